@@ -1,0 +1,171 @@
+/*
+ * mara_gpu.h -- C ABI of the MI355X (gfx950) Godunov finite-volume step.
+ *
+ * The library advances one mesh block of Mara2's method-of-lines TVD scheme
+ * on one GPU.  Every per-zone operation (reconstruction, Riemann fluxes,
+ * flux-CT, divergence, Runge-Kutta stage update, primitive recovery, ghost
+ * fill, CFL reduction) runs in hand-written HIP kernels; the host side only
+ * validates the descriptor, owns device memory and enqueues launches.
+ *
+ * Conventions shared by every entry point:
+ *   - Return value is a status code: MARA_OK (0) or one of MARA_ERR_*.
+ *     mara_error_string() turns it into text.
+ *   - Host arrays of primitive data use the reference MeshData::P layout
+ *     (src/MeshData.cpp:8-28): a row-major array of shape
+ *     (n1+2g1, n2+2g2, n3+2g3, nq), guard zones included, field index
+ *     innermost.  Flat axes have n=1, g=0.
+ *   - Field order is the reference's: density, velocity1-3, pressure,
+ *     magnetic1-3 (src/ConservationLaws.cpp:6-24).
+ *   - No pointer passed in is retained after the call returns.
+ *   - A block handle may be used from one host thread at a time; it owns
+ *     one HIP stream.  Calls are synchronous on return unless noted.
+ */
+#ifndef MARA_GPU_H
+#define MARA_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define MARA_OK                    0
+#define MARA_ERR_INVALID_ARGUMENT  1  /* bad pointer, size or enum value */
+#define MARA_ERR_GUARD_TOO_SMALL   2  /* src/SolutionSchemes.cpp:45-48 logic_error */
+#define MARA_ERR_UNSUPPORTED       3  /* configuration the reference cannot run */
+#define MARA_ERR_HIP               4  /* a HIP runtime call failed */
+#define MARA_ERR_INVERSION_FAILURE 5  /* src/FieldOperator.cpp:57-80 InversionFailure */
+#define MARA_ERR_SUPERLUMINAL      6  /* src/ConservationLaws.cpp:684-687 runtime_error */
+#define MARA_ERR_BAD_EIGENVALUES   7  /* src/ConservationLaws.cpp:796-833 runtime_error */
+#define MARA_ERR_NO_DEVICE         8  /* no usable gfx950 device */
+#define MARA_ERR_CAPACITY          9  /* request exceeds a stated capacity */
+
+/* ---- enumerations (each names the reference class it stands for) -------- */
+#define MARA_LAW_HYDRO   0  /* NewtonianHydro   src/ConservationLaws.cpp:233-343 */
+#define MARA_LAW_MHD     1  /* NewtonianMHD     src/ConservationLaws.cpp:347-605 */
+#define MARA_LAW_RMHD    2  /* RelativisticMHD  src/ConservationLaws.cpp:609-945 */
+
+#define MARA_RECON_PCM   0  /* MethodOfLines     src/IntercellFluxSchemes.cpp:35-63 */
+#define MARA_RECON_PLM   1  /* MethodOfLinesPlm  src/IntercellFluxSchemes.cpp:69-117 */
+#define MARA_RECON_WENO5 2  /* MethodOfLinesPlm structure with Reconstruction::weno5
+                               (WENO5_FD_C2R/C2L, ImprovedShenZha10, A=50),
+                               src/Reconstruction.cpp:68-169 */
+
+#define MARA_RIEMANN_HLLE 0 /* HlleRiemannSolver                src/RiemannSolvers.cpp:38-57 */
+#define MARA_RIEMANN_HLLC 1 /* HllcNewtonianHydroRiemannSolver  src/RiemannSolvers.cpp:63-118 */
+
+#define MARA_BC_PERIODIC   0 /* PeriodicBoundaryCondition   src/BoundaryConditions.cpp:24-60 */
+#define MARA_BC_OUTFLOW    1 /* OutflowBoundaryCondition    src/BoundaryConditions.cpp:66-105 */
+#define MARA_BC_REFLECTING 2 /* ReflectingBoundaryCondition src/BoundaryConditions.cpp:111-176 */
+
+#define MARA_PREC_FP64   0  /* every operation in IEEE binary64, no contraction */
+#define MARA_PREC_MIXED  1  /* reconstruction, Riemann fluxes and flux divergence
+                               in binary32; RK accumulation, flux-CT, primitive
+                               recovery and the RMHD quartic in binary64 */
+
+/* ---- block descriptor ---------------------------------------------------- */
+typedef struct mara_desc
+{
+    int    law;            /* MARA_LAW_* */
+    double gamma;          /* ConservationLaw::setGammaLawIndex; reference default 5/3 */
+    double pressure_floor; /* ConservationLaw::setPressureFloor; <= 0 disables */
+    int    recon;          /* MARA_RECON_*; stencil size PCM 1, PLM 2, WENO5 3 */
+    double plm_theta;      /* MethodOfLinesPlm::setPlmTheta; reference default 1.5 */
+    int    riemann;        /* MARA_RIEMANN_*; HLLC requires MARA_LAW_HYDRO */
+    int    bc;             /* MARA_BC_*, applied on every axis with guards */
+    int    cells[3];       /* interior cells per axis (CartesianMeshGeometry shape) */
+    int    guards[3];      /* MeshData boundary shape; 0 on flat axes */
+    double lower[3];       /* CartesianMeshGeometry::setLowerUpper */
+    double upper[3];
+    int    rk_order;       /* MethodOfLinesTVD::setRungeKuttaOrder, 1..3 */
+    int    field_ct;       /* 1 unless MethodOfLinesTVD::setDisableFieldCT(true) */
+    int    precision;      /* MARA_PREC_* */
+} mara_desc;
+
+/* Per-call diagnostics.  The reference throws at the first failure; the GPU
+   finishes the launch, counts every failing zone and reports the first one
+   (smallest linear interior index), then returns the matching status. */
+typedef struct mara_report
+{
+    int       status;           /* same value the call returns */
+    long long c2p_failures;     /* zones whose fromConserved failed (StateFailure) */
+    int       first_failure[3]; /* interior-relative (i,j,k) of the first of them */
+    long long floor_applied;    /* zones where the pressure floor was used */
+    long long eigen_failures;   /* RMHD quartic gave nr not in {2,4} */
+    long long superluminal;     /* RMHD states with v.v >= 1 */
+} mara_report;
+
+typedef struct mara_block mara_block;
+
+/* ---- library -------------------------------------------------------------- */
+const char* mara_version(void);
+const char* mara_error_string(int status);
+
+/* Number of fields of a law (getNumConserved): 5 hydro, 8 MHD/RMHD. */
+int mara_num_fields(int law);
+
+/* Stencil size of a reconstruction (IntercellFluxScheme::getStencilSize). */
+int mara_stencil_size(int recon);
+
+/* ---- one mesh block --------------------------------------------------------- */
+
+/* Validate the descriptor and allocate the device state of one block.
+   Replaces constructing MeshData (src/MeshData.cpp:8) plus a
+   MethodOfLinesTVD wired with its collaborators (src/Problems.cpp:159-174). */
+int mara_block_create(const mara_desc* desc, mara_block** out);
+int mara_block_destroy(mara_block* block);
+
+/* Copy primitives, guards included, in MeshData::P layout (see above),
+   between host and device.  Count of doubles = prod(n+2g) * nq. */
+int mara_block_upload_prim(mara_block* block, const double* P);
+int mara_block_download_prim(mara_block* block, double* P);
+
+/* One full Runge-Kutta step of size dt.  Replaces
+   MethodOfLinesTVD::advance(MeshData&, double t0, double dt)
+   (src/SolutionSchemes.cpp:108-275).  report may be NULL. */
+int mara_block_advance(mara_block* block, double t0, double dt, mara_report* report);
+
+/* min over interior zones of L / max(lambda_x, lambda_y, lambda_z), no CFL
+   factor.  Replaces FieldOperator::getCourantTimestep(P, L) with
+   L = MeshOperator::linearCellDimension() (src/FieldOperator.cpp:125-169,
+   src/MeshOperator.cpp:292-307). */
+int mara_block_courant_dt(mara_block* block, double* dt);
+
+/* Fill guard zones of the resident primitives.  Replaces
+   MeshData::applyBoundaryCondition -> BoundaryCondition::applySimple
+   (src/MeshData.cpp:148-151, src/BoundaryConditions.cpp:8-18). */
+int mara_block_apply_bc(mara_block* block);
+
+/* max |M| over interior zones of CellCenteredFieldCT::monopole(B, cell)
+   (src/CellCenteredFieldCT.cpp:159-229), B = cell-centred field with
+   guards.  MARA_ERR_UNSUPPORTED for laws without a magnetic field. */
+int mara_block_monopole_max(mara_block* block, double* max_abs);
+
+/* The maraMainLoop contract (src/Mara.cpp:42-88) without a host round trip
+   per step: up to max_steps steps of dt = min(cfl * courant_dt, t_final - t)
+   while t < t_final.  The time step, the clock and the stop condition live
+   on the device; the host waits once at the end.  On return *t holds the
+   new time and *steps the number of steps taken. */
+int mara_block_run(mara_block* block, double cfl, double t_final, int max_steps,
+                   double* t, int* steps, mara_report* report);
+
+/* Kernel timing on the block's stream (HIP events around each launch of
+   the dominant per-stage kernel).  enable=1 starts a fresh tally. */
+int mara_block_set_timing(mara_block* block, int enable);
+/* Average duration in milliseconds and launch count of the stage kernels
+   since timing was enabled; *name receives a static string naming the
+   kernel.  Requires a synchronised stream (every entry point returns so). */
+int mara_block_stage_kernel_time(mara_block* block, double* avg_ms, long long* launches,
+                                 const char** name);
+
+/* Bytes of device memory the block holds (state, fluxes, scratch). */
+long long mara_block_device_bytes(const mara_block* block);
+
+/* Device bytes a block with this descriptor would hold, without allocating
+   (used for admission by HBM budget). */
+long long mara_desc_device_bytes(const mara_desc* desc);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MARA_GPU_H */

@@ -1,0 +1,213 @@
+"""ctypes access to the CPU oracle (oracle/liboracle.so) and, where it was
+built, to the reference bridge (oracle/_ref/libmara_ref.so).
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, __graft_entry__.smoke() and
+bench.py's CPU-baseline leg; the product never imports this package.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from mara2_amd.abi import MaraDesc, MaraReport, array_shape, num_fields
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_REF = None
+
+_d = ctypes.c_double
+_i = ctypes.c_int
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def build():
+    """Compile the oracle (and the reference bridge when the reference tree
+    is present) with oracle/Makefile."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        L.oracle_plm.restype = _d
+        L.oracle_weno5.restype = _d
+        L.oracle_weno5.argtypes = [_dp, _i, _i, _d]
+        L.oracle_plm.argtypes = [_dp, _d, _d]
+        L.oracle_quartic.argtypes = [_dp, _dp]
+        L.oracle_rmhd_c2p.argtypes = [_dp, _d, _d, _dp, _ip]
+        L.oracle_from_primitive.argtypes = [_i, _d, _dp, _dp, _dp, _dp, _dp]
+        L.oracle_from_conserved.argtypes = [_i, _d, _d, _dp, _dp, _ip]
+        L.oracle_riemann.argtypes = [_i, _i, _d, _dp, _dp, _dp, _dp]
+        L.oracle_ct_correct.argtypes = [_dp, _ip, _i, _i]
+        L.oracle_divergence.argtypes = [_dp, _ip, _i, _d, _dp, _d, _dp]
+        L.oracle_monopole_cell.argtypes = [_dp, _ip, _dp]
+        L.oracle_vector_potential_to_B.argtypes = [ctypes.POINTER(MaraDesc), _dp, _dp]
+        L.oracle_apply_bc.argtypes = [ctypes.POINTER(MaraDesc), _dp]
+        L.oracle_advance.argtypes = [ctypes.POINTER(MaraDesc), _dp, _d, _d, ctypes.POINTER(MaraReport)]
+        L.oracle_courant_dt.argtypes = [ctypes.POINTER(MaraDesc), _dp, _dp]
+        L.oracle_monopole_max.argtypes = [ctypes.POINTER(MaraDesc), _dp, _dp]
+        L.oracle_set_threads.argtypes = [_i]
+        _LIB = L
+    return _LIB
+
+
+def ref():
+    """The reference bridge, or None where oracle/_ref was not built."""
+    global _REF
+    if _REF is None:
+        path = os.path.join(_HERE, "_ref", "libmara_ref.so")
+        if not os.path.exists(path):
+            return None
+        R = ctypes.CDLL(path)
+        R.ref_plm.restype = _d
+        R.ref_plm.argtypes = [_dp, _d, _d]
+        R.ref_weno5.restype = _d
+        R.ref_weno5.argtypes = [_dp, _i, _i, _d]
+        R.ref_quartic.argtypes = [_dp, _dp]
+        R.ref_rmhd_c2p.argtypes = [_dp, _d, _d, _dp, _ip]
+        R.ref_unit_vectors.argtypes = [_dp]
+        _REF = R
+    return _REF
+
+
+def _ptr(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+def _arr(x, n=None):
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+    if n is not None and a.size < n:
+        a = np.concatenate([a.ravel(), np.zeros(n - a.size)])
+    return a
+
+
+# ---- point functions ---------------------------------------------------------
+def unit_vectors():
+    out = np.zeros(9)
+    lib().oracle_unit_vectors(_ptr(out))
+    return out.reshape(3, 3)
+
+
+def plm(v, theta, sgn):
+    return lib().oracle_plm(_ptr(_arr(v)), theta, sgn)
+
+
+def weno5(v, op, mode=2, A=50.0):
+    return lib().oracle_weno5(_ptr(_arr(v)), op, mode, A)
+
+
+def quartic(d):
+    roots = np.zeros(4)
+    nr = lib().oracle_quartic(_ptr(_arr(d)), _ptr(roots))
+    return nr, roots[:nr].copy()
+
+
+def rmhd_c2p(U, gamma, pfloor=-1.0):
+    P = np.zeros(8)
+    which = ctypes.c_int(0)
+    err = lib().oracle_rmhd_c2p(_ptr(_arr(U)), gamma, pfloor, _ptr(P), ctypes.byref(which))
+    return err, P, which.value
+
+
+def from_primitive(law, gamma, P, n):
+    nq = num_fields(law)
+    U, F, A = np.zeros(nq), np.zeros(nq), np.zeros(10)
+    err = lib().oracle_from_primitive(law, gamma, _ptr(_arr(P, 10)), _ptr(_arr(n)), _ptr(U), _ptr(F), _ptr(A))
+    return err, U, F, A
+
+
+def from_conserved(law, gamma, U, pfloor=-1.0):
+    P = np.zeros(10)
+    h = ctypes.c_int(0)
+    err = lib().oracle_from_conserved(law, gamma, pfloor, _ptr(_arr(U, 10)), _ptr(P), ctypes.byref(h))
+    return err, P[:num_fields(law)].copy(), h.value
+
+
+def riemann(law, solver, gamma, PL, PR, n):
+    F = np.zeros(10)
+    err = lib().oracle_riemann(law, solver, gamma, _ptr(_arr(PL, 10)), _ptr(_arr(PR, 10)), _ptr(_arr(n)), _ptr(F))
+    return err, F[:num_fields(law)].copy()
+
+
+# ---- array functions -----------------------------------------------------------
+def ct_correct(F, mag_index=5):
+    """In place on F of shape (s0, s1, s2, nq, 3)."""
+    assert F.ndim == 5 and F.shape[4] == 3
+    s = (ctypes.c_int * 3)(*F.shape[:3])
+    lib().oracle_ct_correct(_ptr(F), s, F.shape[3], mag_index)
+    return F
+
+
+def divergence(F, factor, dA, dV):
+    s = (ctypes.c_int * 3)(*F.shape[:3])
+    D = np.zeros((F.shape[0] - 1, F.shape[1] - 1, F.shape[2] - 1, F.shape[3]))
+    lib().oracle_divergence(_ptr(F), s, F.shape[3], factor, _ptr(_arr(dA)), dV, _ptr(D))
+    return D
+
+
+def monopole_cell(B):
+    s = (ctypes.c_int * 3)(*B.shape[:3])
+    M = np.zeros(B.shape[:3])
+    lib().oracle_monopole_cell(_ptr(np.ascontiguousarray(B)), s, _ptr(M))
+    return M
+
+
+def vector_potential_to_B(desc, A):
+    N = [desc.cells[a] + 2 * desc.guards[a] for a in range(3)]
+    B = np.zeros((N[0], N[1], N[2], 3))
+    lib().oracle_vector_potential_to_B(ctypes.byref(desc), _ptr(np.ascontiguousarray(A)), _ptr(B))
+    return B
+
+
+def apply_bc(desc, P):
+    return lib().oracle_apply_bc(ctypes.byref(desc), _ptr(P))
+
+
+def advance(desc, P, t0, dt):
+    """MethodOfLinesTVD::advance on P (MeshData::P layout) in place."""
+    assert P.shape == array_shape(desc)
+    rep = MaraReport()
+    err = lib().oracle_advance(ctypes.byref(desc), _ptr(P), t0, dt, ctypes.byref(rep))
+    return err, rep
+
+
+def courant_dt(desc, P):
+    dt = ctypes.c_double(0)
+    err = lib().oracle_courant_dt(ctypes.byref(desc), _ptr(P), ctypes.byref(dt))
+    return err, dt.value
+
+
+def monopole_max(desc, P):
+    m = ctypes.c_double(0)
+    err = lib().oracle_monopole_max(ctypes.byref(desc), _ptr(P), ctypes.byref(m))
+    return err, m.value
+
+
+def set_threads(n):
+    lib().oracle_set_threads(int(n))
+
+
+def run(desc, P, cfl, t_final, max_steps=10**9):
+    """The maraMainLoop contract (src/Mara.cpp:42-88) on the oracle:
+    dt = min(cfl * courant, t_final - t) until t >= t_final."""
+    t, steps = 0.0, 0
+    while t < t_final and steps < max_steps:
+        err, dtc = courant_dt(desc, P)
+        if err:
+            return err, t, steps
+        dt1 = cfl * dtc
+        dt2 = t_final - t
+        dt = dt1 if dt1 < dt2 else dt2
+        err, rep = advance(desc, P, 0.0, dt)
+        if err:
+            return err, t, steps
+        t += dt
+        steps += 1
+    return 0, t, steps

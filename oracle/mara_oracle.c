@@ -1,0 +1,1301 @@
+/*
+ * mara_oracle.c -- CPU restatement of Mara2's Godunov time step.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker): used by tests/, by
+ * __graft_entry__.smoke() and by bench.py's CPU-baseline leg, never by the
+ * product.  It restates the reference algorithm from its source text, one
+ * function per reference routine, keeping the reference's floating-point
+ * operation order.  Build with -ffp-contract=off (oracle/Makefile does).
+ *
+ * Parity pins for each part are listed in DESIGN.md; the ones compared
+ * bit-for-bit against the reference's own compiled sources live in
+ * tests/test_oracle_ref.py.
+ */
+#include <math.h>
+#include <float.h>
+#include <stdlib.h>
+#include <string.h>
+#include "mara_oracle.h"
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define NF 10 /* MARA_NUM_FIELDS, src/Mara.hpp:15 */
+
+/* Comparison macros with the reference's NaN behaviour
+   (src/RiemannSolvers.cpp:3-6, src/Reconstruction.cpp:5-11). */
+#define MIN2(a, b) ((a) < (b) ? (a) : (b))
+#define MIN3(a, b, c) ((a) < (b) ? MIN2(a, c) : MIN2(b, c))
+#define MAX2(a, b) ((a) > (b) ? (a) : (b))
+#define MAX3(a, b, c) ((a) > (b) ? MAX2(a, c) : MAX2(b, c))
+#define SGN(x) (((x) > 0) - ((x) < 0))
+
+static int g_threads = 1;
+
+void oracle_set_threads(int n)
+{
+    g_threads = n < 1 ? 1 : n;
+#ifdef _OPENMP
+    omp_set_num_threads(g_threads);
+#endif
+}
+
+int oracle_get_threads(void)
+{
+#ifdef _OPENMP
+    return g_threads;
+#else
+    return 1;
+#endif
+}
+
+/* ========================================================================== */
+/* UnitVector (src/UnitVector.cpp:9-39): xhat=(mu 0, phi 0), yhat=(0, pi/2),
+   zhat=(1, 0); cartesian() = (sin th cos phi, sin th sin phi, cos th). */
+static void unit_vector(double mu, double phi, double out[3])
+{
+    const double cosTheta = mu;
+    const double sinTheta = sqrt(1 - cosTheta * cosTheta);
+    out[0] = sinTheta * cos(phi);
+    out[1] = sinTheta * sin(phi);
+    out[2] = cosTheta;
+}
+
+void oracle_unit_vectors(double out[9])
+{
+    unit_vector(0.0, 0.0, out + 0);
+    unit_vector(0.0, M_PI / 2, out + 3);
+    unit_vector(1.0, 0.0, out + 6);
+}
+
+/* ========================================================================== */
+/* Reconstruction (src/Reconstruction.cpp:55-66). */
+static double minmod(double theta, double ul, double u0, double ur)
+{
+    const double a = theta * (u0 - ul);
+    const double b = 0.5 * (ur - ul);
+    const double c = theta * (ur - u0);
+    const double fa = fabs(a), fb = fabs(b), fc = fabs(c);
+    const double m = MIN3(fa, fb, fc);
+    return 0.25 * fabs((double)(SGN(a) + SGN(b))) * (double)(SGN(a) + SGN(c)) * m;
+}
+
+static double plm(const double* v, double theta, double sgn)
+{
+    return v[0] + sgn * 0.5 * minmod(theta, v[-1], v[0], v[1]);
+}
+
+double oracle_plm(const double v[3], double theta, double sgn)
+{
+    return plm(v + 1, theta, sgn);
+}
+
+/* Coefficient tables (src/Reconstruction.cpp:134-169). */
+static const double C_A2C_FV[3][3] = {{23./24., 1./12., -1./24.}, {-1./24., 13./12., -1./24.}, {-1./24., 1./12., 23./24.}};
+static const double C_C2A_FV[3][3] = {{25./24., -1./12., 1./24.}, {1./24., 11./12., 1./24.}, {1./24., -1./12., 25./24.}};
+static const double C_C2L_FV[3][3] = {{15./8., -5./4., 3./8.}, {3./8., 3./4., -1./8.}, {-1./8., 3./4., 3./8.}};
+static const double C_C2R_FV[3][3] = {{3./8., 3./4., -1./8.}, {-1./8., 3./4., 3./8.}, {3./8., -5./4., 15./8.}};
+static const double C_C2L_FD[3][3] = {{11./6., -7./6., 1./3.}, {1./3., 5./6., -1./6.}, {-1./6., 5./6., 1./3.}};
+static const double C_C2R_FD[3][3] = {{1./3., 5./6., -1./6.}, {-1./6., 5./6., 1./3.}, {1./3., -7./6., 11./6.}};
+static const double D_A2C_FV[3] = {-9./80., 49./40., -9./80.};
+static const double D_C2A_FV[3] = {-17./240., 137./120., -17./240.};
+static const double D_C2L_FV[3] = {1./16., 5./8., 5./16.};
+static const double D_C2R_FV[3] = {5./16., 5./8., 1./16.};
+static const double D_C2L_FD[3] = {0.1, 0.6, 0.3};
+static const double D_C2R_FD[3] = {0.3, 0.6, 0.1};
+
+#define SQU(x) ((x) * (x))
+
+/* Reconstruction::weno5 (src/Reconstruction.cpp:68-132); v centred at v[0]. */
+static double weno5(const double* v, const double c[3][3], const double d[3], int mode, double A)
+{
+    double eps_prime = 1e-6;
+    double w[3];
+    const double vs[3] = {
+        c[0][0] * v[+0] + c[0][1] * v[+1] + c[0][2] * v[2],
+        c[1][0] * v[-1] + c[1][1] * v[+0] + c[1][2] * v[1],
+        c[2][0] * v[-2] + c[2][1] * v[-1] + c[2][2] * v[0],
+    };
+    double B[3] = {
+        (13. / 12.) * SQU(1 * v[+0] - 2 * v[+1] + 1 * v[+2]) + (1. / 4.) * SQU(3 * v[+0] - 4 * v[+1] + 1 * v[+2]),
+        (13. / 12.) * SQU(1 * v[-1] - 2 * v[+0] + 1 * v[+1]) + (1. / 4.) * SQU(1 * v[-1] - 0 * v[+0] - 1 * v[+1]),
+        (13. / 12.) * SQU(1 * v[-2] - 2 * v[-1] + 1 * v[+0]) + (1. / 4.) * SQU(1 * v[-2] - 4 * v[-1] + 3 * v[+0])};
+
+    switch (mode)
+    {
+        case 0:
+        {
+            eps_prime = 1e-6;
+            w[0] = d[0] / SQU(eps_prime + B[0]);
+            w[1] = d[1] / SQU(eps_prime + B[1]);
+            w[2] = d[2] / SQU(eps_prime + B[2]);
+            break;
+        }
+        case 1:
+        {
+            const double eps = 1e-14;
+            const double tau5 = fabs(B[0] - B[2]);
+            w[0] = d[0] * (1.0 + (tau5 / (B[0] + eps)));
+            w[1] = d[1] * (1.0 + (tau5 / (B[1] + eps)));
+            w[2] = d[2] * (1.0 + (tau5 / (B[2] + eps)));
+            break;
+        }
+        default:
+        {
+            eps_prime = 1e-10;
+            const double minB = MIN3(B[0], B[1], B[2]);
+            const double maxB = MAX3(B[0], B[1], B[2]);
+            const double R0 = minB / (maxB + eps_prime);
+            B[0] = R0 * A * minB + B[0];
+            B[1] = R0 * A * minB + B[1];
+            B[2] = R0 * A * minB + B[2];
+            w[0] = d[0] / SQU(eps_prime + B[0]);
+            w[1] = d[1] / SQU(eps_prime + B[1]);
+            w[2] = d[2] / SQU(eps_prime + B[2]);
+            break;
+        }
+    }
+    const double wtot = w[0] + w[1] + w[2];
+    return (w[0] * vs[0] + w[1] * vs[1] + w[2] * vs[2]) / wtot;
+}
+
+double oracle_weno5(const double v[5], int op, int mode, double A)
+{
+    const double* c = v + 2;
+    switch (op)
+    {
+        case 0: return weno5(c, C_C2R_FD, D_C2R_FD, mode, A);
+        case 1: return weno5(c, C_C2L_FD, D_C2L_FD, mode, A);
+        case 2: return weno5(c, C_C2R_FV, D_C2R_FV, mode, A);
+        case 3: return weno5(c, C_C2L_FV, D_C2L_FV, mode, A);
+        case 4: return weno5(c, C_C2A_FV, D_C2A_FV, mode, A);
+        case 5: return weno5(c, C_A2C_FV, D_A2C_FV, mode, A);
+        default: return 0.0;
+    }
+}
+
+/* ========================================================================== */
+/* QuarticPolynomial (src/QuarticPolynomial.cpp:19-144). */
+static void solve_cubic(double c3, double c2, double c1, double c0,
+                        double* x1, double* x2, double* x3, int* nr)
+{
+    const double a2 = c2 / c3;
+    const double a1 = c1 / c3;
+    const double a0 = c0 / c3;
+    const double q = a1 / 3.0 - a2 * a2 / 9.0;
+    const double r = (a1 * a2 - 3.0 * a0) / 6.0 - a2 * a2 * a2 / 27.0;
+    const double delta = q * q * q + r * r;
+
+    if (delta > 0.0)
+    {
+        double s1 = r + sqrt(delta);
+        double s2 = r - sqrt(delta);
+        s1 = (s1 >= 0.0) ? pow(s1, 1. / 3) : -pow(-s1, 1. / 3);
+        s2 = (s2 >= 0.0) ? pow(s2, 1. / 3) : -pow(-s2, 1. / 3);
+        *x1 = (s1 + s2) - a2 / 3.0;
+        *x2 = *x3 = -0.5 * (s1 + s2) - a2 / 3.0;
+        *nr = 1;
+    }
+    else if (delta < 0.0)
+    {
+        const double theta = acos(r / sqrt(-q * q * q)) / 3.0;
+        const double costh = cos(theta);
+        const double sinth = sin(theta);
+        const double sq = sqrt(-q);
+        *x1 = 2.0 * sq * costh - a2 / 3.0;
+        *x2 = -sq * costh - a2 / 3.0 - sqrt(3.0) * sq * sinth;
+        *x3 = -sq * costh - a2 / 3.0 + sqrt(3.0) * sq * sinth;
+        *nr = 3;
+    }
+    else
+    {
+        const double s = (r >= 0.0) ? pow(r, 1. / 3) : -pow(-r, 1. / 3);
+        *x1 = 2.0 * s - a2 / 3.0;
+        *x2 = *x3 = -s - a2 / 3.0;
+        *nr = 3;
+    }
+}
+
+int oracle_quartic(const double dd[5], double roots[4])
+{
+    const double d4 = dd[0], d3 = dd[1], d2 = dd[2], d1 = dd[3], d0 = dd[4];
+    const double a3 = d3 / d4;
+    const double a2 = d2 / d4;
+    const double a1 = d1 / d4;
+    const double a0 = d0 / d4;
+    const double au2 = -a2;
+    const double au1 = (a1 * a3 - 4 * a0);
+    const double au0 = 4.0 * a0 * a2 - a1 * a1 - a0 * a3 * a3;
+    int nr, nr12, nr34;
+    double r1, r2, r3, r4, x1, x2, x3, u1;
+
+    solve_cubic(1.0, au2, au1, au0, &x1, &x2, &x3, &nr);
+    u1 = (nr == 1) ? x1 : ((x1 > x3) ? x1 : x3);
+
+    double R2 = 0.25 * a3 * a3 + u1 - a2;
+    double R = (R2 > 0.0) ? sqrt(R2) : 0.0;
+    double D2, E2;
+
+    if (R != 0.0)
+    {
+        const double foo1 = 0.75 * a3 * a3 - R2 - 2.0 * a2;
+        const double foo2 = 0.25 * (4.0 * a3 * a2 - 8.0 * a1 - a3 * a3 * a3) / R;
+        D2 = foo1 + foo2;
+        E2 = foo1 - foo2;
+    }
+    else
+    {
+        const double foo1 = 0.75 * a3 * a3 - 2.0 * a2;
+        const double foo2 = 2.00 * sqrt(u1 * u1 - 4.0 * a0);
+        D2 = foo1 + foo2;
+        E2 = foo1 - foo2;
+    }
+    if (D2 >= 0.0)
+    {
+        const double D = sqrt(D2);
+        r1 = -0.25 * a3 + 0.5 * R - 0.5 * D;
+        r2 = -0.25 * a3 + 0.5 * R + 0.5 * D;
+        nr12 = 2;
+    }
+    else
+    {
+        r1 = r2 = -0.25 * a3 + 0.5 * R;
+        nr12 = 0;
+    }
+    if (E2 >= 0.0)
+    {
+        const double E = sqrt(E2);
+        r3 = -0.25 * a3 - 0.5 * R - 0.5 * E;
+        r4 = -0.25 * a3 - 0.5 * R + 0.5 * E;
+        nr34 = 2;
+    }
+    else
+    {
+        r3 = r4 = -0.25 * a3 - 0.5 * R;
+        nr34 = 0;
+    }
+    int i = 0;
+    if (nr12 != 0) { roots[i++] = r1; roots[i++] = r2; }
+    if (nr34 != 0) { roots[i++] = r3; roots[i++] = r4; }
+
+    /* std::sort of at most four values: insertion sort gives the same order */
+    for (int a = 1; a < i; ++a)
+    {
+        double x = roots[a];
+        int b = a - 1;
+        while (b >= 0 && x < roots[b]) { roots[b + 1] = roots[b]; --b; }
+        roots[b + 1] = x;
+    }
+    return nr12 + nr34;
+}
+
+/* ========================================================================== */
+/* ConsToPrimRMHD (src/ConsToPrimRMHD.cpp:41-349). */
+enum { C2P_SUCCESS, C2P_CONS_NAN, C2P_CONS_NEG_D, C2P_CONS_NEG_TAU, C2P_PRIM_NAN,
+       C2P_PRIM_NEG_P, C2P_PRIM_NEG_RHO, C2P_PRIM_SUPERLUMINAL, C2P_MAXITER };
+
+typedef struct
+{
+    double gamf, pfloor;
+    double D, Tau, S2, B2, BS, BS2;
+    double Cons[8];
+    double Z_start, W_start;
+} c2p_state;
+
+static int check_cons(const double* U)
+{
+    if (U[0] < 0.0) return C2P_CONS_NEG_D;
+    if (U[4] < 0.0) return C2P_CONS_NEG_TAU;
+    for (int i = 0; i < 8; ++i)
+        if (!isfinite(U[i])) return C2P_CONS_NAN;
+    return C2P_SUCCESS;
+}
+
+static int check_prim(const double* P)
+{
+    const double v2 = P[1] * P[1] + P[2] * P[2] + P[3] * P[3];
+    if (v2 >= 1.0) return C2P_PRIM_SUPERLUMINAL;
+    if (P[4] < 0.0) return C2P_PRIM_NEG_P;
+    if (P[0] < 0.0) return C2P_PRIM_NEG_RHO;
+    for (int i = 0; i < 8; ++i)
+        if (!isfinite(P[i])) return C2P_PRIM_NAN;
+    return C2P_SUCCESS;
+}
+
+static int reconstruct_prim(const c2p_state* s, double Z, double W, double* Pout)
+{
+    double P[8];
+    const double b0 = s->BS * W / Z;
+    P[0] = s->D / W;
+    P[4] = (s->D / W) * (Z / (s->D * W) - 1.0) * s->gamf;
+    P[1] = (s->Cons[1] + b0 * s->Cons[5] / W) / (Z + s->B2);
+    P[2] = (s->Cons[2] + b0 * s->Cons[6] / W) / (Z + s->B2);
+    P[3] = (s->Cons[3] + b0 * s->Cons[7] / W) / (Z + s->B2);
+    P[5] = s->Cons[5];
+    P[6] = s->Cons[6];
+    P[7] = s->Cons[7];
+    if (P[4] <= 0.0 && s->pfloor > 0.0)
+        P[4] = P[0] * s->pfloor;
+    int error = check_prim(P);
+    if (error) return error;
+    memcpy(Pout, P, 8 * sizeof(double));
+    return C2P_SUCCESS;
+}
+
+static int solve_anton2dzw(const c2p_state* s, double* P)
+{
+    int bad = check_cons(s->Cons);
+    if (bad) return bad;
+    int iterations = 0;
+    double error = 1.0;
+    double W = s->W_start, Z = s->Z_start;
+    const double D = s->D, Tau = s->Tau, S2 = s->S2, B2 = s->B2, BS2 = s->BS2, gamf = s->gamf;
+
+    while (error > 1e-12)
+    {
+        const double Z2 = Z * Z;
+        const double Z3 = Z * Z2;
+        const double W2 = W * W;
+        const double W3 = W * W2;
+        const double Pre = (D / W) * (Z / (D * W) - 1.0) * gamf;
+        const double df0dZ = 2 * (B2 + Z) * (BS2 * W2 + (W2 - 1) * Z3) / (W2 * Z3);
+        const double df0dW = 2 * (B2 + Z) * (B2 + Z) / W3;
+        const double df1dZ = 1.0 + BS2 / Z3 - gamf / W2;
+        const double df1dW = B2 / W3 + (2 * Z - D * W) / W3 * gamf;
+        double f[2], J[4], G[4];
+        f[0] = -S2 + (Z + B2) * (Z + B2) * (W2 - 1) / W2 - (2 * Z + B2) * BS2 / Z2;
+        f[1] = -Tau + Z + B2 - Pre - 0.5 * B2 / W2 - 0.5 * BS2 / Z2 - D;
+        J[0] = df0dZ; J[1] = df0dW;
+        J[2] = df1dZ; J[3] = df1dW;
+        const double det = J[0] * J[3] - J[1] * J[2];
+        G[0] = J[3] / det; G[1] = -J[1] / det;
+        G[2] = -J[2] / det; G[3] = J[0] / det;
+        const double dZ = -(G[0] * f[0] + G[1] * f[1]);
+        const double dW = -(G[2] * f[0] + G[3] * f[1]);
+        double Z_new = Z + dZ;
+        double W_new = W + dW;
+        Z_new = (Z_new > 0.0) ? Z_new : -Z_new;
+        Z_new = (Z_new < 1e20) ? Z_new : Z;
+        W_new = (W_new > 1.0) ? W_new : 1.0;
+        W_new = (W_new < 1e12) ? W_new : 1e12;
+        Z = Z_new;
+        W = W_new;
+        error = fabs(dZ / Z) + fabs(dW / W);
+        ++iterations;
+        if (iterations == 250) return C2P_MAXITER;
+    }
+    return reconstruct_prim(s, Z, W, P);
+}
+
+static int solve_noble1dw(const c2p_state* s, double* P)
+{
+    int bad = check_cons(s->Cons);
+    if (bad) return bad;
+    int iterations = 0;
+    double error = 1.0;
+    double Z = s->Z_start;
+    const double D = s->D, Tau = s->Tau, S2 = s->S2, B2 = s->B2, BS2 = s->BS2, gamf = s->gamf;
+
+    while (error > 1e-12)
+    {
+        const double Z2 = Z * Z;
+        const double Z3 = Z * Z2;
+        const double a = S2 * Z2 + BS2 * (B2 + 2 * Z);
+        const double b = (B2 + Z) * (B2 + Z) * Z2;
+        const double ap = 2 * (S2 * Z + BS2);
+        const double bp = 2 * Z * (B2 + Z) * (B2 + 2 * Z);
+        const double V2 = a / b;
+        const double W2 = 1.0 / (1.0 - V2);
+        const double W = sqrt(W2);
+        const double W3 = W * W2;
+        const double Pre = (D / W) * (Z / (D * W) - 1.0) * gamf;
+        const double dv2dZ = (ap * b - bp * a) / (b * b);
+        const double delPdelZ = gamf / W2;
+        const double delPdelW = gamf * (D / W2 - 2 * Z / W3);
+        const double dWdv2 = 0.5 * W3;
+        const double dPdZ = delPdelW * dWdv2 * dv2dZ + delPdelZ;
+        const double f = Tau + D - 0.5 * B2 * (1 + V2) + 0.5 * BS2 / Z2 - Z + Pre;
+        const double g = -0.5 * B2 * dv2dZ - BS2 / Z3 - 1.0 + dPdZ;
+        const double dZ = -f / g;
+        double Z_new = Z + dZ;
+        Z_new = (Z_new > 0.0) ? Z_new : -Z_new;
+        Z_new = (Z_new < 1e20) ? Z_new : Z;
+        Z = Z_new;
+        error = fabs(dZ / Z);
+        ++iterations;
+        if (iterations == 250) return C2P_MAXITER;
+    }
+    const double Z2 = Z * Z;
+    const double a = S2 * Z2 + BS2 * (B2 + 2 * Z);
+    const double b = (B2 + Z) * (B2 + Z) * Z2;
+    const double V2 = a / b;
+    const double W2 = 1.0 / (1.0 - V2);
+    const double W = sqrt(W2);
+    return reconstruct_prim(s, Z, W, P);
+}
+
+int oracle_rmhd_c2p(const double U[8], double gamma, double pfloor, double P[8], int* which)
+{
+    c2p_state s;
+    s.gamf = (gamma - 1.0) / gamma;
+    s.pfloor = pfloor;
+    s.D = U[0];
+    s.Tau = U[4];
+    s.S2 = U[1] * U[1] + U[2] * U[2] + U[3] * U[3];
+    s.B2 = U[5] * U[5] + U[6] * U[6] + U[7] * U[7];
+    s.BS = U[5] * U[1] + U[6] * U[2] + U[7] * U[3];
+    s.BS2 = s.BS * s.BS;
+    memcpy(s.Cons, U, 8 * sizeof(double));
+    s.Z_start = sqrt(s.S2 + s.D * s.D);
+    s.W_start = s.Z_start / s.D;
+
+    int error = solve_anton2dzw(&s, P);
+    if (error == 0) { if (which) *which = 0; return 0; }
+    error = solve_noble1dw(&s, P);
+    if (error == 0) { if (which) *which = 1; return 0; }
+    if (which) *which = -1;
+    return error;
+}
+
+/* ========================================================================== */
+/* Conservation laws. */
+typedef struct
+{
+    double P[NF], U[NF], F[NF], A[NF];
+} ostate;
+
+static int law_nq(int law) { return law == MARA_LAW_HYDRO ? 5 : 8; }
+
+/* NewtonianHydro::fromPrimitive (src/ConservationLaws.cpp:253-290) */
+static void hydro_prim(double gamma, const double* P, const double* dAA, ostate* S)
+{
+    const double gm0 = gamma;
+    const double gm1 = gamma - 1.0;
+    const double cs = sqrt(gm0 * P[4] / P[0]);
+    const double vv = P[1] * P[1] + P[2] * P[2] + P[3] * P[3];
+    const double vn = P[1] * dAA[0] + P[2] * dAA[1] + P[3] * dAA[2];
+    memset(S, 0, sizeof *S);
+    for (int q = 0; q < 5; ++q) S->P[q] = P[q];
+    S->U[0] = P[0];
+    S->U[1] = P[0] * P[1];
+    S->U[2] = P[0] * P[2];
+    S->U[3] = P[0] * P[3];
+    S->U[4] = P[0] * 0.5 * vv + P[4] / gm1;
+    S->F[0] = vn * S->U[0];
+    S->F[1] = vn * S->U[1] + P[4] * dAA[0];
+    S->F[2] = vn * S->U[2] + P[4] * dAA[1];
+    S->F[3] = vn * S->U[3] + P[4] * dAA[2];
+    S->F[4] = vn * S->U[4] + P[4] * vn;
+    S->A[0] = vn - cs;
+    S->A[1] = vn;
+    S->A[2] = vn;
+    S->A[3] = vn;
+    S->A[4] = vn + cs;
+}
+
+/* NewtonianMHD::fromPrimitive (src/ConservationLaws.cpp:416-487) */
+static void mhd_prim(double gamma, const double* P, const double* dAA, ostate* S)
+{
+    const double gm0 = gamma;
+    const double gm1 = gamma - 1.0;
+    const double cs2 = gm0 * P[4] / P[0];
+    const double vv = P[1] * P[1] + P[2] * P[2] + P[3] * P[3];
+    const double BB = P[5] * P[5] + P[6] * P[6] + P[7] * P[7];
+    const double Bv = P[5] * P[1] + P[6] * P[2] + P[7] * P[3];
+    const double vn = P[1] * dAA[0] + P[2] * dAA[1] + P[3] * dAA[2];
+    const double Bn = P[5] * dAA[0] + P[6] * dAA[1] + P[7] * dAA[2];
+    const double ps = P[4] + 0.5 * BB;
+    memset(S, 0, sizeof *S);
+    for (int q = 0; q < 8; ++q) S->P[q] = P[q];
+    S->U[0] = P[0];
+    S->U[1] = P[0] * P[1];
+    S->U[2] = P[0] * P[2];
+    S->U[3] = P[0] * P[3];
+    S->U[4] = P[0] * 0.5 * vv + 0.5 * BB + P[4] / gm1;
+    S->U[5] = P[5];
+    S->U[6] = P[6];
+    S->U[7] = P[7];
+    S->F[0] = vn * S->U[0];
+    S->F[1] = vn * S->U[1] - Bn * P[5] + ps * dAA[0];
+    S->F[2] = vn * S->U[2] - Bn * P[6] + ps * dAA[1];
+    S->F[3] = vn * S->U[3] - Bn * P[7] + ps * dAA[2];
+    S->F[4] = vn * S->U[4] - Bn * Bv + ps * vn;
+    S->F[5] = vn * S->U[5] - Bn * P[1];
+    S->F[6] = vn * S->U[6] - Bn * P[2];
+    S->F[7] = vn * S->U[7] - Bn * P[3];
+    const double Bn2 = Bn * Bn;
+    const double ca2 = BB / P[0];
+    const double cn2 = Bn2 / P[0];
+    const double cw4 = (cs2 + ca2) * (cs2 + ca2);
+    const double cF2 = 0.5 * (cs2 + ca2 + sqrt(cw4 - 4 * cs2 * cn2));
+    const double cS2 = 0.5 * (cs2 + ca2 - sqrt(cw4 - 4 * cs2 * cn2));
+    S->A[0] = vn - sqrt(cF2);
+    S->A[1] = vn - sqrt(cn2);
+    S->A[2] = vn - sqrt(cS2);
+    S->A[3] = vn;
+    S->A[4] = vn;
+    S->A[5] = vn + sqrt(cS2);
+    S->A[6] = vn + sqrt(cn2);
+    S->A[7] = vn + sqrt(cF2);
+}
+
+/* RelativisticMHD::fromPrimitive (src/ConservationLaws.cpp:659-836) */
+static int rmhd_prim(double gamma, const double* P, const double* dAA, ostate* S)
+{
+    const double gm0 = gamma;
+    const double gm1 = gamma - 1.0;
+    const double vv = P[1] * P[1] + P[2] * P[2] + P[3] * P[3];
+    const double BB = P[5] * P[5] + P[6] * P[6] + P[7] * P[7];
+    const double Bv = P[5] * P[1] + P[6] * P[2] + P[7] * P[3];
+    const double vn = P[1] * dAA[0] + P[2] * dAA[1] + P[3] * dAA[2];
+    const double Bn = P[5] * dAA[0] + P[6] * dAA[1] + P[7] * dAA[2];
+    const double W2 = 1.0 / (1.0 - vv);
+    const double W0 = sqrt(W2);
+    const double b0 = W0 * Bv;
+    const double bb = (BB + b0 * b0) / W2;
+    const double b1 = (P[5] + b0 * W0 * P[1]) / W0;
+    const double b2 = (P[6] + b0 * W0 * P[2]) / W0;
+    const double b3 = (P[7] + b0 * W0 * P[3]) / W0;
+    const double bn = b1 * dAA[0] + b2 * dAA[1] + b3 * dAA[2];
+    const double e0 = (P[4] / P[0]) / gm1;
+    const double es = e0 + 0.5 * bb / P[0];
+    const double p0 = P[4];
+    const double ps = p0 + 0.5 * bb;
+    const double h0 = 1.0 + e0 + p0 / P[0];
+    const double hs = 1.0 + es + ps / P[0];
+
+    if (vv >= 1) return MARA_ERR_SUPERLUMINAL;
+
+    memset(S, 0, sizeof *S);
+    for (int q = 0; q < 8; ++q) S->P[q] = P[q];
+    S->U[0] = P[0] * W0;
+    S->U[4] = P[0] * hs * W2 - ps - b0 * b0 - S->U[0];
+    S->U[1] = P[0] * hs * W2 * P[1] - b0 * b1;
+    S->U[2] = P[0] * hs * W2 * P[2] - b0 * b2;
+    S->U[3] = P[0] * hs * W2 * P[3] - b0 * b3;
+    S->U[5] = P[5];
+    S->U[6] = P[6];
+    S->U[7] = P[7];
+    S->F[0] = vn * S->U[0];
+    S->F[1] = vn * S->U[1] - Bn * b1 / W0 + ps * dAA[0];
+    S->F[2] = vn * S->U[2] - Bn * b2 / W0 + ps * dAA[1];
+    S->F[3] = vn * S->U[3] - Bn * b3 / W0 + ps * dAA[2];
+    S->F[4] = vn * S->U[4] - Bn * b0 / W0 + ps * vn;
+    S->F[5] = vn * S->U[5] - Bn * P[1];
+    S->F[6] = vn * S->U[6] - Bn * P[2];
+    S->F[7] = vn * S->U[7] - Bn * P[3];
+
+    const double cs2 = gm0 * p0 / (P[0] * h0);
+    const double W4 = W2 * W2;
+    const double v2 = vn * vn;
+    const double v3 = vn * v2;
+    const double v4 = vn * v3;
+
+    if (BB == 0.0)
+    {
+        const double K = sqrt(cs2 * (1 - vv) * (1 - vv * cs2 - v2 * (1 - cs2)));
+        S->A[0] = (vn * (1 - cs2) - K) / (1 - vv * cs2);
+        for (int n = 1; n < 7; ++n) S->A[n] = vn;
+        S->A[7] = (vn * (1 - cs2) + K) / (1 - vv * cs2);
+        return MARA_OK;
+    }
+    const double K = W4 * (P[0] * h0 * (1. / cs2 - 1.));
+    const double L = -W2 * (P[0] * h0 + bb / cs2);
+    double d[5];
+    d[0] = K - L - b0 * b0;
+    d[1] = -4 * K * vn + L * vn * 2 + 2 * b0 * bn;
+    d[2] = 6 * K * v2 + L * (1. - v2) + b0 * b0 - bn * bn;
+    d[3] = -4 * K * v3 - L * vn * 2 - 2 * b0 * bn;
+    d[4] = K * v4 + L * v2 + bn * bn;
+    double roots[4];
+    const int nr = oracle_quartic(d, roots);
+    const double C = sqrt(h0 + bb);
+
+    if (nr == 4)
+    {
+        S->A[0] = roots[0];
+        S->A[1] = (bn - C * vn * W0) / (b0 - C * W0);
+        S->A[2] = roots[1];
+        S->A[3] = vn;
+        S->A[4] = vn;
+        S->A[5] = roots[2];
+        S->A[6] = (bn + C * vn * W0) / (b0 + C * W0);
+        S->A[7] = roots[3];
+    }
+    else if (nr == 2)
+    {
+        S->A[0] = roots[0];
+        S->A[1] = (bn - C * vn * W0) / (b0 - C * W0);
+        S->A[2] = vn;
+        S->A[3] = vn;
+        S->A[4] = vn;
+        S->A[5] = vn;
+        S->A[6] = (bn + C * vn * W0) / (b0 + C * W0);
+        S->A[7] = roots[1];
+    }
+    else
+    {
+        return MARA_ERR_BAD_EIGENVALUES;
+    }
+    return MARA_OK;
+}
+
+static int from_primitive(int law, double gamma, const double* P, const double* n, ostate* S)
+{
+    switch (law)
+    {
+        case MARA_LAW_HYDRO: hydro_prim(gamma, P, n, S); return MARA_OK;
+        case MARA_LAW_MHD: mhd_prim(gamma, P, n, S); return MARA_OK;
+        default: return rmhd_prim(gamma, P, n, S);
+    }
+}
+
+/* fromConserved -> primitive only.  Hydro: src/ConservationLaws.cpp:238-251;
+   MHD: :370-414; RMHD: :627-657. */
+static int from_conserved(int law, double gamma, double pfloor, const double* U, double* P, int* health)
+{
+    *health = 0;
+    if (law == MARA_LAW_HYDRO)
+    {
+        const double gm1 = gamma - 1.0;
+        const double pp = U[1] * U[1] + U[2] * U[2] + U[3] * U[3];
+        P[0] = U[0];
+        P[4] = (U[4] - 0.5 * pp / U[0]) * gm1;
+        P[1] = U[1] / U[0];
+        P[2] = U[2] / U[0];
+        P[3] = U[3] / U[0];
+        return MARA_OK;
+    }
+    if (law == MARA_LAW_MHD)
+    {
+        const double gm1 = gamma - 1.0;
+        const double pp = U[1] * U[1] + U[2] * U[2] + U[3] * U[3];
+        const double BB = U[5] * U[5] + U[6] * U[6] + U[7] * U[7];
+        P[0] = U[0];
+        P[4] = (U[4] - 0.5 * pp / U[0] - 0.5 * BB) * gm1;
+        P[1] = U[1] / U[0];
+        P[2] = U[2] / U[0];
+        P[3] = U[3] / U[0];
+        P[5] = U[5];
+        P[6] = U[6];
+        P[7] = U[7];
+        if (P[4] < 0.0 || P[0] < 0.0 || U[0] < 0.0 || U[4] < 0.0)
+        {
+            if (P[4] < 0.0 && pfloor > 0.0)
+            {
+                P[4] = pfloor * P[0];
+                *health = 1;
+                return MARA_OK;
+            }
+            return MARA_ERR_INVERSION_FAILURE;
+        }
+        return MARA_OK;
+    }
+    int which;
+    if (oracle_rmhd_c2p(U, gamma, pfloor, P, &which) != 0)
+        return MARA_ERR_INVERSION_FAILURE;
+    return MARA_OK;
+}
+
+int oracle_from_primitive(int law, double gamma, const double* P, const double n[3],
+                          double* U, double* F, double* A)
+{
+    ostate S;
+    int e = from_primitive(law, gamma, P, n, &S);
+    if (e) return e;
+    const int nq = law_nq(law);
+    for (int q = 0; q < nq; ++q) { U[q] = S.U[q]; F[q] = S.F[q]; }
+    for (int q = 0; q < NF; ++q) A[q] = S.A[q];
+    return MARA_OK;
+}
+
+int oracle_from_conserved(int law, double gamma, double pfloor, const double* U, double* P, int* health)
+{
+    return from_conserved(law, gamma, pfloor, U, P, health);
+}
+
+/* ========================================================================== */
+/* Riemann solvers (src/RiemannSolvers.cpp:38-118). */
+static double max_elem(const double* a)
+{
+    /* std::max_element over the 10-slot array: first element, then replace
+       while (largest < next) */
+    double m = a[0];
+    for (int n = 1; n < NF; ++n) if (m < a[n]) m = a[n];
+    return m;
+}
+
+static double min_elem(const double* a)
+{
+    double m = a[0];
+    for (int n = 1; n < NF; ++n) if (a[n] < m) m = a[n];
+    return m;
+}
+
+static void hlle(const ostate* L, const ostate* R, double* F, int nq)
+{
+    const double epl = max_elem(L->A);
+    const double eml = min_elem(L->A);
+    const double epr = max_elem(R->A);
+    const double emr = min_elem(R->A);
+    const double ap = MAX3(0.0, epl, epr);
+    const double am = MIN3(0.0, eml, emr);
+    for (int q = 0; q < nq; ++q)
+        F[q] = (ap * L->F[q] - am * R->F[q] + ap * am * (R->U[q] - L->U[q])) / (ap - am);
+}
+
+static void hllc(const ostate* L, const ostate* R, const double* dA, double* Fout)
+{
+    enum { ddd, px, py, pz, nrg };
+    enum { rho, vx, vy, vz, pre };
+    const double epl = max_elem(L->A);
+    const double eml = min_elem(L->A);
+    const double epr = max_elem(R->A);
+    const double emr = min_elem(R->A);
+    const double ap = MAX3(0.0, epl, epr);
+    const double am = MIN3(0.0, eml, emr);
+    const double* Pl = L->P;
+    const double* Pr = R->P;
+    const double* Ul = L->U;
+    const double* Ur = R->U;
+    const double vnr = R->P[vx] * dA[0] + R->P[vy] * dA[1] + R->P[vz] * dA[2];
+    const double vnl = L->P[vx] * dA[0] + L->P[vy] * dA[1] + L->P[vz] * dA[2];
+    double Ul_[5], Ur_[5];
+    double lc = (+(Pr[pre] - Pr[rho] * vnr * (ap - vnr)) - (Pl[pre] - Pl[rho] * vnl * (am - vnl)))
+              / (Pl[rho] * (am - vnl) - Pr[rho] * (ap - vnr));
+    double fl = Pl[rho] * (am - vnl) / (am - lc);
+    double fr = Pr[rho] * (ap - vnr) / (ap - lc);
+    Ul_[ddd] = fl;
+    Ul_[nrg] = fl * (Ul[nrg] / Pl[rho] + (lc - vnl) * (lc + Pl[pre] / (Pl[rho] * (am - vnl))));
+    Ul_[px] = fl * ((lc - vnl) * dA[0] + L->P[vx]);
+    Ul_[py] = fl * ((lc - vnl) * dA[1] + L->P[vy]);
+    Ul_[pz] = fl * ((lc - vnl) * dA[2] + L->P[vz]);
+    Ur_[ddd] = fr;
+    Ur_[nrg] = fr * (Ur[nrg] / Pr[rho] + (lc - vnr) * (lc + Pr[pre] / (Pr[rho] * (ap - vnr))));
+    Ur_[px] = fr * ((lc - vnr) * dA[0] + R->P[vx]);
+    Ur_[py] = fr * ((lc - vnr) * dA[1] + R->P[vy]);
+    Ur_[pz] = fr * ((lc - vnr) * dA[2] + R->P[vz]);
+
+    const double s = 0.0;
+    double F[5] = {0, 0, 0, 0, 0};
+    if (s <= am) for (int i = 0; i < 5; ++i) F[i] = L->F[i];
+    else if (am < s && s <= lc) for (int i = 0; i < 5; ++i) F[i] = L->F[i] + am * (Ul_[i] - Ul[i]);
+    else if (lc < s && s <= ap) for (int i = 0; i < 5; ++i) F[i] = R->F[i] + ap * (Ur_[i] - Ur[i]);
+    else if (ap < s) for (int i = 0; i < 5; ++i) F[i] = R->F[i];
+    for (int i = 0; i < 5; ++i) Fout[i] = F[i];
+}
+
+int oracle_riemann(int law, int solver, double gamma, const double* PL, const double* PR,
+                   const double n[3], double* F)
+{
+    ostate L, R;
+    int e = from_primitive(law, gamma, PL, n, &L);
+    if (e) return e;
+    e = from_primitive(law, gamma, PR, n, &R);
+    if (e) return e;
+    if (solver == MARA_RIEMANN_HLLC) hllc(&L, &R, n, F);
+    else hlle(&L, &R, F, law_nq(law));
+    return MARA_OK;
+}
+
+/* ========================================================================== */
+/* Intercell flux schemes (src/IntercellFluxSchemes.cpp:45-117) with the
+   WENO5 variant defined on the MethodOfLinesPlm structure. */
+static int stencil_size(int recon) { return recon == MARA_RECON_PCM ? 1 : recon == MARA_RECON_PLM ? 2 : 3; }
+
+static int intercell_flux(const mara_desc* d, int nq, const double* S /* [2g][nq] */,
+                          const double* n, double* F)
+{
+    double PL[NF], PR[NF];
+    if (d->recon == MARA_RECON_PCM)
+    {
+        for (int q = 0; q < nq; ++q) { PL[q] = S[0 * nq + q]; PR[q] = S[1 * nq + q]; }
+    }
+    else if (d->recon == MARA_RECON_PLM)
+    {
+        for (int q = 0; q < nq; ++q)
+        {
+            const double ps[4] = {S[0 * nq + q], S[1 * nq + q], S[2 * nq + q], S[3 * nq + q]};
+            PL[q] = plm(&ps[1], d->plm_theta, +1.0);
+            PR[q] = plm(&ps[2], d->plm_theta, -1.0);
+        }
+    }
+    else
+    {
+        for (int q = 0; q < nq; ++q)
+        {
+            double ps[6];
+            for (int m = 0; m < 6; ++m) ps[m] = S[m * nq + q];
+            PL[q] = weno5(&ps[2], C_C2R_FD, D_C2R_FD, 2, 50.0);
+            PR[q] = weno5(&ps[3], C_C2L_FD, D_C2L_FD, 2, 50.0);
+        }
+    }
+    return oracle_riemann(d->law, d->riemann, d->gamma, PL, PR, n, F);
+}
+
+/* ========================================================================== */
+/* Array helpers for the reference layout. */
+typedef struct
+{
+    int N[3];  /* cells incl. guards */
+    int nq;
+} layout;
+
+static inline long cidx(const layout* l, int i, int j, int k)
+{
+    return (((long)i * l->N[1] + j) * l->N[2] + k) * l->nq;
+}
+
+/* flux array (N1+1, N2+1, N3+1, nq, 3) */
+static inline long fidx(const layout* l, int i, int j, int k, int q, int a)
+{
+    return ((((long)i * (l->N[1] + 1) + j) * (l->N[2] + 1) + k) * l->nq + q) * 3 + a;
+}
+
+/* ========================================================================== */
+/* CellCenteredFieldCT::correctGodunovFluxes (src/CellCenteredFieldCT.cpp:14-134) */
+void oracle_ct_correct(double* Fa, const int s[3], int nq, int mag)
+{
+    const long total = (long)s[0] * s[1] * s[2] * nq * 3;
+    double* G = (double*)malloc(total * sizeof(double));
+    memcpy(G, Fa, total * sizeof(double));
+    const int B0 = mag + 0, B1 = mag + 1, B2 = mag + 2;
+#define FX(i, j, k, q, a) Fa[((((long)(i) * s[1] + (j)) * s[2] + (k)) * nq + (q)) * 3 + (a)]
+#define GX(i, j, k, q, a) G[((((long)(i) * s[1] + (j)) * s[2] + (k)) * nq + (q)) * 3 + (a)]
+    for (int i = 0; i < s[0]; ++i)
+    for (int j = 0; j < s[1]; ++j)
+    for (int k = 0; k < s[2]; ++k)
+    {
+        GX(i, j, k, B0, 0) = 0.0;
+        GX(i, j, k, B1, 1) = 0.0;
+        GX(i, j, k, B2, 2) = 0.0;
+    }
+    if (s[0] >= 3)
+    {
+        for (int i = 1; i < s[0] - 0; ++i)
+        for (int j = 1; j < s[1] - 1; ++j)
+        for (int k = 0; k < s[2] - 0; ++k)
+            GX(i, j, k, B1, 0) = 0.125 * (+2 * FX(i + 0, j + 0, k + 0, B1, 0) + 1 * FX(i + 0, j + 1, k + 0, B1, 0)
+                + 1 * FX(i + 0, j - 1, k + 0, B1, 0) - 1 * FX(i + 0, j + 1, k + 0, B0, 1) - 1 * FX(i - 1, j + 1, k + 0, B0, 1)
+                - 1 * FX(i + 0, j + 0, k + 0, B0, 1) - 1 * FX(i - 1, j + 0, k + 0, B0, 1));
+        for (int i = 1; i < s[0] - 0; ++i)
+        for (int j = 0; j < s[1] - 0; ++j)
+        for (int k = 1; k < s[2] - 1; ++k)
+            GX(i, j, k, B2, 0) = 0.125 * (+2 * FX(i + 0, j + 0, k + 0, B2, 0) + 1 * FX(i + 0, j + 0, k + 1, B2, 0)
+                + 1 * FX(i + 0, j + 0, k - 1, B2, 0) - 1 * FX(i + 0, j + 0, k + 1, B0, 2) - 1 * FX(i - 1, j + 0, k + 1, B0, 2)
+                - 1 * FX(i + 0, j + 0, k + 0, B0, 2) - 1 * FX(i - 1, j + 0, k + 0, B0, 2));
+    }
+    if (s[1] >= 3)
+    {
+        for (int i = 0; i < s[0] - 0; ++i)
+        for (int j = 1; j < s[1] - 0; ++j)
+        for (int k = 1; k < s[2] - 1; ++k)
+            GX(i, j, k, B2, 1) = 0.125 * (+2 * FX(i + 0, j + 0, k + 0, B2, 1) + 1 * FX(i + 0, j + 0, k + 1, B2, 1)
+                + 1 * FX(i + 0, j + 0, k - 1, B2, 1) - 1 * FX(i + 0, j + 0, k + 1, B1, 2) - 1 * FX(i + 0, j - 1, k + 1, B1, 2)
+                - 1 * FX(i + 0, j + 0, k + 0, B1, 2) - 1 * FX(i + 0, j - 1, k + 0, B1, 2));
+        for (int i = 1; i < s[0] - 1; ++i)
+        for (int j = 1; j < s[1] - 0; ++j)
+        for (int k = 0; k < s[2] - 0; ++k)
+            GX(i, j, k, B0, 1) = 0.125 * (+2 * FX(i + 0, j + 0, k + 0, B0, 1) + 1 * FX(i + 1, j + 0, k + 0, B0, 1)
+                + 1 * FX(i - 1, j + 0, k + 0, B0, 1) - 1 * FX(i + 1, j + 0, k + 0, B1, 0) - 1 * FX(i + 1, j - 1, k + 0, B1, 0)
+                - 1 * FX(i + 0, j + 0, k + 0, B1, 0) - 1 * FX(i + 0, j - 1, k + 0, B1, 0));
+    }
+    if (s[2] >= 3)
+    {
+        for (int i = 1; i < s[0] - 1; ++i)
+        for (int j = 0; j < s[1] - 0; ++j)
+        for (int k = 1; k < s[2] - 0; ++k)
+            GX(i, j, k, B0, 2) = 0.125 * (+2 * FX(i + 0, j + 0, k + 0, B0, 2) + 1 * FX(i + 1, j + 0, k + 0, B0, 2)
+                + 1 * FX(i - 1, j + 0, k + 0, B0, 2) - 1 * FX(i + 1, j + 0, k + 0, B2, 0) - 1 * FX(i + 1, j + 0, k - 1, B2, 0)
+                - 1 * FX(i + 0, j + 0, k + 0, B2, 0) - 1 * FX(i + 0, j + 0, k - 1, B2, 0));
+        for (int i = 0; i < s[0] - 0; ++i)
+        for (int j = 1; j < s[1] - 1; ++j)
+        for (int k = 1; k < s[2] - 0; ++k)
+            GX(i, j, k, B1, 2) = 0.125 * (+2 * FX(i + 0, j + 0, k + 0, B1, 2) + 1 * FX(i + 0, j + 1, k + 0, B1, 2)
+                + 1 * FX(i + 0, j - 1, k + 0, B1, 2) - 1 * FX(i + 0, j + 1, k + 0, B2, 1) - 1 * FX(i + 0, j + 1, k - 1, B2, 1)
+                - 1 * FX(i + 0, j + 0, k + 0, B2, 1) - 1 * FX(i + 0, j + 0, k - 1, B2, 1));
+    }
+#undef FX
+#undef GX
+    memcpy(Fa, G, total * sizeof(double));
+    free(G);
+}
+
+/* MeshOperator::divergence (src/MeshOperator.cpp:323-381), uniform Cartesian. */
+void oracle_divergence(const double* F, const int s[3], int nq, double factor,
+                       const double dA[3], double dV, double* D)
+{
+    const int n0 = s[0] - 1, n1 = s[1] - 1, n2 = s[2] - 1;
+#define FF(i, j, k, q, a) F[((((long)(i) * s[1] + (j)) * s[2] + (k)) * nq + (q)) * 3 + (a)]
+    for (int i = 0; i < n0; ++i)
+    for (int j = 0; j < n1; ++j)
+    for (int k = 0; k < n2; ++k)
+    {
+        const double A1000 = dA[0], A0000 = dA[0];
+        const double A0101 = dA[1], A0001 = dA[1];
+        const double A0012 = dA[2], A0002 = dA[2];
+        const double vol = dV;
+        for (int q = 0; q < nq; ++q)
+            D[(((long)i * n1 + j) * n2 + k) * nq + q] = factor / vol * (
+                +FF(i + 1, j, k, q, 0) * A1000
+                -FF(i, j, k, q, 0) * A0000
+                +FF(i, j + 1, k, q, 1) * A0101
+                -FF(i, j, k, q, 1) * A0001
+                +FF(i, j, k + 1, q, 2) * A0012
+                -FF(i, j, k, q, 2) * A0002);
+    }
+#undef FF
+}
+
+/* CellCenteredFieldCT::monopole, cell location (src/CellCenteredFieldCT.cpp:159-229) */
+void oracle_monopole_cell(const double* B, const int s[3], double* Mc)
+{
+    const int m0 = s[0] + 1, m1 = s[1] + 1, m2 = s[2] + 1;
+    double* M = (double*)calloc((size_t)m0 * m1 * m2, sizeof(double));
+    const int r = s[0] > 1 ? 1 : 0, ss = s[1] > 1 ? 1 : 0, t = s[2] > 1 ? 1 : 0;
+#define BB(i, j, k, c) B[(((long)(i) * s[1] + (j)) * s[2] + (k)) * 3 + (c)]
+#define MM(i, j, k) M[((long)(i) * m1 + (j)) * m2 + (k)]
+    for (int i = 0; i < s[0] - r; ++i)
+    for (int j = 0; j < s[1] - ss; ++j)
+    for (int k = 0; k < s[2] - t; ++k)
+    {
+        const double B0x00 = BB(i + 0, j + 0, k + 0, 0);
+        const double B0x01 = BB(i + 0, j + 0, k + t, 0);
+        const double B0x10 = BB(i + 0, j + ss, k + 0, 0);
+        const double B0x11 = BB(i + 0, j + ss, k + t, 0);
+        const double B1x00 = BB(i + r, j + 0, k + 0, 0);
+        const double B1x01 = BB(i + r, j + 0, k + t, 0);
+        const double B1x10 = BB(i + r, j + ss, k + 0, 0);
+        const double B1x11 = BB(i + r, j + ss, k + t, 0);
+        const double B0y00 = BB(i + 0, j + 0, k + 0, 1);
+        const double B0y01 = BB(i + r, j + 0, k + 0, 1);
+        const double B0y10 = BB(i + 0, j + 0, k + t, 1);
+        const double B0y11 = BB(i + r, j + 0, k + t, 1);
+        const double B1y00 = BB(i + 0, j + ss, k + 0, 1);
+        const double B1y01 = BB(i + r, j + ss, k + 0, 1);
+        const double B1y10 = BB(i + 0, j + ss, k + t, 1);
+        const double B1y11 = BB(i + r, j + ss, k + t, 1);
+        const double B0z00 = BB(i + 0, j + 0, k + 0, 2);
+        const double B0z01 = BB(i + 0, j + ss, k + 0, 2);
+        const double B0z10 = BB(i + r, j + 0, k + 0, 2);
+        const double B0z11 = BB(i + r, j + ss, k + 0, 2);
+        const double B1z00 = BB(i + 0, j + 0, k + t, 2);
+        const double B1z01 = BB(i + 0, j + ss, k + t, 2);
+        const double B1z10 = BB(i + r, j + 0, k + t, 2);
+        const double B1z11 = BB(i + r, j + ss, k + t, 2);
+        const double Bx0 = B0x00 + B0x01 + B0x10 + B0x11;
+        const double Bx1 = B1x00 + B1x01 + B1x10 + B1x11;
+        const double By0 = B0y00 + B0y01 + B0y10 + B0y11;
+        const double By1 = B1y00 + B1y01 + B1y10 + B1y11;
+        const double Bz0 = B0z00 + B0z01 + B0z10 + B0z11;
+        const double Bz1 = B1z00 + B1z01 + B1z10 + B1z11;
+        MM(i + 1, j + 1, k + 1) = 0.25 * ((Bx1 - Bx0) + (By1 - By0) + (Bz1 - Bz0)) / 1.0;
+    }
+    for (int i = 0; i < s[0]; ++i)
+    for (int j = 0; j < s[1]; ++j)
+    for (int k = 0; k < s[2]; ++k)
+    {
+        const double Mi = MM(i, j, k) + MM(i + 1, j, k);
+        const double Mj = MM(i, j, k) + MM(i, j + 1, k);
+        const double Mk = MM(i, j, k) + MM(i, j, k + 1);
+        Mc[((long)i * s[1] + j) * s[2] + k] = 1. / 6 * (Mi + Mj + Mk);
+    }
+#undef BB
+#undef MM
+    free(M);
+}
+
+/* ========================================================================== */
+static void mesh_spacing(const mara_desc* d, double dx[3], double dA[3], double* dV)
+{
+    for (int n = 0; n < 3; ++n) dx[n] = (d->upper[n] - d->lower[n]) / d->cells[n];
+    dA[0] = dx[1] * dx[2];
+    dA[1] = dx[2] * dx[0];
+    dA[2] = dx[0] * dx[1];
+    *dV = dx[0] * dx[1] * dx[2];
+}
+
+void oracle_vector_potential_to_B(const mara_desc* d, const double* A, double* B)
+{
+    int N[3];
+    for (int a = 0; a < 3; ++a) N[a] = d->cells[a] + 2 * d->guards[a];
+    const int s[3] = {N[0] + 1, N[1] + 1, N[2] + 1};
+    const long nf = (long)s[0] * s[1] * s[2];
+    double* F = (double*)calloc(nf * 9, sizeof(double));
+    /* vectorPotentialToFluxes (src/CellCenteredFieldCT.cpp:136-150) */
+    for (long c = 0; c < nf; ++c)
+    {
+        const double* a = A + c * 9;   /* a[q*3 + axis] */
+        double* f = F + c * 9;
+        f[1 * 3 + 0] = -a[2 * 3 + 0];
+        f[2 * 3 + 0] = +a[1 * 3 + 0];
+        f[2 * 3 + 1] = -a[0 * 3 + 1];
+        f[0 * 3 + 1] = +a[2 * 3 + 1];
+        f[0 * 3 + 2] = -a[1 * 3 + 2];
+        f[1 * 3 + 2] = +a[0 * 3 + 2];
+    }
+    oracle_ct_correct(F, s, 3, 0);
+    double dx[3], dA[3], dV;
+    mesh_spacing(d, dx, dA, &dV);
+    oracle_divergence(F, s, 3, 1.0, dA, dV, B);
+    free(F);
+}
+
+/* ========================================================================== */
+/* BoundaryCondition::applySimple + Periodic/Outflow/Reflecting::apply
+   (src/BoundaryConditions.cpp:8-176).  Each axis with guards: left, then right,
+   over the full transverse extent. */
+int oracle_apply_bc(const mara_desc* d, double* P)
+{
+    layout l;
+    l.nq = law_nq(d->law);
+    for (int a = 0; a < 3; ++a) l.N[a] = d->cells[a] + 2 * d->guards[a];
+    for (int axis = 0; axis < 3; ++axis)
+    {
+        const int g = d->guards[axis];
+        if (g <= 0) continue;
+        if (l.N[axis] == 1) return MARA_ERR_INVALID_ARGUMENT;
+        const int N = l.N[axis];
+        for (int side = 0; side < 2; ++side)
+        for (int n = 0; n < g; ++n)
+        {
+            int dst, src;
+            double sign = 1.0;
+            if (d->bc == MARA_BC_PERIODIC)
+            {
+                dst = side == 0 ? n : N - g + n;
+                src = side == 0 ? N - 2 * g + n : g + n;
+            }
+            else if (d->bc == MARA_BC_OUTFLOW)
+            {
+                dst = side == 0 ? n : N - 1 - n;
+                src = side == 0 ? g : N - g - 1;
+            }
+            else
+            {
+                dst = side == 0 ? n : N - 1 - n;
+                src = side == 0 ? 2 * g - n - 1 : N - 2 * g + n;
+                sign = -1.0;
+            }
+            int lo[3] = {0, 0, 0}, hi[3] = {l.N[0], l.N[1], l.N[2]};
+            lo[axis] = 0; hi[axis] = 1;
+            for (int i = lo[0]; i < hi[0]; ++i)
+            for (int j = lo[1]; j < hi[1]; ++j)
+            for (int k = lo[2]; k < hi[2]; ++k)
+            {
+                int I[3] = {i, j, k}, J[3] = {i, j, k};
+                I[axis] = dst;
+                J[axis] = src;
+                double* pd = P + cidx(&l, I[0], I[1], I[2]);
+                const double* ps = P + cidx(&l, J[0], J[1], J[2]);
+                for (int q = 0; q < l.nq; ++q) pd[q] = ps[q];
+                if (sign < 0) pd[1 + axis] *= -1;
+            }
+        }
+    }
+    return MARA_OK;
+}
+
+/* ========================================================================== */
+/* MethodOfLinesTVD::advance (src/SolutionSchemes.cpp:108-275). */
+int oracle_advance(const mara_desc* d, double* P, double t0, double dt, mara_report* rep)
+{
+    (void)t0;
+    layout l;
+    const int nq = law_nq(d->law);
+    const int ng = stencil_size(d->recon);
+    int fp[3];
+    l.nq = nq;
+    for (int a = 0; a < 3; ++a) l.N[a] = d->cells[a] + 2 * d->guards[a];
+    if (rep) memset(rep, 0, sizeof *rep);
+
+    /* SchemeHelpers::makeFootprint (src/SolutionSchemes.cpp:17-49) */
+    for (int a = 0; a < 3; ++a)
+    {
+        fp[a] = l.N[a] > 1 ? 2 * ng : 0;
+        if (fp[a] / 2 > d->guards[a])
+        {
+            if (rep) rep->status = MARA_ERR_GUARD_TOO_SMALL;
+            return MARA_ERR_GUARD_TOO_SMALL;
+        }
+    }
+    const double b1[1] = {1.0};
+    const double b2[2] = {1.0, 0.5};
+    const double b3[3] = {1.0, 1. / 4, 2. / 3};
+    const double* b = d->rk_order == 1 ? b1 : d->rk_order == 2 ? b2 : b3;
+
+    const long ncell = (long)l.N[0] * l.N[1] * l.N[2];
+    const long nface = (long)(l.N[0] + 1) * (l.N[1] + 1) * (l.N[2] + 1);
+    double* U0 = (double*)malloc(ncell * nq * sizeof(double));
+    double* U = (double*)malloc(ncell * nq * sizeof(double));
+    double* L = (double*)malloc(ncell * nq * sizeof(double));
+    double* F = (double*)malloc(nface * nq * 3 * sizeof(double));
+    double nhat[9];
+    oracle_unit_vectors(nhat);
+    int status = MARA_OK;
+    const double xhat[3] = {1.0, 0.0, 0.0}; /* Request() default areaElement */
+
+    /* generateConserved over the whole array (src/FieldOperator.cpp:91-116) */
+    for (long c = 0; c < ncell && status == MARA_OK; ++c)
+    {
+        ostate S;
+        int e = from_primitive(d->law, d->gamma, P + c * nq, xhat, &S);
+        if (e) { status = e; break; }
+        for (int q = 0; q < nq; ++q) U0[c * nq + q] = S.U[q];
+    }
+    if (status == MARA_OK) memcpy(U, U0, ncell * nq * sizeof(double));
+
+    double dx[3], dA[3], dV;
+    mesh_spacing(d, dx, dA, &dV);
+    const int fs[3] = {l.N[0] + 1, l.N[1] + 1, l.N[2] + 1};
+
+    for (int rk = 0; rk < d->rk_order && status == MARA_OK; ++rk)
+    {
+        memset(F, 0, nface * nq * 3 * sizeof(double));
+
+        /* MeshOperator::godunov (src/MeshOperator.cpp:443-525) */
+        for (int sweep = 0; sweep < 3 && status == MARA_OK; ++sweep)
+        {
+            if (fp[sweep] == 0) continue;
+            int hi[3] = {l.N[0], l.N[1], l.N[2]};
+            hi[sweep] -= fp[sweep] - 1;
+            int err = MARA_OK;
+            #pragma omp parallel for collapse(2) schedule(static) reduction(max:err)
+            for (int i = 0; i < hi[0]; ++i)
+            for (int j = 0; j < hi[1]; ++j)
+            for (int k = 0; k < hi[2]; ++k)
+            {
+                double S[6 * NF];
+                for (int m = 0; m < fp[sweep]; ++m)
+                {
+                    int I[3] = {i, j, k};
+                    I[sweep] += m;
+                    const double* pc = P + cidx(&l, I[0], I[1], I[2]);
+                    for (int q = 0; q < nq; ++q) S[m * nq + q] = pc[q];
+                }
+                double Fq[NF];
+                int e = intercell_flux(d, nq, S, nhat + 3 * sweep, Fq);
+                if (e) { if (e > err) err = e; continue; }
+                int I[3] = {i, j, k};
+                I[sweep] += fp[sweep] / 2;
+                for (int q = 0; q < nq; ++q) F[fidx(&l, I[0], I[1], I[2], q, sweep)] = Fq[q];
+            }
+            if (err) status = err;
+        }
+        if (status) break;
+
+        /* fluxCorrection: flux-CT when the law has B (src/SolutionSchemes.cpp:173-187) */
+        if (d->law != MARA_LAW_HYDRO && d->field_ct)
+            oracle_ct_correct(F, fs, nq, 5);
+
+        /* L = divergence(F, -1), U = U0 (1-b) + (U + dt L) b */
+        oracle_divergence(F, fs, nq, -1.0, dA, dV, L);
+        for (long n = 0; n < ncell * nq; ++n)
+            U[n] = U0[n] * (1 - b[rk]) + (U[n] + dt * L[n]) * b[rk];
+
+        /* recoverPrimitive on the interior (src/FieldOperator.cpp:41-82) */
+        long fails = 0, floors = 0, first = -1;
+        const int g0 = d->guards[0], g1 = d->guards[1], g2 = d->guards[2];
+        for (int i = g0; i < l.N[0] - g0; ++i)
+        for (int j = g1; j < l.N[1] - g1; ++j)
+        for (int k = g2; k < l.N[2] - g2; ++k)
+        {
+            const long c = cidx(&l, i, j, k);
+            double Pq[NF];
+            int health;
+            int e = from_conserved(d->law, d->gamma, d->pressure_floor, U + c, Pq, &health);
+            if (e)
+            {
+                if (first < 0 && rep)
+                {
+                    rep->first_failure[0] = i - g0;
+                    rep->first_failure[1] = j - g1;
+                    rep->first_failure[2] = k - g2;
+                }
+                if (first < 0) first = c;
+                ++fails;
+                continue;
+            }
+            floors += health;
+            for (int q = 0; q < nq; ++q) P[c + q] = Pq[q];
+        }
+        if (rep) { rep->c2p_failures += fails; rep->floor_applied += floors; }
+        if (fails) { status = MARA_ERR_INVERSION_FAILURE; break; }
+
+        /* MeshData::applyBoundaryCondition (src/MeshData.cpp:148-151) */
+        oracle_apply_bc(d, P);
+    }
+    free(U0); free(U); free(L); free(F);
+    if (rep) rep->status = status;
+    return status;
+}
+
+/* ========================================================================== */
+/* FieldOperator::getCourantTimestep (src/FieldOperator.cpp:125-169) */
+int oracle_courant_dt(const mara_desc* d, const double* P, double* dt_out)
+{
+    layout l;
+    l.nq = law_nq(d->law);
+    for (int a = 0; a < 3; ++a) l.N[a] = d->cells[a] + 2 * d->guards[a];
+    double dx[3], dA[3], dV;
+    mesh_spacing(d, dx, dA, &dV);
+    /* linearCellDimension: *std::min_element of (dx, dy, dz) */
+    double Lc = dx[0];
+    if (dx[1] < Lc) Lc = dx[1];
+    if (dx[2] < Lc) Lc = dx[2];
+    double nhat[9];
+    oracle_unit_vectors(nhat);
+    double courant = DBL_MAX;
+    int status = MARA_OK;
+    const int g0 = d->guards[0], g1 = d->guards[1], g2 = d->guards[2];
+    for (int i = g0; i < l.N[0] - g0; ++i)
+    for (int j = g1; j < l.N[1] - g1; ++j)
+    for (int k = g2; k < l.N[2] - g2; ++k)
+    {
+        const double* pc = P + cidx(&l, i, j, k);
+        double lam[3];
+        for (int a = 0; a < 3; ++a)
+        {
+            ostate S;
+            int e = from_primitive(d->law, d->gamma, pc, nhat + 3 * a, &S);
+            if (e) { status = e; lam[a] = 0; continue; }
+            double m = 0.0;
+            for (int n = 0; n < l.nq; ++n)
+                if (m < fabs(S.A[n])) m = fabs(S.A[n]);
+            lam[a] = m;
+        }
+        double lmax = lam[0];
+        if (lmax < lam[1]) lmax = lam[1];
+        if (lmax < lam[2]) lmax = lam[2];
+        const double dtc = Lc / lmax;
+        if (dtc < courant) courant = dtc;
+    }
+    *dt_out = courant;
+    return status;
+}
+
+int oracle_monopole_max(const mara_desc* d, const double* P, double* out)
+{
+    if (d->law == MARA_LAW_HYDRO) return MARA_ERR_UNSUPPORTED;
+    int N[3];
+    for (int a = 0; a < 3; ++a) N[a] = d->cells[a] + 2 * d->guards[a];
+    const long nc = (long)N[0] * N[1] * N[2];
+    double* B = (double*)malloc(nc * 3 * sizeof(double));
+    double* M = (double*)malloc(nc * sizeof(double));
+    for (long c = 0; c < nc; ++c)
+        for (int q = 0; q < 3; ++q) B[c * 3 + q] = P[c * 8 + 5 + q];
+    oracle_monopole_cell(B, N, M);
+    double m = 0.0;
+    for (int i = d->guards[0]; i < N[0] - d->guards[0]; ++i)
+    for (int j = d->guards[1]; j < N[1] - d->guards[1]; ++j)
+    for (int k = d->guards[2]; k < N[2] - d->guards[2]; ++k)
+    {
+        const double v = fabs(M[((long)i * N[1] + j) * N[2] + k]);
+        if (v > m) m = v;
+    }
+    *out = m;
+    free(B);
+    free(M);
+    return MARA_OK;
+}
