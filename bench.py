@@ -1,0 +1,175 @@
+#!/usr/bin/env python3
+"""Throughput of the MI355X Godunov step on the 3-D decaying-turbulence box.
+
+Workload: 256^3 zones, periodic unit cube, gamma 5/3, rho = 1, p = 1, seeded
+Gaussian random velocity field on 1 <= |k| <= 16 with |v_k|^2 ~ k^-4 at rms
+Mach 1; PLM (theta 1.5) + HLLC, RK2, CFL 0.3 (mara2_amd.problems.turbulence_3d).
+
+One step = courant reduction + two RK stages + ghost fills, driven on the
+device by mara_block_run (dt = 0.3 * courant each step).  The state is
+resident in HBM before the timed region starts; the call returns after the
+stream has drained.  Prints one JSON line (value = plain FP64 form; the
+mixed form, its speed-up and its L1 distance to FP64 ride along).
+
+    python bench.py [--gpus 1] [--steps 100] [--warmup 5]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from mara2_amd import abi, problems  # noqa: E402
+from mara2_amd.runtime import Block  # noqa: E402
+
+METRIC = "zone-updates/s, FP32-mixed vs FP64 at stated L1 error; solves/s + p99 latency"
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+CFL = 0.3
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def alg_bytes_per_zone_update(desc):
+    """(3s - 1) * nq * 8: each stage reads the state and writes the new one,
+    stages after the first also read the step's starting state."""
+    s = desc.rk_order
+    return (3 * s - 1) * abi.num_fields(desc.law) * 8
+
+
+def traffic_from_profile(kernel, workload):
+    """HBM bytes per launch measured by rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE,
+    committed under profiles/ (null when no such summary exists)."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def time_form(desc, P0, steps, warmup):
+    """Upload, warm up, then time `steps` device-driven steps."""
+    with Block(desc) as blk:
+        blk.upload(P0)
+        if warmup:
+            blk.run(CFL, 1e300, warmup)
+            blk.upload(P0)
+        t0 = time.perf_counter()
+        t, taken, rep = blk.run(CFL, 1e300, steps)
+        t1 = time.perf_counter()
+        if taken != steps:
+            raise RuntimeError(f"run stopped after {taken} of {steps} steps: {rep.as_dict()}")
+        P = blk.download()
+        # stage-kernel duration from HIP events on the block's stream, outside
+        # the timed region (event timing synchronises after each stage)
+        blk.set_timing(True)
+        blk.run(CFL, 1e300, 4)
+        avg_ms, launches, name = blk.stage_kernel_time()
+    return t1 - t0, P, avg_ms, launches, name
+
+
+def cpu_baseline(n_sample, target_s=15.0):
+    """The oracle (CPU restatement of the reference step) on the same workload
+    at n_sample^3, OpenMP threads = host cores."""
+    import oracle
+    threads = min(os.cpu_count() or 1, 64)
+    oracle.set_threads(threads)
+    desc = problems.turbulence_desc(n_sample)
+    P = problems.turbulence_3d(desc)
+    zones = n_sample ** 3
+    err, dtc = oracle.courant_dt(desc, P)
+    oracle.advance(desc, P, 0.0, CFL * dtc)  # warm-up step
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        err, dtc = oracle.courant_dt(desc, P)
+        err, _ = oracle.advance(desc, P, 0.0, CFL * dtc)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el > target_s or steps >= 50:
+            break
+    return {"value": zones * steps / el, "unit": "zone-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"turbulence_3d at {n_sample}^3 (same generator and scheme), {steps} steps after 1 warm-up "
+                      f"step, {el:.1f} s, CPU restatement of the reference step (oracle/), OpenMP over zones"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--cpu-sample", type=int, default=64)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(independent solves shard by block, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    n = args.n
+    d64 = problems.turbulence_desc(n, "fp64")
+    dmx = problems.turbulence_desc(n, "mixed")
+    log(f"generating {n}^3 initial data")
+    P0 = problems.turbulence_3d(d64)
+    zones = n ** 3
+
+    log("timing FP64 form")
+    el64, P64, ms64, nl64, kname = time_form(d64, P0, args.steps, args.warmup)
+    log(f"FP64: {el64:.3f} s for {args.steps} steps; stage kernel {ms64:.3f} ms ({kname})")
+    log("timing mixed form")
+    elmx, Pmx, msmx, nlmx, _ = time_form(dmx, P0, args.steps, args.warmup)
+    log(f"mixed: {elmx:.3f} s for {args.steps} steps; stage kernel {msmx:.3f} ms")
+
+    g = d64.guards[0]
+    inner = (slice(g, g + n),) * 3
+    U64 = P64[inner]
+    Umx = Pmx[inner]
+    l1 = max(float(np.sum(np.abs(Umx[..., q] - U64[..., q])) / np.sum(np.abs(U64[..., q]))) for q in range(5))
+
+    value = zones * args.steps / el64
+    b_alg = alg_bytes_per_zone_update(d64)
+    per_launch = b_alg * zones / d64.rk_order
+    achieved = per_launch / (ms64 * 1e-3)
+    workload = f"turbulence3d_{n}"
+    roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK, "traffic": traffic_from_profile("stage_fp64", workload),
+                "kernel": kname, "avg_launch_ms": ms64, "launches": nl64,
+                "alg_bytes_per_launch": per_launch}
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el64 / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": f"synthetic: seeded Gaussian random velocity field (seed {problems.SEED_TURBULENCE})",
+        "config": {"workload": f"3-D decaying turbulence {n}^3, hydro gamma 5/3, PLM theta 1.5 + HLLC, "
+                               f"RK2, CFL {CFL}, periodic", "zones": zones, "form": "fp64"},
+        "roofline": roofline,
+        "mixed": {"value": zones * args.steps / elmx, "ms_per_step": elmx / args.steps * 1e3,
+                  "speedup_vs_fp64": el64 / elmx, "rel_l1_vs_fp64": l1,
+                  "avg_stage_launch_ms": msmx,
+                  "roofline_frac": per_launch / (msmx * 1e-3) / HBM_PEAK},
+    }
+    if not args.no_cpu_baseline:
+        log("CPU baseline")
+        out["cpu_baseline"] = cpu_baseline(args.cpu_sample)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

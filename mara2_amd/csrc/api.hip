@@ -1,0 +1,639 @@
+// api.hip -- host side of the C ABI declared in include/mara_gpu.h.
+//
+// Owns the device state of one mesh block and enqueues the per-stage kernels
+// on the block's stream.  Device layout and the stage structure are described
+// in kernels.hpp; the fused hydro stage kernel lives in fused.hpp.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "../../include/mara_gpu.h"
+#include "kernels.hpp"
+#include "fused.hpp"
+
+using namespace mara;
+
+namespace {
+
+constexpr int kThreads = 256;
+
+long long round_up(long long x, long long m) { return (x + m - 1) / m * m; }
+
+unsigned grid_for(long long n) { return (unsigned)((n + kThreads - 1) / kThreads); }
+
+bool hip_ok(hipError_t e)
+{
+    if (e != hipSuccess)
+    {
+        std::fprintf(stderr, "mara_gpu: HIP error %d (%s)\n", (int)e, hipGetErrorString(e));
+        return false;
+    }
+    return true;
+}
+
+#define HIP_TRY(call) do { if (!hip_ok(call)) return MARA_ERR_HIP; } while (0)
+
+int law_nq(int law) { return law == MARA_LAW_HYDRO ? 5 : 8; }
+int recon_ng(int recon) { return recon == MARA_RECON_PCM ? 1 : recon == MARA_RECON_PLM ? 2 : 3; }
+
+// UnitVector::xhat/yhat/zhat cartesian components (src/UnitVector.cpp:9-39):
+// (sin(acos mu) cos phi, sin(acos mu) sin phi, mu) with (mu,phi) = (0,0),
+// (0, pi/2), (1, 0), evaluated with the host libm like the reference.
+void unit_vectors(double out[9])
+{
+    const double mu[3] = {0.0, 0.0, 1.0};
+    const double phi[3] = {0.0, M_PI / 2, 0.0};
+    for (int a = 0; a < 3; ++a)
+    {
+        const double st = std::sqrt(1 - mu[a] * mu[a]);
+        out[3 * a + 0] = st * std::cos(phi[a]);
+        out[3 * a + 1] = st * std::sin(phi[a]);
+        out[3 * a + 2] = mu[a];
+    }
+}
+
+int validate(const mara_desc* d)
+{
+    if (!d) return MARA_ERR_INVALID_ARGUMENT;
+    if (d->law < 0 || d->law > 2) return MARA_ERR_INVALID_ARGUMENT;
+    if (d->recon < 0 || d->recon > 2) return MARA_ERR_INVALID_ARGUMENT;
+    if (d->riemann < 0 || d->riemann > 1) return MARA_ERR_INVALID_ARGUMENT;
+    if (d->bc < 0 || d->bc > 2) return MARA_ERR_INVALID_ARGUMENT;
+    if (d->rk_order < 1 || d->rk_order > 3) return MARA_ERR_INVALID_ARGUMENT;
+    if (d->precision < 0 || d->precision > 1) return MARA_ERR_INVALID_ARGUMENT;
+    if (d->riemann == MARA_RIEMANN_HLLC && d->law != MARA_LAW_HYDRO) return MARA_ERR_UNSUPPORTED;
+    const int ng = recon_ng(d->recon);
+    for (int a = 0; a < 3; ++a)
+    {
+        if (d->cells[a] < 1 || d->guards[a] < 0) return MARA_ERR_INVALID_ARGUMENT;
+        if (!(d->upper[a] > d->lower[a])) return MARA_ERR_INVALID_ARGUMENT;
+        const int N = d->cells[a] + 2 * d->guards[a];
+        // SchemeHelpers::makeFootprint (src/SolutionSchemes.cpp:45-48)
+        if (N > 1 && d->guards[a] < ng) return MARA_ERR_GUARD_TOO_SMALL;
+        // the ghost fill reads only interior zones
+        if (d->guards[a] > 0 && d->cells[a] < d->guards[a]) return MARA_ERR_UNSUPPORTED;
+    }
+    if (d->cells[0] == 1 && (d->cells[1] > 1 || d->cells[2] > 1)) return MARA_ERR_UNSUPPORTED;
+    return MARA_OK;
+}
+
+struct Sizes
+{
+    long long ncell, fs, nface, ffs, real;
+    int nq;
+    bool ct;
+    int fleshed[3];
+};
+
+Sizes sizes_of(const mara_desc* d)
+{
+    Sizes s;
+    int N[3];
+    for (int a = 0; a < 3; ++a)
+    {
+        N[a] = d->cells[a] + 2 * d->guards[a];
+        s.fleshed[a] = N[a] > 1;
+    }
+    s.nq = law_nq(d->law);
+    s.ncell = (long long)N[0] * N[1] * N[2];
+    s.fs = round_up(s.ncell, 64);
+    s.nface = (long long)(N[0] + 1) * (N[1] + 1) * (N[2] + 1);
+    s.ffs = round_up(s.nface, 64);
+    s.real = d->precision == MARA_PREC_MIXED ? 4 : 8;
+    s.ct = d->law != MARA_LAW_HYDRO && d->field_ct;
+    return s;
+}
+
+long long device_bytes(const mara_desc* d, const Sizes& s)
+{
+    long long b = 3ll * s.nq * s.fs * 8;  // Pa, Pb, U
+    if (fused_supported(d) && d->rk_order == 3) b += 1ll * s.nq * s.fs * 8;  // Pc
+    if (!fused_supported(d))
+        b += (long long)(s.fleshed[0] + s.fleshed[1] + s.fleshed[2]) * s.nq * s.ffs * s.real;
+    if (s.ct) b += 9ll * s.ffs * 8;
+    b += 256;  // control block
+    return b;
+}
+
+} // namespace
+
+struct mara_block
+{
+    mara_desc d;
+    Geom G;
+    Sizes S;
+    int ng;
+    bool fused;
+    hipStream_t stream = nullptr;
+    double* Pa = nullptr;
+    double* Pb = nullptr;
+    double* Pc = nullptr;  // third stage buffer of the fused RK3 schedule
+    double* U = nullptr;
+    void* F[3] = {nullptr, nullptr, nullptr};
+    double* Gct = nullptr;
+    Ctl* ctl = nullptr;
+    Ctl* ctl_host = nullptr;
+    long long bytes = 0;
+    bool timing = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double timed_ms = 0.0;
+    long long timed_launches = 0;
+};
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// template dispatch
+// ---------------------------------------------------------------------------
+template <int Law, int Recon, int Riem, typename R>
+void launch_flux_t(mara_block* b, const double* Pin, int axis)
+{
+    const Geom& G = b->G;
+    int lo[3], hi[3];
+    for (int a = 0; a < 3; ++a)
+    {
+        if (a == axis) { lo[a] = G.g[a]; hi[a] = G.N[a] - G.g[a] + 1; }
+        else
+        {
+            const int ring = (b->S.ct && G.N[a] > 1) ? 1 : 0;
+            lo[a] = G.g[a] - ring;
+            hi[a] = G.N[a] - G.g[a] + ring;
+        }
+    }
+    const int m0 = hi[0] - lo[0], m1 = hi[1] - lo[1], m2 = hi[2] - lo[2];
+    const long long total = (long long)m0 * m1 * m2;
+    k_flux<Law, Recon, Riem, R><<<grid_for(total), kThreads, 0, b->stream>>>(
+        Pin, (R*)b->F[axis], G, axis, lo[0], lo[1], lo[2], m0, m1, m2, b->ctl, b->ctl);
+}
+
+template <int Law, typename R>
+void launch_flux_law(mara_block* b, const double* Pin, int axis)
+{
+    const int recon = b->d.recon, riem = b->d.riemann;
+    if (recon == MARA_RECON_PCM)
+    {
+        if (Law == 0 && riem == 1) launch_flux_t<Law, 0, 1, R>(b, Pin, axis);
+        else launch_flux_t<Law, 0, 0, R>(b, Pin, axis);
+    }
+    else if (recon == MARA_RECON_PLM)
+    {
+        if (Law == 0 && riem == 1) launch_flux_t<Law, 1, 1, R>(b, Pin, axis);
+        else launch_flux_t<Law, 1, 0, R>(b, Pin, axis);
+    }
+    else
+    {
+        if (Law == 0 && riem == 1) launch_flux_t<Law, 2, 1, R>(b, Pin, axis);
+        else launch_flux_t<Law, 2, 0, R>(b, Pin, axis);
+    }
+}
+
+template <typename R>
+void launch_flux(mara_block* b, const double* Pin, int axis)
+{
+    switch (b->d.law)
+    {
+        case 0: launch_flux_law<0, R>(b, Pin, axis); break;
+        case 1: launch_flux_law<1, R>(b, Pin, axis); break;
+        default: launch_flux_law<2, R>(b, Pin, axis); break;
+    }
+}
+
+template <typename R>
+void launch_ct(mara_block* b)
+{
+    const Geom& G = b->G;
+    for (int a = 0; a < 3; ++a)
+    {
+        if (G.N[a] < 2) continue;
+        int lo[3], m[3];
+        for (int e = 0; e < 3; ++e)
+        {
+            lo[e] = G.g[e];
+            m[e] = G.n[e] + (e == a ? 1 : 0);
+        }
+        const long long total = (long long)m[0] * m[1] * m[2];
+        k_ct<R><<<grid_for(total), kThreads, 0, b->stream>>>(
+            (const R*)b->F[0], (const R*)b->F[1], (const R*)b->F[2], b->Gct, G, a,
+            lo[0], lo[1], lo[2], m[0], m[1], m[2], b->ctl);
+    }
+}
+
+template <int Law, typename R>
+void launch_update_law(mara_block* b, double* Pout, double bk, int first, int write_u)
+{
+    const Geom& G = b->G;
+    const long long total = (long long)G.n[0] * G.n[1] * G.n[2];
+    if (b->S.ct)
+        k_update<Law, R, true><<<grid_for(total), kThreads, 0, b->stream>>>(
+            b->Pa, b->U, Pout, (const R*)b->F[0], (const R*)b->F[1], (const R*)b->F[2], b->Gct,
+            G, bk, first, write_u, b->ctl);
+    else
+        k_update<Law, R, false><<<grid_for(total), kThreads, 0, b->stream>>>(
+            b->Pa, b->U, Pout, (const R*)b->F[0], (const R*)b->F[1], (const R*)b->F[2], b->Gct,
+            G, bk, first, write_u, b->ctl);
+}
+
+template <typename R>
+void launch_update(mara_block* b, double* Pout, double bk, int first, int write_u)
+{
+    switch (b->d.law)
+    {
+        case 0: launch_update_law<0, R>(b, Pout, bk, first, write_u); break;
+        case 1: launch_update_law<1, R>(b, Pout, bk, first, write_u); break;
+        default: launch_update_law<2, R>(b, Pout, bk, first, write_u); break;
+    }
+}
+
+void launch_bc(mara_block* b, double* P)
+{
+    const Geom& G = b->G;
+    for (int axis = 0; axis < 3; ++axis)
+    {
+        if (G.g[axis] <= 0) continue;
+        const int t1 = axis == 0 ? 1 : 0;
+        const int t2 = axis == 2 ? 1 : 2;
+        const long long total = 2ll * G.g[axis] * G.N[t1] * G.N[t2];
+        k_bc<<<grid_for(total), kThreads, 0, b->stream>>>(P, G, axis, b->d.bc, b->ctl);
+    }
+}
+
+void launch_courant(mara_block* b)
+{
+    const Geom& G = b->G;
+    const long long total = (long long)G.n[0] * G.n[1] * G.n[2];
+    long long blocks = (total + kThreads - 1) / kThreads;
+    if (blocks > 2048) blocks = 2048;  // 8 work-groups per CU, grid-stride beyond
+    const unsigned nb = (unsigned)blocks;
+    switch (b->d.law)
+    {
+        case 0: k_courant<0><<<nb, kThreads, 0, b->stream>>>(b->Pa, G, b->ctl); break;
+        case 1: k_courant<1><<<nb, kThreads, 0, b->stream>>>(b->Pa, G, b->ctl); break;
+        default: k_courant<2><<<nb, kThreads, 0, b->stream>>>(b->Pa, G, b->ctl); break;
+    }
+}
+
+// One RK stage: fluxes, CT, update, ghost fill.
+void launch_stage(mara_block* b, int rk)
+{
+    static const double b1[1] = {1.0};
+    static const double b2[2] = {1.0, 0.5};
+    static const double b3[3] = {1.0, 1. / 4, 2. / 3};
+    const double* bb = b->d.rk_order == 1 ? b1 : b->d.rk_order == 2 ? b2 : b3;
+    const int last = rk == b->d.rk_order - 1;
+    const double* Pin = rk == 0 ? b->Pa : b->Pb;
+    double* Pout = last ? b->Pa : b->Pb;
+    const bool mixed = b->d.precision == MARA_PREC_MIXED;
+    if (b->fused)
+    {
+        // the fused kernel reads neighbouring zones of its input while other
+        // work-groups write their output, so input and output never alias:
+        // RK1 Pa -> Pb (then swap), RK2 Pa -> Pb -> Pa, RK3 Pa -> Pb -> Pc -> Pa
+        const int order = b->d.rk_order;
+        Pin = rk == 0 ? b->Pa : (rk == 1 ? b->Pb : b->Pc);
+        Pout = order == 1 ? b->Pb : (last ? b->Pa : (rk == 0 ? b->Pb : b->Pc));
+    }
+
+    if (b->timing) hipEventRecord(b->ev0, b->stream);
+    if (b->fused)
+    {
+        launch_fused(b->d, b->G, b->stream, Pin, b->Pa, b->U, Pout, bb[rk], rk == 0, !last, b->ctl);
+    }
+    else
+    {
+        for (int axis = 0; axis < 3; ++axis)
+        {
+            if (b->G.N[axis] < 2) continue;
+            if (mixed) launch_flux<float>(b, Pin, axis);
+            else launch_flux<double>(b, Pin, axis);
+        }
+        if (b->S.ct)
+        {
+            if (mixed) launch_ct<float>(b);
+            else launch_ct<double>(b);
+        }
+        if (mixed) launch_update<float>(b, Pout, bb[rk], rk == 0, !last);
+        else launch_update<double>(b, Pout, bb[rk], rk == 0, !last);
+    }
+    if (b->timing)
+    {
+        hipEventRecord(b->ev1, b->stream);
+        hipEventSynchronize(b->ev1);
+        float ms = 0;
+        hipEventElapsedTime(&ms, b->ev0, b->ev1);
+        b->timed_ms += ms;
+        b->timed_launches += 1;
+    }
+    launch_bc(b, Pout);
+    if (b->fused && b->d.rk_order == 1)
+    {
+        double* t = b->Pa;
+        b->Pa = b->Pb;
+        b->Pb = t;
+    }
+}
+
+int read_ctl(mara_block* b)
+{
+    HIP_TRY(hipMemcpyAsync(b->ctl_host, b->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return MARA_OK;
+}
+
+int fill_report(mara_block* b, mara_report* r)
+{
+    const Ctl& c = *b->ctl_host;
+    int status = MARA_OK;
+    if (c.superluminal) status = MARA_ERR_SUPERLUMINAL;
+    else if (c.eigen_fail) status = MARA_ERR_BAD_EIGENVALUES;
+    else if (c.c2p_fail) status = MARA_ERR_INVERSION_FAILURE;
+    if (r)
+    {
+        r->status = status;
+        r->c2p_failures = (long long)c.c2p_fail;
+        r->floor_applied = (long long)c.floor_applied;
+        r->eigen_failures = (long long)c.eigen_fail;
+        r->superluminal = (long long)c.superluminal;
+        r->first_failure[0] = r->first_failure[1] = r->first_failure[2] = -1;
+        if (c.c2p_fail)
+        {
+            const unsigned long long lin = c.first_fail;
+            const unsigned long long n1 = b->G.n[1], n2 = b->G.n[2];
+            r->first_failure[2] = (int)(lin % n2);
+            r->first_failure[1] = (int)((lin / n2) % n1);
+            r->first_failure[0] = (int)(lin / (n1 * n2));
+        }
+    }
+    return status;
+}
+
+} // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char* mara_version(void) { return "mara2-amd 0.1 (gfx950)"; }
+
+const char* mara_error_string(int status)
+{
+    switch (status)
+    {
+        case MARA_OK: return "ok";
+        case MARA_ERR_INVALID_ARGUMENT: return "invalid argument";
+        case MARA_ERR_GUARD_TOO_SMALL: return "Boundary region of mesh data is smaller than the scheme's stencil";
+        case MARA_ERR_UNSUPPORTED: return "configuration not supported";
+        case MARA_ERR_HIP: return "HIP runtime error";
+        case MARA_ERR_INVERSION_FAILURE: return "conserved to primitive inversion failed";
+        case MARA_ERR_SUPERLUMINAL: return "got superluminal velocity";
+        case MARA_ERR_BAD_EIGENVALUES: return "got bad eigenvalues";
+        case MARA_ERR_NO_DEVICE: return "no usable GPU device";
+        case MARA_ERR_CAPACITY: return "request exceeds capacity";
+        default: return "unknown status";
+    }
+}
+
+int mara_num_fields(int law) { return law_nq(law); }
+int mara_stencil_size(int recon) { return recon_ng(recon); }
+
+long long mara_desc_device_bytes(const mara_desc* d)
+{
+    if (validate(d) != MARA_OK) return -1;
+    return device_bytes(d, sizes_of(d));
+}
+
+int mara_block_create(const mara_desc* d, mara_block** out)
+{
+    if (!out) return MARA_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    int v = validate(d);
+    if (v) return v;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MARA_ERR_NO_DEVICE;
+
+    mara_block* b = new (std::nothrow) mara_block();
+    if (!b) return MARA_ERR_HIP;
+    b->d = *d;
+    b->S = sizes_of(d);
+    b->ng = recon_ng(d->recon);
+    b->fused = fused_supported(d);
+    Geom& G = b->G;
+    for (int a = 0; a < 3; ++a)
+    {
+        G.N[a] = d->cells[a] + 2 * d->guards[a];
+        G.g[a] = d->guards[a];
+        G.n[a] = d->cells[a];
+    }
+    G.nq = b->S.nq;
+    G.fs = b->S.fs;
+    G.ffs = b->S.ffs;
+    G.gamma = d->gamma;
+    G.pfloor = d->pressure_floor;
+    G.theta = d->plm_theta;
+    // CartesianMeshGeometry::cacheSpacing (src/CartesianMeshGeometry.cpp:145-157)
+    double dx[3];
+    for (int a = 0; a < 3; ++a) dx[a] = (d->upper[a] - d->lower[a]) / d->cells[a];
+    G.dA[0] = dx[1] * dx[2];
+    G.dA[1] = dx[2] * dx[0];
+    G.dA[2] = dx[0] * dx[1];
+    G.dV = dx[0] * dx[1] * dx[2];
+    // MeshOperator::linearCellDimension: std::min_element of the three lengths
+    double L = dx[0];
+    if (dx[1] < L) L = dx[1];
+    if (dx[2] < L) L = dx[2];
+    G.Lmin = L;
+    unit_vectors(G.nhat);
+
+    auto fail = [&](int code) { mara_block_destroy(b); return code; };
+    if (!hip_ok(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking))) return fail(MARA_ERR_HIP);
+    const size_t cellbytes = (size_t)b->S.nq * b->S.fs * sizeof(double);
+    if (!hip_ok(hipMalloc(&b->Pa, cellbytes))) return fail(MARA_ERR_HIP);
+    if (!hip_ok(hipMalloc(&b->Pb, cellbytes))) return fail(MARA_ERR_HIP);
+    if (!hip_ok(hipMalloc(&b->U, cellbytes))) return fail(MARA_ERR_HIP);
+    if (b->fused && d->rk_order == 3)
+    {
+        if (!hip_ok(hipMalloc(&b->Pc, cellbytes))) return fail(MARA_ERR_HIP);
+        if (!hip_ok(hipMemset(b->Pc, 0, cellbytes))) return fail(MARA_ERR_HIP);
+    }
+    if (!b->fused)
+    {
+        for (int a = 0; a < 3; ++a)
+        {
+            if (!b->S.fleshed[a]) continue;
+            const size_t fb = (size_t)b->S.nq * b->S.ffs * b->S.real;
+            if (!hip_ok(hipMalloc(&b->F[a], fb))) return fail(MARA_ERR_HIP);
+        }
+    }
+    if (b->S.ct && !hip_ok(hipMalloc(&b->Gct, 9 * (size_t)b->S.ffs * sizeof(double)))) return fail(MARA_ERR_HIP);
+    if (!hip_ok(hipMalloc(&b->ctl, sizeof(Ctl)))) return fail(MARA_ERR_HIP);
+    if (!hip_ok(hipHostMalloc((void**)&b->ctl_host, sizeof(Ctl), hipHostMallocDefault))) return fail(MARA_ERR_HIP);
+    std::memset(b->ctl_host, 0, sizeof(Ctl));
+    b->ctl_host->active = 1;
+    b->ctl_host->first_fail = ~0ull;
+    if (!hip_ok(hipMemcpy(b->ctl, b->ctl_host, sizeof(Ctl), hipMemcpyHostToDevice))) return fail(MARA_ERR_HIP);
+    // guard zones and never-written faces start at zero
+    if (!hip_ok(hipMemset(b->Pa, 0, cellbytes))) return fail(MARA_ERR_HIP);
+    if (!hip_ok(hipMemset(b->Pb, 0, cellbytes))) return fail(MARA_ERR_HIP);
+    b->bytes = device_bytes(d, b->S);
+    *out = b;
+    return MARA_OK;
+}
+
+int mara_block_destroy(mara_block* b)
+{
+    if (!b) return MARA_ERR_INVALID_ARGUMENT;
+    if (b->stream) hipStreamSynchronize(b->stream);
+    hipFree(b->Pa);
+    hipFree(b->Pb);
+    hipFree(b->Pc);
+    hipFree(b->U);
+    for (int a = 0; a < 3; ++a) hipFree(b->F[a]);
+    hipFree(b->Gct);
+    hipFree(b->ctl);
+    if (b->ctl_host) hipHostFree(b->ctl_host);
+    if (b->ev0) hipEventDestroy(b->ev0);
+    if (b->ev1) hipEventDestroy(b->ev1);
+    if (b->stream) hipStreamDestroy(b->stream);
+    delete b;
+    return MARA_OK;
+}
+
+long long mara_block_device_bytes(const mara_block* b) { return b ? b->bytes : -1; }
+
+int mara_block_upload_prim(mara_block* b, const double* P)
+{
+    if (!b || !P) return MARA_ERR_INVALID_ARGUMENT;
+    const size_t n = (size_t)b->S.ncell * b->S.nq;
+    HIP_TRY(hipMemcpyAsync(b->U, P, n * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    k_aos_to_soa<<<grid_for(b->S.ncell), kThreads, 0, b->stream>>>(b->U, b->Pa, b->G);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return MARA_OK;
+}
+
+int mara_block_download_prim(mara_block* b, double* P)
+{
+    if (!b || !P) return MARA_ERR_INVALID_ARGUMENT;
+    const size_t n = (size_t)b->S.ncell * b->S.nq;
+    k_soa_to_aos<<<grid_for(b->S.ncell), kThreads, 0, b->stream>>>(b->Pa, b->U, b->G);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(P, b->U, n * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return MARA_OK;
+}
+
+int mara_block_apply_bc(mara_block* b)
+{
+    if (!b) return MARA_ERR_INVALID_ARGUMENT;
+    k_ctl_begin_courant<<<1, 1, 0, b->stream>>>(b->ctl);  // sets active
+    launch_bc(b, b->Pa);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return MARA_OK;
+}
+
+int mara_block_advance(mara_block* b, double t0, double dt, mara_report* report)
+{
+    (void)t0;
+    if (!b) return MARA_ERR_INVALID_ARGUMENT;
+    k_ctl_begin_advance<<<1, 1, 0, b->stream>>>(b->ctl, dt);
+    for (int rk = 0; rk < b->d.rk_order; ++rk) launch_stage(b, rk);
+    HIP_TRY(hipGetLastError());
+    int e = read_ctl(b);
+    if (e) return e;
+    return fill_report(b, report);
+}
+
+int mara_block_courant_dt(mara_block* b, double* dt)
+{
+    if (!b || !dt) return MARA_ERR_INVALID_ARGUMENT;
+    k_ctl_begin_courant<<<1, 1, 0, b->stream>>>(b->ctl);
+    launch_courant(b);
+    HIP_TRY(hipGetLastError());
+    int e = read_ctl(b);
+    if (e) return e;
+    union { unsigned long long u; double d; } cv;
+    cv.u = b->ctl_host->courant_bits;
+    *dt = cv.d;
+    if (b->ctl_host->superluminal) return MARA_ERR_SUPERLUMINAL;
+    if (b->ctl_host->eigen_fail) return MARA_ERR_BAD_EIGENVALUES;
+    return MARA_OK;
+}
+
+int mara_block_monopole_max(mara_block* b, double* max_abs)
+{
+    if (!b || !max_abs) return MARA_ERR_INVALID_ARGUMENT;
+    if (b->d.law == MARA_LAW_HYDRO) return MARA_ERR_UNSUPPORTED;
+    k_ctl_begin_monopole<<<1, 1, 0, b->stream>>>(b->ctl);
+    const long long total = (long long)b->G.n[0] * b->G.n[1] * b->G.n[2];
+    k_monopole<<<grid_for(total), kThreads, 0, b->stream>>>(b->Pa, b->G, b->ctl);
+    HIP_TRY(hipGetLastError());
+    int e = read_ctl(b);
+    if (e) return e;
+    union { unsigned long long u; double d; } cv;
+    cv.u = b->ctl_host->monopole_bits;
+    *max_abs = cv.d;
+    return MARA_OK;
+}
+
+int mara_block_run(mara_block* b, double cfl, double t_final, int max_steps,
+                   double* t, int* steps, mara_report* report)
+{
+    if (!b || !t || !steps || max_steps < 0) return MARA_ERR_INVALID_ARGUMENT;
+    k_ctl_begin_run<<<1, 1, 0, b->stream>>>(b->ctl, *t, t_final, cfl, max_steps);
+    const long long total = (long long)b->G.n[0] * b->G.n[1] * b->G.n[2];
+    (void)total;
+    int queued = 0;
+    while (queued < max_steps)
+    {
+        // enqueue a chunk of steps; steps after the stop condition are no-ops
+        const int chunk = max_steps - queued < 64 ? max_steps - queued : 64;
+        for (int s = 0; s < chunk; ++s)
+        {
+            k_ctl_step_begin<<<1, 1, 0, b->stream>>>(b->ctl);
+            launch_courant(b);
+            k_ctl_step_dt<<<1, 1, 0, b->stream>>>(b->ctl);
+            for (int rk = 0; rk < b->d.rk_order; ++rk) launch_stage(b, rk);
+            k_ctl_step_end<<<1, 1, 0, b->stream>>>(b->ctl);
+        }
+        queued += chunk;
+        HIP_TRY(hipGetLastError());
+        int e = read_ctl(b);
+        if (e) return e;
+        const Ctl& c = *b->ctl_host;
+        if (!c.active || !(c.t < c.t_final) || c.steps >= max_steps) break;
+    }
+    int e = read_ctl(b);
+    if (e) return e;
+    *t = b->ctl_host->t;
+    *steps = b->ctl_host->steps;
+    return fill_report(b, report);
+}
+
+int mara_block_set_timing(mara_block* b, int enable)
+{
+    if (!b) return MARA_ERR_INVALID_ARGUMENT;
+    if (enable && !b->ev0)
+    {
+        HIP_TRY(hipEventCreate(&b->ev0));
+        HIP_TRY(hipEventCreate(&b->ev1));
+    }
+    b->timing = enable != 0;
+    b->timed_ms = 0.0;
+    b->timed_launches = 0;
+    return MARA_OK;
+}
+
+int mara_block_stage_kernel_time(mara_block* b, double* avg_ms, long long* launches, const char** name)
+{
+    if (!b) return MARA_ERR_INVALID_ARGUMENT;
+    if (avg_ms) *avg_ms = b->timed_launches ? b->timed_ms / b->timed_launches : 0.0;
+    if (launches) *launches = b->timed_launches;
+    if (name) *name = b->fused ? fused_kernel_name(b->d) : "k_flux+k_ct+k_update (generic stage)";
+    return MARA_OK;
+}
+
+} // extern "C"

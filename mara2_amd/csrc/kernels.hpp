@@ -1,0 +1,565 @@
+// kernels.hpp -- the generic per-stage kernels of the Godunov step.
+//
+// Device layout (structure of arrays, x fastest):
+//   cell field q of cell (i,j,k):  P[q * fs + (k * N1 + ... )]  with
+//   c = (k * N2 + j) * N1 + i,  N = cells + 2 * guards.
+//   face field q of the face left of cell (i,j,k) along any axis:
+//   f = (k * (N2 + 1) + j) * (N1 + 1) + i, one array per axis.
+//
+// A stage is: flux sweep per fleshed axis -> flux-CT (MHD laws) ->
+// divergence + RK combine + primitive recovery (one kernel) -> ghost fill
+// axis by axis.  The fused hydro stage kernel (fused.hpp) replaces the first
+// three for the hydro law.
+#pragma once
+
+#include "physics.hpp"
+
+namespace mara {
+
+// Control block shared by the kernels of one mesh block: the step size and
+// clock of the device-driven loop, the active flag that lets queued kernels
+// become no-ops once the loop is done, and the failure counters.
+struct Ctl
+{
+    double dt;
+    double t;
+    double t_final;
+    double cfl;
+    unsigned long long courant_bits;  // atomicMin accumulator of a non-negative double
+    unsigned long long monopole_bits; // atomicMax accumulator of a non-negative double
+    unsigned long long first_fail;    // smallest failing interior index (reference order)
+    unsigned long long c2p_fail;
+    unsigned long long floor_applied;
+    unsigned long long eigen_fail;
+    unsigned long long superluminal;
+    int active;
+    int steps;
+    int max_steps;
+    int pad;
+};
+
+struct Geom
+{
+    int N[3];       // cells incl. guards
+    int g[3];       // guards
+    int n[3];       // interior cells
+    int nq;
+    long long fs;   // cell field stride
+    long long ffs;  // face field stride
+    double gamma, pfloor, theta;
+    double dA[3], dV, Lmin;
+    double nhat[9]; // UnitVector xhat, yhat, zhat
+};
+
+__device__ __forceinline__ long long cell_index(const Geom& G, int i, int j, int k)
+{
+    return ((long long)k * G.N[1] + j) * G.N[0] + i;
+}
+
+__device__ __forceinline__ long long face_index(const Geom& G, int i, int j, int k)
+{
+    return ((long long)k * (G.N[1] + 1) + j) * (G.N[0] + 1) + i;
+}
+
+// interior-relative linear index in the reference's iteration order (i slowest)
+__device__ __forceinline__ unsigned long long ref_linear(const Geom& G, int i, int j, int k)
+{
+    return ((unsigned long long)(i - G.g[0]) * G.n[1] + (j - G.g[1])) * G.n[2] + (k - G.g[2]);
+}
+
+__device__ __forceinline__ unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
+__device__ __forceinline__ double bitsd(unsigned long long b) { return __longlong_as_double((long long)b); }
+
+// ---------------------------------------------------------------------------
+// Face flux sweep: MeshOperator::godunov + the IntercellFluxScheme
+// (src/MeshOperator.cpp:443-525, src/IntercellFluxSchemes.cpp:45-117).
+// One thread per face of the box [lo, hi) of face indices.
+// ---------------------------------------------------------------------------
+template <int Law, int Recon, int Riem, typename R>
+__global__ void __launch_bounds__(256)
+k_flux(const double* __restrict__ P, R* __restrict__ F, Geom G, int axis,
+       int lo0, int lo1, int lo2, int m0, int m1, int m2, const Ctl* __restrict__ ctl, Ctl* ctlw)
+{
+    constexpr int NQ = LawNQ<Law>::value;
+    constexpr int NG = Recon == 0 ? 1 : Recon == 1 ? 2 : 3;
+    if (!ctl->active) return;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)m0 * m1 * m2;
+    if (t >= total) return;
+    const int i = lo0 + (int)(t % m0);
+    const int j = lo1 + (int)((t / m0) % m1);
+    const int k = lo2 + (int)(t / ((long long)m0 * m1));
+    const long long stride = axis == 0 ? 1 : axis == 1 ? (long long)G.N[0] : (long long)G.N[0] * G.N[1];
+    const long long c0 = cell_index(G, i, j, k) - NG * stride;  // first stencil cell
+
+    R PL[NQ], PR[NQ];
+    const R theta = R(G.theta);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+    {
+        const double* Pq = P + q * G.fs + c0;
+        if constexpr (NG == 1)
+        {
+            PL[q] = R(Pq[0]);
+            PR[q] = R(Pq[stride]);
+        }
+        else if constexpr (NG == 2)
+        {
+            const R v0 = R(Pq[0]), v1 = R(Pq[stride]), v2 = R(Pq[2 * stride]), v3 = R(Pq[3 * stride]);
+            PL[q] = plm(theta, v0, v1, v2, R(+1.0));
+            PR[q] = plm(theta, v1, v2, v3, R(-1.0));
+        }
+        else
+        {
+            const R v0 = R(Pq[0]), v1 = R(Pq[stride]), v2 = R(Pq[2 * stride]);
+            const R v3 = R(Pq[3 * stride]), v4 = R(Pq[4 * stride]), v5 = R(Pq[5 * stride]);
+            PL[q] = weno5<R, true>(v0, v1, v2, v3, v4);
+            PR[q] = weno5<R, false>(v1, v2, v3, v4, v5);
+        }
+    }
+    const R n[3] = {R(G.nhat[3 * axis + 0]), R(G.nhat[3 * axis + 1]), R(G.nhat[3 * axis + 2])};
+    State<R, NQ> SL, SR;
+    const int eL = law_state<Law, R>(R(G.gamma), PL, n, SL);
+    const int eR = law_state<Law, R>(R(G.gamma), PR, n, SR);
+    R Fq[NQ];
+    if (eL | eR)
+    {
+        if ((eL | eR) & ERR_SUPERLUMINAL) atomicAdd(&ctlw->superluminal, 1ull);
+        if ((eL | eR) & ERR_EIGEN) atomicAdd(&ctlw->eigen_fail, 1ull);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) Fq[q] = R(0);
+    }
+    else if constexpr (Riem == 1 && Law == 0)
+        hllc(SL, SR, n, Fq);
+    else
+        hlle<R, NQ>(SL, SR, Fq);
+
+    const long long f = face_index(G, i, j, k);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) F[q * G.ffs + f] = Fq[q];
+}
+
+// ---------------------------------------------------------------------------
+// Flux-CT (CellCenteredFieldCT::correctGodunovFluxes, Toth 2000 eq. 25,
+// src/CellCenteredFieldCT.cpp:14-134).  For face axis a and transverse
+// component c != a (t = c):
+//   G = 1/8 (2 F_a^c(x) + F_a^c(x+e_t) + F_a^c(x-e_t)
+//            - F_t^a(x+e_t) - F_t^a(x+e_t-e_a) - F_t^a(x) - F_t^a(x-e_a))
+// where F_a^c is the flux along a of B_c.  G_a^a = 0.  When axis t is flat
+// the reference leaves F_a^c as it is.  Evaluated in binary64 in both forms.
+// G layout: (a * 3 + c) * ffs + f.
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ void __launch_bounds__(256)
+k_ct(const R* __restrict__ Fx, const R* __restrict__ Fy, const R* __restrict__ Fz,
+     double* __restrict__ Gct, Geom G, int a, int lo0, int lo1, int lo2, int m0, int m1, int m2,
+     const Ctl* __restrict__ ctl)
+{
+    if (!ctl->active) return;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)m0 * m1 * m2;
+    if (t >= total) return;
+    const int i = lo0 + (int)(t % m0);
+    const int j = lo1 + (int)((t / m0) % m1);
+    const int k = lo2 + (int)(t / ((long long)m0 * m1));
+    const R* Fa[3] = {Fx, Fy, Fz};
+    const long long fstride[3] = {1, (long long)(G.N[0] + 1), (long long)(G.N[0] + 1) * (G.N[1] + 1)};
+    const long long f = face_index(G, i, j, k);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+    {
+        double val;
+        if (c == a)
+            val = 0.0;
+        else if (G.N[c] < 2)
+            val = double(Fa[a][(5 + c) * G.ffs + f]);
+        else
+        {
+            const R* Fac = Fa[a] + (5 + c) * G.ffs;  // flux along a of B_c
+            const R* Fta = Fa[c] + (5 + a) * G.ffs;  // flux along t = c of B_a
+            const long long et = fstride[c], ea = fstride[a];
+            val = 0.125 * (+2 * double(Fac[f]) + 1 * double(Fac[f + et]) + 1 * double(Fac[f - et])
+                           - 1 * double(Fta[f + et]) - 1 * double(Fta[f + et - ea])
+                           - 1 * double(Fta[f]) - 1 * double(Fta[f - ea]));
+        }
+        Gct[(a * 3 + c) * G.ffs + f] = val;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Zone update: L = -div F (src/MeshOperator.cpp:323-381), RK combine
+// U = U0 (1-b) + (U + dt L) b (src/SolutionSchemes.cpp:232-235), and
+// recoverPrimitive (src/FieldOperator.cpp:41-82) on interior zones.
+// U0 is regenerated from the step's starting primitives P0.
+// ---------------------------------------------------------------------------
+template <int Law, typename R, bool CT>
+__global__ void __launch_bounds__(256)
+k_update(const double* __restrict__ P0, double* __restrict__ U, double* __restrict__ Pout,
+         const R* __restrict__ Fx, const R* __restrict__ Fy, const R* __restrict__ Fz,
+         const double* __restrict__ Gct, Geom G, double b, int first_stage, int write_u,
+         Ctl* ctl)
+{
+    constexpr int NQ = LawNQ<Law>::value;
+    if (!ctl->active) return;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)G.n[0] * G.n[1] * G.n[2];
+    if (t >= total) return;
+    const int i = G.g[0] + (int)(t % G.n[0]);
+    const int j = G.g[1] + (int)((t / G.n[0]) % G.n[1]);
+    const int k = G.g[2] + (int)(t / ((long long)G.n[0] * G.n[1]));
+    const long long c = cell_index(G, i, j, k);
+    const long long f = face_index(G, i, j, k);
+    const long long fx = f + 1, fy = f + (G.N[0] + 1), fz = f + (long long)(G.N[0] + 1) * (G.N[1] + 1);
+    const bool hx = G.N[0] > 1, hy = G.N[1] > 1, hz = G.N[2] > 1;
+    const double dt = ctl->dt;
+
+    double p0[NQ], U0[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) p0[q] = P0[q * G.fs + c];
+    if (prim_to_cons<Law>(G.gamma, p0, U0) != ERR_NONE) atomicAdd(&ctl->superluminal, 1ull);
+
+    const double fac = -1.0 / G.dV;
+    double Un[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+    {
+        double L;
+        if (CT && q >= 5)
+        {
+            const int cc = q - 5;
+            const double* Gx = Gct + (0 * 3 + cc) * G.ffs;
+            const double* Gy = Gct + (1 * 3 + cc) * G.ffs;
+            const double* Gz = Gct + (2 * 3 + cc) * G.ffs;
+            const double x1 = hx && cc != 0 ? Gx[fx] : 0.0, x0 = hx && cc != 0 ? Gx[f] : 0.0;
+            const double y1 = hy && cc != 1 ? Gy[fy] : 0.0, y0 = hy && cc != 1 ? Gy[f] : 0.0;
+            const double z1 = hz && cc != 2 ? Gz[fz] : 0.0, z0 = hz && cc != 2 ? Gz[f] : 0.0;
+            L = fac * (+x1 * G.dA[0] - x0 * G.dA[0] + y1 * G.dA[1] - y0 * G.dA[1] + z1 * G.dA[2] - z0 * G.dA[2]);
+        }
+        else
+        {
+            const R A0 = R(G.dA[0]), A1 = R(G.dA[1]), A2 = R(G.dA[2]);
+            const R x1 = hx ? Fx[q * G.ffs + fx] : R(0), x0 = hx ? Fx[q * G.ffs + f] : R(0);
+            const R y1 = hy ? Fy[q * G.ffs + fy] : R(0), y0 = hy ? Fy[q * G.ffs + f] : R(0);
+            const R z1 = hz ? Fz[q * G.ffs + fz] : R(0), z0 = hz ? Fz[q * G.ffs + f] : R(0);
+            L = double(R(fac) * (+x1 * A0 - x0 * A0 + y1 * A1 - y0 * A1 + z1 * A2 - z0 * A2));
+        }
+        const double Uq = first_stage ? U0[q] : U[q * G.fs + c];
+        Un[q] = U0[q] * (1 - b) + (Uq + dt * L) * b;
+    }
+    double Pn[NQ];
+    int floor = 0;
+    if (cons_to_prim<Law>(G.gamma, G.pfloor, Un, Pn, floor) != 0)
+    {
+        atomicAdd(&ctl->c2p_fail, 1ull);
+        atomicMin(&ctl->first_fail, ref_linear(G, i, j, k));
+    }
+    else
+    {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) Pout[q * G.fs + c] = Pn[q];
+    }
+    if (floor) atomicAdd(&ctl->floor_applied, 1ull);
+    if (write_u)
+    {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) U[q * G.fs + c] = Un[q];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Ghost fill of one axis, both sides (src/BoundaryConditions.cpp:24-176).
+// kind: 0 periodic, 1 outflow, 2 reflecting.  Threads: 2 * g * N_t1 * N_t2.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_bc(double* __restrict__ P, Geom G, int axis, int kind, const Ctl* __restrict__ ctl)
+{
+    if (!ctl->active) return;
+    const int g = G.g[axis];
+    const int N = G.N[axis];
+    const int t1 = axis == 0 ? 1 : 0;
+    const int t2 = axis == 2 ? 1 : 2;
+    const long long slab = (long long)G.N[t1] * G.N[t2];
+    const long long total = 2ll * g * slab;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int a1 = (int)(t % G.N[t1]);
+    const int a2 = (int)((t / G.N[t1]) % G.N[t2]);
+    const int layer = (int)((t / slab) % g);
+    const int side = (int)(t / (slab * g));
+    int dst, src;
+    if (kind == 0)
+    {
+        dst = side == 0 ? layer : N - g + layer;
+        src = side == 0 ? N - 2 * g + layer : g + layer;
+    }
+    else if (kind == 1)
+    {
+        dst = side == 0 ? layer : N - 1 - layer;
+        src = side == 0 ? g : N - g - 1;
+    }
+    else
+    {
+        dst = side == 0 ? layer : N - 1 - layer;
+        src = side == 0 ? 2 * g - layer - 1 : N - 2 * g + layer;
+    }
+    int I[3], J[3];
+    I[axis] = dst; J[axis] = src;
+    I[t1] = J[t1] = a1;
+    I[t2] = J[t2] = a2;
+    const long long cd = cell_index(G, I[0], I[1], I[2]);
+    const long long cs = cell_index(G, J[0], J[1], J[2]);
+    for (int q = 0; q < G.nq; ++q)
+    {
+        double v = P[q * G.fs + cs];
+        if (kind == 2 && q == 1 + axis) v *= -1;
+        P[q * G.fs + cd] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// CFL reduction (FieldOperator::getCourantTimestep, src/FieldOperator.cpp:125-169):
+// per interior zone L / max(lambda_x, lambda_y, lambda_z), minimum over zones.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T wave_min(T v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+    {
+        const T w = __shfl_xor(v, o, 64);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_max(T v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+    {
+        const T w = __shfl_xor(v, o, 64);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
+template <int Law>
+__device__ __forceinline__ double zone_courant(const double* __restrict__ P, const Geom& G, long long c, int& err)
+{
+    constexpr int NQ = LawNQ<Law>::value;
+    double p[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) p[q] = P[q * G.fs + c];
+    double lam[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+    {
+        State<double, NQ> S;
+        err |= law_state<Law, double>(G.gamma, p, G.nhat + 3 * a, S);
+        double m = 0.0;
+#pragma unroll
+        for (int n = 0; n < NQ; ++n)
+        {
+            const double x = fabs(S.A[n]);
+            if (m < x) m = x;
+        }
+        lam[a] = m;
+    }
+    double lmax = lam[0];
+    if (lmax < lam[1]) lmax = lam[1];
+    if (lmax < lam[2]) lmax = lam[2];
+    return G.Lmin / lmax;
+}
+
+// Grid-stride over interior zones; wavefront then work-group minimum, one
+// atomic per work-group.
+template <int Law>
+__global__ void __launch_bounds__(256)
+k_courant(const double* __restrict__ P, Geom G, Ctl* ctl)
+{
+    __shared__ double wmin[4];
+    if (!ctl->active) return;
+    const long long total = (long long)G.n[0] * G.n[1] * G.n[2];
+    const long long nxy = (long long)G.n[0] * G.n[1];
+    double dtc = 1.79769313486231570815e+308;  // DBL_MAX
+    int err = 0;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x)
+    {
+        const long long kk = t / nxy;
+        const long long r = t - kk * nxy;
+        const int j = (int)(r / G.n[0]);
+        const int i = (int)(r - (long long)j * G.n[0]);
+        const long long c = cell_index(G, G.g[0] + i, G.g[1] + j, G.g[2] + (int)kk);
+        const double d = zone_courant<Law>(P, G, c, err);
+        if (d < dtc) dtc = d;
+    }
+    if (err & ERR_SUPERLUMINAL) atomicAdd(&ctl->superluminal, 1ull);
+    if (err & ERR_EIGEN) atomicAdd(&ctl->eigen_fail, 1ull);
+    dtc = wave_min(dtc);
+    if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = dtc;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        double m = wmin[0];
+        for (int w = 1; w < 4; ++w) if (wmin[w] < m) m = wmin[w];
+        if (m < 1.79769313486231570815e+308) atomicMin(&ctl->courant_bits, dbits(m));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Cell monopole max (CellCenteredFieldCT::monopole, cell location,
+// src/CellCenteredFieldCT.cpp:159-229) over interior zones.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double vertex_monopole(const double* __restrict__ P, const Geom& G, int I, int J, int K)
+{
+    const int r = G.N[0] > 1, s = G.N[1] > 1, tt = G.N[2] > 1;
+    // vertex (I,J,K) is written from base (I-1,J-1,K-1) when inside S
+    const int i = I - 1, j = J - 1, k = K - 1;
+    if (i < 0 || j < 0 || k < 0 || i >= G.N[0] - r || j >= G.N[1] - s || k >= G.N[2] - tt) return 0.0;
+    const double* Bx = P + 5 * G.fs;
+    const double* By = P + 6 * G.fs;
+    const double* Bz = P + 7 * G.fs;
+#define CI(a, b, c) cell_index(G, a, b, c)
+    const double B0x00 = Bx[CI(i + 0, j + 0, k + 0)], B0x01 = Bx[CI(i + 0, j + 0, k + tt)];
+    const double B0x10 = Bx[CI(i + 0, j + s, k + 0)], B0x11 = Bx[CI(i + 0, j + s, k + tt)];
+    const double B1x00 = Bx[CI(i + r, j + 0, k + 0)], B1x01 = Bx[CI(i + r, j + 0, k + tt)];
+    const double B1x10 = Bx[CI(i + r, j + s, k + 0)], B1x11 = Bx[CI(i + r, j + s, k + tt)];
+    const double B0y00 = By[CI(i + 0, j + 0, k + 0)], B0y01 = By[CI(i + r, j + 0, k + 0)];
+    const double B0y10 = By[CI(i + 0, j + 0, k + tt)], B0y11 = By[CI(i + r, j + 0, k + tt)];
+    const double B1y00 = By[CI(i + 0, j + s, k + 0)], B1y01 = By[CI(i + r, j + s, k + 0)];
+    const double B1y10 = By[CI(i + 0, j + s, k + tt)], B1y11 = By[CI(i + r, j + s, k + tt)];
+    const double B0z00 = Bz[CI(i + 0, j + 0, k + 0)], B0z01 = Bz[CI(i + 0, j + s, k + 0)];
+    const double B0z10 = Bz[CI(i + r, j + 0, k + 0)], B0z11 = Bz[CI(i + r, j + s, k + 0)];
+    const double B1z00 = Bz[CI(i + 0, j + 0, k + tt)], B1z01 = Bz[CI(i + 0, j + s, k + tt)];
+    const double B1z10 = Bz[CI(i + r, j + 0, k + tt)], B1z11 = Bz[CI(i + r, j + s, k + tt)];
+#undef CI
+    const double Bx0 = B0x00 + B0x01 + B0x10 + B0x11;
+    const double Bx1 = B1x00 + B1x01 + B1x10 + B1x11;
+    const double By0 = B0y00 + B0y01 + B0y10 + B0y11;
+    const double By1 = B1y00 + B1y01 + B1y10 + B1y11;
+    const double Bz0 = B0z00 + B0z01 + B0z10 + B0z11;
+    const double Bz1 = B1z00 + B1z01 + B1z10 + B1z11;
+    return 0.25 * ((Bx1 - Bx0) + (By1 - By0) + (Bz1 - Bz0)) / 1.0;
+}
+
+__global__ void __launch_bounds__(256)
+k_monopole(const double* __restrict__ P, Geom G, Ctl* ctl)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)G.n[0] * G.n[1] * G.n[2];
+    double m = 0.0;
+    if (t < total)
+    {
+        const int i = G.g[0] + (int)(t % G.n[0]);
+        const int j = G.g[1] + (int)((t / G.n[0]) % G.n[1]);
+        const int k = G.g[2] + (int)(t / ((long long)G.n[0] * G.n[1]));
+        const double M000 = vertex_monopole(P, G, i, j, k);
+        const double Mi = M000 + vertex_monopole(P, G, i + 1, j, k);
+        const double Mj = M000 + vertex_monopole(P, G, i, j + 1, k);
+        const double Mk = M000 + vertex_monopole(P, G, i, j, k + 1);
+        const double v = fabs(1. / 6 * (Mi + Mj + Mk));
+        if (v > m) m = v;
+    }
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0 && m > 0.0) atomicMax(&ctl->monopole_bits, dbits(m));
+}
+
+// ---------------------------------------------------------------------------
+// Layout transposes between MeshData::P (row-major i,j,k,q) and device SoA.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_aos_to_soa(const double* __restrict__ A, double* __restrict__ P, Geom G)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long ncell = (long long)G.N[0] * G.N[1] * G.N[2];
+    if (t >= ncell) return;
+    // t enumerates the reference order: i slowest, k fastest
+    const int k = (int)(t % G.N[2]);
+    const int j = (int)((t / G.N[2]) % G.N[1]);
+    const int i = (int)(t / ((long long)G.N[1] * G.N[2]));
+    const long long c = cell_index(G, i, j, k);
+    for (int q = 0; q < G.nq; ++q) P[q * G.fs + c] = A[t * G.nq + q];
+}
+
+__global__ void __launch_bounds__(256)
+k_soa_to_aos(const double* __restrict__ P, double* __restrict__ A, Geom G)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long ncell = (long long)G.N[0] * G.N[1] * G.N[2];
+    if (t >= ncell) return;
+    const int k = (int)(t % G.N[2]);
+    const int j = (int)((t / G.N[2]) % G.N[1]);
+    const int i = (int)(t / ((long long)G.N[1] * G.N[2]));
+    const long long c = cell_index(G, i, j, k);
+    for (int q = 0; q < G.nq; ++q) A[t * G.nq + q] = P[q * G.fs + c];
+}
+
+// ---------------------------------------------------------------------------
+// Control kernels (single thread): host-given dt, and the device-driven loop.
+// ---------------------------------------------------------------------------
+__global__ void k_ctl_begin_advance(Ctl* ctl, double dt)
+{
+    ctl->dt = dt;
+    ctl->active = 1;
+    ctl->c2p_fail = 0;
+    ctl->floor_applied = 0;
+    ctl->eigen_fail = 0;
+    ctl->superluminal = 0;
+    ctl->first_fail = ~0ull;
+}
+
+__global__ void k_ctl_begin_courant(Ctl* ctl)
+{
+    ctl->active = 1;
+    ctl->courant_bits = 0x7fefffffffffffffull;  // DBL_MAX
+    ctl->eigen_fail = 0;
+    ctl->superluminal = 0;
+}
+
+__global__ void k_ctl_begin_monopole(Ctl* ctl)
+{
+    ctl->monopole_bits = 0ull;
+}
+
+__global__ void k_ctl_begin_run(Ctl* ctl, double t, double t_final, double cfl, int max_steps)
+{
+    ctl->t = t;
+    ctl->t_final = t_final;
+    ctl->cfl = cfl;
+    ctl->steps = 0;
+    ctl->max_steps = max_steps;
+    ctl->active = 1;
+    ctl->c2p_fail = 0;
+    ctl->floor_applied = 0;
+    ctl->eigen_fail = 0;
+    ctl->superluminal = 0;
+    ctl->first_fail = ~0ull;
+}
+
+// maraMainLoop: while (t < t_final) { dt = min(cfl * courant, t_final - t); ... }
+__global__ void k_ctl_step_begin(Ctl* ctl)
+{
+    const bool go = ctl->active && ctl->t < ctl->t_final && ctl->steps < ctl->max_steps
+                    && ctl->c2p_fail == 0 && ctl->eigen_fail == 0 && ctl->superluminal == 0;
+    ctl->active = go ? 1 : 0;
+    ctl->courant_bits = 0x7fefffffffffffffull;
+}
+
+__global__ void k_ctl_step_dt(Ctl* ctl)
+{
+    if (!ctl->active) return;
+    const double dt1 = ctl->cfl * bitsd(ctl->courant_bits);
+    const double dt2 = ctl->t_final - ctl->t;
+    ctl->dt = dt1 < dt2 ? dt1 : dt2;
+}
+
+__global__ void k_ctl_step_end(Ctl* ctl)
+{
+    if (!ctl->active) return;
+    ctl->t += ctl->dt;
+    ctl->steps += 1;
+}
+
+} // namespace mara

@@ -1,0 +1,791 @@
+// physics.hpp -- per-face and per-zone physics of Mara2's Godunov step as
+// device functions, templated on the working precision R (double for the
+// plain path, float for the flux path of the mixed form).
+//
+// Every routine keeps the operation order of the reference routine it cites,
+// so that with contraction disabled (-ffp-contract=off) the FP64 results are
+// the reference's bit for bit.  NaN behaviour of the reference's comparison
+// macros is kept as well (they are written as ternaries, never fmin/fmax).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+namespace mara {
+
+// ---------------------------------------------------------------------------
+// precision helpers
+// ---------------------------------------------------------------------------
+template <typename R> struct Math;
+
+template <> struct Math<double>
+{
+    static __device__ __forceinline__ double sqrt(double x) { return __builtin_sqrt(x); }
+    static __device__ __forceinline__ double div(double a, double b) { return a / b; }
+    static __device__ __forceinline__ double abs(double x) { return __builtin_fabs(x); }
+};
+
+// binary32 in the mixed form: hardware square root and reciprocal (about one
+// ulp), which the mixed form's L1 budget of 1e-5 against FP64 absorbs.
+template <> struct Math<float>
+{
+    static __device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+    static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+    static __device__ __forceinline__ float abs(float x) { return __builtin_fabsf(x); }
+};
+
+template <typename R> __device__ __forceinline__ R min2(R a, R b) { return a < b ? a : b; }
+template <typename R> __device__ __forceinline__ R max2(R a, R b) { return a > b ? a : b; }
+// MIN3 / MAX3 of src/RiemannSolvers.cpp:3-6
+template <typename R> __device__ __forceinline__ R min3(R a, R b, R c) { return a < b ? min2(a, c) : min2(b, c); }
+template <typename R> __device__ __forceinline__ R max3(R a, R b, R c) { return a > b ? max2(a, c) : max2(b, c); }
+template <typename R> __device__ __forceinline__ int sgn(R x) { return (x > R(0)) - (x < R(0)); }
+
+// ---------------------------------------------------------------------------
+// Reconstruction (src/Reconstruction.cpp:55-132)
+// ---------------------------------------------------------------------------
+template <typename R>
+__device__ __forceinline__ R minmod(R theta, R ul, R u0, R ur)
+{
+    const R a = theta * (u0 - ul);
+    const R b = R(0.5) * (ur - ul);
+    const R c = theta * (ur - u0);
+    const R m = min3(Math<R>::abs(a), Math<R>::abs(b), Math<R>::abs(c));
+    const int sa = sgn(a), sb = sgn(b), sc = sgn(c);
+    return R(0.25) * Math<R>::abs(R(sa + sb)) * R(sa + sc) * m;
+}
+
+// PLM_C2R (sgn=+1) / PLM_C2L (sgn=-1): v0 + sgn * 0.5 * minmod
+template <typename R>
+__device__ __forceinline__ R plm(R theta, R vl, R v0, R vr, R sgnv)
+{
+    return v0 + sgnv * R(0.5) * minmod(theta, vl, v0, vr);
+}
+
+// weno5 with the FD tables and the ImprovedShenZha10 indicator, A = 50
+// (src/Reconstruction.cpp:68-132, 154-169).  Right = true selects C2R
+// (value at the right edge of the centre cell), else C2L.  v[-2..2].
+template <typename R, bool Right>
+__device__ __forceinline__ R weno5(R vm2, R vm1, R v0, R vp1, R vp2)
+{
+    // coefficient rows c[r][0..2] and weights d[r]
+    const R c00 = Right ? R(1. / 3.) : R(11. / 6.), c01 = Right ? R(5. / 6.) : R(-7. / 6.), c02 = Right ? R(-1. / 6.) : R(1. / 3.);
+    const R c10 = Right ? R(-1. / 6.) : R(1. / 3.), c11 = R(5. / 6.), c12 = Right ? R(1. / 3.) : R(-1. / 6.);
+    const R c20 = Right ? R(1. / 3.) : R(-1. / 6.), c21 = Right ? R(-7. / 6.) : R(5. / 6.), c22 = Right ? R(11. / 6.) : R(1. / 3.);
+    const R d0 = Right ? R(0.3) : R(0.1), d1 = R(0.6), d2 = Right ? R(0.1) : R(0.3);
+    const R eps_prime = R(1e-10);
+    const R A = R(50.0);
+    const R vs0 = c00 * v0 + c01 * vp1 + c02 * vp2;
+    const R vs1 = c10 * vm1 + c11 * v0 + c12 * vp1;
+    const R vs2 = c20 * vm2 + c21 * vm1 + c22 * v0;
+    const R t0a = R(1) * v0 - R(2) * vp1 + R(1) * vp2;
+    const R t0b = R(3) * v0 - R(4) * vp1 + R(1) * vp2;
+    const R t1a = R(1) * vm1 - R(2) * v0 + R(1) * vp1;
+    const R t1b = R(1) * vm1 - R(0) * v0 - R(1) * vp1;
+    const R t2a = R(1) * vm2 - R(2) * vm1 + R(1) * v0;
+    const R t2b = R(1) * vm2 - R(4) * vm1 + R(3) * v0;
+    R B0 = R(13. / 12.) * (t0a * t0a) + R(1. / 4.) * (t0b * t0b);
+    R B1 = R(13. / 12.) * (t1a * t1a) + R(1. / 4.) * (t1b * t1b);
+    R B2 = R(13. / 12.) * (t2a * t2a) + R(1. / 4.) * (t2b * t2b);
+    const R minB = min3(B0, B1, B2);
+    const R maxB = max3(B0, B1, B2);
+    const R R0 = Math<R>::div(minB, maxB + eps_prime);
+    B0 = R0 * A * minB + B0;
+    B1 = R0 * A * minB + B1;
+    B2 = R0 * A * minB + B2;
+    const R e0 = eps_prime + B0, e1 = eps_prime + B1, e2 = eps_prime + B2;
+    const R w0 = Math<R>::div(d0, e0 * e0);
+    const R w1 = Math<R>::div(d1, e1 * e1);
+    const R w2 = Math<R>::div(d2, e2 * e2);
+    const R wtot = w0 + w1 + w2;
+    return Math<R>::div(w0 * vs0 + w1 * vs1 + w2 * vs2, wtot);
+}
+
+// ---------------------------------------------------------------------------
+// QuarticPolynomial (src/QuarticPolynomial.cpp:19-144), binary64 only
+// ---------------------------------------------------------------------------
+__device__ inline void solve_cubic(double c3, double c2, double c1, double c0,
+                                   double& x1, double& x2, double& x3, int& nr)
+{
+    const double a2 = c2 / c3;
+    const double a1 = c1 / c3;
+    const double a0 = c0 / c3;
+    const double q = a1 / 3.0 - a2 * a2 / 9.0;
+    const double r = (a1 * a2 - 3.0 * a0) / 6.0 - a2 * a2 * a2 / 27.0;
+    const double delta = q * q * q + r * r;
+    if (delta > 0.0)
+    {
+        double s1 = r + sqrt(delta);
+        double s2 = r - sqrt(delta);
+        s1 = (s1 >= 0.0) ? pow(s1, 1. / 3) : -pow(-s1, 1. / 3);
+        s2 = (s2 >= 0.0) ? pow(s2, 1. / 3) : -pow(-s2, 1. / 3);
+        x1 = (s1 + s2) - a2 / 3.0;
+        x2 = x3 = -0.5 * (s1 + s2) - a2 / 3.0;
+        nr = 1;
+    }
+    else if (delta < 0.0)
+    {
+        const double theta = acos(r / sqrt(-q * q * q)) / 3.0;
+        const double costh = cos(theta);
+        const double sinth = sin(theta);
+        const double sq = sqrt(-q);
+        x1 = 2.0 * sq * costh - a2 / 3.0;
+        x2 = -sq * costh - a2 / 3.0 - sqrt(3.0) * sq * sinth;
+        x3 = -sq * costh - a2 / 3.0 + sqrt(3.0) * sq * sinth;
+        nr = 3;
+    }
+    else
+    {
+        const double s = (r >= 0.0) ? pow(r, 1. / 3) : -pow(-r, 1. / 3);
+        x1 = 2.0 * s - a2 / 3.0;
+        x2 = x3 = -s - a2 / 3.0;
+        nr = 3;
+    }
+}
+
+__device__ inline int quartic_solve(double d4, double d3, double d2, double d1, double d0, double roots[4])
+{
+    const double a3 = d3 / d4;
+    const double a2 = d2 / d4;
+    const double a1 = d1 / d4;
+    const double a0 = d0 / d4;
+    const double au2 = -a2;
+    const double au1 = (a1 * a3 - 4 * a0);
+    const double au0 = 4.0 * a0 * a2 - a1 * a1 - a0 * a3 * a3;
+    int nr, nr12, nr34;
+    double r1, r2, r3, r4, x1, x2, x3;
+    solve_cubic(1.0, au2, au1, au0, x1, x2, x3, nr);
+    const double u1 = (nr == 1) ? x1 : ((x1 > x3) ? x1 : x3);
+    const double R2 = 0.25 * a3 * a3 + u1 - a2;
+    const double R = (R2 > 0.0) ? sqrt(R2) : 0.0;
+    double D2, E2;
+    if (R != 0.0)
+    {
+        const double foo1 = 0.75 * a3 * a3 - R2 - 2.0 * a2;
+        const double foo2 = 0.25 * (4.0 * a3 * a2 - 8.0 * a1 - a3 * a3 * a3) / R;
+        D2 = foo1 + foo2;
+        E2 = foo1 - foo2;
+    }
+    else
+    {
+        const double foo1 = 0.75 * a3 * a3 - 2.0 * a2;
+        const double foo2 = 2.00 * sqrt(u1 * u1 - 4.0 * a0);
+        D2 = foo1 + foo2;
+        E2 = foo1 - foo2;
+    }
+    if (D2 >= 0.0)
+    {
+        const double D = sqrt(D2);
+        r1 = -0.25 * a3 + 0.5 * R - 0.5 * D;
+        r2 = -0.25 * a3 + 0.5 * R + 0.5 * D;
+        nr12 = 2;
+    }
+    else
+    {
+        r1 = r2 = -0.25 * a3 + 0.5 * R;
+        nr12 = 0;
+    }
+    if (E2 >= 0.0)
+    {
+        const double E = sqrt(E2);
+        r3 = -0.25 * a3 - 0.5 * R - 0.5 * E;
+        r4 = -0.25 * a3 - 0.5 * R + 0.5 * E;
+        nr34 = 2;
+    }
+    else
+    {
+        r3 = r4 = -0.25 * a3 - 0.5 * R;
+        nr34 = 0;
+    }
+    int i = 0;
+    if (nr12 != 0) { roots[i++] = r1; roots[i++] = r2; }
+    if (nr34 != 0) { roots[i++] = r3; roots[i++] = r4; }
+    for (int a = 1; a < i; ++a)  // ascending, same order as std::sort
+    {
+        const double x = roots[a];
+        int b = a - 1;
+        while (b >= 0 && x < roots[b]) { roots[b + 1] = roots[b]; --b; }
+        roots[b + 1] = x;
+    }
+    return nr12 + nr34;
+}
+
+// ---------------------------------------------------------------------------
+// Face states.  A State holds what the Riemann solvers read: P, U, F and the
+// eigenvalues A (unused slots of the reference's 10-wide arrays are zero).
+// ---------------------------------------------------------------------------
+template <typename R, int NQ>
+struct State
+{
+    R P[NQ], U[NQ], F[NQ], A[NQ];
+};
+
+enum : int { ERR_NONE = 0, ERR_SUPERLUMINAL = 1, ERR_EIGEN = 2 };
+
+// NewtonianHydro::fromPrimitive (src/ConservationLaws.cpp:253-290)
+template <typename R>
+__device__ __forceinline__ void hydro_state(R gamma, const R* P, const R* dAA, State<R, 5>& S)
+{
+    const R gm0 = gamma;
+    const R gm1 = gamma - R(1.0);
+    const R cs = Math<R>::sqrt(Math<R>::div(gm0 * P[4], P[0]));
+    const R vv = P[1] * P[1] + P[2] * P[2] + P[3] * P[3];
+    const R vn = P[1] * dAA[0] + P[2] * dAA[1] + P[3] * dAA[2];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) S.P[q] = P[q];
+    S.U[0] = P[0];
+    S.U[1] = P[0] * P[1];
+    S.U[2] = P[0] * P[2];
+    S.U[3] = P[0] * P[3];
+    S.U[4] = P[0] * R(0.5) * vv + Math<R>::div(P[4], gm1);
+    S.F[0] = vn * S.U[0];
+    S.F[1] = vn * S.U[1] + P[4] * dAA[0];
+    S.F[2] = vn * S.U[2] + P[4] * dAA[1];
+    S.F[3] = vn * S.U[3] + P[4] * dAA[2];
+    S.F[4] = vn * S.U[4] + P[4] * vn;
+    S.A[0] = vn - cs;
+    S.A[1] = vn;
+    S.A[2] = vn;
+    S.A[3] = vn;
+    S.A[4] = vn + cs;
+}
+
+// NewtonianMHD::fromPrimitive (src/ConservationLaws.cpp:416-487)
+template <typename R>
+__device__ __forceinline__ void mhd_state(R gamma, const R* P, const R* dAA, State<R, 8>& S)
+{
+    const R gm0 = gamma;
+    const R gm1 = gamma - R(1.0);
+    const R cs2 = Math<R>::div(gm0 * P[4], P[0]);
+    const R vv = P[1] * P[1] + P[2] * P[2] + P[3] * P[3];
+    const R BB = P[5] * P[5] + P[6] * P[6] + P[7] * P[7];
+    const R Bv = P[5] * P[1] + P[6] * P[2] + P[7] * P[3];
+    const R vn = P[1] * dAA[0] + P[2] * dAA[1] + P[3] * dAA[2];
+    const R Bn = P[5] * dAA[0] + P[6] * dAA[1] + P[7] * dAA[2];
+    const R ps = P[4] + R(0.5) * BB;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) S.P[q] = P[q];
+    S.U[0] = P[0];
+    S.U[1] = P[0] * P[1];
+    S.U[2] = P[0] * P[2];
+    S.U[3] = P[0] * P[3];
+    S.U[4] = P[0] * R(0.5) * vv + R(0.5) * BB + Math<R>::div(P[4], gm1);
+    S.U[5] = P[5];
+    S.U[6] = P[6];
+    S.U[7] = P[7];
+    S.F[0] = vn * S.U[0];
+    S.F[1] = vn * S.U[1] - Bn * P[5] + ps * dAA[0];
+    S.F[2] = vn * S.U[2] - Bn * P[6] + ps * dAA[1];
+    S.F[3] = vn * S.U[3] - Bn * P[7] + ps * dAA[2];
+    S.F[4] = vn * S.U[4] - Bn * Bv + ps * vn;
+    S.F[5] = vn * S.U[5] - Bn * P[1];
+    S.F[6] = vn * S.U[6] - Bn * P[2];
+    S.F[7] = vn * S.U[7] - Bn * P[3];
+    const R Bn2 = Bn * Bn;
+    const R ca2 = Math<R>::div(BB, P[0]);
+    const R cn2 = Math<R>::div(Bn2, P[0]);
+    const R cw4 = (cs2 + ca2) * (cs2 + ca2);
+    const R root = Math<R>::sqrt(cw4 - R(4) * cs2 * cn2);
+    const R cF2 = R(0.5) * (cs2 + ca2 + root);
+    const R cS2 = R(0.5) * (cs2 + ca2 - root);
+    const R sF = Math<R>::sqrt(cF2), sA = Math<R>::sqrt(cn2), sS = Math<R>::sqrt(cS2);
+    S.A[0] = vn - sF;
+    S.A[1] = vn - sA;
+    S.A[2] = vn - sS;
+    S.A[3] = vn;
+    S.A[4] = vn;
+    S.A[5] = vn + sS;
+    S.A[6] = vn + sA;
+    S.A[7] = vn + sF;
+}
+
+// RelativisticMHD eigenvalues (src/ConservationLaws.cpp:739-835), binary64.
+// Returns ERR_NONE or ERR_EIGEN.
+__device__ inline int rmhd_eigen(double gamma, const double* P, const double* dAA, double* A)
+{
+    const double gm0 = gamma;
+    const double gm1 = gamma - 1.0;
+    const double vv = P[1] * P[1] + P[2] * P[2] + P[3] * P[3];
+    const double BB = P[5] * P[5] + P[6] * P[6] + P[7] * P[7];
+    const double Bv = P[5] * P[1] + P[6] * P[2] + P[7] * P[3];
+    const double vn = P[1] * dAA[0] + P[2] * dAA[1] + P[3] * dAA[2];
+    const double W2 = 1.0 / (1.0 - vv);
+    const double W0 = sqrt(W2);
+    const double b0 = W0 * Bv;
+    const double bb = (BB + b0 * b0) / W2;
+    const double b1 = (P[5] + b0 * W0 * P[1]) / W0;
+    const double b2 = (P[6] + b0 * W0 * P[2]) / W0;
+    const double b3 = (P[7] + b0 * W0 * P[3]) / W0;
+    const double bn = b1 * dAA[0] + b2 * dAA[1] + b3 * dAA[2];
+    const double e0 = (P[4] / P[0]) / gm1;
+    const double p0 = P[4];
+    const double h0 = 1.0 + e0 + p0 / P[0];
+    const double cs2 = gm0 * p0 / (P[0] * h0);
+    const double W4 = W2 * W2;
+    const double v2 = vn * vn;
+    const double v3 = vn * v2;
+    const double v4 = vn * v3;
+    if (BB == 0.0)
+    {
+        const double K = sqrt(cs2 * (1 - vv) * (1 - vv * cs2 - v2 * (1 - cs2)));
+        A[0] = (vn * (1 - cs2) - K) / (1 - vv * cs2);
+        for (int n = 1; n < 7; ++n) A[n] = vn;
+        A[7] = (vn * (1 - cs2) + K) / (1 - vv * cs2);
+        return ERR_NONE;
+    }
+    const double K = W4 * (P[0] * h0 * (1. / cs2 - 1.));
+    const double L = -W2 * (P[0] * h0 + bb / cs2);
+    const double A4 = K - L - b0 * b0;
+    const double A3 = -4 * K * vn + L * vn * 2 + 2 * b0 * bn;
+    const double A2 = 6 * K * v2 + L * (1. - v2) + b0 * b0 - bn * bn;
+    const double A1 = -4 * K * v3 - L * vn * 2 - 2 * b0 * bn;
+    const double A0 = K * v4 + L * v2 + bn * bn;
+    double roots[4];
+    const int nr = quartic_solve(A4, A3, A2, A1, A0, roots);
+    const double C = sqrt(h0 + bb);
+    if (nr == 4)
+    {
+        A[0] = roots[0];
+        A[1] = (bn - C * vn * W0) / (b0 - C * W0);
+        A[2] = roots[1];
+        A[3] = vn;
+        A[4] = vn;
+        A[5] = roots[2];
+        A[6] = (bn + C * vn * W0) / (b0 + C * W0);
+        A[7] = roots[3];
+        return ERR_NONE;
+    }
+    if (nr == 2)
+    {
+        A[0] = roots[0];
+        A[1] = (bn - C * vn * W0) / (b0 - C * W0);
+        A[2] = vn;
+        A[3] = vn;
+        A[4] = vn;
+        A[5] = vn;
+        A[6] = (bn + C * vn * W0) / (b0 + C * W0);
+        A[7] = roots[1];
+        return ERR_NONE;
+    }
+    return ERR_EIGEN;
+}
+
+// RelativisticMHD::fromPrimitive, conserved and flux part
+// (src/ConservationLaws.cpp:659-732); eigenvalues always in binary64.
+template <typename R>
+__device__ __forceinline__ int rmhd_state(R gamma, const R* P, const R* dAA, State<R, 8>& S)
+{
+    const R gm1 = gamma - R(1.0);
+    const R vv = P[1] * P[1] + P[2] * P[2] + P[3] * P[3];
+    const R BB = P[5] * P[5] + P[6] * P[6] + P[7] * P[7];
+    const R Bv = P[5] * P[1] + P[6] * P[2] + P[7] * P[3];
+    const R vn = P[1] * dAA[0] + P[2] * dAA[1] + P[3] * dAA[2];
+    const R Bn = P[5] * dAA[0] + P[6] * dAA[1] + P[7] * dAA[2];
+    const R W2 = Math<R>::div(R(1.0), R(1.0) - vv);
+    const R W0 = Math<R>::sqrt(W2);
+    const R b0 = W0 * Bv;
+    const R bb = Math<R>::div(BB + b0 * b0, W2);
+    const R b1 = Math<R>::div(P[5] + b0 * W0 * P[1], W0);
+    const R b2 = Math<R>::div(P[6] + b0 * W0 * P[2], W0);
+    const R b3 = Math<R>::div(P[7] + b0 * W0 * P[3], W0);
+    const R e0 = Math<R>::div(Math<R>::div(P[4], P[0]), gm1);
+    const R es = e0 + Math<R>::div(R(0.5) * bb, P[0]);
+    const R p0 = P[4];
+    const R ps = p0 + R(0.5) * bb;
+    const R hs = R(1.0) + es + Math<R>::div(ps, P[0]);
+    if (vv >= R(1)) return ERR_SUPERLUMINAL;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) S.P[q] = P[q];
+    S.U[0] = P[0] * W0;
+    S.U[4] = P[0] * hs * W2 - ps - b0 * b0 - S.U[0];
+    S.U[1] = P[0] * hs * W2 * P[1] - b0 * b1;
+    S.U[2] = P[0] * hs * W2 * P[2] - b0 * b2;
+    S.U[3] = P[0] * hs * W2 * P[3] - b0 * b3;
+    S.U[5] = P[5];
+    S.U[6] = P[6];
+    S.U[7] = P[7];
+    S.F[0] = vn * S.U[0];
+    S.F[1] = vn * S.U[1] - Math<R>::div(Bn * b1, W0) + ps * dAA[0];
+    S.F[2] = vn * S.U[2] - Math<R>::div(Bn * b2, W0) + ps * dAA[1];
+    S.F[3] = vn * S.U[3] - Math<R>::div(Bn * b3, W0) + ps * dAA[2];
+    S.F[4] = vn * S.U[4] - Math<R>::div(Bn * b0, W0) + ps * vn;
+    S.F[5] = vn * S.U[5] - Bn * P[1];
+    S.F[6] = vn * S.U[6] - Bn * P[2];
+    S.F[7] = vn * S.U[7] - Bn * P[3];
+    double Pd[8], nd[3], Ad[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) Pd[q] = double(P[q]);
+    nd[0] = double(dAA[0]); nd[1] = double(dAA[1]); nd[2] = double(dAA[2]);
+    if (rmhd_eigen(double(gamma), Pd, nd, Ad) != ERR_NONE) return ERR_EIGEN;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) S.A[q] = R(Ad[q]);
+    return ERR_NONE;
+}
+
+// Law dispatch.  Law: 0 hydro, 1 MHD, 2 RMHD (include/mara_gpu.h).
+template <int Law> struct LawNQ { static constexpr int value = Law == 0 ? 5 : 8; };
+
+template <int Law, typename R>
+__device__ __forceinline__ int law_state(R gamma, const R* P, const R* n, State<R, LawNQ<Law>::value>& S)
+{
+    if constexpr (Law == 0) { hydro_state(gamma, P, n, S); return ERR_NONE; }
+    else if constexpr (Law == 1) { mhd_state(gamma, P, n, S); return ERR_NONE; }
+    else return rmhd_state(gamma, P, n, S);
+}
+
+// ---------------------------------------------------------------------------
+// Riemann solvers (src/RiemannSolvers.cpp:38-118)
+// ---------------------------------------------------------------------------
+// std::max_element / min_element over the reference's 10-slot eigenvalue
+// array: the slots beyond NQ hold zeros.
+template <typename R, int NQ>
+__device__ __forceinline__ R max_elem(const R* A)
+{
+    R m = A[0];
+#pragma unroll
+    for (int n = 1; n < NQ; ++n) if (m < A[n]) m = A[n];
+    if (m < R(0)) m = R(0);
+    return m;
+}
+
+template <typename R, int NQ>
+__device__ __forceinline__ R min_elem(const R* A)
+{
+    R m = A[0];
+#pragma unroll
+    for (int n = 1; n < NQ; ++n) if (A[n] < m) m = A[n];
+    if (R(0) < m) m = R(0);
+    return m;
+}
+
+template <typename R, int NQ>
+__device__ __forceinline__ void hlle(const State<R, NQ>& L, const State<R, NQ>& Rs, R* F)
+{
+    const R epl = max_elem<R, NQ>(L.A);
+    const R eml = min_elem<R, NQ>(L.A);
+    const R epr = max_elem<R, NQ>(Rs.A);
+    const R emr = min_elem<R, NQ>(Rs.A);
+    const R ap = max3(R(0.0), epl, epr);
+    const R am = min3(R(0.0), eml, emr);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+        F[q] = Math<R>::div(ap * L.F[q] - am * Rs.F[q] + ap * am * (Rs.U[q] - L.U[q]), ap - am);
+}
+
+template <typename R>
+__device__ __forceinline__ void hllc(const State<R, 5>& L, const State<R, 5>& Rs, const R* dA, R* Fout)
+{
+    const R epl = max_elem<R, 5>(L.A);
+    const R eml = min_elem<R, 5>(L.A);
+    const R epr = max_elem<R, 5>(Rs.A);
+    const R emr = min_elem<R, 5>(Rs.A);
+    const R ap = max3(R(0.0), epl, epr);
+    const R am = min3(R(0.0), eml, emr);
+    const R* Pl = L.P;
+    const R* Pr = Rs.P;
+    const R* Ul = L.U;
+    const R* Ur = Rs.U;
+    const R vnr = Pr[1] * dA[0] + Pr[2] * dA[1] + Pr[3] * dA[2];
+    const R vnl = Pl[1] * dA[0] + Pl[2] * dA[1] + Pl[3] * dA[2];
+    const R lc = Math<R>::div((Pr[4] - Pr[0] * vnr * (ap - vnr)) - (Pl[4] - Pl[0] * vnl * (am - vnl)),
+                              Pl[0] * (am - vnl) - Pr[0] * (ap - vnr));
+    const R fl = Math<R>::div(Pl[0] * (am - vnl), am - lc);
+    const R fr = Math<R>::div(Pr[0] * (ap - vnr), ap - lc);
+    R Ul_[5], Ur_[5];
+    Ul_[0] = fl;
+    Ul_[4] = fl * (Math<R>::div(Ul[4], Pl[0]) + (lc - vnl) * (lc + Math<R>::div(Pl[4], Pl[0] * (am - vnl))));
+    Ul_[1] = fl * ((lc - vnl) * dA[0] + Pl[1]);
+    Ul_[2] = fl * ((lc - vnl) * dA[1] + Pl[2]);
+    Ul_[3] = fl * ((lc - vnl) * dA[2] + Pl[3]);
+    Ur_[0] = fr;
+    Ur_[4] = fr * (Math<R>::div(Ur[4], Pr[0]) + (lc - vnr) * (lc + Math<R>::div(Pr[4], Pr[0] * (ap - vnr))));
+    Ur_[1] = fr * ((lc - vnr) * dA[0] + Pr[1]);
+    Ur_[2] = fr * ((lc - vnr) * dA[1] + Pr[2]);
+    Ur_[3] = fr * ((lc - vnr) * dA[2] + Pr[3]);
+    const R s = R(0.0);
+    if (s <= am)
+    {
+#pragma unroll
+        for (int i = 0; i < 5; ++i) Fout[i] = L.F[i];
+    }
+    else if (am < s && s <= lc)
+    {
+#pragma unroll
+        for (int i = 0; i < 5; ++i) Fout[i] = L.F[i] + am * (Ul_[i] - Ul[i]);
+    }
+    else if (lc < s && s <= ap)
+    {
+#pragma unroll
+        for (int i = 0; i < 5; ++i) Fout[i] = Rs.F[i] + ap * (Ur_[i] - Ur[i]);
+    }
+    else if (ap < s)
+    {
+#pragma unroll
+        for (int i = 0; i < 5; ++i) Fout[i] = Rs.F[i];
+    }
+    else
+    {
+#pragma unroll
+        for (int i = 0; i < 5; ++i) Fout[i] = R(0);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Primitive <-> conserved for the zone update (binary64 in both forms)
+// ---------------------------------------------------------------------------
+// U part of fromPrimitive, as FieldOperator::generateConserved uses it.
+template <int Law>
+__device__ __forceinline__ int prim_to_cons(double gamma, const double* P, double* U)
+{
+    if constexpr (Law == 0)
+    {
+        const double gm1 = gamma - 1.0;
+        const double vv = P[1] * P[1] + P[2] * P[2] + P[3] * P[3];
+        U[0] = P[0];
+        U[1] = P[0] * P[1];
+        U[2] = P[0] * P[2];
+        U[3] = P[0] * P[3];
+        U[4] = P[0] * 0.5 * vv + P[4] / gm1;
+        return ERR_NONE;
+    }
+    else if constexpr (Law == 1)
+    {
+        const double gm1 = gamma - 1.0;
+        const double vv = P[1] * P[1] + P[2] * P[2] + P[3] * P[3];
+        const double BB = P[5] * P[5] + P[6] * P[6] + P[7] * P[7];
+        U[0] = P[0];
+        U[1] = P[0] * P[1];
+        U[2] = P[0] * P[2];
+        U[3] = P[0] * P[3];
+        U[4] = P[0] * 0.5 * vv + 0.5 * BB + P[4] / gm1;
+        U[5] = P[5];
+        U[6] = P[6];
+        U[7] = P[7];
+        return ERR_NONE;
+    }
+    else
+    {
+        const double gm1 = gamma - 1.0;
+        const double vv = P[1] * P[1] + P[2] * P[2] + P[3] * P[3];
+        const double BB = P[5] * P[5] + P[6] * P[6] + P[7] * P[7];
+        const double Bv = P[5] * P[1] + P[6] * P[2] + P[7] * P[3];
+        const double W2 = 1.0 / (1.0 - vv);
+        const double W0 = sqrt(W2);
+        const double b0 = W0 * Bv;
+        const double bb = (BB + b0 * b0) / W2;
+        const double b1 = (P[5] + b0 * W0 * P[1]) / W0;
+        const double b2 = (P[6] + b0 * W0 * P[2]) / W0;
+        const double b3 = (P[7] + b0 * W0 * P[3]) / W0;
+        const double e0 = (P[4] / P[0]) / gm1;
+        const double es = e0 + 0.5 * bb / P[0];
+        const double p0 = P[4];
+        const double ps = p0 + 0.5 * bb;
+        const double hs = 1.0 + es + ps / P[0];
+        if (vv >= 1) return ERR_SUPERLUMINAL;
+        U[0] = P[0] * W0;
+        U[4] = P[0] * hs * W2 - ps - b0 * b0 - U[0];
+        U[1] = P[0] * hs * W2 * P[1] - b0 * b1;
+        U[2] = P[0] * hs * W2 * P[2] - b0 * b2;
+        U[3] = P[0] * hs * W2 * P[3] - b0 * b3;
+        U[5] = P[5];
+        U[6] = P[6];
+        U[7] = P[7];
+        return ERR_NONE;
+    }
+}
+
+// ConsToPrimRMHD (src/ConsToPrimRMHD.cpp:77-323): Anton 2D Newton, then Noble.
+struct C2P
+{
+    double gamf, pfloor, D, Tau, S2, B2, BS, BS2, Cons[8], Z_start, W_start;
+};
+
+__device__ __forceinline__ bool finite8(const double* x)
+{
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) ok = ok && isfinite(x[i]);
+    return ok;
+}
+
+__device__ __forceinline__ int c2p_check_cons(const double* U)
+{
+    if (U[0] < 0.0) return 2;
+    if (U[4] < 0.0) return 3;
+    if (!finite8(U)) return 1;
+    return 0;
+}
+
+__device__ __forceinline__ int c2p_reconstruct(const C2P& s, double Z, double W, double* Pout)
+{
+    double P[8];
+    const double b0 = s.BS * W / Z;
+    P[0] = s.D / W;
+    P[4] = (s.D / W) * (Z / (s.D * W) - 1.0) * s.gamf;
+    P[1] = (s.Cons[1] + b0 * s.Cons[5] / W) / (Z + s.B2);
+    P[2] = (s.Cons[2] + b0 * s.Cons[6] / W) / (Z + s.B2);
+    P[3] = (s.Cons[3] + b0 * s.Cons[7] / W) / (Z + s.B2);
+    P[5] = s.Cons[5];
+    P[6] = s.Cons[6];
+    P[7] = s.Cons[7];
+    if (P[4] <= 0.0 && s.pfloor > 0.0) P[4] = P[0] * s.pfloor;
+    const double v2 = P[1] * P[1] + P[2] * P[2] + P[3] * P[3];
+    if (v2 >= 1.0) return 7;
+    if (P[4] < 0.0) return 5;
+    if (P[0] < 0.0) return 6;
+    if (!finite8(P)) return 4;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) Pout[i] = P[i];
+    return 0;
+}
+
+__device__ inline int c2p_anton(const C2P& s, double* P)
+{
+    if (int bad = c2p_check_cons(s.Cons)) return bad;
+    int iterations = 0;
+    double error = 1.0;
+    double W = s.W_start, Z = s.Z_start;
+    const double D = s.D, Tau = s.Tau, S2 = s.S2, B2 = s.B2, BS2 = s.BS2, gamf = s.gamf;
+    while (error > 1e-12)
+    {
+        const double Z2 = Z * Z;
+        const double Z3 = Z * Z2;
+        const double W2 = W * W;
+        const double W3 = W * W2;
+        const double Pre = (D / W) * (Z / (D * W) - 1.0) * gamf;
+        const double df0dZ = 2 * (B2 + Z) * (BS2 * W2 + (W2 - 1) * Z3) / (W2 * Z3);
+        const double df0dW = 2 * (B2 + Z) * (B2 + Z) / W3;
+        const double df1dZ = 1.0 + BS2 / Z3 - gamf / W2;
+        const double df1dW = B2 / W3 + (2 * Z - D * W) / W3 * gamf;
+        const double f0 = -S2 + (Z + B2) * (Z + B2) * (W2 - 1) / W2 - (2 * Z + B2) * BS2 / Z2;
+        const double f1 = -Tau + Z + B2 - Pre - 0.5 * B2 / W2 - 0.5 * BS2 / Z2 - D;
+        const double det = df0dZ * df1dW - df0dW * df1dZ;
+        const double G0 = df1dW / det, G1 = -df0dW / det;
+        const double G2 = -df1dZ / det, G3 = df0dZ / det;
+        const double dZ = -(G0 * f0 + G1 * f1);
+        const double dW = -(G2 * f0 + G3 * f1);
+        double Z_new = Z + dZ;
+        double W_new = W + dW;
+        Z_new = (Z_new > 0.0) ? Z_new : -Z_new;
+        Z_new = (Z_new < 1e20) ? Z_new : Z;
+        W_new = (W_new > 1.0) ? W_new : 1.0;
+        W_new = (W_new < 1e12) ? W_new : 1e12;
+        Z = Z_new;
+        W = W_new;
+        error = fabs(dZ / Z) + fabs(dW / W);
+        ++iterations;
+        if (iterations == 250) return 8;
+    }
+    return c2p_reconstruct(s, Z, W, P);
+}
+
+__device__ inline int c2p_noble(const C2P& s, double* P)
+{
+    if (int bad = c2p_check_cons(s.Cons)) return bad;
+    int iterations = 0;
+    double error = 1.0;
+    double Z = s.Z_start;
+    const double D = s.D, Tau = s.Tau, S2 = s.S2, B2 = s.B2, BS2 = s.BS2, gamf = s.gamf;
+    while (error > 1e-12)
+    {
+        const double Z2 = Z * Z;
+        const double Z3 = Z * Z2;
+        const double a = S2 * Z2 + BS2 * (B2 + 2 * Z);
+        const double b = (B2 + Z) * (B2 + Z) * Z2;
+        const double ap = 2 * (S2 * Z + BS2);
+        const double bp = 2 * Z * (B2 + Z) * (B2 + 2 * Z);
+        const double V2 = a / b;
+        const double W2 = 1.0 / (1.0 - V2);
+        const double W = sqrt(W2);
+        const double W3 = W * W2;
+        const double Pre = (D / W) * (Z / (D * W) - 1.0) * gamf;
+        const double dv2dZ = (ap * b - bp * a) / (b * b);
+        const double delPdelZ = gamf / W2;
+        const double delPdelW = gamf * (D / W2 - 2 * Z / W3);
+        const double dWdv2 = 0.5 * W3;
+        const double dPdZ = delPdelW * dWdv2 * dv2dZ + delPdelZ;
+        const double f = Tau + D - 0.5 * B2 * (1 + V2) + 0.5 * BS2 / Z2 - Z + Pre;
+        const double g = -0.5 * B2 * dv2dZ - BS2 / Z3 - 1.0 + dPdZ;
+        const double dZ = -f / g;
+        double Z_new = Z + dZ;
+        Z_new = (Z_new > 0.0) ? Z_new : -Z_new;
+        Z_new = (Z_new < 1e20) ? Z_new : Z;
+        Z = Z_new;
+        error = fabs(dZ / Z);
+        ++iterations;
+        if (iterations == 250) return 8;
+    }
+    const double Z2 = Z * Z;
+    const double a = S2 * Z2 + BS2 * (B2 + 2 * Z);
+    const double b = (B2 + Z) * (B2 + Z) * Z2;
+    const double V2 = a / b;
+    const double W2 = 1.0 / (1.0 - V2);
+    const double W = sqrt(W2);
+    return c2p_reconstruct(s, Z, W, P);
+}
+
+// fromConserved -> primitive (src/ConservationLaws.cpp:238-251, 370-414,
+// 627-657).  Returns 0 on success, 1 on inversion failure; floor = 1 when
+// the MHD pressure floor was used.
+template <int Law>
+__device__ __forceinline__ int cons_to_prim(double gamma, double pfloor, const double* U, double* P, int& floor)
+{
+    floor = 0;
+    if constexpr (Law == 0)
+    {
+        const double gm1 = gamma - 1.0;
+        const double pp = U[1] * U[1] + U[2] * U[2] + U[3] * U[3];
+        P[0] = U[0];
+        P[4] = (U[4] - 0.5 * pp / U[0]) * gm1;
+        P[1] = U[1] / U[0];
+        P[2] = U[2] / U[0];
+        P[3] = U[3] / U[0];
+        return 0;
+    }
+    else if constexpr (Law == 1)
+    {
+        const double gm1 = gamma - 1.0;
+        const double pp = U[1] * U[1] + U[2] * U[2] + U[3] * U[3];
+        const double BB = U[5] * U[5] + U[6] * U[6] + U[7] * U[7];
+        P[0] = U[0];
+        P[4] = (U[4] - 0.5 * pp / U[0] - 0.5 * BB) * gm1;
+        P[1] = U[1] / U[0];
+        P[2] = U[2] / U[0];
+        P[3] = U[3] / U[0];
+        P[5] = U[5];
+        P[6] = U[6];
+        P[7] = U[7];
+        if (P[4] < 0.0 || P[0] < 0.0 || U[0] < 0.0 || U[4] < 0.0)
+        {
+            if (P[4] < 0.0 && pfloor > 0.0)
+            {
+                P[4] = pfloor * P[0];
+                floor = 1;
+                return 0;
+            }
+            return 1;
+        }
+        return 0;
+    }
+    else
+    {
+        C2P s;
+        s.gamf = (gamma - 1.0) / gamma;
+        s.pfloor = pfloor;
+        s.D = U[0];
+        s.Tau = U[4];
+        s.S2 = U[1] * U[1] + U[2] * U[2] + U[3] * U[3];
+        s.B2 = U[5] * U[5] + U[6] * U[6] + U[7] * U[7];
+        s.BS = U[5] * U[1] + U[6] * U[2] + U[7] * U[3];
+        s.BS2 = s.BS * s.BS;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) s.Cons[i] = U[i];
+        s.Z_start = sqrt(s.S2 + s.D * s.D);
+        s.W_start = s.Z_start / s.D;
+        if (c2p_anton(s, P) == 0) return 0;
+        if (c2p_noble(s, P) == 0) return 0;
+        return 1;
+    }
+}
+
+} // namespace mara

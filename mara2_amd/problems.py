@@ -1,0 +1,297 @@
+"""Seeded initial data, in MeshData::P layout with guard zones filled.
+
+Problems of the reference's regression programs (src/Problems.cpp:35-671)
+and the synthetic workloads the benchmark and parity tests run:
+
+  sod_1d          1-D Sod tube, 1024 zones, gamma 1.4, PLM + HLLE, RK2, CFL 0.4
+  turbulence_3d   3-D decaying turbulence, periodic cube, gamma 5/3, PLM + HLLC, RK2
+  mhd_2d          2-D ideal MHD, B = curl(A_z z) by the reference's CT initialisation
+  rmhd_3d         3-D relativistic MHD, WENO5 + HLLE, CT, RK3
+  riemann_problem one Riemann problem of the serving stream (1-D or 2-D, hydro or MHD)
+
+Random fields are sums of Fourier modes with integer wave vectors in a band
+kmin <= |k| <= kmax, complex Gaussian amplitudes with a power-law spectrum
+and a numpy Generator seed.  All arithmetic here is host-side setup, done once.
+"""
+import numpy as np
+
+from . import abi
+
+SEED_TURBULENCE = 2026120202
+SEED_MHD = 2026120203
+SEED_RMHD = 2026120204
+
+
+# ---------------------------------------------------------------------------
+# guard fill (BoundaryCondition::applySimple, src/BoundaryConditions.cpp:8-176)
+# ---------------------------------------------------------------------------
+def apply_bc(desc, P):
+    """Fill the guard zones of P in place, axis by axis, left then right."""
+    nq = P.shape[3]
+    for axis in range(3):
+        g = desc.guards[axis]
+        if g <= 0:
+            continue
+        N = P.shape[axis]
+        for side in range(2):
+            for n in range(g):
+                if desc.bc == abi.BC_PERIODIC:
+                    dst, src = (n, N - 2 * g + n) if side == 0 else (N - g + n, g + n)
+                elif desc.bc == abi.BC_OUTFLOW:
+                    dst, src = (n, g) if side == 0 else (N - 1 - n, N - g - 1)
+                else:
+                    dst, src = (n, 2 * g - n - 1) if side == 0 else (N - 1 - n, N - 2 * g + n)
+                d = [slice(None)] * 4
+                s = [slice(None)] * 4
+                d[axis] = dst
+                s[axis] = src
+                P[tuple(d)] = P[tuple(s)]
+                if desc.bc == abi.BC_REFLECTING:
+                    d[3] = 1 + axis
+                    P[tuple(d)] *= -1
+    assert nq == P.shape[3]
+    return P
+
+
+def empty(desc):
+    return np.zeros(abi.array_shape(desc))
+
+
+def cell_centres(desc, axis):
+    """Cell-centre coordinates along an axis, guards included
+    (CartesianMeshGeometry::coordinateAtIndex, src/CartesianMeshGeometry.cpp:61-67)."""
+    n, g = desc.cells[axis], desc.guards[axis]
+    dx = (desc.upper[axis] - desc.lower[axis]) / n
+    ic = np.arange(n + 2 * g) - g
+    return desc.lower[axis] + dx * (ic + 0.5)
+
+
+# ---------------------------------------------------------------------------
+# Gaussian random fields on a periodic grid
+# ---------------------------------------------------------------------------
+def _spectrum(rng, n_axes, kmin, kmax, slope):
+    """Random complex amplitudes on the integer modes of the band, stored on
+    a small cube of side 2 kmax + 1 (index kmax is k = 0)."""
+    m = 2 * kmax + 1
+    ks = np.arange(m) - kmax
+    grids = np.meshgrid(*([ks] * n_axes), indexing="ij")
+    kk = np.sqrt(sum(g.astype(np.float64) ** 2 for g in grids))
+    band = (kk >= kmin) & (kk <= kmax)
+    amp = np.zeros(kk.shape)
+    amp[band] = kk[band] ** (slope / 2.0)
+    z = rng.standard_normal(kk.shape) + 1j * rng.standard_normal(kk.shape)
+    return z * amp, ks
+
+
+def _evaluate(spec, ks, n, shifts):
+    """Real part of sum_k a_k exp(2 pi i k.(i + s)/n) on an n^d grid, with the
+    shift s given in cells (s = 0.5 samples half a cell over)."""
+    d = spec.ndim
+    full = np.zeros((n,) * d, dtype=np.complex128)
+    idx = np.ix_(*([ks % n] * d))
+    phase = np.ones(spec.shape, dtype=np.complex128)
+    for a in range(d):
+        shape = [1] * d
+        shape[a] = len(ks)
+        phase = phase * np.exp(2j * np.pi * ks.reshape(shape) * shifts[a] / n)
+    full[idx] += spec * phase
+    field = np.fft.ifftn(full) * full.size
+    return field.real.copy()
+
+
+def random_field(rng, n_axes, n, kmin, kmax, slope=0.0, shifts=None):
+    spec, ks = _spectrum(rng, n_axes, kmin, kmax, slope)
+    return _evaluate(spec, ks, n, shifts or (0.0,) * n_axes)
+
+
+# ---------------------------------------------------------------------------
+# vector potential -> cell-centred B (src/Problems.cpp:660-667 with
+# CellCenteredFieldCT::vectorPotentialToFluxes / correctGodunovFluxes and
+# MeshOperator::divergence(G, +1))
+# ---------------------------------------------------------------------------
+def ct_correct(F):
+    """CellCenteredFieldCT::correctGodunovFluxes on F of shape (s0,s1,s2,3,3)
+    with magnetic index 0 (src/CellCenteredFieldCT.cpp:14-134)."""
+    G = F.copy()
+    s0, s1, s2 = F.shape[:3]
+    G[..., 0, 0] = 0.0
+    G[..., 1, 1] = 0.0
+    G[..., 2, 2] = 0.0
+    if s0 >= 3:
+        G[1:, 1:-1, :, 1, 0] = 0.125 * (2 * F[1:, 1:-1, :, 1, 0] + F[1:, 2:, :, 1, 0] + F[1:, :-2, :, 1, 0]
+                                        - F[1:, 2:, :, 0, 1] - F[:-1, 2:, :, 0, 1]
+                                        - F[1:, 1:-1, :, 0, 1] - F[:-1, 1:-1, :, 0, 1])
+        G[1:, :, 1:-1, 2, 0] = 0.125 * (2 * F[1:, :, 1:-1, 2, 0] + F[1:, :, 2:, 2, 0] + F[1:, :, :-2, 2, 0]
+                                        - F[1:, :, 2:, 0, 2] - F[:-1, :, 2:, 0, 2]
+                                        - F[1:, :, 1:-1, 0, 2] - F[:-1, :, 1:-1, 0, 2])
+    if s1 >= 3:
+        G[:, 1:, 1:-1, 2, 1] = 0.125 * (2 * F[:, 1:, 1:-1, 2, 1] + F[:, 1:, 2:, 2, 1] + F[:, 1:, :-2, 2, 1]
+                                        - F[:, 1:, 2:, 1, 2] - F[:, :-1, 2:, 1, 2]
+                                        - F[:, 1:, 1:-1, 1, 2] - F[:, :-1, 1:-1, 1, 2])
+        G[1:-1, 1:, :, 0, 1] = 0.125 * (2 * F[1:-1, 1:, :, 0, 1] + F[2:, 1:, :, 0, 1] + F[:-2, 1:, :, 0, 1]
+                                        - F[2:, 1:, :, 1, 0] - F[2:, :-1, :, 1, 0]
+                                        - F[1:-1, 1:, :, 1, 0] - F[1:-1, :-1, :, 1, 0])
+    if s2 >= 3:
+        G[1:-1, :, 1:, 0, 2] = 0.125 * (2 * F[1:-1, :, 1:, 0, 2] + F[2:, :, 1:, 0, 2] + F[:-2, :, 1:, 0, 2]
+                                        - F[2:, :, 1:, 2, 0] - F[2:, :, :-1, 2, 0]
+                                        - F[1:-1, :, 1:, 2, 0] - F[1:-1, :, :-1, 2, 0])
+        G[:, 1:-1, 1:, 1, 2] = 0.125 * (2 * F[:, 1:-1, 1:, 1, 2] + F[:, 2:, 1:, 1, 2] + F[:, :-2, 1:, 1, 2]
+                                        - F[:, 2:, 1:, 2, 1] - F[:, 2:, :-1, 2, 1]
+                                        - F[:, 1:-1, 1:, 2, 1] - F[:, 1:-1, :-1, 2, 1])
+    return G
+
+
+def divergence(F, factor, dA, dV):
+    """MeshOperator::divergence (src/MeshOperator.cpp:323-381), uniform mesh."""
+    f = factor / dV
+    return f * (+F[1:, :-1, :-1, :, 0] * dA[0] - F[:-1, :-1, :-1, :, 0] * dA[0]
+                + F[:-1, 1:, :-1, :, 1] * dA[1] - F[:-1, :-1, :-1, :, 1] * dA[1]
+                + F[:-1, :-1, 1:, :, 2] * dA[2] - F[:-1, :-1, :-1, :, 2] * dA[2])
+
+
+def spacing(desc):
+    dx = [(desc.upper[a] - desc.lower[a]) / desc.cells[a] for a in range(3)]
+    dA = (dx[1] * dx[2], dx[2] * dx[0], dx[0] * dx[1])
+    return dx, dA, dx[0] * dx[1] * dx[2]
+
+
+def vector_potential_to_B(desc, A):
+    """A: (N1+1, N2+1, N3+1, 3, 3) sampled like MeshOperator::generate(face,
+    scalars); returns cell B with guards, shape (N1, N2, N3, 3)."""
+    F = np.zeros(A.shape)
+    F[..., 1, 0] = -A[..., 2, 0]
+    F[..., 2, 0] = +A[..., 1, 0]
+    F[..., 2, 1] = -A[..., 0, 1]
+    F[..., 0, 1] = +A[..., 2, 1]
+    F[..., 0, 2] = -A[..., 1, 2]
+    F[..., 1, 2] = +A[..., 0, 2]
+    G = ct_correct(F)
+    _, dA, dV = spacing(desc)
+    return divergence(G, 1.0, dA, dV)
+
+
+def _face_samples(desc, field_at):
+    """Build A (N1+1, N2+1, N3+1, 3, 3): component q on the faces of axis a is
+    field_at(q, a) -> periodic array on the interior grid sampled at the face
+    centre; guard faces wrap periodically."""
+    N = [desc.cells[a] + 2 * desc.guards[a] for a in range(3)]
+    A = np.zeros((N[0] + 1, N[1] + 1, N[2] + 1, 3, 3))
+    for a in range(3):
+        for q in range(3):
+            vals = field_at(q, a)
+            if vals is None:
+                continue
+            vals = vals.reshape(desc.cells[0], desc.cells[1], desc.cells[2])
+            idx = [(np.arange(N[e] + 1) - desc.guards[e]) % desc.cells[e] for e in range(3)]
+            A[..., q, a] = vals[np.ix_(*idx)]
+    return A
+
+
+# ---------------------------------------------------------------------------
+# workloads
+# ---------------------------------------------------------------------------
+def sod_desc(n=1024, precision="fp64"):
+    return abi.make_desc(law="hydro", gamma=1.4, recon="plm", plm_theta=1.5, riemann="hlle",
+                         bc="outflow", cells=(n, 1, 1), rk_order=2, precision=precision)
+
+
+def sod_1d(desc):
+    """Sod tube: x < 0.5 (rho, v, p) = (1, 0, 1), else (0.125, 0, 0.1)."""
+    P = empty(desc)
+    x = cell_centres(desc, 0)
+    P[:, 0, 0, 0] = np.where(x < 0.5, 1.0, 0.125)
+    P[:, 0, 0, 4] = np.where(x < 0.5, 1.0, 0.1)
+    return apply_bc(desc, P)
+
+
+def turbulence_desc(n=256, precision="fp64"):
+    return abi.make_desc(law="hydro", gamma=5.0 / 3.0, recon="plm", plm_theta=1.5, riemann="hllc",
+                         bc="periodic", cells=(n, n, n), rk_order=2, precision=precision)
+
+
+def turbulence_3d(desc, seed=SEED_TURBULENCE, kmin=1, kmax=16, slope=-4.0, mach=1.0):
+    """rho = 1, p = 1; each velocity component a Gaussian random field on
+    kmin <= |k| <= kmax with |v_k|^2 ~ k^slope; rescaled to rms Mach `mach`."""
+    n = desc.cells[0]
+    rng = np.random.default_rng(seed)
+    v = np.stack([random_field(rng, 3, n, kmin, kmax, slope) for _ in range(3)], axis=-1)
+    cs = np.sqrt(desc.gamma * 1.0 / 1.0)
+    v *= mach * cs / np.sqrt(np.mean(np.sum(v * v, axis=-1)))
+    P = empty(desc)
+    g = desc.guards[0]
+    inner = (slice(g, g + n),) * 3
+    P[..., 0] = 1.0
+    P[..., 4] = 1.0
+    P[inner + (slice(1, 4),)] = v
+    return apply_bc(desc, P)
+
+
+def mhd_desc(n=2048, precision="fp64"):
+    return abi.make_desc(law="mhd", gamma=5.0 / 3.0, recon="plm", plm_theta=1.5, riemann="hlle",
+                         bc="periodic", cells=(n, n, 1), rk_order=2, field_ct=1, precision=precision)
+
+
+def mhd_2d(desc, seed=SEED_MHD, kmax=8, beta=1.0, vrms=0.5):
+    """rho = 1, p = 1; B = curl(A_z z) via the reference's CT initialisation,
+    A_z a Gaussian random field on 1 <= |k| <= kmax scaled to plasma beta
+    2 p / <B^2> = beta; in-plane v random on the same band with rms |v| = vrms."""
+    n = desc.cells[0]
+    rng = np.random.default_rng(seed)
+    spec, ks = _spectrum(rng, 2, 1, kmax, 0.0)
+
+    def field_at(q, a):
+        if q != 2 or a == 2:
+            return None
+        shifts = (0.0, 0.5) if a == 0 else (0.5, 0.0)  # x-faces / y-faces of a cell
+        return _evaluate(spec, ks, n, shifts)
+
+    A = _face_samples(desc, field_at)
+    B = vector_potential_to_B(desc, A)
+    g = desc.guards
+    inner = (slice(g[0], g[0] + n), slice(g[1], g[1] + n), slice(0, 1))
+    scale = np.sqrt(2.0 * 1.0 / beta / np.mean(np.sum(B[inner] ** 2, axis=-1)))
+    B = vector_potential_to_B(desc, A * scale)
+    v = np.stack([random_field(rng, 2, n, 1, kmax, 0.0) for _ in range(2)], axis=-1)
+    v *= vrms / np.sqrt(np.mean(np.sum(v * v, axis=-1)))
+    P = empty(desc)
+    P[..., 0] = 1.0
+    P[..., 4] = 1.0
+    P[..., 5:8] = B
+    P[inner[0], inner[1], 0, 1:3] = v
+    return apply_bc(desc, P)
+
+
+def rmhd_desc(n=128, precision="fp64"):
+    return abi.make_desc(law="rmhd", gamma=4.0 / 3.0, recon="weno5", riemann="hlle", bc="periodic",
+                         cells=(n, n, n), rk_order=3, field_ct=1, precision=precision)
+
+
+def rmhd_3d(desc, seed=SEED_RMHD, kmax=8, wmax=5.0, sigma=1.0):
+    """rho = 1, p = 0.1; spatial four-velocity u random on 1 <= |k| <= kmax,
+    scaled so max W = sqrt(1 + u^2) = wmax, v = u / W; B = curl A with A a
+    random vector potential on the same band, scaled to <B^2> / rho = sigma."""
+    n = desc.cells[0]
+    rng = np.random.default_rng(seed)
+    u = np.stack([random_field(rng, 3, n, 1, kmax, 0.0) for _ in range(3)], axis=-1)
+    u *= np.sqrt(wmax * wmax - 1.0) / np.sqrt(np.max(np.sum(u * u, axis=-1)))
+    W = np.sqrt(1.0 + np.sum(u * u, axis=-1))
+    v = u / W[..., None]
+    specs = [_spectrum(rng, 3, 1, kmax, 0.0) for _ in range(3)]
+    face_shift = {0: (0.0, 0.5, 0.5), 1: (0.5, 0.0, 0.5), 2: (0.5, 0.5, 0.0)}
+
+    def field_at(q, a):
+        spec, ks = specs[q]
+        return _evaluate(spec, ks, n, face_shift[a])
+
+    A = _face_samples(desc, field_at)
+    B = vector_potential_to_B(desc, A)
+    g = desc.guards
+    inner = tuple(slice(g[a], g[a] + n) for a in range(3))
+    scale = np.sqrt(sigma * 1.0 / np.mean(np.sum(B[inner] ** 2, axis=-1)))
+    B = vector_potential_to_B(desc, A * scale)
+    P = empty(desc)
+    P[..., 0] = 1.0
+    P[..., 4] = 0.1
+    P[..., 5:8] = B
+    P[inner + (slice(1, 4),)] = v
+    return apply_bc(desc, P)

@@ -1,0 +1,156 @@
+"""ctypes binding of libmara_gpu.so (the C ABI of include/mara_gpu.h).
+
+The HIP library is the only compute path: if it is missing, loading raises
+instead of falling back to anything else.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import abi
+from .abi import MaraDesc, MaraReport, array_shape
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmara_gpu.so")
+_LIB = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class MaraError(RuntimeError):
+    """A non-zero status from the C ABI, with the text of mara_error_string."""
+
+    def __init__(self, status, where, report=None):
+        self.status = status
+        self.report = report
+        text = load().mara_error_string(status).decode()
+        super().__init__(f"{where}: {text} (status {status})" + (f" {report}" if report else ""))
+
+
+def load():
+    """Load libmara_gpu.so once; raise if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                          "(make -C mara2_amd/csrc); there is no other compute path")
+    L = ctypes.CDLL(LIB_PATH)
+    blk = ctypes.c_void_p
+    L.mara_version.restype = ctypes.c_char_p
+    L.mara_error_string.restype = ctypes.c_char_p
+    L.mara_error_string.argtypes = [ctypes.c_int]
+    L.mara_num_fields.argtypes = [ctypes.c_int]
+    L.mara_stencil_size.argtypes = [ctypes.c_int]
+    L.mara_desc_device_bytes.argtypes = [ctypes.POINTER(MaraDesc)]
+    L.mara_desc_device_bytes.restype = ctypes.c_longlong
+    L.mara_block_create.argtypes = [ctypes.POINTER(MaraDesc), ctypes.POINTER(blk)]
+    L.mara_block_destroy.argtypes = [blk]
+    L.mara_block_upload_prim.argtypes = [blk, _dp]
+    L.mara_block_download_prim.argtypes = [blk, _dp]
+    L.mara_block_advance.argtypes = [blk, ctypes.c_double, ctypes.c_double, ctypes.POINTER(MaraReport)]
+    L.mara_block_courant_dt.argtypes = [blk, _dp]
+    L.mara_block_apply_bc.argtypes = [blk]
+    L.mara_block_monopole_max.argtypes = [blk, _dp]
+    L.mara_block_run.argtypes = [blk, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp,
+                                 ctypes.POINTER(ctypes.c_int), ctypes.POINTER(MaraReport)]
+    L.mara_block_set_timing.argtypes = [blk, ctypes.c_int]
+    L.mara_block_stage_kernel_time.argtypes = [blk, _dp, ctypes.POINTER(ctypes.c_longlong),
+                                               ctypes.POINTER(ctypes.c_char_p)]
+    L.mara_block_device_bytes.argtypes = [blk]
+    L.mara_block_device_bytes.restype = ctypes.c_longlong
+    _LIB = L
+    return L
+
+
+def _check(status, where, report=None):
+    if status != abi.OK:
+        raise MaraError(status, where, report.as_dict() if report is not None else None)
+
+
+class Block:
+    """One device-resident mesh block (MeshData + MethodOfLinesTVD on the GPU)."""
+
+    def __init__(self, desc: MaraDesc):
+        self.lib = load()
+        self.desc = desc
+        self.shape = array_shape(desc)
+        self._h = ctypes.c_void_p()
+        _check(self.lib.mara_block_create(ctypes.byref(desc), ctypes.byref(self._h)), "mara_block_create")
+
+    def close(self):
+        if self._h:
+            self.lib.mara_block_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def device_bytes(self):
+        return self.lib.mara_block_device_bytes(self._h)
+
+    def upload(self, P):
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        assert P.shape == self.shape, (P.shape, self.shape)
+        _check(self.lib.mara_block_upload_prim(self._h, P.ctypes.data_as(_dp)), "mara_block_upload_prim")
+
+    def download(self):
+        P = np.empty(self.shape)
+        _check(self.lib.mara_block_download_prim(self._h, P.ctypes.data_as(_dp)), "mara_block_download_prim")
+        return P
+
+    def advance(self, dt, t0=0.0, check=True):
+        rep = MaraReport()
+        st = self.lib.mara_block_advance(self._h, t0, dt, ctypes.byref(rep))
+        if check:
+            _check(st, "mara_block_advance", rep)
+        return st, rep
+
+    def courant_dt(self):
+        dt = ctypes.c_double(0)
+        _check(self.lib.mara_block_courant_dt(self._h, ctypes.byref(dt)), "mara_block_courant_dt")
+        return dt.value
+
+    def apply_bc(self):
+        _check(self.lib.mara_block_apply_bc(self._h), "mara_block_apply_bc")
+
+    def monopole_max(self):
+        m = ctypes.c_double(0)
+        _check(self.lib.mara_block_monopole_max(self._h, ctypes.byref(m)), "mara_block_monopole_max")
+        return m.value
+
+    def run(self, cfl, t_final, max_steps, t=0.0, check=True):
+        tt = ctypes.c_double(t)
+        steps = ctypes.c_int(0)
+        rep = MaraReport()
+        st = self.lib.mara_block_run(self._h, cfl, t_final, max_steps, ctypes.byref(tt),
+                                     ctypes.byref(steps), ctypes.byref(rep))
+        if check:
+            _check(st, "mara_block_run", rep)
+        return tt.value, steps.value, rep
+
+    def set_timing(self, on=True):
+        _check(self.lib.mara_block_set_timing(self._h, 1 if on else 0), "mara_block_set_timing")
+
+    def stage_kernel_time(self):
+        ms = ctypes.c_double(0)
+        n = ctypes.c_longlong(0)
+        name = ctypes.c_char_p()
+        _check(self.lib.mara_block_stage_kernel_time(self._h, ctypes.byref(ms), ctypes.byref(n),
+                                                     ctypes.byref(name)), "mara_block_stage_kernel_time")
+        return ms.value, n.value, name.value.decode()
+
+
+def desc_device_bytes(desc):
+    return load().mara_desc_device_bytes(ctypes.byref(desc))

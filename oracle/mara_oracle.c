@@ -13,6 +13,7 @@
  */
 #include <math.h>
 #include <float.h>
+#include <limits.h>
 #include <stdlib.h>
 #include <string.h>
 #include "mara_oracle.h"
@@ -928,6 +929,7 @@ void oracle_divergence(const double* F, const int s[3], int nq, double factor,
 {
     const int n0 = s[0] - 1, n1 = s[1] - 1, n2 = s[2] - 1;
 #define FF(i, j, k, q, a) F[((((long)(i) * s[1] + (j)) * s[2] + (k)) * nq + (q)) * 3 + (a)]
+    #pragma omp parallel for collapse(2) schedule(static)
     for (int i = 0; i < n0; ++i)
     for (int j = 0; j < n1; ++j)
     for (int k = 0; k < n2; ++k)
@@ -1137,12 +1139,17 @@ int oracle_advance(const mara_desc* d, double* P, double t0, double dt, mara_rep
     const double xhat[3] = {1.0, 0.0, 0.0}; /* Request() default areaElement */
 
     /* generateConserved over the whole array (src/FieldOperator.cpp:91-116) */
-    for (long c = 0; c < ncell && status == MARA_OK; ++c)
     {
-        ostate S;
-        int e = from_primitive(d->law, d->gamma, P + c * nq, xhat, &S);
-        if (e) { status = e; break; }
-        for (int q = 0; q < nq; ++q) U0[c * nq + q] = S.U[q];
+        int err = MARA_OK;
+        #pragma omp parallel for schedule(static) reduction(max:err)
+        for (long c = 0; c < ncell; ++c)
+        {
+            ostate S;
+            int e = from_primitive(d->law, d->gamma, P + c * nq, xhat, &S);
+            if (e) { if (e > err) err = e; continue; }
+            for (int q = 0; q < nq; ++q) U0[c * nq + q] = S.U[q];
+        }
+        status = err;
     }
     if (status == MARA_OK) memcpy(U, U0, ncell * nq * sizeof(double));
 
@@ -1191,12 +1198,16 @@ int oracle_advance(const mara_desc* d, double* P, double t0, double dt, mara_rep
 
         /* L = divergence(F, -1), U = U0 (1-b) + (U + dt L) b */
         oracle_divergence(F, fs, nq, -1.0, dA, dV, L);
+        #pragma omp parallel for schedule(static)
         for (long n = 0; n < ncell * nq; ++n)
             U[n] = U0[n] * (1 - b[rk]) + (U[n] + dt * L[n]) * b[rk];
 
         /* recoverPrimitive on the interior (src/FieldOperator.cpp:41-82) */
-        long fails = 0, floors = 0, first = -1;
+        long fails = 0, floors = 0;
+        long first = LONG_MAX; /* smallest failing interior index, reference order */
         const int g0 = d->guards[0], g1 = d->guards[1], g2 = d->guards[2];
+        const int m1 = l.N[1] - 2 * g1, m2 = l.N[2] - 2 * g2;
+        #pragma omp parallel for collapse(2) schedule(static) reduction(+:fails, floors) reduction(min:first)
         for (int i = g0; i < l.N[0] - g0; ++i)
         for (int j = g1; j < l.N[1] - g1; ++j)
         for (int k = g2; k < l.N[2] - g2; ++k)
@@ -1207,18 +1218,19 @@ int oracle_advance(const mara_desc* d, double* P, double t0, double dt, mara_rep
             int e = from_conserved(d->law, d->gamma, d->pressure_floor, U + c, Pq, &health);
             if (e)
             {
-                if (first < 0 && rep)
-                {
-                    rep->first_failure[0] = i - g0;
-                    rep->first_failure[1] = j - g1;
-                    rep->first_failure[2] = k - g2;
-                }
-                if (first < 0) first = c;
+                const long lin = ((long)(i - g0) * m1 + (j - g1)) * m2 + (k - g2);
+                if (lin < first) first = lin;
                 ++fails;
                 continue;
             }
             floors += health;
             for (int q = 0; q < nq; ++q) P[c + q] = Pq[q];
+        }
+        if (fails && rep)
+        {
+            rep->first_failure[2] = (int)(first % m2);
+            rep->first_failure[1] = (int)((first / m2) % m1);
+            rep->first_failure[0] = (int)(first / ((long)m1 * m2));
         }
         if (rep) { rep->c2p_failures += fails; rep->floor_applied += floors; }
         if (fails) { status = MARA_ERR_INVERSION_FAILURE; break; }
@@ -1249,6 +1261,8 @@ int oracle_courant_dt(const mara_desc* d, const double* P, double* dt_out)
     double courant = DBL_MAX;
     int status = MARA_OK;
     const int g0 = d->guards[0], g1 = d->guards[1], g2 = d->guards[2];
+    /* min is exact and order independent, so the reduction is deterministic */
+    #pragma omp parallel for collapse(2) schedule(static) reduction(min:courant) reduction(max:status)
     for (int i = g0; i < l.N[0] - g0; ++i)
     for (int j = g1; j < l.N[1] - g1; ++j)
     for (int k = g2; k < l.N[2] - g2; ++k)
@@ -1259,7 +1273,7 @@ int oracle_courant_dt(const mara_desc* d, const double* P, double* dt_out)
         {
             ostate S;
             int e = from_primitive(d->law, d->gamma, pc, nhat + 3 * a, &S);
-            if (e) { status = e; lam[a] = 0; continue; }
+            if (e) { if (e > status) status = e; lam[a] = 0; continue; }
             double m = 0.0;
             for (int n = 0; n < l.nq; ++n)
                 if (m < fabs(S.A[n])) m = fabs(S.A[n]);

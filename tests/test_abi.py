@@ -1,0 +1,96 @@
+"""The C-ABI library without a GPU: it loads, exports every function that
+include/mara_gpu.h declares, its structs match the ctypes mirror, and
+descriptor validation rejects what the reference rejects."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from mara2_amd import abi, runtime
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "mara_gpu.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(mara_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_function():
+    lib = runtime.load()
+    names = declared_functions()
+    assert len(names) >= 17
+    for name in names:
+        assert hasattr(lib, name), name
+    out = subprocess.run(["nm", "-D", "--defined-only", runtime.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r" T (mara_\w+)", out))
+    assert set(names) <= exported
+
+
+def test_struct_layout_matches_header(tmp_path):
+    src = tmp_path / "layout.c"
+    src.write_text('''#include <stdio.h>
+#include <stddef.h>
+#include "mara_gpu.h"
+#define F(T, m) printf(#T "." #m " %zu\\n", offsetof(T, m));
+int main(void) {
+  printf("mara_desc %zu\\n", sizeof(mara_desc));
+  printf("mara_report %zu\\n", sizeof(mara_report));
+  F(mara_desc, law) F(mara_desc, gamma) F(mara_desc, pressure_floor) F(mara_desc, recon)
+  F(mara_desc, plm_theta) F(mara_desc, riemann) F(mara_desc, bc) F(mara_desc, cells)
+  F(mara_desc, guards) F(mara_desc, lower) F(mara_desc, upper) F(mara_desc, rk_order)
+  F(mara_desc, field_ct) F(mara_desc, precision)
+  F(mara_report, status) F(mara_report, c2p_failures) F(mara_report, first_failure)
+  F(mara_report, floor_applied) F(mara_report, eigen_failures) F(mara_report, superluminal)
+  return 0; }
+''')
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    lines = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = dict(l.rsplit(" ", 1) for l in lines if l)
+    assert int(got["mara_desc"]) == ctypes.sizeof(abi.MaraDesc)
+    assert int(got["mara_report"]) == ctypes.sizeof(abi.MaraReport)
+    for T, name in ((abi.MaraDesc, "mara_desc"), (abi.MaraReport, "mara_report")):
+        for field, _ in T._fields_:
+            assert int(got[f"{name}.{field}"]) == getattr(T, field).offset, field
+
+
+def test_error_strings_and_sizes():
+    lib = runtime.load()
+    assert lib.mara_error_string(abi.ERR_GUARD_TOO_SMALL).decode() == \
+        "Boundary region of mesh data is smaller than the scheme's stencil"
+    assert lib.mara_num_fields(abi.LAW_HYDRO) == 5 and lib.mara_num_fields(abi.LAW_RMHD) == 8
+    assert [lib.mara_stencil_size(r) for r in (0, 1, 2)] == [1, 2, 3]
+
+
+@pytest.mark.parametrize("field,value,status", [
+    ("rk_order", 4, abi.ERR_INVALID_ARGUMENT),      # MethodOfLinesTVD::setRungeKuttaOrder throws
+    ("rk_order", 0, abi.ERR_INVALID_ARGUMENT),
+    ("law", 7, abi.ERR_INVALID_ARGUMENT),
+    ("recon", 5, abi.ERR_INVALID_ARGUMENT),
+])
+def test_create_rejects_bad_descriptors(field, value, status):
+    lib = runtime.load()
+    d = abi.make_desc()
+    setattr(d, field, value)
+    h = ctypes.c_void_p()
+    assert lib.mara_block_create(ctypes.byref(d), ctypes.byref(h)) == status
+
+
+def test_create_rejects_small_guards_and_hllc_mhd():
+    lib = runtime.load()
+    h = ctypes.c_void_p()
+    d = abi.make_desc(recon="weno5", cells=(32, 32, 1), guards=(2, 2, 0))
+    assert lib.mara_block_create(ctypes.byref(d), ctypes.byref(h)) == abi.ERR_GUARD_TOO_SMALL
+    d = abi.make_desc(law="mhd", riemann="hllc")
+    assert lib.mara_block_create(ctypes.byref(d), ctypes.byref(h)) == abi.ERR_UNSUPPORTED
+
+
+def test_device_bytes_estimate():
+    d = abi.make_desc(law="hydro", cells=(256, 256, 256), riemann="hllc")
+    b = runtime.desc_device_bytes(d)
+    assert 3 * 5 * 260 ** 3 * 8 <= b < 4 * 5 * 260 ** 3 * 8

@@ -1,0 +1,77 @@
+"""The oracle against the reference's own compiled code: bit-for-bit on the
+committed golden vectors (tests/golden/ref_vectors.npz, written by
+tests/golden/make_golden.py from oracle/_ref), and, where the reference
+bridge is built, on fresh random inputs."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from mara2_amd import abi
+
+GOLD = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_vectors.npz"))
+dp = ctypes.POINTER(ctypes.c_double)
+
+
+def test_unit_vectors_golden():
+    assert np.array_equal(oracle.unit_vectors(), GOLD["unit_vectors"])
+    # yhat carries cos(pi/2) in its x component (src/UnitVector.cpp:10,32-39)
+    assert GOLD["unit_vectors"][1, 0] == np.cos(np.pi / 2) != 0.0
+
+
+def test_plm_golden():
+    v, th, sg, ref = GOLD["plm_v"], GOLD["plm_theta"], GOLD["plm_sgn"], GOLD["plm_out"]
+    got = np.array([oracle.plm(v[i], th[i], sg[i]) for i in range(len(ref))])
+    assert np.array_equal(got, ref)
+
+
+def test_weno5_golden():
+    v, ref = GOLD["weno_v"], GOLD["weno_out"]
+    for i in range(0, len(v), 3):
+        for op in range(6):
+            for mode in range(3):
+                got = oracle.weno5(v[i], op, mode, 50.0)
+                r = ref[i, op, mode]
+                assert got == r or (np.isnan(got) and np.isnan(r)), (i, op, mode, got, r)
+
+
+def test_quartic_golden():
+    d, nr, roots = GOLD["quartic_d"], GOLD["quartic_nr"], GOLD["quartic_roots"]
+    for i in range(len(d)):
+        n, r = oracle.quartic(d[i])
+        assert n == nr[i]
+        assert np.array_equal(r, roots[i, :n]), i
+
+
+def test_rmhd_c2p_golden():
+    U, gam, err, which, P = GOLD["c2p_U"], GOLD["c2p_gamma"], GOLD["c2p_err"], GOLD["c2p_which"], GOLD["c2p_P"]
+    for i in range(len(U)):
+        e, Po, w = oracle.rmhd_c2p(U[i], gam[i])
+        assert e == err[i] and w == which[i], i
+        if e == 0:
+            assert np.array_equal(Po, P[i]), i
+    assert np.mean(err == 0) > 0.95
+
+
+@pytest.mark.skipif(oracle.ref() is None, reason="reference bridge oracle/_ref not built (no reference tree)")
+def test_fresh_random_against_reference_bridge():
+    R = oracle.ref()
+    rng = np.random.default_rng()
+    for _ in range(2000):
+        v = rng.standard_normal(5) * np.exp(rng.uniform(-8, 8))
+        th = rng.uniform(1, 2)
+        for sg in (-1.0, 1.0):
+            a = np.ascontiguousarray(v[1:4])
+            assert oracle.plm(v[1:4], th, sg) == R.ref_plm(a.ctypes.data_as(dp), th, sg)
+        for op in range(6):
+            for mode in range(3):
+                x = oracle.weno5(v, op, mode, 50.0)
+                y = R.ref_weno5(v.ctypes.data_as(dp), op, mode, 50.0)
+                assert x == y or (np.isnan(x) and np.isnan(y))
+        d = rng.standard_normal(5) * np.exp(rng.uniform(-4, 4, 5))
+        r = np.zeros(4)
+        n = R.ref_quartic(d.ctypes.data_as(dp), r.ctypes.data_as(dp))
+        n2, r2 = oracle.quartic(d)
+        assert n == n2 and np.array_equal(r[:n], r2)
