@@ -24,6 +24,11 @@ long long round_up(long long x, long long m) { return (x + m - 1) / m * m; }
 
 unsigned grid_for(long long n) { return (unsigned)((n + kThreads - 1) / kThreads); }
 
+// 2-D work-groups of 64 x 4 over an (m0, m1, m2) box: one wavefront per x row,
+// z from the grid, so kernels decode indices without integer division.
+dim3 block3() { return dim3(64, 4, 1); }
+dim3 grid3(int m0, int m1, int m2) { return dim3((m0 + 63) / 64, (m1 + 3) / 4, m2); }
+
 bool hip_ok(hipError_t e)
 {
     if (e != hipSuccess)
@@ -164,8 +169,7 @@ void launch_flux_t(mara_block* b, const double* Pin, int axis)
         }
     }
     const int m0 = hi[0] - lo[0], m1 = hi[1] - lo[1], m2 = hi[2] - lo[2];
-    const long long total = (long long)m0 * m1 * m2;
-    k_flux<Law, Recon, Riem, R><<<grid_for(total), kThreads, 0, b->stream>>>(
+    k_flux<Law, Recon, Riem, R><<<grid3(m0, m1, m2), block3(), 0, b->stream>>>(
         Pin, (R*)b->F[axis], G, axis, lo[0], lo[1], lo[2], m0, m1, m2, b->ctl, b->ctl);
 }
 
@@ -214,8 +218,7 @@ void launch_ct(mara_block* b)
             lo[e] = G.g[e];
             m[e] = G.n[e] + (e == a ? 1 : 0);
         }
-        const long long total = (long long)m[0] * m[1] * m[2];
-        k_ct<R><<<grid_for(total), kThreads, 0, b->stream>>>(
+        k_ct<R><<<grid3(m[0], m[1], m[2]), block3(), 0, b->stream>>>(
             (const R*)b->F[0], (const R*)b->F[1], (const R*)b->F[2], b->Gct, G, a,
             lo[0], lo[1], lo[2], m[0], m[1], m[2], b->ctl);
     }
@@ -225,13 +228,13 @@ template <int Law, typename R>
 void launch_update_law(mara_block* b, double* Pout, double bk, int first, int write_u)
 {
     const Geom& G = b->G;
-    const long long total = (long long)G.n[0] * G.n[1] * G.n[2];
+    const dim3 grid = grid3(G.n[0], G.n[1], G.n[2]);
     if (b->S.ct)
-        k_update<Law, R, true><<<grid_for(total), kThreads, 0, b->stream>>>(
+        k_update<Law, R, true><<<grid, block3(), 0, b->stream>>>(
             b->Pa, b->U, Pout, (const R*)b->F[0], (const R*)b->F[1], (const R*)b->F[2], b->Gct,
             G, bk, first, write_u, b->ctl);
     else
-        k_update<Law, R, false><<<grid_for(total), kThreads, 0, b->stream>>>(
+        k_update<Law, R, false><<<grid, block3(), 0, b->stream>>>(
             b->Pa, b->U, Pout, (const R*)b->F[0], (const R*)b->F[1], (const R*)b->F[2], b->Gct,
             G, bk, first, write_u, b->ctl);
 }

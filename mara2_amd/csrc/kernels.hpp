@@ -77,18 +77,16 @@ __device__ __forceinline__ double bitsd(unsigned long long b) { return __longlon
 // ---------------------------------------------------------------------------
 template <int Law, int Recon, int Riem, typename R>
 __global__ void __launch_bounds__(256)
-k_flux(const double* __restrict__ P, R* __restrict__ F, Geom G, int axis,
+k_flux(const double* __restrict__ P, R* __restrict__ F, const Geom G, int axis,
        int lo0, int lo1, int lo2, int m0, int m1, int m2, const Ctl* __restrict__ ctl, Ctl* ctlw)
 {
     constexpr int NQ = LawNQ<Law>::value;
     constexpr int NG = Recon == 0 ? 1 : Recon == 1 ? 2 : 3;
     if (!ctl->active) return;
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long total = (long long)m0 * m1 * m2;
-    if (t >= total) return;
-    const int i = lo0 + (int)(t % m0);
-    const int j = lo1 + (int)((t / m0) % m1);
-    const int k = lo2 + (int)(t / ((long long)m0 * m1));
+    const int ti = blockIdx.x * blockDim.x + threadIdx.x;
+    const int tj = blockIdx.y * blockDim.y + threadIdx.y;
+    if (ti >= m0 || tj >= m1) return;
+    const int i = lo0 + ti, j = lo1 + tj, k = lo2 + (int)blockIdx.z;
     const long long stride = axis == 0 ? 1 : axis == 1 ? (long long)G.N[0] : (long long)G.N[0] * G.N[1];
     const long long c0 = cell_index(G, i, j, k) - NG * stride;  // first stencil cell
 
@@ -156,12 +154,10 @@ k_ct(const R* __restrict__ Fx, const R* __restrict__ Fy, const R* __restrict__ F
      const Ctl* __restrict__ ctl)
 {
     if (!ctl->active) return;
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long total = (long long)m0 * m1 * m2;
-    if (t >= total) return;
-    const int i = lo0 + (int)(t % m0);
-    const int j = lo1 + (int)((t / m0) % m1);
-    const int k = lo2 + (int)(t / ((long long)m0 * m1));
+    const int ti = blockIdx.x * blockDim.x + threadIdx.x;
+    const int tj = blockIdx.y * blockDim.y + threadIdx.y;
+    if (ti >= m0 || tj >= m1) return;
+    const int i = lo0 + ti, j = lo1 + tj, k = lo2 + (int)blockIdx.z;
     const R* Fa[3] = {Fx, Fy, Fz};
     const long long fstride[3] = {1, (long long)(G.N[0] + 1), (long long)(G.N[0] + 1) * (G.N[1] + 1)};
     const long long f = face_index(G, i, j, k);
@@ -201,12 +197,10 @@ k_update(const double* __restrict__ P0, double* __restrict__ U, double* __restri
 {
     constexpr int NQ = LawNQ<Law>::value;
     if (!ctl->active) return;
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long total = (long long)G.n[0] * G.n[1] * G.n[2];
-    if (t >= total) return;
-    const int i = G.g[0] + (int)(t % G.n[0]);
-    const int j = G.g[1] + (int)((t / G.n[0]) % G.n[1]);
-    const int k = G.g[2] + (int)(t / ((long long)G.n[0] * G.n[1]));
+    const int ti = blockIdx.x * blockDim.x + threadIdx.x;
+    const int tj = blockIdx.y * blockDim.y + threadIdx.y;
+    if (ti >= G.n[0] || tj >= G.n[1]) return;
+    const int i = G.g[0] + ti, j = G.g[1] + tj, k = G.g[2] + (int)blockIdx.z;
     const long long c = cell_index(G, i, j, k);
     const long long f = face_index(G, i, j, k);
     const long long fx = f + 1, fy = f + (G.N[0] + 1), fz = f + (long long)(G.N[0] + 1) * (G.N[1] + 1);

@@ -1,0 +1,451 @@
+// mara2.cpp -- the host layer of mara2.hpp.  Only descriptor mapping, host
+// arrays and the ghost fill of initial data live here; every step runs in
+// the HIP library through the C ABI.
+#include "mara2.hpp"
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+namespace mara2 {
+
+// ---------------------------------------------------------------------------
+Array::Array(Shape s) : shape_(s)
+{
+    std::size_t n = 1;
+    for (int a = 0; a < 5; ++a) n *= (std::size_t)(shape_[a] < 1 ? 1 : shape_[a]);
+    for (int a = 0; a < 5; ++a) if (shape_[a] < 1) shape_[a] = 1;
+    data_.assign(n, 0.0);
+}
+
+Array::Array(int n1, int n2, int n3, int n4, int n5) : Array(Shape{{n1, n2, n3, n4, n5}}) {}
+
+double& Array::operator()(int i, int j, int k, int q, int r)
+{
+    return data_[((((std::size_t)i * shape_[1] + j) * shape_[2] + k) * shape_[3] + q) * shape_[4] + r];
+}
+
+double Array::operator()(int i, int j, int k, int q, int r) const
+{
+    return data_[((((std::size_t)i * shape_[1] + j) * shape_[2] + k) * shape_[3] + q) * shape_[4] + r];
+}
+
+// ---------------------------------------------------------------------------
+std::vector<std::string> ConservationLaw::getPrimitiveNames() const
+{
+    std::vector<std::string> names;
+    for (int q = 0; q < getNumConserved(); ++q) names.push_back(getPrimitiveName(q));
+    return names;
+}
+
+int NewtonianHydro::getIndexFor(VariableType type) const
+{
+    switch (type)
+    {
+        case VariableType::density: return 0;
+        case VariableType::velocity: return 1;
+        case VariableType::pressure: return 4;
+        default: return -1;
+    }
+}
+
+std::string NewtonianHydro::getPrimitiveName(int q) const
+{
+    static const char* names[] = {"density", "velocity1", "velocity2", "velocity3", "pressure"};
+    return q >= 0 && q < 5 ? names[q] : "";
+}
+
+int NewtonianMHD::getIndexFor(VariableType type) const
+{
+    switch (type)
+    {
+        case VariableType::density: return 0;
+        case VariableType::velocity: return 1;
+        case VariableType::pressure: return 4;
+        case VariableType::magnetic: return 5;
+        default: return -1;
+    }
+}
+
+std::string NewtonianMHD::getPrimitiveName(int q) const
+{
+    static const char* names[] = {"density", "velocity1", "velocity2", "velocity3", "pressure",
+                                  "magnetic1", "magnetic2", "magnetic3"};
+    return q >= 0 && q < 8 ? names[q] : "";
+}
+
+// ---------------------------------------------------------------------------
+CartesianMeshGeometry::CartesianMeshGeometry() : shape{{128, 1, 1, 1, 1}}, lower{{0, 0, 0}}, upper{{1, 1, 1}}
+{
+    cacheSpacing();
+}
+
+CartesianMeshGeometry::CartesianMeshGeometry(Shape S) : shape{{S[0], S[1], S[2], 1, 1}}, lower{{0, 0, 0}}, upper{{1, 1, 1}}
+{
+    cacheSpacing();
+}
+
+void CartesianMeshGeometry::setCellsShape(Shape S)
+{
+    shape[0] = S[0];
+    shape[1] = S[1];
+    shape[2] = S[2];
+    cacheSpacing();
+}
+
+void CartesianMeshGeometry::setLowerUpper(Coordinate L, Coordinate U)
+{
+    lower = L;
+    upper = U;
+    cacheSpacing();
+}
+
+Coordinate CartesianMeshGeometry::coordinateAtIndex(double i, double j, double k) const
+{
+    return Coordinate{{lower[0] + dx[0] * (i + 0.5), lower[1] + dx[1] * (j + 0.5), lower[2] + dx[2] * (k + 0.5)}};
+}
+
+unsigned long CartesianMeshGeometry::totalCellsInMesh() const
+{
+    return (unsigned long)shape[0] * shape[1] * shape[2];
+}
+
+void CartesianMeshGeometry::cacheSpacing()
+{
+    for (int n = 0; n < 3; ++n) dx[n] = (upper[n] - lower[n]) / shape[n];
+}
+
+// ---------------------------------------------------------------------------
+// Host ghost fill for initial data (src/BoundaryConditions.cpp:8-176); the
+// step's own ghost fills run on the device.
+void BoundaryCondition::applySimple(Array& A, Shape boundaryShape) const
+{
+    for (int axis = 0; axis < 3; ++axis)
+    {
+        if (boundaryShape[axis] > 0)
+        {
+            apply(A, MeshLocation::cell, MeshBoundary::left, axis, boundaryShape[axis]);
+            apply(A, MeshLocation::cell, MeshBoundary::right, axis, boundaryShape[axis]);
+        }
+    }
+}
+
+namespace {
+void copy_layer(Array& A, int axis, int dst, int src, int negate)
+{
+    int n[3] = {A.size(0), A.size(1), A.size(2)};
+    n[axis] = 1;
+    for (int i = 0; i < n[0]; ++i)
+    for (int j = 0; j < n[1]; ++j)
+    for (int k = 0; k < n[2]; ++k)
+    {
+        int d[3] = {i, j, k}, s[3] = {i, j, k};
+        d[axis] = dst;
+        s[axis] = src;
+        for (int q = 0; q < A.size(3); ++q)
+        {
+            double v = A(s[0], s[1], s[2], q);
+            if (q == negate) v *= -1;
+            A(d[0], d[1], d[2], q) = v;
+        }
+    }
+}
+}
+
+void PeriodicBoundaryCondition::apply(Array& A, MeshLocation, MeshBoundary b, int axis, int g) const
+{
+    if (A.size(axis) == 1) throw std::runtime_error("Attempt to apply boundary condition on flattened axis " + std::to_string(axis));
+    const int N = A.size(axis);
+    for (int n = 0; n < g; ++n)
+        if (b == MeshBoundary::left) copy_layer(A, axis, n, N - 2 * g + n, -1);
+        else copy_layer(A, axis, N - g + n, g + n, -1);
+}
+
+void OutflowBoundaryCondition::apply(Array& A, MeshLocation, MeshBoundary b, int axis, int g) const
+{
+    if (A.size(axis) == 1) throw std::runtime_error("Attempt to apply boundary condition on flattened axis " + std::to_string(axis));
+    const int N = A.size(axis);
+    for (int n = 0; n < g; ++n)
+        if (b == MeshBoundary::left) copy_layer(A, axis, n, g, -1);
+        else copy_layer(A, axis, N - 1 - n, N - g - 1, -1);
+}
+
+void ReflectingBoundaryCondition::apply(Array& A, MeshLocation, MeshBoundary b, int axis, int g) const
+{
+    if (A.size(axis) == 1) throw std::logic_error("Attempt to apply boundary condition on flattened axis " + std::to_string(axis));
+    if (!conservationLaw) throw std::logic_error("ReflectingBoundaryCondition needs an instance of ConservationLaw");
+    const int ivel = conservationLaw->getIndexFor(ConservationLaw::VariableType::velocity);
+    const int N = A.size(axis);
+    for (int n = 0; n < g; ++n)
+        if (b == MeshBoundary::left) copy_layer(A, axis, n, 2 * g - n - 1, ivel + axis);
+        else copy_layer(A, axis, N - 1 - n, N - 2 * g + n, ivel + axis);
+}
+
+// ---------------------------------------------------------------------------
+Array MeshOperator::linearCellDimension() const
+{
+    if (!geometry) throw std::logic_error("MeshOperator has no MeshGeometry instance");
+    const Shape s = geometry->cellsShape();
+    Array L(s[0], s[1], s[2]);
+    for (int i = 0; i < s[0]; ++i)
+    for (int j = 0; j < s[1]; ++j)
+    for (int k = 0; k < s[2]; ++k)
+    {
+        double m = geometry->cellLength(i, j, k, 0);
+        if (geometry->cellLength(i, j, k, 1) < m) m = geometry->cellLength(i, j, k, 1);
+        if (geometry->cellLength(i, j, k, 2) < m) m = geometry->cellLength(i, j, k, 2);
+        L(i, j, k) = m;
+    }
+    return L;
+}
+
+Array MeshOperator::generate(InitialDataFunction F, MeshLocation location, Shape bs) const
+{
+    if (!geometry) throw std::logic_error("MeshOperator has no MeshGeometry instance");
+    if (location != MeshLocation::cell) throw std::logic_error("Unsupported mesh location for MeshOperator::generate");
+    const int nq = (int)F(0.0, 0.0, 0.0).size();
+    const Shape s = geometry->cellsShape();
+    Array P(s[0] + 2 * bs[0], s[1] + 2 * bs[1], s[2] + 2 * bs[2], nq);
+    for (int i = 0; i < P.size(0); ++i)
+    for (int j = 0; j < P.size(1); ++j)
+    for (int k = 0; k < P.size(2); ++k)
+    {
+        const Coordinate x = geometry->coordinateAtIndex(i - bs[0], j - bs[1], k - bs[2]);
+        const std::vector<double> v = F(x[0], x[1], x[2]);
+        for (int q = 0; q < nq; ++q) P(i, j, k, q) = v[q];
+    }
+    return P;
+}
+
+// ---------------------------------------------------------------------------
+MeshData::MeshData(Shape cs, Shape bs, int nq) : boundaryShape(bs)
+{
+    P = Array(cs[0] + 2 * bs[0], cs[1] + 2 * bs[1], cs[2] + 2 * bs[2], nq);
+}
+
+void MeshData::assignPrimitive(const Array& Pi)
+{
+    const Shape& b = boundaryShape;
+    for (int i = 0; i < Pi.size(0); ++i)
+    for (int j = 0; j < Pi.size(1); ++j)
+    for (int k = 0; k < Pi.size(2); ++k)
+    for (int q = 0; q < Pi.size(3); ++q)
+        P(i + b[0], j + b[1], k + b[2], q) = Pi(i, j, k, q);
+    deviceVersion = 0;  // host copy replaced
+}
+
+// ---------------------------------------------------------------------------
+namespace {
+std::string status_text(int s) { return mara_error_string(s); }
+
+void throw_for(int status, const char* where, const mara_report* r)
+{
+    if (status == MARA_OK) return;
+    std::string msg = std::string(where) + ": " + status_text(status);
+    if (r && r->c2p_failures)
+        msg += " (" + std::to_string(r->c2p_failures) + " zones, first at zone index [" +
+               std::to_string(r->first_failure[0]) + " " + std::to_string(r->first_failure[1]) + " " +
+               std::to_string(r->first_failure[2]) + "])";
+    switch (status)
+    {
+        case MARA_ERR_GUARD_TOO_SMALL:
+        case MARA_ERR_UNSUPPORTED:
+        case MARA_ERR_INVALID_ARGUMENT: throw std::logic_error(msg);
+        case MARA_ERR_INVERSION_FAILURE: throw ConservationLaw::StateFailure(msg);
+        default: throw std::runtime_error(msg);
+    }
+}
+}
+
+GpuMethodOfLinesTVD::GpuMethodOfLinesTVD()
+{
+    fluxScheme = std::make_shared<MethodOfLines>();
+}
+
+GpuMethodOfLinesTVD::~GpuMethodOfLinesTVD()
+{
+    if (block) mara_block_destroy(block);
+}
+
+void GpuMethodOfLinesTVD::setRungeKuttaOrder(int order)
+{
+    if (0 >= order || order > 3) throw std::logic_error("Runge-Kutta order not 1, 2, or 3");
+    rungeKuttaOrder = order;
+}
+
+int GpuMethodOfLinesTVD::getStencilSize() const
+{
+    if (!fluxScheme) throw std::logic_error("No IntercellFluxScheme instance");
+    return fluxScheme->getStencilSize();
+}
+
+mara_desc GpuMethodOfLinesTVD::describe(const MeshData& md) const
+{
+    if (!fieldOperator) throw std::logic_error("No FieldOperator instance");
+    if (!meshOperator) throw std::logic_error("No MeshOperator instance");
+    if (!boundaryCondition) throw std::logic_error("No BoundaryCondition instance");
+    if (!fluxScheme) throw std::logic_error("No IntercellFluxScheme instance");
+    auto law = fieldOperator->getConservationLaw();
+    if (!law) throw std::logic_error("FieldOperator has no ConservationLaw");
+    auto geom = std::dynamic_pointer_cast<CartesianMeshGeometry>(meshOperator->getMeshGeometry());
+    if (!geom) throw std::logic_error("GPU step needs a CartesianMeshGeometry");
+    mara_desc d;
+    std::memset(&d, 0, sizeof d);
+    d.law = law->lawCode();
+    d.gamma = law->getGammaLawIndex();
+    d.pressure_floor = law->getPressureFloor();
+    d.recon = fluxScheme->reconCode();
+    d.plm_theta = fluxScheme->getPlmTheta();
+    d.riemann = fluxScheme->getRiemannSolver() ? fluxScheme->getRiemannSolver()->solverCode() : MARA_RIEMANN_HLLE;
+    d.bc = boundaryCondition->bcCode();
+    const Shape cs = geom->cellsShape();
+    const Coordinate lo = geom->lowerCorner(), hi = geom->upperCorner();
+    for (int a = 0; a < 3; ++a)
+    {
+        d.cells[a] = cs[a];
+        d.guards[a] = md.boundaryShape[a];
+        d.lower[a] = lo[a];
+        d.upper[a] = hi[a];
+        if (md.P.size(a) != cs[a] + 2 * md.boundaryShape[a])
+            throw std::logic_error("MeshData shape does not match the mesh geometry");
+    }
+    if (md.P.size(3) != law->getNumConserved())
+        throw std::logic_error("MeshData field count does not match the conservation law");
+    d.rk_order = rungeKuttaOrder;
+    d.field_ct = disableFieldCT ? 0 : 1;
+    d.precision = precision;
+    return d;
+}
+
+void GpuMethodOfLinesTVD::check(int status, const char* where) const
+{
+    throw_for(status, where, &report);
+}
+
+mara_block* GpuMethodOfLinesTVD::blockFor(const MeshData& md) const
+{
+    const mara_desc d = describe(md);
+    if (block && (blockOwner != &md || std::memcmp(&d, &blockDesc, sizeof d) != 0))
+    {
+        mara_block_destroy(block);
+        block = nullptr;
+    }
+    if (!block)
+    {
+        check(mara_block_create(&d, &block), "mara_block_create");
+        blockDesc = d;
+        blockOwner = &md;
+    }
+    return block;
+}
+
+void GpuMethodOfLinesTVD::syncToDevice(MeshData& md) const
+{
+    mara_block* b = blockFor(md);
+    check(mara_block_upload_prim(b, md.P.data()), "mara_block_upload_prim");
+    md.deviceVersion = 1;
+}
+
+void GpuMethodOfLinesTVD::syncToHost(MeshData& md) const
+{
+    mara_block* b = blockFor(md);
+    check(mara_block_download_prim(b, md.P.data()), "mara_block_download_prim");
+}
+
+void GpuMethodOfLinesTVD::advance(MeshData& md, double t0, double dt) const
+{
+    mara_block* b = blockFor(md);
+    if (!resident || md.deviceVersion == 0)
+        syncToDevice(md);
+    std::memset(&report, 0, sizeof report);
+    const int st = mara_block_advance(b, t0, dt, &report);
+    md.deviceVersion += 1;
+    if (!resident) check(mara_block_download_prim(b, md.P.data()), "mara_block_download_prim");
+    check(st, "MethodOfLinesTVD::advance");
+}
+
+int GpuMethodOfLinesTVD::run(MeshData& md, double cfl, double tFinal, int maxSteps, double& t) const
+{
+    mara_block* b = blockFor(md);
+    if (!resident || md.deviceVersion == 0) syncToDevice(md);
+    int steps = 0;
+    std::memset(&report, 0, sizeof report);
+    const int st = mara_block_run(b, cfl, tFinal, maxSteps, &t, &steps, &report);
+    md.deviceVersion += 1;
+    if (!resident) check(mara_block_download_prim(b, md.P.data()), "mara_block_download_prim");
+    check(st, "MethodOfLinesTVD::run");
+    return steps;
+}
+
+// ---------------------------------------------------------------------------
+FieldOperator::~FieldOperator()
+{
+    if (block) mara_block_destroy(block);
+}
+
+double FieldOperator::getCourantTimestep(const MeshData& md, const Array& L) const
+{
+    if (!law) throw std::logic_error("FieldOperator has no ConservationLaw");
+    if (!geometry) throw std::logic_error("FieldOperator needs the CartesianMeshGeometry (setMeshGeometry)");
+    const Shape cs = geometry->cellsShape();
+    if (L.size(0) != cs[0] || L.size(1) != cs[1] || L.size(2) != cs[2])
+        throw std::logic_error("Primitive and linear cell dimension arrays have different sizes");
+    // the device reduction uses one length for the uniform Cartesian mesh
+    for (std::size_t n = 1; n < L.size(); ++n)
+        if (L.data()[n] != L.data()[0]) throw std::logic_error("non-uniform linear cell dimension is not supported");
+    mara_desc d;
+    std::memset(&d, 0, sizeof d);
+    d.law = law->lawCode();
+    d.gamma = law->getGammaLawIndex();
+    d.pressure_floor = law->getPressureFloor();
+    d.recon = MARA_RECON_PCM;
+    d.plm_theta = 1.5;
+    d.bc = MARA_BC_OUTFLOW;
+    d.rk_order = 1;
+    const Coordinate lo = geometry->lowerCorner(), hi = geometry->upperCorner();
+    for (int a = 0; a < 3; ++a)
+    {
+        d.cells[a] = cs[a];
+        d.guards[a] = md.boundaryShape[a];
+        d.lower[a] = lo[a];
+        d.upper[a] = hi[a];
+    }
+    if (block && std::memcmp(&d, &blockDesc, sizeof d) != 0)
+    {
+        mara_block_destroy(block);
+        block = nullptr;
+    }
+    if (!block)
+    {
+        throw_for(mara_block_create(&d, &block), "FieldOperator::getCourantTimestep", nullptr);
+        blockDesc = d;
+    }
+    double dt = 0.0;
+    int st = mara_block_upload_prim(block, md.P.data());
+    if (st == MARA_OK) st = mara_block_courant_dt(block, &dt);
+    throw_for(st, "FieldOperator::getCourantTimestep", nullptr);
+    return dt;
+}
+
+// ---------------------------------------------------------------------------
+int maraMainLoop(SimulationStatus& status, std::function<double()> timestep, std::function<bool()> condition,
+                 std::function<void(double)> advance, double* meanKzps)
+{
+    double sessionKzps = 0.0;
+    int sessionIter = 0;
+    while (condition())
+    {
+        const auto t0 = std::chrono::steady_clock::now();
+        const double dt = timestep();
+        advance(dt);
+        status.simulationTime += dt;
+        status.simulationIter += 1;
+        const double age = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        sessionKzps += 1e-3 * status.totalCellsInMesh / age;
+        sessionIter += 1;
+    }
+    if (meanKzps) *meanKzps = sessionIter ? sessionKzps / sessionIter : 0.0;
+    return sessionIter;
+}
+
+} // namespace mara2
