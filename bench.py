@@ -80,7 +80,8 @@ def cpu_baseline(n_sample, target_s=15.0):
     """The oracle (CPU restatement of the reference step) on the same workload
     at n_sample^3, OpenMP threads = host cores."""
     import oracle
-    threads = min(os.cpu_count() or 1, 64)
+    # the host's CPU share: OMP_NUM_THREADS where the environment sets it
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
     oracle.set_threads(threads)
     desc = problems.turbulence_desc(n_sample)
     P = problems.turbulence_3d(desc)
@@ -93,7 +94,7 @@ def cpu_baseline(n_sample, target_s=15.0):
         err, _ = oracle.advance(desc, P, 0.0, CFL * dtc)
         steps += 1
         el = time.perf_counter() - t0
-        if el > target_s or steps >= 50:
+        if el > target_s or steps >= 2000:
             break
     return {"value": zones * steps / el, "unit": "zone-updates/s", "cores": threads,
             "kind": "port",
@@ -101,21 +102,98 @@ def cpu_baseline(n_sample, target_s=15.0):
                       f"step, {el:.1f} s, CPU restatement of the reference step (oracle/), OpenMP over zones"}
 
 
+def other_workload(args):
+    """Secondary configurations (not the headline line): 2-D MHD with flux-CT,
+    3-D relativistic MHD with WENO5, and the 1-D Sod tube to t = 0.2."""
+    from mara2_amd.runtime import MaraError
+    w = args.workload
+    if w == "sod":
+        d = problems.sod_desc(1024)
+        P0 = problems.sod_1d(d)
+        with Block(d) as blk:
+            blk.upload(P0)
+            t0 = time.perf_counter()
+            t, steps, rep = blk.run(0.4, 0.2, 1 << 30)
+            el = time.perf_counter() - t0
+        out = {"metric": METRIC, "value": 1024 * steps / el, "unit": "zone-updates/s", "n_gpus": 1,
+               "steps": steps, "warmup": 0, "ms_per_step": el / steps * 1e3, "higher_is_better": True,
+               "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic: Sod tube",
+               "config": {"workload": "1-D Sod tube 1024 zones, gamma 1.4, PLM + HLLE, RK2, CFL 0.4, to t = 0.2",
+                          "t_reached": t, "time_to_solution_s": el}}
+        print(json.dumps(out), flush=True)
+        return
+    if w == "mhd2d":
+        n = args.n or 2048
+        make_desc, make, cfl, label = problems.mhd_desc, problems.mhd_2d, 0.4, \
+            f"2-D ideal MHD {n}^2, PLM + HLLE, flux-CT, RK2, CFL 0.4, periodic"
+        gen = lambda d: make(d)
+    else:
+        n = args.n or 128
+        make_desc, make, cfl, label = problems.rmhd_desc, problems.rmhd_3d, 0.3, \
+            f"3-D relativistic MHD {n}^3, WENO5 + HLLE, flux-CT, RK3, CFL 0.3, periodic, max W {args.wmax}"
+        gen = lambda d: make(d, wmax=args.wmax)
+    d64 = make_desc(n, "fp64")
+    log(f"generating {label}")
+    P0 = gen(d64)
+    res = {}
+    for form in ("fp64", "mixed"):
+        d = make_desc(n, form)
+        with Block(d) as blk:
+            blk.upload(P0)
+            t0 = time.perf_counter()
+            try:
+                t, steps, rep = blk.run(cfl, 1e300, args.steps, check=False)
+            except MaraError as e:
+                t, steps, rep = 0.0, 0, None
+            el = time.perf_counter() - t0
+            P = blk.download()
+            mono = blk.monopole_max()
+            blk.set_timing(True)
+            blk.run(cfl, 1e300, 2, check=False)
+            ms, nl, name = blk.stage_kernel_time()
+        res[form] = dict(el=el, steps=steps, P=P, rep=rep.as_dict() if rep else None, mono=mono, ms=ms, name=name)
+        log(f"{form}: {steps} steps in {el:.3f} s, report {res[form]['rep']}, max |monopole| {mono:.3e}")
+    zones = int(np.prod([d64.cells[a] for a in range(3)]))
+    g = d64.guards
+    inner = tuple(slice(g[a], g[a] + d64.cells[a]) for a in range(3))
+    a, b = res["mixed"]["P"][inner], res["fp64"]["P"][inner]
+    l1 = max(float(np.sum(np.abs(a[..., q] - b[..., q])) / max(np.sum(np.abs(b[..., q])), 1e-300)) for q in range(8))
+    r64 = res["fp64"]
+    b_alg = alg_bytes_per_zone_update(d64)
+    per_launch = b_alg * zones / d64.rk_order
+    out = {"metric": METRIC, "value": zones * r64["steps"] / r64["el"], "unit": "zone-updates/s", "n_gpus": 1,
+           "steps": r64["steps"], "warmup": 0, "ms_per_step": r64["el"] / max(r64["steps"], 1) * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic: seeded Gaussian random fields", "config": {"workload": label, "zones": zones},
+           "roofline": {"bound": "hbm", "achieved": per_launch / (r64["ms"] * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9,
+                        "unit": "GB/s", "frac": per_launch / (r64["ms"] * 1e-3) / HBM_PEAK, "traffic": None,
+                        "kernel": r64["name"], "avg_launch_ms": r64["ms"]},
+           "fp64": {"max_abs_monopole": r64["mono"], "report": r64["rep"]},
+           "mixed": {"value": zones * res["mixed"]["steps"] / res["mixed"]["el"],
+                     "speedup_vs_fp64": r64["el"] / res["mixed"]["el"] if res["mixed"]["steps"] == r64["steps"] else None,
+                     "rel_l1_vs_fp64": l1, "max_abs_monopole": res["mixed"]["mono"], "report": res["mixed"]["rep"]}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--n", type=int, default=0)
     ap.add_argument("--cpu-sample", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default="turbulence", choices=["turbulence", "mhd2d", "rmhd3d", "sod"])
+    ap.add_argument("--wmax", type=float, default=5.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               f"(independent solves shard by block, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "turbulence":
+        return other_workload(args)
 
-    n = args.n
+    n = args.n or 256
     d64 = problems.turbulence_desc(n, "fp64")
     dmx = problems.turbulence_desc(n, "mixed")
     log(f"generating {n}^3 initial data")

@@ -170,7 +170,7 @@ void launch_flux_t(mara_block* b, const double* Pin, int axis)
     }
     const int m0 = hi[0] - lo[0], m1 = hi[1] - lo[1], m2 = hi[2] - lo[2];
     k_flux<Law, Recon, Riem, R><<<grid3(m0, m1, m2), block3(), 0, b->stream>>>(
-        Pin, (R*)b->F[axis], G, axis, lo[0], lo[1], lo[2], m0, m1, m2, b->ctl, b->ctl);
+        Pin, (R*)b->F[axis], G, axis, lo[0], lo[1], lo[2], m0, m1, m2, b->ctl);
 }
 
 template <int Law, typename R>
@@ -225,28 +225,28 @@ void launch_ct(mara_block* b)
 }
 
 template <int Law, typename R>
-void launch_update_law(mara_block* b, double* Pout, double bk, int first, int write_u)
+void launch_update_law(mara_block* b, double* Pout, double bk, int first, int write_u, int cout)
 {
     const Geom& G = b->G;
     const dim3 grid = grid3(G.n[0], G.n[1], G.n[2]);
     if (b->S.ct)
         k_update<Law, R, true><<<grid, block3(), 0, b->stream>>>(
             b->Pa, b->U, Pout, (const R*)b->F[0], (const R*)b->F[1], (const R*)b->F[2], b->Gct,
-            G, bk, first, write_u, b->ctl);
+            G, bk, first, write_u, cout, b->ctl);
     else
         k_update<Law, R, false><<<grid, block3(), 0, b->stream>>>(
             b->Pa, b->U, Pout, (const R*)b->F[0], (const R*)b->F[1], (const R*)b->F[2], b->Gct,
-            G, bk, first, write_u, b->ctl);
+            G, bk, first, write_u, cout, b->ctl);
 }
 
 template <typename R>
-void launch_update(mara_block* b, double* Pout, double bk, int first, int write_u)
+void launch_update(mara_block* b, double* Pout, double bk, int first, int write_u, int cout)
 {
     switch (b->d.law)
     {
-        case 0: launch_update_law<0, R>(b, Pout, bk, first, write_u); break;
-        case 1: launch_update_law<1, R>(b, Pout, bk, first, write_u); break;
-        default: launch_update_law<2, R>(b, Pout, bk, first, write_u); break;
+        case 0: launch_update_law<0, R>(b, Pout, bk, first, write_u, cout); break;
+        case 1: launch_update_law<1, R>(b, Pout, bk, first, write_u, cout); break;
+        default: launch_update_law<2, R>(b, Pout, bk, first, write_u, cout); break;
     }
 }
 
@@ -278,8 +278,9 @@ void launch_courant(mara_block* b)
     }
 }
 
-// One RK stage: fluxes, CT, update, ghost fill.
-void launch_stage(mara_block* b, int rk)
+// One RK stage: fluxes, CT, update, ghost fill.  With courant_out the last
+// stage also reduces the next step's CFL time step into the control bins.
+void launch_stage(mara_block* b, int rk, int courant_out)
 {
     static const double b1[1] = {1.0};
     static const double b2[2] = {1.0, 0.5};
@@ -302,7 +303,7 @@ void launch_stage(mara_block* b, int rk)
     if (b->timing) hipEventRecord(b->ev0, b->stream);
     if (b->fused)
     {
-        launch_fused(b->d, b->G, b->stream, Pin, b->Pa, b->U, Pout, bb[rk], rk == 0, !last, b->ctl);
+        launch_fused(b->d, b->G, b->stream, Pin, b->Pa, b->U, Pout, bb[rk], rk == 0, !last, last && courant_out, b->ctl);
     }
     else
     {
@@ -317,8 +318,8 @@ void launch_stage(mara_block* b, int rk)
             if (mixed) launch_ct<float>(b);
             else launch_ct<double>(b);
         }
-        if (mixed) launch_update<float>(b, Pout, bb[rk], rk == 0, !last);
-        else launch_update<double>(b, Pout, bb[rk], rk == 0, !last);
+        if (mixed) launch_update<float>(b, Pout, bb[rk], rk == 0, !last, last && courant_out);
+        else launch_update<double>(b, Pout, bb[rk], rk == 0, !last, last && courant_out);
     }
     if (b->timing)
     {
@@ -543,7 +544,7 @@ int mara_block_advance(mara_block* b, double t0, double dt, mara_report* report)
     (void)t0;
     if (!b) return MARA_ERR_INVALID_ARGUMENT;
     k_ctl_begin_advance<<<1, 1, 0, b->stream>>>(b->ctl, dt);
-    for (int rk = 0; rk < b->d.rk_order; ++rk) launch_stage(b, rk);
+    for (int rk = 0; rk < b->d.rk_order; ++rk) launch_stage(b, rk, 0);
     HIP_TRY(hipGetLastError());
     int e = read_ctl(b);
     if (e) return e;
@@ -555,6 +556,7 @@ int mara_block_courant_dt(mara_block* b, double* dt)
     if (!b || !dt) return MARA_ERR_INVALID_ARGUMENT;
     k_ctl_begin_courant<<<1, 1, 0, b->stream>>>(b->ctl);
     launch_courant(b);
+    k_ctl_fold_bins<<<1, 1, 0, b->stream>>>(b->ctl);
     HIP_TRY(hipGetLastError());
     int e = read_ctl(b);
     if (e) return e;
@@ -587,8 +589,8 @@ int mara_block_run(mara_block* b, double cfl, double t_final, int max_steps,
 {
     if (!b || !t || !steps || max_steps < 0) return MARA_ERR_INVALID_ARGUMENT;
     k_ctl_begin_run<<<1, 1, 0, b->stream>>>(b->ctl, *t, t_final, cfl, max_steps);
-    const long long total = (long long)b->G.n[0] * b->G.n[1] * b->G.n[2];
-    (void)total;
+    // the first step's CFL; later steps take it from the previous step's last stage
+    launch_courant(b);
     int queued = 0;
     while (queued < max_steps)
     {
@@ -597,9 +599,8 @@ int mara_block_run(mara_block* b, double cfl, double t_final, int max_steps,
         for (int s = 0; s < chunk; ++s)
         {
             k_ctl_step_begin<<<1, 1, 0, b->stream>>>(b->ctl);
-            launch_courant(b);
             k_ctl_step_dt<<<1, 1, 0, b->stream>>>(b->ctl);
-            for (int rk = 0; rk < b->d.rk_order; ++rk) launch_stage(b, rk);
+            for (int rk = 0; rk < b->d.rk_order; ++rk) launch_stage(b, rk, 1);
             k_ctl_step_end<<<1, 1, 0, b->stream>>>(b->ctl);
         }
         queued += chunk;

@@ -66,7 +66,7 @@ __device__ __forceinline__ void hydro_face_flux(const R (&S)[5][2 * (Recon == 0 
 template <int Recon, int Riem, typename R, bool Z>
 __global__ void __launch_bounds__(FUSED_TX * FUSED_TY)
 k_fused_hydro(const double* __restrict__ Pin, const double* __restrict__ P0, double* __restrict__ U,
-              double* __restrict__ Pout, Geom G, double b, int first_stage, int write_u, Ctl* ctl)
+              double* __restrict__ Pout, Geom G, double b, int first_stage, int write_u, int courant_out, Ctl* ctl)
 {
     constexpr int NG = Recon == 0 ? 1 : Recon == 1 ? 2 : 3;
     constexpr int TX = FUSED_TX, TY = FUSED_TY;
@@ -74,8 +74,10 @@ k_fused_hydro(const double* __restrict__ Pin, const double* __restrict__ P0, dou
     __shared__ R plane[5][HY][HX];
     __shared__ R fy[5][TY + 1][TX];
     __shared__ R fxe[5][TY];
+    __shared__ double wmin[TX * TY / 64];
 
     if (!ctl->active) return;
+    double dmin = 1.79769313486231570815e+308;  // this zone column's CFL minimum (last stage)
     const int lane = threadIdx.x;
     const int row = threadIdx.y;
     const int tid = row * TX + lane;
@@ -239,6 +241,12 @@ k_fused_hydro(const double* __restrict__ Pin, const double* __restrict__ P0, dou
             cons_to_prim<0>(G.gamma, G.pfloor, Un, Pn, floor);
 #pragma unroll
             for (int q = 0; q < 5; ++q) Pout[q * G.fs + c] = Pn[q];
+            if (courant_out)
+            {
+                int err = 0;
+                const double d = courant_of<0>(Pn, G, err);
+                if (d < dmin) dmin = d;
+            }
             if (write_u)
             {
 #pragma unroll
@@ -252,6 +260,300 @@ k_fused_hydro(const double* __restrict__ Pin, const double* __restrict__ P0, dou
         }
         __syncthreads();
     }
+    if (courant_out) courant_to_bins(dmin, wmin, TX * TY / 64, tid, ctl);
+}
+
+// ---------------------------------------------------------------------------
+// PLM variant: every zone's half-slope h = 0.5 * minmod (one per axis and
+// field) is computed once and shared by its two faces, where the reference
+// evaluates it once per face (src/IntercellFluxSchemes.cpp:102-107).  The
+// face states are v + h (PLM_C2R) and v + (-h) (PLM_C2L); since
+// (+1 * 0.5) * m and (-1 * 0.5) * m are exact negations of each other, the
+// states -- and so the fluxes -- are the reference's bit for bit.
+//   x: h of the zone in a register, the left neighbour's by a lane shuffle;
+//   y: h of every row (and the rows above and below the tile) in LDS;
+//   z: h of the zone above computed from the register window and carried.
+// ---------------------------------------------------------------------------
+template <typename R>
+__device__ __forceinline__ R half_slope(R theta, R vl, R v0, R vr)
+{
+    return R(+1.0) * R(0.5) * minmod(theta, vl, v0, vr);
+}
+
+template <int Riem, typename R>
+__device__ __forceinline__ void hydro_states_flux(const R* PL, const R* PR, R gamma, const R* n, R* F)
+{
+    State<R, 5> SL, SR;
+    hydro_state(gamma, PL, n, SL);
+    hydro_state(gamma, PR, n, SR);
+    if constexpr (Riem == 1) hllc(SL, SR, n, F);
+    else hlle<R, 5>(SL, SR, F);
+}
+
+// Tile height: binary64 state needs twice the LDS and more VGPRs per lane, so
+// its tiles are 64 x 4 (three work-groups per CU), binary32 ones 64 x 8.
+template <typename R> struct PlmTile { static constexpr int TY = sizeof(R) == 8 ? 4 : 8; };
+
+template <int Riem, typename R, bool Z>
+__global__ void __launch_bounds__(FUSED_TX * PlmTile<R>::TY) __attribute__((amdgpu_waves_per_eu(3, 8)))
+k_fused_plm(const double* __restrict__ Pin, const double* __restrict__ P0, double* __restrict__ U,
+            double* __restrict__ Pout, Geom G, double b, int first_stage, int write_u, int courant_out, Ctl* ctl)
+{
+    constexpr int NG = 2;
+    constexpr int TX = FUSED_TX, TY = PlmTile<R>::TY;
+    constexpr int HX = TX + 2 * NG, HY = TY + 2 * NG;
+    constexpr int NT = TX * TY;
+    __shared__ R plane[5][HY][HX];   // plane k, halo 2; zone (i, j) at [j - j0 + 2][i - i0 + 2]
+    __shared__ R sy[5][TY + 2][TX];  // y half-slopes of rows -1 .. TY at [r + 1]
+    __shared__ R sxe[5][TY][3];      // x half-slopes of zones i0 - 1, i0 + 63, i0 + 64
+    __shared__ R fy[5][TY + 1][TX];  // y-face fluxes, face j0 + r at [r]
+    __shared__ R fxe[5][TY];         // x-face i0 + 64 of each row
+    __shared__ double wmin[TX * TY / 64];
+
+    if (!ctl->active) return;
+    double dmin = 1.79769313486231570815e+308;  // this zone column's CFL minimum (last stage)
+    const int lane = threadIdx.x;
+    const int row = threadIdx.y;
+    const int tid = row * TX + lane;
+    const int i0 = G.g[0] + blockIdx.x * TX;
+    const int j0 = G.g[1] + blockIdx.y * TY;
+    const int i = i0 + lane, j = j0 + row;
+    const int ie = G.N[0] - G.g[0], je = G.N[1] - G.g[1];
+    const bool cell_xy = i < ie && j < je;
+    const bool xface = i <= ie && j < je;
+    const bool yface = j <= je && i < ie;
+    const int kb = Z ? G.g[2] + blockIdx.z * FUSED_KZ : 0;
+    const int ke = Z ? min(kb + FUSED_KZ, G.N[2] - G.g[2]) : 1;
+    const long long sxy = (long long)G.N[0] * G.N[1];
+    const R theta = R(G.theta), gamma = R(G.gamma);
+    const R nx[3] = {R(G.nhat[0]), R(G.nhat[1]), R(G.nhat[2])};
+    const R ny[3] = {R(G.nhat[3]), R(G.nhat[4]), R(G.nhat[5])};
+    const R nz[3] = {R(G.nhat[6]), R(G.nhat[7]), R(G.nhat[8])};
+    const R A0 = R(G.dA[0]), A1 = R(G.dA[1]), A2 = R(G.dA[2]);
+    const R fac = R(-1.0 / G.dV);
+    const double dt = ctl->dt;
+    const int ic = min(i, G.N[0] - 1), jc = min(j, G.N[1] - 1);
+    const long long col = (long long)jc * G.N[0] + ic;
+
+    // z window W[q][m] = P(k - 1 + m), m = 0..3, and the carried half-slope of zone k
+    R W[5][4];
+    R hzc[5];
+    R Fzlo[5];
+    if constexpr (Z)
+    {
+        R w0[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q)
+        {
+            w0[q] = R(Pin[q * G.fs + col + (long long)(kb - 2) * sxy]);
+#pragma unroll
+            for (int m = 0; m < 3; ++m) W[q][m + 1] = R(Pin[q * G.fs + col + (long long)(kb - 1 + m) * sxy]);
+        }
+        // lower z-face of plane kb: zones kb-1 and kb
+        R PL[5], PR[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q)
+        {
+            const R hlo = half_slope(theta, w0[q], W[q][1], W[q][2]);
+            hzc[q] = half_slope(theta, W[q][1], W[q][2], W[q][3]);
+            PL[q] = W[q][1] + hlo;
+            PR[q] = W[q][2] + (-hzc[q]);
+        }
+        hydro_states_flux<Riem, R>(PL, PR, gamma, nz, Fzlo);
+    }
+
+    for (int k = kb; k < ke; ++k)
+    {
+        const long long pbase = (long long)k * sxy;
+        for (int t = tid; t < HX * HY; t += NT)
+        {
+            const int hx = t % HX, hy = t / HX;
+            const int gi = min(i0 - NG + hx, G.N[0] - 1), gj = min(j0 - NG + hy, G.N[1] - 1);
+            const long long c = pbase + (long long)gj * G.N[0] + gi;
+#pragma unroll
+            for (int q = 0; q < 5; ++q) plane[q][hy][hx] = R(Pin[q * G.fs + c]);
+        }
+        // z: slide the window to k-1..k+2, half-slope of zone k+1, upper z-face
+        R Fzhi[5];
+        if constexpr (Z)
+        {
+            R PL[5], PR[5];
+#pragma unroll
+            for (int q = 0; q < 5; ++q)
+            {
+                W[q][0] = W[q][1];
+                W[q][1] = W[q][2];
+                W[q][2] = W[q][3];
+                W[q][3] = R(Pin[q * G.fs + col + (long long)(k + 2) * sxy]);
+                const R hn = half_slope(theta, W[q][1], W[q][2], W[q][3]);
+                PL[q] = W[q][1] + hzc[q];
+                PR[q] = W[q][2] + (-hn);
+                hzc[q] = hn;
+            }
+            if (cell_xy) hydro_states_flux<Riem, R>(PL, PR, gamma, nz, Fzhi);
+        }
+        __syncthreads();
+
+        // ---- half-slopes: own zone in x (register) and y (LDS); tile-edge zones
+        R hx[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q)
+        {
+            hx[q] = half_slope(theta, plane[q][row + 2][lane + 1], plane[q][row + 2][lane + 2], plane[q][row + 2][lane + 3]);
+            sy[q][row + 1][lane] = half_slope(theta, plane[q][row + 1][lane + 2], plane[q][row + 2][lane + 2], plane[q][row + 3][lane + 2]);
+        }
+        if (lane == TX - 1)
+        {
+#pragma unroll
+            for (int q = 0; q < 5; ++q) sxe[q][row][1] = hx[q];
+        }
+        if (tid < 2 * TX + 2 * TY)
+        {
+            if (tid < 2 * TX)  // rows -1 and TY
+            {
+                const int cx = tid % TX;
+                const int r = tid < TX ? -1 : TY;
+#pragma unroll
+                for (int q = 0; q < 5; ++q)
+                    sy[q][r + 1][cx] = half_slope(theta, plane[q][r + 1][cx + 2], plane[q][r + 2][cx + 2], plane[q][r + 3][cx + 2]);
+            }
+            else  // zones i0 - 1 and i0 + 64 of each row
+            {
+                const int e = tid - 2 * TX;
+                const int r = e % TY;
+                const int xs = e < TY ? 1 : TX + 2;  // LDS x index of the zone
+#pragma unroll
+                for (int q = 0; q < 5; ++q)
+                    sxe[q][r][e < TY ? 0 : 2] = half_slope(theta, plane[q][r + 2][xs - 1], plane[q][r + 2][xs], plane[q][r + 2][xs + 1]);
+            }
+        }
+        __syncthreads();
+
+        // ---- left x-face and lower y-face of the zone; tile-edge faces ---------------
+        R Fx[5], Fy[5];
+        R hl[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q)
+        {
+            const R up = __shfl_up(hx[q], 1, 64);
+            hl[q] = lane == 0 ? sxe[q][row][0] : up;
+        }
+        if (xface)
+        {
+            R PL[5], PR[5];
+#pragma unroll
+            for (int q = 0; q < 5; ++q)
+            {
+                PL[q] = plane[q][row + 2][lane + 1] + hl[q];
+                PR[q] = plane[q][row + 2][lane + 2] + (-hx[q]);
+            }
+            hydro_states_flux<Riem, R>(PL, PR, gamma, nx, Fx);
+        }
+        if (yface)
+        {
+            R PL[5], PR[5];
+#pragma unroll
+            for (int q = 0; q < 5; ++q)
+            {
+                PL[q] = plane[q][row + 1][lane + 2] + sy[q][row][lane];
+                PR[q] = plane[q][row + 2][lane + 2] + (-sy[q][row + 1][lane]);
+            }
+            hydro_states_flux<Riem, R>(PL, PR, gamma, ny, Fy);
+#pragma unroll
+            for (int q = 0; q < 5; ++q) fy[q][row][lane] = Fy[q];
+        }
+        // the edge faces go to the last waves; the edge slopes went to the first ones
+        const int te = tid - (NT - TX - TY);
+        if (te >= 0)
+        {
+            R PL[5], PR[5], F[5];
+            if (te < TY)
+            {
+                const int r = te;
+                if (i0 + TX <= ie && j0 + r < je)
+                {
+#pragma unroll
+                    for (int q = 0; q < 5; ++q)
+                    {
+                        PL[q] = plane[q][r + 2][TX + 1] + sxe[q][r][1];
+                        PR[q] = plane[q][r + 2][TX + 2] + (-sxe[q][r][2]);
+                    }
+                    hydro_states_flux<Riem, R>(PL, PR, gamma, nx, F);
+#pragma unroll
+                    for (int q = 0; q < 5; ++q) fxe[q][r] = F[q];
+                }
+            }
+            else
+            {
+                const int cx = te - TY;
+                if (j0 + TY <= je && i0 + cx < ie)
+                {
+#pragma unroll
+                    for (int q = 0; q < 5; ++q)
+                    {
+                        PL[q] = plane[q][TY + 1][cx + 2] + sy[q][TY][cx];
+                        PR[q] = plane[q][TY + 2][cx + 2] + (-sy[q][TY + 1][cx]);
+                    }
+                    hydro_states_flux<Riem, R>(PL, PR, gamma, ny, F);
+#pragma unroll
+                    for (int q = 0; q < 5; ++q) fy[q][TY][cx] = F[q];
+                }
+            }
+        }
+        __syncthreads();
+
+        // ---- divergence, RK combine, primitive recovery ------------------------------
+        R Fxr[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q)
+        {
+            const R down = __shfl_down(Fx[q], 1, 64);
+            Fxr[q] = lane == TX - 1 ? fxe[q][row] : down;
+        }
+        if (cell_xy)
+        {
+            const long long c = pbase + col;
+            double p0[5], U0[5];
+#pragma unroll
+            for (int q = 0; q < 5; ++q) p0[q] = P0[q * G.fs + c];
+            prim_to_cons<0>(G.gamma, p0, U0);
+            double Un[5];
+#pragma unroll
+            for (int q = 0; q < 5; ++q)
+            {
+                const R fyu = fy[q][row + 1][lane];
+                R acc = +Fxr[q] * A0 - Fx[q] * A0 + fyu * A1 - Fy[q] * A1;
+                if constexpr (Z) acc = acc + Fzhi[q] * A2 - Fzlo[q] * A2;
+                else acc = acc + R(0) * A2 - R(0) * A2;
+                const double L = double(fac * acc);
+                const double Uq = first_stage ? U0[q] : U[q * G.fs + c];
+                Un[q] = U0[q] * (1 - b) + (Uq + dt * L) * b;
+            }
+            double Pn[5];
+            int floor;
+            cons_to_prim<0>(G.gamma, G.pfloor, Un, Pn, floor);
+#pragma unroll
+            for (int q = 0; q < 5; ++q) Pout[q * G.fs + c] = Pn[q];
+            if (courant_out)
+            {
+                int err = 0;
+                const double d = courant_of<0>(Pn, G, err);
+                if (d < dmin) dmin = d;
+            }
+            if (write_u)
+            {
+#pragma unroll
+                for (int q = 0; q < 5; ++q) U[q * G.fs + c] = Un[q];
+            }
+        }
+        if constexpr (Z)
+        {
+#pragma unroll
+            for (int q = 0; q < 5; ++q) Fzlo[q] = Fzhi[q];
+        }
+        __syncthreads();
+    }
+    if (courant_out) courant_to_bins(dmin, wmin, TX * TY / 64, tid, ctl);
 }
 
 // The fused path covers the hydro law on 2-D and 3-D blocks; 1-D blocks and
@@ -263,48 +565,62 @@ inline bool fused_supported(const mara_desc* d)
 
 inline const char* fused_kernel_name(const mara_desc& d)
 {
-    return d.precision == MARA_PREC_MIXED ? "k_fused_hydro<float>" : "k_fused_hydro<double>";
+    const bool mixed = d.precision == MARA_PREC_MIXED;
+    if (d.recon == MARA_RECON_PLM) return mixed ? "k_fused_plm<float>" : "k_fused_plm<double>";
+    return mixed ? "k_fused_hydro<float>" : "k_fused_hydro<double>";
 }
 
 template <int Recon, int Riem, typename R>
 inline void launch_fused_t(const Geom& G, hipStream_t s, const double* Pin, const double* P0, double* U,
-                           double* Pout, double b, int first, int write_u, Ctl* ctl)
+                           double* Pout, double b, int first, int write_u, int cout, Ctl* ctl)
 {
     const bool z = G.N[2] > 1;
     dim3 block(FUSED_TX, FUSED_TY, 1);
     dim3 grid((G.n[0] + FUSED_TX - 1) / FUSED_TX, (G.n[1] + FUSED_TY - 1) / FUSED_TY,
               z ? (G.n[2] + FUSED_KZ - 1) / FUSED_KZ : 1);
-    if (z) k_fused_hydro<Recon, Riem, R, true><<<grid, block, 0, s>>>(Pin, P0, U, Pout, G, b, first, write_u, ctl);
-    else k_fused_hydro<Recon, Riem, R, false><<<grid, block, 0, s>>>(Pin, P0, U, Pout, G, b, first, write_u, ctl);
+    if constexpr (Recon == 1)
+    {
+        constexpr int TY = PlmTile<R>::TY;
+        const dim3 pblock(FUSED_TX, TY, 1);
+        const dim3 pgrid((G.n[0] + FUSED_TX - 1) / FUSED_TX, (G.n[1] + TY - 1) / TY,
+                         z ? (G.n[2] + FUSED_KZ - 1) / FUSED_KZ : 1);
+        if (z) k_fused_plm<Riem, R, true><<<pgrid, pblock, 0, s>>>(Pin, P0, U, Pout, G, b, first, write_u, cout, ctl);
+        else k_fused_plm<Riem, R, false><<<pgrid, pblock, 0, s>>>(Pin, P0, U, Pout, G, b, first, write_u, cout, ctl);
+    }
+    else
+    {
+        if (z) k_fused_hydro<Recon, Riem, R, true><<<grid, block, 0, s>>>(Pin, P0, U, Pout, G, b, first, write_u, cout, ctl);
+        else k_fused_hydro<Recon, Riem, R, false><<<grid, block, 0, s>>>(Pin, P0, U, Pout, G, b, first, write_u, cout, ctl);
+    }
 }
 
 template <typename R>
 inline void launch_fused_r(const mara_desc& d, const Geom& G, hipStream_t s, const double* Pin, const double* P0,
-                           double* U, double* Pout, double b, int first, int write_u, Ctl* ctl)
+                           double* U, double* Pout, double b, int first, int write_u, int cout, Ctl* ctl)
 {
     const bool hllc = d.riemann == MARA_RIEMANN_HLLC;
     switch (d.recon)
     {
         case MARA_RECON_PCM:
-            if (hllc) launch_fused_t<0, 1, R>(G, s, Pin, P0, U, Pout, b, first, write_u, ctl);
-            else launch_fused_t<0, 0, R>(G, s, Pin, P0, U, Pout, b, first, write_u, ctl);
+            if (hllc) launch_fused_t<0, 1, R>(G, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl);
+            else launch_fused_t<0, 0, R>(G, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl);
             break;
         case MARA_RECON_PLM:
-            if (hllc) launch_fused_t<1, 1, R>(G, s, Pin, P0, U, Pout, b, first, write_u, ctl);
-            else launch_fused_t<1, 0, R>(G, s, Pin, P0, U, Pout, b, first, write_u, ctl);
+            if (hllc) launch_fused_t<1, 1, R>(G, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl);
+            else launch_fused_t<1, 0, R>(G, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl);
             break;
         default:
-            if (hllc) launch_fused_t<2, 1, R>(G, s, Pin, P0, U, Pout, b, first, write_u, ctl);
-            else launch_fused_t<2, 0, R>(G, s, Pin, P0, U, Pout, b, first, write_u, ctl);
+            if (hllc) launch_fused_t<2, 1, R>(G, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl);
+            else launch_fused_t<2, 0, R>(G, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl);
             break;
     }
 }
 
 inline void launch_fused(const mara_desc& d, const Geom& G, hipStream_t s, const double* Pin, const double* P0,
-                         double* U, double* Pout, double b, int first, int write_u, Ctl* ctl)
+                         double* U, double* Pout, double b, int first, int write_u, int cout, Ctl* ctl)
 {
-    if (d.precision == MARA_PREC_MIXED) launch_fused_r<float>(d, G, s, Pin, P0, U, Pout, b, first, write_u, ctl);
-    else launch_fused_r<double>(d, G, s, Pin, P0, U, Pout, b, first, write_u, ctl);
+    if (d.precision == MARA_PREC_MIXED) launch_fused_r<float>(d, G, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl);
+    else launch_fused_r<double>(d, G, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl);
 }
 
 } // namespace mara

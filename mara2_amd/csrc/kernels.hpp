@@ -19,6 +19,11 @@ namespace mara {
 // Control block shared by the kernels of one mesh block: the step size and
 // clock of the device-driven loop, the active flag that lets queued kernels
 // become no-ops once the loop is done, and the failure counters.
+// The CFL minimum is reduced into CBINS accumulators 128 bytes apart (one
+// atomic per work-group, spread over L2 channels) and folded by one thread.
+constexpr int CBINS = 64;
+constexpr int CBIN_STRIDE = 16;
+
 struct Ctl
 {
     double dt;
@@ -36,7 +41,10 @@ struct Ctl
     int steps;
     int max_steps;
     int pad;
+    unsigned long long cbins[CBINS * CBIN_STRIDE];
 };
+
+constexpr unsigned long long DBL_MAX_BITS = 0x7fefffffffffffffull;
 
 struct Geom
 {
@@ -70,23 +78,42 @@ __device__ __forceinline__ unsigned long long ref_linear(const Geom& G, int i, i
 __device__ __forceinline__ unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
 __device__ __forceinline__ double bitsd(unsigned long long b) { return __longlong_as_double((long long)b); }
 
+template <typename T>
+__device__ __forceinline__ T wave_min(T v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+    {
+        const T w = __shfl_xor(v, o, 64);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_max(T v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+    {
+        const T w = __shfl_xor(v, o, 64);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
+
 // ---------------------------------------------------------------------------
 // Face flux sweep: MeshOperator::godunov + the IntercellFluxScheme
 // (src/MeshOperator.cpp:443-525, src/IntercellFluxSchemes.cpp:45-117).
 // One thread per face of the box [lo, hi) of face indices.
 // ---------------------------------------------------------------------------
 template <int Law, int Recon, int Riem, typename R>
-__global__ void __launch_bounds__(256)
-k_flux(const double* __restrict__ P, R* __restrict__ F, const Geom G, int axis,
-       int lo0, int lo1, int lo2, int m0, int m1, int m2, const Ctl* __restrict__ ctl, Ctl* ctlw)
+__device__ __forceinline__ void flux_face(const double* __restrict__ P, R* __restrict__ F, const Geom& G, int axis,
+                                          int i, int j, int k, Ctl* ctlw)
 {
     constexpr int NQ = LawNQ<Law>::value;
     constexpr int NG = Recon == 0 ? 1 : Recon == 1 ? 2 : 3;
-    if (!ctl->active) return;
-    const int ti = blockIdx.x * blockDim.x + threadIdx.x;
-    const int tj = blockIdx.y * blockDim.y + threadIdx.y;
-    if (ti >= m0 || tj >= m1) return;
-    const int i = lo0 + ti, j = lo1 + tj, k = lo2 + (int)blockIdx.z;
     const long long stride = axis == 0 ? 1 : axis == 1 ? (long long)G.N[0] : (long long)G.N[0] * G.N[1];
     const long long c0 = cell_index(G, i, j, k) - NG * stride;  // first stencil cell
 
@@ -137,6 +164,18 @@ k_flux(const double* __restrict__ P, R* __restrict__ F, const Geom G, int axis,
     for (int q = 0; q < NQ; ++q) F[q * G.ffs + f] = Fq[q];
 }
 
+template <int Law, int Recon, int Riem, typename R>
+__global__ void __launch_bounds__(256)
+k_flux(const double* __restrict__ P, R* __restrict__ F, const Geom G, int axis,
+       int lo0, int lo1, int lo2, int m0, int m1, int m2, Ctl* ctl)
+{
+    if (!ctl->active) return;
+    const int ti = blockIdx.x * blockDim.x + threadIdx.x;
+    const int tj = blockIdx.y * blockDim.y + threadIdx.y;
+    if (ti >= m0 || tj >= m1) return;
+    flux_face<Law, Recon, Riem, R>(P, F, G, axis, lo0 + ti, lo1 + tj, lo2 + (int)blockIdx.z, ctl);
+}
+
 // ---------------------------------------------------------------------------
 // Flux-CT (CellCenteredFieldCT::correctGodunovFluxes, Toth 2000 eq. 25,
 // src/CellCenteredFieldCT.cpp:14-134).  For face axis a and transverse
@@ -148,16 +187,9 @@ k_flux(const double* __restrict__ P, R* __restrict__ F, const Geom G, int axis,
 // G layout: (a * 3 + c) * ffs + f.
 // ---------------------------------------------------------------------------
 template <typename R>
-__global__ void __launch_bounds__(256)
-k_ct(const R* __restrict__ Fx, const R* __restrict__ Fy, const R* __restrict__ Fz,
-     double* __restrict__ Gct, Geom G, int a, int lo0, int lo1, int lo2, int m0, int m1, int m2,
-     const Ctl* __restrict__ ctl)
+__device__ __forceinline__ void ct_face(const R* __restrict__ Fx, const R* __restrict__ Fy, const R* __restrict__ Fz,
+                                        double* __restrict__ Gct, const Geom& G, int a, int i, int j, int k)
 {
-    if (!ctl->active) return;
-    const int ti = blockIdx.x * blockDim.x + threadIdx.x;
-    const int tj = blockIdx.y * blockDim.y + threadIdx.y;
-    if (ti >= m0 || tj >= m1) return;
-    const int i = lo0 + ti, j = lo1 + tj, k = lo2 + (int)blockIdx.z;
     const R* Fa[3] = {Fx, Fy, Fz};
     const long long fstride[3] = {1, (long long)(G.N[0] + 1), (long long)(G.N[0] + 1) * (G.N[1] + 1)};
     const long long f = face_index(G, i, j, k);
@@ -182,6 +214,73 @@ k_ct(const R* __restrict__ Fx, const R* __restrict__ Fy, const R* __restrict__ F
     }
 }
 
+template <typename R>
+__global__ void __launch_bounds__(256)
+k_ct(const R* __restrict__ Fx, const R* __restrict__ Fy, const R* __restrict__ Fz,
+     double* __restrict__ Gct, Geom G, int a, int lo0, int lo1, int lo2, int m0, int m1, int m2,
+     const Ctl* __restrict__ ctl)
+{
+    if (!ctl->active) return;
+    const int ti = blockIdx.x * blockDim.x + threadIdx.x;
+    const int tj = blockIdx.y * blockDim.y + threadIdx.y;
+    if (ti >= m0 || tj >= m1) return;
+    ct_face<R>(Fx, Fy, Fz, Gct, G, a, lo0 + ti, lo1 + tj, lo2 + (int)blockIdx.z);
+}
+
+// ---------------------------------------------------------------------------
+// CFL time step of one zone (FieldOperator::getCourantTimestep, src/FieldOperator.cpp:125-169)
+// ---------------------------------------------------------------------------
+template <int Law>
+__device__ __forceinline__ double courant_of(const double* p, const Geom& G, int& err)
+{
+    constexpr int NQ = LawNQ<Law>::value;
+    double lam[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+    {
+        State<double, NQ> S;
+        err |= law_state<Law, double>(G.gamma, p, G.nhat + 3 * a, S);
+        double m = 0.0;
+#pragma unroll
+        for (int n = 0; n < NQ; ++n)
+        {
+            const double x = fabs(S.A[n]);
+            if (m < x) m = x;
+        }
+        lam[a] = m;
+    }
+    double lmax = lam[0];
+    if (lmax < lam[1]) lmax = lam[1];
+    if (lmax < lam[2]) lmax = lam[2];
+    return G.Lmin / lmax;
+}
+
+template <int Law>
+__device__ __forceinline__ double zone_courant(const double* __restrict__ P, const Geom& G, long long c, int& err)
+{
+    constexpr int NQ = LawNQ<Law>::value;
+    double p[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) p[q] = P[q * G.fs + c];
+    return courant_of<Law>(p, G, err);
+}
+
+// Work-group minimum of v (every thread of the group must call), then one
+// atomicMin into the group's bin.  wmin: __shared__ double[waves].
+__device__ __forceinline__ void courant_to_bins(double v, double* wmin, int nwaves, int tid, Ctl* ctl)
+{
+    v = wave_min(v);
+    if ((tid & 63) == 0) wmin[tid >> 6] = v;
+    __syncthreads();
+    if (tid == 0)
+    {
+        double m = wmin[0];
+        for (int w = 1; w < nwaves; ++w) if (wmin[w] < m) m = wmin[w];
+        const unsigned bin = (blockIdx.x + 7u * blockIdx.y + 13u * blockIdx.z) % CBINS;
+        if (m < 1.79769313486231570815e+308) atomicMin(&ctl->cbins[bin * CBIN_STRIDE], dbits(m));
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Zone update: L = -div F (src/MeshOperator.cpp:323-381), RK combine
 // U = U0 (1-b) + (U + dt L) b (src/SolutionSchemes.cpp:232-235), and
@@ -189,18 +288,13 @@ k_ct(const R* __restrict__ Fx, const R* __restrict__ Fy, const R* __restrict__ F
 // U0 is regenerated from the step's starting primitives P0.
 // ---------------------------------------------------------------------------
 template <int Law, typename R, bool CT>
-__global__ void __launch_bounds__(256)
-k_update(const double* __restrict__ P0, double* __restrict__ U, double* __restrict__ Pout,
-         const R* __restrict__ Fx, const R* __restrict__ Fy, const R* __restrict__ Fz,
-         const double* __restrict__ Gct, Geom G, double b, int first_stage, int write_u,
-         Ctl* ctl)
+__device__ __forceinline__ double update_zone(const double* __restrict__ P0, double* __restrict__ U, double* __restrict__ Pout,
+                                              const R* __restrict__ Fx, const R* __restrict__ Fy, const R* __restrict__ Fz,
+                                              const double* __restrict__ Gct, const Geom& G, double b, int first_stage,
+                                              int write_u, int courant_out, Ctl* ctl, int i, int j, int k)
 {
+    double dtc = 1.79769313486231570815e+308;
     constexpr int NQ = LawNQ<Law>::value;
-    if (!ctl->active) return;
-    const int ti = blockIdx.x * blockDim.x + threadIdx.x;
-    const int tj = blockIdx.y * blockDim.y + threadIdx.y;
-    if (ti >= G.n[0] || tj >= G.n[1]) return;
-    const int i = G.g[0] + ti, j = G.g[1] + tj, k = G.g[2] + (int)blockIdx.z;
     const long long c = cell_index(G, i, j, k);
     const long long f = face_index(G, i, j, k);
     const long long fx = f + 1, fy = f + (G.N[0] + 1), fz = f + (long long)(G.N[0] + 1) * (G.N[1] + 1);
@@ -251,6 +345,15 @@ k_update(const double* __restrict__ P0, double* __restrict__ U, double* __restri
     {
 #pragma unroll
         for (int q = 0; q < NQ; ++q) Pout[q * G.fs + c] = Pn[q];
+        if (courant_out)
+        {
+            // the next step's CFL from the zone's new primitives (interior only,
+            // as FieldOperator::getCourantTimestep reads them)
+            int err = 0;
+            dtc = courant_of<Law>(Pn, G, err);
+            if (err & ERR_SUPERLUMINAL) atomicAdd(&ctl->superluminal, 1ull);
+            if (err & ERR_EIGEN) atomicAdd(&ctl->eigen_fail, 1ull);
+        }
     }
     if (floor) atomicAdd(&ctl->floor_applied, 1ull);
     if (write_u)
@@ -258,24 +361,38 @@ k_update(const double* __restrict__ P0, double* __restrict__ U, double* __restri
 #pragma unroll
         for (int q = 0; q < NQ; ++q) U[q * G.fs + c] = Un[q];
     }
+    return dtc;
+}
+
+template <int Law, typename R, bool CT>
+__global__ void __launch_bounds__(256)
+k_update(const double* __restrict__ P0, double* __restrict__ U, double* __restrict__ Pout,
+         const R* __restrict__ Fx, const R* __restrict__ Fy, const R* __restrict__ Fz,
+         const double* __restrict__ Gct, Geom G, double b, int first_stage, int write_u,
+         int courant_out, Ctl* ctl)
+{
+    __shared__ double wmin[4];
+    if (!ctl->active) return;
+    const int ti = blockIdx.x * blockDim.x + threadIdx.x;
+    const int tj = blockIdx.y * blockDim.y + threadIdx.y;
+    double dtc = 1.79769313486231570815e+308;
+    if (ti < G.n[0] && tj < G.n[1])
+        dtc = update_zone<Law, R, CT>(P0, U, Pout, Fx, Fy, Fz, Gct, G, b, first_stage, write_u, courant_out, ctl,
+                                      G.g[0] + ti, G.g[1] + tj, G.g[2] + (int)blockIdx.z);
+    if (courant_out) courant_to_bins(dtc, wmin, 4, threadIdx.y * blockDim.x + threadIdx.x, ctl);
 }
 
 // ---------------------------------------------------------------------------
 // Ghost fill of one axis, both sides (src/BoundaryConditions.cpp:24-176).
 // kind: 0 periodic, 1 outflow, 2 reflecting.  Threads: 2 * g * N_t1 * N_t2.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256)
-k_bc(double* __restrict__ P, Geom G, int axis, int kind, const Ctl* __restrict__ ctl)
+__device__ __forceinline__ void bc_guard(double* __restrict__ P, const Geom& G, int axis, int kind, long long t)
 {
-    if (!ctl->active) return;
     const int g = G.g[axis];
     const int N = G.N[axis];
     const int t1 = axis == 0 ? 1 : 0;
     const int t2 = axis == 2 ? 1 : 2;
     const long long slab = (long long)G.N[t1] * G.N[t2];
-    const long long total = 2ll * g * slab;
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= total) return;
     const int a1 = (int)(t % G.N[t1]);
     const int a2 = (int)((t / G.N[t1]) % G.N[t2]);
     const int layer = (int)((t / slab) % g);
@@ -310,62 +427,26 @@ k_bc(double* __restrict__ P, Geom G, int axis, int kind, const Ctl* __restrict__
     }
 }
 
+__device__ __forceinline__ long long bc_count(const Geom& G, int axis)
+{
+    const int t1 = axis == 0 ? 1 : 0;
+    const int t2 = axis == 2 ? 1 : 2;
+    return 2ll * G.g[axis] * G.N[t1] * G.N[t2];
+}
+
+__global__ void __launch_bounds__(256)
+k_bc(double* __restrict__ P, Geom G, int axis, int kind, const Ctl* __restrict__ ctl)
+{
+    if (!ctl->active) return;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= bc_count(G, axis)) return;
+    bc_guard(P, G, axis, kind, t);
+}
+
 // ---------------------------------------------------------------------------
 // CFL reduction (FieldOperator::getCourantTimestep, src/FieldOperator.cpp:125-169):
 // per interior zone L / max(lambda_x, lambda_y, lambda_z), minimum over zones.
 // ---------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ T wave_min(T v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1)
-    {
-        const T w = __shfl_xor(v, o, 64);
-        v = w < v ? w : v;
-    }
-    return v;
-}
-
-template <typename T>
-__device__ __forceinline__ T wave_max(T v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1)
-    {
-        const T w = __shfl_xor(v, o, 64);
-        v = w > v ? w : v;
-    }
-    return v;
-}
-
-template <int Law>
-__device__ __forceinline__ double zone_courant(const double* __restrict__ P, const Geom& G, long long c, int& err)
-{
-    constexpr int NQ = LawNQ<Law>::value;
-    double p[NQ];
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) p[q] = P[q * G.fs + c];
-    double lam[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-    {
-        State<double, NQ> S;
-        err |= law_state<Law, double>(G.gamma, p, G.nhat + 3 * a, S);
-        double m = 0.0;
-#pragma unroll
-        for (int n = 0; n < NQ; ++n)
-        {
-            const double x = fabs(S.A[n]);
-            if (m < x) m = x;
-        }
-        lam[a] = m;
-    }
-    double lmax = lam[0];
-    if (lmax < lam[1]) lmax = lam[1];
-    if (lmax < lam[2]) lmax = lam[2];
-    return G.Lmin / lmax;
-}
-
 // Grid-stride over interior zones; wavefront then work-group minimum, one
 // atomic per work-group.
 template <int Law>
@@ -391,15 +472,7 @@ k_courant(const double* __restrict__ P, Geom G, Ctl* ctl)
     }
     if (err & ERR_SUPERLUMINAL) atomicAdd(&ctl->superluminal, 1ull);
     if (err & ERR_EIGEN) atomicAdd(&ctl->eigen_fail, 1ull);
-    dtc = wave_min(dtc);
-    if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = dtc;
-    __syncthreads();
-    if (threadIdx.x == 0)
-    {
-        double m = wmin[0];
-        for (int w = 1; w < 4; ++w) if (wmin[w] < m) m = wmin[w];
-        if (m < 1.79769313486231570815e+308) atomicMin(&ctl->courant_bits, dbits(m));
-    }
+    courant_to_bins(dtc, wmin, 4, threadIdx.x, ctl);
 }
 
 // ---------------------------------------------------------------------------
@@ -504,13 +577,34 @@ __global__ void k_ctl_begin_advance(Ctl* ctl, double dt)
     ctl->first_fail = ~0ull;
 }
 
+__device__ __forceinline__ void reset_bins(Ctl* ctl)
+{
+    for (int b = 0; b < CBINS; ++b) ctl->cbins[b * CBIN_STRIDE] = DBL_MAX_BITS;
+}
+
+// courant_bits <- min over the bins, bins reset
+__device__ __forceinline__ void fold_bins(Ctl* ctl)
+{
+    unsigned long long m = DBL_MAX_BITS;
+    for (int b = 0; b < CBINS; ++b)
+    {
+        const unsigned long long v = ctl->cbins[b * CBIN_STRIDE];
+        if (v < m) m = v;
+    }
+    ctl->courant_bits = m;
+    reset_bins(ctl);
+}
+
 __global__ void k_ctl_begin_courant(Ctl* ctl)
 {
     ctl->active = 1;
-    ctl->courant_bits = 0x7fefffffffffffffull;  // DBL_MAX
+    ctl->courant_bits = DBL_MAX_BITS;
     ctl->eigen_fail = 0;
     ctl->superluminal = 0;
+    reset_bins(ctl);
 }
+
+__global__ void k_ctl_fold_bins(Ctl* ctl) { fold_bins(ctl); }
 
 __global__ void k_ctl_begin_monopole(Ctl* ctl)
 {
@@ -530,6 +624,7 @@ __global__ void k_ctl_begin_run(Ctl* ctl, double t, double t_final, double cfl, 
     ctl->eigen_fail = 0;
     ctl->superluminal = 0;
     ctl->first_fail = ~0ull;
+    reset_bins(ctl);
 }
 
 // maraMainLoop: while (t < t_final) { dt = min(cfl * courant, t_final - t); ... }
@@ -538,12 +633,15 @@ __global__ void k_ctl_step_begin(Ctl* ctl)
     const bool go = ctl->active && ctl->t < ctl->t_final && ctl->steps < ctl->max_steps
                     && ctl->c2p_fail == 0 && ctl->eigen_fail == 0 && ctl->superluminal == 0;
     ctl->active = go ? 1 : 0;
-    ctl->courant_bits = 0x7fefffffffffffffull;
 }
 
+// the CFL of the state at the start of the step sits in the bins (from the
+// standalone reduction on the first step, from the previous step's last
+// stage afterwards)
 __global__ void k_ctl_step_dt(Ctl* ctl)
 {
     if (!ctl->active) return;
+    fold_bins(ctl);
     const double dt1 = ctl->cfl * bitsd(ctl->courant_bits);
     const double dt2 = ctl->t_final - ctl->t;
     ctl->dt = dt1 < dt2 ? dt1 : dt2;

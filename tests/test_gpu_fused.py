@@ -83,3 +83,31 @@ def test_mixed_mhd_monopole_roundoff():
         blk.run(0.4, 1e30, 10)
         m = blk.monopole_max()
     assert m < 1e-12, m
+
+
+@pytest.mark.parametrize("case", ["turbulence3d", "hydro2d", "mhd2d", "hydro3d_rk3"])
+def test_device_loop_matches_oracle_loop(case):
+    """mara_block_run (device dt, clock and stop flag; each step's CFL reduced
+    by the previous step's last stage) against the oracle's maraMainLoop."""
+    import oracle
+    if case == "turbulence3d":
+        desc = problems.turbulence_desc(32)
+        P0, cfl = problems.turbulence_3d(desc), 0.3
+    elif case == "hydro2d":
+        desc = abi.make_desc(law="hydro", recon="plm", riemann="hllc", bc="outflow", cells=(100, 37, 1))
+        P0, cfl = _random_state(desc, 21), 0.4
+    elif case == "mhd2d":
+        desc = problems.mhd_desc(48)
+        P0, cfl = problems.mhd_2d(desc), 0.4
+    else:
+        desc = abi.make_desc(law="hydro", recon="plm", riemann="hlle", bc="periodic", cells=(40, 24, 20), rk_order=3)
+        P0, cfl = _random_state(desc, 22), 0.3
+    Po = P0.copy()
+    err, to, so = oracle.run(desc, Po, cfl, 0.05, max_steps=12)
+    assert err == 0
+    with Block(desc) as blk:
+        blk.upload(P0)
+        tg, sg, rep = blk.run(cfl, 0.05, 12)
+        Pg = blk.download()
+    assert sg == so and tg == to
+    assert np.array_equal(Pg, Po)
