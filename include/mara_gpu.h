@@ -164,6 +164,61 @@ long long mara_block_device_bytes(const mara_block* block);
    (used for admission by HBM budget). */
 long long mara_desc_device_bytes(const mara_desc* desc);
 
+/* ---------------------------------------------------------------------------
+   Batches: many independent solves resident in one pool of device memory,
+   advanced together with one launch per kernel type and solve class per RK
+   stage.  Each solve keeps the single block's arithmetic, its own time step,
+   clock, stop condition and failure counters, so its result is bitwise the
+   same alone or in any mix.  A solve is the maraMainLoop contract of
+   mara_block_run (src/Mara.cpp:42-88) on one MeshData: dt = min(cfl *
+   courant_dt, t_final - t) while t < t_final, at most max_steps steps.
+   The reference runs such solves one process at a time; this has no single
+   counterpart there and serves callers that run many small problems (the
+   serving loop, parameter sweeps).
+   --------------------------------------------------------------------------- */
+typedef struct mara_batch mara_batch;
+
+/* A batch owning hbm_budget bytes of device memory for solve state and up
+   to max_solves solves; rk_order and precision are shared by its solves.
+   MARA_ERR_CAPACITY if the budget cannot be allocated. */
+int mara_batch_create(long long hbm_budget, int rk_order, int precision, int max_solves,
+                      mara_batch** out);
+int mara_batch_destroy(mara_batch* batch);
+
+/* Device bytes one solve with this descriptor takes from the pool. */
+long long mara_batch_item_bytes(const mara_desc* desc, int precision);
+/* Pool bytes not held by a solve (the largest admissible solve may be
+   smaller: the pool is first-fit). */
+long long mara_batch_free_bytes(const mara_batch* batch);
+
+/* Admit a solve: P in MeshData::P layout with guards filled, as
+   mara_block_upload_prim; the clock starts at 0.  *slot receives its handle.
+   MARA_ERR_CAPACITY when no slot or no contiguous pool range is free (the
+   caller retries after a retire); MARA_ERR_UNSUPPORTED when desc's rk_order
+   or precision differ from the batch's. */
+int mara_batch_admit(mara_batch* batch, const mara_desc* desc, const double* P, double cfl,
+                     double t_final, int max_steps, int* slot);
+
+/* Advance every unfinished solve by up to n_steps of its own steps; returns
+   after the device has drained.  *n_active receives the number of admitted
+   solves not yet finished. */
+int mara_batch_step(mara_batch* batch, int n_steps, int* n_active);
+
+/* State of one solve as of the last mara_batch_step: *done is 1 once it
+   reached t_final, its step limit or a failure.  Returns the solve's status
+   (the codes of mara_block_run) and fills report when non-null. */
+int mara_batch_query(mara_batch* batch, int slot, int* done, double* t, int* steps,
+                     mara_report* report);
+
+/* Slots whose solve finished as of the last mara_batch_step and is not yet
+   retired, in slot order: up to capacity of them into slots, their number
+   into *count. */
+int mara_batch_finished(const mara_batch* batch, int* slots, int capacity, int* count);
+
+/* Copy the solve's primitives into P (MeshData::P layout; skipped when P
+   is null) and release its slot and memory. */
+int mara_batch_retire(mara_batch* batch, int slot, double* P);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,14 +13,14 @@
 #include "../../include/mara_gpu.h"
 #include "kernels.hpp"
 #include "fused.hpp"
+#include "host_common.hpp"
 
 using namespace mara;
+using namespace mara::host;
 
 namespace {
 
 constexpr int kThreads = 256;
-
-long long round_up(long long x, long long m) { return (x + m - 1) / m * m; }
 
 unsigned grid_for(long long n) { return (unsigned)((n + kThreads - 1) / kThreads); }
 
@@ -29,61 +29,7 @@ unsigned grid_for(long long n) { return (unsigned)((n + kThreads - 1) / kThreads
 dim3 block3() { return dim3(64, 4, 1); }
 dim3 grid3(int m0, int m1, int m2) { return dim3((m0 + 63) / 64, (m1 + 3) / 4, m2); }
 
-bool hip_ok(hipError_t e)
-{
-    if (e != hipSuccess)
-    {
-        std::fprintf(stderr, "mara_gpu: HIP error %d (%s)\n", (int)e, hipGetErrorString(e));
-        return false;
-    }
-    return true;
-}
-
 #define HIP_TRY(call) do { if (!hip_ok(call)) return MARA_ERR_HIP; } while (0)
-
-int law_nq(int law) { return law == MARA_LAW_HYDRO ? 5 : 8; }
-int recon_ng(int recon) { return recon == MARA_RECON_PCM ? 1 : recon == MARA_RECON_PLM ? 2 : 3; }
-
-// UnitVector::xhat/yhat/zhat cartesian components (src/UnitVector.cpp:9-39):
-// (sin(acos mu) cos phi, sin(acos mu) sin phi, mu) with (mu,phi) = (0,0),
-// (0, pi/2), (1, 0), evaluated with the host libm like the reference.
-void unit_vectors(double out[9])
-{
-    const double mu[3] = {0.0, 0.0, 1.0};
-    const double phi[3] = {0.0, M_PI / 2, 0.0};
-    for (int a = 0; a < 3; ++a)
-    {
-        const double st = std::sqrt(1 - mu[a] * mu[a]);
-        out[3 * a + 0] = st * std::cos(phi[a]);
-        out[3 * a + 1] = st * std::sin(phi[a]);
-        out[3 * a + 2] = mu[a];
-    }
-}
-
-int validate(const mara_desc* d)
-{
-    if (!d) return MARA_ERR_INVALID_ARGUMENT;
-    if (d->law < 0 || d->law > 2) return MARA_ERR_INVALID_ARGUMENT;
-    if (d->recon < 0 || d->recon > 2) return MARA_ERR_INVALID_ARGUMENT;
-    if (d->riemann < 0 || d->riemann > 1) return MARA_ERR_INVALID_ARGUMENT;
-    if (d->bc < 0 || d->bc > 2) return MARA_ERR_INVALID_ARGUMENT;
-    if (d->rk_order < 1 || d->rk_order > 3) return MARA_ERR_INVALID_ARGUMENT;
-    if (d->precision < 0 || d->precision > 1) return MARA_ERR_INVALID_ARGUMENT;
-    if (d->riemann == MARA_RIEMANN_HLLC && d->law != MARA_LAW_HYDRO) return MARA_ERR_UNSUPPORTED;
-    const int ng = recon_ng(d->recon);
-    for (int a = 0; a < 3; ++a)
-    {
-        if (d->cells[a] < 1 || d->guards[a] < 0) return MARA_ERR_INVALID_ARGUMENT;
-        if (!(d->upper[a] > d->lower[a])) return MARA_ERR_INVALID_ARGUMENT;
-        const int N = d->cells[a] + 2 * d->guards[a];
-        // SchemeHelpers::makeFootprint (src/SolutionSchemes.cpp:45-48)
-        if (N > 1 && d->guards[a] < ng) return MARA_ERR_GUARD_TOO_SMALL;
-        // the ghost fill reads only interior zones
-        if (d->guards[a] > 0 && d->cells[a] < d->guards[a]) return MARA_ERR_UNSUPPORTED;
-    }
-    if (d->cells[0] == 1 && (d->cells[1] > 1 || d->cells[2] > 1)) return MARA_ERR_UNSUPPORTED;
-    return MARA_OK;
-}
 
 struct Sizes
 {
@@ -119,7 +65,7 @@ long long device_bytes(const mara_desc* d, const Sizes& s)
     if (!fused_supported(d))
         b += (long long)(s.fleshed[0] + s.fleshed[1] + s.fleshed[2]) * s.nq * s.ffs * s.real;
     if (s.ct) b += 9ll * s.ffs * 8;
-    b += 256;  // control block
+    b += 256 + CBINS * CBIN_STRIDE * 8;  // control block and CFL bins
     return b;
 }
 
@@ -141,6 +87,7 @@ struct mara_block
     double* Gct = nullptr;
     Ctl* ctl = nullptr;
     Ctl* ctl_host = nullptr;
+    unsigned long long* cbins = nullptr;
     long long bytes = 0;
     bool timing = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -346,32 +293,7 @@ int read_ctl(mara_block* b)
     return MARA_OK;
 }
 
-int fill_report(mara_block* b, mara_report* r)
-{
-    const Ctl& c = *b->ctl_host;
-    int status = MARA_OK;
-    if (c.superluminal) status = MARA_ERR_SUPERLUMINAL;
-    else if (c.eigen_fail) status = MARA_ERR_BAD_EIGENVALUES;
-    else if (c.c2p_fail) status = MARA_ERR_INVERSION_FAILURE;
-    if (r)
-    {
-        r->status = status;
-        r->c2p_failures = (long long)c.c2p_fail;
-        r->floor_applied = (long long)c.floor_applied;
-        r->eigen_failures = (long long)c.eigen_fail;
-        r->superluminal = (long long)c.superluminal;
-        r->first_failure[0] = r->first_failure[1] = r->first_failure[2] = -1;
-        if (c.c2p_fail)
-        {
-            const unsigned long long lin = c.first_fail;
-            const unsigned long long n1 = b->G.n[1], n2 = b->G.n[2];
-            r->first_failure[2] = (int)(lin % n2);
-            r->first_failure[1] = (int)((lin / n2) % n1);
-            r->first_failure[0] = (int)(lin / (n1 * n2));
-        }
-    }
-    return status;
-}
+int fill_report(mara_block* b, mara_report* r) { return host::fill_report(*b->ctl_host, b->G, r); }
 
 } // namespace
 
@@ -424,32 +346,7 @@ int mara_block_create(const mara_desc* d, mara_block** out)
     b->S = sizes_of(d);
     b->ng = recon_ng(d->recon);
     b->fused = fused_supported(d);
-    Geom& G = b->G;
-    for (int a = 0; a < 3; ++a)
-    {
-        G.N[a] = d->cells[a] + 2 * d->guards[a];
-        G.g[a] = d->guards[a];
-        G.n[a] = d->cells[a];
-    }
-    G.nq = b->S.nq;
-    G.fs = b->S.fs;
-    G.ffs = b->S.ffs;
-    G.gamma = d->gamma;
-    G.pfloor = d->pressure_floor;
-    G.theta = d->plm_theta;
-    // CartesianMeshGeometry::cacheSpacing (src/CartesianMeshGeometry.cpp:145-157)
-    double dx[3];
-    for (int a = 0; a < 3; ++a) dx[a] = (d->upper[a] - d->lower[a]) / d->cells[a];
-    G.dA[0] = dx[1] * dx[2];
-    G.dA[1] = dx[2] * dx[0];
-    G.dA[2] = dx[0] * dx[1];
-    G.dV = dx[0] * dx[1] * dx[2];
-    // MeshOperator::linearCellDimension: std::min_element of the three lengths
-    double L = dx[0];
-    if (dx[1] < L) L = dx[1];
-    if (dx[2] < L) L = dx[2];
-    G.Lmin = L;
-    unit_vectors(G.nhat);
+    b->G = make_geom(*d);
 
     auto fail = [&](int code) { mara_block_destroy(b); return code; };
     if (!hip_ok(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking))) return fail(MARA_ERR_HIP);
@@ -473,10 +370,12 @@ int mara_block_create(const mara_desc* d, mara_block** out)
     }
     if (b->S.ct && !hip_ok(hipMalloc(&b->Gct, 9 * (size_t)b->S.ffs * sizeof(double)))) return fail(MARA_ERR_HIP);
     if (!hip_ok(hipMalloc(&b->ctl, sizeof(Ctl)))) return fail(MARA_ERR_HIP);
+    if (!hip_ok(hipMalloc(&b->cbins, CBINS * CBIN_STRIDE * sizeof(unsigned long long)))) return fail(MARA_ERR_HIP);
     if (!hip_ok(hipHostMalloc((void**)&b->ctl_host, sizeof(Ctl), hipHostMallocDefault))) return fail(MARA_ERR_HIP);
     std::memset(b->ctl_host, 0, sizeof(Ctl));
     b->ctl_host->active = 1;
     b->ctl_host->first_fail = ~0ull;
+    b->ctl_host->cbins = b->cbins;
     if (!hip_ok(hipMemcpy(b->ctl, b->ctl_host, sizeof(Ctl), hipMemcpyHostToDevice))) return fail(MARA_ERR_HIP);
     // guard zones and never-written faces start at zero
     if (!hip_ok(hipMemset(b->Pa, 0, cellbytes))) return fail(MARA_ERR_HIP);
@@ -497,6 +396,7 @@ int mara_block_destroy(mara_block* b)
     for (int a = 0; a < 3; ++a) hipFree(b->F[a]);
     hipFree(b->Gct);
     hipFree(b->ctl);
+    hipFree(b->cbins);
     if (b->ctl_host) hipHostFree(b->ctl_host);
     if (b->ev0) hipEventDestroy(b->ev0);
     if (b->ev1) hipEventDestroy(b->ev1);

@@ -1,0 +1,555 @@
+// batch.hip -- host side of the batch (serving) entry points of
+// include/mara_gpu.h: a pool of independent solves in one device allocation,
+// advanced together, one launch per kernel type per RK stage.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <new>
+#include <tuple>
+#include <vector>
+
+#include "../../include/mara_gpu.h"
+#include "batch.hpp"
+#include "host_common.hpp"
+
+using namespace mara;
+using namespace mara::host;
+
+namespace {
+
+#define HIP_TRY(call) do { if (!hip_ok(call)) return MARA_ERR_HIP; } while (0)
+
+// first-fit allocator over one device allocation
+struct Pool
+{
+    char* base = nullptr;
+    long long size = 0;
+    std::map<long long, long long> free_;  // offset -> size
+
+    void init(char* b, long long n) { base = b; size = n; free_.clear(); free_[0] = n; }
+    long long available() const
+    {
+        long long s = 0;
+        for (auto& f : free_) s += f.second;
+        return s;
+    }
+    long long alloc(long long n)
+    {
+        n = round_up(n, 256);
+        for (auto it = free_.begin(); it != free_.end(); ++it)
+        {
+            if (it->second >= n)
+            {
+                const long long off = it->first, rest = it->second - n;
+                free_.erase(it);
+                if (rest > 0) free_[off + n] = rest;
+                return off;
+            }
+        }
+        return -1;
+    }
+    void release(long long off, long long n)
+    {
+        n = round_up(n, 256);
+        auto it = free_.emplace(off, n).first;
+        auto next = std::next(it);
+        if (next != free_.end() && it->first + it->second == next->first)
+        {
+            it->second += next->second;
+            free_.erase(next);
+        }
+        if (it != free_.begin())
+        {
+            auto prev = std::prev(it);
+            if (prev->first + prev->second == it->first)
+            {
+                prev->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+};
+
+struct Slot
+{
+    bool used = false;
+    bool fresh = false;
+    mara_desc d;
+    int cls = -1;
+    long long off = 0, bytes = 0;
+    long long ncell = 0;
+};
+
+// solve class: everything that selects a kernel template
+struct ClassKey
+{
+    int law, recon, riemann, ct;
+    bool operator<(const ClassKey& o) const
+    {
+        return std::tie(law, recon, riemann, ct) < std::tie(o.law, o.recon, o.riemann, o.ct);
+    }
+};
+
+struct Segment { int offset, count; };
+
+} // namespace
+
+struct mara_batch
+{
+    int rk_order = 2;
+    int precision = MARA_PREC_FP64;
+    int max_solves = 0;
+    hipStream_t stream = nullptr;
+    Pool pool;
+    char* pool_mem = nullptr;
+    BItem* d_items = nullptr;
+    Ctl* d_ctl = nullptr;
+    unsigned long long* d_bins = nullptr;
+    BTile* d_tiles = nullptr;
+    int tile_capacity = 0;
+    std::vector<BItem> items;
+    std::vector<Slot> slots;
+    Ctl* h_ctl = nullptr;  // pinned mirror
+    std::vector<ClassKey> classes;
+    bool dirty = true;
+    int n_items = 0;  // highest used slot + 1
+    // launch table
+    std::vector<BTile> tiles;
+    std::map<int, Segment> flux[3], ct[3];  // class -> segment
+    std::map<int, Segment> update;
+    Segment bc[3] = {{0, 0}, {0, 0}, {0, 0}};
+};
+
+namespace {
+
+int class_of(mara_batch* B, const mara_desc& d)
+{
+    const ClassKey k{d.law, d.recon, d.riemann, (d.law != MARA_LAW_HYDRO && d.field_ct) ? 1 : 0};
+    for (size_t c = 0; c < B->classes.size(); ++c)
+        if (!(B->classes[c] < k) && !(k < B->classes[c])) return (int)c;
+    B->classes.push_back(k);
+    return (int)B->classes.size() - 1;
+}
+
+long long item_bytes(const mara_desc& d, int precision)
+{
+    int N[3];
+    for (int a = 0; a < 3; ++a) N[a] = d.cells[a] + 2 * d.guards[a];
+    const int nq = law_nq(d.law);
+    const long long fs = round_up((long long)N[0] * N[1] * N[2], 64);
+    const long long ffs = round_up((long long)(N[0] + 1) * (N[1] + 1) * (N[2] + 1), 64);
+    const int real = precision == MARA_PREC_MIXED ? 4 : 8;
+    long long b = 3 * round_up(nq * fs * 8, 256);
+    for (int a = 0; a < 3; ++a) if (N[a] > 1) b += round_up(nq * ffs * real, 256);
+    if (d.law != MARA_LAW_HYDRO && d.field_ct) b += round_up(9 * ffs * 8, 256);
+    return b;
+}
+
+// tiles covering the box [lo, hi) of one item
+void box_tiles(std::vector<BTile>& out, int item, const int lo[3], const int hi[3])
+{
+    const int mx = hi[0] - lo[0], my = hi[1] - lo[1];
+    for (int k = lo[2]; k < hi[2]; ++k)
+    {
+        if (my == 1)
+        {
+            for (int x = 0; x < mx; x += 256)
+                out.push_back(BTile{item, lo[0] + x, lo[1], k, std::min(256, mx - x), 1, 0, 0});
+        }
+        else
+        {
+            for (int y = 0; y < my; y += 4)
+                for (int x = 0; x < mx; x += 64)
+                    out.push_back(BTile{item, lo[0] + x, lo[1] + y, k, std::min(64, mx - x), std::min(4, my - y), 1, 0});
+        }
+    }
+}
+
+void build_table(mara_batch* B)
+{
+    B->tiles.clear();
+    for (int a = 0; a < 3; ++a) { B->flux[a].clear(); B->ct[a].clear(); }
+    B->update.clear();
+    auto seg_begin = [&]() { return (int)B->tiles.size(); };
+    for (size_t c = 0; c < B->classes.size(); ++c)
+    {
+        for (int axis = 0; axis < 3; ++axis)
+        {
+            const int s0 = seg_begin();
+            for (int n = 0; n < B->n_items; ++n)
+            {
+                const Slot& sl = B->slots[n];
+                if (!sl.used || sl.cls != (int)c) continue;
+                const Geom& G = B->items[n].G;
+                if (G.N[axis] < 2) continue;
+                const bool ctring = B->classes[c].ct;
+                int lo[3], hi[3];
+                for (int e = 0; e < 3; ++e)
+                {
+                    if (e == axis) { lo[e] = G.g[e]; hi[e] = G.N[e] - G.g[e] + 1; }
+                    else
+                    {
+                        const int ring = (ctring && G.N[e] > 1) ? 1 : 0;
+                        lo[e] = G.g[e] - ring;
+                        hi[e] = G.N[e] - G.g[e] + ring;
+                    }
+                }
+                box_tiles(B->tiles, n, lo, hi);
+            }
+            if ((int)B->tiles.size() > s0) B->flux[axis][(int)c] = Segment{s0, (int)B->tiles.size() - s0};
+        }
+        if (B->classes[c].ct)
+        {
+            for (int axis = 0; axis < 3; ++axis)
+            {
+                const int s0 = seg_begin();
+                for (int n = 0; n < B->n_items; ++n)
+                {
+                    const Slot& sl = B->slots[n];
+                    if (!sl.used || sl.cls != (int)c) continue;
+                    const Geom& G = B->items[n].G;
+                    if (G.N[axis] < 2) continue;
+                    int lo[3], hi[3];
+                    for (int e = 0; e < 3; ++e)
+                    {
+                        lo[e] = G.g[e];
+                        hi[e] = G.g[e] + G.n[e] + (e == axis ? 1 : 0);
+                    }
+                    box_tiles(B->tiles, n, lo, hi);
+                }
+                if ((int)B->tiles.size() > s0) B->ct[axis][(int)c] = Segment{s0, (int)B->tiles.size() - s0};
+            }
+        }
+        const int s0 = seg_begin();
+        for (int n = 0; n < B->n_items; ++n)
+        {
+            const Slot& sl = B->slots[n];
+            if (!sl.used || sl.cls != (int)c) continue;
+            const Geom& G = B->items[n].G;
+            int lo[3], hi[3];
+            for (int e = 0; e < 3; ++e) { lo[e] = G.g[e]; hi[e] = G.g[e] + G.n[e]; }
+            box_tiles(B->tiles, n, lo, hi);
+        }
+        if ((int)B->tiles.size() > s0) B->update[(int)c] = Segment{s0, (int)B->tiles.size() - s0};
+    }
+    for (int axis = 0; axis < 3; ++axis)
+    {
+        const int s0 = seg_begin();
+        for (int n = 0; n < B->n_items; ++n)
+        {
+            if (!B->slots[n].used) continue;
+            const Geom& G = B->items[n].G;
+            if (G.g[axis] <= 0) continue;
+            const int t1 = axis == 0 ? 1 : 0, t2 = axis == 2 ? 1 : 2;
+            const long long cnt = 2ll * G.g[axis] * G.N[t1] * G.N[t2];
+            for (long long x = 0; x < cnt; x += 256)
+                B->tiles.push_back(BTile{n, (int)x, 0, 0, (int)std::min<long long>(256, cnt - x), 1, 0, 0});
+        }
+        B->bc[axis] = Segment{s0, (int)B->tiles.size() - s0};
+    }
+}
+
+// ---- template dispatch over a solve class -------------------------------------
+template <int Law, int Recon, int Riem, typename R>
+void flux_launch(mara_batch* B, const Segment& s, int axis, int rk)
+{
+    kb_flux<Law, Recon, Riem, R><<<s.count, 256, 0, B->stream>>>(B->d_items, B->d_tiles + s.offset, axis, rk);
+}
+
+template <typename R>
+void flux_dispatch(mara_batch* B, const ClassKey& k, const Segment& s, int axis, int rk)
+{
+#define FL(L, Rc, Rm) if (k.law == L && k.recon == Rc && k.riemann == Rm) { flux_launch<L, Rc, Rm, R>(B, s, axis, rk); return; }
+    FL(0, 0, 0) FL(0, 0, 1) FL(0, 1, 0) FL(0, 1, 1) FL(0, 2, 0) FL(0, 2, 1)
+    FL(1, 0, 0) FL(1, 1, 0) FL(1, 2, 0)
+    FL(2, 0, 0) FL(2, 1, 0) FL(2, 2, 0)
+#undef FL
+}
+
+template <typename R>
+void update_dispatch(mara_batch* B, const ClassKey& k, const Segment& s, double b, int first, int last, int cout)
+{
+    const BTile* t = B->d_tiles + s.offset;
+#define UP(L, C) if (k.law == L && k.ct == C) { kb_update<L, R, (C != 0)><<<s.count, 256, 0, B->stream>>>(B->d_items, t, b, first, last, cout); return; }
+    UP(0, 0) UP(1, 0) UP(1, 1) UP(2, 0) UP(2, 1)
+#undef UP
+}
+
+template <typename R>
+void run_stage(mara_batch* B, int rk)
+{
+    static const double b1[1] = {1.0};
+    static const double b2[2] = {1.0, 0.5};
+    static const double b3[3] = {1.0, 1. / 4, 2. / 3};
+    const double* bb = B->rk_order == 1 ? b1 : B->rk_order == 2 ? b2 : b3;
+    const int last = rk == B->rk_order - 1;
+    for (int axis = 0; axis < 3; ++axis)
+        for (auto& e : B->flux[axis]) flux_dispatch<R>(B, B->classes[e.first], e.second, axis, rk);
+    for (int axis = 0; axis < 3; ++axis)
+        for (auto& e : B->ct[axis])
+            kb_ct<R><<<e.second.count, 256, 0, B->stream>>>(B->d_items, B->d_tiles + e.second.offset, axis);
+    for (auto& e : B->update) update_dispatch<R>(B, B->classes[e.first], e.second, bb[rk], rk == 0, last, last);
+    for (int axis = 0; axis < 3; ++axis)
+        if (B->bc[axis].count)
+            kb_bc<<<B->bc[axis].count, 256, 0, B->stream>>>(B->d_items, B->d_tiles + B->bc[axis].offset, axis, last);
+}
+
+__global__ void kb_reset_bins(Ctl* ctl) { reset_bins(ctl); }
+
+int upload_table(mara_batch* B)
+{
+    if ((int)B->tiles.size() > B->tile_capacity)
+    {
+        if (B->d_tiles) HIP_TRY(hipFree(B->d_tiles));
+        B->tile_capacity = (int)std::max<size_t>(B->tiles.size() * 2, 1024);
+        HIP_TRY(hipMalloc(&B->d_tiles, B->tile_capacity * sizeof(BTile)));
+    }
+    if (!B->tiles.empty())
+        HIP_TRY(hipMemcpyAsync(B->d_tiles, B->tiles.data(), B->tiles.size() * sizeof(BTile), hipMemcpyHostToDevice, B->stream));
+    if (B->n_items)
+        HIP_TRY(hipMemcpyAsync(B->d_items, B->items.data(), B->n_items * sizeof(BItem), hipMemcpyHostToDevice, B->stream));
+    return MARA_OK;
+}
+
+// a solve stops on the conditions of maraMainLoop (t >= t_final), at its
+// step limit, or at the first failure
+bool finished(const Ctl& c)
+{
+    return !c.active || !(c.t < c.t_final) || c.steps >= c.max_steps || ctl_status(c) != MARA_OK;
+}
+
+int validate_item(mara_batch* B, const mara_desc* d)
+{
+    const int v = validate(d);
+    if (v) return v;
+    if (d->rk_order != B->rk_order || d->precision != B->precision) return MARA_ERR_UNSUPPORTED;
+    return MARA_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int mara_batch_create(long long hbm_budget, int rk_order, int precision, int max_solves, mara_batch** out)
+{
+    if (!out || hbm_budget <= 0 || rk_order < 1 || rk_order > 3 || precision < 0 || precision > 1 || max_solves < 1)
+        return MARA_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MARA_ERR_NO_DEVICE;
+    mara_batch* B = new (std::nothrow) mara_batch();
+    if (!B) return MARA_ERR_HIP;
+    B->rk_order = rk_order;
+    B->precision = precision;
+    B->max_solves = max_solves;
+    auto fail = [&](int code) { mara_batch_destroy(B); return code; };
+    if (!hip_ok(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking))) return fail(MARA_ERR_HIP);
+    if (!hip_ok(hipMalloc(&B->pool_mem, hbm_budget))) return fail(MARA_ERR_CAPACITY);
+    B->pool.init(B->pool_mem, hbm_budget);
+    if (!hip_ok(hipMalloc(&B->d_items, max_solves * sizeof(BItem)))) return fail(MARA_ERR_HIP);
+    if (!hip_ok(hipMalloc(&B->d_ctl, max_solves * sizeof(Ctl)))) return fail(MARA_ERR_HIP);
+    if (!hip_ok(hipMalloc(&B->d_bins, (size_t)max_solves * CBINS * CBIN_STRIDE * sizeof(unsigned long long)))) return fail(MARA_ERR_HIP);
+    if (!hip_ok(hipHostMalloc((void**)&B->h_ctl, max_solves * sizeof(Ctl), hipHostMallocDefault))) return fail(MARA_ERR_HIP);
+    std::memset(B->h_ctl, 0, max_solves * sizeof(Ctl));
+    B->items.resize(max_solves);
+    B->slots.resize(max_solves);
+    *out = B;
+    return MARA_OK;
+}
+
+int mara_batch_destroy(mara_batch* B)
+{
+    if (!B) return MARA_ERR_INVALID_ARGUMENT;
+    if (B->stream) (void)hipStreamSynchronize(B->stream);
+    (void)hipFree(B->pool_mem);
+    (void)hipFree(B->d_items);
+    (void)hipFree(B->d_ctl);
+    (void)hipFree(B->d_bins);
+    (void)hipFree(B->d_tiles);
+    if (B->h_ctl) (void)hipHostFree(B->h_ctl);
+    if (B->stream) (void)hipStreamDestroy(B->stream);
+    delete B;
+    return MARA_OK;
+}
+
+long long mara_batch_free_bytes(const mara_batch* B)
+{
+    return B ? B->pool.available() : -1;
+}
+
+long long mara_batch_item_bytes(const mara_desc* d, int precision)
+{
+    if (!d) return -1;
+    return item_bytes(*d, precision);
+}
+
+int mara_batch_admit(mara_batch* B, const mara_desc* d, const double* P, double cfl, double t_final,
+                     int max_steps, int* slot)
+{
+    if (!B || !P || !slot) return MARA_ERR_INVALID_ARGUMENT;
+    const int v = validate_item(B, d);
+    if (v) return v;
+    int n = -1;
+    for (int s = 0; s < B->max_solves; ++s)
+        if (!B->slots[s].used) { n = s; break; }
+    if (n < 0) return MARA_ERR_CAPACITY;
+    const long long bytes = item_bytes(*d, B->precision);
+    const long long off = B->pool.alloc(bytes);
+    if (off < 0) return MARA_ERR_CAPACITY;
+
+    Slot& sl = B->slots[n];
+    sl.used = true;
+    sl.fresh = true;
+    sl.d = *d;
+    sl.off = off;
+    sl.bytes = bytes;
+    sl.cls = class_of(B, *d);
+    BItem& it = B->items[n];
+    std::memset(&it, 0, sizeof it);
+    it.G = make_geom(*d);
+    sl.ncell = (long long)it.G.N[0] * it.G.N[1] * it.G.N[2];
+    char* p = B->pool.base + off;
+    const long long cb = round_up(it.G.nq * it.G.fs * 8, 256);
+    it.Pa = (double*)p; p += cb;
+    it.Pb = (double*)p; p += cb;
+    it.U = (double*)p; p += cb;
+    const int real = B->precision == MARA_PREC_MIXED ? 4 : 8;
+    for (int a = 0; a < 3; ++a)
+        if (it.G.N[a] > 1) { it.F[a] = p; p += round_up(it.G.nq * it.G.ffs * real, 256); }
+    if (d->law != MARA_LAW_HYDRO && d->field_ct) { it.Gct = (double*)p; p += round_up(9 * it.G.ffs * 8, 256); }
+    it.ctl = B->d_ctl + n;
+    it.bc = d->bc;
+
+    Ctl& c = B->h_ctl[n];
+    std::memset(&c, 0, sizeof c);
+    c.t = 0.0;
+    c.t_final = t_final;
+    c.cfl = cfl;
+    c.max_steps = max_steps;
+    c.active = 1;
+    c.first_fail = ~0ull;
+    c.courant_bits = DBL_MAX_BITS;
+    c.cbins = B->d_bins + (size_t)n * CBINS * CBIN_STRIDE;
+    HIP_TRY(hipMemcpyAsync(B->d_ctl + n, &c, sizeof(Ctl), hipMemcpyHostToDevice, B->stream));
+    kb_reset_bins<<<1, 1, 0, B->stream>>>(B->d_ctl + n);
+    // primitives: host -> staging (U) -> Pa, guards included; Pb starts as a copy
+    HIP_TRY(hipMemcpyAsync(it.U, P, sl.ncell * it.G.nq * sizeof(double), hipMemcpyHostToDevice, B->stream));
+    k_aos_to_soa<<<(unsigned)((sl.ncell + 255) / 256), 256, 0, B->stream>>>(it.U, it.Pa, it.G);
+    HIP_TRY(hipMemcpyAsync(it.Pb, it.Pa, cb, hipMemcpyDeviceToDevice, B->stream));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(B->stream));  // the host buffer may go away after return
+    if (n + 1 > B->n_items) B->n_items = n + 1;
+    B->dirty = true;
+    *slot = n;
+    return MARA_OK;
+}
+
+int mara_batch_step(mara_batch* B, int n_steps, int* n_active)
+{
+    if (!B || n_steps < 0) return MARA_ERR_INVALID_ARGUMENT;
+    if (B->dirty)
+    {
+        build_table(B);
+        int e = upload_table(B);
+        if (e) return e;
+        B->dirty = false;
+    }
+    // first CFL of newly admitted solves, grouped by law
+    std::vector<BTile> fresh[3];
+    for (int n = 0; n < B->n_items; ++n)
+    {
+        Slot& sl = B->slots[n];
+        if (!sl.used || !sl.fresh) continue;
+        const Geom& G = B->items[n].G;
+        int lo[3], hi[3];
+        for (int e = 0; e < 3; ++e) { lo[e] = G.g[e]; hi[e] = G.g[e] + G.n[e]; }
+        box_tiles(fresh[sl.d.law], n, lo, hi);
+        sl.fresh = false;
+    }
+    std::vector<BTile> ftiles;
+    int fofs[4] = {0, 0, 0, 0};
+    for (int l = 0; l < 3; ++l)
+    {
+        ftiles.insert(ftiles.end(), fresh[l].begin(), fresh[l].end());
+        fofs[l + 1] = (int)ftiles.size();
+    }
+    BTile* d_ftiles = nullptr;
+    if (!ftiles.empty())
+    {
+        HIP_TRY(hipMallocAsync((void**)&d_ftiles, ftiles.size() * sizeof(BTile), B->stream));
+        HIP_TRY(hipMemcpyAsync(d_ftiles, ftiles.data(), ftiles.size() * sizeof(BTile), hipMemcpyHostToDevice, B->stream));
+        if (fofs[1] > fofs[0]) kb_courant<0><<<fofs[1] - fofs[0], 256, 0, B->stream>>>(B->d_items, d_ftiles + fofs[0]);
+        if (fofs[2] > fofs[1]) kb_courant<1><<<fofs[2] - fofs[1], 256, 0, B->stream>>>(B->d_items, d_ftiles + fofs[1]);
+        if (fofs[3] > fofs[2]) kb_courant<2><<<fofs[3] - fofs[2], 256, 0, B->stream>>>(B->d_items, d_ftiles + fofs[2]);
+    }
+    const int n = B->n_items;
+    const unsigned ig = (unsigned)((n + 63) / 64);
+    for (int s = 0; s < n_steps && n > 0; ++s)
+    {
+        kb_step_begin<<<ig, 64, 0, B->stream>>>(B->d_items, n);
+        kb_step_dt<<<ig, 64, 0, B->stream>>>(B->d_items, n);
+        for (int rk = 0; rk < B->rk_order; ++rk)
+        {
+            if (B->precision == MARA_PREC_MIXED) run_stage<float>(B, rk);
+            else run_stage<double>(B, rk);
+        }
+        kb_step_end<<<ig, 64, 0, B->stream>>>(B->d_items, n);
+    }
+    HIP_TRY(hipGetLastError());
+    if (n) HIP_TRY(hipMemcpyAsync(B->h_ctl, B->d_ctl, n * sizeof(Ctl), hipMemcpyDeviceToHost, B->stream));
+    if (d_ftiles) HIP_TRY(hipFreeAsync(d_ftiles, B->stream));
+    HIP_TRY(hipStreamSynchronize(B->stream));
+    int active = 0;
+    for (int k = 0; k < n; ++k)
+    {
+        const Ctl& c = B->h_ctl[k];
+        if (!B->slots[k].used) continue;
+        if (!finished(c)) ++active;
+    }
+    if (n_active) *n_active = active;
+    return MARA_OK;
+}
+
+int mara_batch_query(mara_batch* B, int slot, int* done, double* t, int* steps, mara_report* r)
+{
+    if (!B || slot < 0 || slot >= B->max_solves || !B->slots[slot].used) return MARA_ERR_INVALID_ARGUMENT;
+    const Ctl& c = B->h_ctl[slot];
+    if (done) *done = finished(c) ? 1 : 0;
+    if (t) *t = c.t;
+    if (steps) *steps = c.steps;
+    return fill_report(c, B->items[slot].G, r);
+}
+
+int mara_batch_finished(const mara_batch* B, int* slots, int capacity, int* count)
+{
+    if (!B || !count || capacity < 0 || (capacity > 0 && !slots)) return MARA_ERR_INVALID_ARGUMENT;
+    int m = 0;
+    for (int n = 0; n < B->n_items && m < capacity; ++n)
+        if (B->slots[n].used && finished(B->h_ctl[n])) slots[m++] = n;
+    *count = m;
+    return MARA_OK;
+}
+
+int mara_batch_retire(mara_batch* B, int slot, double* P)
+{
+    if (!B || slot < 0 || slot >= B->max_solves || !B->slots[slot].used) return MARA_ERR_INVALID_ARGUMENT;
+    Slot& sl = B->slots[slot];
+    BItem& it = B->items[slot];
+    if (P)
+    {
+        k_soa_to_aos<<<(unsigned)((sl.ncell + 255) / 256), 256, 0, B->stream>>>(it.Pa, it.U, it.G);
+        HIP_TRY(hipMemcpyAsync(P, it.U, sl.ncell * it.G.nq * sizeof(double), hipMemcpyDeviceToHost, B->stream));
+    }
+    // no kernel of this slot is queued any more: mara_batch_step synchronises
+    HIP_TRY(hipStreamSynchronize(B->stream));
+    B->pool.release(sl.off, sl.bytes);
+    sl.used = false;
+    std::memset(&B->h_ctl[slot], 0, sizeof(Ctl));
+    while (B->n_items > 0 && !B->slots[B->n_items - 1].used) --B->n_items;
+    B->dirty = true;
+    return MARA_OK;
+}
+
+} // extern "C"

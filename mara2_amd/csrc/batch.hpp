@@ -1,0 +1,154 @@
+// batch.hpp -- kernels that advance many independent solves in one launch per
+// kernel type per RK stage (the serving path).
+//
+// Each solve ("item") keeps the device layout and the per-zone arithmetic of
+// a single block (kernels.hpp): the batched kernels only replace the launch
+// geometry.  A work-group takes one tile from a table built on the host; a
+// tile names its item and a patch of that item's face, zone or guard range:
+// 256 x 1 zones for 1-D items and for guard ranges, 64 x 4 for 2-D and 3-D
+// items.  Every solve has its own control block (dt, clock, stop flag,
+// failure counters, CFL bins), so solves never exchange data and a solve's
+// results are bitwise the same alone or in any mix.
+#pragma once
+
+#include "kernels.hpp"
+
+namespace mara {
+
+struct BItem
+{
+    Geom G;
+    double* Pa;
+    double* Pb;
+    double* U;
+    void* F[3];
+    double* Gct;
+    Ctl* ctl;
+    int bc;
+    int pad[3];
+};
+
+struct BTile
+{
+    int item;
+    int x0, y0, z0;  // absolute index of the tile's first thread
+    int nx, ny;      // valid extent of the tile
+    int mode;        // 0: 256 x 1, 1: 64 x 4
+    int pad;
+};
+
+__device__ __forceinline__ bool tile_thread(const BTile& T, int& i, int& j, int& k)
+{
+    const int lin = threadIdx.x;
+    const int dx = T.mode ? (lin & 63) : lin;
+    const int dy = T.mode ? (lin >> 6) : 0;
+    i = T.x0 + dx;
+    j = T.y0 + dy;
+    k = T.z0;
+    return dx < T.nx && dy < T.ny;
+}
+
+template <int Law, int Recon, int Riem, typename R>
+__global__ void __launch_bounds__(256)
+kb_flux(const BItem* __restrict__ items, const BTile* __restrict__ tiles, int axis, int rk)
+{
+    const BTile T = tiles[blockIdx.x];
+    const BItem& it = items[T.item];
+    if (!it.ctl->active) return;
+    int i, j, k;
+    if (!tile_thread(T, i, j, k)) return;
+    const double* Pin = rk == 0 ? it.Pa : it.Pb;
+    flux_face<Law, Recon, Riem, R>(Pin, (R*)it.F[axis], it.G, axis, i, j, k, it.ctl);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256)
+kb_ct(const BItem* __restrict__ items, const BTile* __restrict__ tiles, int axis)
+{
+    const BTile T = tiles[blockIdx.x];
+    const BItem& it = items[T.item];
+    if (!it.ctl->active) return;
+    int i, j, k;
+    if (!tile_thread(T, i, j, k)) return;
+    ct_face<R>((const R*)it.F[0], (const R*)it.F[1], (const R*)it.F[2], it.Gct, it.G, axis, i, j, k);
+}
+
+template <int Law, typename R, bool CT>
+__global__ void __launch_bounds__(256)
+kb_update(const BItem* __restrict__ items, const BTile* __restrict__ tiles, double b, int first, int last,
+          int courant_out)
+{
+    __shared__ double wmin[4];
+    const BTile T = tiles[blockIdx.x];
+    const BItem& it = items[T.item];
+    if (!it.ctl->active) return;
+    int i, j, k;
+    double dtc = 1.79769313486231570815e+308;
+    if (tile_thread(T, i, j, k))
+        dtc = update_zone<Law, R, CT>(it.Pa, it.U, last ? it.Pa : it.Pb, (const R*)it.F[0], (const R*)it.F[1],
+                                      (const R*)it.F[2], it.Gct, it.G, b, first, !last, courant_out, it.ctl, i, j, k);
+    if (courant_out) courant_to_bins(dtc, wmin, 4, threadIdx.x, it.ctl);
+}
+
+__global__ void __launch_bounds__(256)
+kb_bc(const BItem* __restrict__ items, const BTile* __restrict__ tiles, int axis, int last)
+{
+    const BTile T = tiles[blockIdx.x];
+    const BItem& it = items[T.item];
+    if (!it.ctl->active) return;
+    const int t = T.x0 + threadIdx.x;
+    if ((int)threadIdx.x >= T.nx) return;
+    bc_guard(last ? it.Pa : it.Pb, it.G, axis, it.bc, t);
+}
+
+template <int Law>
+__global__ void __launch_bounds__(256)
+kb_courant(const BItem* __restrict__ items, const BTile* __restrict__ tiles)
+{
+    __shared__ double wmin[4];
+    const BTile T = tiles[blockIdx.x];
+    const BItem& it = items[T.item];
+    if (!it.ctl->active) return;
+    int i, j, k;
+    double dtc = 1.79769313486231570815e+308;
+    int err = 0;
+    if (tile_thread(T, i, j, k)) dtc = zone_courant<Law>(it.Pa, it.G, cell_index(it.G, i, j, k), err);
+    if (err & ERR_SUPERLUMINAL) atomicAdd(&it.ctl->superluminal, 1ull);
+    if (err & ERR_EIGEN) atomicAdd(&it.ctl->eigen_fail, 1ull);
+    courant_to_bins(dtc, wmin, 4, threadIdx.x, it.ctl);
+}
+
+// per-item control, one thread per item
+__global__ void kb_step_begin(const BItem* __restrict__ items, int n)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    Ctl* ctl = items[t].ctl;
+    const bool go = ctl->active && ctl->t < ctl->t_final && ctl->steps < ctl->max_steps
+                    && ctl->c2p_fail == 0 && ctl->eigen_fail == 0 && ctl->superluminal == 0;
+    ctl->active = go ? 1 : 0;
+}
+
+__global__ void kb_step_dt(const BItem* __restrict__ items, int n)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    Ctl* ctl = items[t].ctl;
+    if (!ctl->active) return;
+    fold_bins(ctl);
+    const double dt1 = ctl->cfl * bitsd(ctl->courant_bits);
+    const double dt2 = ctl->t_final - ctl->t;
+    ctl->dt = dt1 < dt2 ? dt1 : dt2;
+}
+
+__global__ void kb_step_end(const BItem* __restrict__ items, int n)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    Ctl* ctl = items[t].ctl;
+    if (!ctl->active) return;
+    ctl->t += ctl->dt;
+    ctl->steps += 1;
+}
+
+} // namespace mara

@@ -41,7 +41,7 @@ struct Ctl
     int steps;
     int max_steps;
     int pad;
-    unsigned long long cbins[CBINS * CBIN_STRIDE];
+    unsigned long long* cbins;  // CBINS * CBIN_STRIDE accumulators, own allocation
 };
 
 constexpr unsigned long long DBL_MAX_BITS = 0x7fefffffffffffffull;
@@ -165,7 +165,7 @@ __device__ __forceinline__ void flux_face(const double* __restrict__ P, R* __res
 }
 
 template <int Law, int Recon, int Riem, typename R>
-__global__ void __launch_bounds__(256)
+static __global__ void __launch_bounds__(256)
 k_flux(const double* __restrict__ P, R* __restrict__ F, const Geom G, int axis,
        int lo0, int lo1, int lo2, int m0, int m1, int m2, Ctl* ctl)
 {
@@ -215,7 +215,7 @@ __device__ __forceinline__ void ct_face(const R* __restrict__ Fx, const R* __res
 }
 
 template <typename R>
-__global__ void __launch_bounds__(256)
+static __global__ void __launch_bounds__(256)
 k_ct(const R* __restrict__ Fx, const R* __restrict__ Fy, const R* __restrict__ Fz,
      double* __restrict__ Gct, Geom G, int a, int lo0, int lo1, int lo2, int m0, int m1, int m2,
      const Ctl* __restrict__ ctl)
@@ -365,7 +365,7 @@ __device__ __forceinline__ double update_zone(const double* __restrict__ P0, dou
 }
 
 template <int Law, typename R, bool CT>
-__global__ void __launch_bounds__(256)
+static __global__ void __launch_bounds__(256)
 k_update(const double* __restrict__ P0, double* __restrict__ U, double* __restrict__ Pout,
          const R* __restrict__ Fx, const R* __restrict__ Fy, const R* __restrict__ Fz,
          const double* __restrict__ Gct, Geom G, double b, int first_stage, int write_u,
@@ -434,7 +434,7 @@ __device__ __forceinline__ long long bc_count(const Geom& G, int axis)
     return 2ll * G.g[axis] * G.N[t1] * G.N[t2];
 }
 
-__global__ void __launch_bounds__(256)
+static __global__ void __launch_bounds__(256)
 k_bc(double* __restrict__ P, Geom G, int axis, int kind, const Ctl* __restrict__ ctl)
 {
     if (!ctl->active) return;
@@ -450,7 +450,7 @@ k_bc(double* __restrict__ P, Geom G, int axis, int kind, const Ctl* __restrict__
 // Grid-stride over interior zones; wavefront then work-group minimum, one
 // atomic per work-group.
 template <int Law>
-__global__ void __launch_bounds__(256)
+static __global__ void __launch_bounds__(256)
 k_courant(const double* __restrict__ P, Geom G, Ctl* ctl)
 {
     __shared__ double wmin[4];
@@ -511,7 +511,7 @@ __device__ __forceinline__ double vertex_monopole(const double* __restrict__ P, 
     return 0.25 * ((Bx1 - Bx0) + (By1 - By0) + (Bz1 - Bz0)) / 1.0;
 }
 
-__global__ void __launch_bounds__(256)
+static __global__ void __launch_bounds__(256)
 k_monopole(const double* __restrict__ P, Geom G, Ctl* ctl)
 {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -536,7 +536,7 @@ k_monopole(const double* __restrict__ P, Geom G, Ctl* ctl)
 // ---------------------------------------------------------------------------
 // Layout transposes between MeshData::P (row-major i,j,k,q) and device SoA.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256)
+static __global__ void __launch_bounds__(256)
 k_aos_to_soa(const double* __restrict__ A, double* __restrict__ P, Geom G)
 {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -550,7 +550,7 @@ k_aos_to_soa(const double* __restrict__ A, double* __restrict__ P, Geom G)
     for (int q = 0; q < G.nq; ++q) P[q * G.fs + c] = A[t * G.nq + q];
 }
 
-__global__ void __launch_bounds__(256)
+static __global__ void __launch_bounds__(256)
 k_soa_to_aos(const double* __restrict__ P, double* __restrict__ A, Geom G)
 {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -566,7 +566,7 @@ k_soa_to_aos(const double* __restrict__ P, double* __restrict__ A, Geom G)
 // ---------------------------------------------------------------------------
 // Control kernels (single thread): host-given dt, and the device-driven loop.
 // ---------------------------------------------------------------------------
-__global__ void k_ctl_begin_advance(Ctl* ctl, double dt)
+static __global__ void k_ctl_begin_advance(Ctl* ctl, double dt)
 {
     ctl->dt = dt;
     ctl->active = 1;
@@ -595,7 +595,7 @@ __device__ __forceinline__ void fold_bins(Ctl* ctl)
     reset_bins(ctl);
 }
 
-__global__ void k_ctl_begin_courant(Ctl* ctl)
+static __global__ void k_ctl_begin_courant(Ctl* ctl)
 {
     ctl->active = 1;
     ctl->courant_bits = DBL_MAX_BITS;
@@ -604,14 +604,14 @@ __global__ void k_ctl_begin_courant(Ctl* ctl)
     reset_bins(ctl);
 }
 
-__global__ void k_ctl_fold_bins(Ctl* ctl) { fold_bins(ctl); }
+static __global__ void k_ctl_fold_bins(Ctl* ctl) { fold_bins(ctl); }
 
-__global__ void k_ctl_begin_monopole(Ctl* ctl)
+static __global__ void k_ctl_begin_monopole(Ctl* ctl)
 {
     ctl->monopole_bits = 0ull;
 }
 
-__global__ void k_ctl_begin_run(Ctl* ctl, double t, double t_final, double cfl, int max_steps)
+static __global__ void k_ctl_begin_run(Ctl* ctl, double t, double t_final, double cfl, int max_steps)
 {
     ctl->t = t;
     ctl->t_final = t_final;
@@ -628,7 +628,7 @@ __global__ void k_ctl_begin_run(Ctl* ctl, double t, double t_final, double cfl, 
 }
 
 // maraMainLoop: while (t < t_final) { dt = min(cfl * courant, t_final - t); ... }
-__global__ void k_ctl_step_begin(Ctl* ctl)
+static __global__ void k_ctl_step_begin(Ctl* ctl)
 {
     const bool go = ctl->active && ctl->t < ctl->t_final && ctl->steps < ctl->max_steps
                     && ctl->c2p_fail == 0 && ctl->eigen_fail == 0 && ctl->superluminal == 0;
@@ -638,7 +638,7 @@ __global__ void k_ctl_step_begin(Ctl* ctl)
 // the CFL of the state at the start of the step sits in the bins (from the
 // standalone reduction on the first step, from the previous step's last
 // stage afterwards)
-__global__ void k_ctl_step_dt(Ctl* ctl)
+static __global__ void k_ctl_step_dt(Ctl* ctl)
 {
     if (!ctl->active) return;
     fold_bins(ctl);
@@ -647,7 +647,7 @@ __global__ void k_ctl_step_dt(Ctl* ctl)
     ctl->dt = dt1 < dt2 ? dt1 : dt2;
 }
 
-__global__ void k_ctl_step_end(Ctl* ctl)
+static __global__ void k_ctl_step_end(Ctl* ctl)
 {
     if (!ctl->active) return;
     ctl->t += ctl->dt;

@@ -20,6 +20,7 @@ from . import abi
 SEED_TURBULENCE = 2026120202
 SEED_MHD = 2026120203
 SEED_RMHD = 2026120204
+SEED_SERVING = 2026120205
 
 
 # ---------------------------------------------------------------------------
@@ -294,4 +295,130 @@ def rmhd_3d(desc, seed=SEED_RMHD, kmax=8, wmax=5.0, sigma=1.0):
     P[..., 4] = 0.1
     P[..., 5:8] = B
     P[inner + (slice(1, 4),)] = v
+    return apply_bc(desc, P)
+
+
+# ---------------------------------------------------------------------------
+# serving stream: independent Riemann problems
+# ---------------------------------------------------------------------------
+def _log_uniform(rng, lo, hi, size=None):
+    return np.exp(rng.uniform(np.log(lo), np.log(hi), size))
+
+
+def riemann_params(rng):
+    """One request of the serving stream: 70% 1-D (zones log-uniform in
+    256..16384), 30% 2-D (side log-uniform in 128..1024, discontinuity on a
+    line through the centre at a random angle); left/right rho and p
+    log-uniform in [0.1, 10], velocities uniform in [-1, 1]; half MHD with a
+    uniform B (plasma beta log-uniform in [0.1, 10]) plus a random jump of
+    the components transverse to the discontinuity.  Every request draws the
+    same number of variates, so the stream does not depend on its mix."""
+    dim = 1 if rng.random() < 0.7 else 2
+    u = rng.random()
+    if dim == 1:
+        n = int(round(np.exp(np.log(256) + u * (np.log(16384) - np.log(256)))))
+    else:
+        n = int(round(np.exp(np.log(128) + u * (np.log(1024) - np.log(128)))))
+    law = "mhd" if rng.random() < 0.5 else "hydro"
+    rho = _log_uniform(rng, 0.1, 10.0, 2)
+    pre = _log_uniform(rng, 0.1, 10.0, 2)
+    vel = rng.uniform(-1.0, 1.0, (2, 3))
+    angle = rng.uniform(0.0, np.pi) if dim == 2 else 0.0
+    if dim == 1:
+        rng.uniform(0.0, np.pi)
+    beta = _log_uniform(rng, 0.1, 10.0)
+    bdir = rng.standard_normal(3)
+    jump = rng.standard_normal(3)
+    jump_size = rng.random()
+    nhat = np.array([np.cos(angle), np.sin(angle), 0.0])
+    bmag = np.sqrt(2.0 * 0.5 * (pre[0] + pre[1]) / beta)
+    B0 = bmag * bdir / np.linalg.norm(bdir)
+    jump = jump - np.dot(jump, nhat) * nhat
+    jump *= jump_size * bmag / max(np.linalg.norm(jump), 1e-300)
+    BL, BR = B0 - 0.5 * jump, B0 + 0.5 * jump
+    return dict(dim=dim, n=n, law=law, rho=rho, p=pre, v=vel, angle=angle, beta=beta,
+                B=np.stack([BL, BR]) if law == "mhd" else None)
+
+
+def riemann_desc(params, precision="mixed", rk_order=2):
+    """PLM (theta 1.5) + HLLC for hydro, HLLE (flux-CT in 2-D) for MHD, gamma
+    1.4, outflow boundaries on the unit interval or square."""
+    cells = (params["n"], 1, 1) if params["dim"] == 1 else (params["n"], params["n"], 1)
+    mhd = params["law"] == "mhd"
+    return abi.make_desc(law=params["law"], gamma=1.4, recon="plm", plm_theta=1.5,
+                         riemann="hlle" if mhd else "hllc", bc="outflow", cells=cells,
+                         rk_order=rk_order, field_ct=1 if (mhd and params["dim"] == 2) else 0,
+                         precision=precision)
+
+
+def _vector_potential_z_to_B(desc, Ax, Ay):
+    """In-plane cell B of a 2-D mesh from A_z sampled on x faces (Ax, shape
+    (N1+1, N2+1)) and y faces (Ay): the z-potential terms of
+    vector_potential_to_B with the same arithmetic, on 2-D arrays."""
+    Fyx = -Ax          # F[..., 1, 0]
+    Fxy = +Ay          # F[..., 0, 1]
+    Gyx = Fyx.copy()
+    Gxy = Fxy.copy()
+    Gyx[1:, 1:-1] = 0.125 * (2 * Fyx[1:, 1:-1] + Fyx[1:, 2:] + Fyx[1:, :-2]
+                             - Fxy[1:, 2:] - Fxy[:-1, 2:]
+                             - Fxy[1:, 1:-1] - Fxy[:-1, 1:-1])
+    Gxy[1:-1, 1:] = 0.125 * (2 * Fxy[1:-1, 1:] + Fxy[2:, 1:] + Fxy[:-2, 1:]
+                             - Fyx[2:, 1:] - Fyx[2:, :-1]
+                             - Fyx[1:-1, 1:] - Fyx[1:-1, :-1])
+    _, dA, dV = spacing(desc)
+    f = 1.0 / dV
+    # components (0, 1) of divergence(G, 1, dA, dV) with G[..., 0, 0] = G[..., 1, 1] = 0
+    Bx = f * (+0.0 * dA[0] - 0.0 * dA[0] + Gxy[:-1, 1:] * dA[1] - Gxy[:-1, :-1] * dA[1] + 0.0 * dA[2] - 0.0 * dA[2])
+    By = f * (+Gyx[1:, :-1] * dA[0] - Gyx[:-1, :-1] * dA[0] + 0.0 * dA[1] - 0.0 * dA[1] + 0.0 * dA[2] - 0.0 * dA[2])
+    return Bx, By
+
+
+def riemann_problem(params, desc):
+    """Initial primitives (guards filled) of one serving request."""
+    P = empty(desc)
+    rho, pre, vel = params["rho"], params["p"], params["v"]
+    if params["dim"] == 1:
+        x = cell_centres(desc, 0)
+        left = (x < 0.5)[:, None, None]
+    else:
+        th = params["angle"]
+        x = cell_centres(desc, 0)[:, None]
+        y = cell_centres(desc, 1)[None, :]
+        left = ((x - 0.5) * np.cos(th) + (y - 0.5) * np.sin(th) < 0.0)[:, :, None]
+    P[..., 0] = np.where(left, rho[0], rho[1])
+    for c in range(3):
+        P[..., 1 + c] = np.where(left, vel[0, c], vel[1, c])
+    P[..., 4] = np.where(left, pre[0], pre[1])
+    if params["law"] == "mhd":
+        BL, BR = params["B"]
+        if params["dim"] == 1:
+            for c in range(3):
+                P[..., 5 + c] = np.where(left, BL[c], BR[c])
+        else:
+            # A_z = Bn u - Bt(s) s with s, u the coordinates along the normal
+            # and the tangent of the discontinuity: curl(A_z z) is B0 with a
+            # jump of the tangential component, divergence-free on the CT stencil
+            th = params["angle"]
+            n_, t_ = np.array([np.cos(th), np.sin(th)]), np.array([-np.sin(th), np.cos(th)])
+            Bn = BL[0] * n_[0] + BL[1] * n_[1]
+            BtL, BtR = BL[0] * t_[0] + BL[1] * t_[1], BR[0] * t_[0] + BR[1] * t_[1]
+            g = desc.guards
+            dx, _, _ = spacing(desc)
+            N0, N1 = desc.cells[0] + 2 * g[0], desc.cells[1] + 2 * g[1]
+            xf = desc.lower[0] + dx[0] * (np.arange(N0 + 1) - g[0]) - 0.5
+            yf = desc.lower[1] + dx[1] * (np.arange(N1 + 1) - g[1]) - 0.5
+            xc = xf + 0.5 * dx[0]
+            yc = yf + 0.5 * dx[1]
+
+            def Az(X, Y):
+                s = X * n_[0] + Y * n_[1]
+                u = X * t_[0] + Y * t_[1]
+                return Bn * u - np.where(s < 0.0, BtL, BtR) * s
+
+            Ax = Az(xf[:, None], yc[None, :])   # x faces: x on the face, y at the centre
+            Ay = Az(xc[:, None], yf[None, :])   # y faces
+            Bx, By = _vector_potential_z_to_B(desc, Ax, Ay)
+            P[..., 0, 5] = Bx
+            P[..., 0, 6] = By
+            P[..., 7] = np.where(left, BL[2], BR[2])
     return apply_bc(desc, P)

@@ -60,6 +60,21 @@ def load():
                                                ctypes.POINTER(ctypes.c_char_p)]
     L.mara_block_device_bytes.argtypes = [blk]
     L.mara_block_device_bytes.restype = ctypes.c_longlong
+    bat = ctypes.c_void_p
+    L.mara_batch_create.argtypes = [ctypes.c_longlong, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.POINTER(bat)]
+    L.mara_batch_destroy.argtypes = [bat]
+    L.mara_batch_item_bytes.argtypes = [ctypes.POINTER(MaraDesc), ctypes.c_int]
+    L.mara_batch_item_bytes.restype = ctypes.c_longlong
+    L.mara_batch_free_bytes.argtypes = [bat]
+    L.mara_batch_free_bytes.restype = ctypes.c_longlong
+    L.mara_batch_admit.argtypes = [bat, ctypes.POINTER(MaraDesc), _dp, ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    L.mara_batch_step.argtypes = [bat, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    L.mara_batch_query.argtypes = [bat, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp,
+                                   ctypes.POINTER(ctypes.c_int), ctypes.POINTER(MaraReport)]
+    L.mara_batch_retire.argtypes = [bat, ctypes.c_int, _dp]
+    L.mara_batch_finished.argtypes = [bat, ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     _LIB = L
     return L
 
@@ -150,6 +165,95 @@ class Block:
         _check(self.lib.mara_block_stage_kernel_time(self._h, ctypes.byref(ms), ctypes.byref(n),
                                                      ctypes.byref(name)), "mara_block_stage_kernel_time")
         return ms.value, n.value, name.value.decode()
+
+
+class Batch:
+    """Many independent solves resident in one device pool, stepped together
+    (mara_batch_* of include/mara_gpu.h)."""
+
+    def __init__(self, hbm_budget, rk_order=2, precision="fp64", max_solves=1024):
+        self.lib = load()
+        self.precision = abi.PRECISIONS[precision] if isinstance(precision, str) else precision
+        self._h = ctypes.c_void_p()
+        self._shapes = {}
+        _check(self.lib.mara_batch_create(int(hbm_budget), rk_order, self.precision, max_solves,
+                                          ctypes.byref(self._h)), "mara_batch_create")
+
+    def close(self):
+        if self._h:
+            self.lib.mara_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def item_bytes(self, desc):
+        return self.lib.mara_batch_item_bytes(ctypes.byref(desc), self.precision)
+
+    @staticmethod
+    def item_bytes_of(desc, precision):
+        """Pool bytes of one solve (no device needed)."""
+        prec = abi.PRECISIONS[precision] if isinstance(precision, str) else precision
+        return load().mara_batch_item_bytes(ctypes.byref(desc), prec)
+
+    @property
+    def free_bytes(self):
+        return self.lib.mara_batch_free_bytes(self._h)
+
+    def admit(self, desc, P, cfl, t_final, max_steps):
+        """Slot of the admitted solve, or None when the pool or the slot table
+        is full (MARA_ERR_CAPACITY)."""
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        assert P.shape == array_shape(desc), (P.shape, array_shape(desc))
+        slot = ctypes.c_int(-1)
+        st = self.lib.mara_batch_admit(self._h, ctypes.byref(desc), P.ctypes.data_as(_dp), cfl, t_final,
+                                       max_steps, ctypes.byref(slot))
+        if st == abi.ERR_CAPACITY:
+            return None
+        _check(st, "mara_batch_admit")
+        self._shapes[slot.value] = array_shape(desc)
+        return slot.value
+
+    def step(self, n_steps):
+        active = ctypes.c_int(0)
+        _check(self.lib.mara_batch_step(self._h, n_steps, ctypes.byref(active)), "mara_batch_step")
+        return active.value
+
+    def query(self, slot):
+        """(done, t, steps, status, report) of one solve."""
+        done = ctypes.c_int(0)
+        t = ctypes.c_double(0)
+        steps = ctypes.c_int(0)
+        rep = MaraReport()
+        st = self.lib.mara_batch_query(self._h, slot, ctypes.byref(done), ctypes.byref(t), ctypes.byref(steps),
+                                       ctypes.byref(rep))
+        if st == abi.ERR_INVALID_ARGUMENT:
+            _check(st, "mara_batch_query")
+        return bool(done.value), t.value, steps.value, st, rep
+
+    def finished(self):
+        """Slots of solves that are done and not yet retired."""
+        cap = max(len(self._shapes), 1)
+        buf = (ctypes.c_int * cap)()
+        n = ctypes.c_int(0)
+        _check(self.lib.mara_batch_finished(self._h, buf, cap, ctypes.byref(n)), "mara_batch_finished")
+        return list(buf[:n.value])
+
+    def retire(self, slot, download=True):
+        P = np.empty(self._shapes[slot]) if download else None
+        _check(self.lib.mara_batch_retire(self._h, slot, P.ctypes.data_as(_dp) if download else None),
+               "mara_batch_retire")
+        del self._shapes[slot]
+        return P
 
 
 def desc_device_bytes(desc):

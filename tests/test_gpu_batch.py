@@ -1,0 +1,169 @@
+"""Batched solves (mara_batch_*): every solve of a mix gives the same result
+alone and in the mix, bit for bit, and the FP64 batch reproduces the single
+block and the oracle bit for bit.  Requests are the serving stream's own
+(seeded Riemann problems, 1-D and 2-D, hydro and MHD) with the zone counts
+scaled down so the oracle finishes in seconds."""
+import numpy as np
+import pytest
+
+from mara2_amd import abi, problems, serving
+from mara2_amd.runtime import Batch, Block
+from helpers import interior, rel_l1
+
+pytestmark = pytest.mark.gpu
+
+
+def SMALL(dim, n):
+    """1-D 256..16384 -> 10..682 zones, 2-D 128..1024 -> 12..102 per side."""
+    return max(10, n // 24) if dim == 1 else max(12, n // 10)
+
+
+N_REQ = 10
+BUDGET = 1 << 30
+
+
+def _stream(precision):
+    reqs = serving.make_stream(N_REQ, precision=precision, resize=SMALL)
+    kinds = {(r.params["dim"], r.params["law"]) for r in reqs}
+    assert kinds == {(1, "hydro"), (1, "mhd"), (2, "hydro"), (2, "mhd")}, kinds
+    return reqs
+
+
+def _run_batch(reqs, precision, chunk=16, max_solves=64):
+    """Admit every request at once and step until all are done."""
+    out = {}
+    with Batch(BUDGET, rk_order=2, precision=precision, max_solves=max_solves) as b:
+        slots = {}
+        for r in reqs:
+            s = b.admit(r.desc, r.P, serving.CFL, serving.T_FINAL, serving.MAX_STEPS)
+            assert s is not None
+            slots[s] = r
+        while slots:
+            b.step(chunk)
+            for s in b.finished():
+                done, t, steps, st, rep = b.query(s)
+                assert done and st == 0, rep.as_dict()
+                out[slots.pop(s).index] = (b.retire(s), t, steps)
+        assert b.free_bytes == BUDGET
+    return out
+
+
+@pytest.fixture(scope="module")
+def fp64_stream():
+    return _stream("fp64")
+
+
+@pytest.fixture(scope="module")
+def fp64_mix(fp64_stream):
+    return _run_batch(fp64_stream, "fp64")
+
+
+def test_fp64_mix_equals_alone(fp64_stream, fp64_mix):
+    for r in fp64_stream:
+        alone = _run_batch([r], "fp64")[r.index]
+        P, t, steps = fp64_mix[r.index]
+        assert (t, steps) == alone[1:], r.index
+        assert np.array_equal(P, alone[0]), r.index
+
+
+def test_fp64_batch_equals_block(fp64_stream, fp64_mix):
+    for r in fp64_stream:
+        with Block(r.desc) as blk:
+            blk.upload(r.P)
+            t, steps, _ = blk.run(serving.CFL, serving.T_FINAL, serving.MAX_STEPS)
+            Pb = blk.download()
+        P, tb, sb = fp64_mix[r.index]
+        assert (tb, sb) == (t, steps), r.index
+        assert np.array_equal(P, Pb), (r.index, r.params["dim"], r.params["law"])
+
+
+def test_fp64_batch_equals_oracle(fp64_stream, fp64_mix):
+    import oracle
+    for r in fp64_stream:
+        Po = r.P.copy()
+        err, t, steps = oracle.run(r.desc, Po, serving.CFL, serving.T_FINAL)
+        assert err == 0
+        P, tb, sb = fp64_mix[r.index]
+        assert (tb, sb) == (t, steps), r.index
+        assert np.array_equal(P, Po), r.index
+
+
+def test_mixed_mix_equals_alone_and_close_to_fp64(fp64_mix):
+    """Mixed form: alone and in the mix bit for bit; against the FP64 result
+    of the same request, relative L1 of the interior primitives <= 2e-4 (the
+    stream's discontinuities move a few binary32 roundings into shock
+    positions; the smooth-flow bound of the mixed form is 1e-5)."""
+    reqs = _stream("mixed")
+    mix = _run_batch(reqs, "mixed", chunk=7)
+    worst = 0.0
+    for r in reqs:
+        alone = _run_batch([r], "mixed", chunk=16)[r.index]
+        P, t, steps = mix[r.index]
+        assert (t, steps) == alone[1:], r.index
+        assert np.array_equal(P, alone[0]), r.index
+        P64 = fp64_mix[r.index][0]
+        worst = max(worst, rel_l1(interior(r.desc, P), interior(r.desc, P64)))
+    print(f"mixed vs fp64 worst rel L1 {worst:.3e}")
+    assert worst <= 2e-4
+
+
+def test_chunking_does_not_change_results(fp64_stream, fp64_mix):
+    res = _run_batch(fp64_stream[:4], "fp64", chunk=1)
+    for i, (P, t, steps) in res.items():
+        assert (t, steps) == fp64_mix[i][1:]
+        assert np.array_equal(P, fp64_mix[i][0])
+
+
+def test_admission_by_budget_and_slot_reuse(fp64_stream):
+    """A pool that holds only some solves: admit returns None when full,
+    retire gives memory and slots back, and reused slots start clean."""
+    reqs = fp64_stream
+    sizes = [Batch.item_bytes_of(r.desc, "fp64") for r in reqs]
+    budget = max(sizes) + min(sizes) + 4096
+    want = _run_batch(reqs, "fp64")
+    with Batch(budget, rk_order=2, precision="fp64", max_solves=2) as b:
+        queue = list(reqs)
+        slots = {}
+        refused = 0
+        while queue or slots:
+            keep = []
+            for r in queue:
+                s = b.admit(r.desc, r.P, serving.CFL, serving.T_FINAL, serving.MAX_STEPS)
+                if s is None:
+                    keep.append(r)
+                    refused += 1
+                else:
+                    slots[s] = r
+            queue = keep
+            assert slots
+            b.step(32)
+            for s in b.finished():
+                r = slots.pop(s)
+                P = b.retire(s)
+                assert np.array_equal(P, want[r.index][0]), r.index
+        assert refused > 0
+        assert b.free_bytes == budget
+
+
+def test_admit_rejects_other_rk_or_precision():
+    r = serving.make_stream(1, precision="fp64", resize=SMALL)[0]
+    with Batch(BUDGET, rk_order=3, precision="fp64") as b:
+        with pytest.raises(Exception) as e:
+            b.admit(r.desc, r.P, serving.CFL, serving.T_FINAL, 10)
+        assert e.value.status == abi.ERR_UNSUPPORTED
+
+
+def test_serve_loop_continuous_batching():
+    """The serving loop on a short stream with staggered arrivals: every
+    request completes with the same result as alone."""
+    reqs = serving.make_stream(8, precision="mixed", resize=SMALL)
+    serving.set_arrivals(reqs, rate=200.0)
+    wall, done = serving.serve(reqs, BUDGET, max_solves=16, chunk=8, keep_results=True)
+    assert len(done) == len(reqs)
+    serving.check_errors(done)
+    ref = _run_batch(reqs, "mixed")
+    for r in done:
+        assert r.finished >= r.admitted >= r.arrival
+        assert np.array_equal(r.result, ref[r.index][0]), r.index
+    st = serving.latency_stats(done)
+    assert st["p99_s"] >= st["p50_s"] > 0
