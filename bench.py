@@ -175,6 +175,65 @@ def other_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def serving_workload(args):
+    """Continuous batching over the seeded Riemann-problem stream: 70% 1-D
+    (256..16384 zones), 30% 2-D (128^2..1024^2), half MHD, PLM + HLLC / HLLE,
+    RK2, CFL 0.4, to t = 0.2, mixed form.  The one-at-a-time service rate is
+    measured on an evenly spaced sample of the stream (create, upload, run,
+    download, destroy per request); arrivals are Poisson at twice that rate."""
+    from mara2_amd import serving
+    n_req = args.requests
+    log(f"building {n_req} requests")
+    reqs = serving.make_stream(n_req, precision="mixed")
+    work = [r.zones for r in reqs]
+    log("warm-up")
+    warm = serving.make_stream(4, precision="mixed", resize=lambda dim, n: 32 if dim == 1 else 16)
+    serving.sequential(warm)
+    serving.serve(warm, 1 << 28, max_solves=8, chunk=args.chunk)
+    stride = max(1, n_req // args.seq_sample)
+    sample = reqs[::stride][:args.seq_sample]
+    log(f"one-at-a-time service of {len(sample)} requests (every {stride}th)")
+    times = serving.sequential(sample)
+    seq_failed = [r.index for r in serving.failures(sample)]
+    seq_rate = len(sample) / sum(times)
+    seq_zone_rate = sum(r.zones * r.steps for r in sample) / sum(times)
+    log(f"sequential: {seq_rate:.3f} solves/s, mean {np.mean(times):.3f} s, max {max(times):.3f} s")
+    budget = int(args.budget_gib * 2**30)
+    serving.set_arrivals(reqs, 2.0 * seq_rate)
+    log(f"serving {n_req} requests, Poisson arrivals at {2 * seq_rate:.3f}/s "
+        f"(span {reqs[-1].arrival:.0f} s), pool {args.budget_gib:g} GiB")
+    wall, done = serving.serve(reqs, budget, chunk=args.chunk, log=log)
+    failed = [r.index for r in serving.failures(done)]
+    st = serving.latency_stats(done)
+    zone_updates = sum(r.zones * r.steps for r in done)
+    log(f"served in {wall:.1f} s: p50 {st['p50_s']:.3f} s p99 {st['p99_s']:.3f} s")
+    out = {"metric": METRIC, "value": len(done) / wall, "unit": "solves/s", "n_gpus": 1, "steps": None,
+           "warmup": None, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32/f64 mixed",
+           "data": f"synthetic: seeded Riemann-problem stream (seed {problems.SEED_SERVING})",
+           "config": {"workload": f"serving stream of {n_req} Riemann problems, 70% 1-D 256..16384 zones, "
+                                  "30% 2-D 128^2..1024^2, half MHD, PLM + HLLC/HLLE, RK2, CFL 0.4, to t = 0.2, "
+                                  "mixed form, Poisson arrivals at 2x the one-at-a-time rate",
+                      "zones_total": int(sum(work)), "hbm_pool_gib": args.budget_gib, "chunk_steps": args.chunk},
+           "latency_s": st, "wall_s": wall, "zone_updates_per_s": zone_updates / wall,
+           "failed_requests": failed, "floor_applied_zones": int(sum(r.floors for r in done)),
+           "arrival_rate": 2.0 * seq_rate,
+           "sequential": {"solves_per_s": seq_rate, "zone_updates_per_s": seq_zone_rate,
+                          "sample": f"{len(sample)} requests, every {stride}th of the stream",
+                          "mean_s": float(np.mean(times)), "max_s": float(max(times)),
+                          "failed_requests": seq_failed}}
+    if args.saturate:
+        for r in reqs:
+            r.arrival = 0.0
+        log("saturated run: every request at t = 0")
+        wall_s, done_s = serving.serve(reqs, budget, chunk=args.chunk, log=log)
+        out["saturated"] = {"solves_per_s": len(done_s) / wall_s, "wall_s": wall_s,
+                            "speedup_vs_sequential": len(done_s) / wall_s / seq_rate,
+                            "latency_s": serving.latency_stats(done_s),
+                            "failed_requests": [r.index for r in serving.failures(done_s)]}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -183,13 +242,22 @@ def main():
     ap.add_argument("--n", type=int, default=0)
     ap.add_argument("--cpu-sample", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="turbulence", choices=["turbulence", "mhd2d", "rmhd3d", "sod"])
+    ap.add_argument("--workload", default="turbulence",
+                    choices=["turbulence", "mhd2d", "rmhd3d", "sod", "serving"])
     ap.add_argument("--wmax", type=float, default=5.0)
+    ap.add_argument("--requests", type=int, default=4096, help="serving: stream length")
+    ap.add_argument("--seq-sample", type=int, default=128, help="serving: requests timed one at a time")
+    ap.add_argument("--budget-gib", type=float, default=64.0, help="serving: HBM pool")
+    ap.add_argument("--chunk", type=int, default=16, help="serving: steps per batch launch sequence")
+    ap.add_argument("--saturate", action="store_true", help="serving: also run with every request at t = 0")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              f"(independent solves shard by block, see DESIGN.md)", file=sys.stderr)
+              f"(a multi-GPU run would shard the mesh into one block per GPU with a halo exchange)",
+              file=sys.stderr)
         sys.exit(2)
+    if args.workload == "serving":
+        return serving_workload(args)
     if args.workload != "turbulence":
         return other_workload(args)
 

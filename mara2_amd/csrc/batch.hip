@@ -542,11 +542,15 @@ int mara_batch_retire(mara_batch* B, int slot, double* P)
         k_soa_to_aos<<<(unsigned)((sl.ncell + 255) / 256), 256, 0, B->stream>>>(it.Pa, it.U, it.G);
         HIP_TRY(hipMemcpyAsync(P, it.U, sl.ncell * it.G.nq * sizeof(double), hipMemcpyDeviceToHost, B->stream));
     }
-    // no kernel of this slot is queued any more: mara_batch_step synchronises
+    // the slot's control block stays inactive until the slot is reused: the
+    // per-solve control kernels still visit it while lower slots are in use
+    Ctl& c = B->h_ctl[slot];
+    std::memset(&c, 0, sizeof c);
+    c.cbins = B->d_bins + (size_t)slot * CBINS * CBIN_STRIDE;
+    HIP_TRY(hipMemcpyAsync(B->d_ctl + slot, &c, sizeof(Ctl), hipMemcpyHostToDevice, B->stream));
     HIP_TRY(hipStreamSynchronize(B->stream));
     B->pool.release(sl.off, sl.bytes);
     sl.used = false;
-    std::memset(&B->h_ctl[slot], 0, sizeof(Ctl));
     while (B->n_items > 0 && !B->slots[B->n_items - 1].used) --B->n_items;
     B->dirty = true;
     return MARA_OK;

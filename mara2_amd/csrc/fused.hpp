@@ -292,10 +292,17 @@ __device__ __forceinline__ void hydro_states_flux(const R* PL, const R* PR, R ga
 
 // Tile height: binary64 state needs twice the LDS and more VGPRs per lane, so
 // its tiles are 64 x 4 (three work-groups per CU), binary32 ones 64 x 8.
-template <typename R> struct PlmTile { static constexpr int TY = sizeof(R) == 8 ? 4 : 8; };
+// WAVES is the occupancy the register allocator is held to: binary64 at 3
+// waves per SIMD (168 VGPRs), binary32 at 4 (128 VGPRs, no spills; at 5 it
+// spills).
+template <typename R> struct PlmTile
+{
+    static constexpr int TY = sizeof(R) == 8 ? 4 : 8;
+    static constexpr int WAVES = sizeof(R) == 8 ? 3 : 4;
+};
 
 template <int Riem, typename R, bool Z>
-__global__ void __launch_bounds__(FUSED_TX * PlmTile<R>::TY) __attribute__((amdgpu_waves_per_eu(3, 8)))
+__global__ void __launch_bounds__(FUSED_TX * PlmTile<R>::TY) __attribute__((amdgpu_waves_per_eu(PlmTile<R>::WAVES, 8)))
 k_fused_plm(const double* __restrict__ Pin, const double* __restrict__ P0, double* __restrict__ U,
             double* __restrict__ Pout, Geom G, double b, int first_stage, int write_u, int courant_out, Ctl* ctl)
 {

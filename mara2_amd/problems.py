@@ -340,12 +340,20 @@ def riemann_params(rng):
                 B=np.stack([BL, BR]) if law == "mhd" else None)
 
 
+# NewtonianMHD::setPressureFloor for the stream's MHD solves: p = 1e-6 rho
+# where the recovered pressure goes negative (src/ConservationLaws.cpp:394-400).
+# Without it the reference scheme stops with an inversion failure on some of
+# the stream's strong rarefactions; NewtonianHydro ignores the floor.
+RIEMANN_PRESSURE_FLOOR = 1e-6
+
+
 def riemann_desc(params, precision="mixed", rk_order=2):
     """PLM (theta 1.5) + HLLC for hydro, HLLE (flux-CT in 2-D) for MHD, gamma
     1.4, outflow boundaries on the unit interval or square."""
     cells = (params["n"], 1, 1) if params["dim"] == 1 else (params["n"], params["n"], 1)
     mhd = params["law"] == "mhd"
-    return abi.make_desc(law=params["law"], gamma=1.4, recon="plm", plm_theta=1.5,
+    return abi.make_desc(law=params["law"], gamma=1.4, pressure_floor=RIEMANN_PRESSURE_FLOOR,
+                         recon="plm", plm_theta=1.5,
                          riemann="hlle" if mhd else "hllc", bc="outflow", cells=cells,
                          rk_order=rk_order, field_ct=1 if (mhd and params["dim"] == 2) else 0,
                          precision=precision)

@@ -41,6 +41,7 @@ class Request:
     steps: int = 0
     t: float = 0.0
     status: int = 0
+    floors: int = 0            # zones where the pressure floor was applied
     result: np.ndarray = None
 
     @property
@@ -85,7 +86,7 @@ def solve_alone(req, keep_result=True):
         b.upload(req.P)
         t, steps, rep = b.run(CFL, T_FINAL, MAX_STEPS, check=False)
         P = b.download()
-    req.t, req.steps, req.status = t, steps, rep.status
+    req.t, req.steps, req.status, req.floors = t, steps, rep.status, rep.floor_applied
     if keep_result:
         req.result = P
     return time.perf_counter() - t0
@@ -128,7 +129,8 @@ def serve(requests, hbm_budget, max_solves=1024, chunk=16, rk_order=2, precision
             batch.step(chunk)
             for slot in batch.finished():
                 r = in_flight.pop(slot)
-                _, r.t, r.steps, r.status, _ = batch.query(slot)
+                _, r.t, r.steps, r.status, rep = batch.query(slot)
+                r.floors = rep.floor_applied
                 P = batch.retire(slot, download=True)
                 r.finished = clock() - t0
                 if keep_results:
@@ -146,6 +148,10 @@ def latency_stats(done):
     lat = np.array([r.latency for r in done])
     return {"p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
             "mean_s": float(lat.mean()), "max_s": float(lat.max())}
+
+
+def failures(done):
+    return [r for r in done if r.status != 0]
 
 
 def check_errors(done):

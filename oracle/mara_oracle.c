@@ -7,9 +7,10 @@
  * function per reference routine, keeping the reference's floating-point
  * operation order.  Build with -ffp-contract=off (oracle/Makefile does).
  *
- * Parity pins for each part are listed in DESIGN.md; the ones compared
- * bit-for-bit against the reference's own compiled sources live in
- * tests/test_oracle_ref.py.
+ * Each part is pinned by tests: bit for bit against the reference's own
+ * compiled sources (tests/test_oracle_ref.py), against the reference's test
+ * values and closed forms (tests/test_oracle_known_answers.py) and against
+ * mathematical identities (tests/test_oracle_math.py).
  */
 #include <math.h>
 #include <float.h>
