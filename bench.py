@@ -175,6 +175,32 @@ def other_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def copy_workload(args):
+    """A plain device-to-device copy of known size: 1 GiB read and 1 GiB
+    written per launch (torch's elementwise copy kernel).  Under rocprofv3
+    --pmc its FETCH_SIZE / WRITE_SIZE readings calibrate those counters for
+    the stage kernel's traffic; its own rate is the HBM bandwidth a
+    streaming kernel reaches on this GPU."""
+    import torch
+    n = 1 << 27  # doubles: 1 GiB
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(max(args.warmup, 1)):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        y.copy_(x)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    nbytes = 2 * n * 8
+    print(json.dumps({"workload": "device copy, 1 GiB read + 1 GiB written per launch", "launches": args.steps,
+                      "read_bytes_per_launch": n * 8, "write_bytes_per_launch": n * 8, "ms_per_launch": ms,
+                      "GB_per_s": nbytes / (ms * 1e-3) / 1e9}), flush=True)
+
+
 def serving_workload(args):
     """Continuous batching over the seeded Riemann-problem stream: 70% 1-D
     (256..16384 zones), 30% 2-D (128^2..1024^2), half MHD, PLM + HLLC / HLLE,
@@ -243,7 +269,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="turbulence",
-                    choices=["turbulence", "mhd2d", "rmhd3d", "sod", "serving"])
+                    choices=["turbulence", "mhd2d", "rmhd3d", "sod", "serving", "copy"])
     ap.add_argument("--wmax", type=float, default=5.0)
     ap.add_argument("--requests", type=int, default=4096, help="serving: stream length")
     ap.add_argument("--seq-sample", type=int, default=128, help="serving: requests timed one at a time")
@@ -258,6 +284,8 @@ def main():
         sys.exit(2)
     if args.workload == "serving":
         return serving_workload(args)
+    if args.workload == "copy":
+        return copy_workload(args)
     if args.workload != "turbulence":
         return other_workload(args)
 
