@@ -3,8 +3,9 @@
 // primitive recovery in one pass over the zones.
 //
 // Work decomposition (2.5-D marching):
-//   - a work-group owns a tile of TX = 64 (one wavefront per row) by TY = 8
+//   - a work-group owns a tile of TX = 64 (one wavefront per row) by TY
 //     interior zones in the x-y plane and marches along z over KZ planes;
+//     tiles are numbered so that neighbours run on the same XCD (xcd_tile);
 //   - plane k of the stage input, with a halo of NG zones in x and y, is
 //     staged in LDS for the x- and y-face stencils;
 //   - each zone's own z column (k-NG .. k+NG) lives in registers, so the
@@ -28,6 +29,20 @@ namespace mara {
 constexpr int FUSED_TX = 64;
 constexpr int FUSED_TY = 8;
 constexpr int FUSED_KZ = 16;
+
+// Work-group -> tile order that keeps neighbouring tiles on one XCD.  The
+// dispatcher hands work-group L of a 1-D grid to XCD L % 8, and each XCD has
+// its own L2.  Tiles [start(x), start(x + 1)) in x-fastest order go to XCD x,
+// in launch order, so tiles that share halo rows and columns run side by side
+// behind the same L2.  The map is a bijection on [0, ntiles).
+constexpr int NUM_XCD = 8;
+
+__device__ __forceinline__ int xcd_tile(int L, int ntiles)
+{
+    const int x = L % NUM_XCD, l = L / NUM_XCD;
+    const int q = ntiles / NUM_XCD, r = ntiles % NUM_XCD;
+    return x * q + (x < r ? x : r) + l;
+}
 
 // Flux through one face from a stencil of 2 NG zones per field (reference
 // IntercellFluxScheme + RiemannSolver, src/IntercellFluxSchemes.cpp:45-117).
@@ -81,14 +96,18 @@ k_fused_hydro(const double* __restrict__ Pin, const double* __restrict__ P0, dou
     const int lane = threadIdx.x;
     const int row = threadIdx.y;
     const int tid = row * TX + lane;
-    const int i0 = G.g[0] + blockIdx.x * TX;
-    const int j0 = G.g[1] + blockIdx.y * TY;
+    const int ntx = (G.n[0] + TX - 1) / TX, nty = (G.n[1] + TY - 1) / TY;
+    const int ntz = Z ? (G.n[2] + FUSED_KZ - 1) / FUSED_KZ : 1;
+    const int tile = xcd_tile(blockIdx.x, ntx * nty * ntz);
+    const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
+    const int i0 = G.g[0] + tx * TX;
+    const int j0 = G.g[1] + ty * TY;
     const int i = i0 + lane, j = j0 + row;
     const int ie = G.N[0] - G.g[0], je = G.N[1] - G.g[1];  // one past the last interior zone
     const bool cell_xy = i < ie && j < je;
     const bool xface = i <= ie && j < je;                  // left x-face of (i, j) exists
     const bool yface = j <= je && i < ie;                  // lower y-face of (i, j) exists
-    const int kb = Z ? G.g[2] + blockIdx.z * FUSED_KZ : 0;
+    const int kb = Z ? G.g[2] + tz * FUSED_KZ : 0;
     const int ke = Z ? min(kb + FUSED_KZ, G.N[2] - G.g[2]) : 1;
     const long long sxy = (long long)G.N[0] * G.N[1];
     const R theta = R(G.theta), gamma = R(G.gamma);
@@ -322,14 +341,18 @@ k_fused_plm(const double* __restrict__ Pin, const double* __restrict__ P0, doubl
     const int lane = threadIdx.x;
     const int row = threadIdx.y;
     const int tid = row * TX + lane;
-    const int i0 = G.g[0] + blockIdx.x * TX;
-    const int j0 = G.g[1] + blockIdx.y * TY;
+    const int ntx = (G.n[0] + TX - 1) / TX, nty = (G.n[1] + TY - 1) / TY;
+    const int ntz = Z ? (G.n[2] + FUSED_KZ - 1) / FUSED_KZ : 1;
+    const int tile = xcd_tile(blockIdx.x, ntx * nty * ntz);
+    const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
+    const int i0 = G.g[0] + tx * TX;
+    const int j0 = G.g[1] + ty * TY;
     const int i = i0 + lane, j = j0 + row;
     const int ie = G.N[0] - G.g[0], je = G.N[1] - G.g[1];
     const bool cell_xy = i < ie && j < je;
     const bool xface = i <= ie && j < je;
     const bool yface = j <= je && i < ie;
-    const int kb = Z ? G.g[2] + blockIdx.z * FUSED_KZ : 0;
+    const int kb = Z ? G.g[2] + tz * FUSED_KZ : 0;
     const int ke = Z ? min(kb + FUSED_KZ, G.N[2] - G.g[2]) : 1;
     const long long sxy = (long long)G.N[0] * G.N[1];
     const R theta = R(G.theta), gamma = R(G.gamma);
@@ -372,13 +395,32 @@ k_fused_plm(const double* __restrict__ Pin, const double* __restrict__ P0, doubl
     for (int k = kb; k < ke; ++k)
     {
         const long long pbase = (long long)k * sxy;
-        for (int t = tid; t < HX * HY; t += NT)
+        if constexpr (Z)
         {
-            const int hx = t % HX, hy = t / HX;
-            const int gi = min(i0 - NG + hx, G.N[0] - 1), gj = min(j0 - NG + hy, G.N[1] - 1);
-            const long long c = pbase + (long long)gj * G.N[0] + gi;
+            // own zone of plane k from the z window (loaded two planes ago),
+            // only the halo ring from memory
 #pragma unroll
-            for (int q = 0; q < 5; ++q) plane[q][hy][hx] = R(Pin[q * G.fs + c]);
+            for (int q = 0; q < 5; ++q) plane[q][row + NG][lane + NG] = W[q][2];
+            for (int t = tid; t < HX * HY; t += NT)
+            {
+                const int hx = t % HX, hy = t / HX;
+                if (hx >= NG && hx < NG + TX && hy >= NG && hy < NG + TY) continue;
+                const int gi = min(i0 - NG + hx, G.N[0] - 1), gj = min(j0 - NG + hy, G.N[1] - 1);
+                const long long c = pbase + (long long)gj * G.N[0] + gi;
+#pragma unroll
+                for (int q = 0; q < 5; ++q) plane[q][hy][hx] = R(Pin[q * G.fs + c]);
+            }
+        }
+        else
+        {
+            for (int t = tid; t < HX * HY; t += NT)
+            {
+                const int hx = t % HX, hy = t / HX;
+                const int gi = min(i0 - NG + hx, G.N[0] - 1), gj = min(j0 - NG + hy, G.N[1] - 1);
+                const long long c = pbase + (long long)gj * G.N[0] + gi;
+#pragma unroll
+                for (int q = 0; q < 5; ++q) plane[q][hy][hx] = R(Pin[q * G.fs + c]);
+            }
         }
         // z: slide the window to k-1..k+2, half-slope of zone k+1, upper z-face
         R Fzhi[5];
@@ -583,14 +625,14 @@ inline void launch_fused_t(const Geom& G, hipStream_t s, const double* Pin, cons
 {
     const bool z = G.N[2] > 1;
     dim3 block(FUSED_TX, FUSED_TY, 1);
-    dim3 grid((G.n[0] + FUSED_TX - 1) / FUSED_TX, (G.n[1] + FUSED_TY - 1) / FUSED_TY,
-              z ? (G.n[2] + FUSED_KZ - 1) / FUSED_KZ : 1);
+    dim3 grid(((G.n[0] + FUSED_TX - 1) / FUSED_TX) * ((G.n[1] + FUSED_TY - 1) / FUSED_TY)
+              * (z ? (G.n[2] + FUSED_KZ - 1) / FUSED_KZ : 1));
     if constexpr (Recon == 1)
     {
         constexpr int TY = PlmTile<R>::TY;
         const dim3 pblock(FUSED_TX, TY, 1);
-        const dim3 pgrid((G.n[0] + FUSED_TX - 1) / FUSED_TX, (G.n[1] + TY - 1) / TY,
-                         z ? (G.n[2] + FUSED_KZ - 1) / FUSED_KZ : 1);
+        const dim3 pgrid(((G.n[0] + FUSED_TX - 1) / FUSED_TX) * ((G.n[1] + TY - 1) / TY)
+                         * (z ? (G.n[2] + FUSED_KZ - 1) / FUSED_KZ : 1));
         if (z) k_fused_plm<Riem, R, true><<<pgrid, pblock, 0, s>>>(Pin, P0, U, Pout, G, b, first, write_u, cout, ctl);
         else k_fused_plm<Riem, R, false><<<pgrid, pblock, 0, s>>>(Pin, P0, U, Pout, G, b, first, write_u, cout, ctl);
     }

@@ -442,8 +442,18 @@ template <typename R, int NQ>
 __device__ __forceinline__ R max_elem(const R* A)
 {
     R m = A[0];
+    if constexpr (NQ == 5)
+    {
+        // hydro states hold A[1] = A[2] = A[3] = vn (hydro_state): once m has
+        // met A[1], comparing it with A[2] and A[3] never changes it
+        if (m < A[1]) m = A[1];
+        if (m < A[4]) m = A[4];
+    }
+    else
+    {
 #pragma unroll
-    for (int n = 1; n < NQ; ++n) if (m < A[n]) m = A[n];
+        for (int n = 1; n < NQ; ++n) if (m < A[n]) m = A[n];
+    }
     if (m < R(0)) m = R(0);
     return m;
 }
@@ -452,8 +462,16 @@ template <typename R, int NQ>
 __device__ __forceinline__ R min_elem(const R* A)
 {
     R m = A[0];
+    if constexpr (NQ == 5)
+    {
+        if (A[1] < m) m = A[1];
+        if (A[4] < m) m = A[4];
+    }
+    else
+    {
 #pragma unroll
-    for (int n = 1; n < NQ; ++n) if (A[n] < m) m = A[n];
+        for (int n = 1; n < NQ; ++n) if (A[n] < m) m = A[n];
+    }
     if (R(0) < m) m = R(0);
     return m;
 }
@@ -489,34 +507,37 @@ __device__ __forceinline__ void hllc(const State<R, 5>& L, const State<R, 5>& Rs
     const R vnl = Pl[1] * dA[0] + Pl[2] * dA[1] + Pl[3] * dA[2];
     const R lc = Math<R>::div((Pr[4] - Pr[0] * vnr * (ap - vnr)) - (Pl[4] - Pl[0] * vnl * (am - vnl)),
                               Pl[0] * (am - vnl) - Pr[0] * (ap - vnr));
-    const R fl = Math<R>::div(Pl[0] * (am - vnl), am - lc);
-    const R fr = Math<R>::div(Pr[0] * (ap - vnr), ap - lc);
-    R Ul_[5], Ur_[5];
-    Ul_[0] = fl;
-    Ul_[4] = fl * (Math<R>::div(Ul[4], Pl[0]) + (lc - vnl) * (lc + Math<R>::div(Pl[4], Pl[0] * (am - vnl))));
-    Ul_[1] = fl * ((lc - vnl) * dA[0] + Pl[1]);
-    Ul_[2] = fl * ((lc - vnl) * dA[1] + Pl[2]);
-    Ul_[3] = fl * ((lc - vnl) * dA[2] + Pl[3]);
-    Ur_[0] = fr;
-    Ur_[4] = fr * (Math<R>::div(Ur[4], Pr[0]) + (lc - vnr) * (lc + Math<R>::div(Pr[4], Pr[0] * (ap - vnr))));
-    Ur_[1] = fr * ((lc - vnr) * dA[0] + Pr[1]);
-    Ur_[2] = fr * ((lc - vnr) * dA[1] + Pr[2]);
-    Ur_[3] = fr * ((lc - vnr) * dA[2] + Pr[3]);
+    // Only the star state on the contact's side of the face enters the flux
+    // (left when 0 <= lc); each lane builds that one, with the same
+    // expressions as the reference's left and right star states.
     const R s = R(0.0);
+    const bool left = s <= lc;
+    R Ps[5], Us_side[5], Fs[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i)
+    {
+        Ps[i] = left ? Pl[i] : Pr[i];
+        Us_side[i] = left ? Ul[i] : Ur[i];
+        Fs[i] = left ? L.F[i] : Rs.F[i];
+    }
+    const R a = left ? am : ap;
+    const R vn = left ? vnl : vnr;
+    const R f = Math<R>::div(Ps[0] * (a - vn), a - lc);
+    R Ustar[5];
+    Ustar[0] = f;
+    Ustar[4] = f * (Math<R>::div(Us_side[4], Ps[0]) + (lc - vn) * (lc + Math<R>::div(Ps[4], Ps[0] * (a - vn))));
+    Ustar[1] = f * ((lc - vn) * dA[0] + Ps[1]);
+    Ustar[2] = f * ((lc - vn) * dA[1] + Ps[2]);
+    Ustar[3] = f * ((lc - vn) * dA[2] + Ps[3]);
     if (s <= am)
     {
 #pragma unroll
         for (int i = 0; i < 5; ++i) Fout[i] = L.F[i];
     }
-    else if (am < s && s <= lc)
+    else if ((am < s && s <= lc) || (lc < s && s <= ap))
     {
 #pragma unroll
-        for (int i = 0; i < 5; ++i) Fout[i] = L.F[i] + am * (Ul_[i] - Ul[i]);
-    }
-    else if (lc < s && s <= ap)
-    {
-#pragma unroll
-        for (int i = 0; i < 5; ++i) Fout[i] = Rs.F[i] + ap * (Ur_[i] - Ur[i]);
+        for (int i = 0; i < 5; ++i) Fout[i] = Fs[i] + a * (Ustar[i] - Us_side[i]);
     }
     else if (ap < s)
     {
