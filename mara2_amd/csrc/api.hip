@@ -225,7 +225,7 @@ void launch_courant(mara_block* b)
     }
 }
 
-// One RK stage: fluxes, CT, update, ghost fill.  With courant_out the last
+// One RK stage: fluxes, CT, update with ghost fill.  With courant_out the last
 // stage also reduces the next step's CFL time step into the control bins.
 void launch_stage(mara_block* b, int rk, int courant_out)
 {
@@ -277,7 +277,9 @@ void launch_stage(mara_block* b, int rk, int courant_out)
         b->timed_ms += ms;
         b->timed_launches += 1;
     }
-    launch_bc(b, Pout);
+    // the flux-array update writes the ghost zones of Pout itself
+    // (write_guard_images); the fused kernels leave them to a ghost-fill pass
+    if (b->fused) launch_bc(b, Pout);
     if (b->fused && b->d.rk_order == 1)
     {
         double* t = b->Pa;

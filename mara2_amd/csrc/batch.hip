@@ -119,7 +119,6 @@ struct mara_batch
     std::vector<BTile> tiles;
     std::map<int, Segment> flux[3], ct[3];  // class -> segment
     std::map<int, Segment> update;
-    Segment bc[3] = {{0, 0}, {0, 0}, {0, 0}};
 };
 
 namespace {
@@ -234,21 +233,6 @@ void build_table(mara_batch* B)
         }
         if ((int)B->tiles.size() > s0) B->update[(int)c] = Segment{s0, (int)B->tiles.size() - s0};
     }
-    for (int axis = 0; axis < 3; ++axis)
-    {
-        const int s0 = seg_begin();
-        for (int n = 0; n < B->n_items; ++n)
-        {
-            if (!B->slots[n].used) continue;
-            const Geom& G = B->items[n].G;
-            if (G.g[axis] <= 0) continue;
-            const int t1 = axis == 0 ? 1 : 0, t2 = axis == 2 ? 1 : 2;
-            const long long cnt = 2ll * G.g[axis] * G.N[t1] * G.N[t2];
-            for (long long x = 0; x < cnt; x += 256)
-                B->tiles.push_back(BTile{n, (int)x, 0, 0, (int)std::min<long long>(256, cnt - x), 1, 0, 0});
-        }
-        B->bc[axis] = Segment{s0, (int)B->tiles.size() - s0};
-    }
 }
 
 // ---- template dispatch over a solve class -------------------------------------
@@ -291,9 +275,7 @@ void run_stage(mara_batch* B, int rk)
         for (auto& e : B->ct[axis])
             kb_ct<R><<<e.second.count, 256, 0, B->stream>>>(B->d_items, B->d_tiles + e.second.offset, axis);
     for (auto& e : B->update) update_dispatch<R>(B, B->classes[e.first], e.second, bb[rk], rk == 0, last, last);
-    for (int axis = 0; axis < 3; ++axis)
-        if (B->bc[axis].count)
-            kb_bc<<<B->bc[axis].count, 256, 0, B->stream>>>(B->d_items, B->d_tiles + B->bc[axis].offset, axis, last);
+    // ghost zones are written by the zone update (write_guard_images)
 }
 
 __global__ void kb_reset_bins(Ctl* ctl) { reset_bins(ctl); }

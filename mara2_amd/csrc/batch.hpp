@@ -4,11 +4,12 @@
 // Each solve ("item") keeps the device layout and the per-zone arithmetic of
 // a single block (kernels.hpp): the batched kernels only replace the launch
 // geometry.  A work-group takes one tile from a table built on the host; a
-// tile names its item and a patch of that item's face, zone or guard range:
-// 256 x 1 zones for 1-D items and for guard ranges, 64 x 4 for 2-D and 3-D
-// items.  Every solve has its own control block (dt, clock, stop flag,
-// failure counters, CFL bins), so solves never exchange data and a solve's
-// results are bitwise the same alone or in any mix.
+// tile names its item and a patch of that item's faces or zones: 256 x 1
+// for 1-D items, 64 x 4 for 2-D and 3-D items.  Ghost zones are written by
+// the zone update itself (write_guard_images).  Every solve has its own
+// control block (dt, clock, stop flag, failure counters, CFL bins), so
+// solves never exchange data and a solve's results are bitwise the same
+// alone or in any mix.
 #pragma once
 
 #include "kernels.hpp"
@@ -88,17 +89,6 @@ kb_update(const BItem* __restrict__ items, const BTile* __restrict__ tiles, doub
         dtc = update_zone<Law, R, CT>(it.Pa, it.U, last ? it.Pa : it.Pb, (const R*)it.F[0], (const R*)it.F[1],
                                       (const R*)it.F[2], it.Gct, it.G, b, first, !last, courant_out, it.ctl, i, j, k);
     if (courant_out) courant_to_bins(dtc, wmin, 4, threadIdx.x, it.ctl);
-}
-
-__global__ void __launch_bounds__(256)
-kb_bc(const BItem* __restrict__ items, const BTile* __restrict__ tiles, int axis, int last)
-{
-    const BTile T = tiles[blockIdx.x];
-    const BItem& it = items[T.item];
-    if (!it.ctl->active) return;
-    const int t = T.x0 + threadIdx.x;
-    if ((int)threadIdx.x >= T.nx) return;
-    bc_guard(last ? it.Pa : it.Pb, it.G, axis, it.bc, t);
 }
 
 template <int Law>

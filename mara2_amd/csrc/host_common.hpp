@@ -83,6 +83,7 @@ inline Geom make_geom(const mara_desc& d)
         G.n[a] = d.cells[a];
     }
     G.nq = law_nq(d.law);
+    G.bc = d.bc;
     G.fs = round_up((long long)G.N[0] * G.N[1] * G.N[2], 64);
     G.ffs = round_up((long long)(G.N[0] + 1) * (G.N[1] + 1) * (G.N[2] + 1), 64);
     G.gamma = d.gamma;

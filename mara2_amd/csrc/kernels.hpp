@@ -52,6 +52,7 @@ struct Geom
     int g[3];       // guards
     int n[3];       // interior cells
     int nq;
+    int bc;         // boundary condition on every axis with guards: 0 periodic, 1 outflow, 2 reflecting
     long long fs;   // cell field stride
     long long ffs;  // face field stride
     double gamma, pfloor, theta;
@@ -282,6 +283,72 @@ __device__ __forceinline__ void courant_to_bins(double v, double* wmin, int nwav
 }
 
 // ---------------------------------------------------------------------------
+// Ghost images of an interior zone.  BoundaryCondition::applySimple
+// (src/BoundaryConditions.cpp:24-176) fills guard layers axis by axis:
+// periodic copies zone N - 2g + l into left layer l and zone g + l into right
+// layer N - g + l, outflow copies the edge zone, reflecting copies the mirror
+// zone and negates the velocity component along the axis.  Filling axis by
+// axis composes these index maps, so afterwards every guard zone (i, j, k)
+// holds zone (m0(i), m1(j), m2(k)), negated once per reflecting axis it was
+// mapped along.  The thread that writes an interior zone therefore writes the
+// same value to each guard zone that maps to it, which is the whole ghost
+// fill without a separate pass.
+// ---------------------------------------------------------------------------
+// Number of indices along `axis` that hold interior index s after the fill
+// (s itself included).
+__device__ __forceinline__ int axis_image_count(const Geom& G, int axis, int s)
+{
+    const int g = G.g[axis], N = G.N[axis];
+    if (g == 0) return 1;
+    if (G.bc == 0) return 1 + (s >= N - 2 * g ? 1 : 0) + (s < 2 * g ? 1 : 0);
+    if (G.bc == 1) return 1 + (s == g ? g : 0) + (s == N - g - 1 ? g : 0);
+    return 1 + (s < 2 * g ? 1 : 0) + (s >= N - 2 * g ? 1 : 0);
+}
+
+// The a-th of them: a = 0 is s, then left-side images, then right-side ones.
+__device__ __forceinline__ int axis_image(const Geom& G, int axis, int s, int a)
+{
+    if (a == 0) return s;
+    const int g = G.g[axis], N = G.N[axis];
+    if (G.bc == 0) return (a == 1 && s >= N - 2 * g) ? s - (N - 2 * g) : s + (N - 2 * g);
+    if (G.bc == 1)
+    {
+        const int nl = s == g ? g : 0;
+        return a <= nl ? a - 1 : N - (a - nl);
+    }
+    return (a == 1 && s < 2 * g) ? 2 * g - 1 - s : 2 * N - 2 * g - 1 - s;
+}
+
+template <int NQ>
+__device__ __forceinline__ void write_guard_images(double* __restrict__ P, const Geom& G, int i, int j, int k,
+                                                   const double* v)
+{
+    const int nx = axis_image_count(G, 0, i), ny = axis_image_count(G, 1, j), nz = axis_image_count(G, 2, k);
+    if (nx * ny * nz == 1) return;
+    const bool refl = G.bc == 2;
+    for (int c = 0; c < nz; ++c)
+    {
+        const int kk = axis_image(G, 2, k, c);
+        for (int b = 0; b < ny; ++b)
+        {
+            const int jj = axis_image(G, 1, j, b);
+            for (int a = 0; a < nx; ++a)
+            {
+                if ((a | b | c) == 0) continue;
+                const long long d = cell_index(G, axis_image(G, 0, i, a), jj, kk);
+#pragma unroll
+                for (int q = 0; q < NQ; ++q)
+                {
+                    double w = v[q];
+                    if (refl && ((q == 1 && a != 0) || (q == 2 && b != 0) || (q == 3 && c != 0))) w *= -1;
+                    P[q * G.fs + d] = w;
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Zone update: L = -div F (src/MeshOperator.cpp:323-381), RK combine
 // U = U0 (1-b) + (U + dt L) b (src/SolutionSchemes.cpp:232-235), and
 // recoverPrimitive (src/FieldOperator.cpp:41-82) on interior zones.
@@ -340,11 +407,16 @@ __device__ __forceinline__ double update_zone(const double* __restrict__ P0, dou
     {
         atomicAdd(&ctl->c2p_fail, 1ull);
         atomicMin(&ctl->first_fail, ref_linear(G, i, j, k));
+        // the zone keeps its previous value, and so do its ghost images
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) Pn[q] = Pout[q * G.fs + c];
+        write_guard_images<NQ>(Pout, G, i, j, k, Pn);
     }
     else
     {
 #pragma unroll
         for (int q = 0; q < NQ; ++q) Pout[q * G.fs + c] = Pn[q];
+        write_guard_images<NQ>(Pout, G, i, j, k, Pn);
         if (courant_out)
         {
             // the next step's CFL from the zone's new primitives (interior only,
