@@ -64,7 +64,6 @@ long long device_bytes(const mara_desc* d, const Sizes& s)
     if (fused_supported(d) && d->rk_order == 3) b += 1ll * s.nq * s.fs * 8;  // Pc
     if (!fused_supported(d))
         b += (long long)(s.fleshed[0] + s.fleshed[1] + s.fleshed[2]) * s.nq * s.ffs * s.real;
-    if (s.ct) b += 9ll * s.ffs * 8;
     b += 256 + CBINS * CBIN_STRIDE * 8;  // control block and CFL bins
     return b;
 }
@@ -84,7 +83,6 @@ struct mara_block
     double* Pc = nullptr;  // third stage buffer of the fused RK3 schedule
     double* U = nullptr;
     void* F[3] = {nullptr, nullptr, nullptr};
-    double* Gct = nullptr;
     Ctl* ctl = nullptr;
     Ctl* ctl_host = nullptr;
     unsigned long long* cbins = nullptr;
@@ -152,25 +150,6 @@ void launch_flux(mara_block* b, const double* Pin, int axis)
     }
 }
 
-template <typename R>
-void launch_ct(mara_block* b)
-{
-    const Geom& G = b->G;
-    for (int a = 0; a < 3; ++a)
-    {
-        if (G.N[a] < 2) continue;
-        int lo[3], m[3];
-        for (int e = 0; e < 3; ++e)
-        {
-            lo[e] = G.g[e];
-            m[e] = G.n[e] + (e == a ? 1 : 0);
-        }
-        k_ct<R><<<grid3(m[0], m[1], m[2]), block3(), 0, b->stream>>>(
-            (const R*)b->F[0], (const R*)b->F[1], (const R*)b->F[2], b->Gct, G, a,
-            lo[0], lo[1], lo[2], m[0], m[1], m[2], b->ctl);
-    }
-}
-
 template <int Law, typename R>
 void launch_update_law(mara_block* b, double* Pout, double bk, int first, int write_u, int cout)
 {
@@ -178,11 +157,11 @@ void launch_update_law(mara_block* b, double* Pout, double bk, int first, int wr
     const dim3 grid = grid3(G.n[0], G.n[1], G.n[2]);
     if (b->S.ct)
         k_update<Law, R, true><<<grid, block3(), 0, b->stream>>>(
-            b->Pa, b->U, Pout, (const R*)b->F[0], (const R*)b->F[1], (const R*)b->F[2], b->Gct,
+            b->Pa, b->U, Pout, (const R*)b->F[0], (const R*)b->F[1], (const R*)b->F[2],
             G, bk, first, write_u, cout, b->ctl);
     else
         k_update<Law, R, false><<<grid, block3(), 0, b->stream>>>(
-            b->Pa, b->U, Pout, (const R*)b->F[0], (const R*)b->F[1], (const R*)b->F[2], b->Gct,
+            b->Pa, b->U, Pout, (const R*)b->F[0], (const R*)b->F[1], (const R*)b->F[2],
             G, bk, first, write_u, cout, b->ctl);
 }
 
@@ -259,11 +238,6 @@ void launch_stage(mara_block* b, int rk, int courant_out)
             if (b->G.N[axis] < 2) continue;
             if (mixed) launch_flux<float>(b, Pin, axis);
             else launch_flux<double>(b, Pin, axis);
-        }
-        if (b->S.ct)
-        {
-            if (mixed) launch_ct<float>(b);
-            else launch_ct<double>(b);
         }
         if (mixed) launch_update<float>(b, Pout, bb[rk], rk == 0, !last, last && courant_out);
         else launch_update<double>(b, Pout, bb[rk], rk == 0, !last, last && courant_out);
@@ -370,7 +344,6 @@ int mara_block_create(const mara_desc* d, mara_block** out)
             if (!hip_ok(hipMalloc(&b->F[a], fb))) return fail(MARA_ERR_HIP);
         }
     }
-    if (b->S.ct && !hip_ok(hipMalloc(&b->Gct, 9 * (size_t)b->S.ffs * sizeof(double)))) return fail(MARA_ERR_HIP);
     if (!hip_ok(hipMalloc(&b->ctl, sizeof(Ctl)))) return fail(MARA_ERR_HIP);
     if (!hip_ok(hipMalloc(&b->cbins, CBINS * CBIN_STRIDE * sizeof(unsigned long long)))) return fail(MARA_ERR_HIP);
     if (!hip_ok(hipHostMalloc((void**)&b->ctl_host, sizeof(Ctl), hipHostMallocDefault))) return fail(MARA_ERR_HIP);
@@ -396,7 +369,6 @@ int mara_block_destroy(mara_block* b)
     hipFree(b->Pc);
     hipFree(b->U);
     for (int a = 0; a < 3; ++a) hipFree(b->F[a]);
-    hipFree(b->Gct);
     hipFree(b->ctl);
     hipFree(b->cbins);
     if (b->ctl_host) hipHostFree(b->ctl_host);
@@ -538,7 +510,7 @@ int mara_block_stage_kernel_time(mara_block* b, double* avg_ms, long long* launc
     if (!b) return MARA_ERR_INVALID_ARGUMENT;
     if (avg_ms) *avg_ms = b->timed_launches ? b->timed_ms / b->timed_launches : 0.0;
     if (launches) *launches = b->timed_launches;
-    if (name) *name = b->fused ? fused_kernel_name(b->d) : "k_flux+k_ct+k_update (generic stage)";
+    if (name) *name = b->fused ? fused_kernel_name(b->d) : "k_flux+k_update (flux-array stage)";
     return MARA_OK;
 }
 

@@ -86,9 +86,10 @@ struct Slot
 struct ClassKey
 {
     int law, recon, riemann, ct;
+    int pencil;  // 1-D items of an RK2 batch: fused pencil tiles
     bool operator<(const ClassKey& o) const
     {
-        return std::tie(law, recon, riemann, ct) < std::tie(o.law, o.recon, o.riemann, o.ct);
+        return std::tie(law, recon, riemann, ct, pencil) < std::tie(o.law, o.recon, o.riemann, o.ct, o.pencil);
     }
 };
 
@@ -117,7 +118,7 @@ struct mara_batch
     int n_items = 0;  // highest used slot + 1
     // launch table
     std::vector<BTile> tiles;
-    std::map<int, Segment> flux[3], ct[3];  // class -> segment
+    std::map<int, Segment> flux[3];  // class -> segment
     std::map<int, Segment> update;
 };
 
@@ -125,7 +126,9 @@ namespace {
 
 int class_of(mara_batch* B, const mara_desc& d)
 {
-    const ClassKey k{d.law, d.recon, d.riemann, (d.law != MARA_LAW_HYDRO && d.field_ct) ? 1 : 0};
+    const bool one_d = d.cells[1] == 1 && d.cells[2] == 1;
+    const ClassKey k{d.law, d.recon, d.riemann, (d.law != MARA_LAW_HYDRO && d.field_ct) ? 1 : 0,
+                     (one_d && B->rk_order == 2) ? 1 : 0};
     for (size_t c = 0; c < B->classes.size(); ++c)
         if (!(B->classes[c] < k) && !(k < B->classes[c])) return (int)c;
     B->classes.push_back(k);
@@ -142,7 +145,6 @@ long long item_bytes(const mara_desc& d, int precision)
     const int real = precision == MARA_PREC_MIXED ? 4 : 8;
     long long b = 3 * round_up(nq * fs * 8, 256);
     for (int a = 0; a < 3; ++a) if (N[a] > 1) b += round_up(nq * ffs * real, 256);
-    if (d.law != MARA_LAW_HYDRO && d.field_ct) b += round_up(9 * ffs * 8, 256);
     return b;
 }
 
@@ -169,11 +171,27 @@ void box_tiles(std::vector<BTile>& out, int item, const int lo[3], const int hi[
 void build_table(mara_batch* B)
 {
     B->tiles.clear();
-    for (int a = 0; a < 3; ++a) { B->flux[a].clear(); B->ct[a].clear(); }
+    for (int a = 0; a < 3; ++a) B->flux[a].clear();
     B->update.clear();
     auto seg_begin = [&]() { return (int)B->tiles.size(); };
     for (size_t c = 0; c < B->classes.size(); ++c)
     {
+        if (B->classes[c].pencil)
+        {
+            // one tile per PENCIL - 1 interior zones of each 1-D item
+            const int s0 = seg_begin();
+            for (int n = 0; n < B->n_items; ++n)
+            {
+                const Slot& sl = B->slots[n];
+                if (!sl.used || sl.cls != (int)c) continue;
+                const Geom& G = B->items[n].G;
+                const int ie = G.N[0] - G.g[0];
+                for (int x0 = G.g[0]; x0 < ie; x0 += PENCIL - 1)
+                    B->tiles.push_back(BTile{n, x0, 0, 0, std::min(PENCIL - 1, ie - x0), 1, 2, 0});
+            }
+            if ((int)B->tiles.size() > s0) B->update[(int)c] = Segment{s0, (int)B->tiles.size() - s0};
+            continue;
+        }
         for (int axis = 0; axis < 3; ++axis)
         {
             const int s0 = seg_begin();
@@ -198,28 +216,6 @@ void build_table(mara_batch* B)
                 box_tiles(B->tiles, n, lo, hi);
             }
             if ((int)B->tiles.size() > s0) B->flux[axis][(int)c] = Segment{s0, (int)B->tiles.size() - s0};
-        }
-        if (B->classes[c].ct)
-        {
-            for (int axis = 0; axis < 3; ++axis)
-            {
-                const int s0 = seg_begin();
-                for (int n = 0; n < B->n_items; ++n)
-                {
-                    const Slot& sl = B->slots[n];
-                    if (!sl.used || sl.cls != (int)c) continue;
-                    const Geom& G = B->items[n].G;
-                    if (G.N[axis] < 2) continue;
-                    int lo[3], hi[3];
-                    for (int e = 0; e < 3; ++e)
-                    {
-                        lo[e] = G.g[e];
-                        hi[e] = G.g[e] + G.n[e] + (e == axis ? 1 : 0);
-                    }
-                    box_tiles(B->tiles, n, lo, hi);
-                }
-                if ((int)B->tiles.size() > s0) B->ct[axis][(int)c] = Segment{s0, (int)B->tiles.size() - s0};
-            }
         }
         const int s0 = seg_begin();
         for (int n = 0; n < B->n_items; ++n)
@@ -252,9 +248,32 @@ void flux_dispatch(mara_batch* B, const ClassKey& k, const Segment& s, int axis,
 #undef FL
 }
 
+template <int Law, int Recon, int Riem, typename R>
+void pencil_launch(mara_batch* B, const ClassKey& k, const BTile* t, int count, double b, int first, int last, int cout)
+{
+    if (k.ct) kb_pencil<Law, Recon, Riem, R, true><<<count, PENCIL, 0, B->stream>>>(B->d_items, t, b, first, last, cout);
+    else kb_pencil<Law, Recon, Riem, R, false><<<count, PENCIL, 0, B->stream>>>(B->d_items, t, b, first, last, cout);
+}
+
+template <typename R>
+void pencil_dispatch(mara_batch* B, const ClassKey& k, const Segment& s, double b, int first, int last, int cout)
+{
+    const BTile* t = B->d_tiles + s.offset;
+#define PN(L, Rc, Rm) if (k.law == L && k.recon == Rc && k.riemann == Rm) { pencil_launch<L, Rc, Rm, R>(B, k, t, s.count, b, first, last, cout); return; }
+    PN(0, 0, 0) PN(0, 0, 1) PN(0, 1, 0) PN(0, 1, 1) PN(0, 2, 0) PN(0, 2, 1)
+    PN(1, 0, 0) PN(1, 1, 0) PN(1, 2, 0)
+    PN(2, 0, 0) PN(2, 1, 0) PN(2, 2, 0)
+#undef PN
+}
+
 template <typename R>
 void update_dispatch(mara_batch* B, const ClassKey& k, const Segment& s, double b, int first, int last, int cout)
 {
+    if (k.pencil)
+    {
+        pencil_dispatch<R>(B, k, s, b, first, last, cout);
+        return;
+    }
     const BTile* t = B->d_tiles + s.offset;
 #define UP(L, C) if (k.law == L && k.ct == C) { kb_update<L, R, (C != 0)><<<s.count, 256, 0, B->stream>>>(B->d_items, t, b, first, last, cout); return; }
     UP(0, 0) UP(1, 0) UP(1, 1) UP(2, 0) UP(2, 1)
@@ -271,9 +290,6 @@ void run_stage(mara_batch* B, int rk)
     const int last = rk == B->rk_order - 1;
     for (int axis = 0; axis < 3; ++axis)
         for (auto& e : B->flux[axis]) flux_dispatch<R>(B, B->classes[e.first], e.second, axis, rk);
-    for (int axis = 0; axis < 3; ++axis)
-        for (auto& e : B->ct[axis])
-            kb_ct<R><<<e.second.count, 256, 0, B->stream>>>(B->d_items, B->d_tiles + e.second.offset, axis);
     for (auto& e : B->update) update_dispatch<R>(B, B->classes[e.first], e.second, bb[rk], rk == 0, last, last);
     // ghost zones are written by the zone update (write_guard_images)
 }
@@ -400,7 +416,6 @@ int mara_batch_admit(mara_batch* B, const mara_desc* d, const double* P, double 
     const int real = B->precision == MARA_PREC_MIXED ? 4 : 8;
     for (int a = 0; a < 3; ++a)
         if (it.G.N[a] > 1) { it.F[a] = p; p += round_up(it.G.nq * it.G.ffs * real, 256); }
-    if (d->law != MARA_LAW_HYDRO && d->field_ct) { it.Gct = (double*)p; p += round_up(9 * it.G.ffs * 8, 256); }
     it.ctl = B->d_ctl + n;
     it.bc = d->bc;
 

@@ -23,10 +23,9 @@ struct BItem
     double* Pb;
     double* U;
     void* F[3];
-    double* Gct;
     Ctl* ctl;
     int bc;
-    int pad[3];
+    int pad;
 };
 
 struct BTile
@@ -62,18 +61,6 @@ kb_flux(const BItem* __restrict__ items, const BTile* __restrict__ tiles, int ax
     flux_face<Law, Recon, Riem, R>(Pin, (R*)it.F[axis], it.G, axis, i, j, k, it.ctl);
 }
 
-template <typename R>
-__global__ void __launch_bounds__(256)
-kb_ct(const BItem* __restrict__ items, const BTile* __restrict__ tiles, int axis)
-{
-    const BTile T = tiles[blockIdx.x];
-    const BItem& it = items[T.item];
-    if (!it.ctl->active) return;
-    int i, j, k;
-    if (!tile_thread(T, i, j, k)) return;
-    ct_face<R>((const R*)it.F[0], (const R*)it.F[1], (const R*)it.F[2], it.Gct, it.G, axis, i, j, k);
-}
-
 template <int Law, typename R, bool CT>
 __global__ void __launch_bounds__(256)
 kb_update(const BItem* __restrict__ items, const BTile* __restrict__ tiles, double b, int first, int last,
@@ -87,8 +74,29 @@ kb_update(const BItem* __restrict__ items, const BTile* __restrict__ tiles, doub
     double dtc = 1.79769313486231570815e+308;
     if (tile_thread(T, i, j, k))
         dtc = update_zone<Law, R, CT>(it.Pa, it.U, last ? it.Pa : it.Pb, (const R*)it.F[0], (const R*)it.F[1],
-                                      (const R*)it.F[2], it.Gct, it.G, b, first, !last, courant_out, it.ctl, i, j, k);
+                                      (const R*)it.F[2], it.G, b, first, !last, courant_out, it.ctl, i, j, k);
     if (courant_out) courant_to_bins(dtc, wmin, 4, threadIdx.x, it.ctl);
+}
+
+// 1-D items of an RK2 batch: reconstruction, fluxes and update in one pass
+// per tile of PENCIL - 1 zones (stage 1: Pa -> Pb, stage 2: Pb -> Pa).
+template <int Law, int Recon, int Riem, typename R, bool CT>
+__global__ void __launch_bounds__(PENCIL)
+kb_pencil(const BItem* __restrict__ items, const BTile* __restrict__ tiles, double b, int first, int last,
+          int courant_out)
+{
+    constexpr int NQ = PencilLds<Law, Recon>::NQ, W = PencilLds<Law, Recon>::W;
+    __shared__ R sP[NQ * W];
+    __shared__ R sF[NQ * PENCIL];
+    __shared__ double wmin[PENCIL / 64];
+    const BTile T = tiles[blockIdx.x];
+    const BItem& it = items[T.item];
+    if (!it.ctl->active) return;
+    const double* Pin = first ? it.Pa : it.Pb;
+    double* Pout = last ? it.Pa : it.Pb;
+    const double dtc = pencil_tile<Law, Recon, Riem, R, CT>(Pin, it.Pa, it.U, Pout, it.G, b, first, !last,
+                                                            courant_out, it.ctl, T.x0, sP, sF);
+    if (courant_out) courant_to_bins(dtc, wmin, PENCIL / 64, threadIdx.x, it.ctl);
 }
 
 template <int Law>

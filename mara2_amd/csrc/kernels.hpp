@@ -109,45 +109,41 @@ __device__ __forceinline__ T wave_max(T v)
 // (src/MeshOperator.cpp:443-525, src/IntercellFluxSchemes.cpp:45-117).
 // One thread per face of the box [lo, hi) of face indices.
 // ---------------------------------------------------------------------------
+template <int Recon> struct Stencil { static constexpr int NG = Recon == 0 ? 1 : Recon == 1 ? 2 : 3; };
+
+// Flux through one face from its stencil S[q][m] = P(first stencil zone + m),
+// m = 0 .. 2 NG - 1, in R (reconstruction, law state, Riemann solver).
 template <int Law, int Recon, int Riem, typename R>
-__device__ __forceinline__ void flux_face(const double* __restrict__ P, R* __restrict__ F, const Geom& G, int axis,
-                                          int i, int j, int k, Ctl* ctlw)
+__device__ __forceinline__ void face_flux(const R (&S)[LawNQ<Law>::value][2 * Stencil<Recon>::NG], const Geom& G,
+                                          int axis, R* Fq, Ctl* ctlw)
 {
     constexpr int NQ = LawNQ<Law>::value;
-    constexpr int NG = Recon == 0 ? 1 : Recon == 1 ? 2 : 3;
-    const long long stride = axis == 0 ? 1 : axis == 1 ? (long long)G.N[0] : (long long)G.N[0] * G.N[1];
-    const long long c0 = cell_index(G, i, j, k) - NG * stride;  // first stencil cell
-
+    constexpr int NG = Stencil<Recon>::NG;
     R PL[NQ], PR[NQ];
     const R theta = R(G.theta);
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
     {
-        const double* Pq = P + q * G.fs + c0;
         if constexpr (NG == 1)
         {
-            PL[q] = R(Pq[0]);
-            PR[q] = R(Pq[stride]);
+            PL[q] = S[q][0];
+            PR[q] = S[q][1];
         }
         else if constexpr (NG == 2)
         {
-            const R v0 = R(Pq[0]), v1 = R(Pq[stride]), v2 = R(Pq[2 * stride]), v3 = R(Pq[3 * stride]);
-            PL[q] = plm(theta, v0, v1, v2, R(+1.0));
-            PR[q] = plm(theta, v1, v2, v3, R(-1.0));
+            PL[q] = plm(theta, S[q][0], S[q][1], S[q][2], R(+1.0));
+            PR[q] = plm(theta, S[q][1], S[q][2], S[q][3], R(-1.0));
         }
         else
         {
-            const R v0 = R(Pq[0]), v1 = R(Pq[stride]), v2 = R(Pq[2 * stride]);
-            const R v3 = R(Pq[3 * stride]), v4 = R(Pq[4 * stride]), v5 = R(Pq[5 * stride]);
-            PL[q] = weno5<R, true>(v0, v1, v2, v3, v4);
-            PR[q] = weno5<R, false>(v1, v2, v3, v4, v5);
+            PL[q] = weno5<R, true>(S[q][0], S[q][1], S[q][2], S[q][3], S[q][4]);
+            PR[q] = weno5<R, false>(S[q][1], S[q][2], S[q][3], S[q][4], S[q][5]);
         }
     }
     const R n[3] = {R(G.nhat[3 * axis + 0]), R(G.nhat[3 * axis + 1]), R(G.nhat[3 * axis + 2])};
     State<R, NQ> SL, SR;
     const int eL = law_state<Law, R>(R(G.gamma), PL, n, SL);
     const int eR = law_state<Law, R>(R(G.gamma), PR, n, SR);
-    R Fq[NQ];
     if (eL | eR)
     {
         if ((eL | eR) & ERR_SUPERLUMINAL) atomicAdd(&ctlw->superluminal, 1ull);
@@ -159,7 +155,26 @@ __device__ __forceinline__ void flux_face(const double* __restrict__ P, R* __res
         hllc(SL, SR, n, Fq);
     else
         hlle<R, NQ>(SL, SR, Fq);
+}
 
+template <int Law, int Recon, int Riem, typename R>
+__device__ __forceinline__ void flux_face(const double* __restrict__ P, R* __restrict__ F, const Geom& G, int axis,
+                                          int i, int j, int k, Ctl* ctlw)
+{
+    constexpr int NQ = LawNQ<Law>::value;
+    constexpr int NG = Stencil<Recon>::NG;
+    const long long stride = axis == 0 ? 1 : axis == 1 ? (long long)G.N[0] : (long long)G.N[0] * G.N[1];
+    const long long c0 = cell_index(G, i, j, k) - NG * stride;  // first stencil cell
+    R S[NQ][2 * NG];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+    {
+        const double* Pq = P + q * G.fs + c0;
+#pragma unroll
+        for (int m = 0; m < 2 * NG; ++m) S[q][m] = R(Pq[m * stride]);
+    }
+    R Fq[NQ];
+    face_flux<Law, Recon, Riem, R>(S, G, axis, Fq, ctlw);
     const long long f = face_index(G, i, j, k);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) F[q * G.ffs + f] = Fq[q];
@@ -184,48 +199,23 @@ k_flux(const double* __restrict__ P, R* __restrict__ F, const Geom G, int axis,
 //   G = 1/8 (2 F_a^c(x) + F_a^c(x+e_t) + F_a^c(x-e_t)
 //            - F_t^a(x+e_t) - F_t^a(x+e_t-e_a) - F_t^a(x) - F_t^a(x-e_a))
 // where F_a^c is the flux along a of B_c.  G_a^a = 0.  When axis t is flat
-// the reference leaves F_a^c as it is.  Evaluated in binary64 in both forms.
-// G layout: (a * 3 + c) * ffs + f.
+// the reference leaves F_a^c as it is.  Evaluated in binary64 in both forms,
+// by the zone update for the faces it needs (no corrected-flux array).
 // ---------------------------------------------------------------------------
 template <typename R>
-__device__ __forceinline__ void ct_face(const R* __restrict__ Fx, const R* __restrict__ Fy, const R* __restrict__ Fz,
-                                        double* __restrict__ Gct, const Geom& G, int a, int i, int j, int k)
+__device__ __forceinline__ double ct_value(const R* __restrict__ Fx, const R* __restrict__ Fy,
+                                           const R* __restrict__ Fz, const Geom& G, int a, int c, long long f)
 {
-    const R* Fa[3] = {Fx, Fy, Fz};
+    const R* Fa = a == 0 ? Fx : a == 1 ? Fy : Fz;
+    if (G.N[c] < 2) return double(Fa[(5 + c) * G.ffs + f]);
+    const R* Fc = c == 0 ? Fx : c == 1 ? Fy : Fz;
+    const R* Fac = Fa + (5 + c) * G.ffs;  // flux along a of B_c
+    const R* Fta = Fc + (5 + a) * G.ffs;  // flux along t = c of B_a
     const long long fstride[3] = {1, (long long)(G.N[0] + 1), (long long)(G.N[0] + 1) * (G.N[1] + 1)};
-    const long long f = face_index(G, i, j, k);
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-    {
-        double val;
-        if (c == a)
-            val = 0.0;
-        else if (G.N[c] < 2)
-            val = double(Fa[a][(5 + c) * G.ffs + f]);
-        else
-        {
-            const R* Fac = Fa[a] + (5 + c) * G.ffs;  // flux along a of B_c
-            const R* Fta = Fa[c] + (5 + a) * G.ffs;  // flux along t = c of B_a
-            const long long et = fstride[c], ea = fstride[a];
-            val = 0.125 * (+2 * double(Fac[f]) + 1 * double(Fac[f + et]) + 1 * double(Fac[f - et])
-                           - 1 * double(Fta[f + et]) - 1 * double(Fta[f + et - ea])
-                           - 1 * double(Fta[f]) - 1 * double(Fta[f - ea]));
-        }
-        Gct[(a * 3 + c) * G.ffs + f] = val;
-    }
-}
-
-template <typename R>
-static __global__ void __launch_bounds__(256)
-k_ct(const R* __restrict__ Fx, const R* __restrict__ Fy, const R* __restrict__ Fz,
-     double* __restrict__ Gct, Geom G, int a, int lo0, int lo1, int lo2, int m0, int m1, int m2,
-     const Ctl* __restrict__ ctl)
-{
-    if (!ctl->active) return;
-    const int ti = blockIdx.x * blockDim.x + threadIdx.x;
-    const int tj = blockIdx.y * blockDim.y + threadIdx.y;
-    if (ti >= m0 || tj >= m1) return;
-    ct_face<R>(Fx, Fy, Fz, Gct, G, a, lo0 + ti, lo1 + tj, lo2 + (int)blockIdx.z);
+    const long long et = fstride[c], ea = fstride[a];
+    return 0.125 * (+2 * double(Fac[f]) + 1 * double(Fac[f + et]) + 1 * double(Fac[f - et])
+                    - 1 * double(Fta[f + et]) - 1 * double(Fta[f + et - ea])
+                    - 1 * double(Fta[f]) - 1 * double(Fta[f - ea]));
 }
 
 // ---------------------------------------------------------------------------
@@ -354,18 +344,17 @@ __device__ __forceinline__ void write_guard_images(double* __restrict__ P, const
 // recoverPrimitive (src/FieldOperator.cpp:41-82) on interior zones.
 // U0 is regenerated from the step's starting primitives P0.
 // ---------------------------------------------------------------------------
-template <int Law, typename R, bool CT>
-__device__ __forceinline__ double update_zone(const double* __restrict__ P0, double* __restrict__ U, double* __restrict__ Pout,
-                                              const R* __restrict__ Fx, const R* __restrict__ Fy, const R* __restrict__ Fz,
-                                              const double* __restrict__ Gct, const Geom& G, double b, int first_stage,
-                                              int write_u, int courant_out, Ctl* ctl, int i, int j, int k)
+// The zone update once the flux divergence L[q] is known.  Returns the
+// zone's CFL time step when courant_out is set (else DBL_MAX).
+template <int Law>
+__device__ __forceinline__ double finish_zone(const double* __restrict__ P0, double* __restrict__ U,
+                                              double* __restrict__ Pout, const Geom& G, double b, int first_stage,
+                                              int write_u, int courant_out, Ctl* ctl, int i, int j, int k,
+                                              const double* L)
 {
     double dtc = 1.79769313486231570815e+308;
     constexpr int NQ = LawNQ<Law>::value;
     const long long c = cell_index(G, i, j, k);
-    const long long f = face_index(G, i, j, k);
-    const long long fx = f + 1, fy = f + (G.N[0] + 1), fz = f + (long long)(G.N[0] + 1) * (G.N[1] + 1);
-    const bool hx = G.N[0] > 1, hy = G.N[1] > 1, hz = G.N[2] > 1;
     const double dt = ctl->dt;
 
     double p0[NQ], U0[NQ];
@@ -373,33 +362,12 @@ __device__ __forceinline__ double update_zone(const double* __restrict__ P0, dou
     for (int q = 0; q < NQ; ++q) p0[q] = P0[q * G.fs + c];
     if (prim_to_cons<Law>(G.gamma, p0, U0) != ERR_NONE) atomicAdd(&ctl->superluminal, 1ull);
 
-    const double fac = -1.0 / G.dV;
     double Un[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
     {
-        double L;
-        if (CT && q >= 5)
-        {
-            const int cc = q - 5;
-            const double* Gx = Gct + (0 * 3 + cc) * G.ffs;
-            const double* Gy = Gct + (1 * 3 + cc) * G.ffs;
-            const double* Gz = Gct + (2 * 3 + cc) * G.ffs;
-            const double x1 = hx && cc != 0 ? Gx[fx] : 0.0, x0 = hx && cc != 0 ? Gx[f] : 0.0;
-            const double y1 = hy && cc != 1 ? Gy[fy] : 0.0, y0 = hy && cc != 1 ? Gy[f] : 0.0;
-            const double z1 = hz && cc != 2 ? Gz[fz] : 0.0, z0 = hz && cc != 2 ? Gz[f] : 0.0;
-            L = fac * (+x1 * G.dA[0] - x0 * G.dA[0] + y1 * G.dA[1] - y0 * G.dA[1] + z1 * G.dA[2] - z0 * G.dA[2]);
-        }
-        else
-        {
-            const R A0 = R(G.dA[0]), A1 = R(G.dA[1]), A2 = R(G.dA[2]);
-            const R x1 = hx ? Fx[q * G.ffs + fx] : R(0), x0 = hx ? Fx[q * G.ffs + f] : R(0);
-            const R y1 = hy ? Fy[q * G.ffs + fy] : R(0), y0 = hy ? Fy[q * G.ffs + f] : R(0);
-            const R z1 = hz ? Fz[q * G.ffs + fz] : R(0), z0 = hz ? Fz[q * G.ffs + f] : R(0);
-            L = double(R(fac) * (+x1 * A0 - x0 * A0 + y1 * A1 - y0 * A1 + z1 * A2 - z0 * A2));
-        }
         const double Uq = first_stage ? U0[q] : U[q * G.fs + c];
-        Un[q] = U0[q] * (1 - b) + (Uq + dt * L) * b;
+        Un[q] = U0[q] * (1 - b) + (Uq + dt * L[q]) * b;
     }
     double Pn[NQ];
     int floor = 0;
@@ -436,11 +404,143 @@ __device__ __forceinline__ double update_zone(const double* __restrict__ P0, dou
     return dtc;
 }
 
+// L = -div F of one field (src/MeshOperator.cpp:366-372), in R for the
+// Godunov fluxes; the flux-CT magnetic terms in binary64.
+template <typename R>
+__device__ __forceinline__ double divergence_R(const Geom& G, R x1, R x0, R y1, R y0, R z1, R z0)
+{
+    const R A0 = R(G.dA[0]), A1 = R(G.dA[1]), A2 = R(G.dA[2]);
+    const double fac = -1.0 / G.dV;
+    return double(R(fac) * (+x1 * A0 - x0 * A0 + y1 * A1 - y0 * A1 + z1 * A2 - z0 * A2));
+}
+
+__device__ __forceinline__ double divergence_ct(const Geom& G, double x1, double x0, double y1, double y0,
+                                                double z1, double z0)
+{
+    const double fac = -1.0 / G.dV;
+    return fac * (+x1 * G.dA[0] - x0 * G.dA[0] + y1 * G.dA[1] - y0 * G.dA[1] + z1 * G.dA[2] - z0 * G.dA[2]);
+}
+
+// Zone update from the face-flux arrays: L = -div F (src/MeshOperator.cpp:323-381),
+// U = U0 (1-b) + (U + dt L) b (src/SolutionSchemes.cpp:232-235), and
+// recoverPrimitive (src/FieldOperator.cpp:41-82) on interior zones.
+// U0 is regenerated from the step's starting primitives P0.
+template <int Law, typename R, bool CT>
+__device__ __forceinline__ double update_zone(const double* __restrict__ P0, double* __restrict__ U, double* __restrict__ Pout,
+                                              const R* __restrict__ Fx, const R* __restrict__ Fy, const R* __restrict__ Fz,
+                                              const Geom& G, double b, int first_stage,
+                                              int write_u, int courant_out, Ctl* ctl, int i, int j, int k)
+{
+    constexpr int NQ = LawNQ<Law>::value;
+    const long long f = face_index(G, i, j, k);
+    const long long fx = f + 1, fy = f + (G.N[0] + 1), fz = f + (long long)(G.N[0] + 1) * (G.N[1] + 1);
+    const bool hx = G.N[0] > 1, hy = G.N[1] > 1, hz = G.N[2] > 1;
+    double L[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+    {
+        if (CT && q >= 5)
+        {
+            const int cc = q - 5;
+            const double x1 = hx && cc != 0 ? ct_value(Fx, Fy, Fz, G, 0, cc, fx) : 0.0;
+            const double x0 = hx && cc != 0 ? ct_value(Fx, Fy, Fz, G, 0, cc, f) : 0.0;
+            const double y1 = hy && cc != 1 ? ct_value(Fx, Fy, Fz, G, 1, cc, fy) : 0.0;
+            const double y0 = hy && cc != 1 ? ct_value(Fx, Fy, Fz, G, 1, cc, f) : 0.0;
+            const double z1 = hz && cc != 2 ? ct_value(Fx, Fy, Fz, G, 2, cc, fz) : 0.0;
+            const double z0 = hz && cc != 2 ? ct_value(Fx, Fy, Fz, G, 2, cc, f) : 0.0;
+            L[q] = divergence_ct(G, x1, x0, y1, y0, z1, z0);
+        }
+        else
+        {
+            const R x1 = hx ? Fx[q * G.ffs + fx] : R(0), x0 = hx ? Fx[q * G.ffs + f] : R(0);
+            const R y1 = hy ? Fy[q * G.ffs + fy] : R(0), y0 = hy ? Fy[q * G.ffs + f] : R(0);
+            const R z1 = hz ? Fz[q * G.ffs + fz] : R(0), z0 = hz ? Fz[q * G.ffs + f] : R(0);
+            L[q] = divergence_R<R>(G, x1, x0, y1, y0, z1, z0);
+        }
+    }
+    return finish_zone<Law>(P0, U, Pout, G, b, first_stage, write_u, courant_out, ctl, i, j, k, L);
+}
+
+// ---------------------------------------------------------------------------
+// 1-D pencil tile: one work-group of PENCIL threads computes the fluxes of
+// PENCIL consecutive x-faces from a stencil staged in LDS and updates the
+// PENCIL - 1 zones between them, with the same per-face and per-zone
+// arithmetic as the flux-array path (face_flux, divergence_*, finish_zone),
+// so the face fluxes never go through memory.  Input and output must be
+// different arrays (the tile reads neighbouring zones of Pin).
+// ---------------------------------------------------------------------------
+constexpr int PENCIL = 256;
+
+template <int Law, int Recon> struct PencilLds
+{
+    static constexpr int NQ = LawNQ<Law>::value;
+    static constexpr int W = PENCIL + 2 * Stencil<Recon>::NG - 1;  // staged zones per tile
+};
+
+template <int Law, int Recon, int Riem, typename R, bool CT>
+__device__ __forceinline__ double pencil_tile(const double* __restrict__ Pin, const double* __restrict__ P0,
+                                              double* __restrict__ U, double* __restrict__ Pout, const Geom& G,
+                                              double b, int first_stage, int write_u, int courant_out, Ctl* ctl,
+                                              int x0, R* __restrict__ sP, R* __restrict__ sF)
+{
+    constexpr int NQ = LawNQ<Law>::value;
+    constexpr int NG = Stencil<Recon>::NG;
+    constexpr int W = PencilLds<Law, Recon>::W;
+    const int t = threadIdx.x;
+    const int ie = G.N[0] - G.g[0];  // one past the last interior zone
+    // zones x0 - NG .. x0 + PENCIL + NG - 2 (clamped to the array; the clamped
+    // ones feed no computed face)
+    for (int m = t; m < W; m += PENCIL)
+    {
+        const int zi = min(x0 - NG + m, G.N[0] - 1);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) sP[q * W + m] = R(Pin[q * G.fs + zi]);
+    }
+    __syncthreads();
+    if (x0 + t <= ie)  // face x0 + t, between zones x0 + t - 1 and x0 + t
+    {
+        R S[NQ][2 * NG];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+        {
+#pragma unroll
+            for (int m = 0; m < 2 * NG; ++m) S[q][m] = sP[q * W + t + m];
+        }
+        R Fq[NQ];
+        face_flux<Law, Recon, Riem, R>(S, G, 0, Fq, ctl);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) sF[q * PENCIL + t] = Fq[q];
+    }
+    __syncthreads();
+    double dtc = 1.79769313486231570815e+308;
+    const int zi = x0 + t;
+    if (t < PENCIL - 1 && zi < ie)
+    {
+        double L[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+        {
+            const R fr = sF[q * PENCIL + t + 1], fl = sF[q * PENCIL + t];
+            if (CT && q >= 5)
+            {
+                // flux-CT on a 1-D mesh: the transverse axes are flat, so the
+                // reference keeps F for B_y, B_z and zeroes it for B_x
+                const double x1 = q != 5 ? double(fr) : 0.0, xl = q != 5 ? double(fl) : 0.0;
+                L[q] = divergence_ct(G, x1, xl, 0.0, 0.0, 0.0, 0.0);
+            }
+            else
+                L[q] = divergence_R<R>(G, fr, fl, R(0), R(0), R(0), R(0));
+        }
+        dtc = finish_zone<Law>(P0, U, Pout, G, b, first_stage, write_u, courant_out, ctl, zi, 0, 0, L);
+    }
+    return dtc;
+}
+
 template <int Law, typename R, bool CT>
 static __global__ void __launch_bounds__(256)
 k_update(const double* __restrict__ P0, double* __restrict__ U, double* __restrict__ Pout,
          const R* __restrict__ Fx, const R* __restrict__ Fy, const R* __restrict__ Fz,
-         const double* __restrict__ Gct, Geom G, double b, int first_stage, int write_u,
+         Geom G, double b, int first_stage, int write_u,
          int courant_out, Ctl* ctl)
 {
     __shared__ double wmin[4];
@@ -449,7 +549,7 @@ k_update(const double* __restrict__ P0, double* __restrict__ U, double* __restri
     const int tj = blockIdx.y * blockDim.y + threadIdx.y;
     double dtc = 1.79769313486231570815e+308;
     if (ti < G.n[0] && tj < G.n[1])
-        dtc = update_zone<Law, R, CT>(P0, U, Pout, Fx, Fy, Fz, Gct, G, b, first_stage, write_u, courant_out, ctl,
+        dtc = update_zone<Law, R, CT>(P0, U, Pout, Fx, Fy, Fz, G, b, first_stage, write_u, courant_out, ctl,
                                       G.g[0] + ti, G.g[1] + tj, G.g[2] + (int)blockIdx.z);
     if (courant_out) courant_to_bins(dtc, wmin, 4, threadIdx.y * blockDim.x + threadIdx.x, ctl);
 }

@@ -285,7 +285,13 @@ __device__ __forceinline__ void mhd_state(R gamma, const R* P, const R* dAA, Sta
     const R ca2 = Math<R>::div(BB, P[0]);
     const R cn2 = Math<R>::div(Bn2, P[0]);
     const R cw4 = (cs2 + ca2) * (cs2 + ca2);
-    const R root = Math<R>::sqrt(cw4 - R(4) * cs2 * cn2);
+    // (cs2 + ca2)^2 - 4 cs2 cn2 >= (cs2 - ca2)^2 >= 0 since cn2 <= ca2; in
+    // binary32 it can round below zero where cs2 ~ ca2 and B is along the
+    // normal, and the square root would then poison the fast speed.  The
+    // mixed form clamps it; binary64 keeps the reference's expression.
+    R disc = cw4 - R(4) * cs2 * cn2;
+    if constexpr (sizeof(R) == 4) disc = disc < R(0) ? R(0) : disc;
+    const R root = Math<R>::sqrt(disc);
     const R cF2 = R(0.5) * (cs2 + ca2 + root);
     const R cS2 = R(0.5) * (cs2 + ca2 - root);
     const R sF = Math<R>::sqrt(cF2), sA = Math<R>::sqrt(cn2), sS = Math<R>::sqrt(cS2);
