@@ -148,12 +148,14 @@ int mara_block_monopole_max(mara_block* block, double* max_abs);
 int mara_block_run(mara_block* block, double cfl, double t_final, int max_steps,
                    double* t, int* steps, mara_report* report);
 
-/* Kernel timing on the block's stream (HIP events around each launch of
-   the dominant per-stage kernel).  enable=1 starts a fresh tally. */
+/* Kernel timing on the block's stream: a pair of HIP events around each
+   launch of the stage kernels (the fused kernel, or the flux and update
+   kernels), recorded without host waits, for up to 512 stages.
+   enable=1 starts a fresh tally. */
 int mara_block_set_timing(mara_block* block, int enable);
 /* Average duration in milliseconds and launch count of the stage kernels
-   since timing was enabled; *name receives a static string naming the
-   kernel.  Requires a synchronised stream (every entry point returns so). */
+   since timing was enabled (waits for the last recorded event); *name
+   receives a static string naming the kernel. */
 int mara_block_stage_kernel_time(mara_block* block, double* avg_ms, long long* launches,
                                  const char** name);
 

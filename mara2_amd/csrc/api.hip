@@ -14,6 +14,7 @@
 #include "kernels.hpp"
 #include "fused.hpp"
 #include "host_common.hpp"
+#include "stage.hpp"
 
 using namespace mara;
 using namespace mara::host;
@@ -87,106 +88,24 @@ struct mara_block
     Ctl* ctl_host = nullptr;
     unsigned long long* cbins = nullptr;
     long long bytes = 0;
+    // stage-kernel timing: event pairs recorded around each stage launch with
+    // no host wait in between (the GPU stays busy), summed when read
+    static constexpr int kTimingPairs = 512;
     bool timing = false;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev[2 * kTimingPairs] = {};
+    int ev_used = 0;
     double timed_ms = 0.0;
     long long timed_launches = 0;
 };
 
 namespace {
 
-// ---------------------------------------------------------------------------
-// template dispatch
-// ---------------------------------------------------------------------------
-template <int Law, int Recon, int Riem, typename R>
-void launch_flux_t(mara_block* b, const double* Pin, int axis)
-{
-    const Geom& G = b->G;
-    int lo[3], hi[3];
-    for (int a = 0; a < 3; ++a)
-    {
-        if (a == axis) { lo[a] = G.g[a]; hi[a] = G.N[a] - G.g[a] + 1; }
-        else
-        {
-            const int ring = (b->S.ct && G.N[a] > 1) ? 1 : 0;
-            lo[a] = G.g[a] - ring;
-            hi[a] = G.N[a] - G.g[a] + ring;
-        }
-    }
-    const int m0 = hi[0] - lo[0], m1 = hi[1] - lo[1], m2 = hi[2] - lo[2];
-    k_flux<Law, Recon, Riem, R><<<grid3(m0, m1, m2), block3(), 0, b->stream>>>(
-        Pin, (R*)b->F[axis], G, axis, lo[0], lo[1], lo[2], m0, m1, m2, b->ctl);
-}
-
-template <int Law, typename R>
-void launch_flux_law(mara_block* b, const double* Pin, int axis)
-{
-    const int recon = b->d.recon, riem = b->d.riemann;
-    if (recon == MARA_RECON_PCM)
-    {
-        if (Law == 0 && riem == 1) launch_flux_t<Law, 0, 1, R>(b, Pin, axis);
-        else launch_flux_t<Law, 0, 0, R>(b, Pin, axis);
-    }
-    else if (recon == MARA_RECON_PLM)
-    {
-        if (Law == 0 && riem == 1) launch_flux_t<Law, 1, 1, R>(b, Pin, axis);
-        else launch_flux_t<Law, 1, 0, R>(b, Pin, axis);
-    }
-    else
-    {
-        if (Law == 0 && riem == 1) launch_flux_t<Law, 2, 1, R>(b, Pin, axis);
-        else launch_flux_t<Law, 2, 0, R>(b, Pin, axis);
-    }
-}
-
-template <typename R>
-void launch_flux(mara_block* b, const double* Pin, int axis)
-{
-    switch (b->d.law)
-    {
-        case 0: launch_flux_law<0, R>(b, Pin, axis); break;
-        case 1: launch_flux_law<1, R>(b, Pin, axis); break;
-        default: launch_flux_law<2, R>(b, Pin, axis); break;
-    }
-}
-
-template <int Law, typename R>
-void launch_update_law(mara_block* b, double* Pout, double bk, int first, int write_u, int cout)
-{
-    const Geom& G = b->G;
-    const dim3 grid = grid3(G.n[0], G.n[1], G.n[2]);
-    if (b->S.ct)
-        k_update<Law, R, true><<<grid, block3(), 0, b->stream>>>(
-            b->Pa, b->U, Pout, (const R*)b->F[0], (const R*)b->F[1], (const R*)b->F[2],
-            G, bk, first, write_u, cout, b->ctl);
-    else
-        k_update<Law, R, false><<<grid, block3(), 0, b->stream>>>(
-            b->Pa, b->U, Pout, (const R*)b->F[0], (const R*)b->F[1], (const R*)b->F[2],
-            G, bk, first, write_u, cout, b->ctl);
-}
-
-template <typename R>
-void launch_update(mara_block* b, double* Pout, double bk, int first, int write_u, int cout)
-{
-    switch (b->d.law)
-    {
-        case 0: launch_update_law<0, R>(b, Pout, bk, first, write_u, cout); break;
-        case 1: launch_update_law<1, R>(b, Pout, bk, first, write_u, cout); break;
-        default: launch_update_law<2, R>(b, Pout, bk, first, write_u, cout); break;
-    }
-}
-
 void launch_bc(mara_block* b, double* P)
 {
     const Geom& G = b->G;
-    for (int axis = 0; axis < 3; ++axis)
-    {
-        if (G.g[axis] <= 0) continue;
-        const int t1 = axis == 0 ? 1 : 0;
-        const int t2 = axis == 2 ? 1 : 2;
-        const long long total = 2ll * G.g[axis] * G.N[t1] * G.N[t2];
-        k_bc<<<grid_for(total), kThreads, 0, b->stream>>>(P, G, axis, b->d.bc, b->ctl);
-    }
+    const long long total = 2ll * G.g[0] * G.N[1] * G.N[2] + (long long)G.n[0] * 2 * G.g[1] * G.N[2]
+                            + (long long)G.n[0] * G.n[1] * 2 * G.g[2];
+    if (total > 0) k_ghost<<<grid_for(total), kThreads, 0, b->stream>>>(P, G, total, b->ctl);
 }
 
 void launch_courant(mara_block* b)
@@ -226,30 +145,30 @@ void launch_stage(mara_block* b, int rk, int courant_out)
         Pout = order == 1 ? b->Pb : (last ? b->Pa : (rk == 0 ? b->Pb : b->Pc));
     }
 
-    if (b->timing) hipEventRecord(b->ev0, b->stream);
-    if (b->fused)
+    const bool timed = b->timing && b->ev_used < mara_block::kTimingPairs;
+    if (timed) hipEventRecord(b->ev[2 * b->ev_used], b->stream);
+    StageArgs s;
+    s.d = &b->d;
+    s.G = &b->G;
+    s.stream = b->stream;
+    s.Pin = Pin;
+    s.P0 = b->Pa;
+    s.U = b->U;
+    s.Pout = Pout;
+    for (int a = 0; a < 3; ++a) s.F[a] = b->F[a];
+    s.b = bb[rk];
+    s.first = rk == 0;
+    s.write_u = !last;
+    s.cout = last && courant_out;
+    s.ctl = b->ctl;
+    s.fused = b->fused;
+    s.ct = b->S.ct;
+    if (mixed) stage_kernels_f32(s);
+    else stage_kernels<double>(s);
+    if (timed)
     {
-        launch_fused(b->d, b->G, b->stream, Pin, b->Pa, b->U, Pout, bb[rk], rk == 0, !last, last && courant_out, b->ctl);
-    }
-    else
-    {
-        for (int axis = 0; axis < 3; ++axis)
-        {
-            if (b->G.N[axis] < 2) continue;
-            if (mixed) launch_flux<float>(b, Pin, axis);
-            else launch_flux<double>(b, Pin, axis);
-        }
-        if (mixed) launch_update<float>(b, Pout, bb[rk], rk == 0, !last, last && courant_out);
-        else launch_update<double>(b, Pout, bb[rk], rk == 0, !last, last && courant_out);
-    }
-    if (b->timing)
-    {
-        hipEventRecord(b->ev1, b->stream);
-        hipEventSynchronize(b->ev1);
-        float ms = 0;
-        hipEventElapsedTime(&ms, b->ev0, b->ev1);
-        b->timed_ms += ms;
-        b->timed_launches += 1;
+        hipEventRecord(b->ev[2 * b->ev_used + 1], b->stream);
+        b->ev_used += 1;
     }
     // the flux-array update writes the ghost zones of Pout itself
     // (write_guard_images); the fused kernels leave them to a ghost-fill pass
@@ -372,8 +291,8 @@ int mara_block_destroy(mara_block* b)
     hipFree(b->ctl);
     hipFree(b->cbins);
     if (b->ctl_host) hipHostFree(b->ctl_host);
-    if (b->ev0) hipEventDestroy(b->ev0);
-    if (b->ev1) hipEventDestroy(b->ev1);
+    for (int e = 0; e < 2 * mara_block::kTimingPairs; ++e)
+        if (b->ev[e]) hipEventDestroy(b->ev[e]);
     if (b->stream) hipStreamDestroy(b->stream);
     delete b;
     return MARA_OK;
@@ -494,12 +413,10 @@ int mara_block_run(mara_block* b, double cfl, double t_final, int max_steps,
 int mara_block_set_timing(mara_block* b, int enable)
 {
     if (!b) return MARA_ERR_INVALID_ARGUMENT;
-    if (enable && !b->ev0)
-    {
-        HIP_TRY(hipEventCreate(&b->ev0));
-        HIP_TRY(hipEventCreate(&b->ev1));
-    }
+    if (enable && !b->ev[0])
+        for (int e = 0; e < 2 * mara_block::kTimingPairs; ++e) HIP_TRY(hipEventCreate(&b->ev[e]));
     b->timing = enable != 0;
+    b->ev_used = 0;
     b->timed_ms = 0.0;
     b->timed_launches = 0;
     return MARA_OK;
@@ -508,6 +425,18 @@ int mara_block_set_timing(mara_block* b, int enable)
 int mara_block_stage_kernel_time(mara_block* b, double* avg_ms, long long* launches, const char** name)
 {
     if (!b) return MARA_ERR_INVALID_ARGUMENT;
+    if (b->ev_used > 0)
+    {
+        HIP_TRY(hipEventSynchronize(b->ev[2 * b->ev_used - 1]));
+        for (int p = 0; p < b->ev_used; ++p)
+        {
+            float ms = 0;
+            HIP_TRY(hipEventElapsedTime(&ms, b->ev[2 * p], b->ev[2 * p + 1]));
+            b->timed_ms += ms;
+            b->timed_launches += 1;
+        }
+        b->ev_used = 0;
+    }
     if (avg_ms) *avg_ms = b->timed_launches ? b->timed_ms / b->timed_launches : 0.0;
     if (launches) *launches = b->timed_launches;
     if (name) *name = b->fused ? fused_kernel_name(b->d) : "k_flux+k_update (flux-array stage)";

@@ -11,116 +11,13 @@
 #include <vector>
 
 #include "../../include/mara_gpu.h"
-#include "batch.hpp"
-#include "host_common.hpp"
+#include "batch_state.hpp"
 
 using namespace mara;
 using namespace mara::host;
-
-namespace {
+using namespace mara::batch;
 
 #define HIP_TRY(call) do { if (!hip_ok(call)) return MARA_ERR_HIP; } while (0)
-
-// first-fit allocator over one device allocation
-struct Pool
-{
-    char* base = nullptr;
-    long long size = 0;
-    std::map<long long, long long> free_;  // offset -> size
-
-    void init(char* b, long long n) { base = b; size = n; free_.clear(); free_[0] = n; }
-    long long available() const
-    {
-        long long s = 0;
-        for (auto& f : free_) s += f.second;
-        return s;
-    }
-    long long alloc(long long n)
-    {
-        n = round_up(n, 256);
-        for (auto it = free_.begin(); it != free_.end(); ++it)
-        {
-            if (it->second >= n)
-            {
-                const long long off = it->first, rest = it->second - n;
-                free_.erase(it);
-                if (rest > 0) free_[off + n] = rest;
-                return off;
-            }
-        }
-        return -1;
-    }
-    void release(long long off, long long n)
-    {
-        n = round_up(n, 256);
-        auto it = free_.emplace(off, n).first;
-        auto next = std::next(it);
-        if (next != free_.end() && it->first + it->second == next->first)
-        {
-            it->second += next->second;
-            free_.erase(next);
-        }
-        if (it != free_.begin())
-        {
-            auto prev = std::prev(it);
-            if (prev->first + prev->second == it->first)
-            {
-                prev->second += it->second;
-                free_.erase(it);
-            }
-        }
-    }
-};
-
-struct Slot
-{
-    bool used = false;
-    bool fresh = false;
-    mara_desc d;
-    int cls = -1;
-    long long off = 0, bytes = 0;
-    long long ncell = 0;
-};
-
-// solve class: everything that selects a kernel template
-struct ClassKey
-{
-    int law, recon, riemann, ct;
-    int pencil;  // 1-D items of an RK2 batch: fused pencil tiles
-    bool operator<(const ClassKey& o) const
-    {
-        return std::tie(law, recon, riemann, ct, pencil) < std::tie(o.law, o.recon, o.riemann, o.ct, o.pencil);
-    }
-};
-
-struct Segment { int offset, count; };
-
-} // namespace
-
-struct mara_batch
-{
-    int rk_order = 2;
-    int precision = MARA_PREC_FP64;
-    int max_solves = 0;
-    hipStream_t stream = nullptr;
-    Pool pool;
-    char* pool_mem = nullptr;
-    BItem* d_items = nullptr;
-    Ctl* d_ctl = nullptr;
-    unsigned long long* d_bins = nullptr;
-    BTile* d_tiles = nullptr;
-    int tile_capacity = 0;
-    std::vector<BItem> items;
-    std::vector<Slot> slots;
-    Ctl* h_ctl = nullptr;  // pinned mirror
-    std::vector<ClassKey> classes;
-    bool dirty = true;
-    int n_items = 0;  // highest used slot + 1
-    // launch table
-    std::vector<BTile> tiles;
-    std::map<int, Segment> flux[3];  // class -> segment
-    std::map<int, Segment> update;
-};
 
 namespace {
 
@@ -229,69 +126,6 @@ void build_table(mara_batch* B)
         }
         if ((int)B->tiles.size() > s0) B->update[(int)c] = Segment{s0, (int)B->tiles.size() - s0};
     }
-}
-
-// ---- template dispatch over a solve class -------------------------------------
-template <int Law, int Recon, int Riem, typename R>
-void flux_launch(mara_batch* B, const Segment& s, int axis, int rk)
-{
-    kb_flux<Law, Recon, Riem, R><<<s.count, 256, 0, B->stream>>>(B->d_items, B->d_tiles + s.offset, axis, rk);
-}
-
-template <typename R>
-void flux_dispatch(mara_batch* B, const ClassKey& k, const Segment& s, int axis, int rk)
-{
-#define FL(L, Rc, Rm) if (k.law == L && k.recon == Rc && k.riemann == Rm) { flux_launch<L, Rc, Rm, R>(B, s, axis, rk); return; }
-    FL(0, 0, 0) FL(0, 0, 1) FL(0, 1, 0) FL(0, 1, 1) FL(0, 2, 0) FL(0, 2, 1)
-    FL(1, 0, 0) FL(1, 1, 0) FL(1, 2, 0)
-    FL(2, 0, 0) FL(2, 1, 0) FL(2, 2, 0)
-#undef FL
-}
-
-template <int Law, int Recon, int Riem, typename R>
-void pencil_launch(mara_batch* B, const ClassKey& k, const BTile* t, int count, double b, int first, int last, int cout)
-{
-    if (k.ct) kb_pencil<Law, Recon, Riem, R, true><<<count, PENCIL, 0, B->stream>>>(B->d_items, t, b, first, last, cout);
-    else kb_pencil<Law, Recon, Riem, R, false><<<count, PENCIL, 0, B->stream>>>(B->d_items, t, b, first, last, cout);
-}
-
-template <typename R>
-void pencil_dispatch(mara_batch* B, const ClassKey& k, const Segment& s, double b, int first, int last, int cout)
-{
-    const BTile* t = B->d_tiles + s.offset;
-#define PN(L, Rc, Rm) if (k.law == L && k.recon == Rc && k.riemann == Rm) { pencil_launch<L, Rc, Rm, R>(B, k, t, s.count, b, first, last, cout); return; }
-    PN(0, 0, 0) PN(0, 0, 1) PN(0, 1, 0) PN(0, 1, 1) PN(0, 2, 0) PN(0, 2, 1)
-    PN(1, 0, 0) PN(1, 1, 0) PN(1, 2, 0)
-    PN(2, 0, 0) PN(2, 1, 0) PN(2, 2, 0)
-#undef PN
-}
-
-template <typename R>
-void update_dispatch(mara_batch* B, const ClassKey& k, const Segment& s, double b, int first, int last, int cout)
-{
-    if (k.pencil)
-    {
-        pencil_dispatch<R>(B, k, s, b, first, last, cout);
-        return;
-    }
-    const BTile* t = B->d_tiles + s.offset;
-#define UP(L, C) if (k.law == L && k.ct == C) { kb_update<L, R, (C != 0)><<<s.count, 256, 0, B->stream>>>(B->d_items, t, b, first, last, cout); return; }
-    UP(0, 0) UP(1, 0) UP(1, 1) UP(2, 0) UP(2, 1)
-#undef UP
-}
-
-template <typename R>
-void run_stage(mara_batch* B, int rk)
-{
-    static const double b1[1] = {1.0};
-    static const double b2[2] = {1.0, 0.5};
-    static const double b3[3] = {1.0, 1. / 4, 2. / 3};
-    const double* bb = B->rk_order == 1 ? b1 : B->rk_order == 2 ? b2 : b3;
-    const int last = rk == B->rk_order - 1;
-    for (int axis = 0; axis < 3; ++axis)
-        for (auto& e : B->flux[axis]) flux_dispatch<R>(B, B->classes[e.first], e.second, axis, rk);
-    for (auto& e : B->update) update_dispatch<R>(B, B->classes[e.first], e.second, bb[rk], rk == 0, last, last);
-    // ghost zones are written by the zone update (write_guard_images)
 }
 
 __global__ void kb_reset_bins(Ctl* ctl) { reset_bins(ctl); }
@@ -489,7 +323,7 @@ int mara_batch_step(mara_batch* B, int n_steps, int* n_active)
         kb_step_dt<<<ig, 64, 0, B->stream>>>(B->d_items, n);
         for (int rk = 0; rk < B->rk_order; ++rk)
         {
-            if (B->precision == MARA_PREC_MIXED) run_stage<float>(B, rk);
+            if (B->precision == MARA_PREC_MIXED) run_stage_f32(B, rk);
             else run_stage<double>(B, rk);
         }
         kb_step_end<<<ig, 64, 0, B->stream>>>(B->d_items, n);

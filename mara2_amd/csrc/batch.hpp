@@ -49,7 +49,7 @@ __device__ __forceinline__ bool tile_thread(const BTile& T, int& i, int& j, int&
 }
 
 template <int Law, int Recon, int Riem, typename R>
-__global__ void __launch_bounds__(256)
+static __global__ void __launch_bounds__(256)
 kb_flux(const BItem* __restrict__ items, const BTile* __restrict__ tiles, int axis, int rk)
 {
     const BTile T = tiles[blockIdx.x];
@@ -62,7 +62,7 @@ kb_flux(const BItem* __restrict__ items, const BTile* __restrict__ tiles, int ax
 }
 
 template <int Law, typename R, bool CT>
-__global__ void __launch_bounds__(256)
+static __global__ void __launch_bounds__(256)
 kb_update(const BItem* __restrict__ items, const BTile* __restrict__ tiles, double b, int first, int last,
           int courant_out)
 {
@@ -81,7 +81,7 @@ kb_update(const BItem* __restrict__ items, const BTile* __restrict__ tiles, doub
 // 1-D items of an RK2 batch: reconstruction, fluxes and update in one pass
 // per tile of PENCIL - 1 zones (stage 1: Pa -> Pb, stage 2: Pb -> Pa).
 template <int Law, int Recon, int Riem, typename R, bool CT>
-__global__ void __launch_bounds__(PENCIL)
+static __global__ void __launch_bounds__(PENCIL)
 kb_pencil(const BItem* __restrict__ items, const BTile* __restrict__ tiles, double b, int first, int last,
           int courant_out)
 {
@@ -100,7 +100,7 @@ kb_pencil(const BItem* __restrict__ items, const BTile* __restrict__ tiles, doub
 }
 
 template <int Law>
-__global__ void __launch_bounds__(256)
+static __global__ void __launch_bounds__(256)
 kb_courant(const BItem* __restrict__ items, const BTile* __restrict__ tiles)
 {
     __shared__ double wmin[4];
@@ -117,7 +117,7 @@ kb_courant(const BItem* __restrict__ items, const BTile* __restrict__ tiles)
 }
 
 // per-item control, one thread per item
-__global__ void kb_step_begin(const BItem* __restrict__ items, int n)
+static __global__ void kb_step_begin(const BItem* __restrict__ items, int n)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
@@ -127,7 +127,7 @@ __global__ void kb_step_begin(const BItem* __restrict__ items, int n)
     ctl->active = go ? 1 : 0;
 }
 
-__global__ void kb_step_dt(const BItem* __restrict__ items, int n)
+static __global__ void kb_step_dt(const BItem* __restrict__ items, int n)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
@@ -139,7 +139,7 @@ __global__ void kb_step_dt(const BItem* __restrict__ items, int n)
     ctl->dt = dt1 < dt2 ? dt1 : dt2;
 }
 
-__global__ void kb_step_end(const BItem* __restrict__ items, int n)
+static __global__ void kb_step_end(const BItem* __restrict__ items, int n)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;

@@ -79,7 +79,7 @@ __device__ __forceinline__ void hydro_face_flux(const R (&S)[5][2 * (Recon == 0 
 }
 
 template <int Recon, int Riem, typename R, bool Z>
-__global__ void __launch_bounds__(FUSED_TX * FUSED_TY)
+static __global__ void __launch_bounds__(FUSED_TX * FUSED_TY)
 k_fused_hydro(const double* __restrict__ Pin, const double* __restrict__ P0, double* __restrict__ U,
               double* __restrict__ Pout, Geom G, double b, int first_stage, int write_u, int courant_out, Ctl* ctl)
 {
@@ -321,7 +321,7 @@ template <typename R> struct PlmTile
 };
 
 template <int Riem, typename R, bool Z>
-__global__ void __launch_bounds__(FUSED_TX * PlmTile<R>::TY) __attribute__((amdgpu_waves_per_eu(PlmTile<R>::WAVES, 8)))
+static __global__ void __launch_bounds__(FUSED_TX * PlmTile<R>::TY) __attribute__((amdgpu_waves_per_eu(PlmTile<R>::WAVES, 8)))
 k_fused_plm(const double* __restrict__ Pin, const double* __restrict__ P0, double* __restrict__ U,
             double* __restrict__ Pout, Geom G, double b, int first_stage, int write_u, int courant_out, Ctl* ctl)
 {

@@ -599,6 +599,81 @@ __device__ __forceinline__ void bc_guard(double* __restrict__ P, const Geom& G, 
     }
 }
 
+// Source index along `axis` of index i after the ghost fill (i itself when
+// interior); sets flip when the map reflects.
+__device__ __forceinline__ int ghost_source(const Geom& G, int axis, int i, bool& flip)
+{
+    const int g = G.g[axis], N = G.N[axis];
+    flip = false;
+    if (g == 0 || (i >= g && i < N - g)) return i;
+    const bool left = i < g;
+    if (G.bc == 0) return left ? i + (N - 2 * g) : i - (N - 2 * g);
+    if (G.bc == 1) return left ? g : N - g - 1;
+    flip = true;
+    return left ? 2 * g - 1 - i : 2 * N - 2 * g - 1 - i;
+}
+
+// All guard zones in one pass: zone (i, j, k) takes (m0(i), m1(j), m2(k)) of
+// the axis maps, negating the velocity component of each reflecting axis it
+// was mapped along -- the composition the axis-by-axis fill produces.
+// Threads: x slabs 2 g0 N1 N2, then y slabs n0 2 g1 N2, then z slabs n0 n1 2 g2.
+__device__ __forceinline__ void ghost_zone(double* __restrict__ P, const Geom& G, long long t)
+{
+    const long long sx = 2ll * G.g[0] * G.N[1] * G.N[2];
+    const long long sy = (long long)G.n[0] * 2 * G.g[1] * G.N[2];
+    int i, j, k;
+    if (t < sx)
+    {
+        const int l = (int)(t % (2 * G.g[0]));
+        const long long r = t / (2 * G.g[0]);
+        i = l < G.g[0] ? l : G.N[0] - 2 * G.g[0] + l;
+        j = (int)(r % G.N[1]);
+        k = (int)(r / G.N[1]);
+    }
+    else if (t < sx + sy)
+    {
+        const long long u = t - sx;
+        i = G.g[0] + (int)(u % G.n[0]);
+        const long long r = u / G.n[0];
+        const int l = (int)(r % (2 * G.g[1]));
+        j = l < G.g[1] ? l : G.N[1] - 2 * G.g[1] + l;
+        k = (int)(r / (2 * G.g[1]));
+    }
+    else
+    {
+        const long long u = t - sx - sy;
+        i = G.g[0] + (int)(u % G.n[0]);
+        const long long r = u / G.n[0];
+        j = G.g[1] + (int)(r % G.n[1]);
+        const int l = (int)(r / G.n[1]);
+        k = l < G.g[2] ? l : G.N[2] - 2 * G.g[2] + l;
+    }
+    bool fx, fy, fz;
+    const int si = ghost_source(G, 0, i, fx), sj = ghost_source(G, 1, j, fy), sk = ghost_source(G, 2, k, fz);
+    const long long cd = cell_index(G, i, j, k), cs = cell_index(G, si, sj, sk);
+    for (int q = 0; q < G.nq; ++q)
+    {
+        double v = P[q * G.fs + cs];
+        if ((q == 1 && fx) || (q == 2 && fy) || (q == 3 && fz)) v *= -1;
+        P[q * G.fs + cd] = v;
+    }
+}
+
+__device__ __forceinline__ long long ghost_count(const Geom& G)
+{
+    return 2ll * G.g[0] * G.N[1] * G.N[2] + (long long)G.n[0] * 2 * G.g[1] * G.N[2]
+           + (long long)G.n[0] * G.n[1] * 2 * G.g[2];
+}
+
+static __global__ void __launch_bounds__(256)
+k_ghost(double* __restrict__ P, Geom G, long long total, const Ctl* __restrict__ ctl)
+{
+    if (!ctl->active) return;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    ghost_zone(P, G, t);
+}
+
 __device__ __forceinline__ long long bc_count(const Geom& G, int axis)
 {
     const int t1 = axis == 0 ? 1 : 0;

@@ -1,0 +1,19 @@
+// mixed.hip -- the binary32 (mixed-form) instantiations of the stage launches
+// of a single block (stage.hpp) and of a batch (batch_state.hpp).  This file
+// is compiled with floating-point contraction (fused multiply-add), which
+// the mixed form allows: its fluxes are binary32 and its RK accumulation,
+// primitive recovery and flux-CT stay binary64, with no claim to the
+// reference's rounding.  The binary64 kernels live in api.hip and batch.hip,
+// compiled without contraction.
+#include "stage.hpp"
+#include "batch_state.hpp"
+
+namespace mara {
+
+void stage_kernels_f32(const StageArgs& s) { stage_kernels<float>(s); }
+
+namespace batch {
+void run_stage_f32(mara_batch* B, int rk) { run_stage<float>(B, rk); }
+} // namespace batch
+
+} // namespace mara

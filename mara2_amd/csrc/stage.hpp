@@ -1,0 +1,126 @@
+// stage.hpp -- kernel launches of one RK stage of a single block, templated
+// on the flux arithmetic R.  Instantiated for binary64 in api.hip and for
+// binary32 in mixed.hip, which is compiled with floating-point contraction
+// (the mixed form is not bound to the reference's rounding).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/mara_gpu.h"
+#include "kernels.hpp"
+#include "fused.hpp"
+
+namespace mara {
+
+// Everything one stage's launches need: the stage reads Pin (neighbours) and
+// P0, U (own zone), writes Pout and, when write_u, U.
+struct StageArgs
+{
+    const mara_desc* d;
+    const Geom* G;
+    hipStream_t stream;
+    const double* Pin;
+    const double* P0;
+    double* U;
+    double* Pout;
+    void* F[3];
+    double b;
+    int first, write_u, cout;
+    Ctl* ctl;
+    bool fused, ct;
+};
+
+// 2-D work-groups of 64 x 4 over an (m0, m1, m2) box: one wavefront per x row,
+// z from the grid, so kernels decode indices without integer division.
+inline dim3 stage_block3() { return dim3(64, 4, 1); }
+inline dim3 stage_grid3(int m0, int m1, int m2) { return dim3((m0 + 63) / 64, (m1 + 3) / 4, m2); }
+
+template <int Law, int Recon, int Riem, typename R>
+void stage_flux_t(const StageArgs& s, int axis)
+{
+    const Geom& G = *s.G;
+    int lo[3], hi[3];
+    for (int a = 0; a < 3; ++a)
+    {
+        if (a == axis) { lo[a] = G.g[a]; hi[a] = G.N[a] - G.g[a] + 1; }
+        else
+        {
+            // flux-CT reads the transverse neighbours of each face: one ring more
+            const int ring = (s.ct && G.N[a] > 1) ? 1 : 0;
+            lo[a] = G.g[a] - ring;
+            hi[a] = G.N[a] - G.g[a] + ring;
+        }
+    }
+    const int m0 = hi[0] - lo[0], m1 = hi[1] - lo[1], m2 = hi[2] - lo[2];
+    k_flux<Law, Recon, Riem, R><<<stage_grid3(m0, m1, m2), stage_block3(), 0, s.stream>>>(
+        s.Pin, (R*)s.F[axis], G, axis, lo[0], lo[1], lo[2], m0, m1, m2, s.ctl);
+}
+
+template <int Law, typename R>
+void stage_flux_law(const StageArgs& s, int axis)
+{
+    const int recon = s.d->recon, riem = s.d->riemann;
+    if (recon == MARA_RECON_PCM)
+    {
+        if (Law == 0 && riem == 1) stage_flux_t<Law, 0, 1, R>(s, axis);
+        else stage_flux_t<Law, 0, 0, R>(s, axis);
+    }
+    else if (recon == MARA_RECON_PLM)
+    {
+        if (Law == 0 && riem == 1) stage_flux_t<Law, 1, 1, R>(s, axis);
+        else stage_flux_t<Law, 1, 0, R>(s, axis);
+    }
+    else
+    {
+        if (Law == 0 && riem == 1) stage_flux_t<Law, 2, 1, R>(s, axis);
+        else stage_flux_t<Law, 2, 0, R>(s, axis);
+    }
+}
+
+template <int Law, typename R>
+void stage_update_law(const StageArgs& s)
+{
+    const Geom& G = *s.G;
+    const dim3 grid = stage_grid3(G.n[0], G.n[1], G.n[2]);
+    if (s.ct)
+        k_update<Law, R, true><<<grid, stage_block3(), 0, s.stream>>>(
+            s.P0, s.U, s.Pout, (const R*)s.F[0], (const R*)s.F[1], (const R*)s.F[2],
+            G, s.b, s.first, s.write_u, s.cout, s.ctl);
+    else
+        k_update<Law, R, false><<<grid, stage_block3(), 0, s.stream>>>(
+            s.P0, s.U, s.Pout, (const R*)s.F[0], (const R*)s.F[1], (const R*)s.F[2],
+            G, s.b, s.first, s.write_u, s.cout, s.ctl);
+}
+
+// The stage's kernels: the fused hydro kernel, or one flux sweep per axis
+// followed by the zone update (which also writes the ghost zones).
+template <typename R>
+void stage_kernels(const StageArgs& s)
+{
+    if (s.fused)
+    {
+        launch_fused_r<R>(*s.d, *s.G, s.stream, s.Pin, s.P0, s.U, s.Pout, s.b, s.first, s.write_u, s.cout, s.ctl);
+        return;
+    }
+    for (int axis = 0; axis < 3; ++axis)
+    {
+        if (s.G->N[axis] < 2) continue;
+        switch (s.d->law)
+        {
+            case 0: stage_flux_law<0, R>(s, axis); break;
+            case 1: stage_flux_law<1, R>(s, axis); break;
+            default: stage_flux_law<2, R>(s, axis); break;
+        }
+    }
+    switch (s.d->law)
+    {
+        case 0: stage_update_law<0, R>(s); break;
+        case 1: stage_update_law<1, R>(s); break;
+        default: stage_update_law<2, R>(s); break;
+    }
+}
+
+// binary32 instantiation (mixed.hip)
+void stage_kernels_f32(const StageArgs& s);
+
+} // namespace mara
