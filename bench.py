@@ -55,10 +55,14 @@ def traffic_from_profile(kernel, workload):
     return None
 
 
-def time_form(desc, P0, steps, warmup):
-    """Upload, warm up, then time `steps` device-driven steps."""
+def time_form(desc, P0, steps, warmup, transfers=None):
+    """Upload, warm up, then time `steps` device-driven steps.  transfers, if
+    a dict, receives the host<->device times of the MeshData::P arrays."""
     with Block(desc) as blk:
+        t0 = time.perf_counter()
         blk.upload(P0)
+        if transfers is not None:
+            transfers["upload_s"] = time.perf_counter() - t0
         if warmup:
             blk.run(CFL, 1e300, warmup)
             blk.upload(P0)
@@ -72,7 +76,10 @@ def time_form(desc, P0, steps, warmup):
         if taken != steps:
             raise RuntimeError(f"run stopped after {taken} of {steps} steps: {rep.as_dict()}")
         avg_ms, launches, name = blk.stage_kernel_time()
+        t2 = time.perf_counter()
         P = blk.download()
+        if transfers is not None:
+            transfers["download_s"] = time.perf_counter() - t2
     return t1 - t0, P, avg_ms, launches, name
 
 
@@ -260,7 +267,9 @@ def serving_workload(args):
            "sequential": {"solves_per_s": seq_rate, "zone_updates_per_s": seq_zone_rate,
                           "sample": f"{len(sample)} requests, every {stride}th of the stream",
                           "mean_s": float(np.mean(times)), "max_s": float(max(times)),
-                          "failed_requests": seq_failed}}
+                          "failed_requests": seq_failed,
+                          "truncated_requests": [r.index for r in serving.truncated(sample)]},
+           "work_bound_per_request": {"steps": serving.MAX_STEPS, "zone_updates": serving.MAX_ZONE_UPDATES}}
     if not args.no_open_loop:
         serving.set_arrivals(reqs, 2.0 * seq_rate)
         log(f"serving {n_req} requests, Poisson arrivals at {2 * seq_rate:.3f}/s "
@@ -272,6 +281,8 @@ def serving_workload(args):
         log(f"served in {wall:.1f} s: p50 {st['p50_s']:.3f} s p99 {st['p99_s']:.3f} s")
         out.update({"value": len(done) / wall, "latency_s": st, "wall_s": wall,
                     "zone_updates_per_s": zone_updates / wall, "failed_requests": failed,
+                    "truncated_requests": [r.index for r in serving.truncated(done)],
+                    "slowest": serving.slowest(done),
                     "floor_applied_zones": int(sum(r.floors for r in done))})
     if args.saturate:
         for r in reqs:
@@ -281,7 +292,10 @@ def serving_workload(args):
         out["saturated"] = {"solves_per_s": len(done_s) / wall_s, "wall_s": wall_s,
                             "speedup_vs_sequential": len(done_s) / wall_s / seq_rate,
                             "latency_s": serving.latency_stats(done_s),
-                            "failed_requests": [r.index for r in serving.failures(done_s)]}
+                            "zone_updates_per_s": sum(r.zones * r.steps for r in done_s) / wall_s,
+                            "failed_requests": [r.index for r in serving.failures(done_s)],
+                            "truncated_requests": [r.index for r in serving.truncated(done_s)],
+                            "slowest": serving.slowest(done_s)}
         if out["value"] is None:
             out["value"] = out["saturated"]["solves_per_s"]
     print(json.dumps(out), flush=True)
@@ -299,7 +313,7 @@ def main():
                     choices=["turbulence", "mhd2d", "rmhd3d", "sod", "serving", "copy"])
     ap.add_argument("--wmax", type=float, default=5.0)
     ap.add_argument("--requests", type=int, default=4096, help="serving: stream length")
-    ap.add_argument("--seq-sample", type=int, default=128, help="serving: requests timed one at a time")
+    ap.add_argument("--seq-sample", type=int, default=256, help="serving: requests timed one at a time")
     ap.add_argument("--budget-gib", type=float, default=64.0, help="serving: HBM pool")
     ap.add_argument("--chunk", type=int, default=16, help="serving: steps per batch launch sequence")
     ap.add_argument("--saturate", action="store_true", help="serving: also run with every request at t = 0")
@@ -325,7 +339,8 @@ def main():
     zones = n ** 3
 
     log("timing FP64 form")
-    el64, P64, ms64, nl64, kname = time_form(d64, P0, args.steps, args.warmup)
+    transfers = {}
+    el64, P64, ms64, nl64, kname = time_form(d64, P0, args.steps, args.warmup, transfers)
     log(f"FP64: {el64:.3f} s for {args.steps} steps; stage kernel {ms64:.3f} ms ({kname})")
     log("timing mixed form")
     elmx, Pmx, msmx, nlmx, _ = time_form(dmx, P0, args.steps, args.warmup)
@@ -362,6 +377,11 @@ def main():
         "config": {"workload": f"3-D decaying turbulence {n}^3, hydro gamma 5/3, PLM theta 1.5 + HLLC, "
                                f"RK2, CFL {CFL}, periodic", "zones": zones, "form": "fp64"},
         "roofline": roofline,
+        # the C ABI takes host arrays: PCIe-inclusive rate of one upload, the
+        # timed steps and one download of MeshData::P (not the headline value)
+        "host_transfer": {"upload_s": transfers["upload_s"], "download_s": transfers["download_s"],
+                          "pcie_inclusive_value": zones * args.steps
+                          / (el64 + transfers["upload_s"] + transfers["download_s"])},
         "mixed": {"value": zones * args.steps / elmx, "ms_per_step": elmx / args.steps * 1e3,
                   "speedup_vs_fp64": el64 / elmx, "rel_l1_vs_fp64": l1,
                   "avg_stage_launch_ms": msmx,

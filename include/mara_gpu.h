@@ -202,8 +202,12 @@ int mara_batch_admit(mara_batch* batch, const mara_desc* desc, const double* P, 
                      double t_final, int max_steps, int* slot);
 
 /* Advance every unfinished solve by up to n_steps of its own steps; returns
-   after the device has drained.  *n_active receives the number of admitted
-   solves not yet finished. */
+   after the device has drained.  1-D solves of an RK2 batch run on a second
+   stream and may take up to 32 * n_steps steps in one call (more while large
+   solves make the others' steps slow), so that they are not held to the pace
+   of the large ones; a solve's result does not depend on how its steps are
+   grouped into calls.  *n_active receives the number of admitted solves not
+   yet finished. */
 int mara_batch_step(mara_batch* batch, int n_steps, int* n_active);
 
 /* State of one solve as of the last mara_batch_step: *done is 1 once it

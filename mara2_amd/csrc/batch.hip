@@ -24,8 +24,11 @@ namespace {
 int class_of(mara_batch* B, const mara_desc& d)
 {
     const bool one_d = d.cells[1] == 1 && d.cells[2] == 1;
+    const bool rk2 = B->rk_order == 2;
+    const int fused = (!one_d && rk2 && d.law == MARA_LAW_HYDRO && d.recon == MARA_RECON_PLM)
+                          ? (d.cells[2] > 1 ? 2 : 1) : 0;
     const ClassKey k{d.law, d.recon, d.riemann, (d.law != MARA_LAW_HYDRO && d.field_ct) ? 1 : 0,
-                     (one_d && B->rk_order == 2) ? 1 : 0};
+                     (one_d && rk2) ? 1 : 0, fused};
     for (size_t c = 0; c < B->classes.size(); ++c)
         if (!(B->classes[c] < k) && !(k < B->classes[c])) return (int)c;
     B->classes.push_back(k);
@@ -70,9 +73,42 @@ void build_table(mara_batch* B)
     B->tiles.clear();
     for (int a = 0; a < 3; ++a) B->flux[a].clear();
     B->update.clear();
+    B->ghost.clear();
     auto seg_begin = [&]() { return (int)B->tiles.size(); };
     for (size_t c = 0; c < B->classes.size(); ++c)
     {
+        if (B->classes[c].fused)
+        {
+            // one tile per FUSED_TX x TY x KZ block of each item, then the
+            // ghost-zone tiles of the same items
+            const int TY = B->precision == MARA_PREC_MIXED ? PlmTile<float>::TY : PlmTile<double>::TY;
+            int s0 = seg_begin();
+            for (int n = 0; n < B->n_items; ++n)
+            {
+                const Slot& sl = B->slots[n];
+                if (!sl.used || sl.cls != (int)c) continue;
+                const Geom& G = B->items[n].G;
+                const int ntx = (G.n[0] + FUSED_TX - 1) / FUSED_TX, nty = (G.n[1] + TY - 1) / TY;
+                const int ntz = G.N[2] > 1 ? (G.n[2] + FUSED_KZ - 1) / FUSED_KZ : 1;
+                for (int tz = 0; tz < ntz; ++tz)
+                    for (int ty = 0; ty < nty; ++ty)
+                        for (int tx = 0; tx < ntx; ++tx) B->tiles.push_back(BTile{n, tx, ty, tz, 0, 0, 3, 0});
+            }
+            if ((int)B->tiles.size() > s0) B->update[(int)c] = Segment{s0, (int)B->tiles.size() - s0};
+            s0 = seg_begin();
+            for (int n = 0; n < B->n_items; ++n)
+            {
+                const Slot& sl = B->slots[n];
+                if (!sl.used || sl.cls != (int)c) continue;
+                const Geom& G = B->items[n].G;
+                const long long total = 2ll * G.g[0] * G.N[1] * G.N[2] + (long long)G.n[0] * 2 * G.g[1] * G.N[2]
+                                        + (long long)G.n[0] * G.n[1] * 2 * G.g[2];
+                for (long long x = 0; x < total; x += 256)
+                    B->tiles.push_back(BTile{n, (int)x, 0, 0, (int)std::min<long long>(256, total - x), 1, 0, 0});
+            }
+            if ((int)B->tiles.size() > s0) B->ghost[(int)c] = Segment{s0, (int)B->tiles.size() - s0};
+            continue;
+        }
         if (B->classes[c].pencil)
         {
             // one tile per PENCIL - 1 interior zones of each 1-D item
@@ -178,6 +214,9 @@ int mara_batch_create(long long hbm_budget, int rk_order, int precision, int max
     B->max_solves = max_solves;
     auto fail = [&](int code) { mara_batch_destroy(B); return code; };
     if (!hip_ok(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking))) return fail(MARA_ERR_HIP);
+    if (!hip_ok(hipStreamCreateWithFlags(&B->stream2, hipStreamNonBlocking))) return fail(MARA_ERR_HIP);
+    if (!hip_ok(hipEventCreateWithFlags(&B->fork, hipEventDisableTiming))) return fail(MARA_ERR_HIP);
+    if (!hip_ok(hipEventCreateWithFlags(&B->join, hipEventDisableTiming))) return fail(MARA_ERR_HIP);
     if (!hip_ok(hipMalloc(&B->pool_mem, hbm_budget))) return fail(MARA_ERR_CAPACITY);
     B->pool.init(B->pool_mem, hbm_budget);
     if (!hip_ok(hipMalloc(&B->d_items, max_solves * sizeof(BItem)))) return fail(MARA_ERR_HIP);
@@ -195,6 +234,7 @@ int mara_batch_destroy(mara_batch* B)
 {
     if (!B) return MARA_ERR_INVALID_ARGUMENT;
     if (B->stream) (void)hipStreamSynchronize(B->stream);
+    if (B->stream2) (void)hipStreamSynchronize(B->stream2);
     (void)hipFree(B->pool_mem);
     (void)hipFree(B->d_items);
     (void)hipFree(B->d_ctl);
@@ -202,6 +242,9 @@ int mara_batch_destroy(mara_batch* B)
     (void)hipFree(B->d_tiles);
     if (B->h_ctl) (void)hipHostFree(B->h_ctl);
     if (B->stream) (void)hipStreamDestroy(B->stream);
+    if (B->stream2) (void)hipStreamDestroy(B->stream2);
+    if (B->fork) (void)hipEventDestroy(B->fork);
+    if (B->join) (void)hipEventDestroy(B->join);
     delete B;
     return MARA_OK;
 }
@@ -252,6 +295,7 @@ int mara_batch_admit(mara_batch* B, const mara_desc* d, const double* P, double 
         if (it.G.N[a] > 1) { it.F[a] = p; p += round_up(it.G.nq * it.G.ffs * real, 256); }
     it.ctl = B->d_ctl + n;
     it.bc = d->bc;
+    it.lane = B->classes[sl.cls].pencil;
 
     Ctl& c = B->h_ctl[n];
     std::memset(&c, 0, sizeof c);
@@ -317,17 +361,57 @@ int mara_batch_step(mara_batch* B, int n_steps, int* n_active)
     }
     const int n = B->n_items;
     const unsigned ig = (unsigned)((n + 63) / 64);
-    for (int s = 0; s < n_steps && n > 0; ++s)
+    // Two lanes: tile kernels (2-D/3-D solves, and 1-D ones outside RK2) on
+    // the first stream, pencil kernels of 1-D RK2 solves on the second, so
+    // that small 1-D solves are not held to the pace of large ones.  The
+    // pencil lane takes `ratio` steps per tile-lane step, sized from the
+    // tile lane's estimated step time (its active zones at ~5e9 zone-stages
+    // per second) against the pencil lane's launch-bound ~20 us per step.
+    long long tile_zones = 0;
+    bool any_tile = false, any_pencil = false;
+    for (int k = 0; k < n; ++k)
     {
-        kb_step_begin<<<ig, 64, 0, B->stream>>>(B->d_items, n);
-        kb_step_dt<<<ig, 64, 0, B->stream>>>(B->d_items, n);
+        if (!B->slots[k].used || finished(B->h_ctl[k])) continue;
+        if (B->items[k].lane) any_pencil = true;
+        else
+        {
+            any_tile = true;
+            tile_zones += (long long)B->items[k].G.n[0] * B->items[k].G.n[1] * B->items[k].G.n[2];
+        }
+    }
+    int ratio = 1;
+    if (any_tile && any_pencil)
+    {
+        const double tile_step = (double)tile_zones * B->rk_order / 5e9 + 20e-6;
+        ratio = (int)std::min(32.0, std::max(1.0, tile_step / 20e-6));
+    }
+    auto run_lane = [&](int lane) {
+        kb_step_begin<<<ig, 64, 0, B->cur>>>(B->d_items, n, lane);
+        kb_step_dt<<<ig, 64, 0, B->cur>>>(B->d_items, n, lane);
         for (int rk = 0; rk < B->rk_order; ++rk)
         {
-            if (B->precision == MARA_PREC_MIXED) run_stage_f32(B, rk);
-            else run_stage<double>(B, rk);
+            if (B->precision == MARA_PREC_MIXED) run_stage_f32(B, rk, lane);
+            else run_stage<double>(B, rk, lane);
         }
-        kb_step_end<<<ig, 64, 0, B->stream>>>(B->d_items, n);
+        kb_step_end<<<ig, 64, 0, B->cur>>>(B->d_items, n, lane);
+    };
+    HIP_TRY(hipEventRecord(B->fork, B->stream));
+    HIP_TRY(hipStreamWaitEvent(B->stream2, B->fork, 0));
+    for (int s = 0; s < n_steps && n > 0; ++s)
+    {
+        if (any_tile || !any_pencil)
+        {
+            B->cur = B->stream;
+            run_lane(0);
+        }
+        if (any_pencil)
+        {
+            B->cur = B->stream2;
+            for (int r = 0; r < ratio; ++r) run_lane(1);
+        }
     }
+    HIP_TRY(hipEventRecord(B->join, B->stream2));
+    HIP_TRY(hipStreamWaitEvent(B->stream, B->join, 0));
     HIP_TRY(hipGetLastError());
     if (n) HIP_TRY(hipMemcpyAsync(B->h_ctl, B->d_ctl, n * sizeof(Ctl), hipMemcpyDeviceToHost, B->stream));
     if (d_ftiles) HIP_TRY(hipFreeAsync(d_ftiles, B->stream));

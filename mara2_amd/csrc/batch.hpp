@@ -13,6 +13,7 @@
 #pragma once
 
 #include "kernels.hpp"
+#include "fused.hpp"
 
 namespace mara {
 
@@ -25,7 +26,7 @@ struct BItem
     void* F[3];
     Ctl* ctl;
     int bc;
-    int pad;
+    int lane;  // 0: tile kernels on the batch's first stream, 1: pencil kernels on the second
 };
 
 struct BTile
@@ -99,6 +100,35 @@ kb_pencil(const BItem* __restrict__ items, const BTile* __restrict__ tiles, doub
     if (courant_out) courant_to_bins(dtc, wmin, PENCIL / 64, threadIdx.x, it.ctl);
 }
 
+// 2-D and 3-D hydro + PLM items of an RK2 batch: the single block's fused
+// stage kernel, one tile (x0, y0, z0 = tile coordinates) per work-group;
+// launch order is permuted so neighbouring tiles share an XCD.
+template <int Riem, typename R, bool Z>
+static __global__ void __launch_bounds__(FUSED_TX * PlmTile<R>::TY) __attribute__((amdgpu_waves_per_eu(PlmTile<R>::WAVES, 8)))
+kb_fused_plm(const BItem* __restrict__ items, const BTile* __restrict__ tiles, double b, int first, int last,
+             int courant_out)
+{
+    const BTile T = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    const BItem& it = items[T.item];
+    if (!it.ctl->active) return;
+    const Geom G = it.G;
+    const double* Pin = first ? it.Pa : it.Pb;
+    double* Pout = last ? it.Pa : it.Pb;
+    fused_plm_tile<Riem, R, Z>(Pin, it.Pa, it.U, Pout, G, b, first, !last, courant_out, it.ctl, T.x0, T.y0, T.z0);
+}
+
+// Ghost zones of the fused items (their kernel leaves them): tile x0 is the
+// first guard-zone index of ghost_zone's enumeration, nx the count.
+static __global__ void __launch_bounds__(256)
+kb_ghost(const BItem* __restrict__ items, const BTile* __restrict__ tiles, int last)
+{
+    const BTile T = tiles[blockIdx.x];
+    const BItem& it = items[T.item];
+    if (!it.ctl->active) return;
+    if ((int)threadIdx.x >= T.nx) return;
+    ghost_zone(last ? it.Pa : it.Pb, it.G, (long long)T.x0 + threadIdx.x);
+}
+
 template <int Law>
 static __global__ void __launch_bounds__(256)
 kb_courant(const BItem* __restrict__ items, const BTile* __restrict__ tiles)
@@ -117,20 +147,20 @@ kb_courant(const BItem* __restrict__ items, const BTile* __restrict__ tiles)
 }
 
 // per-item control, one thread per item
-static __global__ void kb_step_begin(const BItem* __restrict__ items, int n)
+static __global__ void kb_step_begin(const BItem* __restrict__ items, int n, int lane)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
+    if (t >= n || items[t].lane != lane) return;
     Ctl* ctl = items[t].ctl;
     const bool go = ctl->active && ctl->t < ctl->t_final && ctl->steps < ctl->max_steps
                     && ctl->c2p_fail == 0 && ctl->eigen_fail == 0 && ctl->superluminal == 0;
     ctl->active = go ? 1 : 0;
 }
 
-static __global__ void kb_step_dt(const BItem* __restrict__ items, int n)
+static __global__ void kb_step_dt(const BItem* __restrict__ items, int n, int lane)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
+    if (t >= n || items[t].lane != lane) return;
     Ctl* ctl = items[t].ctl;
     if (!ctl->active) return;
     fold_bins(ctl);
@@ -139,10 +169,10 @@ static __global__ void kb_step_dt(const BItem* __restrict__ items, int n)
     ctl->dt = dt1 < dt2 ? dt1 : dt2;
 }
 
-static __global__ void kb_step_end(const BItem* __restrict__ items, int n)
+static __global__ void kb_step_end(const BItem* __restrict__ items, int n, int lane)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
+    if (t >= n || items[t].lane != lane) return;
     Ctl* ctl = items[t].ctl;
     if (!ctl->active) return;
     ctl->t += ctl->dt;

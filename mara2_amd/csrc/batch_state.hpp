@@ -82,9 +82,11 @@ struct ClassKey
 {
     int law, recon, riemann, ct;
     int pencil;  // 1-D items of an RK2 batch: fused pencil tiles
+    int fused;   // hydro + PLM items of an RK2 batch on 2-D (1) or 3-D (2) meshes: fused stage tiles
     bool operator<(const ClassKey& o) const
     {
-        return std::tie(law, recon, riemann, ct, pencil) < std::tie(o.law, o.recon, o.riemann, o.ct, o.pencil);
+        return std::tie(law, recon, riemann, ct, pencil, fused)
+               < std::tie(o.law, o.recon, o.riemann, o.ct, o.pencil, o.fused);
     }
 };
 
@@ -98,7 +100,10 @@ struct mara_batch
     int rk_order = 2;
     int precision = MARA_PREC_FP64;
     int max_solves = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;   // tile kernels, transfers, control
+    hipStream_t stream2 = nullptr;  // pencil kernels of 1-D solves
+    hipStream_t cur = nullptr;      // stream the stage launches go to
+    hipEvent_t fork = nullptr, join = nullptr;
     mara::batch::Pool pool;
     char* pool_mem = nullptr;
     mara::BItem* d_items = nullptr;
@@ -116,6 +121,7 @@ struct mara_batch
     std::vector<mara::BTile> tiles;
     std::map<int, mara::batch::Segment> flux[3];  // class -> segment
     std::map<int, mara::batch::Segment> update;
+    std::map<int, mara::batch::Segment> ghost;     // fused classes: ghost-zone tiles
 };
 
 namespace mara {
@@ -125,7 +131,7 @@ namespace batch {
 template <int Law, int Recon, int Riem, typename R>
 void flux_launch(mara_batch* B, const Segment& s, int axis, int rk)
 {
-    kb_flux<Law, Recon, Riem, R><<<s.count, 256, 0, B->stream>>>(B->d_items, B->d_tiles + s.offset, axis, rk);
+    kb_flux<Law, Recon, Riem, R><<<s.count, 256, 0, B->cur>>>(B->d_items, B->d_tiles + s.offset, axis, rk);
 }
 
 template <typename R>
@@ -141,8 +147,8 @@ void flux_dispatch(mara_batch* B, const ClassKey& k, const Segment& s, int axis,
 template <int Law, int Recon, int Riem, typename R>
 void pencil_launch(mara_batch* B, const ClassKey& k, const BTile* t, int count, double b, int first, int last, int cout)
 {
-    if (k.ct) kb_pencil<Law, Recon, Riem, R, true><<<count, PENCIL, 0, B->stream>>>(B->d_items, t, b, first, last, cout);
-    else kb_pencil<Law, Recon, Riem, R, false><<<count, PENCIL, 0, B->stream>>>(B->d_items, t, b, first, last, cout);
+    if (k.ct) kb_pencil<Law, Recon, Riem, R, true><<<count, PENCIL, 0, B->cur>>>(B->d_items, t, b, first, last, cout);
+    else kb_pencil<Law, Recon, Riem, R, false><<<count, PENCIL, 0, B->cur>>>(B->d_items, t, b, first, last, cout);
 }
 
 template <typename R>
@@ -156,6 +162,12 @@ void pencil_dispatch(mara_batch* B, const ClassKey& k, const Segment& s, double 
 #undef PN
 }
 
+template <int Riem, typename R, bool Z>
+void fused_launch(mara_batch* B, const BTile* t, int count, double b, int first, int last, int cout)
+{
+    kb_fused_plm<Riem, R, Z><<<count, dim3(FUSED_TX, PlmTile<R>::TY), 0, B->cur>>>(B->d_items, t, b, first, last, cout);
+}
+
 template <typename R>
 void update_dispatch(mara_batch* B, const ClassKey& k, const Segment& s, double b, int first, int last, int cout)
 {
@@ -164,14 +176,31 @@ void update_dispatch(mara_batch* B, const ClassKey& k, const Segment& s, double 
         pencil_dispatch<R>(B, k, s, b, first, last, cout);
         return;
     }
+    if (k.fused)
+    {
+        const BTile* t = B->d_tiles + s.offset;
+        if (k.fused == 2)
+        {
+            if (k.riemann) fused_launch<1, R, true>(B, t, s.count, b, first, last, cout);
+            else fused_launch<0, R, true>(B, t, s.count, b, first, last, cout);
+        }
+        else
+        {
+            if (k.riemann) fused_launch<1, R, false>(B, t, s.count, b, first, last, cout);
+            else fused_launch<0, R, false>(B, t, s.count, b, first, last, cout);
+        }
+        return;
+    }
     const BTile* t = B->d_tiles + s.offset;
-#define UP(L, C) if (k.law == L && k.ct == C) { kb_update<L, R, (C != 0)><<<s.count, 256, 0, B->stream>>>(B->d_items, t, b, first, last, cout); return; }
+#define UP(L, C) if (k.law == L && k.ct == C) { kb_update<L, R, (C != 0)><<<s.count, 256, 0, B->cur>>>(B->d_items, t, b, first, last, cout); return; }
     UP(0, 0) UP(1, 0) UP(1, 1) UP(2, 0) UP(2, 1)
 #undef UP
 }
 
+// The stage's launches for the classes of one lane (0: tile kernels, 1:
+// pencil kernels), on B->cur.
 template <typename R>
-void run_stage(mara_batch* B, int rk)
+void run_stage(mara_batch* B, int rk, int lane)
 {
     static const double b1[1] = {1.0};
     static const double b2[2] = {1.0, 0.5};
@@ -179,14 +208,20 @@ void run_stage(mara_batch* B, int rk)
     const double* bb = B->rk_order == 1 ? b1 : B->rk_order == 2 ? b2 : b3;
     const int last = rk == B->rk_order - 1;
     for (int axis = 0; axis < 3; ++axis)
-        for (auto& e : B->flux[axis]) flux_dispatch<R>(B, B->classes[e.first], e.second, axis, rk);
-    for (auto& e : B->update) update_dispatch<R>(B, B->classes[e.first], e.second, bb[rk], rk == 0, last, last);
-    // ghost zones are written by the zone update (write_guard_images)
+        for (auto& e : B->flux[axis])
+            if (B->classes[e.first].pencil == lane) flux_dispatch<R>(B, B->classes[e.first], e.second, axis, rk);
+    for (auto& e : B->update)
+        if (B->classes[e.first].pencil == lane)
+            update_dispatch<R>(B, B->classes[e.first], e.second, bb[rk], rk == 0, last, last);
+    // the flux-array and pencil updates write ghost zones themselves
+    // (write_guard_images); the fused tiles leave them to this pass
+    for (auto& e : B->ghost)
+        if (B->classes[e.first].pencil == lane) kb_ghost<<<e.second.count, 256, 0, B->cur>>>(B->d_items, B->d_tiles + e.second.offset, last);
 }
 
 
 // binary32 stage launches, compiled with contraction in mixed.hip
-void run_stage_f32(mara_batch* B, int rk);
+void run_stage_f32(mara_batch* B, int rk, int lane);
 
 } // namespace batch
 } // namespace mara

@@ -320,10 +320,13 @@ template <typename R> struct PlmTile
     static constexpr int WAVES = sizeof(R) == 8 ? 3 : 4;
 };
 
+// One tile (tx, ty, tz) of the fused PLM stage; every thread of the
+// work-group calls it (it synchronises internally).
 template <int Riem, typename R, bool Z>
-static __global__ void __launch_bounds__(FUSED_TX * PlmTile<R>::TY) __attribute__((amdgpu_waves_per_eu(PlmTile<R>::WAVES, 8)))
-k_fused_plm(const double* __restrict__ Pin, const double* __restrict__ P0, double* __restrict__ U,
-            double* __restrict__ Pout, Geom G, double b, int first_stage, int write_u, int courant_out, Ctl* ctl)
+__device__ __forceinline__ void fused_plm_tile(const double* __restrict__ Pin, const double* __restrict__ P0,
+                                               double* __restrict__ U, double* __restrict__ Pout, const Geom& G,
+                                               double b, int first_stage, int write_u, int courant_out, Ctl* ctl,
+                                               int tx, int ty, int tz)
 {
     constexpr int NG = 2;
     constexpr int TX = FUSED_TX, TY = PlmTile<R>::TY;
@@ -336,15 +339,10 @@ k_fused_plm(const double* __restrict__ Pin, const double* __restrict__ P0, doubl
     __shared__ R fxe[5][TY];         // x-face i0 + 64 of each row
     __shared__ double wmin[TX * TY / 64];
 
-    if (!ctl->active) return;
     double dmin = 1.79769313486231570815e+308;  // this zone column's CFL minimum (last stage)
     const int lane = threadIdx.x;
     const int row = threadIdx.y;
     const int tid = row * TX + lane;
-    const int ntx = (G.n[0] + TX - 1) / TX, nty = (G.n[1] + TY - 1) / TY;
-    const int ntz = Z ? (G.n[2] + FUSED_KZ - 1) / FUSED_KZ : 1;
-    const int tile = xcd_tile(blockIdx.x, ntx * nty * ntz);
-    const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
     const int i0 = G.g[0] + tx * TX;
     const int j0 = G.g[1] + ty * TY;
     const int i = i0 + lane, j = j0 + row;
@@ -603,6 +601,20 @@ k_fused_plm(const double* __restrict__ Pin, const double* __restrict__ P0, doubl
         __syncthreads();
     }
     if (courant_out) courant_to_bins(dmin, wmin, TX * TY / 64, tid, ctl);
+}
+
+template <int Riem, typename R, bool Z>
+static __global__ void __launch_bounds__(FUSED_TX * PlmTile<R>::TY) __attribute__((amdgpu_waves_per_eu(PlmTile<R>::WAVES, 8)))
+k_fused_plm(const double* __restrict__ Pin, const double* __restrict__ P0, double* __restrict__ U,
+            double* __restrict__ Pout, Geom G, double b, int first_stage, int write_u, int courant_out, Ctl* ctl)
+{
+    if (!ctl->active) return;
+    constexpr int TX = FUSED_TX, TY = PlmTile<R>::TY;
+    const int ntx = (G.n[0] + TX - 1) / TX, nty = (G.n[1] + TY - 1) / TY;
+    const int ntz = Z ? (G.n[2] + FUSED_KZ - 1) / FUSED_KZ : 1;
+    const int tile = xcd_tile(blockIdx.x, ntx * nty * ntz);
+    const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
+    fused_plm_tile<Riem, R, Z>(Pin, P0, U, Pout, G, b, first_stage, write_u, courant_out, ctl, tx, ty, tz);
 }
 
 // The fused path covers the hydro law on 2-D and 3-D blocks; 1-D blocks and

@@ -13,7 +13,7 @@ namespace mara {
 void stage_kernels_f32(const StageArgs& s) { stage_kernels<float>(s); }
 
 namespace batch {
-void run_stage_f32(mara_batch* B, int rk) { run_stage<float>(B, rk); }
+void run_stage_f32(mara_batch* B, int rk, int lane) { run_stage<float>(B, rk, lane); }
 } // namespace batch
 
 } // namespace mara

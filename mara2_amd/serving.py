@@ -26,7 +26,18 @@ from .runtime import Batch, Block, MaraError
 
 T_FINAL = 0.2
 CFL = 0.4
-MAX_STEPS = 10**7
+# Work bound per request.  The stream's regular solves need at most about
+# 1.6e5 steps (1-D, 16384 zones) or 7e9 zone-updates (2-D, 1024^2); an MHD
+# problem that rarefies towards vacuum drives the Alfven speed B / sqrt(rho)
+# up and dt down without bound, and is stopped here with its clock short of
+# t_final (reported as truncated), in the batch and one-at-a-time alike.
+MAX_STEPS = 200000
+MAX_ZONE_UPDATES = 10**10
+
+
+def max_steps(req):
+    """Step bound of one request: MAX_STEPS and MAX_ZONE_UPDATES / zones."""
+    return int(min(MAX_STEPS, MAX_ZONE_UPDATES // req.zones))
 
 
 @dataclasses.dataclass
@@ -84,7 +95,7 @@ def solve_alone(req, keep_result=True):
     t0 = time.perf_counter()
     with Block(req.desc) as b:
         b.upload(req.P)
-        t, steps, rep = b.run(CFL, T_FINAL, MAX_STEPS, check=False)
+        t, steps, rep = b.run(CFL, T_FINAL, max_steps(req), check=False)
         P = b.download()
     req.t, req.steps, req.status, req.floors = t, steps, rep.status, rep.floor_applied
     if keep_result:
@@ -115,7 +126,7 @@ def serve(requests, hbm_budget, max_solves=1024, chunk=16, rk_order=2, precision
             # admit in arrival order, skipping requests that do not fit yet
             still = []
             for r in waiting:
-                slot = batch.admit(r.desc, r.P, CFL, T_FINAL, MAX_STEPS) if len(in_flight) < max_solves else None
+                slot = batch.admit(r.desc, r.P, CFL, T_FINAL, max_steps(r)) if len(in_flight) < max_solves else None
                 if slot is None:
                     still.append(r)
                 else:
@@ -148,6 +159,20 @@ def latency_stats(done):
     lat = np.array([r.latency for r in done])
     return {"p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
             "mean_s": float(lat.mean()), "max_s": float(lat.max())}
+
+
+def truncated(done):
+    """Requests stopped by the step bound before t_final."""
+    return [r for r in done if r.status == 0 and r.t < T_FINAL]
+
+
+def slowest(done, n=5):
+    """The n longest-latency requests: index, mesh, law, steps, clock."""
+    out = []
+    for r in sorted(done, key=lambda r: -r.latency)[:n]:
+        out.append({"index": r.index, "dim": r.params["dim"], "n": r.params["n"], "law": r.params["law"],
+                    "steps": r.steps, "t": r.t, "latency_s": r.latency})
+    return out
 
 
 def failures(done):
