@@ -83,16 +83,21 @@ def time_form(desc, P0, steps, warmup, transfers=None):
     return t1 - t0, P, avg_ms, launches, name
 
 
-def cpu_baseline(n_sample, target_s=15.0):
-    """The oracle (CPU restatement of the reference step) on the same workload
-    at n_sample^3, OpenMP threads = host cores."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _oracle_rate(n_sample, threads, target_s, max_steps=2000):
     import oracle
-    # the host's CPU share: OMP_NUM_THREADS where the environment sets it
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
     oracle.set_threads(threads)
     desc = problems.turbulence_desc(n_sample)
     P = problems.turbulence_3d(desc)
-    zones = n_sample ** 3
     err, dtc = oracle.courant_dt(desc, P)
     oracle.advance(desc, P, 0.0, CFL * dtc)  # warm-up step
     steps, t0 = 0, time.perf_counter()
@@ -101,12 +106,24 @@ def cpu_baseline(n_sample, target_s=15.0):
         err, _ = oracle.advance(desc, P, 0.0, CFL * dtc)
         steps += 1
         el = time.perf_counter() - t0
-        if el > target_s or steps >= 2000:
+        if el > target_s or steps >= max_steps:
             break
-    return {"value": zones * steps / el, "unit": "zone-updates/s", "cores": threads,
-            "kind": "port",
+    return n_sample ** 3 * steps / el, steps, el
+
+
+def cpu_baseline(n_sample, target_s=15.0):
+    """The oracle (CPU restatement of the reference step) on the same workload
+    at n_sample^3, OpenMP threads = the host's CPU share, plus a shorter
+    1-thread run."""
+    # the host's CPU share: OMP_NUM_THREADS where the environment sets it
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+    rate, steps, el = _oracle_rate(n_sample, threads, target_s)
+    rate1, steps1, el1 = _oracle_rate(32, 1, 5.0, 200)
+    return {"value": rate, "unit": "zone-updates/s", "cores": threads, "kind": "port",
+            "cpu": cpu_model(), "one_thread_value": rate1,
             "sample": f"turbulence_3d at {n_sample}^3 (same generator and scheme), {steps} steps after 1 warm-up "
-                      f"step, {el:.1f} s, CPU restatement of the reference step (oracle/), OpenMP over zones"}
+                      f"step, {el:.1f} s, CPU restatement of the reference step (oracle/), OpenMP over zones; "
+                      f"one thread: 32^3, {steps1} steps, {el1:.1f} s"}
 
 
 def other_workload(args):

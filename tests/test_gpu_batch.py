@@ -167,3 +167,84 @@ def test_serve_loop_continuous_batching():
         assert np.array_equal(r.result, ref[r.index][0]), r.index
     st = serving.latency_stats(done)
     assert st["p99_s"] >= st["p50_s"] > 0
+
+
+def _alone_vs_block(desc, P, steps=40, cfl=0.4):
+    with Block(desc) as blk:
+        blk.upload(P)
+        t, n, _ = blk.run(cfl, 1e300, steps)
+        return blk.download(), t, n
+
+
+def test_two_lanes_with_a_large_tile_solve():
+    """A 256^2 hydro solve makes the tile lane's steps slow, so the pencil
+    lane takes several steps per batch step: every 1-D solve still ends
+    bitwise equal to its single-block run."""
+    prm = serving.make_stream(12, precision="fp64", resize=SMALL)
+    ones = [r for r in prm if r.params["dim"] == 1][:3]
+    assert len(ones) == 3
+    rng = np.random.default_rng(5)
+    params = problems.riemann_params(rng)
+    params.update(dim=2, n=256, law="hydro")
+    big = problems.riemann_desc(params, precision="fp64")
+    Pbig = problems.riemann_problem(params, big)
+    with Batch(BUDGET, rk_order=2, precision="fp64", max_solves=8) as b:
+        sb = b.admit(big, Pbig, serving.CFL, 1e300, 60)
+        slots = {b.admit(r.desc, r.P, serving.CFL, 1e300, 200): r for r in ones}
+        left = 4
+        got = {}
+        while left:
+            b.step(4)
+            for s in b.finished():
+                done, t, steps, st, rep = b.query(s)
+                got[s] = (b.retire(s), t, steps)
+                left -= 1
+    P, t, n = got[sb]
+    Pw, tw, nw = _alone_vs_block(big, Pbig, 60)
+    assert (t, n) == (tw, nw) and np.array_equal(P, Pw)
+    for s, r in slots.items():
+        P, t, n = got[s]
+        Pw, tw, nw = _alone_vs_block(r.desc, r.P, 200)
+        assert (t, n) == (tw, nw) and np.array_equal(P, Pw), r.index
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_fused_3d_solve_in_a_batch(precision):
+    """3-D hydro + PLM solves take the batch's fused tiles (z-marching): equal
+    to the single block bit for bit, beside a 1-D solve."""
+    desc = problems.turbulence_desc(24, precision)
+    P0 = problems.turbulence_3d(problems.turbulence_desc(24, "fp64"), kmax=4)
+    one = serving.make_stream(2, precision=precision, resize=SMALL)[1]
+    with Batch(BUDGET, rk_order=2, precision=precision) as b:
+        s3 = b.admit(desc, P0, 0.3, 1e300, 6)
+        b.admit(one.desc, one.P, serving.CFL, serving.T_FINAL, serving.MAX_STEPS)
+        got = {}
+        while len(got) < 2:
+            b.step(5)
+            for s in b.finished():
+                got[s] = b.retire(s)
+    with Block(desc) as blk:
+        blk.upload(P0)
+        blk.run(0.3, 1e300, 6)
+        Pw = blk.download()
+    assert np.array_equal(got[s3], Pw)
+
+
+def test_rk3_batch_uses_flux_arrays_and_matches_block():
+    """Outside RK2 every solve takes the flux-array kernels: 1-D and 2-D
+    hydro and MHD at RK3 equal their single-block runs."""
+    reqs = serving.make_stream(N_REQ, precision="fp64", resize=SMALL)[:6]
+    descs = []
+    for r in reqs:
+        d = problems.riemann_desc(r.params, precision="fp64", rk_order=3)
+        descs.append(d)
+    with Batch(BUDGET, rk_order=3, precision="fp64") as b:
+        slots = {b.admit(d, r.P, serving.CFL, 1e300, 30): (d, r) for d, r in zip(descs, reqs)}
+        got = {}
+        while len(got) < len(slots):
+            b.step(7)
+            for s in b.finished():
+                got[s] = b.retire(s)
+    for s, (d, r) in slots.items():
+        Pw, _, _ = _alone_vs_block(d, r.P, 30)
+        assert np.array_equal(got[s], Pw), r.index

@@ -302,3 +302,29 @@ def test_xy_symmetry():
     Qt = Q.transpose(1, 0, 2, 3).copy()
     Qt[..., [1, 2]] = Qt[..., [2, 1]]
     assert np.allclose(P, Qt, rtol=0, atol=1e-13)
+
+
+def test_toro_tubes_converge():
+    """Toro's five shock tubes (3rd ed., table 6.2) with PLM + HLLC: the
+    density error against the exact solution falls with resolution."""
+    tubes = [((1.0, 0.75, 1.0), (0.125, 0.0, 0.1), 0.3, 0.2),
+             ((1.0, -2.0, 0.4), (1.0, 2.0, 0.4), 0.5, 0.15),
+             ((1.0, 0.0, 1000.0), (1.0, 0.0, 0.01), 0.5, 0.012),
+             ((5.99924, 19.5975, 460.894), (5.99242, -6.19633, 46.0950), 0.4, 0.035),
+             ((1.0, -19.59745, 1000.0), (1.0, -19.59745, 0.01), 0.8, 0.012)]
+    for L, R, x0, tf in tubes:
+        errs = []
+        for n in (200, 400):
+            d = abi.make_desc(law="hydro", gamma=1.4, recon="plm", plm_theta=1.5, riemann="hllc",
+                              bc="outflow", cells=(n, 1, 1), rk_order=2)
+            P = problems.empty(d)
+            x = problems.cell_centres(d, 0)
+            for q, (a, b) in enumerate(zip(L, R)):
+                P[:, 0, 0, (0, 1, 4)[q]] = np.where(x < x0, a, b)
+            P = problems.apply_bc(d, P)
+            err, t, _ = oracle.run(d, P, 0.4, tf)
+            assert err == 0
+            xi = problems.cell_centres(d, 0)[2:-2]
+            ex = sample(L, R, 1.4, (xi - x0) / tf)
+            errs.append(np.mean(np.abs(P[2:-2, 0, 0, 0] - ex[:, 0])) / np.mean(np.abs(ex[:, 0])))
+        assert errs[0] < 0.1 and errs[1] < errs[0] / 1.3
