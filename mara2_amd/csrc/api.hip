@@ -391,8 +391,7 @@ int mara_block_run(mara_block* b, double cfl, double t_final, int max_steps,
         const int chunk = max_steps - queued < 64 ? max_steps - queued : 64;
         for (int s = 0; s < chunk; ++s)
         {
-            k_ctl_step_begin<<<1, 1, 0, b->stream>>>(b->ctl);
-            k_ctl_step_dt<<<1, 1, 0, b->stream>>>(b->ctl);
+            k_ctl_step_start<<<1, 1, 0, b->stream>>>(b->ctl);
             for (int rk = 0; rk < b->d.rk_order; ++rk) launch_stage(b, rk, 1);
             k_ctl_step_end<<<1, 1, 0, b->stream>>>(b->ctl);
         }

@@ -875,19 +875,15 @@ static __global__ void k_ctl_begin_run(Ctl* ctl, double t, double t_final, doubl
 }
 
 // maraMainLoop: while (t < t_final) { dt = min(cfl * courant, t_final - t); ... }
-static __global__ void k_ctl_step_begin(Ctl* ctl)
+// The CFL of the state at the start of the step sits in the bins (from the
+// standalone reduction on the first step, from the previous step's last
+// stage afterwards).
+static __global__ void k_ctl_step_start(Ctl* ctl)
 {
     const bool go = ctl->active && ctl->t < ctl->t_final && ctl->steps < ctl->max_steps
                     && ctl->c2p_fail == 0 && ctl->eigen_fail == 0 && ctl->superluminal == 0;
     ctl->active = go ? 1 : 0;
-}
-
-// the CFL of the state at the start of the step sits in the bins (from the
-// standalone reduction on the first step, from the previous step's last
-// stage afterwards)
-static __global__ void k_ctl_step_dt(Ctl* ctl)
-{
-    if (!ctl->active) return;
+    if (!go) return;
     fold_bins(ctl);
     const double dt1 = ctl->cfl * bitsd(ctl->courant_bits);
     const double dt2 = ctl->t_final - ctl->t;
