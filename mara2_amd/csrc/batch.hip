@@ -307,14 +307,24 @@ int mara_batch_admit(mara_batch* B, const mara_desc* d, const double* P, double 
     c.first_fail = ~0ull;
     c.courant_bits = DBL_MAX_BITS;
     c.cbins = B->d_bins + (size_t)n * CBINS * CBIN_STRIDE;
-    HIP_TRY(hipMemcpyAsync(B->d_ctl + n, &c, sizeof(Ctl), hipMemcpyHostToDevice, B->stream));
+    // a failed transfer gives the slot and the pool range back
+    auto undo = [&]() {
+        (void)hipStreamSynchronize(B->stream);
+        B->pool.release(sl.off, sl.bytes);
+        sl.used = false;
+        std::memset(&c, 0, sizeof c);
+        return MARA_ERR_HIP;
+    };
+    if (!hip_ok(hipMemcpyAsync(B->d_ctl + n, &c, sizeof(Ctl), hipMemcpyHostToDevice, B->stream))) return undo();
     kb_reset_bins<<<1, 1, 0, B->stream>>>(B->d_ctl + n);
     // primitives: host -> staging (U) -> Pa, guards included; Pb starts as a copy
-    HIP_TRY(hipMemcpyAsync(it.U, P, sl.ncell * it.G.nq * sizeof(double), hipMemcpyHostToDevice, B->stream));
+    if (!hip_ok(hipMemcpyAsync(it.U, P, sl.ncell * it.G.nq * sizeof(double), hipMemcpyHostToDevice, B->stream)))
+        return undo();
     k_aos_to_soa<<<(unsigned)((sl.ncell + 255) / 256), 256, 0, B->stream>>>(it.U, it.Pa, it.G);
-    HIP_TRY(hipMemcpyAsync(it.Pb, it.Pa, cb, hipMemcpyDeviceToDevice, B->stream));
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipStreamSynchronize(B->stream));  // the host buffer may go away after return
+    if (!hip_ok(hipMemcpyAsync(it.Pb, it.Pa, cb, hipMemcpyDeviceToDevice, B->stream))) return undo();
+    if (!hip_ok(hipGetLastError())) return undo();
+    // the host buffer may go away after return
+    if (!hip_ok(hipStreamSynchronize(B->stream))) return undo();
     if (n + 1 > B->n_items) B->n_items = n + 1;
     B->dirty = true;
     *slot = n;
