@@ -78,6 +78,21 @@ __device__ __forceinline__ void hydro_face_flux(const R (&S)[5][2 * (Recon == 0 
     else hlle<R, 5>(SL, SR, F);
 }
 
+// U = U(P0) of the binary32 flux form: p / (gamma - 1) of
+// NewtonianHydro::fromPrimitive (src/ConservationLaws.cpp:253-290) as a
+// product with 1 / (gamma - 1), which the tile computes once.  The mixed form
+// is held to a tolerance, not to the reference's rounding; the binary64 form
+// keeps prim_to_cons.
+__device__ __forceinline__ void hydro_cons_mixed(double igm1, const double* P, double* U)
+{
+    const double vv = P[1] * P[1] + P[2] * P[2] + P[3] * P[3];
+    U[0] = P[0];
+    U[1] = P[0] * P[1];
+    U[2] = P[0] * P[2];
+    U[3] = P[0] * P[3];
+    U[4] = P[0] * 0.5 * vv + P[4] * igm1;
+}
+
 template <int Recon, int Riem, typename R, bool Z>
 static __global__ void __launch_bounds__(FUSED_TX * FUSED_TY)
 k_fused_hydro(const double* __restrict__ Pin, const double* __restrict__ P0, double* __restrict__ U,
@@ -111,6 +126,7 @@ k_fused_hydro(const double* __restrict__ Pin, const double* __restrict__ P0, dou
     const int ke = Z ? min(kb + FUSED_KZ, G.N[2] - G.g[2]) : 1;
     const long long sxy = (long long)G.N[0] * G.N[1];
     const R theta = R(G.theta), gamma = R(G.gamma);
+    const double igm1 = 1.0 / (G.gamma - 1.0);  // binary32 form's U(P0)
     const R nx[3] = {R(G.nhat[0]), R(G.nhat[1]), R(G.nhat[2])};
     const R ny[3] = {R(G.nhat[3]), R(G.nhat[4]), R(G.nhat[5])};
     const R nz[3] = {R(G.nhat[6]), R(G.nhat[7]), R(G.nhat[8])};
@@ -242,7 +258,8 @@ k_fused_hydro(const double* __restrict__ Pin, const double* __restrict__ P0, dou
             double p0[5], U0[5];
 #pragma unroll
             for (int q = 0; q < 5; ++q) p0[q] = P0[q * G.fs + c];
-            prim_to_cons<0>(G.gamma, p0, U0);
+            if constexpr (sizeof(R) == 4) hydro_cons_mixed(igm1, p0, U0);
+            else prim_to_cons<0>(G.gamma, p0, U0);
             double Un[5];
 #pragma unroll
             for (int q = 0; q < 5; ++q)
@@ -257,13 +274,14 @@ k_fused_hydro(const double* __restrict__ Pin, const double* __restrict__ P0, dou
             }
             double Pn[5];
             int floor;
-            cons_to_prim<0>(G.gamma, G.pfloor, Un, Pn, floor);
+            if constexpr (sizeof(R) == 4) cons_to_prim_mixed<0>(G.gamma, G.pfloor, Un, Pn, floor);
+            else cons_to_prim<0>(G.gamma, G.pfloor, Un, Pn, floor);
 #pragma unroll
             for (int q = 0; q < 5; ++q) Pout[q * G.fs + c] = Pn[q];
             if (courant_out)
             {
                 int err = 0;
-                const double d = courant_of<0>(Pn, G, err);
+                const double d = courant_of<0, R>(Pn, G, err);
                 if (d < dmin) dmin = d;
             }
             if (write_u)
@@ -354,6 +372,7 @@ __device__ __forceinline__ void fused_plm_tile(const double* __restrict__ Pin, c
     const int ke = Z ? min(kb + FUSED_KZ, G.N[2] - G.g[2]) : 1;
     const long long sxy = (long long)G.N[0] * G.N[1];
     const R theta = R(G.theta), gamma = R(G.gamma);
+    const double igm1 = 1.0 / (G.gamma - 1.0);  // binary32 form's U(P0)
     const R nx[3] = {R(G.nhat[0]), R(G.nhat[1]), R(G.nhat[2])};
     const R ny[3] = {R(G.nhat[3]), R(G.nhat[4]), R(G.nhat[5])};
     const R nz[3] = {R(G.nhat[6]), R(G.nhat[7]), R(G.nhat[8])};
@@ -563,7 +582,8 @@ __device__ __forceinline__ void fused_plm_tile(const double* __restrict__ Pin, c
             double p0[5], U0[5];
 #pragma unroll
             for (int q = 0; q < 5; ++q) p0[q] = P0[q * G.fs + c];
-            prim_to_cons<0>(G.gamma, p0, U0);
+            if constexpr (sizeof(R) == 4) hydro_cons_mixed(igm1, p0, U0);
+            else prim_to_cons<0>(G.gamma, p0, U0);
             double Un[5];
 #pragma unroll
             for (int q = 0; q < 5; ++q)
@@ -578,13 +598,14 @@ __device__ __forceinline__ void fused_plm_tile(const double* __restrict__ Pin, c
             }
             double Pn[5];
             int floor;
-            cons_to_prim<0>(G.gamma, G.pfloor, Un, Pn, floor);
+            if constexpr (sizeof(R) == 4) cons_to_prim_mixed<0>(G.gamma, G.pfloor, Un, Pn, floor);
+            else cons_to_prim<0>(G.gamma, G.pfloor, Un, Pn, floor);
 #pragma unroll
             for (int q = 0; q < 5; ++q) Pout[q * G.fs + c] = Pn[q];
             if (courant_out)
             {
                 int err = 0;
-                const double d = courant_of<0>(Pn, G, err);
+                const double d = courant_of<0, R>(Pn, G, err);
                 if (d < dmin) dmin = d;
             }
             if (write_u)
@@ -675,13 +696,6 @@ inline void launch_fused_r(const mara_desc& d, const Geom& G, hipStream_t s, con
             else launch_fused_t<2, 0, R>(G, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl);
             break;
     }
-}
-
-inline void launch_fused(const mara_desc& d, const Geom& G, hipStream_t s, const double* Pin, const double* P0,
-                         double* U, double* Pout, double b, int first, int write_u, int cout, Ctl* ctl)
-{
-    if (d.precision == MARA_PREC_MIXED) launch_fused_r<float>(d, G, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl);
-    else launch_fused_r<double>(d, G, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl);
 }
 
 } // namespace mara

@@ -221,29 +221,36 @@ __device__ __forceinline__ double ct_value(const R* __restrict__ Fx, const R* __
 // ---------------------------------------------------------------------------
 // CFL time step of one zone (FieldOperator::getCourantTimestep, src/FieldOperator.cpp:125-169)
 // ---------------------------------------------------------------------------
-template <int Law>
+// R is the arithmetic of the wave speeds: binary64 for the FP64 form (the
+// reference's), binary32 for the hydro and MHD laws in the mixed form.
+template <int Law, typename R = double>
 __device__ __forceinline__ double courant_of(const double* p, const Geom& G, int& err)
 {
     constexpr int NQ = LawNQ<Law>::value;
-    double lam[3];
+    R pr[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) pr[q] = R(p[q]);
+    R lam[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a)
     {
-        State<double, NQ> S;
-        err |= law_state<Law, double>(G.gamma, p, G.nhat + 3 * a, S);
-        double m = 0.0;
+        const R n[3] = {R(G.nhat[3 * a]), R(G.nhat[3 * a + 1]), R(G.nhat[3 * a + 2])};
+        State<R, NQ> S;
+        err |= law_state<Law, R>(R(G.gamma), pr, n, S);
+        R m = R(0);
 #pragma unroll
-        for (int n = 0; n < NQ; ++n)
+        for (int q = 0; q < NQ; ++q)
         {
-            const double x = fabs(S.A[n]);
+            const R x = Math<R>::abs(S.A[q]);
             if (m < x) m = x;
         }
         lam[a] = m;
     }
-    double lmax = lam[0];
+    R lmax = lam[0];
     if (lmax < lam[1]) lmax = lam[1];
     if (lmax < lam[2]) lmax = lam[2];
-    return G.Lmin / lmax;
+    if constexpr (sizeof(R) == 8) return G.Lmin / lmax;
+    else return double(Math<R>::div(R(G.Lmin), lmax));
 }
 
 template <int Law>
@@ -345,8 +352,11 @@ __device__ __forceinline__ void write_guard_images(double* __restrict__ P, const
 // U0 is regenerated from the step's starting primitives P0.
 // ---------------------------------------------------------------------------
 // The zone update once the flux divergence L[q] is known.  Returns the
-// zone's CFL time step when courant_out is set (else DBL_MAX).
-template <int Law>
+// zone's CFL time step when courant_out is set (else DBL_MAX).  R is the flux
+// arithmetic: with binary32 fluxes (the mixed form) the hydro and MHD laws
+// recover primitives with cons_to_prim_mixed and take the CFL wave speeds in
+// binary32.
+template <int Law, typename R = double>
 __device__ __forceinline__ double finish_zone(const double* __restrict__ P0, double* __restrict__ U,
                                               double* __restrict__ Pout, const Geom& G, double b, int first_stage,
                                               int write_u, int courant_out, Ctl* ctl, int i, int j, int k,
@@ -369,9 +379,13 @@ __device__ __forceinline__ double finish_zone(const double* __restrict__ P0, dou
         const double Uq = first_stage ? U0[q] : U[q * G.fs + c];
         Un[q] = U0[q] * (1 - b) + (Uq + dt * L[q]) * b;
     }
+    constexpr bool MIX = sizeof(R) == 4 && Law < 2;
     double Pn[NQ];
     int floor = 0;
-    if (cons_to_prim<Law>(G.gamma, G.pfloor, Un, Pn, floor) != 0)
+    int failed;
+    if constexpr (MIX) failed = cons_to_prim_mixed<Law>(G.gamma, G.pfloor, Un, Pn, floor);
+    else failed = cons_to_prim<Law>(G.gamma, G.pfloor, Un, Pn, floor);
+    if (failed != 0)
     {
         atomicAdd(&ctl->c2p_fail, 1ull);
         atomicMin(&ctl->first_fail, ref_linear(G, i, j, k));
@@ -390,7 +404,8 @@ __device__ __forceinline__ double finish_zone(const double* __restrict__ P0, dou
             // the next step's CFL from the zone's new primitives (interior only,
             // as FieldOperator::getCourantTimestep reads them)
             int err = 0;
-            dtc = courant_of<Law>(Pn, G, err);
+            if constexpr (MIX) dtc = courant_of<Law, float>(Pn, G, err);
+            else dtc = courant_of<Law>(Pn, G, err);
             if (err & ERR_SUPERLUMINAL) atomicAdd(&ctl->superluminal, 1ull);
             if (err & ERR_EIGEN) atomicAdd(&ctl->eigen_fail, 1ull);
         }
@@ -458,7 +473,7 @@ __device__ __forceinline__ double update_zone(const double* __restrict__ P0, dou
             L[q] = divergence_R<R>(G, x1, x0, y1, y0, z1, z0);
         }
     }
-    return finish_zone<Law>(P0, U, Pout, G, b, first_stage, write_u, courant_out, ctl, i, j, k, L);
+    return finish_zone<Law, R>(P0, U, Pout, G, b, first_stage, write_u, courant_out, ctl, i, j, k, L);
 }
 
 // ---------------------------------------------------------------------------
@@ -531,7 +546,7 @@ __device__ __forceinline__ double pencil_tile(const double* __restrict__ Pin, co
             else
                 L[q] = divergence_R<R>(G, fr, fl, R(0), R(0), R(0), R(0));
         }
-        dtc = finish_zone<Law>(P0, U, Pout, G, b, first_stage, write_u, courant_out, ctl, zi, 0, 0, L);
+        dtc = finish_zone<Law, R>(P0, U, Pout, G, b, first_stage, write_u, courant_out, ctl, zi, 0, 0, L);
     }
     return dtc;
 }

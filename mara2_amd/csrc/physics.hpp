@@ -815,4 +815,48 @@ __device__ __forceinline__ int cons_to_prim(double gamma, double pfloor, const d
     }
 }
 
+// fromConserved of the mixed form for the hydro and MHD laws: binary64 like
+// cons_to_prim, with one reciprocal of the density in place of its four
+// quotients (the mixed form is held to a tolerance, not to the reference's
+// rounding).  Failure and floor semantics as cons_to_prim.  The RMHD law
+// keeps cons_to_prim.
+template <int Law>
+__device__ __forceinline__ int cons_to_prim_mixed(double gamma, double pfloor, const double* U, double* P,
+                                                  int& floor)
+{
+    static_assert(Law == 0 || Law == 1, "hydro and MHD only");
+    floor = 0;
+    const double gm1 = gamma - 1.0;
+    const double ir = 1.0 / U[0];
+    const double pp = U[1] * U[1] + U[2] * U[2] + U[3] * U[3];
+    P[0] = U[0];
+    P[1] = U[1] * ir;
+    P[2] = U[2] * ir;
+    P[3] = U[3] * ir;
+    if constexpr (Law == 0)
+    {
+        P[4] = (U[4] - 0.5 * pp * ir) * gm1;
+        return 0;
+    }
+    else
+    {
+        const double BB = U[5] * U[5] + U[6] * U[6] + U[7] * U[7];
+        P[4] = (U[4] - 0.5 * pp * ir - 0.5 * BB) * gm1;
+        P[5] = U[5];
+        P[6] = U[6];
+        P[7] = U[7];
+        if (P[4] < 0.0 || P[0] < 0.0 || U[0] < 0.0 || U[4] < 0.0)
+        {
+            if (P[4] < 0.0 && pfloor > 0.0)
+            {
+                P[4] = pfloor * P[0];
+                floor = 1;
+                return 0;
+            }
+            return 1;
+        }
+        return 0;
+    }
+}
+
 } // namespace mara
