@@ -55,8 +55,9 @@ def traffic_from_profile(kernel, workload):
     return None
 
 
-def time_form(desc, P0, steps, warmup, transfers=None):
-    """Upload, warm up, then time `steps` device-driven steps.  transfers, if
+def time_form(desc, P0, steps, warmup, transfers=None, tune=None):
+    """Upload, warm up, let the autotuner pick the launch shape (tune, if a
+    dict, receives it), then time `steps` device-driven steps.  transfers, if
     a dict, receives the host<->device times of the MeshData::P arrays."""
     with Block(desc) as blk:
         t0 = time.perf_counter()
@@ -66,6 +67,9 @@ def time_form(desc, P0, steps, warmup, transfers=None):
         if warmup:
             blk.run(CFL, 1e300, warmup)
             blk.upload(P0)
+        if tune is not None:
+            shape, ms = blk.autotune(repeats=5)
+            tune.update(shape.as_dict(), stage_ms=ms)
         # stage-kernel durations from HIP event pairs recorded on the block's
         # stream around every stage launch of the timed region (no host wait
         # between them; read after the run)
@@ -135,7 +139,16 @@ def other_workload(args):
         d = problems.sod_desc(1024)
         P0 = problems.sod_1d(d)
         with Block(d) as blk:
+            blk.set_graphs(not args.no_graphs)
             blk.upload(P0)
+            blk.run(0.4, 0.2, 1 << 30)  # warm-up: kernel loading and graph capture
+            blk.upload(P0)
+            shape = None
+            if not args.no_autotune:
+                shape, _ = blk.autotune(repeats=20)
+                shape = shape.as_dict()
+                blk.run(0.4, 0.2, 1 << 30)  # graph capture in the tuned shape
+                blk.upload(P0)
             t0 = time.perf_counter()
             t, steps, rep = blk.run(0.4, 0.2, 1 << 30)
             el = time.perf_counter() - t0
@@ -143,7 +156,8 @@ def other_workload(args):
                "steps": steps, "warmup": 0, "ms_per_step": el / steps * 1e3, "higher_is_better": True,
                "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic: Sod tube",
                "config": {"workload": "1-D Sod tube 1024 zones, gamma 1.4, PLM + HLLE, RK2, CFL 0.4, to t = 0.2",
-                          "t_reached": t, "time_to_solution_s": el}}
+                          "t_reached": t, "time_to_solution_s": el, "graphs": not args.no_graphs,
+                          "launch_shape": shape}}
         print(json.dumps(out), flush=True)
         return
     if w == "mhd2d":
@@ -163,6 +177,12 @@ def other_workload(args):
     for form in ("fp64", "mixed"):
         d = make_desc(n, form)
         with Block(d) as blk:
+            # warm-up (kernel loading, graph capture) and the launch shape
+            blk.upload(P0)
+            blk.run(cfl, 1e300, 2, check=False)
+            shape = None
+            if not args.no_autotune:
+                shape = blk.autotune(repeats=5)[0].as_dict()
             # the maraMainLoop contract: stops at the first failure, as the reference does
             blk.upload(P0)
             t0 = time.perf_counter()
@@ -190,7 +210,7 @@ def other_workload(args):
             blk.run(cfl, 1e300, 2, check=False)
             ms, nl, name = blk.stage_kernel_time()
         res[form] = dict(el=el, steps=steps, P=P, rep=rep.as_dict() if rep else None, mono=mono, ms=ms, name=name,
-                         timed=timed)
+                         timed=timed, shape=shape)
         log(f"{form}: {steps} steps in {el:.3f} s, report {res[form]['rep']}, max |monopole| {mono:.3e}"
             + (f"; fixed-dt timing {timed[1]} steps in {timed[0]:.3f} s" if timed else ""))
     zones = int(np.prod([d64.cells[a] for a in range(3)]))
@@ -212,7 +232,9 @@ def other_workload(args):
            "steps": st64, "warmup": 0, "ms_per_step": ms64,
            "steps_before_failure": {"fp64": r64["steps"], "mixed": res["mixed"]["steps"]},
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-           "data": "synthetic: seeded Gaussian random fields", "config": {"workload": label, "zones": zones},
+           "data": "synthetic: seeded Gaussian random fields",
+           "config": {"workload": label, "zones": zones, "launch_shape": r64["shape"],
+                      "launch_shape_mixed": res["mixed"]["shape"]},
            "roofline": {"bound": "hbm", "achieved": per_launch / (r64["ms"] * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9,
                         "unit": "GB/s", "frac": per_launch / (r64["ms"] * 1e-3) / HBM_PEAK, "traffic": None,
                         "kernel": r64["name"], "avg_launch_ms": r64["ms"]},
@@ -261,17 +283,27 @@ def serving_workload(args):
     work = [r.zones for r in reqs]
     log("warm-up")
     warm = serving.make_stream(4, precision="mixed", resize=lambda dim, n: 32 if dim == 1 else 16)
-    serving.sequential(warm)
-    serving.serve(warm, 1 << 28, max_solves=8, chunk=args.chunk)
+    graphs = not args.no_graphs
+    serving.sequential(warm, graphs=graphs)
+    serving.serve(warm, 1 << 28, max_solves=8, chunk=args.chunk, graphs=graphs)
     stride = max(1, n_req // args.seq_sample)
     sample = reqs[::stride][:args.seq_sample]
     log(f"one-at-a-time service of {len(sample)} requests (every {stride}th)")
-    times = serving.sequential(sample)
+    times = serving.sequential(sample, graphs=graphs)
     seq_failed = [r.index for r in serving.failures(sample)]
     seq_rate = len(sample) / sum(times)
     seq_zone_rate = sum(r.zones * r.steps for r in sample) / sum(times)
     log(f"sequential: {seq_rate:.3f} solves/s, mean {np.mean(times):.3f} s, max {max(times):.3f} s")
     budget = int(args.budget_gib * 2**30)
+    shape = None
+    if not args.no_autotune:
+        # the batch's pencil length, tuned on the stream's median 1-D request
+        ones = sorted((r for r in reqs if r.params["dim"] == 1), key=lambda r: r.zones)
+        rep = ones[len(ones) // 2]
+        with Block(rep.desc) as blk:
+            blk.upload(rep.P)
+            shape, _ = blk.autotune(repeats=20)
+        log(f"pencil length {shape.pencil} (tuned on request {rep.index}, {rep.zones} zones)")
     out = {"metric": METRIC, "value": None, "unit": "solves/s", "n_gpus": 1, "steps": None,
            "warmup": None, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f32/f64 mixed",
@@ -279,7 +311,8 @@ def serving_workload(args):
            "config": {"workload": f"serving stream of {n_req} Riemann problems, 70% 1-D 256..16384 zones, "
                                   "30% 2-D 128^2..1024^2, half MHD, PLM + HLLC/HLLE, RK2, CFL 0.4, to t = 0.2, "
                                   "mixed form, Poisson arrivals at 2x the one-at-a-time rate",
-                      "zones_total": int(sum(work)), "hbm_pool_gib": args.budget_gib, "chunk_steps": args.chunk},
+                      "zones_total": int(sum(work)), "hbm_pool_gib": args.budget_gib, "chunk_steps": args.chunk,
+                      "graphs": graphs, "launch_shape": shape.as_dict() if shape is not None else None},
            "arrival_rate": 2.0 * seq_rate,
            "sequential": {"solves_per_s": seq_rate, "zone_updates_per_s": seq_zone_rate,
                           "sample": f"{len(sample)} requests, every {stride}th of the stream",
@@ -291,7 +324,7 @@ def serving_workload(args):
         serving.set_arrivals(reqs, 2.0 * seq_rate)
         log(f"serving {n_req} requests, Poisson arrivals at {2 * seq_rate:.3f}/s "
             f"(span {reqs[-1].arrival:.0f} s), pool {args.budget_gib:g} GiB")
-        wall, done = serving.serve(reqs, budget, chunk=args.chunk, log=log)
+        wall, done = serving.serve(reqs, budget, chunk=args.chunk, log=log, graphs=graphs, shape=shape)
         failed = [r.index for r in serving.failures(done)]
         st = serving.latency_stats(done)
         zone_updates = sum(r.zones * r.steps for r in done)
@@ -305,7 +338,7 @@ def serving_workload(args):
         for r in reqs:
             r.arrival = 0.0
         log("saturated run: every request at t = 0")
-        wall_s, done_s = serving.serve(reqs, budget, chunk=args.chunk, log=log)
+        wall_s, done_s = serving.serve(reqs, budget, chunk=args.chunk, log=log, graphs=graphs, shape=shape)
         out["saturated"] = {"solves_per_s": len(done_s) / wall_s, "wall_s": wall_s,
                             "speedup_vs_sequential": len(done_s) / wall_s / seq_rate,
                             "latency_s": serving.latency_stats(done_s),
@@ -335,6 +368,9 @@ def main():
     ap.add_argument("--chunk", type=int, default=16, help="serving: steps per batch launch sequence")
     ap.add_argument("--saturate", action="store_true", help="serving: also run with every request at t = 0")
     ap.add_argument("--no-open-loop", action="store_true", help="serving: skip the Poisson-arrival run")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="sod, serving: launch every kernel from the host instead of replaying HIP graphs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -357,10 +393,13 @@ def main():
 
     log("timing FP64 form")
     transfers = {}
-    el64, P64, ms64, nl64, kname = time_form(d64, P0, args.steps, args.warmup, transfers)
-    log(f"FP64: {el64:.3f} s for {args.steps} steps; stage kernel {ms64:.3f} ms ({kname})")
+    tune64 = None if args.no_autotune else {}
+    tunemx = None if args.no_autotune else {}
+    el64, P64, ms64, nl64, kname = time_form(d64, P0, args.steps, args.warmup, transfers, tune64)
+    log(f"FP64: {el64:.3f} s for {args.steps} steps; stage kernel {ms64:.3f} ms ({kname}); shape {tune64}")
     log("timing mixed form")
-    elmx, Pmx, msmx, nlmx, _ = time_form(dmx, P0, args.steps, args.warmup)
+    elmx, Pmx, msmx, nlmx, _ = time_form(dmx, P0, args.steps, args.warmup, None, tunemx)
+    log(f"mixed: shape {tunemx}")
     log(f"mixed: {elmx:.3f} s for {args.steps} steps; stage kernel {msmx:.3f} ms")
 
     g = d64.guards[0]
@@ -392,7 +431,8 @@ def main():
         "dtype": "f64",
         "data": f"synthetic: seeded Gaussian random velocity field (seed {problems.SEED_TURBULENCE})",
         "config": {"workload": f"3-D decaying turbulence {n}^3, hydro gamma 5/3, PLM theta 1.5 + HLLC, "
-                               f"RK2, CFL {CFL}, periodic", "zones": zones, "form": "fp64"},
+                               f"RK2, CFL {CFL}, periodic", "zones": zones, "form": "fp64",
+                   "launch_shape": tune64, "launch_shape_mixed": tunemx},
         "roofline": roofline,
         # the C ABI takes host arrays: PCIe-inclusive rate of one upload, the
         # timed steps and one download of MeshData::P (not the headline value)

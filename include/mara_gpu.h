@@ -148,6 +148,13 @@ int mara_block_monopole_max(mara_block* block, double* max_abs);
 int mara_block_run(mara_block* block, double cfl, double t_final, int max_steps,
                    double* t, int* steps, mara_report* report);
 
+/* mara_block_run replays its steps from a HIP graph of 16 steps, captured on
+   the first run (enable=1, the default), or launches every kernel from the
+   host (enable=0).  Results are the same either way; runs with stage timing
+   enabled launch from the host.  No counterpart in the reference: the
+   reference steps on the host. */
+int mara_block_set_graphs(mara_block* block, int enable);
+
 /* Kernel timing on the block's stream: a pair of HIP events around each
    launch of the stage kernels (the fused kernel, or the flux and update
    kernels), recorded without host waits, for up to 512 stages.
@@ -158,6 +165,45 @@ int mara_block_set_timing(mara_block* block, int enable);
    receives a static string naming the kernel. */
 int mara_block_stage_kernel_time(mara_block* block, double* avg_ms, long long* launches,
                                  const char** name);
+
+/* ---------------------------------------------------------------------------
+   Launch shapes and the autotuner (no counterpart in the reference, which
+   has no kernels).  A launch shape divides a stage's zones among
+   work-groups differently; it never changes the arithmetic, so every shape
+   gives bit-identical results.
+   --------------------------------------------------------------------------- */
+typedef struct mara_shape
+{
+    int pencil;  /* 1-D stages: faces per pencil work-group, 128, 256 or 512
+                    (the relativistic law runs 256 whatever is set) */
+    int zplanes; /* fused 3-D hydro stages: z planes a work-group marches,
+                    4, 8, 16 or 32 */
+    int rows;    /* flux-array stages: work-group rows of 64 zones, 1, 2 or 4 */
+} mara_shape;
+
+/* The block's launch shape.  A new block starts from the tuning table's
+   entry for its descriptor, else from pencil 256, zplanes 16, rows 4.
+   set: MARA_ERR_INVALID_ARGUMENT for a value outside the lists above. */
+int mara_block_get_shape(const mara_block* block, mara_shape* shape);
+int mara_block_set_shape(mara_block* block, const mara_shape* shape);
+
+/* Autotuner: times the candidates of the shape the block's stages use
+   (pencil for 1-D blocks of two or three stages, zplanes for fused 3-D
+   hydro blocks, rows for the flux-array path; nothing to choose for fused
+   2-D blocks), `repeats` launches of a first RK stage each after one
+   untimed launch, on the block's own state with dt = 0 and a scratch
+   control block: the state, clock and counters are untouched.  Sets the
+   fastest candidate on the block and records it in the process's tuning
+   table under the block's descriptor.  *chosen (optional) receives the
+   shape, *best_ms (optional) the fastest mean stage time in ms. */
+int mara_block_autotune(mara_block* block, int repeats, mara_shape* chosen, double* best_ms);
+
+/* The process's tuning table: look up the shape recorded for a descriptor
+   (returns 1 and fills *shape when found, else 0), write every entry to a
+   text file, or merge the entries of such a file. */
+int mara_tune_lookup(const mara_desc* desc, mara_shape* shape);
+int mara_tune_save(const char* path);
+int mara_tune_load(const char* path);
 
 /* Bytes of device memory the block holds (state, fluxes, scratch). */
 long long mara_block_device_bytes(const mara_block* block);
@@ -209,6 +255,18 @@ int mara_batch_admit(mara_batch* batch, const mara_desc* desc, const double* P, 
    grouped into calls.  *n_active receives the number of admitted solves not
    yet finished. */
 int mara_batch_step(mara_batch* batch, int n_steps, int* n_active);
+
+/* With enable=1 (the default) mara_batch_step launches one round of kernels
+   (a step of the large solves and the small 1-D solves' steps beside it) as a
+   HIP graph, captured when the set of solves changed and replayed n_steps
+   times; with enable=0 it launches every kernel from the host.  Results are
+   the same either way.  No counterpart in the reference. */
+int mara_batch_set_graphs(mara_batch* batch, int enable);
+
+/* Launch shape of the batch's pencil tiles (1-D solves) and fused 3-D
+   tiles; rows is not used by batches.  Takes effect at the next step. */
+int mara_batch_get_shape(const mara_batch* batch, mara_shape* shape);
+int mara_batch_set_shape(mara_batch* batch, const mara_shape* shape);
 
 /* State of one solve as of the last mara_batch_step: *done is 1 once it
    reached t_final, its step limit or a failure.  Returns the solve's status

@@ -55,6 +55,28 @@ class MaraDesc(ctypes.Structure):
     ]
 
 
+class MaraShape(ctypes.Structure):
+    """mara_shape: launch shape of the stage kernels (pencil length, z planes
+    per fused work-group, flux-array work-group rows)."""
+    _fields_ = [
+        ("pencil", ctypes.c_int),
+        ("zplanes", ctypes.c_int),
+        ("rows", ctypes.c_int),
+    ]
+
+    def as_dict(self):
+        return {"pencil": self.pencil, "zplanes": self.zplanes, "rows": self.rows}
+
+
+PENCIL_SHAPES = (128, 256, 512)
+ZPLANE_SHAPES = (4, 8, 16, 32)
+ROW_SHAPES = (1, 2, 4)
+
+
+def make_shape(pencil=256, zplanes=16, rows=4):
+    return MaraShape(pencil, zplanes, rows)
+
+
 class MaraReport(ctypes.Structure):
     _fields_ = [
         ("status", ctypes.c_int),

@@ -8,7 +8,11 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <new>
+#include <string>
+#include <vector>
 
 #include "../../include/mara_gpu.h"
 #include "kernels.hpp"
@@ -61,9 +65,12 @@ Sizes sizes_of(const mara_desc* d)
 
 long long device_bytes(const mara_desc* d, const Sizes& s)
 {
+    // fused and pencil blocks compute each stage's fluxes inside the stage
+    // pass (no flux arrays) and step Pa -> Pb (-> Pc) -> Pa
+    const bool in_pass = fused_supported(d) || pencil_supported(d);
     long long b = 3ll * s.nq * s.fs * 8;  // Pa, Pb, U
-    if (fused_supported(d) && d->rk_order == 3) b += 1ll * s.nq * s.fs * 8;  // Pc
-    if (!fused_supported(d))
+    if (in_pass && d->rk_order == 3) b += 1ll * s.nq * s.fs * 8;  // Pc
+    if (!in_pass)
         b += (long long)(s.fleshed[0] + s.fleshed[1] + s.fleshed[2]) * s.nq * s.ffs * s.real;
     b += 256 + CBINS * CBIN_STRIDE * 8;  // control block and CFL bins
     return b;
@@ -84,6 +91,10 @@ struct mara_block
     double* Pc = nullptr;  // third stage buffer of the fused RK3 schedule
     double* U = nullptr;
     void* F[3] = {nullptr, nullptr, nullptr};
+    int pencil = 0;  // 1-D pencil tile length (pencil_supported), else 0
+    int kz = FUSED_KZ;           // fused 3-D kernels: z planes per work-group
+    int rows = ROWS_DEFAULT;     // flux-array kernels: work-group rows
+    int pencil_shape = PENCIL_DEFAULT;  // requested pencil length (all laws)
     Ctl* ctl = nullptr;
     Ctl* ctl_host = nullptr;
     unsigned long long* cbins = nullptr;
@@ -96,6 +107,13 @@ struct mara_block
     int ev_used = 0;
     double timed_ms = 0.0;
     long long timed_launches = 0;
+    // HIP graph of kGraphSteps device-driven steps (control kernels and stage
+    // launches), captured by the first mara_block_run that may use it and
+    // replayed by later ones; the step's buffers and arguments never change
+    // over the block's life, and steps past the stop condition are no-ops
+    static constexpr int kGraphSteps = 16;
+    bool graphs = true;
+    hipGraphExec_t steps_exec = nullptr;
 };
 
 namespace {
@@ -135,14 +153,13 @@ void launch_stage(mara_block* b, int rk, int courant_out)
     const double* Pin = rk == 0 ? b->Pa : b->Pb;
     double* Pout = last ? b->Pa : b->Pb;
     const bool mixed = b->d.precision == MARA_PREC_MIXED;
-    if (b->fused)
+    if (b->fused || b->pencil)
     {
-        // the fused kernel reads neighbouring zones of its input while other
-        // work-groups write their output, so input and output never alias:
-        // RK1 Pa -> Pb (then swap), RK2 Pa -> Pb -> Pa, RK3 Pa -> Pb -> Pc -> Pa
-        const int order = b->d.rk_order;
+        // the fused and pencil kernels read neighbouring zones of their input
+        // while other work-groups write their output, so input and output
+        // never alias: RK2 Pa -> Pb -> Pa, RK3 Pa -> Pb -> Pc -> Pa
         Pin = rk == 0 ? b->Pa : (rk == 1 ? b->Pb : b->Pc);
-        Pout = order == 1 ? b->Pb : (last ? b->Pa : (rk == 0 ? b->Pb : b->Pc));
+        Pout = last ? b->Pa : (rk == 0 ? b->Pb : b->Pc);
     }
 
     const bool timed = b->timing && b->ev_used < mara_block::kTimingPairs;
@@ -163,6 +180,9 @@ void launch_stage(mara_block* b, int rk, int courant_out)
     s.ctl = b->ctl;
     s.fused = b->fused;
     s.ct = b->S.ct;
+    s.pencil = b->pencil;
+    s.kz = b->kz;
+    s.rows = b->rows;
     if (mixed) stage_kernels_f32(s);
     else stage_kernels<double>(s);
     if (timed)
@@ -173,12 +193,84 @@ void launch_stage(mara_block* b, int rk, int courant_out)
     // the flux-array update writes the ghost zones of Pout itself
     // (write_guard_images); the fused kernels leave them to a ghost-fill pass
     if (b->fused) launch_bc(b, Pout);
-    if (b->fused && b->d.rk_order == 1)
+}
+
+// ---- tuning table ----------------------------------------------------------
+// Shapes chosen by the autotuner, keyed by what selects and sizes a block's
+// stage kernels: law, reconstruction, Riemann solver, flux-CT, precision, RK
+// order and cell counts.
+std::mutex tune_mutex;
+std::map<std::string, mara_shape> tune_table;
+
+std::string tune_key(const mara_desc& d)
+{
+    char k[160];
+    std::snprintf(k, sizeof k, "%d %d %d %d %d %d %d %d %d", d.law, d.recon, d.riemann,
+                  d.law != MARA_LAW_HYDRO && d.field_ct ? 1 : 0, d.precision, d.rk_order, d.cells[0], d.cells[1],
+                  d.cells[2]);
+    return k;
+}
+
+bool tune_find(const mara_desc& d, mara_shape* s)
+{
+    std::lock_guard<std::mutex> lock(tune_mutex);
+    auto it = tune_table.find(tune_key(d));
+    if (it == tune_table.end()) return false;
+    *s = it->second;
+    return true;
+}
+
+mara_shape block_shape(const mara_block* b)
+{
+    mara_shape s;
+    s.pencil = b->pencil_shape;
+    s.zplanes = b->kz;
+    s.rows = b->rows;
+    return s;
+}
+
+void apply_shape(mara_block* b, const mara_shape& s)
+{
+    b->pencil_shape = s.pencil;
+    if (b->pencil) b->pencil = pencil_length(b->d.law, s.pencil);
+    b->kz = s.zplanes;
+    b->rows = s.rows;
+    // the captured steps hold the old launch shape
+    if (b->steps_exec) (void)hipGraphExecDestroy(b->steps_exec);
+    b->steps_exec = nullptr;
+}
+
+// One device-driven step: dt from the CFL bins and the clock, the RK stages
+// (the last one reduces the next step's CFL), then the clock advance.
+void enqueue_step(mara_block* b)
+{
+    k_ctl_step_start<<<1, 1, 0, b->stream>>>(b->ctl);
+    for (int rk = 0; rk < b->d.rk_order; ++rk) launch_stage(b, rk, 1);
+    k_ctl_step_end<<<1, 1, 0, b->stream>>>(b->ctl);
+}
+
+// Capture kGraphSteps steps of the block's stream into an executable graph.
+// Nothing in a step is a memset or a host transfer.
+int capture_steps(mara_block* b)
+{
+    hipGraph_t g = nullptr;
+    HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    for (int s = 0; s < mara_block::kGraphSteps; ++s) enqueue_step(b);
+    const hipError_t launched = hipGetLastError();
+    const hipError_t ended = hipStreamEndCapture(b->stream, &g);
+    if (!hip_ok(launched) || !hip_ok(ended))
     {
-        double* t = b->Pa;
-        b->Pa = b->Pb;
-        b->Pb = t;
+        if (g) (void)hipGraphDestroy(g);
+        return MARA_ERR_HIP;
     }
+    const hipError_t made = hipGraphInstantiate(&b->steps_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (!hip_ok(made))
+    {
+        b->steps_exec = nullptr;
+        return MARA_ERR_HIP;
+    }
+    return MARA_OK;
 }
 
 int read_ctl(mara_block* b)
@@ -241,7 +333,12 @@ int mara_block_create(const mara_desc* d, mara_block** out)
     b->S = sizes_of(d);
     b->ng = recon_ng(d->recon);
     b->fused = fused_supported(d);
+    b->pencil = pencil_supported(d) ? PENCIL_DEFAULT : 0;
     b->G = make_geom(*d);
+    {
+        mara_shape tuned;
+        if (tune_find(*d, &tuned)) apply_shape(b, tuned);
+    }
 
     auto fail = [&](int code) { mara_block_destroy(b); return code; };
     if (!hip_ok(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking))) return fail(MARA_ERR_HIP);
@@ -249,12 +346,12 @@ int mara_block_create(const mara_desc* d, mara_block** out)
     if (!hip_ok(hipMalloc(&b->Pa, cellbytes))) return fail(MARA_ERR_HIP);
     if (!hip_ok(hipMalloc(&b->Pb, cellbytes))) return fail(MARA_ERR_HIP);
     if (!hip_ok(hipMalloc(&b->U, cellbytes))) return fail(MARA_ERR_HIP);
-    if (b->fused && d->rk_order == 3)
+    if ((b->fused || b->pencil) && d->rk_order == 3)
     {
         if (!hip_ok(hipMalloc(&b->Pc, cellbytes))) return fail(MARA_ERR_HIP);
         if (!hip_ok(hipMemset(b->Pc, 0, cellbytes))) return fail(MARA_ERR_HIP);
     }
-    if (!b->fused)
+    if (!b->fused && !b->pencil)
     {
         for (int a = 0; a < 3; ++a)
         {
@@ -290,6 +387,7 @@ int mara_block_destroy(mara_block* b)
     for (int a = 0; a < 3; ++a) hipFree(b->F[a]);
     hipFree(b->ctl);
     hipFree(b->cbins);
+    if (b->steps_exec) (void)hipGraphExecDestroy(b->steps_exec);
     if (b->ctl_host) hipHostFree(b->ctl_host);
     for (int e = 0; e < 2 * mara_block::kTimingPairs; ++e)
         if (b->ev[e]) hipEventDestroy(b->ev[e]);
@@ -299,6 +397,169 @@ int mara_block_destroy(mara_block* b)
 }
 
 long long mara_block_device_bytes(const mara_block* b) { return b ? b->bytes : -1; }
+
+int mara_block_get_shape(const mara_block* b, mara_shape* shape)
+{
+    if (!b || !shape) return MARA_ERR_INVALID_ARGUMENT;
+    *shape = block_shape(b);
+    return MARA_OK;
+}
+
+int mara_block_set_shape(mara_block* b, const mara_shape* shape)
+{
+    if (!b || !shape || !valid_shape(*shape)) return MARA_ERR_INVALID_ARGUMENT;
+    apply_shape(b, *shape);
+    return MARA_OK;
+}
+
+int mara_block_autotune(mara_block* b, int repeats, mara_shape* chosen, double* best_ms)
+{
+    if (!b || repeats < 1) return MARA_ERR_INVALID_ARGUMENT;
+    // the candidates of the one shape the block's stages use
+    std::vector<mara_shape> cands;
+    const mara_shape cur = block_shape(b);
+    if (b->pencil && b->d.law != MARA_LAW_RMHD)
+        for (int p : PENCIL_SHAPES) { mara_shape s = cur; s.pencil = p; cands.push_back(s); }
+    else if (b->fused && b->G.N[2] > 1)
+        for (int z : KZ_SHAPES) { mara_shape s = cur; s.zplanes = z; cands.push_back(s); }
+    else if (!b->fused && !b->pencil)
+        for (int r : ROWS_SHAPES) { mara_shape s = cur; s.rows = r; cands.push_back(s); }
+    else
+        cands.push_back(cur);
+
+    // scratch control block: active, dt = 0 (the stage reproduces U0), no CFL
+    Ctl h;
+    std::memset(&h, 0, sizeof h);
+    h.active = 1;
+    h.t_final = 1.0;
+    h.max_steps = 1;
+    h.first_fail = ~0ull;
+    h.cbins = b->cbins;
+    Ctl* sc = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int status = MARA_OK;
+    double best = 1e300;
+    mara_shape pick = cur;
+    auto cleanup = [&]() {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (sc) (void)hipFree(sc);
+    };
+    if (!hip_ok(hipMalloc(&sc, sizeof(Ctl))) || !hip_ok(hipMemcpy(sc, &h, sizeof(Ctl), hipMemcpyHostToDevice))
+        || !hip_ok(hipEventCreate(&e0)) || !hip_ok(hipEventCreate(&e1)))
+    {
+        cleanup();
+        return MARA_ERR_HIP;
+    }
+    for (const mara_shape& c : cands)
+    {
+        StageArgs s;
+        s.d = &b->d;
+        s.G = &b->G;
+        s.stream = b->stream;
+        s.Pin = b->Pa;  // stage 1 reads the state and writes only scratch
+        s.P0 = b->Pa;
+        s.U = b->U;
+        s.Pout = b->Pb;
+        for (int a = 0; a < 3; ++a) s.F[a] = b->F[a];
+        s.b = 1.0;
+        s.first = 1;
+        s.write_u = 1;
+        s.cout = 0;
+        s.ctl = sc;
+        s.fused = b->fused;
+        s.ct = b->S.ct;
+        s.pencil = b->pencil ? pencil_length(b->d.law, c.pencil) : 0;
+        s.kz = c.zplanes;
+        s.rows = c.rows;
+        const bool mixed = b->d.precision == MARA_PREC_MIXED;
+        for (int r = 0; r <= repeats; ++r)
+        {
+            if (r == 1) (void)hipEventRecord(e0, b->stream);
+            if (mixed) stage_kernels_f32(s);
+            else stage_kernels<double>(s);
+        }
+        (void)hipEventRecord(e1, b->stream);
+        if (!hip_ok(hipGetLastError()) || !hip_ok(hipEventSynchronize(e1)))
+        {
+            status = MARA_ERR_HIP;
+            break;
+        }
+        float ms = 0.0f;
+        if (!hip_ok(hipEventElapsedTime(&ms, e0, e1)))
+        {
+            status = MARA_ERR_HIP;
+            break;
+        }
+        const double mean = ms / repeats;
+        if (mean < best)
+        {
+            best = mean;
+            pick = c;
+        }
+    }
+    cleanup();
+    if (status) return status;
+    apply_shape(b, pick);
+    {
+        std::lock_guard<std::mutex> lock(tune_mutex);
+        tune_table[tune_key(b->d)] = pick;
+    }
+    if (chosen) *chosen = pick;
+    if (best_ms) *best_ms = best;
+    return MARA_OK;
+}
+
+int mara_tune_lookup(const mara_desc* d, mara_shape* shape)
+{
+    if (!d || !shape) return MARA_ERR_INVALID_ARGUMENT;
+    return tune_find(*d, shape) ? 1 : 0;
+}
+
+int mara_tune_save(const char* path)
+{
+    if (!path) return MARA_ERR_INVALID_ARGUMENT;
+    std::FILE* f = std::fopen(path, "w");
+    if (!f) return MARA_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(tune_mutex);
+    std::fprintf(f, "# law recon riemann ct precision rk_order n0 n1 n2 : pencil zplanes rows\n");
+    for (auto& e : tune_table)
+        std::fprintf(f, "%s : %d %d %d\n", e.first.c_str(), e.second.pencil, e.second.zplanes, e.second.rows);
+    std::fclose(f);
+    return MARA_OK;
+}
+
+int mara_tune_load(const char* path)
+{
+    if (!path) return MARA_ERR_INVALID_ARGUMENT;
+    std::FILE* f = std::fopen(path, "r");
+    if (!f) return MARA_ERR_INVALID_ARGUMENT;
+    char line[256];
+    int status = MARA_OK;
+    std::map<std::string, mara_shape> got;
+    while (std::fgets(line, sizeof line, f))
+    {
+        if (line[0] == '#' || line[0] == '\n') continue;
+        int k[9];
+        mara_shape s;
+        if (std::sscanf(line, "%d %d %d %d %d %d %d %d %d : %d %d %d", &k[0], &k[1], &k[2], &k[3], &k[4], &k[5],
+                        &k[6], &k[7], &k[8], &s.pencil, &s.zplanes, &s.rows) != 12
+            || !valid_shape(s))
+        {
+            status = MARA_ERR_INVALID_ARGUMENT;
+            break;
+        }
+        char key[160];
+        std::snprintf(key, sizeof key, "%d %d %d %d %d %d %d %d %d", k[0], k[1], k[2], k[3], k[4], k[5], k[6], k[7],
+                      k[8]);
+        got[key] = s;
+    }
+    std::fclose(f);
+    if (status) return status;
+    std::lock_guard<std::mutex> lock(tune_mutex);
+    for (auto& e : got) tune_table[e.first] = e.second;
+    return MARA_OK;
+}
 
 int mara_block_upload_prim(mara_block* b, const double* P)
 {
@@ -381,32 +642,52 @@ int mara_block_run(mara_block* b, double cfl, double t_final, int max_steps,
                    double* t, int* steps, mara_report* report)
 {
     if (!b || !t || !steps || max_steps < 0) return MARA_ERR_INVALID_ARGUMENT;
+    // stage timing records events around each stage launch, which a graph
+    // replay would overwrite: timed runs launch eagerly
+    const bool graph = b->graphs && !b->timing;
+    if (graph && !b->steps_exec)
+    {
+        const int e = capture_steps(b);
+        if (e) return e;
+    }
     k_ctl_begin_run<<<1, 1, 0, b->stream>>>(b->ctl, *t, t_final, cfl, max_steps);
     // the first step's CFL; later steps take it from the previous step's last stage
     launch_courant(b);
-    int queued = 0;
+    constexpr int kChunk = 64;
+    int queued = 0, chunk = kChunk;
     while (queued < max_steps)
     {
         // enqueue a chunk of steps; steps after the stop condition are no-ops
-        const int chunk = max_steps - queued < 64 ? max_steps - queued : 64;
-        for (int s = 0; s < chunk; ++s)
-        {
-            k_ctl_step_start<<<1, 1, 0, b->stream>>>(b->ctl);
-            for (int rk = 0; rk < b->d.rk_order; ++rk) launch_stage(b, rk, 1);
-            k_ctl_step_end<<<1, 1, 0, b->stream>>>(b->ctl);
-        }
+        if (chunk > max_steps - queued) chunk = max_steps - queued;
+        int s = 0;
+        if (graph)
+            for (; s + mara_block::kGraphSteps <= chunk; s += mara_block::kGraphSteps)
+                HIP_TRY(hipGraphLaunch(b->steps_exec, b->stream));
+        for (; s < chunk; ++s) enqueue_step(b);
         queued += chunk;
         HIP_TRY(hipGetLastError());
         int e = read_ctl(b);
         if (e) return e;
         const Ctl& c = *b->ctl_host;
         if (!c.active || !(c.t < c.t_final) || c.steps >= max_steps) break;
+        // size the next chunk to the steps the clock still needs at the last
+        // step's dt (plus a margin), so few no-op steps are queued at the end
+        const double left = (c.t_final - c.t) / c.dt;
+        chunk = kChunk;
+        if (c.dt > 0 && left + 8 < kChunk) chunk = (int)left + 8;
     }
     int e = read_ctl(b);
     if (e) return e;
     *t = b->ctl_host->t;
     *steps = b->ctl_host->steps;
     return fill_report(b, report);
+}
+
+int mara_block_set_graphs(mara_block* b, int enable)
+{
+    if (!b) return MARA_ERR_INVALID_ARGUMENT;
+    b->graphs = enable != 0;
+    return MARA_OK;
 }
 
 int mara_block_set_timing(mara_block* b, int enable)
@@ -438,7 +719,9 @@ int mara_block_stage_kernel_time(mara_block* b, double* avg_ms, long long* launc
     }
     if (avg_ms) *avg_ms = b->timed_launches ? b->timed_ms / b->timed_launches : 0.0;
     if (launches) *launches = b->timed_launches;
-    if (name) *name = b->fused ? fused_kernel_name(b->d) : "k_flux+k_update (flux-array stage)";
+    if (name)
+        *name = b->fused ? fused_kernel_name(b->d)
+                         : b->pencil ? "k_pencil (1-D stage)" : "k_flux+k_update (flux-array stage)";
     return MARA_OK;
 }
 

@@ -70,6 +70,7 @@ void box_tiles(std::vector<BTile>& out, int item, const int lo[3], const int hi[
 
 void build_table(mara_batch* B)
 {
+    B->table_version += 1;
     B->tiles.clear();
     for (int a = 0; a < 3; ++a) B->flux[a].clear();
     B->update.clear();
@@ -89,7 +90,7 @@ void build_table(mara_batch* B)
                 if (!sl.used || sl.cls != (int)c) continue;
                 const Geom& G = B->items[n].G;
                 const int ntx = (G.n[0] + FUSED_TX - 1) / FUSED_TX, nty = (G.n[1] + TY - 1) / TY;
-                const int ntz = G.N[2] > 1 ? (G.n[2] + FUSED_KZ - 1) / FUSED_KZ : 1;
+                const int ntz = G.N[2] > 1 ? (G.n[2] + B->kz - 1) / B->kz : 1;
                 for (int tz = 0; tz < ntz; ++tz)
                     for (int ty = 0; ty < nty; ++ty)
                         for (int tx = 0; tx < ntx; ++tx) B->tiles.push_back(BTile{n, tx, ty, tz, 0, 0, 3, 0});
@@ -111,7 +112,8 @@ void build_table(mara_batch* B)
         }
         if (B->classes[c].pencil)
         {
-            // one tile per PENCIL - 1 interior zones of each 1-D item
+            // one tile per pl - 1 interior zones of each 1-D item
+            const int pl = pencil_length(B->classes[c].law, B->pencil);
             const int s0 = seg_begin();
             for (int n = 0; n < B->n_items; ++n)
             {
@@ -119,8 +121,8 @@ void build_table(mara_batch* B)
                 if (!sl.used || sl.cls != (int)c) continue;
                 const Geom& G = B->items[n].G;
                 const int ie = G.N[0] - G.g[0];
-                for (int x0 = G.g[0]; x0 < ie; x0 += PENCIL - 1)
-                    B->tiles.push_back(BTile{n, x0, 0, 0, std::min(PENCIL - 1, ie - x0), 1, 2, 0});
+                for (int x0 = G.g[0]; x0 < ie; x0 += pl - 1)
+                    B->tiles.push_back(BTile{n, x0, 0, 0, std::min(pl - 1, ie - x0), 1, 2, 0});
             }
             if ((int)B->tiles.size() > s0) B->update[(int)c] = Segment{s0, (int)B->tiles.size() - s0};
             continue;
@@ -240,12 +242,38 @@ int mara_batch_destroy(mara_batch* B)
     (void)hipFree(B->d_ctl);
     (void)hipFree(B->d_bins);
     (void)hipFree(B->d_tiles);
+    if (B->round_exec) (void)hipGraphExecDestroy(B->round_exec);
     if (B->h_ctl) (void)hipHostFree(B->h_ctl);
     if (B->stream) (void)hipStreamDestroy(B->stream);
     if (B->stream2) (void)hipStreamDestroy(B->stream2);
     if (B->fork) (void)hipEventDestroy(B->fork);
     if (B->join) (void)hipEventDestroy(B->join);
     delete B;
+    return MARA_OK;
+}
+
+int mara_batch_get_shape(const mara_batch* B, mara_shape* shape)
+{
+    if (!B || !shape) return MARA_ERR_INVALID_ARGUMENT;
+    shape->pencil = B->pencil;
+    shape->zplanes = B->kz;
+    shape->rows = ROWS_DEFAULT;
+    return MARA_OK;
+}
+
+int mara_batch_set_shape(mara_batch* B, const mara_shape* shape)
+{
+    if (!B || !shape || !valid_shape(*shape)) return MARA_ERR_INVALID_ARGUMENT;
+    if (shape->pencil != B->pencil || shape->zplanes != B->kz) B->dirty = true;  // tiles change
+    B->pencil = shape->pencil;
+    B->kz = shape->zplanes;
+    return MARA_OK;
+}
+
+int mara_batch_set_graphs(mara_batch* B, int enable)
+{
+    if (!B) return MARA_ERR_INVALID_ARGUMENT;
+    B->graphs = enable != 0;
     return MARA_OK;
 }
 
@@ -405,10 +433,7 @@ int mara_batch_step(mara_batch* B, int n_steps, int* n_active)
         }
         kb_step_end<<<ig, 64, 0, B->cur>>>(B->d_items, n, lane);
     };
-    HIP_TRY(hipEventRecord(B->fork, B->stream));
-    HIP_TRY(hipStreamWaitEvent(B->stream2, B->fork, 0));
-    for (int s = 0; s < n_steps && n > 0; ++s)
-    {
+    auto run_round = [&]() {
         if (any_tile || !any_pencil)
         {
             B->cur = B->stream;
@@ -419,9 +444,53 @@ int mara_batch_step(mara_batch* B, int n_steps, int* n_active)
             B->cur = B->stream2;
             for (int r = 0; r < ratio; ++r) run_lane(1);
         }
+    };
+    const int lanes = (any_tile ? 1 : 0) | (any_pencil ? 2 : 0);
+    if (B->graphs && n > 0 && n_steps > 1)
+    {
+        if (!B->round_exec || B->round_version != B->table_version || B->round_n != n || B->round_ratio != ratio
+            || B->round_lanes != lanes)
+        {
+            if (B->round_exec) HIP_TRY(hipGraphExecDestroy(B->round_exec));
+            B->round_exec = nullptr;
+            // the round forks the second stream off the first and joins it
+            // back, so one replay ends when both lanes have
+            hipGraph_t g = nullptr;
+            HIP_TRY(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
+            (void)hipEventRecord(B->fork, B->stream);
+            (void)hipStreamWaitEvent(B->stream2, B->fork, 0);
+            run_round();
+            (void)hipEventRecord(B->join, B->stream2);
+            (void)hipStreamWaitEvent(B->stream, B->join, 0);
+            const hipError_t launched = hipGetLastError();
+            const hipError_t ended = hipStreamEndCapture(B->stream, &g);
+            if (!hip_ok(launched) || !hip_ok(ended))
+            {
+                if (g) (void)hipGraphDestroy(g);
+                return MARA_ERR_HIP;
+            }
+            const hipError_t made = hipGraphInstantiate(&B->round_exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (!hip_ok(made))
+            {
+                B->round_exec = nullptr;
+                return MARA_ERR_HIP;
+            }
+            B->round_version = B->table_version;
+            B->round_n = n;
+            B->round_ratio = ratio;
+            B->round_lanes = lanes;
+        }
+        for (int s = 0; s < n_steps; ++s) HIP_TRY(hipGraphLaunch(B->round_exec, B->stream));
     }
-    HIP_TRY(hipEventRecord(B->join, B->stream2));
-    HIP_TRY(hipStreamWaitEvent(B->stream, B->join, 0));
+    else
+    {
+        HIP_TRY(hipEventRecord(B->fork, B->stream));
+        HIP_TRY(hipStreamWaitEvent(B->stream2, B->fork, 0));
+        for (int s = 0; s < n_steps && n > 0; ++s) run_round();
+        HIP_TRY(hipEventRecord(B->join, B->stream2));
+        HIP_TRY(hipStreamWaitEvent(B->stream, B->join, 0));
+    }
     HIP_TRY(hipGetLastError());
     if (n) HIP_TRY(hipMemcpyAsync(B->h_ctl, B->d_ctl, n * sizeof(Ctl), hipMemcpyDeviceToHost, B->stream));
     if (d_ftiles) HIP_TRY(hipFreeAsync(d_ftiles, B->stream));

@@ -81,12 +81,12 @@ kb_update(const BItem* __restrict__ items, const BTile* __restrict__ tiles, doub
 
 // 1-D items of an RK2 batch: reconstruction, fluxes and update in one pass
 // per tile of PENCIL - 1 zones (stage 1: Pa -> Pb, stage 2: Pb -> Pa).
-template <int Law, int Recon, int Riem, typename R, bool CT>
+template <int Law, int Recon, int Riem, typename R, bool CT, int PENCIL>
 static __global__ void __launch_bounds__(PENCIL)
 kb_pencil(const BItem* __restrict__ items, const BTile* __restrict__ tiles, double b, int first, int last,
           int courant_out)
 {
-    constexpr int NQ = PencilLds<Law, Recon>::NQ, W = PencilLds<Law, Recon>::W;
+    constexpr int NQ = PencilLds<Law, Recon, PENCIL>::NQ, W = PencilLds<Law, Recon, PENCIL>::W;
     __shared__ R sP[NQ * W];
     __shared__ R sF[NQ * PENCIL];
     __shared__ double wmin[PENCIL / 64];
@@ -95,8 +95,8 @@ kb_pencil(const BItem* __restrict__ items, const BTile* __restrict__ tiles, doub
     if (!it.ctl->active) return;
     const double* Pin = first ? it.Pa : it.Pb;
     double* Pout = last ? it.Pa : it.Pb;
-    const double dtc = pencil_tile<Law, Recon, Riem, R, CT>(Pin, it.Pa, it.U, Pout, it.G, b, first, !last,
-                                                            courant_out, it.ctl, T.x0, sP, sF);
+    const double dtc = pencil_tile<Law, Recon, Riem, R, CT, PENCIL>(Pin, it.Pa, it.U, Pout, it.G, b, first, !last,
+                                                                    courant_out, it.ctl, T.x0, sP, sF);
     if (courant_out) courant_to_bins(dtc, wmin, PENCIL / 64, threadIdx.x, it.ctl);
 }
 
@@ -106,7 +106,7 @@ kb_pencil(const BItem* __restrict__ items, const BTile* __restrict__ tiles, doub
 template <int Riem, typename R, bool Z>
 static __global__ void __launch_bounds__(FUSED_TX * PlmTile<R>::TY) __attribute__((amdgpu_waves_per_eu(PlmTile<R>::WAVES, 8)))
 kb_fused_plm(const BItem* __restrict__ items, const BTile* __restrict__ tiles, double b, int first, int last,
-             int courant_out)
+             int courant_out, int kz)
 {
     const BTile T = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     const BItem& it = items[T.item];
@@ -114,7 +114,8 @@ kb_fused_plm(const BItem* __restrict__ items, const BTile* __restrict__ tiles, d
     const Geom G = it.G;
     const double* Pin = first ? it.Pa : it.Pb;
     double* Pout = last ? it.Pa : it.Pb;
-    fused_plm_tile<Riem, R, Z>(Pin, it.Pa, it.U, Pout, G, b, first, !last, courant_out, it.ctl, T.x0, T.y0, T.z0);
+    fused_plm_tile<Riem, R, Z>(Pin, it.Pa, it.U, Pout, G, b, first, !last, courant_out, it.ctl, T.x0, T.y0, T.z0,
+                               kz);
 }
 
 // Ghost zones of the fused items (their kernel leaves them): tile x0 is the

@@ -100,6 +100,8 @@ struct mara_batch
     int rk_order = 2;
     int precision = MARA_PREC_FP64;
     int max_solves = 0;
+    int pencil = mara::PENCIL_DEFAULT;  // faces per pencil tile of the 1-D solves
+    int kz = mara::FUSED_KZ;            // z planes per fused tile of the 3-D hydro solves
     hipStream_t stream = nullptr;   // tile kernels, transfers, control
     hipStream_t stream2 = nullptr;  // pencil kernels of 1-D solves
     hipStream_t cur = nullptr;      // stream the stage launches go to
@@ -122,6 +124,14 @@ struct mara_batch
     std::map<int, mara::batch::Segment> flux[3];  // class -> segment
     std::map<int, mara::batch::Segment> update;
     std::map<int, mara::batch::Segment> ghost;     // fused classes: ghost-zone tiles
+    long long table_version = 0;                   // bumped by every table build
+    // HIP graph of one round (a tile-lane step and `ratio` pencil-lane steps),
+    // replayed n_steps times by mara_batch_step; recaptured when the table,
+    // the slot count, the ratio or the set of busy lanes changes
+    bool graphs = true;
+    hipGraphExec_t round_exec = nullptr;
+    long long round_version = -1;
+    int round_n = -1, round_ratio = -1, round_lanes = -1;
 };
 
 namespace mara {
@@ -144,11 +154,30 @@ void flux_dispatch(mara_batch* B, const ClassKey& k, const Segment& s, int axis,
 #undef FL
 }
 
+template <int Law, int Recon, int Riem, typename R, int PL>
+void pencil_launch_pl(mara_batch* B, const ClassKey& k, const BTile* t, int count, double b, int first, int last,
+                      int cout)
+{
+    if (k.ct) kb_pencil<Law, Recon, Riem, R, true, PL><<<count, PL, 0, B->cur>>>(B->d_items, t, b, first, last, cout);
+    else kb_pencil<Law, Recon, Riem, R, false, PL><<<count, PL, 0, B->cur>>>(B->d_items, t, b, first, last, cout);
+}
+
+// the batch's pencil length (one of PENCIL_SHAPES) selects the instantiation
 template <int Law, int Recon, int Riem, typename R>
 void pencil_launch(mara_batch* B, const ClassKey& k, const BTile* t, int count, double b, int first, int last, int cout)
 {
-    if (k.ct) kb_pencil<Law, Recon, Riem, R, true><<<count, PENCIL, 0, B->cur>>>(B->d_items, t, b, first, last, cout);
-    else kb_pencil<Law, Recon, Riem, R, false><<<count, PENCIL, 0, B->cur>>>(B->d_items, t, b, first, last, cout);
+    if constexpr (Law == 2)
+    {
+        // relativistic solves: the default length only (pencil_length)
+        pencil_launch_pl<Law, Recon, Riem, R, PENCIL_DEFAULT>(B, k, t, count, b, first, last, cout);
+        return;
+    }
+    switch (B->pencil)
+    {
+        case 128: pencil_launch_pl<Law, Recon, Riem, R, 128>(B, k, t, count, b, first, last, cout); break;
+        case 512: pencil_launch_pl<Law, Recon, Riem, R, 512>(B, k, t, count, b, first, last, cout); break;
+        default: pencil_launch_pl<Law, Recon, Riem, R, 256>(B, k, t, count, b, first, last, cout); break;
+    }
 }
 
 template <typename R>
@@ -165,7 +194,8 @@ void pencil_dispatch(mara_batch* B, const ClassKey& k, const Segment& s, double 
 template <int Riem, typename R, bool Z>
 void fused_launch(mara_batch* B, const BTile* t, int count, double b, int first, int last, int cout)
 {
-    kb_fused_plm<Riem, R, Z><<<count, dim3(FUSED_TX, PlmTile<R>::TY), 0, B->cur>>>(B->d_items, t, b, first, last, cout);
+    kb_fused_plm<Riem, R, Z><<<count, dim3(FUSED_TX, PlmTile<R>::TY), 0, B->cur>>>(B->d_items, t, b, first, last, cout,
+                                                                                  B->kz);
 }
 
 template <typename R>

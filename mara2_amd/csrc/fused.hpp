@@ -28,7 +28,8 @@ namespace mara {
 
 constexpr int FUSED_TX = 64;
 constexpr int FUSED_TY = 8;
-constexpr int FUSED_KZ = 16;
+constexpr int FUSED_KZ = 16;  // default z planes per work-group (a launch shape)
+constexpr int KZ_SHAPES[4] = {4, 8, 16, 32};
 
 // Work-group -> tile order that keeps neighbouring tiles on one XCD.  The
 // dispatcher hands work-group L of a 1-D grid to XCD L % 8, and each XCD has
@@ -96,7 +97,8 @@ __device__ __forceinline__ void hydro_cons_mixed(double igm1, const double* P, d
 template <int Recon, int Riem, typename R, bool Z>
 static __global__ void __launch_bounds__(FUSED_TX * FUSED_TY)
 k_fused_hydro(const double* __restrict__ Pin, const double* __restrict__ P0, double* __restrict__ U,
-              double* __restrict__ Pout, Geom G, double b, int first_stage, int write_u, int courant_out, Ctl* ctl)
+              double* __restrict__ Pout, Geom G, double b, int first_stage, int write_u, int courant_out, Ctl* ctl,
+              int kz)
 {
     constexpr int NG = Recon == 0 ? 1 : Recon == 1 ? 2 : 3;
     constexpr int TX = FUSED_TX, TY = FUSED_TY;
@@ -112,7 +114,7 @@ k_fused_hydro(const double* __restrict__ Pin, const double* __restrict__ P0, dou
     const int row = threadIdx.y;
     const int tid = row * TX + lane;
     const int ntx = (G.n[0] + TX - 1) / TX, nty = (G.n[1] + TY - 1) / TY;
-    const int ntz = Z ? (G.n[2] + FUSED_KZ - 1) / FUSED_KZ : 1;
+    const int ntz = Z ? (G.n[2] + kz - 1) / kz : 1;
     const int tile = xcd_tile(blockIdx.x, ntx * nty * ntz);
     const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
     const int i0 = G.g[0] + tx * TX;
@@ -122,8 +124,8 @@ k_fused_hydro(const double* __restrict__ Pin, const double* __restrict__ P0, dou
     const bool cell_xy = i < ie && j < je;
     const bool xface = i <= ie && j < je;                  // left x-face of (i, j) exists
     const bool yface = j <= je && i < ie;                  // lower y-face of (i, j) exists
-    const int kb = Z ? G.g[2] + tz * FUSED_KZ : 0;
-    const int ke = Z ? min(kb + FUSED_KZ, G.N[2] - G.g[2]) : 1;
+    const int kb = Z ? G.g[2] + tz * kz : 0;
+    const int ke = Z ? min(kb + kz, G.N[2] - G.g[2]) : 1;
     const long long sxy = (long long)G.N[0] * G.N[1];
     const R theta = R(G.theta), gamma = R(G.gamma);
     const double igm1 = 1.0 / (G.gamma - 1.0);  // binary32 form's U(P0)
@@ -344,7 +346,7 @@ template <int Riem, typename R, bool Z>
 __device__ __forceinline__ void fused_plm_tile(const double* __restrict__ Pin, const double* __restrict__ P0,
                                                double* __restrict__ U, double* __restrict__ Pout, const Geom& G,
                                                double b, int first_stage, int write_u, int courant_out, Ctl* ctl,
-                                               int tx, int ty, int tz)
+                                               int tx, int ty, int tz, int kz)
 {
     constexpr int NG = 2;
     constexpr int TX = FUSED_TX, TY = PlmTile<R>::TY;
@@ -368,8 +370,8 @@ __device__ __forceinline__ void fused_plm_tile(const double* __restrict__ Pin, c
     const bool cell_xy = i < ie && j < je;
     const bool xface = i <= ie && j < je;
     const bool yface = j <= je && i < ie;
-    const int kb = Z ? G.g[2] + tz * FUSED_KZ : 0;
-    const int ke = Z ? min(kb + FUSED_KZ, G.N[2] - G.g[2]) : 1;
+    const int kb = Z ? G.g[2] + tz * kz : 0;
+    const int ke = Z ? min(kb + kz, G.N[2] - G.g[2]) : 1;
     const long long sxy = (long long)G.N[0] * G.N[1];
     const R theta = R(G.theta), gamma = R(G.gamma);
     const double igm1 = 1.0 / (G.gamma - 1.0);  // binary32 form's U(P0)
@@ -627,22 +629,36 @@ __device__ __forceinline__ void fused_plm_tile(const double* __restrict__ Pin, c
 template <int Riem, typename R, bool Z>
 static __global__ void __launch_bounds__(FUSED_TX * PlmTile<R>::TY) __attribute__((amdgpu_waves_per_eu(PlmTile<R>::WAVES, 8)))
 k_fused_plm(const double* __restrict__ Pin, const double* __restrict__ P0, double* __restrict__ U,
-            double* __restrict__ Pout, Geom G, double b, int first_stage, int write_u, int courant_out, Ctl* ctl)
+            double* __restrict__ Pout, Geom G, double b, int first_stage, int write_u, int courant_out, Ctl* ctl,
+            int kz)
 {
     if (!ctl->active) return;
     constexpr int TX = FUSED_TX, TY = PlmTile<R>::TY;
     const int ntx = (G.n[0] + TX - 1) / TX, nty = (G.n[1] + TY - 1) / TY;
-    const int ntz = Z ? (G.n[2] + FUSED_KZ - 1) / FUSED_KZ : 1;
+    const int ntz = Z ? (G.n[2] + kz - 1) / kz : 1;
     const int tile = xcd_tile(blockIdx.x, ntx * nty * ntz);
     const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
-    fused_plm_tile<Riem, R, Z>(Pin, P0, U, Pout, G, b, first_stage, write_u, courant_out, ctl, tx, ty, tz);
+    fused_plm_tile<Riem, R, Z>(Pin, P0, U, Pout, G, b, first_stage, write_u, courant_out, ctl, tx, ty, tz, kz);
 }
 
 // The fused path covers the hydro law on 2-D and 3-D blocks; 1-D blocks and
 // the MHD laws (which need the flux arrays for flux-CT) use the flux-array path.
+// The fused kernel reads neighbouring zones of its input while other
+// work-groups write their output, so a stage needs a second buffer; with two
+// or three stages the step still ends in the state buffer (Pa -> Pb -> Pa,
+// Pa -> Pb -> Pc -> Pa).  A one-stage (RK1) step would end in the other
+// buffer, so RK1 blocks take the flux-array path, which updates in place.
 inline bool fused_supported(const mara_desc* d)
 {
-    return d->law == MARA_LAW_HYDRO && d->cells[1] > 1;
+    return d->law == MARA_LAW_HYDRO && d->cells[1] > 1 && d->rk_order >= 2;
+}
+
+// 1-D blocks of two or three stages run each stage as pencil tiles
+// (k_pencil): fluxes and update in one pass, on the fused kernels' buffer
+// schedule, so they hold no flux arrays either.
+inline bool pencil_supported(const mara_desc* d)
+{
+    return d->cells[1] == 1 && d->cells[2] == 1 && d->cells[0] > 1 && d->guards[0] > 0 && d->rk_order >= 2;
 }
 
 inline const char* fused_kernel_name(const mara_desc& d)
@@ -652,48 +668,50 @@ inline const char* fused_kernel_name(const mara_desc& d)
     return mixed ? "k_fused_hydro<float>" : "k_fused_hydro<double>";
 }
 
+// kz: z planes per work-group (one of KZ_SHAPES; a launch shape, the
+// result does not depend on it)
 template <int Recon, int Riem, typename R>
 inline void launch_fused_t(const Geom& G, hipStream_t s, const double* Pin, const double* P0, double* U,
-                           double* Pout, double b, int first, int write_u, int cout, Ctl* ctl)
+                           double* Pout, double b, int first, int write_u, int cout, Ctl* ctl, int kz)
 {
     const bool z = G.N[2] > 1;
     dim3 block(FUSED_TX, FUSED_TY, 1);
     dim3 grid(((G.n[0] + FUSED_TX - 1) / FUSED_TX) * ((G.n[1] + FUSED_TY - 1) / FUSED_TY)
-              * (z ? (G.n[2] + FUSED_KZ - 1) / FUSED_KZ : 1));
+              * (z ? (G.n[2] + kz - 1) / kz : 1));
     if constexpr (Recon == 1)
     {
         constexpr int TY = PlmTile<R>::TY;
         const dim3 pblock(FUSED_TX, TY, 1);
         const dim3 pgrid(((G.n[0] + FUSED_TX - 1) / FUSED_TX) * ((G.n[1] + TY - 1) / TY)
-                         * (z ? (G.n[2] + FUSED_KZ - 1) / FUSED_KZ : 1));
-        if (z) k_fused_plm<Riem, R, true><<<pgrid, pblock, 0, s>>>(Pin, P0, U, Pout, G, b, first, write_u, cout, ctl);
-        else k_fused_plm<Riem, R, false><<<pgrid, pblock, 0, s>>>(Pin, P0, U, Pout, G, b, first, write_u, cout, ctl);
+                         * (z ? (G.n[2] + kz - 1) / kz : 1));
+        if (z) k_fused_plm<Riem, R, true><<<pgrid, pblock, 0, s>>>(Pin, P0, U, Pout, G, b, first, write_u, cout, ctl, kz);
+        else k_fused_plm<Riem, R, false><<<pgrid, pblock, 0, s>>>(Pin, P0, U, Pout, G, b, first, write_u, cout, ctl, kz);
     }
     else
     {
-        if (z) k_fused_hydro<Recon, Riem, R, true><<<grid, block, 0, s>>>(Pin, P0, U, Pout, G, b, first, write_u, cout, ctl);
-        else k_fused_hydro<Recon, Riem, R, false><<<grid, block, 0, s>>>(Pin, P0, U, Pout, G, b, first, write_u, cout, ctl);
+        if (z) k_fused_hydro<Recon, Riem, R, true><<<grid, block, 0, s>>>(Pin, P0, U, Pout, G, b, first, write_u, cout, ctl, kz);
+        else k_fused_hydro<Recon, Riem, R, false><<<grid, block, 0, s>>>(Pin, P0, U, Pout, G, b, first, write_u, cout, ctl, kz);
     }
 }
 
 template <typename R>
 inline void launch_fused_r(const mara_desc& d, const Geom& G, hipStream_t s, const double* Pin, const double* P0,
-                           double* U, double* Pout, double b, int first, int write_u, int cout, Ctl* ctl)
+                           double* U, double* Pout, double b, int first, int write_u, int cout, Ctl* ctl, int kz)
 {
     const bool hllc = d.riemann == MARA_RIEMANN_HLLC;
     switch (d.recon)
     {
         case MARA_RECON_PCM:
-            if (hllc) launch_fused_t<0, 1, R>(G, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl);
-            else launch_fused_t<0, 0, R>(G, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl);
+            if (hllc) launch_fused_t<0, 1, R>(G, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, kz);
+            else launch_fused_t<0, 0, R>(G, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, kz);
             break;
         case MARA_RECON_PLM:
-            if (hllc) launch_fused_t<1, 1, R>(G, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl);
-            else launch_fused_t<1, 0, R>(G, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl);
+            if (hllc) launch_fused_t<1, 1, R>(G, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, kz);
+            else launch_fused_t<1, 0, R>(G, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, kz);
             break;
         default:
-            if (hllc) launch_fused_t<2, 1, R>(G, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl);
-            else launch_fused_t<2, 0, R>(G, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl);
+            if (hllc) launch_fused_t<2, 1, R>(G, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, kz);
+            else launch_fused_t<2, 0, R>(G, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, kz);
             break;
     }
 }

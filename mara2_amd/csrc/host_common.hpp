@@ -11,6 +11,8 @@
 
 #include "../../include/mara_gpu.h"
 #include "kernels.hpp"
+#include "fused.hpp"
+#include "stage.hpp"
 
 namespace mara {
 namespace host {
@@ -105,6 +107,17 @@ inline Geom make_geom(const mara_desc& d)
 }
 
 // status of a finished call, in the reference's order of precedence
+// A launch shape whose values are all in the candidate lists.
+inline bool valid_shape(const mara_shape& s)
+{
+    auto in = [](int v, const int* list, int n) {
+        for (int i = 0; i < n; ++i)
+            if (list[i] == v) return true;
+        return false;
+    };
+    return in(s.pencil, PENCIL_SHAPES, 3) && in(s.zplanes, KZ_SHAPES, 4) && in(s.rows, ROWS_SHAPES, 3);
+}
+
 inline int ctl_status(const Ctl& c)
 {
     if (c.superluminal) return MARA_ERR_SUPERLUMINAL;

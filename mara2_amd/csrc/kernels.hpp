@@ -482,17 +482,31 @@ __device__ __forceinline__ double update_zone(const double* __restrict__ P0, dou
 // PENCIL - 1 zones between them, with the same per-face and per-zone
 // arithmetic as the flux-array path (face_flux, divergence_*, finish_zone),
 // so the face fluxes never go through memory.  Input and output must be
-// different arrays (the tile reads neighbouring zones of Pin).
+// different arrays (the tile reads neighbouring zones of Pin).  The pencil
+// length is a launch shape (PENCIL_SHAPES); every length gives the same
+// result.
 // ---------------------------------------------------------------------------
-constexpr int PENCIL = 256;
+constexpr int PENCIL_DEFAULT = 256;
+constexpr int PENCIL_SHAPES[3] = {128, 256, 512};
 
-template <int Law, int Recon> struct PencilLds
+// The pencil length a law runs with: the requested one of PENCIL_SHAPES for
+// the hydro and MHD laws, the default for the relativistic law (whose
+// pencils are instantiated in that length only).
+inline int pencil_length(int law, int requested)
+{
+    if (law == 2) return PENCIL_DEFAULT;
+    for (int p : PENCIL_SHAPES)
+        if (p == requested) return p;
+    return PENCIL_DEFAULT;
+}
+
+template <int Law, int Recon, int PENCIL> struct PencilLds
 {
     static constexpr int NQ = LawNQ<Law>::value;
     static constexpr int W = PENCIL + 2 * Stencil<Recon>::NG - 1;  // staged zones per tile
 };
 
-template <int Law, int Recon, int Riem, typename R, bool CT>
+template <int Law, int Recon, int Riem, typename R, bool CT, int PENCIL>
 __device__ __forceinline__ double pencil_tile(const double* __restrict__ Pin, const double* __restrict__ P0,
                                               double* __restrict__ U, double* __restrict__ Pout, const Geom& G,
                                               double b, int first_stage, int write_u, int courant_out, Ctl* ctl,
@@ -500,7 +514,7 @@ __device__ __forceinline__ double pencil_tile(const double* __restrict__ Pin, co
 {
     constexpr int NQ = LawNQ<Law>::value;
     constexpr int NG = Stencil<Recon>::NG;
-    constexpr int W = PencilLds<Law, Recon>::W;
+    constexpr int W = PencilLds<Law, Recon, PENCIL>::W;
     const int t = threadIdx.x;
     const int ie = G.N[0] - G.g[0];  // one past the last interior zone
     // zones x0 - NG .. x0 + PENCIL + NG - 2 (clamped to the array; the clamped
@@ -551,6 +565,24 @@ __device__ __forceinline__ double pencil_tile(const double* __restrict__ Pin, co
     return dtc;
 }
 
+// The 1-D stage of a single block in one pass: tile b covers interior zones
+// g + b (PENCIL - 1) ...; RK schedules as the fused kernels (Pin != Pout).
+template <int Law, int Recon, int Riem, typename R, bool CT, int PENCIL>
+static __global__ void __launch_bounds__(PENCIL)
+k_pencil(const double* __restrict__ Pin, const double* __restrict__ P0, double* __restrict__ U,
+         double* __restrict__ Pout, Geom G, double b, int first_stage, int write_u, int courant_out, Ctl* ctl)
+{
+    constexpr int NQ = PencilLds<Law, Recon, PENCIL>::NQ, W = PencilLds<Law, Recon, PENCIL>::W;
+    __shared__ R sP[NQ * W];
+    __shared__ R sF[NQ * PENCIL];
+    __shared__ double wmin[PENCIL / 64];
+    if (!ctl->active) return;
+    const int x0 = G.g[0] + (int)blockIdx.x * (PENCIL - 1);
+    const double dtc = pencil_tile<Law, Recon, Riem, R, CT, PENCIL>(Pin, P0, U, Pout, G, b, first_stage, write_u,
+                                                                    courant_out, ctl, x0, sP, sF);
+    if (courant_out) courant_to_bins(dtc, wmin, PENCIL / 64, threadIdx.x, ctl);
+}
+
 template <int Law, typename R, bool CT>
 static __global__ void __launch_bounds__(256)
 k_update(const double* __restrict__ P0, double* __restrict__ U, double* __restrict__ Pout,
@@ -566,7 +598,7 @@ k_update(const double* __restrict__ P0, double* __restrict__ U, double* __restri
     if (ti < G.n[0] && tj < G.n[1])
         dtc = update_zone<Law, R, CT>(P0, U, Pout, Fx, Fy, Fz, G, b, first_stage, write_u, courant_out, ctl,
                                       G.g[0] + ti, G.g[1] + tj, G.g[2] + (int)blockIdx.z);
-    if (courant_out) courant_to_bins(dtc, wmin, 4, threadIdx.y * blockDim.x + threadIdx.x, ctl);
+    if (courant_out) courant_to_bins(dtc, wmin, (int)blockDim.y, threadIdx.y * blockDim.x + threadIdx.x, ctl);
 }
 
 // ---------------------------------------------------------------------------

@@ -1,7 +1,8 @@
 // mixed.hip -- the binary32 (mixed-form) instantiations of the stage launches
 // of a single block (stage.hpp) and of a batch (batch_state.hpp).  This file
-// is compiled with floating-point contraction (fused multiply-add), which
-// the mixed form allows: its fluxes are binary32 and its RK accumulation,
+// is compiled with floating-point contraction within expressions (fused
+// multiply-add, -ffp-contract=on), which the mixed form allows: its fluxes
+// are binary32 and its RK accumulation,
 // primitive recovery and flux-CT stay binary64, with no claim to the
 // reference's rounding.  The binary64 kernels live in api.hip and batch.hip,
 // compiled without contraction.

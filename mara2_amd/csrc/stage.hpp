@@ -28,12 +28,18 @@ struct StageArgs
     int first, write_u, cout;
     Ctl* ctl;
     bool fused, ct;
+    int pencil;  // 1-D pencil tiles of this length, or 0
+    int kz;      // fused 3-D kernels: z planes per work-group
+    int rows;    // flux-array kernels: work-group rows of 64 zones
 };
 
-// 2-D work-groups of 64 x 4 over an (m0, m1, m2) box: one wavefront per x row,
-// z from the grid, so kernels decode indices without integer division.
-inline dim3 stage_block3() { return dim3(64, 4, 1); }
-inline dim3 stage_grid3(int m0, int m1, int m2) { return dim3((m0 + 63) / 64, (m1 + 3) / 4, m2); }
+// 2-D work-groups of 64 x rows over an (m0, m1, m2) box: one wavefront per x
+// row, z from the grid, so kernels decode indices without integer division.
+// rows (1, 2 or 4) is a launch shape; the result does not depend on it.
+constexpr int ROWS_DEFAULT = 4;
+constexpr int ROWS_SHAPES[3] = {1, 2, 4};
+inline dim3 stage_block3(int rows) { return dim3(64, rows, 1); }
+inline dim3 stage_grid3(int m0, int m1, int m2, int rows) { return dim3((m0 + 63) / 64, (m1 + rows - 1) / rows, m2); }
 
 template <int Law, int Recon, int Riem, typename R>
 void stage_flux_t(const StageArgs& s, int axis)
@@ -52,7 +58,7 @@ void stage_flux_t(const StageArgs& s, int axis)
         }
     }
     const int m0 = hi[0] - lo[0], m1 = hi[1] - lo[1], m2 = hi[2] - lo[2];
-    k_flux<Law, Recon, Riem, R><<<stage_grid3(m0, m1, m2), stage_block3(), 0, s.stream>>>(
+    k_flux<Law, Recon, Riem, R><<<stage_grid3(m0, m1, m2, s.rows), stage_block3(s.rows), 0, s.stream>>>(
         s.Pin, (R*)s.F[axis], G, axis, lo[0], lo[1], lo[2], m0, m1, m2, s.ctl);
 }
 
@@ -81,25 +87,92 @@ template <int Law, typename R>
 void stage_update_law(const StageArgs& s)
 {
     const Geom& G = *s.G;
-    const dim3 grid = stage_grid3(G.n[0], G.n[1], G.n[2]);
+    const dim3 grid = stage_grid3(G.n[0], G.n[1], G.n[2], s.rows);
     if (s.ct)
-        k_update<Law, R, true><<<grid, stage_block3(), 0, s.stream>>>(
+        k_update<Law, R, true><<<grid, stage_block3(s.rows), 0, s.stream>>>(
             s.P0, s.U, s.Pout, (const R*)s.F[0], (const R*)s.F[1], (const R*)s.F[2],
             G, s.b, s.first, s.write_u, s.cout, s.ctl);
     else
-        k_update<Law, R, false><<<grid, stage_block3(), 0, s.stream>>>(
+        k_update<Law, R, false><<<grid, stage_block3(s.rows), 0, s.stream>>>(
             s.P0, s.U, s.Pout, (const R*)s.F[0], (const R*)s.F[1], (const R*)s.F[2],
             G, s.b, s.first, s.write_u, s.cout, s.ctl);
 }
 
-// The stage's kernels: the fused hydro kernel, or one flux sweep per axis
-// followed by the zone update (which also writes the ghost zones).
+template <int Law, int Recon, int Riem, typename R, int PL>
+void stage_pencil_pl(const StageArgs& s)
+{
+    const Geom& G = *s.G;
+    const int tiles = (G.n[0] + PL - 2) / (PL - 1);
+    if constexpr (Law != 0)
+        if (s.ct)
+        {
+            k_pencil<Law, Recon, Riem, R, true, PL><<<tiles, PL, 0, s.stream>>>(s.Pin, s.P0, s.U, s.Pout, G, s.b,
+                                                                               s.first, s.write_u, s.cout, s.ctl);
+            return;
+        }
+    k_pencil<Law, Recon, Riem, R, false, PL><<<tiles, PL, 0, s.stream>>>(s.Pin, s.P0, s.U, s.Pout, G, s.b, s.first,
+                                                                        s.write_u, s.cout, s.ctl);
+}
+
+template <int Law, int Recon, int Riem, typename R>
+void stage_pencil_t(const StageArgs& s)
+{
+    // the relativistic law's pencils (Newton c2p and quartic inlined) come in
+    // the default length only
+    if constexpr (Law == 2)
+    {
+        stage_pencil_pl<Law, Recon, Riem, R, PENCIL_DEFAULT>(s);
+        return;
+    }
+    switch (s.pencil)
+    {
+        case 128: stage_pencil_pl<Law, Recon, Riem, R, 128>(s); break;
+        case 512: stage_pencil_pl<Law, Recon, Riem, R, 512>(s); break;
+        default: stage_pencil_pl<Law, Recon, Riem, R, 256>(s); break;
+    }
+}
+
+template <int Law, typename R>
+void stage_pencil_law(const StageArgs& s)
+{
+    const int recon = s.d->recon, riem = s.d->riemann;
+    if (recon == MARA_RECON_PCM)
+    {
+        if (Law == 0 && riem == 1) stage_pencil_t<Law, 0, 1, R>(s);
+        else stage_pencil_t<Law, 0, 0, R>(s);
+    }
+    else if (recon == MARA_RECON_PLM)
+    {
+        if (Law == 0 && riem == 1) stage_pencil_t<Law, 1, 1, R>(s);
+        else stage_pencil_t<Law, 1, 0, R>(s);
+    }
+    else
+    {
+        if (Law == 0 && riem == 1) stage_pencil_t<Law, 2, 1, R>(s);
+        else stage_pencil_t<Law, 2, 0, R>(s);
+    }
+}
+
+// The stage's kernels: the fused hydro kernel, the 1-D pencil kernel, or one
+// flux sweep per axis followed by the zone update (the update and the pencil
+// kernel write the ghost zones themselves).
 template <typename R>
 void stage_kernels(const StageArgs& s)
 {
     if (s.fused)
     {
-        launch_fused_r<R>(*s.d, *s.G, s.stream, s.Pin, s.P0, s.U, s.Pout, s.b, s.first, s.write_u, s.cout, s.ctl);
+        launch_fused_r<R>(*s.d, *s.G, s.stream, s.Pin, s.P0, s.U, s.Pout, s.b, s.first, s.write_u, s.cout, s.ctl,
+                          s.kz);
+        return;
+    }
+    if (s.pencil)
+    {
+        switch (s.d->law)
+        {
+            case 0: stage_pencil_law<0, R>(s); break;
+            case 1: stage_pencil_law<1, R>(s); break;
+            default: stage_pencil_law<2, R>(s); break;
+        }
         return;
     }
     for (int axis = 0; axis < 3; ++axis)

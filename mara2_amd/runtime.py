@@ -9,7 +9,7 @@ import os
 import numpy as np
 
 from . import abi
-from .abi import MaraDesc, MaraReport, array_shape
+from .abi import MaraDesc, MaraReport, MaraShape, array_shape
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libmara_gpu.so")
@@ -56,6 +56,14 @@ def load():
     L.mara_block_run.argtypes = [blk, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp,
                                  ctypes.POINTER(ctypes.c_int), ctypes.POINTER(MaraReport)]
     L.mara_block_set_timing.argtypes = [blk, ctypes.c_int]
+    L.mara_block_set_graphs.argtypes = [blk, ctypes.c_int]
+    shp = ctypes.POINTER(MaraShape)
+    L.mara_block_get_shape.argtypes = [blk, shp]
+    L.mara_block_set_shape.argtypes = [blk, shp]
+    L.mara_block_autotune.argtypes = [blk, ctypes.c_int, shp, _dp]
+    L.mara_tune_lookup.argtypes = [ctypes.POINTER(MaraDesc), shp]
+    L.mara_tune_save.argtypes = [ctypes.c_char_p]
+    L.mara_tune_load.argtypes = [ctypes.c_char_p]
     L.mara_block_stage_kernel_time.argtypes = [blk, _dp, ctypes.POINTER(ctypes.c_longlong),
                                                ctypes.POINTER(ctypes.c_char_p)]
     L.mara_block_device_bytes.argtypes = [blk]
@@ -63,6 +71,9 @@ def load():
     bat = ctypes.c_void_p
     L.mara_batch_create.argtypes = [ctypes.c_longlong, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.POINTER(bat)]
+    L.mara_batch_set_graphs.argtypes = [bat, ctypes.c_int]
+    L.mara_batch_get_shape.argtypes = [bat, ctypes.POINTER(MaraShape)]
+    L.mara_batch_set_shape.argtypes = [bat, ctypes.POINTER(MaraShape)]
     L.mara_batch_destroy.argtypes = [bat]
     L.mara_batch_item_bytes.argtypes = [ctypes.POINTER(MaraDesc), ctypes.c_int]
     L.mara_batch_item_bytes.restype = ctypes.c_longlong
@@ -82,6 +93,23 @@ def load():
 def _check(status, where, report=None):
     if status != abi.OK:
         raise MaraError(status, where, report.as_dict() if report is not None else None)
+
+
+def tune_lookup(desc):
+    """Shape the tuning table holds for desc, or None."""
+    s = MaraShape()
+    r = load().mara_tune_lookup(ctypes.byref(desc), ctypes.byref(s))
+    if r < 0:
+        _check(r, "mara_tune_lookup")
+    return s if r == 1 else None
+
+
+def tune_save(path):
+    _check(load().mara_tune_save(os.fsencode(path)), "mara_tune_save")
+
+
+def tune_load(path):
+    _check(load().mara_tune_load(os.fsencode(path)), "mara_tune_load")
 
 
 class Block:
@@ -155,8 +183,30 @@ class Block:
             _check(st, "mara_block_run", rep)
         return tt.value, steps.value, rep
 
+    def set_graphs(self, on=True):
+        """Replay run() steps from a captured HIP graph (default) or launch
+        every kernel from the host."""
+        _check(self.lib.mara_block_set_graphs(self._h, 1 if on else 0), "mara_block_set_graphs")
+
     def set_timing(self, on=True):
         _check(self.lib.mara_block_set_timing(self._h, 1 if on else 0), "mara_block_set_timing")
+
+    def get_shape(self):
+        s = MaraShape()
+        _check(self.lib.mara_block_get_shape(self._h, ctypes.byref(s)), "mara_block_get_shape")
+        return s
+
+    def set_shape(self, shape):
+        _check(self.lib.mara_block_set_shape(self._h, ctypes.byref(shape)), "mara_block_set_shape")
+
+    def autotune(self, repeats=5):
+        """Time the candidate launch shapes of the block's stage kernels, keep
+        the fastest; returns (shape, mean stage ms)."""
+        s = MaraShape()
+        ms = ctypes.c_double(0)
+        _check(self.lib.mara_block_autotune(self._h, repeats, ctypes.byref(s), ctypes.byref(ms)),
+               "mara_block_autotune")
+        return s, ms.value
 
     def stage_kernel_time(self):
         ms = ctypes.c_double(0)
@@ -208,6 +258,19 @@ class Batch:
     @property
     def free_bytes(self):
         return self.lib.mara_batch_free_bytes(self._h)
+
+    def set_graphs(self, on=True):
+        """Replay each round of step() from a captured HIP graph (default) or
+        launch every kernel from the host."""
+        _check(self.lib.mara_batch_set_graphs(self._h, 1 if on else 0), "mara_batch_set_graphs")
+
+    def get_shape(self):
+        s = MaraShape()
+        _check(self.lib.mara_batch_get_shape(self._h, ctypes.byref(s)), "mara_batch_get_shape")
+        return s
+
+    def set_shape(self, shape):
+        _check(self.lib.mara_batch_set_shape(self._h, ctypes.byref(shape)), "mara_batch_set_shape")
 
     def admit(self, desc, P, cfl, t_final, max_steps):
         """Slot of the admitted solve, or None when the pool or the slot table

@@ -90,10 +90,11 @@ def set_arrivals(requests, rate, seed=problems.SEED_SERVING + 1):
         r.arrival = float(a)
 
 
-def solve_alone(req, keep_result=True):
+def solve_alone(req, keep_result=True, graphs=True):
     """Serve one request on its own block: create, upload, run, download."""
     t0 = time.perf_counter()
     with Block(req.desc) as b:
+        b.set_graphs(graphs)
         b.upload(req.P)
         t, steps, rep = b.run(CFL, T_FINAL, max_steps(req), check=False)
         P = b.download()
@@ -103,20 +104,24 @@ def solve_alone(req, keep_result=True):
     return time.perf_counter() - t0
 
 
-def sequential(requests, keep_result=False):
+def sequential(requests, keep_result=False, graphs=True):
     """Service time of each request served one at a time (seconds)."""
-    return [solve_alone(r, keep_result) for r in requests]
+    return [solve_alone(r, keep_result, graphs) for r in requests]
 
 
 def serve(requests, hbm_budget, max_solves=1024, chunk=16, rk_order=2, precision="mixed",
-          keep_results=False, log=None, clock=time.perf_counter):
-    """Continuous batching over a request stream with arrival times set.
-    Returns (wall seconds, list of requests with admitted/finished stamps)."""
+          keep_results=False, log=None, clock=time.perf_counter, graphs=True, shape=None):
+    """Continuous batching over a request stream with arrival times set;
+    shape (optional) is the batch's launch shape.  Returns (wall seconds,
+    list of requests with admitted/finished stamps)."""
     pending = deque(sorted(requests, key=lambda r: r.arrival))
     waiting = []
     in_flight = {}
     done = []
     with Batch(hbm_budget, rk_order=rk_order, precision=precision, max_solves=max_solves) as batch:
+        batch.set_graphs(graphs)
+        if shape is not None:
+            batch.set_shape(shape)
         t0 = clock()
         last_log = t0
         while pending or waiting or in_flight:

@@ -40,6 +40,8 @@ def test_struct_layout_matches_header(tmp_path):
 int main(void) {
   printf("mara_desc %zu\\n", sizeof(mara_desc));
   printf("mara_report %zu\\n", sizeof(mara_report));
+  printf("mara_shape %zu\\n", sizeof(mara_shape));
+  F(mara_shape, pencil) F(mara_shape, zplanes) F(mara_shape, rows)
   F(mara_desc, law) F(mara_desc, gamma) F(mara_desc, pressure_floor) F(mara_desc, recon)
   F(mara_desc, plm_theta) F(mara_desc, riemann) F(mara_desc, bc) F(mara_desc, cells)
   F(mara_desc, guards) F(mara_desc, lower) F(mara_desc, upper) F(mara_desc, rk_order)
@@ -54,7 +56,8 @@ int main(void) {
     got = dict(l.rsplit(" ", 1) for l in lines if l)
     assert int(got["mara_desc"]) == ctypes.sizeof(abi.MaraDesc)
     assert int(got["mara_report"]) == ctypes.sizeof(abi.MaraReport)
-    for T, name in ((abi.MaraDesc, "mara_desc"), (abi.MaraReport, "mara_report")):
+    assert int(got["mara_shape"]) == ctypes.sizeof(abi.MaraShape)
+    for T, name in ((abi.MaraDesc, "mara_desc"), (abi.MaraReport, "mara_report"), (abi.MaraShape, "mara_shape")):
         for field, _ in T._fields_:
             assert int(got[f"{name}.{field}"]) == getattr(T, field).offset, field
 
@@ -94,3 +97,25 @@ def test_device_bytes_estimate():
     d = abi.make_desc(law="hydro", cells=(256, 256, 256), riemann="hllc")
     b = runtime.desc_device_bytes(d)
     assert 3 * 5 * 260 ** 3 * 8 <= b < 4 * 5 * 260 ** 3 * 8
+
+
+def test_tuning_table_file_round_trip(tmp_path):
+    """mara_tune_load / lookup / save without a GPU: entries are keyed by the
+    descriptor's kernel selectors and cell counts, a bad shape is refused."""
+    d = abi.make_desc(law="mhd", recon="plm", riemann="hlle", bc="periodic", cells=(96, 1, 1), field_ct=1)
+    other = abi.make_desc(law="mhd", recon="plm", riemann="hlle", bc="periodic", cells=(97, 1, 1), field_ct=1)
+    f = tmp_path / "tune.txt"
+    f.write_text("# comment\n1 1 0 1 0 2 96 1 1 : 512 8 2\n")
+    runtime.tune_load(str(f))
+    got = runtime.tune_lookup(d)
+    assert got is not None and got.as_dict() == {"pencil": 512, "zplanes": 8, "rows": 2}
+    assert runtime.tune_lookup(other) is None
+    out = tmp_path / "saved.txt"
+    runtime.tune_save(str(out))
+    assert "1 1 0 1 0 2 96 1 1 : 512 8 2" in out.read_text().splitlines()
+    bad = tmp_path / "bad.txt"
+    bad.write_text("1 1 0 1 0 2 96 1 1 : 300 8 2\n")
+    with pytest.raises(runtime.MaraError):
+        runtime.tune_load(str(bad))
+    with pytest.raises(runtime.MaraError):
+        runtime.tune_load(str(tmp_path / "missing.txt"))

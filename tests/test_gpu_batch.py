@@ -29,10 +29,14 @@ def _stream(precision):
     return reqs
 
 
-def _run_batch(reqs, precision, chunk=16, max_solves=64):
+def _run_batch(reqs, precision, chunk=16, max_solves=64, graphs=True, shape=None):
     """Admit every request at once and step until all are done."""
     out = {}
     with Batch(BUDGET, rk_order=2, precision=precision, max_solves=max_solves) as b:
+        b.set_graphs(graphs)
+        if shape is not None:
+            b.set_shape(shape)
+            assert b.get_shape().pencil == shape.pencil
         slots = {}
         for r in reqs:
             s = b.admit(r.desc, r.P, serving.CFL, serving.T_FINAL, serving.MAX_STEPS)
@@ -112,6 +116,30 @@ def test_chunking_does_not_change_results(fp64_stream, fp64_mix):
     for i, (P, t, steps) in res.items():
         assert (t, steps) == fp64_mix[i][1:]
         assert np.array_equal(P, fp64_mix[i][0])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_graph_rounds_match_host_launches(fp64_stream, fp64_mix, precision):
+    """Rounds replayed from captured graphs (recaptured as solves finish and
+    leave the table) against every kernel launched from the host."""
+    if precision == "fp64":
+        reqs, ref = fp64_stream, fp64_mix
+    else:
+        reqs = _stream("mixed")
+        ref = _run_batch(reqs, "mixed", graphs=True)
+    eager = _run_batch(reqs, precision, graphs=False)
+    assert eager.keys() == ref.keys()
+    for i, (P, t, steps) in eager.items():
+        assert (t, steps) == ref[i][1:], i
+        assert np.array_equal(P, ref[i][0]), i
+
+
+@pytest.mark.parametrize("pencil", [128, 512])
+def test_pencil_length_does_not_change_results(fp64_stream, fp64_mix, pencil):
+    res = _run_batch(fp64_stream, "fp64", shape=abi.make_shape(pencil=pencil, zplanes=8))
+    for i, (P, t, steps) in res.items():
+        assert (t, steps) == fp64_mix[i][1:], i
+        assert np.array_equal(P, fp64_mix[i][0]), i
 
 
 def test_admission_by_budget_and_slot_reuse(fp64_stream):
@@ -209,13 +237,16 @@ def test_two_lanes_with_a_large_tile_solve():
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_fused_3d_solve_in_a_batch(precision):
-    """3-D hydro + PLM solves take the batch's fused tiles (z-marching): equal
-    to the single block bit for bit, beside a 1-D solve."""
+@pytest.mark.parametrize("zplanes", [16, 4])
+def test_fused_3d_solve_in_a_batch(precision, zplanes):
+    """3-D hydro + PLM solves take the batch's fused tiles (z-marching, 16 or
+    4 planes per work-group): equal to the single block bit for bit, beside a
+    1-D solve."""
     desc = problems.turbulence_desc(24, precision)
     P0 = problems.turbulence_3d(problems.turbulence_desc(24, "fp64"), kmax=4)
     one = serving.make_stream(2, precision=precision, resize=SMALL)[1]
     with Batch(BUDGET, rk_order=2, precision=precision) as b:
+        b.set_shape(abi.make_shape(zplanes=zplanes))
         s3 = b.admit(desc, P0, 0.3, 1e300, 6)
         b.admit(one.desc, one.P, serving.CFL, serving.T_FINAL, serving.MAX_STEPS)
         got = {}
