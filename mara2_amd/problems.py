@@ -8,11 +8,14 @@ and the synthetic workloads the benchmark and parity tests run:
   mhd_2d          2-D ideal MHD, B = curl(A_z z) by the reference's CT initialisation
   rmhd_3d         3-D relativistic MHD, WENO5 + HLLE, CT, RK3
   riemann_problem one Riemann problem of the serving stream (1-D or 2-D, hydro or MHD)
+  regress_mhd_2d  FieldLoop / AbcField of the reference's 2-D MHD regression program
 
 Random fields are sums of Fourier modes with integer wave vectors in a band
 kmin <= |k| <= kmax, complex Gaussian amplitudes with a power-law spectrum
 and a numpy Generator seed.  All arithmetic here is host-side setup, done once.
 """
+import math
+
 import numpy as np
 
 from . import abi
@@ -430,3 +433,108 @@ def riemann_problem(params, desc):
             P[..., 0, 6] = By
             P[..., 7] = np.where(left, BL[2], BR[2])
     return apply_bc(desc, P)
+
+
+# ---------------------------------------------------------------------------
+# 2-D MHD regression problems with a vector potential
+# (NewtonianMHD2DTestProgram, src/Problems.cpp:480-672)
+# ---------------------------------------------------------------------------
+def _generate_cells(desc, f):
+    """MeshOperator::generate(F, cell) (src/MeshOperator.cpp:168-189): f at
+    every cell centre, guards included; f is evaluated with the host libm."""
+    xs = [cell_centres(desc, a) for a in range(3)]
+    out = np.empty(abi.array_shape(desc))
+    for i, x in enumerate(xs[0]):
+        for j, y in enumerate(xs[1]):
+            for k, z in enumerate(xs[2]):
+                out[i, j, k, :] = f(float(x), float(y), float(z))
+    return out
+
+
+def _generate_faces(desc, f):
+    """MeshOperator::generate(F, face, scalars) (src/MeshOperator.cpp:119-166):
+    the 3 components of f on the faces of each axis, at
+    coordinateAtIndex(ic - 1/2, jc, kc) and its permutations, guards included.
+    Returns A of shape (N1+1, N2+1, N3+1, 3, 3) [..., component, axis]."""
+    N = [desc.cells[a] + 2 * desc.guards[a] for a in range(3)]
+    dx = [(desc.upper[a] - desc.lower[a]) / desc.cells[a] for a in range(3)]
+    A = np.empty((N[0] + 1, N[1] + 1, N[2] + 1, 3, 3))
+
+    def coord(a, i):
+        return desc.lower[a] + dx[a] * (i + 0.5)
+
+    for i in range(N[0] + 1):
+        ic = i - desc.guards[0]
+        for j in range(N[1] + 1):
+            jc = j - desc.guards[1]
+            for k in range(N[2] + 1):
+                kc = k - desc.guards[2]
+                A[i, j, k, :, 0] = f(coord(0, ic - 0.5), coord(1, jc), coord(2, kc))
+                A[i, j, k, :, 1] = f(coord(0, ic), coord(1, jc - 0.5), coord(2, kc))
+                A[i, j, k, :, 2] = f(coord(0, ic), coord(1, jc), coord(2, kc - 0.5))
+    return A
+
+
+def _field_loop_p(x, y, z):
+    x -= 0.5
+    y -= 0.5
+    B, k = 1e-5, 10.0
+    R = math.sqrt(x * x + y * y)
+    bf = B * 3 * k * math.pow(k * R, 2) * math.exp(-math.pow(k * R, 3))
+    return (1.0, 1.0, 1.0, 0.0, 1.0, bf * (-y / R), bf * (x / R), 0.0)
+
+
+def _field_loop_a(x, y, z):
+    x -= 0.5
+    y -= 0.5
+    B, k = 1e-5, 10.0
+    R = math.sqrt(x * x + y * y)
+    return (0.0, 0.0, B * math.exp(-math.pow(k * R, 3)))
+
+
+_ABC = (1.0, 1.0, 0.0, 2 * math.pi)  # A, B, C, k
+
+
+def _abc_field_p(x, y, z):
+    A, B, C, k = _ABC
+    bx = C * math.cos(k * z) - B * math.sin(k * y)
+    by = A * math.cos(k * x) - C * math.sin(k * z)
+    bz = B * math.cos(k * y) - A * math.sin(k * x)
+    return (1.0, 0.0, 0.0, 0.0, 1.0, bx, by, bz)
+
+
+def _abc_field_a(x, y, z):
+    A, B, C, k = _ABC
+    bx = C * math.cos(k * z) - B * math.sin(k * y)
+    by = A * math.cos(k * x) - C * math.sin(k * z)
+    bz = B * math.cos(k * y) - A * math.sin(k * x)
+    return (bx / k, by / k, bz / k)
+
+
+REGRESS_MHD_2D = {
+    # name: (final time, primitive function, vector potential)
+    "FieldLoop": (1.0, _field_loop_p, _field_loop_a),
+    "AbcField": (0.1, _abc_field_p, _abc_field_a),
+}
+REGRESS_MHD_2D_CFL = 0.5
+
+
+def regress_mhd_desc(n=64, precision="fp64"):
+    """The program's mesh and scheme: n x n cells on the unit square, 2 guards,
+    periodic, NewtonianMHD (gamma 5/3), PLM + HLLE with flux-CT, RK2
+    (src/Problems.cpp:569-611)."""
+    return abi.make_desc(law="mhd", gamma=5.0 / 3.0, recon="plm", riemann="hlle", bc="periodic",
+                         cells=(n, n, 1), rk_order=2, field_ct=1, precision=precision)
+
+
+def regress_mhd_2d(name, desc):
+    """Initial MeshData::P of a vector-potential problem as runProblem builds
+    it (src/Problems.cpp:655-667): primitives at cell centres, then the cell
+    B of the whole array from A on faces -> vectorPotentialToFluxes -> CT ->
+    divergence (factor +1), then the boundary condition.  Returns
+    (P, final time)."""
+    t_final, fp, fa = REGRESS_MHD_2D[name]
+    P = _generate_cells(desc, fp)
+    A = _generate_faces(desc, fa)
+    P[..., 5:8] = vector_potential_to_B(desc, A)
+    return apply_bc(desc, P), t_final

@@ -328,3 +328,25 @@ def test_toro_tubes_converge():
             ex = sample(L, R, 1.4, (xi - x0) / tf)
             errs.append(np.mean(np.abs(P[2:-2, 0, 0, 0] - ex[:, 0])) / np.mean(np.abs(ex[:, 0])))
         assert errs[0] < 0.1 and errs[1] < errs[0] / 1.3
+
+
+def test_field_loop_returns_and_converges():
+    """The 2-D MHD program's FieldLoop (src/Problems.cpp:500-527), initialised
+    through the reference's vector-potential path: after one crossing of the
+    periodic box (t = 1, v = (1, 1)) the field is back in place up to the
+    scheme's error, which falls with resolution; the CT update keeps the
+    corner div B at roundoff throughout."""
+    errs = []
+    for n in (32, 64):
+        d = problems.regress_mhd_desc(n)
+        P0, tf = problems.regress_mhd_2d("FieldLoop", d)
+        P = P0.copy()
+        err, t, steps = oracle.run(d, P, problems.REGRESS_MHD_2D_CFL, tf, max_steps=100000)
+        assert err == 0 and t == tf
+        g = d.guards
+        inner = (slice(g[0], g[0] + n), slice(g[1], g[1] + n), slice(0, 1))
+        B0, B1 = P0[inner][..., 5:7], P[inner][..., 5:7]
+        errs.append(float(np.sum(np.abs(B1 - B0)) / np.sum(np.abs(B0))))
+        e, mono = oracle.monopole_max(d, P)
+        assert e == 0 and mono <= 1e-12 * np.abs(B0).max() * n
+    assert errs[1] < 0.5 * errs[0] and errs[1] < 0.4, errs
