@@ -244,7 +244,7 @@ void apply_shape(mara_block* b, const mara_shape& s)
 // (the last one reduces the next step's CFL), then the clock advance.
 void enqueue_step(mara_block* b)
 {
-    k_ctl_step_start<<<1, 1, 0, b->stream>>>(b->ctl);
+    k_ctl_step_start<<<1, 64, 0, b->stream>>>(b->ctl);
     for (int rk = 0; rk < b->d.rk_order; ++rk) launch_stage(b, rk, 1);
     k_ctl_step_end<<<1, 1, 0, b->stream>>>(b->ctl);
 }

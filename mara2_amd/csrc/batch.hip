@@ -423,9 +423,9 @@ int mara_batch_step(mara_batch* B, int n_steps, int* n_active)
         const double tile_step = (double)tile_zones * B->rk_order / 5e9 + 20e-6;
         ratio = (int)std::min(32.0, std::max(1.0, tile_step / 20e-6));
     }
+    const unsigned sg = (unsigned)((n + 3) / 4);  // kb_step_start: 4 items per work-group
     auto run_lane = [&](int lane) {
-        kb_step_begin<<<ig, 64, 0, B->cur>>>(B->d_items, n, lane);
-        kb_step_dt<<<ig, 64, 0, B->cur>>>(B->d_items, n, lane);
+        kb_step_start<<<sg, 256, 0, B->cur>>>(B->d_items, n, lane);
         for (int rk = 0; rk < B->rk_order; ++rk)
         {
             if (B->precision == MARA_PREC_MIXED) run_stage_f32(B, rk, lane);

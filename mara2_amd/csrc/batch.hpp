@@ -147,27 +147,31 @@ kb_courant(const BItem* __restrict__ items, const BTile* __restrict__ tiles)
     courant_to_bins(dtc, wmin, 4, threadIdx.x, it.ctl);
 }
 
-// per-item control, one thread per item
-static __global__ void kb_step_begin(const BItem* __restrict__ items, int n, int lane)
+// Per-item step start, one wavefront per item (4 items per work-group of
+// 256): the stop condition of k_ctl_step_start, then the CFL bins folded by
+// the wavefront and dt = min(cfl * courant, t_final - t).
+static __global__ void __launch_bounds__(256) kb_step_start(const BItem* __restrict__ items, int n, int lane)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n || items[t].lane != lane) return;
-    Ctl* ctl = items[t].ctl;
+    const int w = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int l = (int)(threadIdx.x & 63);
+    if (w >= n || items[w].lane != lane) return;
+    Ctl* ctl = items[w].ctl;
     const bool go = ctl->active && ctl->t < ctl->t_final && ctl->steps < ctl->max_steps
                     && ctl->c2p_fail == 0 && ctl->eigen_fail == 0 && ctl->superluminal == 0;
-    ctl->active = go ? 1 : 0;
-}
-
-static __global__ void kb_step_dt(const BItem* __restrict__ items, int n, int lane)
-{
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n || items[t].lane != lane) return;
-    Ctl* ctl = items[t].ctl;
-    if (!ctl->active) return;
-    fold_bins(ctl);
-    const double dt1 = ctl->cfl * bitsd(ctl->courant_bits);
-    const double dt2 = ctl->t_final - ctl->t;
-    ctl->dt = dt1 < dt2 ? dt1 : dt2;
+    if (!go)
+    {
+        if (l == 0) ctl->active = 0;
+        return;
+    }
+    const unsigned long long m = fold_bins_wave(ctl, l);
+    if (l == 0)
+    {
+        ctl->active = 1;
+        ctl->courant_bits = m;
+        const double dt1 = ctl->cfl * bitsd(m);
+        const double dt2 = ctl->t_final - ctl->t;
+        ctl->dt = dt1 < dt2 ? dt1 : dt2;
+    }
 }
 
 static __global__ void kb_step_end(const BItem* __restrict__ items, int n, int lane)

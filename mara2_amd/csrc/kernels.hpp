@@ -889,6 +889,17 @@ __device__ __forceinline__ void fold_bins(Ctl* ctl)
     reset_bins(ctl);
 }
 
+// The same with one wavefront: lane b takes bin b (CBINS == 64), the minimum
+// of unsigned bit patterns of non-negative doubles is order-free, so the
+// result equals fold_bins.  Every lane returns the minimum.
+static_assert(CBINS == 64, "one bin per lane");
+__device__ __forceinline__ unsigned long long fold_bins_wave(Ctl* ctl, int lane)
+{
+    const unsigned long long v = ctl->cbins[lane * CBIN_STRIDE];
+    ctl->cbins[lane * CBIN_STRIDE] = DBL_MAX_BITS;
+    return wave_min(v);
+}
+
 static __global__ void k_ctl_begin_courant(Ctl* ctl)
 {
     ctl->active = 1;
@@ -925,16 +936,26 @@ static __global__ void k_ctl_begin_run(Ctl* ctl, double t, double t_final, doubl
 // The CFL of the state at the start of the step sits in the bins (from the
 // standalone reduction on the first step, from the previous step's last
 // stage afterwards).
-static __global__ void k_ctl_step_start(Ctl* ctl)
+// One wavefront (64 threads): every lane reads the clock, lane b folds bin b.
+static __global__ void __launch_bounds__(64) k_ctl_step_start(Ctl* ctl)
 {
+    const int lane = threadIdx.x;
     const bool go = ctl->active && ctl->t < ctl->t_final && ctl->steps < ctl->max_steps
                     && ctl->c2p_fail == 0 && ctl->eigen_fail == 0 && ctl->superluminal == 0;
-    ctl->active = go ? 1 : 0;
-    if (!go) return;
-    fold_bins(ctl);
-    const double dt1 = ctl->cfl * bitsd(ctl->courant_bits);
-    const double dt2 = ctl->t_final - ctl->t;
-    ctl->dt = dt1 < dt2 ? dt1 : dt2;
+    if (!go)
+    {
+        if (lane == 0) ctl->active = 0;
+        return;
+    }
+    const unsigned long long m = fold_bins_wave(ctl, lane);
+    if (lane == 0)
+    {
+        ctl->active = 1;
+        ctl->courant_bits = m;
+        const double dt1 = ctl->cfl * bitsd(m);
+        const double dt2 = ctl->t_final - ctl->t;
+        ctl->dt = dt1 < dt2 ? dt1 : dt2;
+    }
 }
 
 static __global__ void k_ctl_step_end(Ctl* ctl)
