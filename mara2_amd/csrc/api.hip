@@ -67,7 +67,7 @@ long long device_bytes(const mara_desc* d, const Sizes& s)
 {
     // fused and pencil blocks compute each stage's fluxes inside the stage
     // pass (no flux arrays) and step Pa -> Pb (-> Pc) -> Pa
-    const bool in_pass = fused_supported(d) || pencil_supported(d);
+    const bool in_pass = fused_supported(d) || pencil_supported(d) || mhd2d_supported(d);
     long long b = 3ll * s.nq * s.fs * 8;  // Pa, Pb, U
     if (in_pass && d->rk_order == 3) b += 1ll * s.nq * s.fs * 8;  // Pc
     if (!in_pass)
@@ -92,6 +92,7 @@ struct mara_block
     double* U = nullptr;
     void* F[3] = {nullptr, nullptr, nullptr};
     int pencil = 0;  // 1-D pencil tile length (pencil_supported), else 0
+    bool mhd2d = false;  // fused 2-D MHD tiles (mhd2d_supported)
     int kz = FUSED_KZ;           // fused 3-D kernels: z planes per work-group
     int rows = ROWS_DEFAULT;     // flux-array kernels: work-group rows
     int pencil_shape = PENCIL_DEFAULT;  // requested pencil length (all laws)
@@ -153,7 +154,7 @@ void launch_stage(mara_block* b, int rk, int courant_out)
     const double* Pin = rk == 0 ? b->Pa : b->Pb;
     double* Pout = last ? b->Pa : b->Pb;
     const bool mixed = b->d.precision == MARA_PREC_MIXED;
-    if (b->fused || b->pencil)
+    if (b->fused || b->pencil || b->mhd2d)
     {
         // the fused and pencil kernels read neighbouring zones of their input
         // while other work-groups write their output, so input and output
@@ -181,6 +182,7 @@ void launch_stage(mara_block* b, int rk, int courant_out)
     s.fused = b->fused;
     s.ct = b->S.ct;
     s.pencil = b->pencil;
+    s.mhd2d = b->mhd2d;
     s.kz = b->kz;
     s.rows = b->rows;
     if (mixed) stage_kernels_f32(s);
@@ -334,6 +336,7 @@ int mara_block_create(const mara_desc* d, mara_block** out)
     b->ng = recon_ng(d->recon);
     b->fused = fused_supported(d);
     b->pencil = pencil_supported(d) ? PENCIL_DEFAULT : 0;
+    b->mhd2d = mhd2d_supported(d);
     b->G = make_geom(*d);
     {
         mara_shape tuned;
@@ -346,12 +349,12 @@ int mara_block_create(const mara_desc* d, mara_block** out)
     if (!hip_ok(hipMalloc(&b->Pa, cellbytes))) return fail(MARA_ERR_HIP);
     if (!hip_ok(hipMalloc(&b->Pb, cellbytes))) return fail(MARA_ERR_HIP);
     if (!hip_ok(hipMalloc(&b->U, cellbytes))) return fail(MARA_ERR_HIP);
-    if ((b->fused || b->pencil) && d->rk_order == 3)
+    if ((b->fused || b->pencil || b->mhd2d) && d->rk_order == 3)
     {
         if (!hip_ok(hipMalloc(&b->Pc, cellbytes))) return fail(MARA_ERR_HIP);
         if (!hip_ok(hipMemset(b->Pc, 0, cellbytes))) return fail(MARA_ERR_HIP);
     }
-    if (!b->fused && !b->pencil)
+    if (!b->fused && !b->pencil && !b->mhd2d)
     {
         for (int a = 0; a < 3; ++a)
         {
@@ -422,7 +425,7 @@ int mara_block_autotune(mara_block* b, int repeats, mara_shape* chosen, double* 
         for (int p : PENCIL_SHAPES) { mara_shape s = cur; s.pencil = p; cands.push_back(s); }
     else if (b->fused && b->G.N[2] > 1)
         for (int z : KZ_SHAPES) { mara_shape s = cur; s.zplanes = z; cands.push_back(s); }
-    else if (!b->fused && !b->pencil)
+    else if (!b->fused && !b->pencil && !b->mhd2d)
         for (int r : ROWS_SHAPES) { mara_shape s = cur; s.rows = r; cands.push_back(s); }
     else
         cands.push_back(cur);
@@ -470,6 +473,7 @@ int mara_block_autotune(mara_block* b, int repeats, mara_shape* chosen, double* 
         s.fused = b->fused;
         s.ct = b->S.ct;
         s.pencil = b->pencil ? pencil_length(b->d.law, c.pencil) : 0;
+        s.mhd2d = b->mhd2d;
         s.kz = c.zplanes;
         s.rows = c.rows;
         const bool mixed = b->d.precision == MARA_PREC_MIXED;
@@ -720,8 +724,10 @@ int mara_block_stage_kernel_time(mara_block* b, double* avg_ms, long long* launc
     if (avg_ms) *avg_ms = b->timed_launches ? b->timed_ms / b->timed_launches : 0.0;
     if (launches) *launches = b->timed_launches;
     if (name)
-        *name = b->fused ? fused_kernel_name(b->d)
-                         : b->pencil ? "k_pencil (1-D stage)" : "k_flux+k_update (flux-array stage)";
+        *name = b->fused   ? fused_kernel_name(b->d)
+                : b->mhd2d ? "k_mhd2d (2-D MHD stage)"
+                : b->pencil ? "k_pencil (1-D stage)"
+                            : "k_flux+k_update (flux-array stage)";
     return MARA_OK;
 }
 

@@ -25,8 +25,10 @@ int class_of(mara_batch* B, const mara_desc& d)
 {
     const bool one_d = d.cells[1] == 1 && d.cells[2] == 1;
     const bool rk2 = B->rk_order == 2;
-    const int fused = (!one_d && rk2 && d.law == MARA_LAW_HYDRO && d.recon == MARA_RECON_PLM)
-                          ? (d.cells[2] > 1 ? 2 : 1) : 0;
+    // fused tiles in RK2 batches: hydro + PLM in 2-D (1) and 3-D (2), MHD in 2-D (3)
+    int fused = 0;
+    if (!one_d && rk2 && d.law == MARA_LAW_HYDRO && d.recon == MARA_RECON_PLM) fused = d.cells[2] > 1 ? 2 : 1;
+    else if (!one_d && rk2 && d.law == MARA_LAW_MHD && d.cells[2] == 1) fused = 3;
     const ClassKey k{d.law, d.recon, d.riemann, (d.law != MARA_LAW_HYDRO && d.field_ct) ? 1 : 0,
                      (one_d && rk2) ? 1 : 0, fused};
     for (size_t c = 0; c < B->classes.size(); ++c)
@@ -78,6 +80,24 @@ void build_table(mara_batch* B)
     auto seg_begin = [&]() { return (int)B->tiles.size(); };
     for (size_t c = 0; c < B->classes.size(); ++c)
     {
+        if (B->classes[c].fused == 3)
+        {
+            // 2-D MHD: one 64 x TY tile per work-group; the kernel writes the
+            // ghost images itself
+            const int TY = B->precision == MARA_PREC_MIXED ? MhdTile<float>::TY : MhdTile<double>::TY;
+            const int s0 = seg_begin();
+            for (int n = 0; n < B->n_items; ++n)
+            {
+                const Slot& sl = B->slots[n];
+                if (!sl.used || sl.cls != (int)c) continue;
+                const Geom& G = B->items[n].G;
+                const int ntx = (G.n[0] + 63) / 64, nty = (G.n[1] + TY - 1) / TY;
+                for (int ty = 0; ty < nty; ++ty)
+                    for (int tx = 0; tx < ntx; ++tx) B->tiles.push_back(BTile{n, tx, ty, 0, 0, 0, 3, 0});
+            }
+            if ((int)B->tiles.size() > s0) B->update[(int)c] = Segment{s0, (int)B->tiles.size() - s0};
+            continue;
+        }
         if (B->classes[c].fused)
         {
             // one tile per FUSED_TX x TY x KZ block of each item, then the

@@ -82,7 +82,7 @@ struct ClassKey
 {
     int law, recon, riemann, ct;
     int pencil;  // 1-D items of an RK2 batch: fused pencil tiles
-    int fused;   // hydro + PLM items of an RK2 batch on 2-D (1) or 3-D (2) meshes: fused stage tiles
+    int fused;   // RK2 batch fused stage tiles: hydro + PLM on 2-D (1) or 3-D (2) meshes, MHD on 2-D (3)
     bool operator<(const ClassKey& o) const
     {
         return std::tie(law, recon, riemann, ct, pencil, fused)
@@ -198,9 +198,26 @@ void fused_launch(mara_batch* B, const BTile* t, int count, double b, int first,
                                                                                   B->kz);
 }
 
+template <int Recon, typename R>
+void mhd2d_launch(mara_batch* B, const ClassKey& k, const BTile* t, int count, double b, int first, int last,
+                  int cout)
+{
+    const dim3 block(64, MhdTile<R>::TY);
+    if (k.ct) kb_mhd2d<Recon, R, true><<<count, block, 0, B->cur>>>(B->d_items, t, b, first, last, cout);
+    else kb_mhd2d<Recon, R, false><<<count, block, 0, B->cur>>>(B->d_items, t, b, first, last, cout);
+}
+
 template <typename R>
 void update_dispatch(mara_batch* B, const ClassKey& k, const Segment& s, double b, int first, int last, int cout)
 {
+    if (k.fused == 3)
+    {
+        const BTile* t = B->d_tiles + s.offset;
+        if (k.recon == MARA_RECON_PCM) mhd2d_launch<0, R>(B, k, t, s.count, b, first, last, cout);
+        else if (k.recon == MARA_RECON_PLM) mhd2d_launch<1, R>(B, k, t, s.count, b, first, last, cout);
+        else mhd2d_launch<2, R>(B, k, t, s.count, b, first, last, cout);
+        return;
+    }
     if (k.pencil)
     {
         pencil_dispatch<R>(B, k, s, b, first, last, cout);

@@ -9,6 +9,7 @@
 #include "../../include/mara_gpu.h"
 #include "kernels.hpp"
 #include "fused.hpp"
+#include "fused_mhd.hpp"
 
 namespace mara {
 
@@ -29,6 +30,7 @@ struct StageArgs
     Ctl* ctl;
     bool fused, ct;
     int pencil;  // 1-D pencil tiles of this length, or 0
+    bool mhd2d;  // 2-D MHD: fused flux + CT + update tiles
     int kz;      // fused 3-D kernels: z planes per work-group
     int rows;    // flux-array kernels: work-group rows of 64 zones
 };
@@ -153,9 +155,9 @@ void stage_pencil_law(const StageArgs& s)
     }
 }
 
-// The stage's kernels: the fused hydro kernel, the 1-D pencil kernel, or one
-// flux sweep per axis followed by the zone update (the update and the pencil
-// kernel write the ghost zones themselves).
+// The stage's kernels: the fused hydro kernel, the fused 2-D MHD kernel, the
+// 1-D pencil kernel, or one flux sweep per axis followed by the zone update
+// (all but the fused hydro kernel write the ghost zones themselves).
 template <typename R>
 void stage_kernels(const StageArgs& s)
 {
@@ -163,6 +165,11 @@ void stage_kernels(const StageArgs& s)
     {
         launch_fused_r<R>(*s.d, *s.G, s.stream, s.Pin, s.P0, s.U, s.Pout, s.b, s.first, s.write_u, s.cout, s.ctl,
                           s.kz);
+        return;
+    }
+    if (s.mhd2d)
+    {
+        launch_mhd2d<R>(*s.d, *s.G, s.ct, s.stream, s.Pin, s.P0, s.U, s.Pout, s.b, s.first, s.write_u, s.cout, s.ctl);
         return;
     }
     if (s.pencil)
