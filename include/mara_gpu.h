@@ -189,8 +189,8 @@ int mara_block_set_shape(mara_block* block, const mara_shape* shape);
 
 /* Autotuner: times the candidates of the shape the block's stages use
    (pencil for 1-D blocks of two or three stages, zplanes for fused 3-D
-   hydro blocks, rows for the flux-array path; nothing to choose for fused
-   2-D blocks), `repeats` launches of a first RK stage each after one
+   hydro blocks, rows for the flux-array path; nothing to choose for the
+   fused 2-D hydro and MHD blocks), `repeats` launches of a first RK stage each after one
    untimed launch, on the block's own state with dt = 0 and a scratch
    control block: the state, clock and counters are untouched.  Sets the
    fastest candidate on the block and records it in the process's tuning
