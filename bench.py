@@ -177,6 +177,7 @@ def other_workload(args):
     for form in ("fp64", "mixed"):
         d = make_desc(n, form)
         with Block(d) as blk:
+            blk.set_graphs(not args.no_graphs)
             # warm-up (kernel loading, graph capture) and the launch shape
             blk.upload(P0)
             blk.run(cfl, 1e300, 2, check=False)
@@ -195,16 +196,20 @@ def other_workload(args):
             mono = blk.monopole_max()
             timed = None
             if steps < args.steps:
-                # throughput on the same work when the loop stopped early: the
-                # same number of steps at the first step's dt, failures ignored
-                # (the kernels do identical work per step either way)
+                # throughput when the loop stopped early: up to 10 steps from
+                # the initial state at its first dt, failures ignored.  Zones
+                # past an inversion failure run the Newton solver to its
+                # iteration limit every step and their number grows, so a
+                # longer fixed-dt run would time the failure, not the scheme
+                # (100 steps: 9.8 ms/step against 5.7 over the first 10).
+                n_fixed = min(args.steps, 10)
                 blk.upload(P0)
                 dt = cfl * blk.courant_dt()
                 blk.advance(dt, check=False)
                 t1 = time.perf_counter()
-                for _ in range(args.steps):
+                for _ in range(n_fixed):
                     blk.advance(dt, check=False)
-                timed = (time.perf_counter() - t1, args.steps)
+                timed = (time.perf_counter() - t1, n_fixed)
             blk.upload(P0)
             blk.set_timing(True)
             blk.run(cfl, 1e300, 2, check=False)
@@ -370,7 +375,7 @@ def main():
     ap.add_argument("--no-open-loop", action="store_true", help="serving: skip the Poisson-arrival run")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--no-graphs", action="store_true",
-                    help="sod, serving: launch every kernel from the host instead of replaying HIP graphs")
+                    help="sod, mhd2d, rmhd3d, serving: launch every kernel from the host instead of replaying HIP graphs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
