@@ -148,6 +148,23 @@ int mara_block_monopole_max(mara_block* block, double* max_abs);
 int mara_block_run(mara_block* block, double cfl, double t_final, int max_steps,
                    double* t, int* steps, mara_report* report);
 
+/* Source terms: the counterpart of MethodOfLinesTVD::setSourceTermsFunction
+   (src/SolutionSchemes.hpp:36, applied in src/SolutionSchemes.cpp:203-217).
+   The per-zone function is device code in a gfx950 code object written with
+   include/mara_source.h; the block loads `code_object`, takes its kernel
+   `kernel`, and before each RK stage evaluates S at every interior zone from
+   the stage's primitives, then adds S to the flux divergence L in the zone
+   update.  params (up to 16) reach the function as given; code_object = NULL
+   removes the source.  A block of the fused hydro path switches to the
+   flux-array path while a source is set (its fused update has no source
+   term).  MARA_ERR_INVALID_ARGUMENT when the file or the kernel cannot be
+   loaded.  Batches take no sources. */
+int mara_block_set_source(mara_block* block, const char* code_object, const char* kernel, const double* params,
+                          int n_params);
+/* The 8 auxiliary values passed to the source function (the reference
+   passes its star-particle data there); they start at zero. */
+int mara_block_set_source_aux(mara_block* block, const double* t);
+
 /* mara_block_run replays its steps from a HIP graph of 16 steps, captured on
    the first run (enable=1, the default), or launches every kernel from the
    host (enable=0).  Results are the same either way; runs with stage timing

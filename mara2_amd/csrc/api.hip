@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/mara_gpu.h"
+#include "../../include/mara_source.h"
 #include "kernels.hpp"
 #include "fused.hpp"
 #include "host_common.hpp"
@@ -115,6 +116,12 @@ struct mara_block
     static constexpr int kGraphSteps = 16;
     bool graphs = true;
     hipGraphExec_t steps_exec = nullptr;
+    // source terms (mara_block_set_source): the loaded code object, its
+    // kernel, the launch record and the S array (field layout of P)
+    hipModule_t src_module = nullptr;
+    hipFunction_t src_fn = nullptr;
+    mara_source_args src_args = {};
+    double* Ssrc = nullptr;
 };
 
 namespace {
@@ -163,6 +170,16 @@ void launch_stage(mara_block* b, int rk, int courant_out)
         Pout = last ? b->Pa : (rk == 0 ? b->Pb : b->Pc);
     }
 
+    // source terms of the stage's primitives (MeshOperator::generateSourceTerms)
+    if (b->src_fn)
+    {
+        b->src_args.P = Pin;
+        b->src_args.S = b->Ssrc;
+        void* args[] = {&b->src_args};
+        const long long zones = (long long)b->G.n[0] * b->G.n[1] * b->G.n[2];
+        (void)hipModuleLaunchKernel(b->src_fn, (unsigned)((zones + 255) / 256), 1, 1, 256, 1, 1, 0, b->stream, args,
+                                    nullptr);
+    }
     const bool timed = b->timing && b->ev_used < mara_block::kTimingPairs;
     if (timed) hipEventRecord(b->ev[2 * b->ev_used], b->stream);
     StageArgs s;
@@ -185,6 +202,7 @@ void launch_stage(mara_block* b, int rk, int courant_out)
     s.mhd2d = b->mhd2d;
     s.kz = b->kz;
     s.rows = b->rows;
+    s.S = b->src_fn ? b->Ssrc : nullptr;
     if (mixed) stage_kernels_f32(s);
     else stage_kernels<double>(s);
     if (timed)
@@ -391,6 +409,8 @@ int mara_block_destroy(mara_block* b)
     hipFree(b->ctl);
     hipFree(b->cbins);
     if (b->steps_exec) (void)hipGraphExecDestroy(b->steps_exec);
+    if (b->src_module) (void)hipModuleUnload(b->src_module);
+    hipFree(b->Ssrc);
     if (b->ctl_host) hipHostFree(b->ctl_host);
     for (int e = 0; e < 2 * mara_block::kTimingPairs; ++e)
         if (b->ev[e]) hipEventDestroy(b->ev[e]);
@@ -476,6 +496,7 @@ int mara_block_autotune(mara_block* b, int repeats, mara_shape* chosen, double* 
         s.mhd2d = b->mhd2d;
         s.kz = c.zplanes;
         s.rows = c.rows;
+        s.S = b->src_fn ? b->Ssrc : nullptr;
         const bool mixed = b->d.precision == MARA_PREC_MIXED;
         for (int r = 0; r <= repeats; ++r)
         {
@@ -685,6 +706,81 @@ int mara_block_run(mara_block* b, double cfl, double t_final, int max_steps,
     *t = b->ctl_host->t;
     *steps = b->ctl_host->steps;
     return fill_report(b, report);
+}
+
+int mara_block_set_source(mara_block* b, const char* code_object, const char* kernel, const double* params,
+                          int n_params)
+{
+    if (!b || n_params < 0 || n_params > MARA_SOURCE_MAX_PARAMS || (n_params > 0 && !params)
+        || (code_object && !kernel))
+        return MARA_ERR_INVALID_ARGUMENT;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    if (b->src_module) (void)hipModuleUnload(b->src_module);
+    b->src_module = nullptr;
+    b->src_fn = nullptr;
+    if (b->steps_exec) (void)hipGraphExecDestroy(b->steps_exec);  // captured without / with the old source
+    b->steps_exec = nullptr;
+    if (!code_object)
+    {
+        b->fused = fused_supported(&b->d);
+        return MARA_OK;
+    }
+    if (hipModuleLoad(&b->src_module, code_object) != hipSuccess)
+    {
+        std::fprintf(stderr, "mara_gpu: cannot load source code object %s\n", code_object);
+        b->src_module = nullptr;
+        return MARA_ERR_INVALID_ARGUMENT;
+    }
+    if (hipModuleGetFunction(&b->src_fn, b->src_module, kernel) != hipSuccess)
+    {
+        std::fprintf(stderr, "mara_gpu: no kernel %s in %s\n", kernel, code_object);
+        (void)hipModuleUnload(b->src_module);
+        b->src_module = nullptr;
+        b->src_fn = nullptr;
+        return MARA_ERR_INVALID_ARGUMENT;
+    }
+    const size_t cellbytes = (size_t)b->S.nq * b->S.fs * sizeof(double);
+    if (!b->Ssrc)
+    {
+        HIP_TRY(hipMalloc(&b->Ssrc, cellbytes));
+        HIP_TRY(hipMemset(b->Ssrc, 0, cellbytes));
+        b->bytes += (long long)cellbytes;
+    }
+    // the fused hydro kernels update zones without a source term: such a
+    // block takes the flux-array path while a source is set
+    if (b->fused)
+    {
+        for (int a = 0; a < 3; ++a)
+        {
+            if (!b->S.fleshed[a] || b->F[a]) continue;
+            const size_t fb = (size_t)b->S.nq * b->S.ffs * b->S.real;
+            HIP_TRY(hipMalloc(&b->F[a], fb));
+            b->bytes += (long long)fb;
+        }
+        b->fused = false;
+    }
+    mara_source_args& a = b->src_args;
+    a.fs = b->G.fs;
+    for (int e = 0; e < 3; ++e)
+    {
+        a.N[e] = b->G.N[e];
+        a.g[e] = b->G.g[e];
+        a.n[e] = b->G.n[e];
+        a.lower[e] = b->d.lower[e];
+        a.dx[e] = (b->d.upper[e] - b->d.lower[e]) / b->d.cells[e];  // CartesianMeshGeometry
+    }
+    a.nq = b->S.nq;
+    for (int q = 0; q < MARA_SOURCE_MAX_PARAMS; ++q) a.params[q] = q < n_params ? params[q] : 0.0;
+    return MARA_OK;
+}
+
+int mara_block_set_source_aux(mara_block* b, const double* t)
+{
+    if (!b || !t) return MARA_ERR_INVALID_ARGUMENT;
+    for (int q = 0; q < 8; ++q) b->src_args.t[q] = t[q];
+    if (b->steps_exec) (void)hipGraphExecDestroy(b->steps_exec);  // launch records are captured by value
+    b->steps_exec = nullptr;
+    return MARA_OK;
 }
 
 int mara_block_set_graphs(mara_block* b, int enable)

@@ -76,7 +76,7 @@ kb_update(const BItem* __restrict__ items, const BTile* __restrict__ tiles, doub
     double dtc = 1.79769313486231570815e+308;
     if (tile_thread(T, i, j, k))
         dtc = update_zone<Law, R, CT>(it.Pa, it.U, last ? it.Pa : it.Pb, (const R*)it.F[0], (const R*)it.F[1],
-                                      (const R*)it.F[2], it.G, b, first, !last, courant_out, it.ctl, i, j, k);
+                                      (const R*)it.F[2], it.G, b, first, !last, courant_out, it.ctl, i, j, k, nullptr);
     if (courant_out) courant_to_bins(dtc, wmin, 4, threadIdx.x, it.ctl);
 }
 
@@ -97,7 +97,7 @@ kb_pencil(const BItem* __restrict__ items, const BTile* __restrict__ tiles, doub
     const double* Pin = first ? it.Pa : it.Pb;
     double* Pout = last ? it.Pa : it.Pb;
     const double dtc = pencil_tile<Law, Recon, Riem, R, CT, PENCIL>(Pin, it.Pa, it.U, Pout, it.G, b, first, !last,
-                                                                    courant_out, it.ctl, T.x0, sP, sF);
+                                                                    courant_out, it.ctl, T.x0, sP, sF, nullptr);
     if (courant_out) courant_to_bins(dtc, wmin, PENCIL / 64, threadIdx.x, it.ctl);
 }
 
@@ -132,7 +132,7 @@ kb_mhd2d(const BItem* __restrict__ items, const BTile* __restrict__ tiles, doubl
     const Geom G = it.G;
     const double* Pin = first ? it.Pa : it.Pb;
     double* Pout = last ? it.Pa : it.Pb;
-    mhd2d_tile<Recon, R, CT>(Pin, it.Pa, it.U, Pout, G, b, first, !last, courant_out, it.ctl, T.x0, T.y0);
+    mhd2d_tile<Recon, R, CT>(Pin, it.Pa, it.U, Pout, G, b, first, !last, courant_out, it.ctl, T.x0, T.y0, nullptr);
 }
 
 // Ghost zones of the fused items (their kernel leaves them): tile x0 is the

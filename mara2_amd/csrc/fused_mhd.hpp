@@ -33,7 +33,8 @@ template <typename R> struct MhdTile
 template <int Recon, typename R, bool CT>
 __device__ __forceinline__ void mhd2d_tile(const double* __restrict__ Pin, const double* __restrict__ P0,
                                            double* __restrict__ U, double* __restrict__ Pout, const Geom& G, double b,
-                                           int first_stage, int write_u, int courant_out, Ctl* ctl, int tx, int ty)
+                                           int first_stage, int write_u, int courant_out, Ctl* ctl, int tx, int ty,
+                                           const double* __restrict__ Ssrc)
 {
     constexpr int NQ = 8;
     constexpr int TX = 64, TY = MhdTile<R>::TY, NT = TX * TY;
@@ -117,7 +118,7 @@ __device__ __forceinline__ void mhd2d_tile(const double* __restrict__ Pin, const
                 L[q] = divergence_R<R>(G, fx[q][xr][xc + 1], fx[q][xr][xc], fy[q][yr + 1][yc], fy[q][yr][yc], R(0),
                                        R(0));
         }
-        dtc = finish_zone<1, R>(P0, U, Pout, G, b, first_stage, write_u, courant_out, ctl, i, j, k, L);
+        dtc = finish_zone<1, R>(P0, U, Pout, G, b, first_stage, write_u, courant_out, ctl, i, j, k, L, Ssrc);
     }
     if (courant_out) courant_to_bins(dtc, wmin, NT / 64, tid, ctl);
 }
@@ -125,13 +126,15 @@ __device__ __forceinline__ void mhd2d_tile(const double* __restrict__ Pin, const
 template <int Recon, typename R, bool CT>
 static __global__ void __launch_bounds__(64 * MhdTile<R>::TY)
 k_mhd2d(const double* __restrict__ Pin, const double* __restrict__ P0, double* __restrict__ U,
-        double* __restrict__ Pout, Geom G, double b, int first_stage, int write_u, int courant_out, Ctl* ctl)
+        double* __restrict__ Pout, Geom G, double b, int first_stage, int write_u, int courant_out, Ctl* ctl,
+        const double* __restrict__ Ssrc)
 {
     if (!ctl->active) return;
     constexpr int TY = MhdTile<R>::TY;
     const int ntx = (G.n[0] + 63) / 64, nty = (G.n[1] + TY - 1) / TY;
     const int tile = xcd_tile(blockIdx.x, ntx * nty);
-    mhd2d_tile<Recon, R, CT>(Pin, P0, U, Pout, G, b, first_stage, write_u, courant_out, ctl, tile % ntx, tile / ntx);
+    mhd2d_tile<Recon, R, CT>(Pin, P0, U, Pout, G, b, first_stage, write_u, courant_out, ctl, tile % ntx, tile / ntx,
+                             Ssrc);
 }
 
 // 2-D MHD blocks of two or three stages (the z axis flat, x and y fleshed):
@@ -144,25 +147,25 @@ inline bool mhd2d_supported(const mara_desc* d)
 
 template <int Recon, typename R>
 inline void launch_mhd2d_t(const Geom& G, bool ct, hipStream_t s, const double* Pin, const double* P0, double* U,
-                           double* Pout, double b, int first, int write_u, int cout, Ctl* ctl)
+                           double* Pout, double b, int first, int write_u, int cout, Ctl* ctl, const double* Ssrc)
 {
     constexpr int TY = MhdTile<R>::TY;
     const dim3 block(64, TY, 1);
     const dim3 grid(((G.n[0] + 63) / 64) * ((G.n[1] + TY - 1) / TY));
-    if (ct) k_mhd2d<Recon, R, true><<<grid, block, 0, s>>>(Pin, P0, U, Pout, G, b, first, write_u, cout, ctl);
-    else k_mhd2d<Recon, R, false><<<grid, block, 0, s>>>(Pin, P0, U, Pout, G, b, first, write_u, cout, ctl);
+    if (ct) k_mhd2d<Recon, R, true><<<grid, block, 0, s>>>(Pin, P0, U, Pout, G, b, first, write_u, cout, ctl, Ssrc);
+    else k_mhd2d<Recon, R, false><<<grid, block, 0, s>>>(Pin, P0, U, Pout, G, b, first, write_u, cout, ctl, Ssrc);
 }
 
 template <typename R>
 inline void launch_mhd2d(const mara_desc& d, const Geom& G, bool ct, hipStream_t s, const double* Pin,
                          const double* P0, double* U, double* Pout, double b, int first, int write_u, int cout,
-                         Ctl* ctl)
+                         Ctl* ctl, const double* Ssrc)
 {
     switch (d.recon)
     {
-        case MARA_RECON_PCM: launch_mhd2d_t<0, R>(G, ct, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl); break;
-        case MARA_RECON_PLM: launch_mhd2d_t<1, R>(G, ct, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl); break;
-        default: launch_mhd2d_t<2, R>(G, ct, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl); break;
+        case MARA_RECON_PCM: launch_mhd2d_t<0, R>(G, ct, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, Ssrc); break;
+        case MARA_RECON_PLM: launch_mhd2d_t<1, R>(G, ct, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, Ssrc); break;
+        default: launch_mhd2d_t<2, R>(G, ct, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, Ssrc); break;
     }
 }
 

@@ -379,7 +379,7 @@ template <int Law, typename R = double>
 __device__ __forceinline__ double finish_zone(const double* __restrict__ P0, double* __restrict__ U,
                                               double* __restrict__ Pout, const Geom& G, double b, int first_stage,
                                               int write_u, int courant_out, Ctl* ctl, int i, int j, int k,
-                                              const double* L)
+                                              const double* L, const double* __restrict__ Ssrc)
 {
     double dtc = 1.79769313486231570815e+308;
     constexpr int NQ = LawNQ<Law>::value;
@@ -396,7 +396,9 @@ __device__ __forceinline__ double finish_zone(const double* __restrict__ P0, dou
     for (int q = 0; q < NQ; ++q)
     {
         const double Uq = first_stage ? U0[q] : U[q * G.fs + c];
-        Un[q] = U0[q] * (1 - b) + (Uq + dt * L[q]) * b;
+        // L += S (src/SolutionSchemes.cpp:209-217) when a source is set
+        const double Lq = Ssrc ? L[q] + Ssrc[q * G.fs + c] : L[q];
+        Un[q] = U0[q] * (1 - b) + (Uq + dt * Lq) * b;
     }
     constexpr bool MIX = sizeof(R) == 4 && Law < 2;
     double Pn[NQ];
@@ -463,7 +465,8 @@ template <int Law, typename R, bool CT>
 __device__ __forceinline__ double update_zone(const double* __restrict__ P0, double* __restrict__ U, double* __restrict__ Pout,
                                               const R* __restrict__ Fx, const R* __restrict__ Fy, const R* __restrict__ Fz,
                                               const Geom& G, double b, int first_stage,
-                                              int write_u, int courant_out, Ctl* ctl, int i, int j, int k)
+                                              int write_u, int courant_out, Ctl* ctl, int i, int j, int k,
+                                              const double* __restrict__ Ssrc)
 {
     constexpr int NQ = LawNQ<Law>::value;
     const long long f = face_index(G, i, j, k);
@@ -492,7 +495,7 @@ __device__ __forceinline__ double update_zone(const double* __restrict__ P0, dou
             L[q] = divergence_R<R>(G, x1, x0, y1, y0, z1, z0);
         }
     }
-    return finish_zone<Law, R>(P0, U, Pout, G, b, first_stage, write_u, courant_out, ctl, i, j, k, L);
+    return finish_zone<Law, R>(P0, U, Pout, G, b, first_stage, write_u, courant_out, ctl, i, j, k, L, Ssrc);
 }
 
 // ---------------------------------------------------------------------------
@@ -529,7 +532,8 @@ template <int Law, int Recon, int Riem, typename R, bool CT, int PENCIL>
 __device__ __forceinline__ double pencil_tile(const double* __restrict__ Pin, const double* __restrict__ P0,
                                               double* __restrict__ U, double* __restrict__ Pout, const Geom& G,
                                               double b, int first_stage, int write_u, int courant_out, Ctl* ctl,
-                                              int x0, R* __restrict__ sP, R* __restrict__ sF)
+                                              int x0, R* __restrict__ sP, R* __restrict__ sF,
+                                              const double* __restrict__ Ssrc)
 {
     constexpr int NQ = LawNQ<Law>::value;
     constexpr int NG = Stencil<Recon>::NG;
@@ -579,7 +583,7 @@ __device__ __forceinline__ double pencil_tile(const double* __restrict__ Pin, co
             else
                 L[q] = divergence_R<R>(G, fr, fl, R(0), R(0), R(0), R(0));
         }
-        dtc = finish_zone<Law, R>(P0, U, Pout, G, b, first_stage, write_u, courant_out, ctl, zi, 0, 0, L);
+        dtc = finish_zone<Law, R>(P0, U, Pout, G, b, first_stage, write_u, courant_out, ctl, zi, 0, 0, L, Ssrc);
     }
     return dtc;
 }
@@ -589,7 +593,8 @@ __device__ __forceinline__ double pencil_tile(const double* __restrict__ Pin, co
 template <int Law, int Recon, int Riem, typename R, bool CT, int PENCIL>
 static __global__ void __launch_bounds__(PENCIL)
 k_pencil(const double* __restrict__ Pin, const double* __restrict__ P0, double* __restrict__ U,
-         double* __restrict__ Pout, Geom G, double b, int first_stage, int write_u, int courant_out, Ctl* ctl)
+         double* __restrict__ Pout, Geom G, double b, int first_stage, int write_u, int courant_out, Ctl* ctl,
+         const double* __restrict__ Ssrc)
 {
     constexpr int NQ = PencilLds<Law, Recon, PENCIL>::NQ, W = PencilLds<Law, Recon, PENCIL>::W;
     __shared__ R sP[NQ * W];
@@ -598,7 +603,7 @@ k_pencil(const double* __restrict__ Pin, const double* __restrict__ P0, double* 
     if (!ctl->active) return;
     const int x0 = G.g[0] + (int)blockIdx.x * (PENCIL - 1);
     const double dtc = pencil_tile<Law, Recon, Riem, R, CT, PENCIL>(Pin, P0, U, Pout, G, b, first_stage, write_u,
-                                                                    courant_out, ctl, x0, sP, sF);
+                                                                    courant_out, ctl, x0, sP, sF, Ssrc);
     if (courant_out) courant_to_bins(dtc, wmin, PENCIL / 64, threadIdx.x, ctl);
 }
 
@@ -607,7 +612,7 @@ static __global__ void __launch_bounds__(256)
 k_update(const double* __restrict__ P0, double* __restrict__ U, double* __restrict__ Pout,
          const R* __restrict__ Fx, const R* __restrict__ Fy, const R* __restrict__ Fz,
          Geom G, double b, int first_stage, int write_u,
-         int courant_out, Ctl* ctl)
+         int courant_out, Ctl* ctl, const double* __restrict__ Ssrc)
 {
     __shared__ double wmin[4];
     if (!ctl->active) return;
@@ -616,7 +621,7 @@ k_update(const double* __restrict__ P0, double* __restrict__ U, double* __restri
     double dtc = 1.79769313486231570815e+308;
     if (ti < G.n[0] && tj < G.n[1])
         dtc = update_zone<Law, R, CT>(P0, U, Pout, Fx, Fy, Fz, G, b, first_stage, write_u, courant_out, ctl,
-                                      G.g[0] + ti, G.g[1] + tj, G.g[2] + (int)blockIdx.z);
+                                      G.g[0] + ti, G.g[1] + tj, G.g[2] + (int)blockIdx.z, Ssrc);
     if (courant_out) courant_to_bins(dtc, wmin, (int)blockDim.y, threadIdx.y * blockDim.x + threadIdx.x, ctl);
 }
 

@@ -33,6 +33,7 @@ struct StageArgs
     bool mhd2d;  // 2-D MHD: fused flux + CT + update tiles
     int kz;      // fused 3-D kernels: z planes per work-group
     int rows;    // flux-array kernels: work-group rows of 64 zones
+    const double* S;  // source terms added to L (same layout as P), or null
 };
 
 // 2-D work-groups of 64 x rows over an (m0, m1, m2) box: one wavefront per x
@@ -93,11 +94,11 @@ void stage_update_law(const StageArgs& s)
     if (s.ct)
         k_update<Law, R, true><<<grid, stage_block3(s.rows), 0, s.stream>>>(
             s.P0, s.U, s.Pout, (const R*)s.F[0], (const R*)s.F[1], (const R*)s.F[2],
-            G, s.b, s.first, s.write_u, s.cout, s.ctl);
+            G, s.b, s.first, s.write_u, s.cout, s.ctl, s.S);
     else
         k_update<Law, R, false><<<grid, stage_block3(s.rows), 0, s.stream>>>(
             s.P0, s.U, s.Pout, (const R*)s.F[0], (const R*)s.F[1], (const R*)s.F[2],
-            G, s.b, s.first, s.write_u, s.cout, s.ctl);
+            G, s.b, s.first, s.write_u, s.cout, s.ctl, s.S);
 }
 
 template <int Law, int Recon, int Riem, typename R, int PL>
@@ -109,11 +110,11 @@ void stage_pencil_pl(const StageArgs& s)
         if (s.ct)
         {
             k_pencil<Law, Recon, Riem, R, true, PL><<<tiles, PL, 0, s.stream>>>(s.Pin, s.P0, s.U, s.Pout, G, s.b,
-                                                                               s.first, s.write_u, s.cout, s.ctl);
+                                                                               s.first, s.write_u, s.cout, s.ctl, s.S);
             return;
         }
     k_pencil<Law, Recon, Riem, R, false, PL><<<tiles, PL, 0, s.stream>>>(s.Pin, s.P0, s.U, s.Pout, G, s.b, s.first,
-                                                                        s.write_u, s.cout, s.ctl);
+                                                                        s.write_u, s.cout, s.ctl, s.S);
 }
 
 template <int Law, int Recon, int Riem, typename R>
@@ -169,7 +170,8 @@ void stage_kernels(const StageArgs& s)
     }
     if (s.mhd2d)
     {
-        launch_mhd2d<R>(*s.d, *s.G, s.ct, s.stream, s.Pin, s.P0, s.U, s.Pout, s.b, s.first, s.write_u, s.cout, s.ctl);
+        launch_mhd2d<R>(*s.d, *s.G, s.ct, s.stream, s.Pin, s.P0, s.U, s.Pout, s.b, s.first, s.write_u, s.cout, s.ctl,
+                        s.S);
         return;
     }
     if (s.pencil)

@@ -57,6 +57,8 @@ def load():
                                  ctypes.POINTER(ctypes.c_int), ctypes.POINTER(MaraReport)]
     L.mara_block_set_timing.argtypes = [blk, ctypes.c_int]
     L.mara_block_set_graphs.argtypes = [blk, ctypes.c_int]
+    L.mara_block_set_source.argtypes = [blk, ctypes.c_char_p, ctypes.c_char_p, _dp, ctypes.c_int]
+    L.mara_block_set_source_aux.argtypes = [blk, _dp]
     shp = ctypes.POINTER(MaraShape)
     L.mara_block_get_shape.argtypes = [blk, shp]
     L.mara_block_set_shape.argtypes = [blk, shp]
@@ -182,6 +184,20 @@ class Block:
         if check:
             _check(st, "mara_block_run", rep)
         return tt.value, steps.value, rep
+
+    def set_source(self, code_object, kernel=None, params=()):
+        """Load a source-term code object (include/mara_source.h) and its
+        kernel; S is added to L in every stage.  code_object=None removes it."""
+        pr = np.ascontiguousarray(params, dtype=np.float64)
+        _check(self.lib.mara_block_set_source(
+            self._h, os.fsencode(code_object) if code_object else None, kernel.encode() if kernel else None,
+            pr.ctypes.data_as(_dp) if len(pr) else None, len(pr)), "mara_block_set_source")
+
+    def set_source_aux(self, t):
+        """The 8 auxiliary values passed to the source function."""
+        t8 = np.zeros(8)
+        t8[:len(t)] = t
+        _check(self.lib.mara_block_set_source_aux(self._h, t8.ctypes.data_as(_dp)), "mara_block_set_source_aux")
 
     def set_graphs(self, on=True):
         """Replay run() steps from a captured HIP graph (default) or launch

@@ -50,6 +50,8 @@ def lib():
         L.oracle_vector_potential_to_B.argtypes = [ctypes.POINTER(MaraDesc), _dp, _dp]
         L.oracle_apply_bc.argtypes = [ctypes.POINTER(MaraDesc), _dp]
         L.oracle_advance.argtypes = [ctypes.POINTER(MaraDesc), _dp, _d, _d, ctypes.POINTER(MaraReport)]
+        L.oracle_advance_src.argtypes = [ctypes.POINTER(MaraDesc), _dp, _d, _d, ctypes.c_void_p, _dp, _dp,
+                                         ctypes.POINTER(MaraReport)]
         L.oracle_courant_dt.argtypes = [ctypes.POINTER(MaraDesc), _dp, _dp]
         L.oracle_monopole_max.argtypes = [ctypes.POINTER(MaraDesc), _dp, _dp]
         L.oracle_set_threads.argtypes = [_i]
@@ -170,11 +172,27 @@ def apply_bc(desc, P):
     return lib().oracle_apply_bc(ctypes.byref(desc), _ptr(P))
 
 
-def advance(desc, P, t0, dt):
-    """MethodOfLinesTVD::advance on P (MeshData::P layout) in place."""
+# test sources compiled into the oracle (mara_oracle.c): name -> C symbol
+SOURCES = {"uniform_gravity": "oracle_source_uniform_gravity", "point_mass": "oracle_source_point_mass"}
+
+
+def advance(desc, P, t0, dt, source=None):
+    """MethodOfLinesTVD::advance on P (MeshData::P layout) in place.
+    source: None or (name, t[8], params) -- a sourceTermsFunction of SOURCES
+    with its auxiliary data and parameters."""
     assert P.shape == array_shape(desc)
     rep = MaraReport()
-    err = lib().oracle_advance(ctypes.byref(desc), _ptr(P), t0, dt, ctypes.byref(rep))
+    if source is None:
+        err = lib().oracle_advance(ctypes.byref(desc), _ptr(P), t0, dt, ctypes.byref(rep))
+    else:
+        name, taux, params = source
+        fn = ctypes.cast(getattr(lib(), SOURCES[name]), ctypes.c_void_p)
+        t8 = np.zeros(8)
+        t8[:len(taux)] = taux
+        pr = np.zeros(16)
+        pr[:len(params)] = params
+        err = lib().oracle_advance_src(ctypes.byref(desc), _ptr(P), t0, dt, fn, _ptr(t8), _ptr(pr),
+                                       ctypes.byref(rep))
     return err, rep
 
 
@@ -194,7 +212,7 @@ def set_threads(n):
     lib().oracle_set_threads(int(n))
 
 
-def run(desc, P, cfl, t_final, max_steps=10**9):
+def run(desc, P, cfl, t_final, max_steps=10**9, source=None):
     """The maraMainLoop contract (src/Mara.cpp:42-88) on the oracle:
     dt = min(cfl * courant, t_final - t) until t >= t_final."""
     t, steps = 0.0, 0
@@ -205,7 +223,7 @@ def run(desc, P, cfl, t_final, max_steps=10**9):
         dt1 = cfl * dtc
         dt2 = t_final - t
         dt = dt1 if dt1 < dt2 else dt2
-        err, rep = advance(desc, P, 0.0, dt)
+        err, rep = advance(desc, P, 0.0, dt, source)
         if err:
             return err, t, steps
         t += dt

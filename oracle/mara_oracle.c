@@ -1101,10 +1101,65 @@ int oracle_apply_bc(const mara_desc* d, double* P)
 }
 
 /* ========================================================================== */
+/* Source terms of the reference's SourceTermsFunction shape (src/Mara.hpp:67):
+   S(x, y, z, t[8], P) for one zone, here with a parameter array.  Two test
+   sources, restated on the device in mara2_amd/sources/: uniform gravity
+   g = params[0..2], and a point mass GM = params[0] with softening
+   eps^2 = params[1] at (t[0], t[1], t[2]) (the auxiliary data play the role
+   of the reference's star-particle positions). */
+typedef void (*oracle_source_fn)(double x, double y, double z, const double* t, const double* P, double* S,
+                                 const double* params);
+
+void oracle_source_uniform_gravity(double x, double y, double z, const double* t, const double* P, double* S,
+                                   const double* params)
+{
+    (void)x; (void)y; (void)z; (void)t;
+    const double g0 = params[0], g1 = params[1], g2 = params[2];
+    S[0] = 0.0;
+    S[1] = P[0] * g0;
+    S[2] = P[0] * g1;
+    S[3] = P[0] * g2;
+    S[4] = P[0] * (P[1] * g0 + P[2] * g1 + P[3] * g2);
+}
+
+void oracle_source_point_mass(double x, double y, double z, const double* t, const double* P, double* S,
+                              const double* params)
+{
+    const double dx = x - t[0], dy = y - t[1], dz = z - t[2];
+    const double r2 = dx * dx + dy * dy + dz * dz + params[1];
+    const double f = -params[0] / (r2 * sqrt(r2));
+    const double a0 = f * dx, a1 = f * dy, a2 = f * dz;
+    S[0] = 0.0;
+    S[1] = P[0] * a0;
+    S[2] = P[0] * a1;
+    S[3] = P[0] * a2;
+    S[4] = P[0] * (P[1] * a0 + P[2] * a1 + P[3] * a2);
+}
+
+static int advance_impl(const mara_desc* d, double* P, double dt, oracle_source_fn src, const double* taux,
+                        const double* params, mara_report* rep);
+
 /* MethodOfLinesTVD::advance (src/SolutionSchemes.cpp:108-275). */
 int oracle_advance(const mara_desc* d, double* P, double t0, double dt, mara_report* rep)
 {
     (void)t0;
+    return advance_impl(d, P, dt, NULL, NULL, NULL, rep);
+}
+
+/* The same with a sourceTermsFunction: after the divergence, S is evaluated
+   at every cell centre from the stage's primitives and added to L
+   (src/SolutionSchemes.cpp:203-217, MeshOperator::generateSourceTerms,
+   src/MeshOperator.cpp:197-219). */
+int oracle_advance_src(const mara_desc* d, double* P, double t0, double dt, oracle_source_fn src,
+                       const double* taux, const double* params, mara_report* rep)
+{
+    (void)t0;
+    return advance_impl(d, P, dt, src, taux, params, rep);
+}
+
+static int advance_impl(const mara_desc* d, double* P, double dt, oracle_source_fn src, const double* taux,
+                        const double* params, mara_report* rep)
+{
     layout l;
     const int nq = law_nq(d->law);
     const int ng = stencil_size(d->recon);
@@ -1199,6 +1254,23 @@ int oracle_advance(const mara_desc* d, double* P, double t0, double dt, mara_rep
 
         /* L = divergence(F, -1), U = U0 (1-b) + (U + dt L) b */
         oracle_divergence(F, fs, nq, -1.0, dA, dV, L);
+        if (src)
+        {
+            /* coordinateAtIndex(i + start) with start = -guards (src/CartesianMeshGeometry.cpp:61-67) */
+            #pragma omp parallel for collapse(2) schedule(static)
+            for (int i = 0; i < l.N[0]; ++i)
+            for (int j = 0; j < l.N[1]; ++j)
+            for (int k = 0; k < l.N[2]; ++k)
+            {
+                const double x = d->lower[0] + dx[0] * ((i - d->guards[0]) + 0.5);
+                const double y = d->lower[1] + dx[1] * ((j - d->guards[1]) + 0.5);
+                const double z = d->lower[2] + dx[2] * ((k - d->guards[2]) + 0.5);
+                const long c = cidx(&l, i, j, k);
+                double Sq[NF] = {0};
+                src(x, y, z, taux, P + c, Sq, params);
+                for (int q = 0; q < nq; ++q) L[c + q] += Sq[q];
+            }
+        }
         #pragma omp parallel for schedule(static)
         for (long n = 0; n < ncell * nq; ++n)
             U[n] = U0[n] * (1 - b[rk]) + (U[n] + dt * L[n]) * b[rk];

@@ -350,3 +350,34 @@ def test_field_loop_returns_and_converges():
         e, mono = oracle.monopole_max(d, P)
         assert e == 0 and mono <= 1e-12 * np.abs(B0).max() * n
     assert errs[1] < 0.5 * errs[0] and errs[1] < 0.4, errs
+
+
+def test_uniform_gravity_source_on_a_uniform_state():
+    """sourceTermsFunction path (src/SolutionSchemes.cpp:203-217) with uniform
+    gravity: no gradients, so the fluxes cancel exactly, rho v grows as
+    rho g t, and the energy source rho v . g adds exactly the kinetic energy
+    (p unchanged); a point mass pulls the gas towards the position held in
+    the auxiliary data."""
+    d = abi.make_desc(law="hydro", recon="plm", riemann="hllc", bc="periodic", cells=(16, 8, 1))
+    P = problems.empty(d)
+    P[..., 0] = 2.0
+    P[..., 4] = 1.5
+    g = np.array([0.25, -0.5, 0.0])
+    t = 0.0
+    for _ in range(6):
+        err, dtc = oracle.courant_dt(d, P)
+        dt = 0.4 * dtc
+        err, _ = oracle.advance(d, P, 0.0, dt, ("uniform_gravity", [], list(g)))
+        assert err == 0
+        t += dt
+    np.testing.assert_allclose(P[..., 1:4], np.broadcast_to(g * t, P[..., 1:4].shape), rtol=0, atol=1e-14)
+    np.testing.assert_allclose(P[..., 4], 1.5, rtol=0, atol=1e-13)
+    assert np.all(P[..., 0] == 2.0)
+    # point mass at (0.25, 0.5): after one step the gas left of it moves right
+    P = problems.empty(d)
+    P[..., 0] = 1.0
+    P[..., 4] = 1.0
+    err, _ = oracle.advance(d, P, 0.0, 1e-3, ("point_mass", [0.25, 0.5, 0.5], [0.1, 0.01]))
+    x = problems.cell_centres(d, 0)
+    g0 = d.guards[0]
+    assert err == 0 and P[g0, 5, 0, 1] > 0 and P[g0 + 7, 5, 0, 1] < 0 and x[g0] < 0.25 < x[g0 + 7]
