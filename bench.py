@@ -317,7 +317,8 @@ def serving_workload(args):
                                   "30% 2-D 128^2..1024^2, half MHD, PLM + HLLC/HLLE, RK2, CFL 0.4, to t = 0.2, "
                                   "mixed form, Poisson arrivals at 2x the one-at-a-time rate",
                       "zones_total": int(sum(work)), "hbm_pool_gib": args.budget_gib, "chunk_steps": args.chunk,
-                      "graphs": graphs, "launch_shape": shape.as_dict() if shape is not None else None},
+                      "graphs": graphs, "launch_shape": shape.as_dict() if shape is not None else None,
+                      "express_lane_max": args.express},
            "arrival_rate": 2.0 * seq_rate,
            "sequential": {"solves_per_s": seq_rate, "zone_updates_per_s": seq_zone_rate,
                           "sample": f"{len(sample)} requests, every {stride}th of the stream",
@@ -329,7 +330,8 @@ def serving_workload(args):
         serving.set_arrivals(reqs, 2.0 * seq_rate)
         log(f"serving {n_req} requests, Poisson arrivals at {2 * seq_rate:.3f}/s "
             f"(span {reqs[-1].arrival:.0f} s), pool {args.budget_gib:g} GiB")
-        wall, done = serving.serve(reqs, budget, chunk=args.chunk, log=log, graphs=graphs, shape=shape)
+        wall, done = serving.serve(reqs, budget, chunk=args.chunk, log=log, graphs=graphs, shape=shape,
+                                   express=args.express)
         failed = [r.index for r in serving.failures(done)]
         st = serving.latency_stats(done)
         zone_updates = sum(r.zones * r.steps for r in done)
@@ -343,7 +345,8 @@ def serving_workload(args):
         for r in reqs:
             r.arrival = 0.0
         log("saturated run: every request at t = 0")
-        wall_s, done_s = serving.serve(reqs, budget, chunk=args.chunk, log=log, graphs=graphs, shape=shape)
+        wall_s, done_s = serving.serve(reqs, budget, chunk=args.chunk, log=log, graphs=graphs, shape=shape,
+                                       express=args.express)
         out["saturated"] = {"solves_per_s": len(done_s) / wall_s, "wall_s": wall_s,
                             "speedup_vs_sequential": len(done_s) / wall_s / seq_rate,
                             "latency_s": serving.latency_stats(done_s),
@@ -374,6 +377,7 @@ def main():
     ap.add_argument("--saturate", action="store_true", help="serving: also run with every request at t = 0")
     ap.add_argument("--no-open-loop", action="store_true", help="serving: skip the Poisson-arrival run")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--express", type=int, default=0, help="serving: most solves on the batch's express lane")
     ap.add_argument("--no-graphs", action="store_true",
                     help="sod, mhd2d, rmhd3d, serving: launch every kernel from the host instead of replaying HIP graphs")
     args = ap.parse_args()

@@ -280,6 +280,20 @@ int mara_batch_step(mara_batch* batch, int n_steps, int* n_active);
    the same either way.  No counterpart in the reference. */
 int mara_batch_set_graphs(mara_batch* batch, int enable);
 
+/* The most solves the express lane may hold (default 0: every 2-D and 3-D
+   solve stays on the tile lane).  Solves already there stay. */
+int mara_batch_set_express(mara_batch* batch, int max_solves);
+
+/* *lane receives the lane a solve runs on: 0 the tile lane (2-D and 3-D solves), 1 the
+   pencil lane (1-D solves of an RK2 batch, up to 32 of their steps per
+   round on a second stream), 2 the express lane (a 2-D or 3-D solve whose
+   remaining steps exceed 4x the median of the running tile-lane solves and
+   1000, moved at the start of a mara_batch_step and kept there, up to 32 of
+   its steps per round on a third stream; see mara_batch_set_express).  A solve's
+   result does not depend on its lane.  MARA_ERR_INVALID_ARGUMENT for a
+   free slot. */
+int mara_batch_lane(const mara_batch* batch, int slot, int* lane);
+
 /* Launch shape of the batch's pencil tiles (1-D solves) and fused 3-D
    tiles; rows is not used by batches.  Takes effect at the next step. */
 int mara_batch_get_shape(const mara_batch* batch, mara_shape* shape);

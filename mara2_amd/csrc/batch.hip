@@ -21,7 +21,7 @@ using namespace mara::batch;
 
 namespace {
 
-int class_of(mara_batch* B, const mara_desc& d)
+int class_of(mara_batch* B, const mara_desc& d, bool express = false)
 {
     const bool one_d = d.cells[1] == 1 && d.cells[2] == 1;
     const bool rk2 = B->rk_order == 2;
@@ -29,8 +29,9 @@ int class_of(mara_batch* B, const mara_desc& d)
     int fused = 0;
     if (!one_d && rk2 && d.law == MARA_LAW_HYDRO && d.recon == MARA_RECON_PLM) fused = d.cells[2] > 1 ? 2 : 1;
     else if (!one_d && rk2 && d.law == MARA_LAW_MHD && d.cells[2] == 1) fused = 3;
+    const int pencil = (one_d && rk2) ? 1 : 0;
     const ClassKey k{d.law, d.recon, d.riemann, (d.law != MARA_LAW_HYDRO && d.field_ct) ? 1 : 0,
-                     (one_d && rk2) ? 1 : 0, fused};
+                     pencil, fused, pencil ? 1 : (express ? 2 : 0)};
     for (size_t c = 0; c < B->classes.size(); ++c)
         if (!(B->classes[c] < k) && !(k < B->classes[c])) return (int)c;
     B->classes.push_back(k);
@@ -210,6 +211,51 @@ bool finished(const Ctl& c)
     return !c.active || !(c.t < c.t_final) || c.steps >= c.max_steps || ctl_status(c) != MARA_OK;
 }
 
+// Long chains.  Every tile-lane solve takes one step per round, so the solve
+// with the most steps left sets the batch's makespan while the rest drain.
+// A solve whose remaining steps, estimated from its last dt and clock, exceed
+// 4x the median of the running tile-lane solves (and 1000) moves to the
+// express lane, which takes several of its steps per round on a stream of
+// its own; at most express_max at a time (mara_batch_set_express), and a
+// solve stays there until it finishes.  A solve's result does not depend on
+// its lane (its steps are its own).
+
+void promote_long_chains(mara_batch* B)
+{
+    std::vector<std::pair<double, int>> rem;
+    int express = 0;
+    for (int k = 0; k < B->n_items; ++k)
+    {
+        const Slot& sl = B->slots[k];
+        const Ctl& c = B->h_ctl[k];
+        if (!sl.used || sl.fresh || finished(c) || B->classes[sl.cls].pencil) continue;
+        if (sl.express)
+        {
+            ++express;
+            continue;
+        }
+        const double by_clock = c.dt > 0 ? (c.t_final - c.t) / c.dt : 0.0;
+        rem.push_back({std::min(by_clock, (double)(c.max_steps - c.steps)), k});
+    }
+    if (rem.size() < 2 || express >= B->express_max) return;
+    std::vector<std::pair<double, int>> sorted = rem;
+    std::nth_element(sorted.begin(), sorted.begin() + sorted.size() / 2, sorted.end());
+    const double threshold = std::max(1000.0, 4.0 * sorted[sorted.size() / 2].first);
+    std::sort(rem.begin(), rem.end(), [](const std::pair<double, int>& a, const std::pair<double, int>& b) {
+        return a.first > b.first;
+    });
+    for (const auto& e : rem)
+    {
+        if (express >= B->express_max || e.first <= threshold) break;
+        Slot& sl = B->slots[e.second];
+        sl.express = true;
+        sl.cls = class_of(B, sl.d, true);
+        B->items[e.second].lane = B->classes[sl.cls].lane;
+        B->dirty = true;
+        ++express;
+    }
+}
+
 int validate_item(mara_batch* B, const mara_desc* d)
 {
     const int v = validate(d);
@@ -237,8 +283,10 @@ int mara_batch_create(long long hbm_budget, int rk_order, int precision, int max
     auto fail = [&](int code) { mara_batch_destroy(B); return code; };
     if (!hip_ok(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking))) return fail(MARA_ERR_HIP);
     if (!hip_ok(hipStreamCreateWithFlags(&B->stream2, hipStreamNonBlocking))) return fail(MARA_ERR_HIP);
+    if (!hip_ok(hipStreamCreateWithFlags(&B->stream3, hipStreamNonBlocking))) return fail(MARA_ERR_HIP);
     if (!hip_ok(hipEventCreateWithFlags(&B->fork, hipEventDisableTiming))) return fail(MARA_ERR_HIP);
     if (!hip_ok(hipEventCreateWithFlags(&B->join, hipEventDisableTiming))) return fail(MARA_ERR_HIP);
+    if (!hip_ok(hipEventCreateWithFlags(&B->join3, hipEventDisableTiming))) return fail(MARA_ERR_HIP);
     if (!hip_ok(hipMalloc(&B->pool_mem, hbm_budget))) return fail(MARA_ERR_CAPACITY);
     B->pool.init(B->pool_mem, hbm_budget);
     if (!hip_ok(hipMalloc(&B->d_items, max_solves * sizeof(BItem)))) return fail(MARA_ERR_HIP);
@@ -257,6 +305,7 @@ int mara_batch_destroy(mara_batch* B)
     if (!B) return MARA_ERR_INVALID_ARGUMENT;
     if (B->stream) (void)hipStreamSynchronize(B->stream);
     if (B->stream2) (void)hipStreamSynchronize(B->stream2);
+    if (B->stream3) (void)hipStreamSynchronize(B->stream3);
     (void)hipFree(B->pool_mem);
     (void)hipFree(B->d_items);
     (void)hipFree(B->d_ctl);
@@ -266,9 +315,25 @@ int mara_batch_destroy(mara_batch* B)
     if (B->h_ctl) (void)hipHostFree(B->h_ctl);
     if (B->stream) (void)hipStreamDestroy(B->stream);
     if (B->stream2) (void)hipStreamDestroy(B->stream2);
+    if (B->stream3) (void)hipStreamDestroy(B->stream3);
     if (B->fork) (void)hipEventDestroy(B->fork);
     if (B->join) (void)hipEventDestroy(B->join);
+    if (B->join3) (void)hipEventDestroy(B->join3);
     delete B;
+    return MARA_OK;
+}
+
+int mara_batch_set_express(mara_batch* B, int max_solves)
+{
+    if (!B || max_solves < 0) return MARA_ERR_INVALID_ARGUMENT;
+    B->express_max = max_solves;
+    return MARA_OK;
+}
+
+int mara_batch_lane(const mara_batch* B, int slot, int* lane)
+{
+    if (!B || !lane || slot < 0 || slot >= B->max_solves || !B->slots[slot].used) return MARA_ERR_INVALID_ARGUMENT;
+    *lane = B->items[slot].lane;
     return MARA_OK;
 }
 
@@ -343,7 +408,8 @@ int mara_batch_admit(mara_batch* B, const mara_desc* d, const double* P, double 
         if (it.G.N[a] > 1) { it.F[a] = p; p += round_up(it.G.nq * it.G.ffs * real, 256); }
     it.ctl = B->d_ctl + n;
     it.bc = d->bc;
-    it.lane = B->classes[sl.cls].pencil;
+    it.lane = B->classes[sl.cls].lane;
+    sl.express = false;
 
     Ctl& c = B->h_ctl[n];
     std::memset(&c, 0, sizeof c);
@@ -382,6 +448,7 @@ int mara_batch_admit(mara_batch* B, const mara_desc* d, const double* P, double 
 int mara_batch_step(mara_batch* B, int n_steps, int* n_active)
 {
     if (!B || n_steps < 0) return MARA_ERR_INVALID_ARGUMENT;
+    promote_long_chains(B);
     if (B->dirty)
     {
         build_table(B);
@@ -419,30 +486,28 @@ int mara_batch_step(mara_batch* B, int n_steps, int* n_active)
     }
     const int n = B->n_items;
     const unsigned ig = (unsigned)((n + 63) / 64);
-    // Two lanes: tile kernels (2-D/3-D solves, and 1-D ones outside RK2) on
+    // Three lanes: tile kernels (2-D/3-D solves, and 1-D ones outside RK2) on
     // the first stream, pencil kernels of 1-D RK2 solves on the second, so
-    // that small 1-D solves are not held to the pace of large ones.  The
-    // pencil lane takes `ratio` steps per tile-lane step, sized from the
-    // tile lane's estimated step time (its active zones at ~5e9 zone-stages
-    // per second) against the pencil lane's launch-bound ~20 us per step.
-    long long tile_zones = 0;
-    bool any_tile = false, any_pencil = false;
+    // that small 1-D solves are not held to the pace of large ones, and the
+    // express lane (promote_long_chains) on the third.  A round is one
+    // tile-lane step; the other lanes take `ratio` / `ratio3` steps in it,
+    // sized from the tile lane's estimated step time (its active zones at
+    // ~5e9 zone-stages per second, plus ~20 us of launches) against theirs.
+    long long zones[3] = {0, 0, 0};
+    bool busy[3] = {false, false, false};
     for (int k = 0; k < n; ++k)
     {
         if (!B->slots[k].used || finished(B->h_ctl[k])) continue;
-        if (B->items[k].lane) any_pencil = true;
-        else
-        {
-            any_tile = true;
-            tile_zones += (long long)B->items[k].G.n[0] * B->items[k].G.n[1] * B->items[k].G.n[2];
-        }
+        const int l = B->items[k].lane;
+        busy[l] = true;
+        zones[l] += (long long)B->items[k].G.n[0] * B->items[k].G.n[1] * B->items[k].G.n[2];
     }
-    int ratio = 1;
-    if (any_tile && any_pencil)
-    {
-        const double tile_step = (double)tile_zones * B->rk_order / 5e9 + 20e-6;
-        ratio = (int)std::min(32.0, std::max(1.0, tile_step / 20e-6));
-    }
+    const bool any_tile = busy[0], any_pencil = busy[1], any_express = busy[2];
+    auto step_time = [&](int l) { return (double)zones[l] * B->rk_order / 5e9 + 20e-6; };
+    const double round_time = any_tile ? step_time(0) : any_express ? step_time(2) : 20e-6;
+    int ratio = 1, ratio3 = 1;
+    if (any_pencil && (any_tile || any_express)) ratio = (int)std::min(32.0, std::max(1.0, round_time / 20e-6));
+    if (any_express && any_tile) ratio3 = (int)std::min(32.0, std::max(1.0, round_time / step_time(2)));
     const unsigned sg = (unsigned)((n + 3) / 4);  // kb_step_start: 4 items per work-group
     auto run_lane = [&](int lane) {
         kb_step_start<<<sg, 256, 0, B->cur>>>(B->d_items, n, lane);
@@ -454,7 +519,7 @@ int mara_batch_step(mara_batch* B, int n_steps, int* n_active)
         kb_step_end<<<ig, 64, 0, B->cur>>>(B->d_items, n, lane);
     };
     auto run_round = [&]() {
-        if (any_tile || !any_pencil)
+        if (any_tile || (!any_pencil && !any_express))
         {
             B->cur = B->stream;
             run_lane(0);
@@ -464,24 +529,32 @@ int mara_batch_step(mara_batch* B, int n_steps, int* n_active)
             B->cur = B->stream2;
             for (int r = 0; r < ratio; ++r) run_lane(1);
         }
+        if (any_express)
+        {
+            B->cur = B->stream3;
+            for (int r = 0; r < ratio3; ++r) run_lane(2);
+        }
     };
-    const int lanes = (any_tile ? 1 : 0) | (any_pencil ? 2 : 0);
+    const int lanes = (any_tile ? 1 : 0) | (any_pencil ? 2 : 0) | (any_express ? 4 : 0);
     if (B->graphs && n > 0 && n_steps > 1)
     {
         if (!B->round_exec || B->round_version != B->table_version || B->round_n != n || B->round_ratio != ratio
-            || B->round_lanes != lanes)
+            || B->round_ratio3 != ratio3 || B->round_lanes != lanes)
         {
             if (B->round_exec) HIP_TRY(hipGraphExecDestroy(B->round_exec));
             B->round_exec = nullptr;
-            // the round forks the second stream off the first and joins it
-            // back, so one replay ends when both lanes have
+            // the round forks the other streams off the first and joins them
+            // back, so one replay ends when every lane has
             hipGraph_t g = nullptr;
             HIP_TRY(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
             (void)hipEventRecord(B->fork, B->stream);
             (void)hipStreamWaitEvent(B->stream2, B->fork, 0);
+            (void)hipStreamWaitEvent(B->stream3, B->fork, 0);
             run_round();
             (void)hipEventRecord(B->join, B->stream2);
+            (void)hipEventRecord(B->join3, B->stream3);
             (void)hipStreamWaitEvent(B->stream, B->join, 0);
+            (void)hipStreamWaitEvent(B->stream, B->join3, 0);
             const hipError_t launched = hipGetLastError();
             const hipError_t ended = hipStreamEndCapture(B->stream, &g);
             if (!hip_ok(launched) || !hip_ok(ended))
@@ -499,6 +572,7 @@ int mara_batch_step(mara_batch* B, int n_steps, int* n_active)
             B->round_version = B->table_version;
             B->round_n = n;
             B->round_ratio = ratio;
+            B->round_ratio3 = ratio3;
             B->round_lanes = lanes;
         }
         for (int s = 0; s < n_steps; ++s) HIP_TRY(hipGraphLaunch(B->round_exec, B->stream));
@@ -507,9 +581,12 @@ int mara_batch_step(mara_batch* B, int n_steps, int* n_active)
     {
         HIP_TRY(hipEventRecord(B->fork, B->stream));
         HIP_TRY(hipStreamWaitEvent(B->stream2, B->fork, 0));
+        HIP_TRY(hipStreamWaitEvent(B->stream3, B->fork, 0));
         for (int s = 0; s < n_steps && n > 0; ++s) run_round();
         HIP_TRY(hipEventRecord(B->join, B->stream2));
+        HIP_TRY(hipEventRecord(B->join3, B->stream3));
         HIP_TRY(hipStreamWaitEvent(B->stream, B->join, 0));
+        HIP_TRY(hipStreamWaitEvent(B->stream, B->join3, 0));
     }
     HIP_TRY(hipGetLastError());
     if (n) HIP_TRY(hipMemcpyAsync(B->h_ctl, B->d_ctl, n * sizeof(Ctl), hipMemcpyDeviceToHost, B->stream));

@@ -71,6 +71,7 @@ struct Slot
 {
     bool used = false;
     bool fresh = false;
+    bool express = false;  // a long chain moved to the express lane
     mara_desc d;
     int cls = -1;
     long long off = 0, bytes = 0;
@@ -83,10 +84,11 @@ struct ClassKey
     int law, recon, riemann, ct;
     int pencil;  // 1-D items of an RK2 batch: fused pencil tiles
     int fused;   // RK2 batch fused stage tiles: hydro + PLM on 2-D (1) or 3-D (2) meshes, MHD on 2-D (3)
+    int lane;    // 0 tile lane, 1 pencil lane, 2 express lane (long chains)
     bool operator<(const ClassKey& o) const
     {
-        return std::tie(law, recon, riemann, ct, pencil, fused)
-               < std::tie(o.law, o.recon, o.riemann, o.ct, o.pencil, o.fused);
+        return std::tie(law, recon, riemann, ct, pencil, fused, lane)
+               < std::tie(o.law, o.recon, o.riemann, o.ct, o.pencil, o.fused, o.lane);
     }
 };
 
@@ -102,10 +104,12 @@ struct mara_batch
     int max_solves = 0;
     int pencil = mara::PENCIL_DEFAULT;  // faces per pencil tile of the 1-D solves
     int kz = mara::FUSED_KZ;            // z planes per fused tile of the 3-D hydro solves
+    int express_max = 0;                // solves the express lane may hold (0: none)
     hipStream_t stream = nullptr;   // tile kernels, transfers, control
     hipStream_t stream2 = nullptr;  // pencil kernels of 1-D solves
+    hipStream_t stream3 = nullptr;  // express lane: the longest chains of 2-D/3-D solves
     hipStream_t cur = nullptr;      // stream the stage launches go to
-    hipEvent_t fork = nullptr, join = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr, join3 = nullptr;
     mara::batch::Pool pool;
     char* pool_mem = nullptr;
     mara::BItem* d_items = nullptr;
@@ -131,7 +135,7 @@ struct mara_batch
     bool graphs = true;
     hipGraphExec_t round_exec = nullptr;
     long long round_version = -1;
-    int round_n = -1, round_ratio = -1, round_lanes = -1;
+    int round_n = -1, round_ratio = -1, round_ratio3 = -1, round_lanes = -1;
 };
 
 namespace mara {
@@ -245,7 +249,7 @@ void update_dispatch(mara_batch* B, const ClassKey& k, const Segment& s, double 
 }
 
 // The stage's launches for the classes of one lane (0: tile kernels, 1:
-// pencil kernels), on B->cur.
+// pencil kernels, 2: express tile kernels), on B->cur.
 template <typename R>
 void run_stage(mara_batch* B, int rk, int lane)
 {
@@ -256,14 +260,14 @@ void run_stage(mara_batch* B, int rk, int lane)
     const int last = rk == B->rk_order - 1;
     for (int axis = 0; axis < 3; ++axis)
         for (auto& e : B->flux[axis])
-            if (B->classes[e.first].pencil == lane) flux_dispatch<R>(B, B->classes[e.first], e.second, axis, rk);
+            if (B->classes[e.first].lane == lane) flux_dispatch<R>(B, B->classes[e.first], e.second, axis, rk);
     for (auto& e : B->update)
-        if (B->classes[e.first].pencil == lane)
+        if (B->classes[e.first].lane == lane)
             update_dispatch<R>(B, B->classes[e.first], e.second, bb[rk], rk == 0, last, last);
     // the flux-array and pencil updates write ghost zones themselves
     // (write_guard_images); the fused tiles leave them to this pass
     for (auto& e : B->ghost)
-        if (B->classes[e.first].pencil == lane) kb_ghost<<<e.second.count, 256, 0, B->cur>>>(B->d_items, B->d_tiles + e.second.offset, last);
+        if (B->classes[e.first].lane == lane) kb_ghost<<<e.second.count, 256, 0, B->cur>>>(B->d_items, B->d_tiles + e.second.offset, last);
 }
 
 

@@ -75,6 +75,8 @@ def load():
                                     ctypes.POINTER(bat)]
     L.mara_batch_set_graphs.argtypes = [bat, ctypes.c_int]
     L.mara_batch_get_shape.argtypes = [bat, ctypes.POINTER(MaraShape)]
+    L.mara_batch_lane.argtypes = [bat, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    L.mara_batch_set_express.argtypes = [bat, ctypes.c_int]
     L.mara_batch_set_shape.argtypes = [bat, ctypes.POINTER(MaraShape)]
     L.mara_batch_destroy.argtypes = [bat]
     L.mara_batch_item_bytes.argtypes = [ctypes.POINTER(MaraDesc), ctypes.c_int]
@@ -284,6 +286,16 @@ class Batch:
         s = MaraShape()
         _check(self.lib.mara_batch_get_shape(self._h, ctypes.byref(s)), "mara_batch_get_shape")
         return s
+
+    def set_express(self, max_solves):
+        """The most long-chain solves the express lane may hold (0: none)."""
+        _check(self.lib.mara_batch_set_express(self._h, max_solves), "mara_batch_set_express")
+
+    def lane(self, slot):
+        """0 tile lane, 1 pencil lane, 2 express lane."""
+        lane = ctypes.c_int(-1)
+        _check(self.lib.mara_batch_lane(self._h, slot, ctypes.byref(lane)), "mara_batch_lane")
+        return lane.value
 
     def set_shape(self, shape):
         _check(self.lib.mara_batch_set_shape(self._h, ctypes.byref(shape)), "mara_batch_set_shape")

@@ -110,7 +110,7 @@ def sequential(requests, keep_result=False, graphs=True):
 
 
 def serve(requests, hbm_budget, max_solves=1024, chunk=16, rk_order=2, precision="mixed",
-          keep_results=False, log=None, clock=time.perf_counter, graphs=True, shape=None):
+          keep_results=False, log=None, clock=time.perf_counter, graphs=True, shape=None, express=0):
     """Continuous batching over a request stream with arrival times set;
     shape (optional) is the batch's launch shape.  Returns (wall seconds,
     list of requests with admitted/finished stamps)."""
@@ -120,6 +120,7 @@ def serve(requests, hbm_budget, max_solves=1024, chunk=16, rk_order=2, precision
     done = []
     with Batch(hbm_budget, rk_order=rk_order, precision=precision, max_solves=max_solves) as batch:
         batch.set_graphs(graphs)
+        batch.set_express(express)
         if shape is not None:
             batch.set_shape(shape)
         t0 = clock()

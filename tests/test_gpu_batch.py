@@ -236,6 +236,55 @@ def test_two_lanes_with_a_large_tile_solve():
         assert (t, n) == (tw, nw) and np.array_equal(P, Pw), r.index
 
 
+@pytest.mark.parametrize("graphs", [True, False])
+def test_long_chain_moves_to_the_express_lane(graphs):
+    """A 2-D MHD solve with far more steps left (1500) than the large hydro
+    solves beside it (60 each) is promoted to the express lane after its
+    first call and, while their steps are slow, takes several of its own per
+    round; every solve ends bitwise equal to its single-block run."""
+    rng = np.random.default_rng(7)
+    big_p = problems.riemann_params(rng)
+    while not (big_p["law"] == "mhd" and big_p["dim"] == 2):
+        big_p = problems.riemann_params(rng)
+    big_p.update(n=64)
+    big = problems.riemann_desc(big_p, precision="fp64")
+    Pbig = problems.riemann_problem(big_p, big)
+    smalls = []
+    while len(smalls) < 6:
+        prm = problems.riemann_params(rng)
+        if not (prm["law"] == "hydro" and prm["dim"] == 2):
+            continue
+        prm.update(n=256)
+        d = problems.riemann_desc(prm, precision="fp64")
+        smalls.append((d, problems.riemann_problem(prm, d)))
+    with Batch(BUDGET, rk_order=2, precision="fp64", max_solves=16) as b:
+        b.set_graphs(graphs)
+        b.set_express(8)
+        sb = b.admit(big, Pbig, serving.CFL, 1e300, 1500)
+        ss = [b.admit(d, P, serving.CFL, 1e300, 60) for d, P in smalls]
+        got = {}
+        lanes = set()
+        big_steps_when_others_done = None
+        while len(got) < 7:
+            b.step(4)
+            if sb not in got:
+                lanes.add(b.lane(sb))
+            for s in b.finished():
+                done, t, steps, st, rep = b.query(s)
+                got[s] = (b.retire(s), t, steps, st)
+            if big_steps_when_others_done is None and all(s in got for s in ss):
+                big_steps_when_others_done = got[sb][2] if sb in got else b.query(sb)[2]
+    assert 2 in lanes, lanes
+    assert big_steps_when_others_done > 120, big_steps_when_others_done
+    for s, (d, P0, n_max) in [(sb, (big, Pbig, 1500))] + [(s, (d, P, 60)) for s, (d, P) in zip(ss, smalls)]:
+        P, t, n, st = got[s]
+        with Block(d) as blk:
+            blk.upload(P0)
+            tw, nw, rep = blk.run(serving.CFL, 1e300, n_max, check=False)
+            Pw = blk.download()
+        assert (t, n, st) == (tw, nw, rep.status) and np.array_equal(P, Pw), s
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("zplanes", [16, 4])
 def test_fused_3d_solve_in_a_batch(precision, zplanes):
