@@ -1,8 +1,12 @@
 // api.hip -- host side of the C ABI declared in include/mara_gpu.h.
 //
 // Owns the device state of one mesh block and enqueues the per-stage kernels
-// on the block's stream.  Device layout and the stage structure are described
-// in kernels.hpp; the fused hydro stage kernel lives in fused.hpp.
+// on the block's stream (stage.hpp picks them: the fused hydro kernel of
+// fused.hpp, the fused 2-D MHD kernel of fused_mhd.hpp, the 1-D pencils or
+// the flux-array kernels of kernels.hpp), replays device-driven steps from a
+// captured HIP graph, runs the launch-shape autotuner and loads source-term
+// code objects.  Device layout and the stage structure are described in
+// kernels.hpp.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -29,11 +33,6 @@ namespace {
 constexpr int kThreads = 256;
 
 unsigned grid_for(long long n) { return (unsigned)((n + kThreads - 1) / kThreads); }
-
-// 2-D work-groups of 64 x 4 over an (m0, m1, m2) box: one wavefront per x row,
-// z from the grid, so kernels decode indices without integer division.
-dim3 block3() { return dim3(64, 4, 1); }
-dim3 grid3(int m0, int m1, int m2) { return dim3((m0 + 63) / 64, (m1 + 3) / 4, m2); }
 
 #define HIP_TRY(call) do { if (!hip_ok(call)) return MARA_ERR_HIP; } while (0)
 

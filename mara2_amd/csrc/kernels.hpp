@@ -6,10 +6,11 @@
 //   face field q of the face left of cell (i,j,k) along any axis:
 //   f = (k * (N2 + 1) + j) * (N1 + 1) + i, one array per axis.
 //
-// A stage is: flux sweep per fleshed axis -> flux-CT (MHD laws) ->
-// divergence + RK combine + primitive recovery (one kernel) -> ghost fill
-// axis by axis.  The fused hydro stage kernel (fused.hpp) replaces the first
-// three for the hydro law.
+// A stage is: flux sweep per fleshed axis -> divergence with flux-CT (MHD
+// laws) + RK combine + primitive recovery + ghost images (one kernel).  The
+// fused kernels replace both for hydro on 2-D/3-D blocks (fused.hpp, ghost
+// zones by a separate pass) and MHD on 2-D blocks (fused_mhd.hpp); 1-D
+// blocks of two or three stages run each stage as pencil tiles (below).
 #pragma once
 
 #include "physics.hpp"
