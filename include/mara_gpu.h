@@ -159,6 +159,7 @@ int mara_block_run(mara_block* block, double cfl, double t_final, int max_steps,
    flux-array path while a source is set (its fused update has no source
    term).  MARA_ERR_INVALID_ARGUMENT when the file or the kernel cannot be
    loaded.  Batches take no sources. */
+#define MARA_SOURCE_MAX_PARAMS 16
 int mara_block_set_source(mara_block* block, const char* code_object, const char* kernel, const double* params,
                           int n_params);
 /* The 8 auxiliary values passed to the source function (the reference

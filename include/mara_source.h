@@ -25,7 +25,7 @@
 
 #include <hip/hip_runtime.h>
 
-#define MARA_SOURCE_MAX_PARAMS 16
+#include "mara_gpu.h" /* MARA_SOURCE_MAX_PARAMS */
 
 /* The launch record the block passes to a source kernel (by value). */
 typedef struct mara_source_args

@@ -335,8 +335,45 @@ mara_block* GpuMethodOfLinesTVD::blockFor(const MeshData& md) const
         check(mara_block_create(&d, &block), "mara_block_create");
         blockDesc = d;
         blockOwner = &md;
+        applySource(block);
     }
     return block;
+}
+
+void GpuMethodOfLinesTVD::applySource(mara_block* b) const
+{
+    if (sourcePath.empty())
+        check(mara_block_set_source(b, nullptr, nullptr, nullptr, 0), "mara_block_set_source");
+    else
+        check(mara_block_set_source(b, sourcePath.c_str(), sourceKernel.c_str(), sourceParams.data(),
+                                    (int)sourceParams.size()),
+              "mara_block_set_source");
+    check(mara_block_set_source_aux(b, sourceAux.data()), "mara_block_set_source_aux");
+}
+
+void GpuMethodOfLinesTVD::setSourceTermsCodeObject(const std::string& path, const std::string& kernel,
+                                                   std::vector<double> params)
+{
+    if (params.size() > MARA_SOURCE_MAX_PARAMS) throw std::logic_error("at most 16 source-term parameters");
+    sourcePath = path;
+    sourceKernel = kernel;
+    sourceParams = std::move(params);
+    if (block) applySource(block);
+}
+
+void GpuMethodOfLinesTVD::setStarParticleData(const std::array<double, 8>& t)
+{
+    sourceAux = t;
+    if (block) check(mara_block_set_source_aux(block, sourceAux.data()), "mara_block_set_source_aux");
+}
+
+mara_shape GpuMethodOfLinesTVD::autotune(const MeshData& md, int repeats) const
+{
+    mara_block* b = blockFor(md);
+    if (!resident || md.deviceVersion == 0) check(mara_block_upload_prim(b, md.P.data()), "mara_block_upload_prim");
+    mara_shape s{};
+    check(mara_block_autotune(b, repeats, &s, nullptr), "mara_block_autotune");
+    return s;
 }
 
 void GpuMethodOfLinesTVD::syncToDevice(MeshData& md) const

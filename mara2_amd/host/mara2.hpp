@@ -318,9 +318,24 @@ public:
     int run(MeshData& solution, double cfl, double tFinal, int maxSteps, double& t) const;
     mara_desc describe(const MeshData& solution) const;
     const mara_report& lastReport() const { return report; }
+    // setSourceTermsFunction (src/SolutionSchemes.hpp:36) on the GPU: the
+    // per-zone function compiled into a gfx950 code object written with
+    // include/mara_source.h, its kernel name and up to 16 parameters; an
+    // empty path removes it.  setStarParticleData stands for the 8 auxiliary
+    // values the reference passes to the function (its star-particle data).
+    void setSourceTermsCodeObject(const std::string& path, const std::string& kernel,
+                                  std::vector<double> params = {});
+    void setStarParticleData(const std::array<double, 8>& t);
+    // launch-shape autotuner on the block of md (mara_block_autotune); the
+    // state is left as it is
+    mara_shape autotune(const MeshData& md, int repeats = 5) const;
 private:
     mara_block* blockFor(const MeshData& md) const;
+    void applySource(mara_block* b) const;
     void check(int status, const char* where) const;
+    std::string sourcePath, sourceKernel;
+    std::vector<double> sourceParams;
+    std::array<double, 8> sourceAux{};
     int rungeKuttaOrder = 1;
     bool disableFieldCT = false;
     int precision = MARA_PREC_FP64;

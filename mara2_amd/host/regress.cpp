@@ -1,8 +1,11 @@
 // regress.cpp -- the reference's regression programs (src/Problems.cpp:35-671)
 // written against the host layer, running every step on the GPU.
 //
-//   mara_regress <program> <problem> <scheme> <out.bin> [max_steps]
+//   mara_regress <program> <problem> <scheme> <out.bin> [max_steps
+//                [source.hsaco kernel [parameters...]]]
 //   program: regress-1d | regress-rel | regress-2d | regress-mhd | sod
+//   The optional source is a source-term code object (include/mara_source.h)
+//   set with GpuMethodOfLinesTVD::setSourceTermsCodeObject.
 //
 // Writes the initial primitives (after the ghost fill), the final ones and
 // the loop's clock to out.bin, so a test can replay the same initial data on
@@ -15,6 +18,7 @@
 #include <iostream>
 #include <map>
 #include <string>
+#include <vector>
 
 #include "mara2.hpp"
 
@@ -85,7 +89,8 @@ std::map<std::string, Problem> problemsMhd()
 
 int usage()
 {
-    std::cerr << "usage: mara_regress <regress-1d|regress-rel|regress-2d|regress-mhd|sod> <problem> <scheme> <out.bin> [max_steps]\n";
+    std::cerr << "usage: mara_regress <regress-1d|regress-rel|regress-2d|regress-mhd|sod> <problem> <scheme> <out.bin>"
+                 " [max_steps [source.hsaco kernel [parameters...]]]\n";
     return 2;
 }
 
@@ -96,6 +101,15 @@ int main(int argc, const char* argv[])
     if (argc < 5) return usage();
     const std::string program = argv[1], pname = argv[2], sname = argv[3], out = argv[4];
     const int maxSteps = argc > 5 ? std::atoi(argv[5]) : 1 << 30;
+    // optional source terms: <code object> <kernel> [parameters...]
+    std::string srcPath, srcKernel;
+    std::vector<double> srcParams;
+    if (argc > 7)
+    {
+        srcPath = argv[6];
+        srcKernel = argv[7];
+        for (int a = 8; a < argc; ++a) srcParams.push_back(std::atof(argv[a]));
+    }
 
     // scheme: pcmN / plmN with N the Runge-Kutta order (src/Problems.cpp:102-135)
     if (sname.size() != 4) return usage();
@@ -166,6 +180,7 @@ int main(int argc, const char* argv[])
     ss->setMeshOperator(mo);
     ss->setBoundaryCondition(problem.bc);
     ss->setIntercellFluxScheme(fs);
+    if (!srcPath.empty()) ss->setSourceTermsCodeObject(srcPath, srcKernel, srcParams);
     problem.bc->setConservationLaw(cl);
 
     SimulationStatus status;

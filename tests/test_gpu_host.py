@@ -55,3 +55,24 @@ def test_regression_program_matches_oracle(tmp_path, program, problem, scheme, k
         assert np.sum(np.abs(Pg - Po)) / np.sum(np.abs(Po)) < 1e-12
     else:
         assert np.array_equal(Pg, Po)
+
+
+def test_regression_program_with_source_terms_matches_oracle(tmp_path):
+    """GpuMethodOfLinesTVD::setSourceTermsCodeObject through the C++ layer:
+    the 2-D shock tube under uniform gravity (the uniform_gravity code
+    object) against the oracle's source path, bit for bit."""
+    out = tmp_path / "out.bin"
+    src = os.path.join(ROOT, "mara2_amd", "sources", "uniform_gravity.hsaco")
+    g = (0.5, -1.0, 0.0)
+    r = subprocess.run([EXE, "regress-2d", "Shocktube2D", "plm2", str(out), "1000000", src,
+                        "mara_source_uniform_gravity"] + [repr(x) for x in g],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    steps, t, P0, Pg = _read(out)
+    desc = abi.make_desc(law="hydro", recon="plm", riemann="hlle", bc="outflow", cells=(128, 128, 1),
+                         guards=(2, 2, 0), rk_order=2)
+    Po = P0.copy()
+    err, to, so = oracle.run(desc, Po, 0.5, 0.1, source=("uniform_gravity", [], list(g)))
+    assert err == 0
+    assert so == steps and to == t
+    assert np.array_equal(Pg, Po)
