@@ -302,13 +302,19 @@ def serving_workload(args):
     budget = int(args.budget_gib * 2**30)
     shape = None
     if not args.no_autotune:
-        # the batch's pencil length, tuned on the stream's median 1-D request
-        ones = sorted((r for r in reqs if r.params["dim"] == 1), key=lambda r: r.zones)
-        rep = ones[len(ones) // 2]
-        with Block(rep.desc) as blk:
-            blk.upload(rep.P)
-            shape, _ = blk.autotune(repeats=20)
+        # the batch's pencil length, tuned on the stream's median 1-D request,
+        # and its 2-D MHD tile height, on the median 2-D MHD request
+        def median_tune(keep):
+            sel = sorted((r for r in reqs if keep(r.params)), key=lambda r: r.zones)
+            rep = sel[len(sel) // 2]
+            with Block(rep.desc) as blk:
+                blk.upload(rep.P)
+                return blk.autotune(repeats=20)[0], rep
+        shape, rep = median_tune(lambda p: p["dim"] == 1)
         log(f"pencil length {shape.pencil} (tuned on request {rep.index}, {rep.zones} zones)")
+        tiles, rep = median_tune(lambda p: p["dim"] == 2 and p["law"] == "mhd")
+        shape.rows = tiles.rows
+        log(f"2-D MHD tile rows {shape.rows} (tuned on request {rep.index}, {rep.zones} zones)")
     out = {"metric": METRIC, "value": None, "unit": "solves/s", "n_gpus": 1, "steps": None,
            "warmup": None, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f32/f64 mixed",

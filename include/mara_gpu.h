@@ -196,24 +196,28 @@ typedef struct mara_shape
                     (the relativistic law runs 256 whatever is set) */
     int zplanes; /* fused 3-D hydro stages: z planes a work-group marches,
                     4, 8, 16 or 32 */
-    int rows;    /* flux-array stages: work-group rows of 64 zones, 1, 2 or 4 */
+    int rows;    /* flux-array stages: work-group rows of 64 zones; fused
+                    2-D MHD stages: tiles of 64 x (4 * rows) zones; 1, 2 or 4 */
 } mara_shape;
 
 /* The block's launch shape.  A new block starts from the tuning table's
-   entry for its descriptor, else from pencil 256, zplanes 16, rows 4.
+   entry for its descriptor, else from pencil 256, zplanes 16 and rows 4
+   (rows 2 for a fused 2-D MHD block).
    set: MARA_ERR_INVALID_ARGUMENT for a value outside the lists above. */
 int mara_block_get_shape(const mara_block* block, mara_shape* shape);
 int mara_block_set_shape(mara_block* block, const mara_shape* shape);
 
 /* Autotuner: times the candidates of the shape the block's stages use
    (pencil for 1-D blocks of two or three stages, zplanes for fused 3-D
-   hydro blocks, rows for the flux-array path; nothing to choose for the
-   fused 2-D hydro and MHD blocks), `repeats` launches of a first RK stage each after one
-   untimed launch, on the block's own state with dt = 0 and a scratch
-   control block: the state, clock and counters are untouched.  Sets the
-   fastest candidate on the block and records it in the process's tuning
-   table under the block's descriptor.  *chosen (optional) receives the
-   shape, *best_ms (optional) the fastest mean stage time in ms. */
+   hydro blocks, rows for the flux-array path and the fused 2-D MHD tiles;
+   nothing to choose for the fused 2-D hydro blocks): `repeats` launches of
+   a first RK stage per candidate after one untimed launch, candidates in
+   turn over three rounds, each candidate's fastest round counting; on the
+   block's own state with dt = 0 and a scratch control block, so the state,
+   clock and counters are untouched.  Sets the fastest candidate on the
+   block and records it in the process's tuning table under the block's
+   descriptor.  *chosen (optional) receives the shape, *best_ms (optional)
+   the fastest mean stage time in ms. */
 int mara_block_autotune(mara_block* block, int repeats, mara_shape* chosen, double* best_ms);
 
 /* The process's tuning table: look up the shape recorded for a descriptor
@@ -295,8 +299,9 @@ int mara_batch_set_express(mara_batch* batch, int max_solves);
    free slot. */
 int mara_batch_lane(const mara_batch* batch, int slot, int* lane);
 
-/* Launch shape of the batch's pencil tiles (1-D solves) and fused 3-D
-   tiles; rows is not used by batches.  Takes effect at the next step. */
+/* Launch shape of the batch's pencil tiles (1-D solves), fused 3-D tiles
+   (zplanes) and fused 2-D MHD tiles (rows; 2 at create).  Takes effect at
+   the next step. */
 int mara_batch_get_shape(const mara_batch* batch, mara_shape* shape);
 int mara_batch_set_shape(mara_batch* batch, const mara_shape* shape);
 

@@ -57,7 +57,7 @@ class MaraDesc(ctypes.Structure):
 
 class MaraShape(ctypes.Structure):
     """mara_shape: launch shape of the stage kernels (pencil length, z planes
-    per fused work-group, flux-array work-group rows)."""
+    per fused work-group, flux-array work-group rows or 2-D MHD tile height)."""
     _fields_ = [
         ("pencil", ctypes.c_int),
         ("zplanes", ctypes.c_int),

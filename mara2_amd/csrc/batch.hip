@@ -85,7 +85,7 @@ void build_table(mara_batch* B)
         {
             // 2-D MHD: one 64 x TY tile per work-group; the kernel writes the
             // ghost images itself
-            const int TY = B->precision == MARA_PREC_MIXED ? MhdTile<float>::TY : MhdTile<double>::TY;
+            const int TY = mhd_tile_height(B->rows);
             const int s0 = seg_begin();
             for (int n = 0; n < B->n_items; ++n)
             {
@@ -279,6 +279,7 @@ int mara_batch_create(long long hbm_budget, int rk_order, int precision, int max
     if (!B) return MARA_ERR_HIP;
     B->rk_order = rk_order;
     B->precision = precision;
+    B->rows = MHD_ROWS_DEFAULT;
     B->max_solves = max_solves;
     auto fail = [&](int code) { mara_batch_destroy(B); return code; };
     if (!hip_ok(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking))) return fail(MARA_ERR_HIP);
@@ -342,16 +343,18 @@ int mara_batch_get_shape(const mara_batch* B, mara_shape* shape)
     if (!B || !shape) return MARA_ERR_INVALID_ARGUMENT;
     shape->pencil = B->pencil;
     shape->zplanes = B->kz;
-    shape->rows = ROWS_DEFAULT;
+    shape->rows = B->rows;
     return MARA_OK;
 }
 
 int mara_batch_set_shape(mara_batch* B, const mara_shape* shape)
 {
     if (!B || !shape || !valid_shape(*shape)) return MARA_ERR_INVALID_ARGUMENT;
-    if (shape->pencil != B->pencil || shape->zplanes != B->kz) B->dirty = true;  // tiles change
+    if (shape->pencil != B->pencil || shape->zplanes != B->kz || shape->rows != B->rows)
+        B->dirty = true;  // tiles change
     B->pencil = shape->pencil;
     B->kz = shape->zplanes;
+    B->rows = shape->rows;
     return MARA_OK;
 }
 

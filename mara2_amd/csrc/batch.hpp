@@ -121,8 +121,8 @@ kb_fused_plm(const BItem* __restrict__ items, const BTile* __restrict__ tiles, d
 
 // 2-D MHD items of an RK2 batch: the single block's fused MHD tiles (fluxes,
 // flux-CT and update in one pass, ghost images included), tile (x0, y0).
-template <int Recon, typename R, bool CT>
-static __global__ void __launch_bounds__(64 * MhdTile<R>::TY)
+template <int Recon, typename R, bool CT, int TY>
+static __global__ void __launch_bounds__(64 * TY)
 kb_mhd2d(const BItem* __restrict__ items, const BTile* __restrict__ tiles, double b, int first, int last,
          int courant_out)
 {
@@ -132,7 +132,7 @@ kb_mhd2d(const BItem* __restrict__ items, const BTile* __restrict__ tiles, doubl
     const Geom G = it.G;
     const double* Pin = first ? it.Pa : it.Pb;
     double* Pout = last ? it.Pa : it.Pb;
-    mhd2d_tile<Recon, R, CT>(Pin, it.Pa, it.U, Pout, G, b, first, !last, courant_out, it.ctl, T.x0, T.y0, nullptr);
+    mhd2d_tile<Recon, R, CT, TY>(Pin, it.Pa, it.U, Pout, G, b, first, !last, courant_out, it.ctl, T.x0, T.y0, nullptr);
 }
 
 // Ghost zones of the fused items (their kernel leaves them): tile x0 is the

@@ -7,6 +7,7 @@
 
 #include <map>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/mara_gpu.h"
@@ -104,6 +105,7 @@ struct mara_batch
     int max_solves = 0;
     int pencil = mara::PENCIL_DEFAULT;  // faces per pencil tile of the 1-D solves
     int kz = mara::FUSED_KZ;            // z planes per fused tile of the 3-D hydro solves
+    int rows = mara::MHD_ROWS_DEFAULT;  // 2-D MHD tiles of 64 x (4 * rows) zones
     int express_max = 0;                // solves the express lane may hold (0: none)
     hipStream_t stream = nullptr;   // tile kernels, transfers, control
     hipStream_t stream2 = nullptr;  // pencil kernels of 1-D solves
@@ -206,9 +208,18 @@ template <int Recon, typename R>
 void mhd2d_launch(mara_batch* B, const ClassKey& k, const BTile* t, int count, double b, int first, int last,
                   int cout)
 {
-    const dim3 block(64, MhdTile<R>::TY);
-    if (k.ct) kb_mhd2d<Recon, R, true><<<count, block, 0, B->cur>>>(B->d_items, t, b, first, last, cout);
-    else kb_mhd2d<Recon, R, false><<<count, block, 0, B->cur>>>(B->d_items, t, b, first, last, cout);
+    auto go = [&](auto ty) {
+        constexpr int TY = decltype(ty)::value;
+        const dim3 block(64, TY);
+        if (k.ct) kb_mhd2d<Recon, R, true, TY><<<count, block, 0, B->cur>>>(B->d_items, t, b, first, last, cout);
+        else kb_mhd2d<Recon, R, false, TY><<<count, block, 0, B->cur>>>(B->d_items, t, b, first, last, cout);
+    };
+    switch (B->rows)
+    {
+        case 1: go(std::integral_constant<int, 4>{}); break;
+        case 2: go(std::integral_constant<int, 8>{}); break;
+        default: go(std::integral_constant<int, 16>{}); break;
+    }
 }
 
 template <typename R>

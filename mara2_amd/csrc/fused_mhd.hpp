@@ -3,13 +3,15 @@
 // recovery and ghost images in one pass, without flux arrays in HBM.
 //
 // A work-group owns a tile of 64 x TY interior zones (x fastest, one
-// wavefront per row).  Phase 1 computes into LDS
+// wavefront per row; TY = 4, 8 or 16, a launch shape).  Phase 1 computes
+// into LDS
 //   - the x-face fluxes of the tile's rows and of one row below and above
 //     (faces i0 .. i0 + 64), and
 //   - the y-face fluxes of the tile's columns and of one column left and
 //     right (faces j0 .. j0 + TY),
 // each from its stencil read from the stage input (face_flux_at, the
-// arithmetic of the flux-array sweep).  The extra row and column are the
+// arithmetic of the flux-array sweep), as one list of faces dealt out to
+// the work-group's lanes.  The extra row and column are the
 // transverse neighbours flux-CT reads (Toth 2000 eq. 25,
 // CellCenteredFieldCT::correctGodunovFluxes, src/CellCenteredFieldCT.cpp:14-134).
 // Phase 2 forms each zone's divergence from LDS in the order of update_zone
@@ -24,20 +26,22 @@
 
 namespace mara {
 
-// Tile height: binary64 tiles 64 x 4 (LDS 46 KB), binary32 64 x 8 (40 KB).
-template <typename R> struct MhdTile
-{
-    static constexpr int TY = sizeof(R) == 8 ? 4 : 8;
-};
+// Tile height TY = 4 * rows for the launch shape's rows (1, 2 or 4).  LDS per
+// work-group: binary64 46 / 80 / 147 KB, binary32 23 / 40 / 73 KB.  64 x 8
+// tiles are the default in both forms: at 2048^2 their stage takes 0.51 ms
+// (binary64) and 0.33 ms (binary32) against 0.60 / 0.35 ms for 64 x 4 and
+// 0.54 / 0.36 ms for 64 x 16 (one work-group per CU).
+constexpr int MHD_ROWS_DEFAULT = 2;
+inline constexpr int mhd_tile_height(int rows) { return 4 * rows; }
 
-template <int Recon, typename R, bool CT>
+template <int Recon, typename R, bool CT, int TY>
 __device__ __forceinline__ void mhd2d_tile(const double* __restrict__ Pin, const double* __restrict__ P0,
                                            double* __restrict__ U, double* __restrict__ Pout, const Geom& G, double b,
                                            int first_stage, int write_u, int courant_out, Ctl* ctl, int tx, int ty,
                                            const double* __restrict__ Ssrc)
 {
     constexpr int NQ = 8;
-    constexpr int TX = 64, TY = MhdTile<R>::TY, NT = TX * TY;
+    constexpr int TX = 64, NT = TX * TY;
     constexpr int FXC = TX + 1;  // x-faces per row
     constexpr int FXR = TY + 2;  // rows j0 - 1 .. j0 + TY
     constexpr int FYC = TX + 2;  // columns i0 - 1 .. i0 + 64
@@ -57,27 +61,33 @@ __device__ __forceinline__ void mhd2d_tile(const double* __restrict__ Pin, const
     const int xr_lo = CT ? 0 : 1;
     const int xr_hi = min(CT ? FXR : FXR - 1, (CT ? je + 1 : je) - (j0 - 1));  // exclusive
     const int xc_hi = min(FXC, ie + 1 - i0);
-    for (int t = tid; t < FXR * FXC; t += NT)
-    {
-        const int c = t % FXC, r = t / FXC;
-        if (r < xr_lo || r >= xr_hi || c >= xc_hi) continue;
-        R Fq[NQ];
-        face_flux_at<1, Recon, 0, R>(Pin, G, 0, i0 + c, j0 - 1 + r, k, Fq, ctl);
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) fx[q][r][c] = Fq[q];
-    }
     // y-faces (c, r): face j0 + r of column i0 - 1 + c
     const int yc_lo = CT ? 0 : 1;
     const int yc_hi = min(CT ? FYC : FYC - 1, (CT ? ie + 1 : ie) - (i0 - 1));
     const int yr_hi = min(FYR, je + 1 - j0);
-    for (int t = tid; t < FYR * FYC; t += NT)
+    // the faces needed, x-faces first, row by row, dealt out to the lanes as
+    // one list; every extent is >= 2 since the tile holds an interior zone.
+    // Each kind keeps its own call, so its stencil stride stays a constant.
+    const int nxw = xc_hi, nx = (xr_hi - xr_lo) * nxw;
+    const int nyw = yc_hi - yc_lo, nf = nx + yr_hi * nyw;
+    for (int t = tid; t < nf; t += NT)
     {
-        const int c = t % FYC, r = t / FYC;
-        if (c < yc_lo || c >= yc_hi || r >= yr_hi) continue;
+        const bool xf = t < nx;
+        const int u = xf ? t : t - nx, w = xf ? nxw : nyw;
+        const int r = u / w, c = u - r * w;
         R Fq[NQ];
-        face_flux_at<1, Recon, 0, R>(Pin, G, 1, i0 - 1 + c, j0 + r, k, Fq, ctl);
+        if (xf)
+        {
+            face_flux_at<1, Recon, 0, R>(Pin, G, 0, i0 + c, j0 - 1 + xr_lo + r, k, Fq, ctl);
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) fy[q][r][c] = Fq[q];
+            for (int q = 0; q < NQ; ++q) fx[q][xr_lo + r][c] = Fq[q];
+        }
+        else
+        {
+            face_flux_at<1, Recon, 0, R>(Pin, G, 1, i0 - 1 + yc_lo + c, j0 + r, k, Fq, ctl);
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) fy[q][r][yc_lo + c] = Fq[q];
+        }
     }
     __syncthreads();
 
@@ -123,18 +133,17 @@ __device__ __forceinline__ void mhd2d_tile(const double* __restrict__ Pin, const
     if (courant_out) courant_to_bins(dtc, wmin, NT / 64, tid, ctl);
 }
 
-template <int Recon, typename R, bool CT>
-static __global__ void __launch_bounds__(64 * MhdTile<R>::TY)
+template <int Recon, typename R, bool CT, int TY>
+static __global__ void __launch_bounds__(64 * TY)
 k_mhd2d(const double* __restrict__ Pin, const double* __restrict__ P0, double* __restrict__ U,
         double* __restrict__ Pout, Geom G, double b, int first_stage, int write_u, int courant_out, Ctl* ctl,
         const double* __restrict__ Ssrc)
 {
     if (!ctl->active) return;
-    constexpr int TY = MhdTile<R>::TY;
     const int ntx = (G.n[0] + 63) / 64, nty = (G.n[1] + TY - 1) / TY;
     const int tile = xcd_tile(blockIdx.x, ntx * nty);
-    mhd2d_tile<Recon, R, CT>(Pin, P0, U, Pout, G, b, first_stage, write_u, courant_out, ctl, tile % ntx, tile / ntx,
-                             Ssrc);
+    mhd2d_tile<Recon, R, CT, TY>(Pin, P0, U, Pout, G, b, first_stage, write_u, courant_out, ctl, tile % ntx,
+                                 tile / ntx, Ssrc);
 }
 
 // 2-D MHD blocks of two or three stages (the z axis flat, x and y fleshed):
@@ -145,27 +154,45 @@ inline bool mhd2d_supported(const mara_desc* d)
            && d->guards[1] > 0 && d->rk_order >= 2;
 }
 
-template <int Recon, typename R>
-inline void launch_mhd2d_t(const Geom& G, bool ct, hipStream_t s, const double* Pin, const double* P0, double* U,
-                           double* Pout, double b, int first, int write_u, int cout, Ctl* ctl, const double* Ssrc)
+template <int Recon, typename R, int TY>
+inline void launch_mhd2d_ty(const Geom& G, bool ct, hipStream_t s, const double* Pin, const double* P0, double* U,
+                            double* Pout, double b, int first, int write_u, int cout, Ctl* ctl, const double* Ssrc)
 {
-    constexpr int TY = MhdTile<R>::TY;
     const dim3 block(64, TY, 1);
     const dim3 grid(((G.n[0] + 63) / 64) * ((G.n[1] + TY - 1) / TY));
-    if (ct) k_mhd2d<Recon, R, true><<<grid, block, 0, s>>>(Pin, P0, U, Pout, G, b, first, write_u, cout, ctl, Ssrc);
-    else k_mhd2d<Recon, R, false><<<grid, block, 0, s>>>(Pin, P0, U, Pout, G, b, first, write_u, cout, ctl, Ssrc);
+    if (ct)
+        k_mhd2d<Recon, R, true, TY><<<grid, block, 0, s>>>(Pin, P0, U, Pout, G, b, first, write_u, cout, ctl, Ssrc);
+    else
+        k_mhd2d<Recon, R, false, TY><<<grid, block, 0, s>>>(Pin, P0, U, Pout, G, b, first, write_u, cout, ctl, Ssrc);
+}
+
+template <int Recon, typename R>
+inline void launch_mhd2d_t(const Geom& G, bool ct, int rows, hipStream_t s, const double* Pin, const double* P0,
+                           double* U, double* Pout, double b, int first, int write_u, int cout, Ctl* ctl,
+                           const double* Ssrc)
+{
+    switch (rows)
+    {
+        case 1: launch_mhd2d_ty<Recon, R, 4>(G, ct, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, Ssrc); break;
+        case 2: launch_mhd2d_ty<Recon, R, 8>(G, ct, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, Ssrc); break;
+        default: launch_mhd2d_ty<Recon, R, 16>(G, ct, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, Ssrc); break;
+    }
 }
 
 template <typename R>
-inline void launch_mhd2d(const mara_desc& d, const Geom& G, bool ct, hipStream_t s, const double* Pin,
+inline void launch_mhd2d(const mara_desc& d, const Geom& G, bool ct, int rows, hipStream_t s, const double* Pin,
                          const double* P0, double* U, double* Pout, double b, int first, int write_u, int cout,
                          Ctl* ctl, const double* Ssrc)
 {
     switch (d.recon)
     {
-        case MARA_RECON_PCM: launch_mhd2d_t<0, R>(G, ct, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, Ssrc); break;
-        case MARA_RECON_PLM: launch_mhd2d_t<1, R>(G, ct, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, Ssrc); break;
-        default: launch_mhd2d_t<2, R>(G, ct, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, Ssrc); break;
+        case MARA_RECON_PCM:
+            launch_mhd2d_t<0, R>(G, ct, rows, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, Ssrc);
+            break;
+        case MARA_RECON_PLM:
+            launch_mhd2d_t<1, R>(G, ct, rows, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, Ssrc);
+            break;
+        default: launch_mhd2d_t<2, R>(G, ct, rows, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, Ssrc); break;
     }
 }
 

@@ -32,7 +32,8 @@ struct StageArgs
     int pencil;  // 1-D pencil tiles of this length, or 0
     bool mhd2d;  // 2-D MHD: fused flux + CT + update tiles
     int kz;      // fused 3-D kernels: z planes per work-group
-    int rows;    // flux-array kernels: work-group rows of 64 zones
+    int rows;    // flux-array kernels: work-group rows of 64 zones; fused 2-D
+                 // MHD tiles: 64 x (4 * rows) zones
     const double* S;  // source terms added to L (same layout as P), or null
 };
 
@@ -170,7 +171,7 @@ void stage_kernels(const StageArgs& s)
     }
     if (s.mhd2d)
     {
-        launch_mhd2d<R>(*s.d, *s.G, s.ct, s.stream, s.Pin, s.P0, s.U, s.Pout, s.b, s.first, s.write_u, s.cout, s.ctl,
+        launch_mhd2d<R>(*s.d, *s.G, s.ct, s.rows, s.stream, s.Pin, s.P0, s.U, s.Pout, s.b, s.first, s.write_u, s.cout, s.ctl,
                         s.S);
         return;
     }

@@ -36,7 +36,7 @@ def _run_batch(reqs, precision, chunk=16, max_solves=64, graphs=True, shape=None
         b.set_graphs(graphs)
         if shape is not None:
             b.set_shape(shape)
-            assert b.get_shape().pencil == shape.pencil
+            assert b.get_shape().as_dict() == shape.as_dict()
         slots = {}
         for r in reqs:
             s = b.admit(r.desc, r.P, serving.CFL, serving.T_FINAL, serving.MAX_STEPS)
@@ -134,9 +134,10 @@ def test_graph_rounds_match_host_launches(fp64_stream, fp64_mix, precision):
         assert np.array_equal(P, ref[i][0]), i
 
 
-@pytest.mark.parametrize("pencil", [128, 512])
-def test_pencil_length_does_not_change_results(fp64_stream, fp64_mix, pencil):
-    res = _run_batch(fp64_stream, "fp64", shape=abi.make_shape(pencil=pencil, zplanes=8))
+@pytest.mark.parametrize("pencil,rows", [(128, 1), (512, 4)])
+def test_launch_shape_does_not_change_results(fp64_stream, fp64_mix, pencil, rows):
+    """Pencil length, fused 3-D z planes and 2-D MHD tile height of a batch."""
+    res = _run_batch(fp64_stream, "fp64", shape=abi.make_shape(pencil=pencil, zplanes=8, rows=rows))
     for i, (P, t, steps) in res.items():
         assert (t, steps) == fp64_mix[i][1:], i
         assert np.array_equal(P, fp64_mix[i][0]), i

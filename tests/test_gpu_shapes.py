@@ -1,6 +1,6 @@
 """Launch shapes and the autotuner.  A shape (pencil length of 1-D stages,
 z planes per work-group of the fused 3-D kernels, work-group rows of the
-flux-array kernels) only redistributes zones among work-groups, so every
+flux-array kernels and tile height of the fused 2-D MHD kernel) only redistributes zones among work-groups, so every
 shape must give the default shape's result bit for bit; the autotuner must
 leave the state and clock untouched and record its choice for blocks
 created later with the same descriptor."""
@@ -48,6 +48,14 @@ def _case(name):
     if name == "mhd2d_rows":
         d = problems.mhd_desc(48)
         return d, problems.mhd_2d(d), "rows", abi.ROW_SHAPES
+    if name == "mhd2d_rows_ragged_mixed":
+        d = abi.make_desc(law="mhd", recon="plm", riemann="hlle", bc="outflow", cells=(70, 37, 1), field_ct=1,
+                          precision="mixed")
+        return d, _random_state(d, 43), "rows", abi.ROW_SHAPES
+    if name == "mhd2d_rows_weno_noct":
+        d = abi.make_desc(law="mhd", recon="weno5", riemann="hlle", bc="periodic", cells=(130, 21, 1), field_ct=0,
+                          rk_order=3)
+        return d, _random_state(d, 44), "rows", abi.ROW_SHAPES
     raise KeyError(name)
 
 
@@ -62,7 +70,8 @@ def _run(desc, P0, shape, steps=6, cfl=0.3):
 
 
 @pytest.mark.parametrize("name", ["sod_pencil", "sod_pencil_mixed_rk3", "mhd1d_pencil_ct", "turbulence3d_zplanes",
-                                  "turbulence3d_zplanes_mixed", "weno3d_zplanes", "mhd2d_rows"])
+                                  "turbulence3d_zplanes_mixed", "weno3d_zplanes", "mhd2d_rows",
+                                  "mhd2d_rows_ragged_mixed", "mhd2d_rows_weno_noct"])
 def test_every_shape_gives_the_default_result(name):
     desc, P0, knob, values = _case(name)
     t0, n0, ref = _run(desc, P0, None)
