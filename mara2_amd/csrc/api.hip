@@ -268,13 +268,21 @@ void enqueue_step(mara_block* b)
     k_ctl_step_end<<<1, 1, 0, b->stream>>>(b->ctl);
 }
 
-// Capture kGraphSteps steps of the block's stream into an executable graph.
-// Nothing in a step is a memset or a host transfer.
+// Capture kGraphSteps steps of the block's stream into an executable graph:
+// the steps of enqueue_step, with each step's end and the next one's start
+// in one launch (k_ctl_step_next).  Nothing in a step is a memset or a host
+// transfer.
 int capture_steps(mara_block* b)
 {
     hipGraph_t g = nullptr;
     HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    for (int s = 0; s < mara_block::kGraphSteps; ++s) enqueue_step(b);
+    k_ctl_step_start<<<1, 64, 0, b->stream>>>(b->ctl);
+    for (int s = 0; s < mara_block::kGraphSteps; ++s)
+    {
+        for (int rk = 0; rk < b->d.rk_order; ++rk) launch_stage(b, rk, 1);
+        if (s + 1 < mara_block::kGraphSteps) k_ctl_step_next<<<1, 64, 0, b->stream>>>(b->ctl);
+        else k_ctl_step_end<<<1, 1, 0, b->stream>>>(b->ctl);
+    }
     const hipError_t launched = hipGetLastError();
     const hipError_t ended = hipStreamEndCapture(b->stream, &g);
     if (!hip_ok(launched) || !hip_ok(ended))

@@ -962,11 +962,11 @@ static __global__ void k_ctl_begin_run(Ctl* ctl, double t, double t_final, doubl
 // standalone reduction on the first step, from the previous step's last
 // stage afterwards).
 // One wavefront (64 threads): every lane reads the clock, lane b folds bin b.
-static __global__ void __launch_bounds__(64) k_ctl_step_start(Ctl* ctl)
+// Step start on one wavefront, from the clock (active, t, steps) it is given.
+__device__ __forceinline__ void step_start_wave(Ctl* ctl, int lane, int active, double t, int steps)
 {
-    const int lane = threadIdx.x;
-    const bool go = ctl->active && ctl->t < ctl->t_final && ctl->steps < ctl->max_steps
-                    && ctl->c2p_fail == 0 && ctl->eigen_fail == 0 && ctl->superluminal == 0;
+    const bool go = active && t < ctl->t_final && steps < ctl->max_steps && ctl->c2p_fail == 0
+                    && ctl->eigen_fail == 0 && ctl->superluminal == 0;
     if (!go)
     {
         if (lane == 0) ctl->active = 0;
@@ -978,9 +978,14 @@ static __global__ void __launch_bounds__(64) k_ctl_step_start(Ctl* ctl)
         ctl->active = 1;
         ctl->courant_bits = m;
         const double dt1 = ctl->cfl * bitsd(m);
-        const double dt2 = ctl->t_final - ctl->t;
+        const double dt2 = ctl->t_final - t;
         ctl->dt = dt1 < dt2 ? dt1 : dt2;
     }
+}
+
+static __global__ void __launch_bounds__(64) k_ctl_step_start(Ctl* ctl)
+{
+    step_start_wave(ctl, threadIdx.x, ctl->active, ctl->t, ctl->steps);
 }
 
 static __global__ void k_ctl_step_end(Ctl* ctl)
@@ -988,6 +993,28 @@ static __global__ void k_ctl_step_end(Ctl* ctl)
     if (!ctl->active) return;
     ctl->t += ctl->dt;
     ctl->steps += 1;
+}
+
+// k_ctl_step_end of one step and k_ctl_step_start of the next in one launch:
+// every lane forms the advanced clock from the same loads, lane 0 stores it,
+// and the start reads the clock from registers.
+static __global__ void __launch_bounds__(64) k_ctl_step_next(Ctl* ctl)
+{
+    const int lane = threadIdx.x;
+    const int active = ctl->active;
+    double t = ctl->t;
+    int steps = ctl->steps;
+    if (active)
+    {
+        t += ctl->dt;
+        steps += 1;
+        if (lane == 0)
+        {
+            ctl->t = t;
+            ctl->steps = steps;
+        }
+    }
+    step_start_wave(ctl, lane, active, t, steps);
 }
 
 } // namespace mara
