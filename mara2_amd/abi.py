@@ -71,6 +71,7 @@ class MaraShape(ctypes.Structure):
 PENCIL_SHAPES = (128, 256, 512)
 ZPLANE_SHAPES = (4, 8, 16, 32)
 ROW_SHAPES = (1, 2, 4)
+MHD_ROWS_DEFAULT = 2  # fused 2-D MHD tiles of 64 x 8 zones
 
 
 def make_shape(pencil=256, zplanes=16, rows=4):

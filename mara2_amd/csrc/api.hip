@@ -485,10 +485,10 @@ int mara_block_autotune(mara_block* b, int repeats, mara_shape* chosen, double* 
     // candidates in turn, three rounds, each candidate's fastest round counts:
     // a slow moment of the device then penalises one round, not one candidate
     std::vector<double> fastest(cands.size(), 1e300);
-    const int rounds = cands.size() > 1 ? 3 : 1;
-    for (int round = 0; round < rounds && status == MARA_OK; ++round)
-    for (size_t ci = 0; ci < cands.size(); ++ci)
+    const size_t rounds = cands.size() > 1 ? 3 : 1;
+    for (size_t it = 0; it < rounds * cands.size(); ++it)
     {
+        const size_t ci = it % cands.size();
         const mara_shape& c = cands[ci];
         StageArgs s;
         s.d = &b->d;
