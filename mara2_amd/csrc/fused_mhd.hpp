@@ -27,7 +27,8 @@
 namespace mara {
 
 // Tile height TY = 4 * rows for the launch shape's rows (1, 2 or 4).  LDS per
-// work-group: binary64 46 / 80 / 147 KB, binary32 23 / 40 / 73 KB.  64 x 8
+// work-group (PLM): binary64 46 / 80 / 147 KB; binary32, with its staged
+// stencil zones, 40 / 66 / 116 KB.  64 x 8
 // tiles are the default in both forms: at 2048^2 their stage takes 0.51 ms
 // (binary64) and 0.33 ms (binary32) against 0.60 / 0.35 ms for 64 x 4 and
 // 0.54 / 0.36 ms for 64 x 16 (one work-group per CU).
@@ -55,6 +56,26 @@ __device__ __forceinline__ void mhd2d_tile(const double* __restrict__ Pin, const
     const int ie = G.N[0] - G.g[0], je = G.N[1] - G.g[1];  // one past the last interior zone
     const int k = 0;
 
+    // ---- phase 0 (binary32): the tile's stencil zones into LDS ----------------------
+    // zones i0 - NG .. i0 + 63 + NG of rows j0 - NG .. j0 + TY - 1 + NG, converted
+    // once; indices past the block are clamped and feed no face that is used
+    constexpr int NG = Stencil<Recon>::NG;
+    constexpr bool STAGE = sizeof(R) == 4;
+    constexpr int HX = TX + 2 * NG, HY = TY + 2 * NG;
+    __shared__ R sP[STAGE ? NQ : 1][STAGE ? HY : 1][STAGE ? HX : 1];
+    if constexpr (STAGE)
+    {
+        for (int t = tid; t < HX * HY; t += NT)
+        {
+            const int hx = t % HX, hy = t / HX;
+            const int gi = min(i0 - NG + hx, G.N[0] - 1), gj = min(j0 - NG + hy, G.N[1] - 1);
+            const long long c = cell_index(G, gi, gj, k);
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) sP[q][hy][hx] = R(Pin[q * G.fs + c]);
+        }
+        __syncthreads();
+    }
+
     // ---- phase 1: face fluxes into LDS ----------------------------------------------
     // x-faces (c, r): face i0 + c of row j0 - 1 + r; with CT the rows j0 - 1 and
     // je feed the B_y correction, without CT only the tile's rows are needed
@@ -78,13 +99,39 @@ __device__ __forceinline__ void mhd2d_tile(const double* __restrict__ Pin, const
         R Fq[NQ];
         if (xf)
         {
-            face_flux_at<1, Recon, 0, R>(Pin, G, 0, i0 + c, j0 - 1 + xr_lo + r, k, Fq, ctl);
+            const int jj = j0 - 1 + xr_lo + r;
+            if constexpr (STAGE)
+            {
+                // stencil zones i0 + c - NG .. i0 + c + NG - 1 of row jj
+                R S[NQ][2 * NG];
+                const int ly = jj - (j0 - NG);
+#pragma unroll
+                for (int q = 0; q < NQ; ++q)
+#pragma unroll
+                    for (int m = 0; m < 2 * NG; ++m) S[q][m] = sP[q][ly][c + m];
+                face_flux<1, Recon, 0, R>(S, G, 0, Fq, ctl);
+            }
+            else
+                face_flux_at<1, Recon, 0, R>(Pin, G, 0, i0 + c, jj, k, Fq, ctl);
 #pragma unroll
             for (int q = 0; q < NQ; ++q) fx[q][xr_lo + r][c] = Fq[q];
         }
         else
         {
-            face_flux_at<1, Recon, 0, R>(Pin, G, 1, i0 - 1 + yc_lo + c, j0 + r, k, Fq, ctl);
+            const int ii = i0 - 1 + yc_lo + c;
+            if constexpr (STAGE)
+            {
+                // stencil rows j0 + r - NG .. j0 + r + NG - 1 of column ii
+                R S[NQ][2 * NG];
+                const int lx = ii - (i0 - NG);
+#pragma unroll
+                for (int q = 0; q < NQ; ++q)
+#pragma unroll
+                    for (int m = 0; m < 2 * NG; ++m) S[q][m] = sP[q][r + m][lx];
+                face_flux<1, Recon, 0, R>(S, G, 1, Fq, ctl);
+            }
+            else
+                face_flux_at<1, Recon, 0, R>(Pin, G, 1, ii, j0 + r, k, Fq, ctl);
 #pragma unroll
             for (int q = 0; q < NQ; ++q) fy[q][r][yc_lo + c] = Fq[q];
         }
