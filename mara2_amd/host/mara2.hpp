@@ -93,6 +93,14 @@ public:
 class NewtonianMHD : public ConservationLaw
 {
 public:
+    // NewtonianMHD::setCoolingRate (src/ConservationLaws.cpp:354-358): a
+    // positive rate adds an auxiliary ninth field and the cooling source,
+    // which the GPU step does not carry; zero or negative (the default, off)
+    // is accepted.
+    void setCoolingRate(double cr)
+    {
+        if (cr > 0.0) throw std::logic_error("GPU step: NewtonianMHD cooling (auxiliary field) is not supported");
+    }
     int getNumConserved() const override { return 8; }
     int getIndexFor(VariableType type) const override;
     std::string getPrimitiveName(int fieldIndex) const override;

@@ -1,0 +1,57 @@
+// The host layer's configuration errors, which need no GPU: the reference's
+// logic_error cases (MethodOfLinesTVD::setRungeKuttaOrder,
+// src/SolutionSchemes.cpp:77-84) and the configurations the GPU step
+// refuses (NewtonianMHD cooling).  Prints one line per check; exit status 0
+// when every check holds.
+#include <cstdio>
+#include <memory>
+#include <stdexcept>
+
+#include "../../mara2_amd/host/mara2.hpp"
+
+using namespace mara2;
+
+template <typename F> static int expect_logic_error(const char* what, F f)
+{
+    try
+    {
+        f();
+    }
+    catch (const std::logic_error&)
+    {
+        std::printf("ok   %s\n", what);
+        return 0;
+    }
+    std::printf("FAIL %s: no logic_error\n", what);
+    return 1;
+}
+
+template <typename F> static int expect_no_error(const char* what, F f)
+{
+    try
+    {
+        f();
+    }
+    catch (const std::exception& e)
+    {
+        std::printf("FAIL %s: %s\n", what, e.what());
+        return 1;
+    }
+    std::printf("ok   %s\n", what);
+    return 0;
+}
+
+int main()
+{
+    int bad = 0;
+    auto ss = std::make_shared<GpuMethodOfLinesTVD>();
+    bad += expect_logic_error("rk order 0", [&] { ss->setRungeKuttaOrder(0); });
+    bad += expect_logic_error("rk order 4", [&] { ss->setRungeKuttaOrder(4); });
+    bad += expect_no_error("rk orders 1..3", [&] {
+        for (int k = 1; k <= 3; ++k) ss->setRungeKuttaOrder(k);
+    });
+    auto mhd = std::make_shared<NewtonianMHD>();
+    bad += expect_logic_error("MHD cooling rate 0.1", [&] { mhd->setCoolingRate(0.1); });
+    bad += expect_no_error("MHD cooling off", [&] { mhd->setCoolingRate(-1.0); });
+    return bad == 0 ? 0 : 1;
+}
