@@ -166,6 +166,20 @@ int mara_block_set_source(mara_block* block, const char* code_object, const char
    passes its star-particle data there); they start at zero. */
 int mara_block_set_source_aux(mara_block* block, const double* t);
 
+/* Viscous fluxes (MethodOfLinesTVD::setViscousFluxFunction,
+   src/SolutionSchemes.cpp:91-94 and :183-186): a per-face function in a
+   gfx950 code object (written with include/mara_source.h,
+   MARA_VISCOUS_FLUX_KERNEL), run on every stage after the Godunov fluxes and
+   before the divergence, with up to MARA_SOURCE_MAX_PARAMS parameters and the
+   auxiliary data of mara_block_set_source_aux.  The block takes the
+   flux-array path while it is set.  A null code_object removes it.
+   MARA_ERR_UNSUPPORTED in the mixed form (binary32 flux arrays) and for a
+   magnetic law with flux-CT (the reference applies it to the CT-corrected
+   fluxes, which this path forms inside the zone update);
+   MARA_ERR_INVALID_ARGUMENT for a file or kernel that cannot be loaded. */
+int mara_block_set_viscous_flux(mara_block* block, const char* code_object, const char* kernel,
+                                const double* params, int n_params);
+
 /* mara_block_run replays its steps from a HIP graph of 16 steps, captured on
    the first run (enable=1, the default), or launches every kernel from the
    host (enable=0).  Results are the same either way; runs with stage timing

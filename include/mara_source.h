@@ -68,4 +68,60 @@ typedef struct mara_source_args
         for (int q = 0; q < a.nq; ++q) a.S[q * a.fs + c] = S[q];                                  \
     }
 
+/* ---------------------------------------------------------------------------
+   Viscous fluxes for mara_block_set_viscous_flux.  The reference takes a
+   viscousFlux function (P, fluxes, T) that adds to the Godunov fluxes after
+   flux-CT and before the divergence (MethodOfLinesTVD::setViscousFluxFunction,
+   src/SolutionSchemes.cpp:91-94, called at :183-186).  On the GPU it is a
+   per-face __device__ function
+       void fn (const mara_flux_args& a, int axis, int i, int j, int k)
+   that updates the fluxes of the face below cell (i, j, k) along `axis`
+   (between cells (i, j, k) - e_axis and (i, j, k)), wrapped with
+   MARA_VISCOUS_FLUX_KERNEL and compiled to a code object as above.
+   --------------------------------------------------------------------------- */
+typedef struct mara_flux_args
+{
+    const double* P;   /* stage primitives, field q of cell c at P[q * fs + c] */
+    double* F[3];      /* Godunov fluxes along each axis (null for a flat
+                          axis): field q of face f at F[a][q * ffs + f],
+                          f = (k * (N[1] + 1) + j) * (N[0] + 1) + i */
+    long long fs;      /* cell field stride */
+    long long ffs;     /* face field stride */
+    int N[3];          /* cells including guards */
+    int g[3];          /* guard zones */
+    int n[3];          /* interior cells */
+    int nq;            /* fields of the law */
+    int pad;
+    double lower[3];   /* domain lower corner */
+    double dx[3];      /* cell size */
+    double t[8];       /* auxiliary data (mara_block_set_source_aux) */
+    double params[MARA_SOURCE_MAX_PARAMS];
+} mara_flux_args;
+
+/* MARA_VISCOUS_FLUX_KERNEL(name, fn): an extern "C" kernel `name` that calls
+   fn(a, axis, i, j, k) once for every face the zone update reads: along each
+   axis with fluxes, faces g .. g + n of that axis on the interior rows and
+   columns of the others; one thread per face, 256 per work-group, the axes
+   in order x, y, z. */
+#define MARA_VISCOUS_FLUX_KERNEL(name, fn)                                                        \
+    extern "C" __global__ void __launch_bounds__(256) name(mara_flux_args a)                      \
+    {                                                                                             \
+        long long u = (long long)blockIdx.x * blockDim.x + threadIdx.x;                            \
+        for (int ax = 0; ax < 3; ++ax)                                                            \
+        {                                                                                         \
+            if (!a.F[ax]) continue;                                                               \
+            const int m0 = a.n[0] + (ax == 0), m1 = a.n[1] + (ax == 1), m2 = a.n[2] + (ax == 2);   \
+            const long long cnt = (long long)m0 * m1 * m2;                                        \
+            if (u < cnt)                                                                          \
+            {                                                                                     \
+                const int i = a.g[0] + (int)(u % m0);                                             \
+                const int j = a.g[1] + (int)((u / m0) % m1);                                      \
+                const int k = a.g[2] + (int)(u / ((long long)m0 * m1));                           \
+                fn(a, ax, i, j, k);                                                               \
+                return;                                                                           \
+            }                                                                                     \
+            u -= cnt;                                                                             \
+        }                                                                                         \
+    }
+
 #endif /* MARA_SOURCE_H */

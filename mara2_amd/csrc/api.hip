@@ -121,9 +121,26 @@ struct mara_block
     hipFunction_t src_fn = nullptr;
     mara_source_args src_args = {};
     double* Ssrc = nullptr;
+    // viscous fluxes (mara_block_set_viscous_flux): code object, kernel,
+    // launch record and face count; while set, the stages take the
+    // flux-array path
+    hipModule_t vis_module = nullptr;
+    hipFunction_t vis_fn = nullptr;
+    mara_flux_args vis_args = {};
+    long long vis_faces = 0;
 };
 
 namespace {
+
+// the viscous-flux kernel of the block, on the stage's primitives and the
+// flux arrays the sweeps just wrote (StageArgs::after_flux)
+void launch_viscous(void* ctx)
+{
+    mara_block* b = static_cast<mara_block*>(ctx);
+    void* args[] = {&b->vis_args};
+    (void)hipModuleLaunchKernel(b->vis_fn, (unsigned)((b->vis_faces + 255) / 256), 1, 1, 256, 1, 1, 0, b->stream,
+                                args, nullptr);
+}
 
 void launch_bc(mara_block* b, double* P)
 {
@@ -202,6 +219,9 @@ void launch_stage(mara_block* b, int rk, int courant_out)
     s.kz = b->kz;
     s.rows = b->rows;
     s.S = b->src_fn ? b->Ssrc : nullptr;
+    b->vis_args.P = Pin;
+    s.after_flux = b->vis_fn ? launch_viscous : nullptr;
+    s.after_flux_ctx = b;
     if (mixed) stage_kernels_f32(s);
     else stage_kernels<double>(s);
     if (timed)
@@ -418,6 +438,7 @@ int mara_block_destroy(mara_block* b)
     hipFree(b->cbins);
     if (b->steps_exec) (void)hipGraphExecDestroy(b->steps_exec);
     if (b->src_module) (void)hipModuleUnload(b->src_module);
+    if (b->vis_module) (void)hipModuleUnload(b->vis_module);
     hipFree(b->Ssrc);
     if (b->ctl_host) hipHostFree(b->ctl_host);
     for (int e = 0; e < 2 * mara_block::kTimingPairs; ++e)
@@ -511,6 +532,8 @@ int mara_block_autotune(mara_block* b, int repeats, mara_shape* chosen, double* 
         s.kz = c.zplanes;
         s.rows = c.rows;
         s.S = b->src_fn ? b->Ssrc : nullptr;
+        s.after_flux = nullptr;  // the tuner times the stage kernels only
+        s.after_flux_ctx = nullptr;
         const bool mixed = b->d.precision == MARA_PREC_MIXED;
         for (int r = 0; r <= repeats; ++r)
         {
@@ -737,18 +760,20 @@ int mara_block_set_source(mara_block* b, const char* code_object, const char* ke
     b->steps_exec = nullptr;
     if (!code_object)
     {
-        b->fused = fused_supported(&b->d);
+        b->fused = fused_supported(&b->d) && !b->vis_fn;
         return MARA_OK;
     }
     if (hipModuleLoad(&b->src_module, code_object) != hipSuccess)
     {
         std::fprintf(stderr, "mara_gpu: cannot load source code object %s\n", code_object);
+        (void)hipGetLastError();  // the failure is reported here, not by the next launch check
         b->src_module = nullptr;
         return MARA_ERR_INVALID_ARGUMENT;
     }
     if (hipModuleGetFunction(&b->src_fn, b->src_module, kernel) != hipSuccess)
     {
         std::fprintf(stderr, "mara_gpu: no kernel %s in %s\n", kernel, code_object);
+        (void)hipGetLastError();
         (void)hipModuleUnload(b->src_module);
         b->src_module = nullptr;
         b->src_fn = nullptr;
@@ -792,9 +817,78 @@ int mara_block_set_source(mara_block* b, const char* code_object, const char* ke
 int mara_block_set_source_aux(mara_block* b, const double* t)
 {
     if (!b || !t) return MARA_ERR_INVALID_ARGUMENT;
-    for (int q = 0; q < 8; ++q) b->src_args.t[q] = t[q];
+    for (int q = 0; q < 8; ++q) b->src_args.t[q] = b->vis_args.t[q] = t[q];
     if (b->steps_exec) (void)hipGraphExecDestroy(b->steps_exec);  // launch records are captured by value
     b->steps_exec = nullptr;
+    return MARA_OK;
+}
+
+int mara_block_set_viscous_flux(mara_block* b, const char* code_object, const char* kernel, const double* params,
+                                int n_params)
+{
+    if (!b || n_params < 0 || n_params > MARA_SOURCE_MAX_PARAMS || (n_params > 0 && !params)
+        || (code_object && !kernel))
+        return MARA_ERR_INVALID_ARGUMENT;
+    if (code_object && (b->d.precision == MARA_PREC_MIXED || b->S.ct)) return MARA_ERR_UNSUPPORTED;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    if (b->vis_module) (void)hipModuleUnload(b->vis_module);
+    b->vis_module = nullptr;
+    b->vis_fn = nullptr;
+    if (b->steps_exec) (void)hipGraphExecDestroy(b->steps_exec);  // captured without / with the old hook
+    b->steps_exec = nullptr;
+    if (!code_object)
+    {
+        // back to the block's own stage kernels
+        b->fused = fused_supported(&b->d) && !b->src_fn;
+        b->pencil = pencil_supported(&b->d) ? pencil_length(b->d.law, b->pencil_shape) : 0;
+        b->mhd2d = mhd2d_supported(&b->d);
+        return MARA_OK;
+    }
+    if (hipModuleLoad(&b->vis_module, code_object) != hipSuccess)
+    {
+        std::fprintf(stderr, "mara_gpu: cannot load viscous-flux code object %s\n", code_object);
+        (void)hipGetLastError();  // the failure is reported here, not by the next launch check
+        b->vis_module = nullptr;
+        return MARA_ERR_INVALID_ARGUMENT;
+    }
+    if (hipModuleGetFunction(&b->vis_fn, b->vis_module, kernel) != hipSuccess)
+    {
+        std::fprintf(stderr, "mara_gpu: no kernel %s in %s\n", kernel, code_object);
+        (void)hipGetLastError();
+        (void)hipModuleUnload(b->vis_module);
+        b->vis_module = nullptr;
+        b->vis_fn = nullptr;
+        return MARA_ERR_INVALID_ARGUMENT;
+    }
+    // the flux-array path: flux arrays of every axis with zones
+    for (int a = 0; a < 3; ++a)
+    {
+        if (!b->S.fleshed[a] || b->F[a]) continue;
+        const size_t fb = (size_t)b->S.nq * b->S.ffs * b->S.real;
+        HIP_TRY(hipMalloc(&b->F[a], fb));
+        b->bytes += (long long)fb;
+    }
+    b->fused = false;
+    b->pencil = 0;
+    b->mhd2d = false;
+    mara_flux_args& a = b->vis_args;
+    a.fs = b->G.fs;
+    a.ffs = b->G.ffs;
+    a.nq = b->S.nq;
+    b->vis_faces = 0;
+    for (int e = 0; e < 3; ++e)
+    {
+        a.F[e] = b->S.fleshed[e] ? static_cast<double*>(b->F[e]) : nullptr;
+        a.N[e] = b->G.N[e];
+        a.g[e] = b->G.g[e];
+        a.n[e] = b->G.n[e];
+        a.lower[e] = b->d.lower[e];
+        a.dx[e] = (b->d.upper[e] - b->d.lower[e]) / b->d.cells[e];  // CartesianMeshGeometry
+        if (a.F[e])
+            b->vis_faces += (long long)(b->G.n[0] + (e == 0)) * (b->G.n[1] + (e == 1)) * (b->G.n[2] + (e == 2));
+    }
+    for (int q = 0; q < 8; ++q) a.t[q] = b->src_args.t[q];
+    for (int q = 0; q < MARA_SOURCE_MAX_PARAMS; ++q) a.params[q] = q < n_params ? params[q] : 0.0;
     return MARA_OK;
 }
 

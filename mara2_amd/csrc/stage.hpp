@@ -35,6 +35,10 @@ struct StageArgs
     int rows;    // flux-array kernels: work-group rows of 64 zones; fused 2-D
                  // MHD tiles: 64 x (4 * rows) zones
     const double* S;  // source terms added to L (same layout as P), or null
+    // flux-array path: called between the flux sweeps and the zone update
+    // (the viscous-flux hook enqueues its kernel there), or null
+    void (*after_flux)(void*);
+    void* after_flux_ctx;
 };
 
 // 2-D work-groups of 64 x rows over an (m0, m1, m2) box: one wavefront per x
@@ -195,6 +199,7 @@ void stage_kernels(const StageArgs& s)
             default: stage_flux_law<2, R>(s, axis); break;
         }
     }
+    if (s.after_flux) s.after_flux(s.after_flux_ctx);
     switch (s.d->law)
     {
         case 0: stage_update_law<0, R>(s); break;

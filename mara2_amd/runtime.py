@@ -59,6 +59,7 @@ def load():
     L.mara_block_set_graphs.argtypes = [blk, ctypes.c_int]
     L.mara_block_set_source.argtypes = [blk, ctypes.c_char_p, ctypes.c_char_p, _dp, ctypes.c_int]
     L.mara_block_set_source_aux.argtypes = [blk, _dp]
+    L.mara_block_set_viscous_flux.argtypes = [blk, ctypes.c_char_p, ctypes.c_char_p, _dp, ctypes.c_int]
     shp = ctypes.POINTER(MaraShape)
     L.mara_block_get_shape.argtypes = [blk, shp]
     L.mara_block_set_shape.argtypes = [blk, shp]
@@ -195,8 +196,18 @@ class Block:
             self._h, os.fsencode(code_object) if code_object else None, kernel.encode() if kernel else None,
             pr.ctypes.data_as(_dp) if len(pr) else None, len(pr)), "mara_block_set_source")
 
+    def set_viscous_flux(self, code_object, kernel=None, params=()):
+        """Load a viscous-flux code object (include/mara_source.h,
+        MARA_VISCOUS_FLUX_KERNEL) and its kernel; it adds to the Godunov fluxes
+        of every stage before the divergence.  code_object=None removes it."""
+        pr = np.ascontiguousarray(params, dtype=np.float64)
+        _check(self.lib.mara_block_set_viscous_flux(
+            self._h, os.fsencode(code_object) if code_object else None, kernel.encode() if kernel else None,
+            pr.ctypes.data_as(_dp) if len(pr) else None, len(pr)), "mara_block_set_viscous_flux")
+
     def set_source_aux(self, t):
-        """The 8 auxiliary values passed to the source function."""
+        """The 8 auxiliary values passed to the source and viscous-flux
+        functions."""
         t8 = np.zeros(8)
         t8[:len(t)] = t
         _check(self.lib.mara_block_set_source_aux(self._h, t8.ctypes.data_as(_dp)), "mara_block_set_source_aux")

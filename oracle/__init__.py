@@ -52,6 +52,8 @@ def lib():
         L.oracle_advance.argtypes = [ctypes.POINTER(MaraDesc), _dp, _d, _d, ctypes.POINTER(MaraReport)]
         L.oracle_advance_src.argtypes = [ctypes.POINTER(MaraDesc), _dp, _d, _d, ctypes.c_void_p, _dp, _dp,
                                          ctypes.POINTER(MaraReport)]
+        L.oracle_advance_visc.argtypes = [ctypes.POINTER(MaraDesc), _dp, _d, _d, ctypes.c_void_p, _dp, _dp,
+                                          ctypes.POINTER(MaraReport)]
         L.oracle_courant_dt.argtypes = [ctypes.POINTER(MaraDesc), _dp, _dp]
         L.oracle_monopole_max.argtypes = [ctypes.POINTER(MaraDesc), _dp, _dp]
         L.oracle_set_threads.argtypes = [_i]
@@ -174,15 +176,26 @@ def apply_bc(desc, P):
 
 # test sources compiled into the oracle (mara_oracle.c): name -> C symbol
 SOURCES = {"uniform_gravity": "oracle_source_uniform_gravity", "point_mass": "oracle_source_point_mass"}
+# test viscous-flux functions compiled into the oracle
+VISCOUS = {"constant": "oracle_viscous_constant"}
 
 
-def advance(desc, P, t0, dt, source=None):
+def advance(desc, P, t0, dt, source=None, viscous=None):
     """MethodOfLinesTVD::advance on P (MeshData::P layout) in place.
     source: None or (name, t[8], params) -- a sourceTermsFunction of SOURCES
-    with its auxiliary data and parameters."""
+    with its auxiliary data and parameters; viscous: None or (name, params)
+    -- a viscous-flux function of VISCOUS (not both)."""
     assert P.shape == array_shape(desc)
+    assert source is None or viscous is None
     rep = MaraReport()
-    if source is None:
+    if viscous is not None:
+        name, params = viscous
+        fn = ctypes.cast(getattr(lib(), VISCOUS[name]), ctypes.c_void_p)
+        pr = np.zeros(16)
+        pr[:len(params)] = params
+        err = lib().oracle_advance_visc(ctypes.byref(desc), _ptr(P), t0, dt, fn, _ptr(np.zeros(8)), _ptr(pr),
+                                        ctypes.byref(rep))
+    elif source is None:
         err = lib().oracle_advance(ctypes.byref(desc), _ptr(P), t0, dt, ctypes.byref(rep))
     else:
         name, taux, params = source
@@ -212,7 +225,7 @@ def set_threads(n):
     lib().oracle_set_threads(int(n))
 
 
-def run(desc, P, cfl, t_final, max_steps=10**9, source=None):
+def run(desc, P, cfl, t_final, max_steps=10**9, source=None, viscous=None):
     """The maraMainLoop contract (src/Mara.cpp:42-88) on the oracle:
     dt = min(cfl * courant, t_final - t) until t >= t_final."""
     t, steps = 0.0, 0
@@ -223,7 +236,7 @@ def run(desc, P, cfl, t_final, max_steps=10**9, source=None):
         dt1 = cfl * dtc
         dt2 = t_final - t
         dt = dt1 if dt1 < dt2 else dt2
-        err, rep = advance(desc, P, 0.0, dt, source)
+        err, rep = advance(desc, P, 0.0, dt, source, viscous)
         if err:
             return err, t, steps
         t += dt

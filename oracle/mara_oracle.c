@@ -1136,14 +1136,68 @@ void oracle_source_point_mass(double x, double y, double z, const double* t, con
     S[4] = P[0] * (P[1] * a0 + P[2] * a1 + P[3] * a2);
 }
 
+/* A viscous-flux function of the reference's setViscousFluxFunction shape
+   (src/SolutionSchemes.cpp:91-94, called after flux-CT at :183-186):
+   (P, fluxes, T), here with the mesh descriptor and a parameter array; P in
+   the MeshData::P layout, F in the (N1+1, N2+1, N3+1, nq, 3) face layout. */
+typedef void (*oracle_viscous_fn)(const mara_desc* d, const double* P, double* F, const double* t,
+                                  const double* params);
+
+/* Test viscous flux, restated on the device in mara2_amd/sources/: constant
+   kinematic viscosity nu = params[0].  At each face between cells m (lower)
+   and p (upper) along an axis of spacing h, with rho = (rho_m + rho_p) / 2,
+   mu = nu rho, dv = (v_p - v_m) / h and vb = (v_m + v_p) / 2 per component:
+   F(momentum c) -= mu dv_c and F(energy) -= mu (vb . dv). */
+void oracle_viscous_constant(const mara_desc* d, const double* P, double* F, const double* t, const double* params)
+{
+    (void)t;
+    layout l;
+    l.nq = law_nq(d->law);
+    for (int a = 0; a < 3; ++a) l.N[a] = d->cells[a] + 2 * d->guards[a];
+    double dx[3], dA[3], dV;
+    mesh_spacing(d, dx, dA, &dV);
+    for (int a = 0; a < 3; ++a)
+    {
+        if (l.N[a] < 2) continue;
+        const double h = dx[a];
+        int lo[3] = {0, 0, 0};
+        lo[a] = 1;
+        #pragma omp parallel for collapse(2) schedule(static)
+        for (int i = lo[0]; i < l.N[0]; ++i)
+        for (int j = lo[1]; j < l.N[1]; ++j)
+        for (int k = lo[2]; k < l.N[2]; ++k)
+        {
+            int I[3] = {i, j, k};
+            I[a] -= 1;
+            const double* Pp = P + cidx(&l, i, j, k);
+            const double* Pm = P + cidx(&l, I[0], I[1], I[2]);
+            const double rho = 0.5 * (Pm[0] + Pp[0]);
+            const double mu = params[0] * rho;
+            double dv[3], vb[3];
+            for (int c = 0; c < 3; ++c)
+            {
+                dv[c] = (Pp[1 + c] - Pm[1 + c]) / h;
+                vb[c] = 0.5 * (Pm[1 + c] + Pp[1 + c]);
+            }
+            for (int c = 0; c < 3; ++c)
+            {
+                double* Fc = F + fidx(&l, i, j, k, 1 + c, a);
+                *Fc = *Fc - mu * dv[c];
+            }
+            double* Fe = F + fidx(&l, i, j, k, 4, a);
+            *Fe = *Fe - mu * (vb[0] * dv[0] + vb[1] * dv[1] + vb[2] * dv[2]);
+        }
+    }
+}
+
 static int advance_impl(const mara_desc* d, double* P, double dt, oracle_source_fn src, const double* taux,
-                        const double* params, mara_report* rep);
+                        const double* params, oracle_viscous_fn visc, mara_report* rep);
 
 /* MethodOfLinesTVD::advance (src/SolutionSchemes.cpp:108-275). */
 int oracle_advance(const mara_desc* d, double* P, double t0, double dt, mara_report* rep)
 {
     (void)t0;
-    return advance_impl(d, P, dt, NULL, NULL, NULL, rep);
+    return advance_impl(d, P, dt, NULL, NULL, NULL, NULL, rep);
 }
 
 /* The same with a sourceTermsFunction: after the divergence, S is evaluated
@@ -1154,11 +1208,20 @@ int oracle_advance_src(const mara_desc* d, double* P, double t0, double dt, orac
                        const double* taux, const double* params, mara_report* rep)
 {
     (void)t0;
-    return advance_impl(d, P, dt, src, taux, params, rep);
+    return advance_impl(d, P, dt, src, taux, params, NULL, rep);
+}
+
+/* The same with a viscous-flux function applied to the Godunov fluxes after
+   flux-CT and before the divergence (src/SolutionSchemes.cpp:183-186). */
+int oracle_advance_visc(const mara_desc* d, double* P, double t0, double dt, oracle_viscous_fn visc,
+                        const double* taux, const double* params, mara_report* rep)
+{
+    (void)t0;
+    return advance_impl(d, P, dt, NULL, taux, params, visc, rep);
 }
 
 static int advance_impl(const mara_desc* d, double* P, double dt, oracle_source_fn src, const double* taux,
-                        const double* params, mara_report* rep)
+                        const double* params, oracle_viscous_fn visc, mara_report* rep)
 {
     layout l;
     const int nq = law_nq(d->law);
@@ -1251,6 +1314,9 @@ static int advance_impl(const mara_desc* d, double* P, double dt, oracle_source_
         /* fluxCorrection: flux-CT when the law has B (src/SolutionSchemes.cpp:173-187) */
         if (d->law != MARA_LAW_HYDRO && d->field_ct)
             oracle_ct_correct(F, fs, nq, 5);
+
+        /* viscousFlux (P, fluxes, T) (src/SolutionSchemes.cpp:183-186) */
+        if (visc) visc(d, P, F, taux, params);
 
         /* L = divergence(F, -1), U = U0 (1-b) + (U + dt L) b */
         oracle_divergence(F, fs, nq, -1.0, dA, dV, L);

@@ -129,6 +129,7 @@ def test_removing_the_source_restores_the_plain_step():
 
 
 def test_bad_code_object_or_kernel_is_refused():
+    """Refused, and the block (with its HIP error state) stays usable."""
     desc = abi.make_desc(**CASES["hydro1d_pencil"])
     with Block(desc) as blk:
         with pytest.raises(runtime.MaraError):
@@ -137,3 +138,5 @@ def test_bad_code_object_or_kernel_is_refused():
             blk.set_source(GRAVITY[0], "no_such_kernel")
         with pytest.raises(runtime.MaraError):
             blk.set_source(GRAVITY[0], GRAVITY[1], np.zeros(17))
+        blk.upload(_state(desc, 65))
+        blk.advance(0.3 * blk.courant_dt())

@@ -381,3 +381,45 @@ def test_uniform_gravity_source_on_a_uniform_state():
     x = problems.cell_centres(d, 0)
     g0 = d.guards[0]
     assert err == 0 and P[g0, 5, 0, 1] > 0 and P[g0 + 7, 5, 0, 1] < 0 and x[g0] < 0.25 < x[g0 + 7]
+
+
+def test_constant_viscosity_flux_is_the_discrete_shear_diffusion():
+    """viscousFlux path (src/SolutionSchemes.cpp:183-186) with the oracle's
+    constant-viscosity function: on a shear flow v_y(x) at uniform density
+    and pressure, one forward-Euler step with and without viscosity differs
+    by dt nu (v_{i+1} - 2 v_i + v_{i-1}) / dx^2 in v_y (the Godunov fluxes
+    are the same in both runs), and the energy by the divergence of
+    nu rho vb dv; density is untouched.  A wrong sign, a missing 1/dx or a
+    swapped face would fail it."""
+    n, nu, dt, gamma = 64, 0.02, 1e-3, 5 / 3
+    d = abi.make_desc(law="hydro", recon="pcm", riemann="hlle", bc="periodic", cells=(n, 1, 1), rk_order=1,
+                      gamma=gamma)
+    P0 = problems.empty(d)
+    x = problems.cell_centres(d, 0)
+    P0[..., 0] = 1.0
+    P0[..., 4] = 1.0
+    P0[:, 0, 0, 2] = 0.1 * np.sin(2 * np.pi * x)
+    P0 = problems.apply_bc(d, P0)
+    runs = []
+    for visc in (None, ("constant", [nu])):
+        P = P0.copy()
+        err, _ = oracle.advance(d, P, 0.0, dt, None, visc)
+        assert err == 0
+        runs.append(P)
+    g = d.guards[0]
+    inner = slice(g, g + n)
+    dx = 1.0 / n
+    v = P0[:, 0, 0, 2]
+    lap = (v[g + 1:g + n + 1] - 2 * v[inner] + v[g - 1:g + n - 1]) / dx**2
+    dvy = runs[1][inner, 0, 0, 2] - runs[0][inner, 0, 0, 2]
+    np.testing.assert_allclose(dvy, dt * nu * lap, rtol=1e-8, atol=1e-13)
+    assert np.array_equal(runs[1][..., 0], runs[0][..., 0])
+
+    def energy(P):
+        return P[..., 4] / (gamma - 1) + 0.5 * P[..., 0] * np.sum(P[..., 1:4] ** 2, axis=-1)
+
+    vb = 0.5 * (v[g:g + n + 1] + v[g - 1:g + n])       # faces g .. g + n
+    dvf = (v[g:g + n + 1] - v[g - 1:g + n]) / dx
+    G = nu * 1.0 * vb * dvf
+    dE = (energy(runs[1]) - energy(runs[0]))[inner, 0, 0]
+    np.testing.assert_allclose(dE, dt * (G[1:] - G[:-1]) / dx, rtol=1e-6, atol=1e-13)
