@@ -349,6 +349,12 @@ void GpuMethodOfLinesTVD::applySource(mara_block* b) const
                                     (int)sourceParams.size()),
               "mara_block_set_source");
     check(mara_block_set_source_aux(b, sourceAux.data()), "mara_block_set_source_aux");
+    if (viscousPath.empty())
+        check(mara_block_set_viscous_flux(b, nullptr, nullptr, nullptr, 0), "mara_block_set_viscous_flux");
+    else
+        check(mara_block_set_viscous_flux(b, viscousPath.c_str(), viscousKernel.c_str(), viscousParams.data(),
+                                          (int)viscousParams.size()),
+              "mara_block_set_viscous_flux");
 }
 
 void GpuMethodOfLinesTVD::setSourceTermsCodeObject(const std::string& path, const std::string& kernel,
@@ -358,6 +364,16 @@ void GpuMethodOfLinesTVD::setSourceTermsCodeObject(const std::string& path, cons
     sourcePath = path;
     sourceKernel = kernel;
     sourceParams = std::move(params);
+    if (block) applySource(block);
+}
+
+void GpuMethodOfLinesTVD::setViscousFluxCodeObject(const std::string& path, const std::string& kernel,
+                                                   std::vector<double> params)
+{
+    if (params.size() > MARA_SOURCE_MAX_PARAMS) throw std::logic_error("at most 16 viscous-flux parameters");
+    viscousPath = path;
+    viscousKernel = kernel;
+    viscousParams = std::move(params);
     if (block) applySource(block);
 }
 

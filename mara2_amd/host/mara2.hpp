@@ -334,6 +334,13 @@ public:
     void setSourceTermsCodeObject(const std::string& path, const std::string& kernel,
                                   std::vector<double> params = {});
     void setStarParticleData(const std::array<double, 8>& t);
+    // setViscousFluxFunction (src/SolutionSchemes.cpp:91-94) on the GPU: the
+    // per-face function in a code object (MARA_VISCOUS_FLUX_KERNEL of
+    // include/mara_source.h), its kernel name and up to 16 parameters; an
+    // empty path removes it.  The C ABI refuses it in the mixed form and for
+    // a magnetic law with flux-CT (mara_block_set_viscous_flux).
+    void setViscousFluxCodeObject(const std::string& path, const std::string& kernel,
+                                  std::vector<double> params = {});
     // launch-shape autotuner on the block of md (mara_block_autotune); the
     // state is left as it is
     mara_shape autotune(const MeshData& md, int repeats = 5) const;
@@ -344,6 +351,8 @@ private:
     std::string sourcePath, sourceKernel;
     std::vector<double> sourceParams;
     std::array<double, 8> sourceAux{};
+    std::string viscousPath, viscousKernel;
+    std::vector<double> viscousParams;
     int rungeKuttaOrder = 1;
     bool disableFieldCT = false;
     int precision = MARA_PREC_FP64;

@@ -2,10 +2,12 @@
 // written against the host layer, running every step on the GPU.
 //
 //   mara_regress <program> <problem> <scheme> <out.bin> [max_steps
-//                [source.hsaco kernel [parameters...]]]
+//                [[viscous=]code.hsaco kernel [parameters...]]]
 //   program: regress-1d | regress-rel | regress-2d | regress-mhd | sod
-//   The optional source is a source-term code object (include/mara_source.h)
-//   set with GpuMethodOfLinesTVD::setSourceTermsCodeObject.
+//   The optional code object (include/mara_source.h) is a source-term
+//   function set with GpuMethodOfLinesTVD::setSourceTermsCodeObject, or with
+//   the viscous= prefix a viscous-flux function set with
+//   GpuMethodOfLinesTVD::setViscousFluxCodeObject.
 //
 // Writes the initial primitives (after the ghost fill), the final ones and
 // the loop's clock to out.bin, so a test can replay the same initial data on
@@ -90,7 +92,7 @@ std::map<std::string, Problem> problemsMhd()
 int usage()
 {
     std::cerr << "usage: mara_regress <regress-1d|regress-rel|regress-2d|regress-mhd|sod> <problem> <scheme> <out.bin>"
-                 " [max_steps [source.hsaco kernel [parameters...]]]\n";
+                 " [max_steps [[viscous=]code.hsaco kernel [parameters...]]]\n";
     return 2;
 }
 
@@ -101,12 +103,19 @@ int main(int argc, const char* argv[])
     if (argc < 5) return usage();
     const std::string program = argv[1], pname = argv[2], sname = argv[3], out = argv[4];
     const int maxSteps = argc > 5 ? std::atoi(argv[5]) : 1 << 30;
-    // optional source terms: <code object> <kernel> [parameters...]
+    // optional source terms: <code object> <kernel> [parameters...], or a
+    // viscous-flux function: viscous=<code object> <kernel> [parameters...]
     std::string srcPath, srcKernel;
     std::vector<double> srcParams;
+    bool viscous = false;
     if (argc > 7)
     {
         srcPath = argv[6];
+        if (srcPath.rfind("viscous=", 0) == 0)
+        {
+            viscous = true;
+            srcPath = srcPath.substr(8);
+        }
         srcKernel = argv[7];
         for (int a = 8; a < argc; ++a) srcParams.push_back(std::atof(argv[a]));
     }
@@ -180,7 +189,8 @@ int main(int argc, const char* argv[])
     ss->setMeshOperator(mo);
     ss->setBoundaryCondition(problem.bc);
     ss->setIntercellFluxScheme(fs);
-    if (!srcPath.empty()) ss->setSourceTermsCodeObject(srcPath, srcKernel, srcParams);
+    if (!srcPath.empty() && viscous) ss->setViscousFluxCodeObject(srcPath, srcKernel, srcParams);
+    else if (!srcPath.empty()) ss->setSourceTermsCodeObject(srcPath, srcKernel, srcParams);
     problem.bc->setConservationLaw(cl);
 
     SimulationStatus status;

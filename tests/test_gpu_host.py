@@ -76,3 +76,22 @@ def test_regression_program_with_source_terms_matches_oracle(tmp_path):
     assert err == 0
     assert so == steps and to == t
     assert np.array_equal(Pg, Po)
+
+
+def test_regression_program_with_viscous_flux_matches_oracle(tmp_path):
+    """GpuMethodOfLinesTVD::setViscousFluxCodeObject through the C++ layer:
+    the 2-D shock tube with constant viscosity (the viscous_constant code
+    object) against the oracle's viscous-flux path, bit for bit."""
+    out = tmp_path / "out.bin"
+    obj = os.path.join(ROOT, "mara2_amd", "sources", "viscous_constant.hsaco")
+    r = subprocess.run([EXE, "regress-2d", "Shocktube2D", "plm2", str(out), "1000000", "viscous=" + obj,
+                        "mara_viscous_constant", "0.002"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    steps, t, P0, Pg = _read(out)
+    desc = abi.make_desc(law="hydro", recon="plm", riemann="hlle", bc="outflow", cells=(128, 128, 1),
+                         guards=(2, 2, 0), rk_order=2)
+    Po = P0.copy()
+    err, to, so = oracle.run(desc, Po, 0.5, 0.1, viscous=("constant", [0.002]))
+    assert err == 0
+    assert so == steps and to == t
+    assert np.array_equal(Pg, Po)
