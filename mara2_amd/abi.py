@@ -52,6 +52,7 @@ class MaraDesc(ctypes.Structure):
         ("rk_order", ctypes.c_int),
         ("field_ct", ctypes.c_int),
         ("precision", ctypes.c_int),
+        ("cooling_rate", ctypes.c_double),
     ]
 
 
@@ -102,7 +103,7 @@ class MaraReport(ctypes.Structure):
 def make_desc(law="hydro", gamma=5.0 / 3.0, pressure_floor=-1.0, recon="plm",
               plm_theta=1.5, riemann="hlle", bc="periodic", cells=(128, 1, 1),
               guards=None, lower=(0.0, 0.0, 0.0), upper=(1.0, 1.0, 1.0),
-              rk_order=2, field_ct=1, precision="fp64"):
+              rk_order=2, field_ct=1, precision="fp64", cooling_rate=0.0):
     """Build a MaraDesc from names.  guards defaults to the scheme's stencil
     size on every axis with more than one cell (the reference programs'
     boundary shapes, src/Problems.cpp:155-156)."""
@@ -126,9 +127,16 @@ def make_desc(law="hydro", gamma=5.0 / 3.0, pressure_floor=-1.0, recon="plm",
     d.rk_order = rk_order
     d.field_ct = field_ct
     d.precision = PRECISIONS[precision] if isinstance(precision, str) else precision
+    d.cooling_rate = cooling_rate
     return d
 
 
 def array_shape(desc):
     """Shape of MeshData::P for a descriptor: (n1+2g1, n2+2g2, n3+2g3, nq)."""
-    return tuple(desc.cells[a] + 2 * desc.guards[a] for a in range(3)) + (num_fields(desc.law),)
+    return tuple(desc.cells[a] + 2 * desc.guards[a] for a in range(3)) + (desc_num_fields(desc),)
+
+
+def desc_num_fields(desc):
+    """Fields of a descriptor's state: num_fields(law), plus the auxiliary
+    one of NewtonianMHD with a positive cooling rate."""
+    return 9 if desc.law == LAW_MHD and desc.cooling_rate > 0 else num_fields(desc.law)

@@ -53,7 +53,7 @@ Sizes sizes_of(const mara_desc* d)
         N[a] = d->cells[a] + 2 * d->guards[a];
         s.fleshed[a] = N[a] > 1;
     }
-    s.nq = law_nq(d->law);
+    s.nq = law_nq(law_code(*d));
     s.ncell = (long long)N[0] * N[1] * N[2];
     s.fs = round_up(s.ncell, 64);
     s.nface = (long long)(N[0] + 1) * (N[1] + 1) * (N[2] + 1);
@@ -72,6 +72,7 @@ long long device_bytes(const mara_desc* d, const Sizes& s)
     if (in_pass && d->rk_order == 3) b += 1ll * s.nq * s.fs * 8;  // Pc
     if (!in_pass)
         b += (long long)(s.fleshed[0] + s.fleshed[1] + s.fleshed[2]) * s.nq * s.ffs * s.real;
+    if (s.nq == 9) b += 1ll * s.nq * s.fs * 8;  // cooling terms
     b += 256 + CBINS * CBIN_STRIDE * 8;  // control block and CFL bins
     return b;
 }
@@ -160,7 +161,10 @@ void launch_courant(mara_block* b)
     switch (b->d.law)
     {
         case 0: k_courant<0><<<nb, kThreads, 0, b->stream>>>(b->Pa, G, b->ctl); break;
-        case 1: k_courant<1><<<nb, kThreads, 0, b->stream>>>(b->Pa, G, b->ctl); break;
+        case 1:
+            if (G.nq == 9) k_courant<3><<<nb, kThreads, 0, b->stream>>>(b->Pa, G, b->ctl);
+            else k_courant<1><<<nb, kThreads, 0, b->stream>>>(b->Pa, G, b->ctl);
+            break;
         default: k_courant<2><<<nb, kThreads, 0, b->stream>>>(b->Pa, G, b->ctl); break;
     }
 }
@@ -186,6 +190,13 @@ void launch_stage(mara_block* b, int rk, int courant_out)
         Pout = last ? b->Pa : (rk == 0 ? b->Pb : b->Pc);
     }
 
+    // NewtonianMHD cooling (ConservationLaw::addSourceTerms) into S
+    const bool cooling = b->G.nq == 9;
+    if (cooling)
+    {
+        const long long zones = (long long)b->G.n[0] * b->G.n[1] * b->G.n[2];
+        k_cooling_source<<<(unsigned)((zones + 255) / 256), 256, 0, b->stream>>>(Pin, b->Ssrc, b->G);
+    }
     // source terms of the stage's primitives (MeshOperator::generateSourceTerms)
     if (b->src_fn)
     {
@@ -218,7 +229,7 @@ void launch_stage(mara_block* b, int rk, int courant_out)
     s.mhd2d = b->mhd2d;
     s.kz = b->kz;
     s.rows = b->rows;
-    s.S = b->src_fn ? b->Ssrc : nullptr;
+    s.S = b->src_fn || cooling ? b->Ssrc : nullptr;
     b->vis_args.P = Pin;
     s.after_flux = b->vis_fn ? launch_viscous : nullptr;
     s.after_flux_ctx = b;
@@ -409,6 +420,12 @@ int mara_block_create(const mara_desc* d, mara_block** out)
             if (!hip_ok(hipMalloc(&b->F[a], fb))) return fail(MARA_ERR_HIP);
         }
     }
+    if (b->G.nq == 9)
+    {
+        // the cooling terms of each stage (k_cooling_source)
+        if (!hip_ok(hipMalloc(&b->Ssrc, cellbytes))) return fail(MARA_ERR_HIP);
+        if (!hip_ok(hipMemset(b->Ssrc, 0, cellbytes))) return fail(MARA_ERR_HIP);
+    }
     if (!hip_ok(hipMalloc(&b->ctl, sizeof(Ctl)))) return fail(MARA_ERR_HIP);
     if (!hip_ok(hipMalloc(&b->cbins, CBINS * CBIN_STRIDE * sizeof(unsigned long long)))) return fail(MARA_ERR_HIP);
     if (!hip_ok(hipHostMalloc((void**)&b->ctl_host, sizeof(Ctl), hipHostMallocDefault))) return fail(MARA_ERR_HIP);
@@ -531,7 +548,7 @@ int mara_block_autotune(mara_block* b, int repeats, mara_shape* chosen, double* 
         s.mhd2d = b->mhd2d;
         s.kz = c.zplanes;
         s.rows = c.rows;
-        s.S = b->src_fn ? b->Ssrc : nullptr;
+        s.S = b->src_fn || b->G.nq == 9 ? b->Ssrc : nullptr;
         s.after_flux = nullptr;  // the tuner times the stage kernels only
         s.after_flux_ctx = nullptr;
         const bool mixed = b->d.precision == MARA_PREC_MIXED;
@@ -752,6 +769,9 @@ int mara_block_set_source(mara_block* b, const char* code_object, const char* ke
     if (!b || n_params < 0 || n_params > MARA_SOURCE_MAX_PARAMS || (n_params > 0 && !params)
         || (code_object && !kernel))
         return MARA_ERR_INVALID_ARGUMENT;
+    // the cooling terms use the S array; the reference adds them to L before
+    // the source function's, an order one array cannot keep
+    if (code_object && b->G.nq == 9) return MARA_ERR_UNSUPPORTED;
     HIP_TRY(hipStreamSynchronize(b->stream));
     if (b->src_module) (void)hipModuleUnload(b->src_module);
     b->src_module = nullptr;

@@ -261,6 +261,7 @@ int validate_item(mara_batch* B, const mara_desc* d)
     const int v = validate(d);
     if (v) return v;
     if (d->rk_order != B->rk_order || d->precision != B->precision) return MARA_ERR_UNSUPPORTED;
+    if (law_code(*d) == LAW_MHD_COOLING) return MARA_ERR_UNSUPPORTED;  // single blocks only
     return MARA_OK;
 }
 

@@ -658,7 +658,8 @@ inline bool fused_supported(const mara_desc* d)
 // schedule, so they hold no flux arrays either.
 inline bool pencil_supported(const mara_desc* d)
 {
-    return d->cells[1] == 1 && d->cells[2] == 1 && d->cells[0] > 1 && d->guards[0] > 0 && d->rk_order >= 2;
+    return d->cells[1] == 1 && d->cells[2] == 1 && d->cells[0] > 1 && d->guards[0] > 0 && d->rk_order >= 2
+           && !(d->cooling_rate > 0.0);
 }
 
 inline const char* fused_kernel_name(const mara_desc& d)

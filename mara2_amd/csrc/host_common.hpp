@@ -28,7 +28,14 @@ inline bool hip_ok(hipError_t e)
 }
 
 inline long long round_up(long long x, long long m) { return (x + m - 1) / m * m; }
-inline int law_nq(int law) { return law == MARA_LAW_HYDRO ? 5 : 8; }
+// Law codes of the kernels: the three of the ABI, and 3 for NewtonianMHD
+// with a positive cooling rate (nine fields: the auxiliary one last).
+constexpr int LAW_MHD_COOLING = 3;
+inline int law_nq(int law) { return law == MARA_LAW_HYDRO ? 5 : law == LAW_MHD_COOLING ? 9 : 8; }
+inline int law_code(const mara_desc& d)
+{
+    return d.law == MARA_LAW_MHD && d.cooling_rate > 0.0 ? LAW_MHD_COOLING : d.law;
+}
 inline int recon_ng(int recon) { return recon == MARA_RECON_PCM ? 1 : recon == MARA_RECON_PLM ? 2 : 3; }
 
 // UnitVector::xhat/yhat/zhat cartesian components (src/UnitVector.cpp:9-39):
@@ -57,6 +64,9 @@ inline int validate(const mara_desc* d)
     if (d->rk_order < 1 || d->rk_order > 3) return MARA_ERR_INVALID_ARGUMENT;
     if (d->precision < 0 || d->precision > 1) return MARA_ERR_INVALID_ARGUMENT;
     if (d->riemann == MARA_RIEMANN_HLLC && d->law != MARA_LAW_HYDRO) return MARA_ERR_UNSUPPORTED;
+    // the cooling source is a NewtonianMHD option, built for the binary64 form
+    if (d->cooling_rate > 0.0 && (d->law != MARA_LAW_MHD || d->precision != MARA_PREC_FP64))
+        return MARA_ERR_UNSUPPORTED;
     const int ng = recon_ng(d->recon);
     for (int a = 0; a < 3; ++a)
     {
@@ -84,12 +94,13 @@ inline Geom make_geom(const mara_desc& d)
         G.g[a] = d.guards[a];
         G.n[a] = d.cells[a];
     }
-    G.nq = law_nq(d.law);
+    G.nq = law_nq(law_code(d));
     G.bc = d.bc;
     G.fs = round_up((long long)G.N[0] * G.N[1] * G.N[2], 64);
     G.ffs = round_up((long long)(G.N[0] + 1) * (G.N[1] + 1) * (G.N[2] + 1), 64);
     G.gamma = d.gamma;
     G.pfloor = d.pressure_floor;
+    G.cool = law_code(d) == LAW_MHD_COOLING ? d.cooling_rate : 0.0;
     G.theta = d.plm_theta;
     double dx[3];
     for (int a = 0; a < 3; ++a) dx[a] = (d.upper[a] - d.lower[a]) / d.cells[a];

@@ -57,6 +57,7 @@ struct Geom
     long long fs;   // cell field stride
     long long ffs;  // face field stride
     double gamma, pfloor, theta;
+    double cool;    // NewtonianMHD cooling rate (law 3), else 0
     double dA[3], dV, Lmin;
     double nhat[9]; // UnitVector xhat, yhat, zhat
 };
@@ -441,6 +442,25 @@ __device__ __forceinline__ double finish_zone(const double* __restrict__ P0, dou
     return dtc;
 }
 
+// NewtonianMHD::addSourceTerms (src/ConservationLaws.cpp:489-501) as a
+// source array for the zone update: with c = p / (gamma - 1) * rate from the
+// stage's primitives, S(energy) = -c and S(auxiliary) = +c, so that
+// L + S equals the reference's L -= c, L += c.  Interior zones; the other
+// fields of S stay zero.
+static __global__ void __launch_bounds__(256) k_cooling_source(const double* __restrict__ P, double* __restrict__ S,
+                                                               const Geom G)
+{
+    const long long z = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long nx = G.n[0], nxy = nx * G.n[1];
+    if (z >= nxy * G.n[2]) return;
+    const int i = G.g[0] + (int)(z % nx), j = G.g[1] + (int)((z / nx) % G.n[1]), k = G.g[2] + (int)(z / nxy);
+    const long long c = cell_index(G, i, j, k);
+    const double gm1 = G.gamma - 1.0;
+    const double q = P[4 * G.fs + c] / gm1 * G.cool;
+    S[4 * G.fs + c] = -q;
+    S[8 * G.fs + c] = q;
+}
+
 // L = -div F of one field (src/MeshOperator.cpp:366-372), in R for the
 // Godunov fluxes; the flux-CT magnetic terms in binary64.
 template <typename R>
@@ -477,7 +497,7 @@ __device__ __forceinline__ double update_zone(const double* __restrict__ P0, dou
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
     {
-        if (CT && q >= 5)
+        if (CT && q >= 5 && q < 8)  // B; an auxiliary field (q = 8) is not one
         {
             const int cc = q - 5;
             const double x1 = hx && cc != 0 ? ct_value(Fx, Fy, Fz, G, 0, cc, fx) : 0.0;

@@ -428,14 +428,35 @@ __device__ __forceinline__ int rmhd_state(R gamma, const R* P, const R* dAA, Sta
     return ERR_NONE;
 }
 
-// Law dispatch.  Law: 0 hydro, 1 MHD, 2 RMHD (include/mara_gpu.h).
-template <int Law> struct LawNQ { static constexpr int value = Law == 0 ? 5 : 8; };
+// Law dispatch.  Law: 0 hydro, 1 MHD, 2 RMHD (include/mara_gpu.h), 3 MHD
+// with a positive cooling rate: the MHD fields and an auxiliary ninth one,
+// U = P, zero flux and eigenvalue (NewtonianMHD::fromPrimitive with
+// numAuxiliary = 1, src/ConservationLaws.cpp:479-485).
+template <int Law> struct LawNQ { static constexpr int value = Law == 0 ? 5 : Law == 3 ? 9 : 8; };
 
 template <int Law, typename R>
 __device__ __forceinline__ int law_state(R gamma, const R* P, const R* n, State<R, LawNQ<Law>::value>& S)
 {
     if constexpr (Law == 0) { hydro_state(gamma, P, n, S); return ERR_NONE; }
     else if constexpr (Law == 1) { mhd_state(gamma, P, n, S); return ERR_NONE; }
+    else if constexpr (Law == 3)
+    {
+        State<R, 8> m;
+        mhd_state(gamma, P, n, m);
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+        {
+            S.P[q] = m.P[q];
+            S.U[q] = m.U[q];
+            S.F[q] = m.F[q];
+            S.A[q] = m.A[q];
+        }
+        S.P[8] = P[8];
+        S.U[8] = P[8];
+        S.F[8] = R(0);
+        S.A[8] = R(0);
+        return ERR_NONE;
+    }
     else return rmhd_state(gamma, P, n, S);
 }
 
@@ -588,6 +609,12 @@ __device__ __forceinline__ int prim_to_cons(double gamma, const double* P, doubl
         U[5] = P[5];
         U[6] = P[6];
         U[7] = P[7];
+        return ERR_NONE;
+    }
+    else if constexpr (Law == 3)
+    {
+        prim_to_cons<1>(gamma, P, U);
+        U[8] = P[8];
         return ERR_NONE;
     }
     else
@@ -768,6 +795,13 @@ __device__ __forceinline__ int cons_to_prim(double gamma, double pfloor, const d
         P[2] = U[2] / U[0];
         P[3] = U[3] / U[0];
         return 0;
+    }
+    else if constexpr (Law == 3)
+    {
+        // the auxiliary field is copied before the bad-state test
+        // (src/ConservationLaws.cpp:386-389)
+        P[8] = U[8];
+        return cons_to_prim<1>(gamma, pfloor, U, P, floor);
     }
     else if constexpr (Law == 1)
     {

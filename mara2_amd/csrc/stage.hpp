@@ -189,21 +189,29 @@ void stage_kernels(const StageArgs& s)
         }
         return;
     }
+    // law 3: NewtonianMHD with cooling (nine fields), binary64 only
+    const int law = s.G->nq == 9 ? 3 : s.d->law;
     for (int axis = 0; axis < 3; ++axis)
     {
         if (s.G->N[axis] < 2) continue;
-        switch (s.d->law)
+        switch (law)
         {
             case 0: stage_flux_law<0, R>(s, axis); break;
             case 1: stage_flux_law<1, R>(s, axis); break;
+            case 3:
+                if constexpr (sizeof(R) == 8) stage_flux_law<3, R>(s, axis);
+                break;
             default: stage_flux_law<2, R>(s, axis); break;
         }
     }
     if (s.after_flux) s.after_flux(s.after_flux_ctx);
-    switch (s.d->law)
+    switch (law)
     {
         case 0: stage_update_law<0, R>(s); break;
         case 1: stage_update_law<1, R>(s); break;
+        case 3:
+            if constexpr (sizeof(R) == 8) stage_update_law<3, R>(s);
+            break;
         default: stage_update_law<2, R>(s); break;
     }
 }

@@ -467,7 +467,14 @@ typedef struct
     double P[NF], U[NF], F[NF], A[NF];
 } ostate;
 
-static int law_nq(int law) { return law == MARA_LAW_HYDRO ? 5 : 8; }
+/* Law 3: NewtonianMHD with a positive cooling rate (setCoolingRate,
+   src/ConservationLaws.cpp:354-358): the MHD fields and one auxiliary. */
+#define LAW_MHD_COOLING 3
+static int law_nq(int law) { return law == MARA_LAW_HYDRO ? 5 : law == LAW_MHD_COOLING ? 9 : 8; }
+static int law_of(const mara_desc* d)
+{
+    return d->law == MARA_LAW_MHD && d->cooling_rate > 0.0 ? LAW_MHD_COOLING : d->law;
+}
 
 /* NewtonianHydro::fromPrimitive (src/ConservationLaws.cpp:253-290) */
 static void hydro_prim(double gamma, const double* P, const double* dAA, ostate* S)
@@ -649,6 +656,14 @@ static int from_primitive(int law, double gamma, const double* P, const double* 
     {
         case MARA_LAW_HYDRO: hydro_prim(gamma, P, n, S); return MARA_OK;
         case MARA_LAW_MHD: mhd_prim(gamma, P, n, S); return MARA_OK;
+        case LAW_MHD_COOLING:
+            /* the auxiliary field: P = U = P[8], F = A = 0 (:479-485) */
+            mhd_prim(gamma, P, n, S);
+            S->P[8] = P[8];
+            S->U[8] = P[8];
+            S->F[8] = 0.0;
+            S->A[8] = 0.0;
+            return MARA_OK;
         default: return rmhd_prim(gamma, P, n, S);
     }
 }
@@ -668,6 +683,11 @@ static int from_conserved(int law, double gamma, double pfloor, const double* U,
         P[2] = U[2] / U[0];
         P[3] = U[3] / U[0];
         return MARA_OK;
+    }
+    if (law == LAW_MHD_COOLING)
+    {
+        P[8] = U[8];  /* copied before the bad-state test (:386-389) */
+        law = MARA_LAW_MHD;
     }
     if (law == MARA_LAW_MHD)
     {
@@ -833,7 +853,7 @@ static int intercell_flux(const mara_desc* d, int nq, const double* S /* [2g][nq
             PR[q] = weno5(&ps[3], C_C2L_FD, D_C2L_FD, 2, 50.0);
         }
     }
-    return oracle_riemann(d->law, d->riemann, d->gamma, PL, PR, n, F);
+    return oracle_riemann(law_of(d), d->riemann, d->gamma, PL, PR, n, F);
 }
 
 /* ========================================================================== */
@@ -1052,7 +1072,7 @@ void oracle_vector_potential_to_B(const mara_desc* d, const double* A, double* B
 int oracle_apply_bc(const mara_desc* d, double* P)
 {
     layout l;
-    l.nq = law_nq(d->law);
+    l.nq = law_nq(law_of(d));
     for (int a = 0; a < 3; ++a) l.N[a] = d->cells[a] + 2 * d->guards[a];
     for (int axis = 0; axis < 3; ++axis)
     {
@@ -1152,7 +1172,7 @@ void oracle_viscous_constant(const mara_desc* d, const double* P, double* F, con
 {
     (void)t;
     layout l;
-    l.nq = law_nq(d->law);
+    l.nq = law_nq(law_of(d));
     for (int a = 0; a < 3; ++a) l.N[a] = d->cells[a] + 2 * d->guards[a];
     double dx[3], dA[3], dV;
     mesh_spacing(d, dx, dA, &dV);
@@ -1224,7 +1244,7 @@ static int advance_impl(const mara_desc* d, double* P, double dt, oracle_source_
                         const double* params, oracle_viscous_fn visc, mara_report* rep)
 {
     layout l;
-    const int nq = law_nq(d->law);
+    const int nq = law_nq(law_of(d));
     const int ng = stencil_size(d->recon);
     int fp[3];
     l.nq = nq;
@@ -1264,7 +1284,7 @@ static int advance_impl(const mara_desc* d, double* P, double dt, oracle_source_
         for (long c = 0; c < ncell; ++c)
         {
             ostate S;
-            int e = from_primitive(d->law, d->gamma, P + c * nq, xhat, &S);
+            int e = from_primitive(law_of(d), d->gamma, P + c * nq, xhat, &S);
             if (e) { if (e > err) err = e; continue; }
             for (int q = 0; q < nq; ++q) U0[c * nq + q] = S.U[q];
         }
@@ -1320,6 +1340,19 @@ static int advance_impl(const mara_desc* d, double* P, double dt, oracle_source_
 
         /* L = divergence(F, -1), U = U0 (1-b) + (U + dt L) b */
         oracle_divergence(F, fs, nq, -1.0, dA, dV, L);
+        /* ConservationLaw::addSourceTerms: NewtonianMHD cooling
+           (src/ConservationLaws.cpp:489-501), every cell */
+        if (law_of(d) == LAW_MHD_COOLING)
+        {
+            const double gm1 = d->gamma - 1.0;
+            #pragma omp parallel for schedule(static)
+            for (long c = 0; c < ncell; ++c)
+            {
+                const double p = P[c * nq + 4];
+                L[c * nq + 4] -= p / gm1 * d->cooling_rate;
+                L[c * nq + 8] += p / gm1 * d->cooling_rate;
+            }
+        }
         if (src)
         {
             /* coordinateAtIndex(i + start) with start = -guards (src/CartesianMeshGeometry.cpp:61-67) */
@@ -1354,7 +1387,7 @@ static int advance_impl(const mara_desc* d, double* P, double dt, oracle_source_
             const long c = cidx(&l, i, j, k);
             double Pq[NF];
             int health;
-            int e = from_conserved(d->law, d->gamma, d->pressure_floor, U + c, Pq, &health);
+            int e = from_conserved(law_of(d), d->gamma, d->pressure_floor, U + c, Pq, &health);
             if (e)
             {
                 const long lin = ((long)(i - g0) * m1 + (j - g1)) * m2 + (k - g2);
@@ -1387,7 +1420,7 @@ static int advance_impl(const mara_desc* d, double* P, double dt, oracle_source_
 int oracle_courant_dt(const mara_desc* d, const double* P, double* dt_out)
 {
     layout l;
-    l.nq = law_nq(d->law);
+    l.nq = law_nq(law_of(d));
     for (int a = 0; a < 3; ++a) l.N[a] = d->cells[a] + 2 * d->guards[a];
     double dx[3], dA[3], dV;
     mesh_spacing(d, dx, dA, &dV);
@@ -1411,7 +1444,7 @@ int oracle_courant_dt(const mara_desc* d, const double* P, double* dt_out)
         for (int a = 0; a < 3; ++a)
         {
             ostate S;
-            int e = from_primitive(d->law, d->gamma, pc, nhat + 3 * a, &S);
+            int e = from_primitive(law_of(d), d->gamma, pc, nhat + 3 * a, &S);
             if (e) { if (e > status) status = e; lam[a] = 0; continue; }
             double m = 0.0;
             for (int n = 0; n < l.nq; ++n)
@@ -1436,8 +1469,9 @@ int oracle_monopole_max(const mara_desc* d, const double* P, double* out)
     const long nc = (long)N[0] * N[1] * N[2];
     double* B = (double*)malloc(nc * 3 * sizeof(double));
     double* M = (double*)malloc(nc * sizeof(double));
+    const int nq = law_nq(law_of(d));
     for (long c = 0; c < nc; ++c)
-        for (int q = 0; q < 3; ++q) B[c * 3 + q] = P[c * 8 + 5 + q];
+        for (int q = 0; q < 3; ++q) B[c * 3 + q] = P[c * nq + 5 + q];
     oracle_monopole_cell(B, N, M);
     double m = 0.0;
     for (int i = d->guards[0]; i < N[0] - d->guards[0]; ++i)

@@ -45,7 +45,7 @@ int main(void) {
   F(mara_desc, law) F(mara_desc, gamma) F(mara_desc, pressure_floor) F(mara_desc, recon)
   F(mara_desc, plm_theta) F(mara_desc, riemann) F(mara_desc, bc) F(mara_desc, cells)
   F(mara_desc, guards) F(mara_desc, lower) F(mara_desc, upper) F(mara_desc, rk_order)
-  F(mara_desc, field_ct) F(mara_desc, precision)
+  F(mara_desc, field_ct) F(mara_desc, precision) F(mara_desc, cooling_rate)
   F(mara_report, status) F(mara_report, c2p_failures) F(mara_report, first_failure)
   F(mara_report, floor_applied) F(mara_report, eigen_failures) F(mara_report, superluminal)
   return 0; }

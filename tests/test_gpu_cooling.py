@@ -1,0 +1,74 @@
+"""NewtonianMHD with a cooling rate (mara_desc.cooling_rate; setCoolingRate,
+src/ConservationLaws.cpp:354-358, addSourceTerms :489-501): the auxiliary
+ninth field and the cooling terms on the flux-array path, bit for bit against
+the oracle in FP64 -- 1-D, 2-D and 3-D, flux-CT on and off, RK 1 to 3, the
+device loop -- and the configurations it is refused in."""
+import numpy as np
+import pytest
+
+from mara2_amd import abi, problems, runtime
+from mara2_amd.runtime import Batch, Block
+from helpers import run_pair
+
+pytestmark = pytest.mark.gpu
+
+
+def _state(desc, seed, amp=0.3):
+    P = problems.empty(desc)
+    assert P.shape[3] == 9
+    rng = np.random.default_rng(seed)
+    shp = P.shape[:3]
+    P[..., 0] = 1.0 + 0.5 * rng.random(shp)
+    P[..., 4] = 1.0 + 0.5 * rng.random(shp)
+    P[..., 1:4] = amp * rng.standard_normal(shp + (3,))
+    P[..., 5:8] = 0.5 + 0.2 * rng.standard_normal(shp + (3,))
+    P[..., 8] = rng.random(shp)
+    return problems.apply_bc(desc, P)
+
+
+CASES = {
+    "1d_plm_rk2": dict(cells=(200, 1, 1), recon="plm", bc="outflow", rk_order=2, field_ct=1),
+    "2d_plm_ct_rk2": dict(cells=(70, 37, 1), recon="plm", bc="periodic", rk_order=2, field_ct=1),
+    "2d_weno_noct_rk3": dict(cells=(40, 24, 1), recon="weno5", bc="periodic", rk_order=3, field_ct=0),
+    "2d_pcm_ct_rk1": dict(cells=(48, 30, 1), recon="pcm", bc="outflow", rk_order=1, field_ct=1),
+    "3d_plm_ct_rk2": dict(cells=(16, 12, 10), recon="plm", bc="periodic", rk_order=2, field_ct=1),
+}
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_cooling_matches_oracle(case):
+    desc = abi.make_desc(law="mhd", riemann="hlle", cooling_rate=0.8, **CASES[case])
+    P0 = _state(desc, 81)
+    Pg, Po, dg, do = run_pair(desc, P0, 4, 0.3, Block)
+    assert dg == do
+    diff = np.argwhere(Pg != Po)
+    assert diff.size == 0, f"{len(diff)} differ, first {diff[0]}"
+
+
+def test_device_loop_with_cooling_matches_oracle_loop():
+    import oracle
+    desc = abi.make_desc(law="mhd", riemann="hlle", cooling_rate=2.0, **CASES["2d_plm_ct_rk2"])
+    P0 = _state(desc, 82)
+    Po = P0.copy()
+    err, to, so = oracle.run(desc, Po, 0.4, 0.05, max_steps=40)
+    assert err == 0
+    with Block(desc) as blk:
+        blk.upload(P0)
+        tg, sg, rep = blk.run(0.4, 0.05, 40)
+        Pg = blk.download()
+        mono = blk.monopole_max()
+    assert (sg, tg) == (so, to)
+    assert np.array_equal(Pg, Po)
+    e, mo = oracle.monopole_max(desc, Po)
+    assert e == 0 and mono == mo
+
+
+def test_refused_configurations():
+    with pytest.raises(runtime.MaraError):
+        Block(abi.make_desc(law="mhd", riemann="hlle", cells=(32, 32, 1), cooling_rate=0.5, precision="mixed"))
+    with pytest.raises(runtime.MaraError):
+        Block(abi.make_desc(law="hydro", cells=(32, 32, 1), cooling_rate=0.5))
+    desc = abi.make_desc(law="mhd", riemann="hlle", cooling_rate=0.5, **CASES["2d_plm_ct_rk2"])
+    with Batch(1 << 28, rk_order=2, precision="fp64", max_solves=4) as b:
+        with pytest.raises(runtime.MaraError):
+            b.admit(desc, _state(desc, 83), 0.4, 0.1, 10)

@@ -423,3 +423,37 @@ def test_constant_viscosity_flux_is_the_discrete_shear_diffusion():
     G = nu * 1.0 * vb * dvf
     dE = (energy(runs[1]) - energy(runs[0]))[inner, 0, 0]
     np.testing.assert_allclose(dE, dt * (G[1:] - G[:-1]) / dx, rtol=1e-6, atol=1e-13)
+
+
+def test_mhd_cooling_moves_thermal_energy_into_the_auxiliary_field():
+    """NewtonianMHD::addSourceTerms with a cooling rate (src/ConservationLaws.cpp:
+    489-501) on a uniform state: the fluxes cancel, so one forward-Euler step
+    gives p' = p (1 - rate dt) and aux' = aux + dt p / (gamma - 1) rate, and
+    the total U(energy) + U(aux) is unchanged; density, velocity and B stay
+    put.  A wrong sign, a missing 1/(gamma - 1) or the wrong field would
+    fail it."""
+    gamma, rate, dt = 5 / 3, 0.7, 1e-3
+    d = abi.make_desc(law="mhd", recon="plm", riemann="hlle", bc="periodic", cells=(8, 8, 1), rk_order=1,
+                      gamma=gamma, cooling_rate=rate)
+    P = problems.empty(d)
+    assert P.shape[3] == 9
+    P[..., 0] = 1.3
+    P[..., 1:4] = (0.2, -0.1, 0.05)
+    P[..., 4] = 0.9
+    P[..., 5:8] = (0.3, 0.4, -0.2)
+    P[..., 8] = 0.25
+    P0 = P.copy()
+    err, _ = oracle.advance(d, P, 0.0, dt)
+    assert err == 0
+    g = d.guards
+    inner = P[g[0]:g[0] + 8, g[1]:g[1] + 8, 0]
+    np.testing.assert_allclose(inner[..., 4], 0.9 * (1 - rate * dt), rtol=1e-13)
+    np.testing.assert_allclose(inner[..., 8], 0.25 + dt * 0.9 / (gamma - 1) * rate, rtol=1e-14)
+    for q in (0, 1, 2, 3, 5, 6, 7):
+        np.testing.assert_allclose(inner[..., q], P0[g[0], g[1], 0, q], rtol=1e-14, atol=1e-15)
+
+    def total(Q):
+        kin = 0.5 * Q[..., 0] * np.sum(Q[..., 1:4] ** 2, axis=-1)
+        mag = 0.5 * np.sum(Q[..., 5:8] ** 2, axis=-1)
+        return Q[..., 4] / (gamma - 1) + kin + mag + Q[..., 8]
+    np.testing.assert_allclose(total(inner), total(P0[g[0]:g[0] + 8, g[1]:g[1] + 8, 0]), rtol=1e-14)
