@@ -105,7 +105,8 @@ typedef struct mara_block mara_block;
 const char* mara_version(void);
 const char* mara_error_string(int status);
 
-/* Number of fields of a law (getNumConserved): 5 hydro, 8 MHD/RMHD. */
+/* Number of fields of a law (getNumConserved): 5 hydro, 8 MHD/RMHD; a
+   descriptor with a positive cooling_rate has 9 (the auxiliary field last). */
 int mara_num_fields(int law);
 
 /* Stencil size of a reconstruction (IntercellFluxScheme::getStencilSize). */

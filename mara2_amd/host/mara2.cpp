@@ -71,7 +71,8 @@ std::string NewtonianMHD::getPrimitiveName(int q) const
 {
     static const char* names[] = {"density", "velocity1", "velocity2", "velocity3", "pressure",
                                   "magnetic1", "magnetic2", "magnetic3"};
-    return q >= 0 && q < 8 ? names[q] : "";
+    if (q >= 0 && q < 8) return names[q];
+    return "auxiliary" + std::to_string(q - 7);  // NewtonianMHD::getPrimitiveName
 }
 
 // ---------------------------------------------------------------------------
@@ -294,6 +295,7 @@ mara_desc GpuMethodOfLinesTVD::describe(const MeshData& md) const
     d.law = law->lawCode();
     d.gamma = law->getGammaLawIndex();
     d.pressure_floor = law->getPressureFloor();
+    d.cooling_rate = law->getCoolingRate();
     d.recon = fluxScheme->reconCode();
     d.plm_theta = fluxScheme->getPlmTheta();
     d.riemann = fluxScheme->getRiemannSolver() ? fluxScheme->getRiemannSolver()->solverCode() : MARA_RIEMANN_HLLE;
@@ -451,6 +453,7 @@ double FieldOperator::getCourantTimestep(const MeshData& md, const Array& L) con
     d.law = law->lawCode();
     d.gamma = law->getGammaLawIndex();
     d.pressure_floor = law->getPressureFloor();
+    d.cooling_rate = law->getCoolingRate();
     d.recon = MARA_RECON_PCM;
     d.plm_theta = 1.5;
     d.bc = MARA_BC_OUTFLOW;

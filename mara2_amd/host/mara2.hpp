@@ -75,6 +75,7 @@ public:
     std::vector<std::string> getPrimitiveNames() const;
     double getGammaLawIndex() const { return gammaLawIndex; }
     double getPressureFloor() const { return pressureFloor; }
+    virtual double getCoolingRate() const { return -1.0; }  // NewtonianMHD only
     virtual int lawCode() const = 0;
 protected:
     double gammaLawIndex = 5. / 3;  // reference default (src/ConservationLaws.cpp:233,351,611)
@@ -94,22 +95,23 @@ class NewtonianMHD : public ConservationLaw
 {
 public:
     // NewtonianMHD::setCoolingRate (src/ConservationLaws.cpp:354-358): a
-    // positive rate adds an auxiliary ninth field and the cooling source,
-    // which the GPU step does not carry; zero or negative (the default, off)
-    // is accepted.
-    void setCoolingRate(double cr)
-    {
-        if (cr > 0.0) throw std::logic_error("GPU step: NewtonianMHD cooling (auxiliary field) is not supported");
-    }
-    int getNumConserved() const override { return 8; }
+    // positive rate adds the auxiliary ninth field ("auxiliary1") and the
+    // cooling source (mara_desc.cooling_rate)
+    void setCoolingRate(double cr) { coolingRate = cr; }
+    double getCoolingRate() const override { return coolingRate; }
+    int getNumConserved() const override { return coolingRate > 0.0 ? 9 : 8; }
     int getIndexFor(VariableType type) const override;
     std::string getPrimitiveName(int fieldIndex) const override;
     int lawCode() const override { return MARA_LAW_MHD; }
+private:
+    double coolingRate = -1.0;  // reference default: off
 };
 
 class RelativisticMHD : public NewtonianMHD
 {
 public:
+    double getCoolingRate() const override { return -1.0; }
+    int getNumConserved() const override { return 8; }
     int lawCode() const override { return MARA_LAW_RMHD; }
 };
 

@@ -1,8 +1,8 @@
 // The host layer's configuration errors, which need no GPU: the reference's
 // logic_error cases (MethodOfLinesTVD::setRungeKuttaOrder,
-// src/SolutionSchemes.cpp:77-84) and the configurations the GPU step
-// refuses (NewtonianMHD cooling).  Prints one line per check; exit status 0
-// when every check holds.
+// src/SolutionSchemes.cpp:77-84) and the field count of NewtonianMHD with
+// cooling (src/ConservationLaws.cpp:354-358,503-506).  Prints one line per
+// check; exit status 0 when every check holds.
 #include <cstdio>
 #include <memory>
 #include <stdexcept>
@@ -51,7 +51,12 @@ int main()
         for (int k = 1; k <= 3; ++k) ss->setRungeKuttaOrder(k);
     });
     auto mhd = std::make_shared<NewtonianMHD>();
-    bad += expect_logic_error("MHD cooling rate 0.1", [&] { mhd->setCoolingRate(0.1); });
-    bad += expect_no_error("MHD cooling off", [&] { mhd->setCoolingRate(-1.0); });
+    bad += expect_no_error("MHD cooling adds the auxiliary field", [&] {
+        mhd->setCoolingRate(0.1);
+        if (mhd->getNumConserved() != 9 || mhd->getPrimitiveName(8) != "auxiliary1")
+            throw std::runtime_error("expected 9 fields, the last auxiliary1");
+        mhd->setCoolingRate(-1.0);
+        if (mhd->getNumConserved() != 8) throw std::runtime_error("expected 8 fields with cooling off");
+    });
     return bad == 0 ? 0 : 1;
 }
