@@ -64,9 +64,8 @@ inline int validate(const mara_desc* d)
     if (d->rk_order < 1 || d->rk_order > 3) return MARA_ERR_INVALID_ARGUMENT;
     if (d->precision < 0 || d->precision > 1) return MARA_ERR_INVALID_ARGUMENT;
     if (d->riemann == MARA_RIEMANN_HLLC && d->law != MARA_LAW_HYDRO) return MARA_ERR_UNSUPPORTED;
-    // the cooling source is a NewtonianMHD option, built for the binary64 form
-    if (d->cooling_rate > 0.0 && (d->law != MARA_LAW_MHD || d->precision != MARA_PREC_FP64))
-        return MARA_ERR_UNSUPPORTED;
+    // the cooling source is a NewtonianMHD option
+    if (d->cooling_rate > 0.0 && d->law != MARA_LAW_MHD) return MARA_ERR_UNSUPPORTED;
     const int ng = recon_ng(d->recon);
     for (int a = 0; a < 3; ++a)
     {

@@ -189,7 +189,7 @@ void stage_kernels(const StageArgs& s)
         }
         return;
     }
-    // law 3: NewtonianMHD with cooling (nine fields), binary64 only
+    // law 3: NewtonianMHD with cooling (nine fields)
     const int law = s.G->nq == 9 ? 3 : s.d->law;
     for (int axis = 0; axis < 3; ++axis)
     {
@@ -198,9 +198,7 @@ void stage_kernels(const StageArgs& s)
         {
             case 0: stage_flux_law<0, R>(s, axis); break;
             case 1: stage_flux_law<1, R>(s, axis); break;
-            case 3:
-                if constexpr (sizeof(R) == 8) stage_flux_law<3, R>(s, axis);
-                break;
+            case 3: stage_flux_law<3, R>(s, axis); break;
             default: stage_flux_law<2, R>(s, axis); break;
         }
     }
@@ -209,9 +207,7 @@ void stage_kernels(const StageArgs& s)
     {
         case 0: stage_update_law<0, R>(s); break;
         case 1: stage_update_law<1, R>(s); break;
-        case 3:
-            if constexpr (sizeof(R) == 8) stage_update_law<3, R>(s);
-            break;
+        case 3: stage_update_law<3, R>(s); break;
         default: stage_update_law<2, R>(s); break;
     }
 }
