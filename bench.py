@@ -25,7 +25,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from mara2_amd import abi, problems  # noqa: E402
-from mara2_amd.runtime import Block  # noqa: E402
+from mara2_amd.runtime import Batch, Block  # noqa: E402
 
 METRIC = "zone-updates/s, FP32-mixed vs FP64 at stated L1 error; solves/s + p99 latency"
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
@@ -300,19 +300,12 @@ def serving_workload(args):
     seq_zone_rate = sum(r.zones * r.steps for r in sample) / sum(times)
     log(f"sequential: {seq_rate:.3f} solves/s, mean {np.mean(times):.3f} s, max {max(times):.3f} s")
     budget = int(args.budget_gib * 2**30)
+    # the batch keeps its own launch shape (pencils of 128 faces, 64 x 8 MHD
+    # tiles): tuned on one request alone, a block prefers longer pencils and
+    # taller tiles, which are slower once the batch fills the device
     shape = None
-    if not args.no_autotune:
-        # the batch's pencil length, tuned on the stream's median 1-D request;
-        # the 2-D MHD tiles keep the batch's default height (64 x 8), the
-        # fastest on a full device (a median request alone, 370^2, prefers
-        # 64 x 16, which is slower once the batch fills the device)
-        ones = sorted((r for r in reqs if r.params["dim"] == 1), key=lambda r: r.zones)
-        rep = ones[len(ones) // 2]
-        with Block(rep.desc) as blk:
-            blk.upload(rep.P)
-            shape, _ = blk.autotune(repeats=20)
-        shape.rows = abi.MHD_ROWS_DEFAULT
-        log(f"pencil length {shape.pencil} (tuned on request {rep.index}, {rep.zones} zones)")
+    with Batch(1 << 20, rk_order=2, precision="mixed", max_solves=1) as b0:
+        batch_shape = b0.get_shape().as_dict()
     out = {"metric": METRIC, "value": None, "unit": "solves/s", "n_gpus": 1, "steps": None,
            "warmup": None, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f32/f64 mixed",
@@ -321,7 +314,7 @@ def serving_workload(args):
                                   "30% 2-D 128^2..1024^2, half MHD, PLM + HLLC/HLLE, RK2, CFL 0.4, to t = 0.2, "
                                   "mixed form, Poisson arrivals at 2x the one-at-a-time rate",
                       "zones_total": int(sum(work)), "hbm_pool_gib": args.budget_gib, "chunk_steps": args.chunk,
-                      "graphs": graphs, "launch_shape": shape.as_dict() if shape is not None else None,
+                      "graphs": graphs, "launch_shape": batch_shape,
                       "express_lane_max": args.express},
            "arrival_rate": 2.0 * seq_rate,
            "sequential": {"solves_per_s": seq_rate, "zone_updates_per_s": seq_zone_rate,

@@ -103,7 +103,11 @@ struct mara_batch
     int rk_order = 2;
     int precision = MARA_PREC_FP64;
     int max_solves = 0;
-    int pencil = mara::PENCIL_DEFAULT;  // faces per pencil tile of the 1-D solves
+    // faces per pencil tile of the 1-D solves: 128, the shortest, is the
+    // fastest once the batch fills the device (a saturated 1024-request
+    // serving run spent 11.4 s in 1-D MHD and hydro pencils of 128 faces,
+    // 13.5 s in pencils of 512)
+    int pencil = 128;
     int kz = mara::FUSED_KZ;            // z planes per fused tile of the 3-D hydro solves
     int rows = mara::MHD_ROWS_DEFAULT;  // 2-D MHD tiles of 64 x (4 * rows) zones
     int express_max = 0;                // solves the express lane may hold (0: none)
