@@ -513,30 +513,35 @@ int mara_batch_step(mara_batch* B, int n_steps, int* n_active)
     if (any_pencil && (any_tile || any_express)) ratio = (int)std::min(32.0, std::max(1.0, round_time / 20e-6));
     if (any_express && any_tile) ratio3 = (int)std::min(32.0, std::max(1.0, round_time / step_time(2)));
     const unsigned sg = (unsigned)((n + 3) / 4);  // kb_step_start: 4 items per work-group
-    auto run_lane = [&](int lane) {
+    // `count` steps of one lane: one kernel ends a step and starts the next
+    auto run_lane = [&](int lane, int count) {
         kb_step_start<<<sg, 256, 0, B->cur>>>(B->d_items, n, lane);
-        for (int rk = 0; rk < B->rk_order; ++rk)
+        for (int s = 0; s < count; ++s)
         {
-            if (B->precision == MARA_PREC_MIXED) run_stage_f32(B, rk, lane);
-            else run_stage<double>(B, rk, lane);
+            for (int rk = 0; rk < B->rk_order; ++rk)
+            {
+                if (B->precision == MARA_PREC_MIXED) run_stage_f32(B, rk, lane);
+                else run_stage<double>(B, rk, lane);
+            }
+            if (s + 1 < count) kb_step_next<<<sg, 256, 0, B->cur>>>(B->d_items, n, lane);
+            else kb_step_end<<<ig, 64, 0, B->cur>>>(B->d_items, n, lane);
         }
-        kb_step_end<<<ig, 64, 0, B->cur>>>(B->d_items, n, lane);
     };
     auto run_round = [&]() {
         if (any_tile || (!any_pencil && !any_express))
         {
             B->cur = B->stream;
-            run_lane(0);
+            run_lane(0, 1);
         }
         if (any_pencil)
         {
             B->cur = B->stream2;
-            for (int r = 0; r < ratio; ++r) run_lane(1);
+            run_lane(1, ratio);
         }
         if (any_express)
         {
             B->cur = B->stream3;
-            for (int r = 0; r < ratio3; ++r) run_lane(2);
+            run_lane(2, ratio3);
         }
     };
     const int lanes = (any_tile ? 1 : 0) | (any_pencil ? 2 : 0) | (any_express ? 4 : 0);

@@ -173,22 +173,33 @@ static __global__ void __launch_bounds__(256) kb_step_start(const BItem* __restr
     const int l = (int)(threadIdx.x & 63);
     if (w >= n || items[w].lane != lane) return;
     Ctl* ctl = items[w].ctl;
-    const bool go = ctl->active && ctl->t < ctl->t_final && ctl->steps < ctl->max_steps
-                    && ctl->c2p_fail == 0 && ctl->eigen_fail == 0 && ctl->superluminal == 0;
-    if (!go)
+    step_start_wave(ctl, l, ctl->active, ctl->t, ctl->steps);
+}
+
+// kb_step_end of one step and kb_step_start of the next for every solve of
+// a lane, one wavefront per solve: every lane of the wave forms the advanced
+// clock from the same loads, lane 0 stores it, the start reads it from
+// registers (k_ctl_step_next per solve).
+static __global__ void __launch_bounds__(256) kb_step_next(const BItem* __restrict__ items, int n, int lane)
+{
+    const int w = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int l = (int)(threadIdx.x & 63);
+    if (w >= n || items[w].lane != lane) return;
+    Ctl* ctl = items[w].ctl;
+    const int active = ctl->active;
+    double t = ctl->t;
+    int steps = ctl->steps;
+    if (active)
     {
-        if (l == 0) ctl->active = 0;
-        return;
+        t += ctl->dt;
+        steps += 1;
+        if (l == 0)
+        {
+            ctl->t = t;
+            ctl->steps = steps;
+        }
     }
-    const unsigned long long m = fold_bins_wave(ctl, l);
-    if (l == 0)
-    {
-        ctl->active = 1;
-        ctl->courant_bits = m;
-        const double dt1 = ctl->cfl * bitsd(m);
-        const double dt2 = ctl->t_final - ctl->t;
-        ctl->dt = dt1 < dt2 ? dt1 : dt2;
-    }
+    step_start_wave(ctl, l, active, t, steps);
 }
 
 static __global__ void kb_step_end(const BItem* __restrict__ items, int n, int lane)
