@@ -320,8 +320,8 @@ int mara_batch_set_express(mara_batch* batch, int max_solves);
 int mara_batch_lane(const mara_batch* batch, int slot, int* lane);
 
 /* Launch shape of the batch's pencil tiles (1-D solves), fused 3-D tiles
-   (zplanes) and fused 2-D MHD tiles (rows; 2 at create).  Takes effect at
-   the next step. */
+   (zplanes) and fused 2-D MHD tiles (rows); at create pencil 128, zplanes
+   16, rows 2.  Takes effect at the next step. */
 int mara_batch_get_shape(const mara_batch* batch, mara_shape* shape);
 int mara_batch_set_shape(mara_batch* batch, const mara_shape* shape);
 
