@@ -94,3 +94,27 @@ def test_refused_configurations():
     with Batch(1 << 28, rk_order=2, precision="fp64", max_solves=4) as b:
         with pytest.raises(runtime.MaraError):
             b.admit(desc, _state(desc, 83), 0.4, 0.1, 10)
+
+
+def test_cooling_with_viscous_fluxes_matches_oracle():
+    """Both hooks of the stage on one block (flux-CT off): the viscous fluxes
+    go in before the divergence, the cooling terms after it, as in the
+    reference's advance (src/SolutionSchemes.cpp:183-207)."""
+    import os
+    import oracle
+    src = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "mara2_amd", "sources")
+    desc = abi.make_desc(law="mhd", riemann="hlle", cooling_rate=0.6, **CASES["2d_weno_noct_rk3"])
+    P0 = _state(desc, 85)
+    Po = P0.copy()
+    with Block(desc) as blk:
+        blk.set_viscous_flux(os.path.join(src, "viscous_constant.hsaco"), "mara_viscous_constant", (0.01,))
+        blk.upload(P0)
+        for _ in range(4):
+            dg = 0.3 * blk.courant_dt()
+            err, do = oracle.courant_dt(desc, Po)
+            assert err == 0 and dg == 0.3 * do
+            blk.advance(dg)
+            err, _ = oracle.advance(desc, Po, 0.0, dg, None, ("constant", [0.01]))
+            assert err == 0
+        Pg = blk.download()
+    assert np.array_equal(Pg, Po)
