@@ -105,9 +105,14 @@ typedef struct mara_block mara_block;
 const char* mara_version(void);
 const char* mara_error_string(int status);
 
-/* Number of fields of a law (getNumConserved): 5 hydro, 8 MHD/RMHD; a
-   descriptor with a positive cooling_rate has 9 (the auxiliary field last). */
+/* Number of fields of a law (getNumConserved): 5 hydro, 8 MHD/RMHD. */
 int mara_num_fields(int law);
+
+/* Number of fields of a descriptor's state (NewtonianMHD::getNumConserved,
+   src/ConservationLaws.cpp:503-506): mara_num_fields(law), or 9 for the MHD
+   law with a positive cooling_rate (the auxiliary field last); -1 for a
+   null descriptor. */
+int mara_desc_num_fields(const mara_desc* desc);
 
 /* Stencil size of a reconstruction (IntercellFluxScheme::getStencilSize). */
 int mara_stencil_size(int recon);

@@ -368,6 +368,8 @@ const char* mara_error_string(int status)
 }
 
 int mara_num_fields(int law) { return law_nq(law); }
+
+int mara_desc_num_fields(const mara_desc* d) { return d ? law_nq(law_code(*d)) : -1; }
 int mara_stencil_size(int recon) { return recon_ng(recon); }
 
 long long mara_desc_device_bytes(const mara_desc* d)

@@ -42,6 +42,7 @@ def load():
     L.mara_error_string.restype = ctypes.c_char_p
     L.mara_error_string.argtypes = [ctypes.c_int]
     L.mara_num_fields.argtypes = [ctypes.c_int]
+    L.mara_desc_num_fields.argtypes = [ctypes.POINTER(MaraDesc)]
     L.mara_stencil_size.argtypes = [ctypes.c_int]
     L.mara_desc_device_bytes.argtypes = [ctypes.POINTER(MaraDesc)]
     L.mara_desc_device_bytes.restype = ctypes.c_longlong

@@ -67,6 +67,10 @@ def test_error_strings_and_sizes():
     assert lib.mara_error_string(abi.ERR_GUARD_TOO_SMALL).decode() == \
         "Boundary region of mesh data is smaller than the scheme's stencil"
     assert lib.mara_num_fields(abi.LAW_HYDRO) == 5 and lib.mara_num_fields(abi.LAW_RMHD) == 8
+    for kw, nq in ((dict(law="mhd"), 8), (dict(law="mhd", cooling_rate=0.5), 9), (dict(law="mhd", cooling_rate=-1.0), 8),
+                   (dict(law="hydro"), 5)):
+        d = abi.make_desc(**kw)
+        assert lib.mara_desc_num_fields(ctypes.byref(d)) == nq == abi.desc_num_fields(d), kw
     assert [lib.mara_stencil_size(r) for r in (0, 1, 2)] == [1, 2, 3]
 
 
