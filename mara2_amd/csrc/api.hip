@@ -247,24 +247,36 @@ void launch_stage(mara_block* b, int rk, int courant_out)
 
 // ---- tuning table ----------------------------------------------------------
 // Shapes chosen by the autotuner, keyed by what selects and sizes a block's
-// stage kernels: law, reconstruction, Riemann solver, flux-CT, precision, RK
-// order and cell counts.
+// stage kernels: the kernel law (cooling included), reconstruction, Riemann
+// solver, flux-CT, precision, RK order, cell counts and the stage path the
+// block takes (the same knob means different tiles on different paths).
 std::mutex tune_mutex;
 std::map<std::string, mara_shape> tune_table;
 
-std::string tune_key(const mara_desc& d)
+enum StagePath { PATH_FLUX_ARRAYS = 0, PATH_FUSED = 1, PATH_PENCIL = 2, PATH_MHD2D = 3 };
+
+// the path a block with no source or viscous hook takes
+int desc_path(const mara_desc& d)
+{
+    if (fused_supported(&d)) return PATH_FUSED;
+    if (pencil_supported(&d)) return PATH_PENCIL;
+    if (mhd2d_supported(&d)) return PATH_MHD2D;
+    return PATH_FLUX_ARRAYS;
+}
+
+std::string tune_key(const mara_desc& d, int path)
 {
     char k[160];
-    std::snprintf(k, sizeof k, "%d %d %d %d %d %d %d %d %d", d.law, d.recon, d.riemann,
+    std::snprintf(k, sizeof k, "%d %d %d %d %d %d %d %d %d p%d", law_code(d), d.recon, d.riemann,
                   d.law != MARA_LAW_HYDRO && d.field_ct ? 1 : 0, d.precision, d.rk_order, d.cells[0], d.cells[1],
-                  d.cells[2]);
+                  d.cells[2], path);
     return k;
 }
 
 bool tune_find(const mara_desc& d, mara_shape* s)
 {
     std::lock_guard<std::mutex> lock(tune_mutex);
-    auto it = tune_table.find(tune_key(d));
+    auto it = tune_table.find(tune_key(d, desc_path(d)));
     if (it == tune_table.end()) return false;
     *s = it->second;
     return true;
@@ -585,7 +597,9 @@ int mara_block_autotune(mara_block* b, int repeats, mara_shape* chosen, double* 
     apply_shape(b, pick);
     {
         std::lock_guard<std::mutex> lock(tune_mutex);
-        tune_table[tune_key(b->d)] = pick;
+        // the path the block took while it was timed (a hook moves it to the flux arrays)
+        const int path = b->fused ? PATH_FUSED : b->pencil ? PATH_PENCIL : b->mhd2d ? PATH_MHD2D : PATH_FLUX_ARRAYS;
+        tune_table[tune_key(b->d, path)] = pick;
     }
     if (chosen) *chosen = pick;
     if (best_ms) *best_ms = best;
@@ -604,7 +618,8 @@ int mara_tune_save(const char* path)
     std::FILE* f = std::fopen(path, "w");
     if (!f) return MARA_ERR_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lock(tune_mutex);
-    std::fprintf(f, "# law recon riemann ct precision rk_order n0 n1 n2 : pencil zplanes rows\n");
+    std::fprintf(f, "# kernel-law recon riemann ct precision rk_order n0 n1 n2 p<path> : pencil zplanes rows\n"
+                    "# path: 0 flux arrays, 1 fused hydro, 2 1-D pencils, 3 fused 2-D MHD\n");
     for (auto& e : tune_table)
         std::fprintf(f, "%s : %d %d %d\n", e.first.c_str(), e.second.pencil, e.second.zplanes, e.second.rows);
     std::fclose(f);
@@ -622,18 +637,18 @@ int mara_tune_load(const char* path)
     while (std::fgets(line, sizeof line, f))
     {
         if (line[0] == '#' || line[0] == '\n') continue;
-        int k[9];
+        int k[10];
         mara_shape s;
-        if (std::sscanf(line, "%d %d %d %d %d %d %d %d %d : %d %d %d", &k[0], &k[1], &k[2], &k[3], &k[4], &k[5],
-                        &k[6], &k[7], &k[8], &s.pencil, &s.zplanes, &s.rows) != 12
-            || !valid_shape(s))
+        if (std::sscanf(line, "%d %d %d %d %d %d %d %d %d p%d : %d %d %d", &k[0], &k[1], &k[2], &k[3], &k[4], &k[5],
+                        &k[6], &k[7], &k[8], &k[9], &s.pencil, &s.zplanes, &s.rows) != 13
+            || !valid_shape(s) || k[9] < PATH_FLUX_ARRAYS || k[9] > PATH_MHD2D)
         {
             status = MARA_ERR_INVALID_ARGUMENT;
             break;
         }
         char key[160];
-        std::snprintf(key, sizeof key, "%d %d %d %d %d %d %d %d %d", k[0], k[1], k[2], k[3], k[4], k[5], k[6], k[7],
-                      k[8]);
+        std::snprintf(key, sizeof key, "%d %d %d %d %d %d %d %d %d p%d", k[0], k[1], k[2], k[3], k[4], k[5], k[6],
+                      k[7], k[8], k[9]);
         got[key] = s;
     }
     std::fclose(f);

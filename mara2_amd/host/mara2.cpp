@@ -334,16 +334,34 @@ mara_block* GpuMethodOfLinesTVD::blockFor(const MeshData& md) const
     }
     if (!block)
     {
-        check(mara_block_create(&d, &block), "mara_block_create");
+        // the block is kept only once its hooks are on it: a failed hook
+        // leaves no cached block that would step without them
+        mara_block* fresh = nullptr;
+        check(mara_block_create(&d, &fresh), "mara_block_create");
+        try
+        {
+            applySource(fresh);
+        }
+        catch (...)
+        {
+            mara_block_destroy(fresh);
+            throw;
+        }
+        block = fresh;
         blockDesc = d;
         blockOwner = &md;
-        applySource(block);
     }
+    else if (!hooksApplied)
+        applySource(block);
     return block;
 }
 
 void GpuMethodOfLinesTVD::applySource(mara_block* b) const
 {
+    hooksApplied = false;
+    if (!sourcePath.empty() && fieldOperator && fieldOperator->getConservationLaw() &&
+        fieldOperator->getConservationLaw()->getCoolingRate() > 0.0)
+        throw std::logic_error("a source-terms function and NewtonianMHD cooling on one scheme are not supported");
     if (sourcePath.empty())
         check(mara_block_set_source(b, nullptr, nullptr, nullptr, 0), "mara_block_set_source");
     else
@@ -357,26 +375,57 @@ void GpuMethodOfLinesTVD::applySource(mara_block* b) const
         check(mara_block_set_viscous_flux(b, viscousPath.c_str(), viscousKernel.c_str(), viscousParams.data(),
                                           (int)viscousParams.size()),
               "mara_block_set_viscous_flux");
+    hooksApplied = true;
 }
 
 void GpuMethodOfLinesTVD::setSourceTermsCodeObject(const std::string& path, const std::string& kernel,
                                                    std::vector<double> params)
 {
     if (params.size() > MARA_SOURCE_MAX_PARAMS) throw std::logic_error("at most 16 source-term parameters");
+    auto law = fieldOperator ? fieldOperator->getConservationLaw() : nullptr;
+    if (!path.empty() && law && law->getCoolingRate() > 0.0)
+        throw std::logic_error("a source-terms function and NewtonianMHD cooling on one scheme are not supported");
+    const std::string oldPath = sourcePath, oldKernel = sourceKernel;
+    const std::vector<double> oldParams = sourceParams;
     sourcePath = path;
     sourceKernel = kernel;
     sourceParams = std::move(params);
-    if (block) applySource(block);
+    if (!block) return;
+    try
+    {
+        applySource(block);
+    }
+    catch (...)
+    {
+        // keep the previous hooks; the next use of the block puts them back
+        sourcePath = oldPath;
+        sourceKernel = oldKernel;
+        sourceParams = oldParams;
+        throw;
+    }
 }
 
 void GpuMethodOfLinesTVD::setViscousFluxCodeObject(const std::string& path, const std::string& kernel,
                                                    std::vector<double> params)
 {
     if (params.size() > MARA_SOURCE_MAX_PARAMS) throw std::logic_error("at most 16 viscous-flux parameters");
+    const std::string oldPath = viscousPath, oldKernel = viscousKernel;
+    const std::vector<double> oldParams = viscousParams;
     viscousPath = path;
     viscousKernel = kernel;
     viscousParams = std::move(params);
-    if (block) applySource(block);
+    if (!block) return;
+    try
+    {
+        applySource(block);
+    }
+    catch (...)
+    {
+        viscousPath = oldPath;
+        viscousKernel = oldKernel;
+        viscousParams = oldParams;
+        throw;
+    }
 }
 
 void GpuMethodOfLinesTVD::setStarParticleData(const std::array<double, 8>& t)

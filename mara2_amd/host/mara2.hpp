@@ -362,6 +362,7 @@ private:
     mutable mara_block* block = nullptr;
     mutable mara_desc blockDesc{};
     mutable const MeshData* blockOwner = nullptr;
+    mutable bool hooksApplied = false;  // the block carries the current source and viscous hooks
     mutable mara_report report{};
 };
 

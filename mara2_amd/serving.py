@@ -21,7 +21,7 @@ from collections import deque
 
 import numpy as np
 
-from . import problems
+from . import abi, problems
 from .runtime import Batch, Block, MaraError
 
 T_FINAL = 0.2
@@ -110,15 +110,20 @@ def sequential(requests, keep_result=False, graphs=True):
 
 
 def serve(requests, hbm_budget, max_solves=1024, chunk=16, rk_order=2, precision="mixed",
-          keep_results=False, log=None, clock=time.perf_counter, graphs=True, shape=None, express=0):
+          keep_results=False, log=None, clock=time.perf_counter, graphs=True, shape=None, express=0,
+          batch_factory=Batch):
     """Continuous batching over a request stream with arrival times set;
     shape (optional) is the batch's launch shape.  Returns (wall seconds,
-    list of requests with admitted/finished stamps)."""
+    list of requests with admitted/finished stamps).  A request the pool
+    cannot take even when it is empty (larger than the HBM budget) is not
+    waited on: it finishes at once with status MARA_ERR_CAPACITY.
+    batch_factory(hbm_budget, rk_order=, precision=, max_solves=) makes the
+    batch (the device Batch by default)."""
     pending = deque(sorted(requests, key=lambda r: r.arrival))
     waiting = []
     in_flight = {}
     done = []
-    with Batch(hbm_budget, rk_order=rk_order, precision=precision, max_solves=max_solves) as batch:
+    with batch_factory(hbm_budget, rk_order=rk_order, precision=precision, max_solves=max_solves) as batch:
         batch.set_graphs(graphs)
         batch.set_express(express)
         if shape is not None:
@@ -140,6 +145,12 @@ def serve(requests, hbm_budget, max_solves=1024, chunk=16, rk_order=2, precision
                     in_flight[slot] = r
             waiting = still
             if not in_flight:
+                # the pool is empty: whatever still waits can never be admitted
+                for r in waiting:
+                    r.admitted = r.finished = clock() - t0
+                    r.status = abi.ERR_CAPACITY
+                    done.append(r)
+                waiting = []
                 if pending:
                     time.sleep(max(0.0, min(pending[0].arrival - (clock() - t0), 0.05)))
                 continue

@@ -58,5 +58,16 @@ int main()
         mhd->setCoolingRate(-1.0);
         if (mhd->getNumConserved() != 8) throw std::runtime_error("expected 8 fields with cooling off");
     });
+    // a source function on a scheme whose law cools is refused before any
+    // block is touched (the reference adds cooling to L before the
+    // function's S, an order one S array cannot keep)
+    auto cooled = std::make_shared<NewtonianMHD>();
+    cooled->setCoolingRate(0.3);
+    auto ss2 = std::make_shared<GpuMethodOfLinesTVD>();
+    ss2->setFieldOperator(std::make_shared<FieldOperator>(cooled));
+    bad += expect_logic_error("source with cooling", [&] {
+        ss2->setSourceTermsCodeObject("any.hsaco", "any_kernel", {1.0});
+    });
+    bad += expect_no_error("removing a source with cooling", [&] { ss2->setSourceTermsCodeObject("", ""); });
     return bad == 0 ? 0 : 1;
 }

@@ -105,20 +105,24 @@ def test_device_bytes_estimate():
 
 def test_tuning_table_file_round_trip(tmp_path):
     """mara_tune_load / lookup / save without a GPU: entries are keyed by the
-    descriptor's kernel selectors and cell counts, a bad shape is refused."""
+    descriptor's kernel law (cooling included), kernel selectors, cell counts
+    and stage path (p2: 1-D pencils), a bad shape is refused."""
     d = abi.make_desc(law="mhd", recon="plm", riemann="hlle", bc="periodic", cells=(96, 1, 1), field_ct=1)
     other = abi.make_desc(law="mhd", recon="plm", riemann="hlle", bc="periodic", cells=(97, 1, 1), field_ct=1)
+    cooled = abi.make_desc(law="mhd", recon="plm", riemann="hlle", bc="periodic", cells=(96, 1, 1), field_ct=1,
+                           cooling_rate=0.5)
     f = tmp_path / "tune.txt"
-    f.write_text("# comment\n1 1 0 1 0 2 96 1 1 : 512 8 2\n")
+    f.write_text("# comment\n1 1 0 1 0 2 96 1 1 p2 : 512 8 2\n")
     runtime.tune_load(str(f))
     got = runtime.tune_lookup(d)
     assert got is not None and got.as_dict() == {"pencil": 512, "zplanes": 8, "rows": 2}
     assert runtime.tune_lookup(other) is None
+    assert runtime.tune_lookup(cooled) is None
     out = tmp_path / "saved.txt"
     runtime.tune_save(str(out))
-    assert "1 1 0 1 0 2 96 1 1 : 512 8 2" in out.read_text().splitlines()
+    assert "1 1 0 1 0 2 96 1 1 p2 : 512 8 2" in out.read_text().splitlines()
     bad = tmp_path / "bad.txt"
-    bad.write_text("1 1 0 1 0 2 96 1 1 : 300 8 2\n")
+    bad.write_text("1 1 0 1 0 2 96 1 1 p2 : 300 8 2\n")
     with pytest.raises(runtime.MaraError):
         runtime.tune_load(str(bad))
     with pytest.raises(runtime.MaraError):

@@ -22,4 +22,4 @@ def test_host_layer_configuration_errors(tmp_path):
                     f"-Wl,-rpath,{PKG}"], check=True, capture_output=True, text=True)
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stdout + r.stderr
-    assert r.stdout.count("ok ") == 4, r.stdout
+    assert r.stdout.count("ok ") == 6, r.stdout

@@ -55,9 +55,11 @@ def test_rmhd_c2p_golden():
     assert np.mean(err == 0) > 0.95
 
 
-@pytest.mark.skipif(oracle.ref() is None, reason="reference bridge oracle/_ref not built (no reference tree)")
 def test_fresh_random_against_reference_bridge():
+    # loaded here, not at collection: a run that deselects this test never maps the bridge
     R = oracle.ref()
+    if R is None:
+        pytest.skip("reference bridge oracle/_ref not built (no reference tree)")
     rng = np.random.default_rng()
     for _ in range(2000):
         v = rng.standard_normal(5) * np.exp(rng.uniform(-8, 8))

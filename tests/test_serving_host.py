@@ -91,3 +91,74 @@ def test_work_bound():
     # 2-D 1024^2 at 7e3 steps
     assert 16384 * 160000 <= serving.MAX_ZONE_UPDATES and 160000 <= serving.MAX_STEPS
     assert 1024 * 1024 * 7000 <= serving.MAX_ZONE_UPDATES
+
+
+class _FakeBatch:
+    """Stands in for the device batch: admits a solve while its bytes fit the
+    budget (item bytes = zones * 100), finishes every admitted solve on its
+    first step."""
+
+    def __init__(self, hbm_budget, rk_order=2, precision="mixed", max_solves=8):
+        self.budget, self.used, self.slots, self.max = hbm_budget, 0, {}, max_solves
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        pass
+
+    def set_graphs(self, on):
+        pass
+
+    def set_express(self, n):
+        pass
+
+    def set_shape(self, s):
+        pass
+
+    @property
+    def free_bytes(self):
+        return self.budget - self.used
+
+    def admit(self, desc, P, cfl, t_final, max_steps):
+        nbytes = int(np.prod(list(desc.cells))) * 100
+        if self.used + nbytes > self.budget or len(self.slots) >= self.max:
+            return None
+        slot = min(set(range(self.max)) - set(self.slots))
+        self.slots[slot] = nbytes
+        self.used += nbytes
+        return slot
+
+    def step(self, n):
+        return 0
+
+    def finished(self):
+        return sorted(self.slots)
+
+    def query(self, slot):
+        rep = abi.MaraReport()
+        return 1, serving.T_FINAL, 10, 0, rep
+
+    def retire(self, slot, download=True):
+        self.used -= self.slots.pop(slot)
+        return None
+
+
+def test_serve_finishes_when_a_request_can_never_be_admitted():
+    """A request larger than the whole budget is reported with
+    MARA_ERR_CAPACITY instead of keeping the loop alive; the others are
+    served."""
+    reqs = serving.make_stream(6, with_data=False, resize=lambda dim, n: 300 if dim == 1 else 20)
+    for r in reqs:
+        r.P = np.zeros(abi.array_shape(r.desc))
+    big = serving.make_stream(1, with_data=False, resize=lambda dim, n: 100000 if dim == 1 else 1000)[0]
+    big.index = 99
+    big.P = None
+    reqs.append(big)
+    serving.set_arrivals(reqs, rate=1e6)
+    budget = 300 * 100 * 2
+    wall, done = serving.serve(reqs, budget, max_solves=8, batch_factory=_FakeBatch)
+    assert len(done) == len(reqs)
+    by_index = {r.index: r for r in done}
+    assert by_index[99].status == abi.ERR_CAPACITY
+    assert all(r.status == 0 for r in done if r.index != 99)
