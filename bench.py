@@ -87,6 +87,33 @@ def time_form(desc, P0, steps, warmup, transfers=None, tune=None):
     return t1 - t0, P, avg_ms, launches, name
 
 
+def fast_vs_plain_after(d64, dfast, P0, steps):
+    """Relative L1 of the conserved interior between the fast and the plain
+    binary64 forms after `steps` steps of the plain form's dt."""
+    res = []
+    dts = []
+    for d in (d64, dfast):
+        with Block(d) as blk:
+            blk.upload(P0)
+            for s in range(steps):
+                if d is d64:
+                    dts.append(CFL * blk.courant_dt())
+                blk.advance(dts[s])
+            res.append(blk.download())
+    g = d64.guards
+    inner = tuple(slice(g[a], g[a] + d64.cells[a]) for a in range(3))
+
+    def cons(P):
+        P = P[inner]
+        U = np.empty_like(P)
+        U[..., 0] = P[..., 0]
+        U[..., 1:4] = P[..., :1] * P[..., 1:4]
+        U[..., 4] = 0.5 * P[..., 0] * np.sum(P[..., 1:4] ** 2, axis=-1) + P[..., 4] / (d64.gamma - 1)
+        return U
+    a, b = cons(res[1]), cons(res[0])
+    return max(float(np.sum(np.abs(a[..., q] - b[..., q])) / np.sum(np.abs(b[..., q]))) for q in range(5))
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -410,6 +437,11 @@ def main():
     tunemx = None if args.no_autotune else {}
     el64, P64, ms64, nl64, kname = time_form(d64, P0, args.steps, args.warmup, transfers, tune64)
     log(f"FP64: {el64:.3f} s for {args.steps} steps; stage kernel {ms64:.3f} ms ({kname}); shape {tune64}")
+    dfast = problems.turbulence_desc(n, "fp64_fast")
+    tunefast = None if args.no_autotune else {}
+    log("timing fast FP64 form")
+    elfa, Pfa, msfa, nlfa, kfa = time_form(dfast, P0, args.steps, args.warmup, None, tunefast)
+    log(f"fast FP64: {elfa:.3f} s for {args.steps} steps; stage kernel {msfa:.3f} ms ({kfa}); shape {tunefast}")
     log("timing mixed form")
     elmx, Pmx, msmx, nlmx, _ = time_form(dmx, P0, args.steps, args.warmup, None, tunemx)
     log(f"mixed: shape {tunemx}")
@@ -457,6 +489,13 @@ def main():
                   "avg_stage_launch_ms": msmx,
                   "roofline_frac": per_launch / (msmx * 1e-3) / HBM_PEAK},
     }
+    # the fast form against the plain one (bit-identical to the reference
+    # restatement) after 10 steps of the same dt: the FP64 bound of 1e-12
+    l1_fast10 = fast_vs_plain_after(d64, dfast, P0, 10)
+    out["fp64_fast"] = {"value": zones * args.steps / elfa, "ms_per_step": elfa / args.steps * 1e3,
+                        "speedup_vs_fp64": el64 / elfa, "avg_stage_launch_ms": msfa, "kernel": kfa,
+                        "roofline_frac": per_launch / (msfa * 1e-3) / HBM_PEAK,
+                        "rel_l1_vs_fp64_after_10_steps": l1_fast10, "launch_shape": tunefast}
     if not args.no_cpu_baseline:
         log("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample)

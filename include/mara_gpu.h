@@ -61,6 +61,15 @@ extern "C" {
 #define MARA_PREC_MIXED  1  /* reconstruction, Riemann fluxes and flux divergence
                                in binary32; RK accumulation, flux-CT, primitive
                                recovery and the RMHD quartic in binary64 */
+#define MARA_PREC_FP64_FAST 2 /* every operation in binary64, but not the
+                               reference's rounding: fused multiply-add,
+                               reciprocals and square roots refined from the
+                               hardware approximations, axis-aligned face
+                               normals; held to relative L1 <= 1e-12 of the
+                               reference's conserved fields after 10 steps.
+                               Hydro blocks with PLM on 2-D and 3-D meshes run
+                               their own stage kernel; every other block runs
+                               the MARA_PREC_FP64 kernels.  Single blocks only. */
 
 /* ---- block descriptor ---------------------------------------------------- */
 typedef struct mara_desc

@@ -17,13 +17,13 @@ LAW_HYDRO, LAW_MHD, LAW_RMHD = 0, 1, 2
 RECON_PCM, RECON_PLM, RECON_WENO5 = 0, 1, 2
 RIEMANN_HLLE, RIEMANN_HLLC = 0, 1
 BC_PERIODIC, BC_OUTFLOW, BC_REFLECTING = 0, 1, 2
-PREC_FP64, PREC_MIXED = 0, 1
+PREC_FP64, PREC_MIXED, PREC_FP64_FAST = 0, 1, 2
 
 LAWS = {"hydro": LAW_HYDRO, "mhd": LAW_MHD, "rmhd": LAW_RMHD}
 RECONS = {"pcm": RECON_PCM, "plm": RECON_PLM, "weno5": RECON_WENO5}
 RIEMANNS = {"hlle": RIEMANN_HLLE, "hllc": RIEMANN_HLLC}
 BCS = {"periodic": BC_PERIODIC, "outflow": BC_OUTFLOW, "reflecting": BC_REFLECTING}
-PRECISIONS = {"fp64": PREC_FP64, "mixed": PREC_MIXED}
+PRECISIONS = {"fp64": PREC_FP64, "mixed": PREC_MIXED, "fp64_fast": PREC_FP64_FAST}
 
 
 def num_fields(law):

@@ -9,6 +9,7 @@
 // kernels.hpp.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -169,6 +170,16 @@ void launch_courant(mara_block* b)
     }
 }
 
+// The stage kernels of the block's form: binary32 fluxes (mixed), the fast
+// binary64 kernel where it serves the block (its fused hydro PLM stage), else
+// the plain binary64 kernels.
+void run_stage_kernels(const StageArgs& s, bool mixed)
+{
+    if (mixed) stage_kernels_f32(s);
+    else if (s.d->precision == MARA_PREC_FP64_FAST && s.fused && fast64_supported(s.d)) stage_kernels_fast64(s);
+    else stage_kernels<double>(s);
+}
+
 // One RK stage: fluxes, CT, update with ghost fill.  With courant_out the last
 // stage also reduces the next step's CFL time step into the control bins.
 void launch_stage(mara_block* b, int rk, int courant_out)
@@ -233,8 +244,7 @@ void launch_stage(mara_block* b, int rk, int courant_out)
     b->vis_args.P = Pin;
     s.after_flux = b->vis_fn ? launch_viscous : nullptr;
     s.after_flux_ctx = b;
-    if (mixed) stage_kernels_f32(s);
-    else stage_kernels<double>(s);
+    run_stage_kernels(s, mixed);
     if (timed)
     {
         hipEventRecord(b->ev[2 * b->ev_used + 1], b->stream);
@@ -569,8 +579,7 @@ int mara_block_autotune(mara_block* b, int repeats, mara_shape* chosen, double* 
         for (int r = 0; r <= repeats; ++r)
         {
             if (r == 1) (void)hipEventRecord(e0, b->stream);
-            if (mixed) stage_kernels_f32(s);
-            else stage_kernels<double>(s);
+            run_stage_kernels(s, mixed);
         }
         (void)hipEventRecord(e1, b->stream);
         if (!hip_ok(hipGetLastError()) || !hip_ok(hipEventSynchronize(e1)))
@@ -974,3 +983,4 @@ int mara_block_stage_kernel_time(mara_block* b, double* avg_ms, long long* launc
 }
 
 } // extern "C"
+

@@ -62,7 +62,7 @@ inline int validate(const mara_desc* d)
     if (d->riemann < 0 || d->riemann > 1) return MARA_ERR_INVALID_ARGUMENT;
     if (d->bc < 0 || d->bc > 2) return MARA_ERR_INVALID_ARGUMENT;
     if (d->rk_order < 1 || d->rk_order > 3) return MARA_ERR_INVALID_ARGUMENT;
-    if (d->precision < 0 || d->precision > 1) return MARA_ERR_INVALID_ARGUMENT;
+    if (d->precision < 0 || d->precision > 2) return MARA_ERR_INVALID_ARGUMENT;
     if (d->riemann == MARA_RIEMANN_HLLC && d->law != MARA_LAW_HYDRO) return MARA_ERR_UNSUPPORTED;
     // the cooling source is a NewtonianMHD option
     if (d->cooling_rate > 0.0 && d->law != MARA_LAW_MHD) return MARA_ERR_UNSUPPORTED;

@@ -215,4 +215,9 @@ void stage_kernels(const StageArgs& s)
 // binary32 instantiation (mixed.hip)
 void stage_kernels_f32(const StageArgs& s);
 
+// the fast binary64 form's stage kernel (fast.hip) for the blocks it serves
+// (fast64_supported); the caller runs stage_kernels<double> for the others
+void stage_kernels_fast64(const StageArgs& s);
+bool fast64_supported(const mara_desc* d);
+
 } // namespace mara

@@ -123,6 +123,20 @@ def test_mhd_2d_ct(ct):
         assert mg < 1e-12
 
 
+def _monopole_pair(desc, Pg, Po):
+    """max |monopole| of the GPU state (mara_block_monopole_max) and of the
+    oracle's; both must sit at binary64 roundoff relative to max |B|."""
+    import oracle
+    with Block(desc) as blk:
+        blk.upload(Pg)
+        mg = blk.monopole_max()
+    err, mo = oracle.monopole_max(desc, Po)
+    assert err == 0
+    bmax = np.abs(Pg[..., 5:8]).max()
+    assert mg <= 1e-12 * bmax and mo <= 1e-12 * bmax, (mg, mo, bmax)
+    return mg, mo
+
+
 def test_mhd_3d_ct():
     desc = abi.make_desc(law="mhd", recon="plm", riemann="hlle", bc="periodic",
                          cells=(16, 16, 16), rk_order=2, field_ct=1)
@@ -131,17 +145,45 @@ def test_mhd_3d_ct():
     Pg, Po, dg, do = run_pair(desc, P0, 6, 0.3, Block)
     assert dg == do
     _check_fp64(desc, Pg, Po)
+    mg, mo = _monopole_pair(desc, Pg, Po)
+    assert mg == mo
 
 
 def test_rmhd_3d_weno_ct():
     """The 128^3 workload scaled to 32^3: modes kept at >= 16 zones per
     wavelength (kmax = 2) and max W = 3, where the reference scheme runs
-    without inversion failures."""
+    without inversion failures; 10 steps, relative L1 <= 1e-12 against the
+    oracle, corner monopole at roundoff."""
     desc = problems.rmhd_desc(32)
     P0 = problems.rmhd_3d(desc, kmax=2, wmax=3.0)
-    Pg, Po, dg, do = run_pair(desc, P0, 4, 0.3, Block)
+    Pg, Po, dg, do = run_pair(desc, P0, 10, 0.3, Block)
     assert abs(dg[0] - do[0]) <= 1e-14 * do[0]
     _check_fp64(desc, Pg, Po, exact=False)
+    _monopole_pair(desc, Pg, Po)
+
+
+def test_monopole_of_a_linear_field_is_its_closed_form():
+    """B = (a x, b y, c z): every interior cell's corner monopole is
+    a dx + b dy + c dz (meshSpacing 1, src/CellCenteredFieldCT.cpp:11,205),
+    and one sixth of the in-plane value on a 2-D block (the vertex array's
+    k = 0 layer is never written)."""
+    a, b, c = 0.7, -1.9, 2.3
+    for cells, upper, want in (((6, 5, 4), (1.2, 0.5, 2.0), a * 0.2 + b * 0.1 + c * 0.5),
+                               ((8, 6, 1), (1.0, 3.0, 1.0), (a / 8 + b * 0.5) / 6)):
+        desc = abi.make_desc(law="mhd", recon="plm", cells=cells, upper=upper)
+        P = problems.empty(desc)
+        P[..., 0] = 1.0
+        P[..., 4] = 1.0
+        x = problems.cell_centres(desc, 0)[:, None, None]
+        y = problems.cell_centres(desc, 1)[None, :, None]
+        z = problems.cell_centres(desc, 2)[None, None, :]
+        P[..., 5] = a * x + 0 * (y + z)
+        P[..., 6] = b * y + 0 * (x + z)
+        P[..., 7] = (c if cells[2] > 1 else 0.0) * z + 0 * (x + y)
+        with Block(desc) as blk:
+            blk.upload(P)
+            m = blk.monopole_max()
+        np.testing.assert_allclose(m, abs(want), rtol=1e-12)
 
 
 def test_rmhd_1d_shocktube():
