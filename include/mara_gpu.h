@@ -269,6 +269,28 @@ long long mara_block_device_bytes(const mara_block* block);
 long long mara_desc_device_bytes(const mara_desc* desc);
 
 /* ---------------------------------------------------------------------------
+   One step with no device allocation.  No counterpart in the reference: it
+   serves callers that cannot allocate device memory (code under HIP graph
+   capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+   arithmetic as mara_block_advance on a block created from desc and loaded
+   with P_in, with every working buffer (the state in the device layout, its
+   stage copies, the RK state, flux arrays where the path needs them, the
+   control block and the CFL bins) carved from one statically sized device
+   array of MARA_SCRATCH_BYTES per GPU, which the library holds from its
+   first use on that GPU until the process ends.  P_in and P_out are device
+   pointers to primitives in MeshData::P layout (guards included); they may
+   be the same array.  stream is a hipStream_t (NULL: the null stream); the
+   call waits for it before returning.  MARA_ERR_CAPACITY when the
+   descriptor's buffers exceed MARA_SCRATCH_BYTES.  Every call on one GPU
+   shares that GPU's array: calls that overlap on one GPU (other host
+   threads, other streams) race and must not be made.  The launch shape is
+   the default one (the tuning table is not consulted).
+   --------------------------------------------------------------------------- */
+#define MARA_SCRATCH_BYTES (32ll * 1024 * 1024)
+int mara_advance_noalloc(const mara_desc* desc, const double* P_in, double* P_out, double t0, double dt,
+                         void* stream, mara_report* report);
+
+/* ---------------------------------------------------------------------------
    Batches: many independent solves resident in one pool of device memory,
    advanced together with one launch per kernel type and solve class per RK
    stage.  Each solve keeps the single block's arithmetic, its own time step,

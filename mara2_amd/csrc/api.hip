@@ -984,3 +984,167 @@ int mara_block_stage_kernel_time(mara_block* b, double* avg_ms, long long* launc
 
 } // extern "C"
 
+// ===========================================================================
+// One step with no device allocation (mara_advance_noalloc)
+// ===========================================================================
+// Every working buffer of the step lives in this file-scope array, one copy
+// per device, held from the code object's load until the process ends.  In
+// host code the name `scratch` is a host-side stand-in, not a device
+// address: the address of a device's copy comes from hipGetSymbolAddress,
+// taken once per device with that device current, and only that address is
+// passed to kernels.
+static __device__ __align__(256) unsigned char scratch[MARA_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kScratchDevices = 64;
+std::mutex scratch_mutex;
+unsigned char* scratch_ptr[kScratchDevices] = {};
+
+// the current device's copy of the scratch array
+int scratch_on_device(int dev, unsigned char** out)
+{
+    if (dev < 0 || dev >= kScratchDevices) return MARA_ERR_NO_DEVICE;
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (!scratch_ptr[dev])
+    {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+        {
+            (void)hipGetLastError();
+            return MARA_ERR_HIP;
+        }
+        scratch_ptr[dev] = static_cast<unsigned char*>(p);
+    }
+    *out = scratch_ptr[dev];
+    return MARA_OK;
+}
+
+// Byte offsets of a step's buffers in the scratch array, each 256-aligned:
+// the same buffers mara_block_create allocates for the descriptor.
+struct ScratchPlan
+{
+    long long Pa, Pb, Pc, U, F[3], S, ctl, cbins, total;
+};
+
+ScratchPlan plan_scratch(const mara_desc& d, const Sizes& s, bool in_pass)
+{
+    ScratchPlan p;
+    long long at = 0;
+    auto take = [&](long long bytes) {
+        const long long here = at;
+        at += round_up(bytes, 256);
+        return here;
+    };
+    const long long cellbytes = (long long)s.nq * s.fs * 8;
+    p.Pa = take(cellbytes);
+    p.Pb = take(cellbytes);
+    p.U = take(cellbytes);
+    p.Pc = in_pass && d.rk_order == 3 ? take(cellbytes) : -1;
+    for (int a = 0; a < 3; ++a) p.F[a] = !in_pass && s.fleshed[a] ? take((long long)s.nq * s.ffs * s.real) : -1;
+    p.S = s.nq == 9 ? take(cellbytes) : -1;
+    p.ctl = take(sizeof(Ctl));
+    p.cbins = take((long long)CBINS * CBIN_STRIDE * 8);
+    p.total = at;
+    return p;
+}
+
+} // namespace
+
+// the control block of a fresh step: counters at zero, the CFL bins reset
+static __global__ void k_scratch_ctl_reset(Ctl* ctl, unsigned long long* cbins)
+{
+    const int t = threadIdx.x;
+    if (t == 0)
+    {
+        ctl->dt = 0.0;
+        ctl->t = 0.0;
+        ctl->t_final = 0.0;
+        ctl->cfl = 0.0;
+        ctl->courant_bits = DBL_MAX_BITS;
+        ctl->monopole_bits = 0;
+        ctl->first_fail = ~0ull;
+        ctl->c2p_fail = 0;
+        ctl->floor_applied = 0;
+        ctl->eigen_fail = 0;
+        ctl->superluminal = 0;
+        ctl->active = 1;
+        ctl->steps = 0;
+        ctl->max_steps = 0;
+        ctl->pad = 0;
+        ctl->cbins = cbins;
+    }
+    for (int b = t; b < CBINS; b += blockDim.x) cbins[b * CBIN_STRIDE] = DBL_MAX_BITS;
+}
+
+static __global__ void __launch_bounds__(256) k_scratch_zero(double* __restrict__ p, long long n)
+{
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+        p[t] = 0.0;
+}
+
+extern "C" int mara_advance_noalloc(const mara_desc* d, const double* P_in, double* P_out, double t0, double dt,
+                                    void* stream, mara_report* report)
+{
+    (void)t0;
+    if (!P_in || !P_out) return MARA_ERR_INVALID_ARGUMENT;
+    const int v = validate(d);
+    if (v) return v;
+    // the block's state, set up as mara_block_create does, on the stack; its
+    // buffers are carved from the scratch array below
+    mara_block b;
+    b.d = *d;
+    b.S = sizes_of(d);
+    b.ng = recon_ng(d->recon);
+    b.fused = fused_supported(d);
+    b.pencil = pencil_supported(d) ? PENCIL_DEFAULT : 0;
+    b.mhd2d = mhd2d_supported(d);
+    if (b.mhd2d) b.rows = MHD_ROWS_DEFAULT;
+    b.G = make_geom(*d);
+    const ScratchPlan plan = plan_scratch(*d, b.S, b.fused || b.pencil || b.mhd2d);
+    if (plan.total > MARA_SCRATCH_BYTES) return MARA_ERR_CAPACITY;
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess)
+    {
+        (void)hipGetLastError();
+        return MARA_ERR_NO_DEVICE;
+    }
+    unsigned char* base = nullptr;
+    const int sa = scratch_on_device(dev, &base);
+    if (sa) return sa;
+    auto at = [&](long long off) { return off < 0 ? nullptr : reinterpret_cast<double*>(base + off); };
+    b.Pa = at(plan.Pa);
+    b.Pb = at(plan.Pb);
+    b.Pc = at(plan.Pc);
+    b.U = at(plan.U);
+    for (int a = 0; a < 3; ++a) b.F[a] = plan.F[a] < 0 ? nullptr : static_cast<void*>(base + plan.F[a]);
+    b.Ssrc = at(plan.S);
+    b.ctl = reinterpret_cast<Ctl*>(base + plan.ctl);
+    b.cbins = reinterpret_cast<unsigned long long*>(base + plan.cbins);
+    Ctl ctl_host;
+    b.ctl_host = &ctl_host;
+    b.stream = static_cast<hipStream_t>(stream);
+    b.graphs = false;
+
+    // what mara_block_create zeroes, zeroed again: the array keeps the last
+    // call's contents
+    const long long cellwords = (long long)b.S.nq * b.S.fs;
+    const unsigned zgrid = (unsigned)std::min<long long>((cellwords + 255) / 256, 4096);
+    k_scratch_ctl_reset<<<1, 64, 0, b.stream>>>(b.ctl, b.cbins);
+    k_scratch_zero<<<zgrid, 256, 0, b.stream>>>(b.Pb, cellwords);
+    if (b.Pc) k_scratch_zero<<<zgrid, 256, 0, b.stream>>>(b.Pc, cellwords);
+    if (b.Ssrc) k_scratch_zero<<<zgrid, 256, 0, b.stream>>>(b.Ssrc, cellwords);
+    // MeshData::P -> the device layout, one step, and back
+    k_aos_to_soa<<<grid_for(b.S.ncell), kThreads, 0, b.stream>>>(P_in, b.Pa, b.G);
+    k_ctl_begin_advance<<<1, 1, 0, b.stream>>>(b.ctl, dt);
+    for (int rk = 0; rk < b.d.rk_order; ++rk) launch_stage(&b, rk, 0);
+    k_soa_to_aos<<<grid_for(b.S.ncell), kThreads, 0, b.stream>>>(b.Pa, P_out, b.G);
+    int status = MARA_OK;
+    if (!hip_ok(hipGetLastError())) status = MARA_ERR_HIP;
+    else if (!hip_ok(hipMemcpyAsync(&ctl_host, b.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, b.stream)))
+        status = MARA_ERR_HIP;
+    else if (!hip_ok(hipStreamSynchronize(b.stream))) status = MARA_ERR_HIP;
+    if (status) return status;
+    return fill_report(&b, report);
+}
