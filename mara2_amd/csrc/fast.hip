@@ -23,20 +23,20 @@ void stage_kernels_fast64(const StageArgs& s)
     if (z)
     {
         if (hllc)
-            fast64::k_fast_plm<1, true><<<grid, block, 0, s.stream>>>(s.Pin, s.P0, s.U, s.Pout, G, s.b, s.first,
-                                                                     s.write_u, s.cout, s.ctl, s.kz);
+            fast64::k_fast_plm<1, true><<<grid, block, 0, s.stream>>>(s.Pin, s.P0, s.Pout, G, s.b, s.first, s.cout,
+                                                                     s.ctl, s.kz);
         else
-            fast64::k_fast_plm<0, true><<<grid, block, 0, s.stream>>>(s.Pin, s.P0, s.U, s.Pout, G, s.b, s.first,
-                                                                     s.write_u, s.cout, s.ctl, s.kz);
+            fast64::k_fast_plm<0, true><<<grid, block, 0, s.stream>>>(s.Pin, s.P0, s.Pout, G, s.b, s.first, s.cout,
+                                                                     s.ctl, s.kz);
     }
     else
     {
         if (hllc)
-            fast64::k_fast_plm<1, false><<<grid, block, 0, s.stream>>>(s.Pin, s.P0, s.U, s.Pout, G, s.b, s.first,
-                                                                      s.write_u, s.cout, s.ctl, s.kz);
+            fast64::k_fast_plm<1, false><<<grid, block, 0, s.stream>>>(s.Pin, s.P0, s.Pout, G, s.b, s.first, s.cout,
+                                                                      s.ctl, s.kz);
         else
-            fast64::k_fast_plm<0, false><<<grid, block, 0, s.stream>>>(s.Pin, s.P0, s.U, s.Pout, G, s.b, s.first,
-                                                                      s.write_u, s.cout, s.ctl, s.kz);
+            fast64::k_fast_plm<0, false><<<grid, block, 0, s.stream>>>(s.Pin, s.P0, s.Pout, G, s.b, s.first, s.cout,
+                                                                      s.ctl, s.kz);
     }
 }
 

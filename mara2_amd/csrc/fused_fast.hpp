@@ -16,7 +16,12 @@
 //     changes nothing above roundoff), the wave-speed extremes are vn -/+ cs;
 //   - HLLC forms the conserved state and flux of the contact's side only and
 //     needs one reciprocal for its star state, HLLE one per face;
-//   - the zone update takes one reciprocal of the density.
+//   - the zone update takes one reciprocal of the density;
+//   - the RK stage state is carried as the primitives alone: a stage's U is
+//     U(Pin) of the zone's own input value (where the reference carries U
+//     between stages; the two differ by the rounding of one P(U) -> U(P)
+//     round trip), so the kernel neither reads nor writes the U array, and
+//     the first stage takes U0 from that value instead of re-reading P0.
 // The form is held to relative L1 <= 1e-12 of the conserved fields against
 // the reference step after 10 steps, not to bit equality.
 #pragma once
@@ -186,9 +191,8 @@ constexpr int FTX = 64, FTY = 4;
 
 template <int Riem, bool Z>
 static __global__ void __launch_bounds__(FTX * FTY) __attribute__((amdgpu_waves_per_eu(3, 8)))
-k_fast_plm(const double* __restrict__ Pin, const double* __restrict__ P0, double* __restrict__ U,
-           double* __restrict__ Pout, Geom G, double b, int first_stage, int write_u, int courant_out, Ctl* ctl,
-           int kz)
+k_fast_plm(const double* __restrict__ Pin, const double* __restrict__ P0, double* __restrict__ Pout, Geom G,
+           double b, int first_stage, int courant_out, Ctl* ctl, int kz)
 {
     if (!ctl->active) return;
     constexpr int NG = 2, TX = FTX, TY = FTY;
@@ -424,10 +428,25 @@ k_fast_plm(const double* __restrict__ Pin, const double* __restrict__ P0, double
         if (cell_xy)
         {
             const long long c = pbase + col;
-            double p0[5], U0[5];
+            // the stage's own state U(Pin) from the zone's input value (in the z
+            // window, or the LDS plane in 2-D); the step's start state U(P0) is
+            // the same value in the first stage and read from P0 after it
+            double pin[5], Us[5], U0[5];
 #pragma unroll
-            for (int q = 0; q < 5; ++q) p0[q] = P0[q * G.fs + c];
-            cons_of(K, p0, U0);
+            for (int q = 0; q < 5; ++q) pin[q] = Z ? W[q][1] : plane[q][row + 2][lane + 2];
+            cons_of(K, pin, Us);
+            if (first_stage)
+            {
+#pragma unroll
+                for (int q = 0; q < 5; ++q) U0[q] = Us[q];
+            }
+            else
+            {
+                double p0[5];
+#pragma unroll
+                for (int q = 0; q < 5; ++q) p0[q] = P0[q * G.fs + c];
+                cons_of(K, p0, U0);
+            }
             double Un[5];
 #pragma unroll
             for (int q = 0; q < 5; ++q)
@@ -435,8 +454,7 @@ k_fast_plm(const double* __restrict__ Pin, const double* __restrict__ P0, double
                 double acc = (Fxr[q] - Fx[q]) * A0 + (fy[q][row + 1][lane] - Fy[q]) * A1;
                 if constexpr (Z) acc += (Fzhi[q] - Fzlo[q]) * A2;
                 const double L = fac * acc;
-                const double Uq = first_stage ? U0[q] : U[q * G.fs + c];
-                Un[q] = U0[q] * (1 - b) + (Uq + dt * L) * b;
+                Un[q] = U0[q] * (1 - b) + (Us[q] + dt * L) * b;
             }
             double Pn[5];
             prim_of(K, Un, Pn);
@@ -446,11 +464,6 @@ k_fast_plm(const double* __restrict__ Pin, const double* __restrict__ P0, double
             {
                 const double d = courant(K, Pn, G.Lmin);
                 if (d < dmin) dmin = d;
-            }
-            if (write_u)
-            {
-#pragma unroll
-                for (int q = 0; q < 5; ++q) U[q * G.fs + c] = Un[q];
             }
         }
         if constexpr (Z)

@@ -270,17 +270,36 @@ def rmhd_desc(n=128, precision="fp64"):
                          cells=(n, n, n), rk_order=3, field_ct=1, precision=precision)
 
 
-def rmhd_3d(desc, seed=SEED_RMHD, kmax=8, wmax=5.0, sigma=1.0):
-    """rho = 1, p = 0.1; spatial four-velocity u random on 1 <= |k| <= kmax,
-    scaled so max W = sqrt(1 + u^2) = wmax, v = u / W; B = curl A with A a
-    random vector potential on the same band, scaled to <B^2> / rho = sigma."""
+def _solenoidal(specs, ks):
+    """Project the complex amplitudes of a vector field onto the plane normal
+    to each wave vector (k . a_k = 0), so its real part is divergence-free."""
+    kx, ky, kz = np.meshgrid(ks, ks, ks, indexing="ij")
+    kv = np.stack([kx, ky, kz]).astype(np.float64)
+    k2 = np.sum(kv * kv, axis=0)
+    k2[k2 == 0] = 1.0
+    a = np.stack(specs)
+    proj = np.sum(kv * a, axis=0) / k2
+    return [a[c] - kv[c] * proj for c in range(3)]
+
+
+def rmhd_3d(desc, seed=SEED_RMHD, kmax=8, wmax=5.0, sigma=1.0, u_slope=0.0, a_slope=0.0, solenoidal=False):
+    """rho = 1, p = 0.1; spatial four-velocity u random on 1 <= |k| <= kmax
+    with power spectrum |u_k|^2 ~ k^u_slope (flat by default), optionally
+    projected to be divergence-free, scaled so max W = sqrt(1 + u^2) = wmax,
+    v = u / W; B = curl A with A a random vector potential on the same band
+    (|A_k|^2 ~ k^a_slope), scaled to <B^2> / rho = sigma."""
     n = desc.cells[0]
     rng = np.random.default_rng(seed)
-    u = np.stack([random_field(rng, 3, n, 1, kmax, 0.0) for _ in range(3)], axis=-1)
+    uspecs = [_spectrum(rng, 3, 1, kmax, u_slope) for _ in range(3)]
+    ks_u = uspecs[0][1]
+    ua = [sp for sp, _ in uspecs]
+    if solenoidal:
+        ua = _solenoidal(ua, ks_u)
+    u = np.stack([_evaluate(a, ks_u, n, (0.0, 0.0, 0.0)) for a in ua], axis=-1)
     u *= np.sqrt(wmax * wmax - 1.0) / np.sqrt(np.max(np.sum(u * u, axis=-1)))
     W = np.sqrt(1.0 + np.sum(u * u, axis=-1))
     v = u / W[..., None]
-    specs = [_spectrum(rng, 3, 1, kmax, 0.0) for _ in range(3)]
+    specs = [_spectrum(rng, 3, 1, kmax, a_slope) for _ in range(3)]
     face_shift = {0: (0.0, 0.5, 0.5), 1: (0.5, 0.0, 0.5), 2: (0.5, 0.5, 0.0)}
 
     def field_at(q, a):

@@ -57,6 +57,8 @@ def lib():
         L.oracle_courant_dt.argtypes = [ctypes.POINTER(MaraDesc), _dp, _dp]
         L.oracle_monopole_max.argtypes = [ctypes.POINTER(MaraDesc), _dp, _dp]
         L.oracle_set_threads.argtypes = [_i]
+        L.oracle_rmhd_c2p_codes.argtypes = [_dp, _d, _d, _ip]
+        L.oracle_last_failure.argtypes = [_dp]
         _LIB = L
     return _LIB
 
@@ -219,6 +221,21 @@ def monopole_max(desc, P):
     m = ctypes.c_double(0)
     err = lib().oracle_monopole_max(ctypes.byref(desc), _ptr(P), ctypes.byref(m))
     return err, m.value
+
+
+def rmhd_c2p_codes(U, gamma, pfloor=-1.0):
+    """(Anton status, Noble status) of the inversion of U (C2P_* codes)."""
+    codes = (ctypes.c_int * 2)()
+    lib().oracle_rmhd_c2p_codes(_ptr(_arr(U)), gamma, pfloor, codes)
+    return codes[0], codes[1]
+
+
+def last_failure():
+    """(RK stage, conserved state) of the first failing zone of the last
+    advance that failed an inversion; stage -1 if the last one did not."""
+    U = np.zeros(10)
+    stage = lib().oracle_last_failure(_ptr(U))
+    return stage, U
 
 
 def set_threads(n):

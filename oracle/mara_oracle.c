@@ -460,6 +460,29 @@ int oracle_rmhd_c2p(const double U[8], double gamma, double pfloor, double P[8],
     return error;
 }
 
+/* Both solvers' status codes for one conserved state (diagnostics of a
+   failed inversion): codes[0] Anton, codes[1] Noble, each a C2P_* value
+   (0 success, 1 NaN in U, 2 D < 0, 3 tau < 0, 4 NaN in P, 5 p < 0,
+   6 rho < 0, 7 v.v >= 1, 8 iteration limit). */
+void oracle_rmhd_c2p_codes(const double U[8], double gamma, double pfloor, int codes[2])
+{
+    c2p_state s;
+    double P[8];
+    s.gamf = (gamma - 1.0) / gamma;
+    s.pfloor = pfloor;
+    s.D = U[0];
+    s.Tau = U[4];
+    s.S2 = U[1] * U[1] + U[2] * U[2] + U[3] * U[3];
+    s.B2 = U[5] * U[5] + U[6] * U[6] + U[7] * U[7];
+    s.BS = U[5] * U[1] + U[6] * U[2] + U[7] * U[3];
+    s.BS2 = s.BS * s.BS;
+    memcpy(s.Cons, U, 8 * sizeof(double));
+    s.Z_start = sqrt(s.S2 + s.D * s.D);
+    s.W_start = s.Z_start / s.D;
+    codes[0] = solve_anton2dzw(&s, P);
+    codes[1] = solve_noble1dw(&s, P);
+}
+
 /* ========================================================================== */
 /* Conservation laws. */
 typedef struct
@@ -1240,10 +1263,22 @@ int oracle_advance_visc(const mara_desc* d, double* P, double t0, double dt, ora
     return advance_impl(d, P, dt, NULL, taux, params, visc, rep);
 }
 
+/* The RK stage (0-based) and conserved state of the first failing zone of
+   the last advance that failed; stage -1 after one that did not. */
+static int last_fail_stage = -1;
+static double last_fail_U[NF];
+
+int oracle_last_failure(double* U)
+{
+    if (U) memcpy(U, last_fail_U, sizeof last_fail_U);
+    return last_fail_stage;
+}
+
 static int advance_impl(const mara_desc* d, double* P, double dt, oracle_source_fn src, const double* taux,
                         const double* params, oracle_viscous_fn visc, mara_report* rep)
 {
     layout l;
+    last_fail_stage = -1;
     const int nq = law_nq(law_of(d));
     const int ng = stencil_size(d->recon);
     int fp[3];
@@ -1398,11 +1433,17 @@ static int advance_impl(const mara_desc* d, double* P, double dt, oracle_source_
             floors += health;
             for (int q = 0; q < nq; ++q) P[c + q] = Pq[q];
         }
-        if (fails && rep)
+        if (fails)
         {
-            rep->first_failure[2] = (int)(first % m2);
-            rep->first_failure[1] = (int)((first / m2) % m1);
-            rep->first_failure[0] = (int)(first / ((long)m1 * m2));
+            const int fk = (int)(first % m2), fj = (int)((first / m2) % m1), fi = (int)(first / ((long)m1 * m2));
+            if (rep)
+            {
+                rep->first_failure[2] = fk;
+                rep->first_failure[1] = fj;
+                rep->first_failure[0] = fi;
+            }
+            last_fail_stage = rk;
+            memcpy(last_fail_U, U + cidx(&l, fi + g0, fj + g1, fk + g2), nq * sizeof(double));
         }
         if (rep) { rep->c2p_failures += fails; rep->floor_applied += floors; }
         if (fails) { status = MARA_ERR_INVERSION_FAILURE; break; }

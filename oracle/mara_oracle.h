@@ -39,6 +39,15 @@ int oracle_quartic(const double d[5], double roots[4]);
    error code; *which = 0 if Anton succeeded, 1 if Noble did, -1 if none. */
 int oracle_rmhd_c2p(const double U[8], double gamma, double pfloor, double P[8], int* which);
 
+/* Both inversion solvers' status codes for U: codes[0] Anton, codes[1]
+   Noble (0 success, 3 tau < 0, 5 p < 0, 7 v.v >= 1, 8 iteration limit, ...). */
+void oracle_rmhd_c2p_codes(const double U[8], double gamma, double pfloor, int codes[2]);
+
+/* Diagnostics of the last oracle_advance that failed an inversion: returns
+   the 0-based RK stage (-1 when the last advance had no failure) and copies
+   the first failing zone's conserved state into U (nq doubles) if U != NULL. */
+int oracle_last_failure(double* U);
+
 /* ConservationLaw::fromPrimitive for a law along unit normal n: writes
    U[nq], F[nq], A[10].  Returns MARA_OK, MARA_ERR_SUPERLUMINAL or
    MARA_ERR_BAD_EIGENVALUES. */
