@@ -1,17 +1,24 @@
 #!/usr/bin/env python3
-"""Throughput of the MI355X Godunov step on the 3-D decaying-turbulence box.
+"""Throughput of the MI355X Godunov step, every configuration in one line.
 
-Workload: 256^3 zones, periodic unit cube, gamma 5/3, rho = 1, p = 1, seeded
-Gaussian random velocity field on 1 <= |k| <= 16 with |v_k|^2 ~ k^-4 at rms
-Mach 1; PLM (theta 1.5) + HLLC, RK2, CFL 0.3 (mara2_amd.problems.turbulence_3d).
+Headline workload: 3-D decaying turbulence, 256^3 zones, periodic unit cube,
+gamma 5/3, rho = 1, p = 1, seeded Gaussian random velocity field on
+1 <= |k| <= 16 with |v_k|^2 ~ k^-4 at rms Mach 1; PLM (theta 1.5) + HLLC,
+RK2, CFL 0.3 (mara2_amd.problems.turbulence_3d).  One step = the CFL
+reduction, two RK stages and the ghost fills, driven on the device by
+mara_block_run; the state is resident in HBM before the timed region starts
+and the call returns after the stream has drained.
 
-One step = courant reduction + two RK stages + ghost fills, driven on the
-device by mara_block_run (dt = 0.3 * courant each step).  The state is
-resident in HBM before the timed region starts; the call returns after the
-stream has drained.  Prints one JSON line (value = plain FP64 form; the
-mixed form, its speed-up and its L1 distance to FP64 ride along).
+`value` is the fast binary64 form (every operation binary64, held to
+relative L1 <= 1e-12 of the reference's conserved fields after 10 steps; the
+distance to the bit-exact binary64 form is measured in the same run and
+reported beside it).  The bit-exact binary64 form and the mixed form (binary32
+fluxes, binary64 accumulation) ride along.  The other configurations follow
+in `configs`: the Sod tube's time to solution, 2-D MHD 2048^2 with flux-CT,
+3-D relativistic MHD 128^3, and a 1024-request serving stream; then the CPU
+baseline (the oracle's -O3 build on the host's CPU share).
 
-    python bench.py [--gpus 1] [--steps 100] [--warmup 5]
+    python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--workload all]
 """
 import argparse
 import json
@@ -25,7 +32,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from mara2_amd import abi, problems  # noqa: E402
-from mara2_amd.runtime import Batch, Block  # noqa: E402
+from mara2_amd.runtime import Batch, Block, MaraError  # noqa: E402
 
 METRIC = "zone-updates/s, FP32-mixed vs FP64 at stated L1 error; solves/s + p99 latency"
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
@@ -55,225 +62,285 @@ def traffic_from_profile(kernel, workload):
     return None
 
 
-def time_form(desc, P0, steps, warmup, transfers=None, tune=None):
-    """Upload, warm up, let the autotuner pick the launch shape (tune, if a
-    dict, receives it), then time `steps` device-driven steps.  transfers, if
-    a dict, receives the host<->device times of the MeshData::P arrays."""
+def with_form(desc, form):
+    d = type(desc).from_buffer_copy(desc)
+    d.precision = abi.PRECISIONS[form]
+    return d
+
+
+def rel_l1_cons(desc, Pa, Pb):
+    """max over fields of sum |U(Pa) - U(Pb)| / sum |U(Pb)| on the interior
+    (hydro and MHD laws; the primitives for the relativistic one)."""
+    g = desc.guards
+    inner = tuple(slice(g[a], g[a] + desc.cells[a]) for a in range(3))
+    a, b = Pa[inner], Pb[inner]
+
+    def cons(P):
+        if desc.law == abi.LAW_RMHD:
+            return P
+        U = np.empty_like(P)
+        U[..., 0] = P[..., 0]
+        U[..., 1:4] = P[..., :1] * P[..., 1:4]
+        e = 0.5 * P[..., 0] * np.sum(P[..., 1:4] ** 2, axis=-1) + P[..., 4] / (desc.gamma - 1)
+        if desc.law == abi.LAW_MHD:
+            e = e + 0.5 * np.sum(P[..., 5:8] ** 2, axis=-1)
+            U[..., 5:8] = P[..., 5:8]
+        U[..., 4] = e
+        return U
+    ua, ub = cons(a), cons(b)
+    return max(float(np.sum(np.abs(ua[..., q] - ub[..., q])) / max(np.sum(np.abs(ub[..., q])), 1e-300))
+               for q in range(ua.shape[-1]))
+
+
+def time_form(desc, P0, steps, warmup, cfl=CFL, tune=True, transfers=None):
+    """Upload, warm up, let the autotuner pick the launch shape, then time
+    `steps` device-driven steps with stage-kernel events.  Returns a dict."""
+    out = {}
     with Block(desc) as blk:
         t0 = time.perf_counter()
         blk.upload(P0)
         if transfers is not None:
             transfers["upload_s"] = time.perf_counter() - t0
         if warmup:
-            blk.run(CFL, 1e300, warmup)
+            blk.run(cfl, 1e300, warmup, check=False)
             blk.upload(P0)
-        if tune is not None:
+        if tune:
             shape, ms = blk.autotune(repeats=5)
-            tune.update(shape.as_dict(), stage_ms=ms)
-        # stage-kernel durations from HIP event pairs recorded on the block's
-        # stream around every stage launch of the timed region (no host wait
-        # between them; read after the run)
+            out["launch_shape"] = dict(shape.as_dict(), stage_ms=ms)
         blk.set_timing(True)
         t0 = time.perf_counter()
-        t, taken, rep = blk.run(CFL, 1e300, steps)
+        t, taken, rep = blk.run(cfl, 1e300, steps, check=False)
         t1 = time.perf_counter()
-        if taken != steps:
-            raise RuntimeError(f"run stopped after {taken} of {steps} steps: {rep.as_dict()}")
         avg_ms, launches, name = blk.stage_kernel_time()
         t2 = time.perf_counter()
-        P = blk.download()
+        out["P"] = blk.download()
         if transfers is not None:
             transfers["download_s"] = time.perf_counter() - t2
-    return t1 - t0, P, avg_ms, launches, name
+    out.update(el=t1 - t0, steps=taken, report=rep.as_dict(), stage_ms=avg_ms, launches=launches, kernel=name)
+    return out
 
 
-def fast_vs_plain_after(d64, dfast, P0, steps):
-    """Relative L1 of the conserved interior between the fast and the plain
-    binary64 forms after `steps` steps of the plain form's dt."""
-    res = []
+def run_fixed(desc, P0, dts):
+    """The state after steps of the given sizes (mara_block_advance)."""
+    with Block(desc) as blk:
+        blk.upload(P0)
+        for dt in dts:
+            blk.advance(dt, check=False)
+        return blk.download()
+
+
+def courant_dts(desc, P0, steps, cfl):
+    """The dt sequence of the bit-exact binary64 form over `steps` steps."""
+    d = with_form(desc, "fp64")
     dts = []
-    for d in (d64, dfast):
-        with Block(d) as blk:
+    with Block(d) as blk:
+        blk.upload(P0)
+        for _ in range(steps):
+            dts.append(cfl * blk.courant_dt())
+            blk.advance(dts[-1], check=False)
+    return dts
+
+
+def roofline(desc, stage_ms, kernel, traffic=None):
+    zones = int(np.prod([desc.cells[a] for a in range(3)]))
+    per_launch = alg_bytes_per_zone_update(desc) * zones / desc.rk_order
+    achieved = per_launch / (stage_ms * 1e-3)
+    return {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK, "traffic": traffic, "kernel": kernel, "avg_launch_ms": stage_ms,
+            "alg_bytes_per_launch": per_launch}
+
+
+# ---------------------------------------------------------------------------
+# headline: 3-D turbulence 256^3
+# ---------------------------------------------------------------------------
+def turbulence_leg(args):
+    n = args.n or 256
+    d64 = problems.turbulence_desc(n, "fp64")
+    log(f"generating {n}^3 turbulence")
+    P0 = problems.turbulence_3d(d64)
+    zones = n ** 3
+    res = {}
+    transfers = {}
+    for form in ("fp64_fast", "fp64", "mixed"):
+        log(f"turbulence: timing {form}")
+        r = time_form(with_form(d64, form), P0, args.steps, args.warmup, tune=not args.no_autotune,
+                      transfers=transfers if form == "fp64_fast" else None)
+        if r["steps"] != args.steps:
+            raise RuntimeError(f"{form}: run stopped after {r['steps']} of {args.steps} steps: {r['report']}")
+        r["value"] = zones * args.steps / r["el"]
+        r.pop("P")
+        log(f"{form}: {r['el']:.3f} s for {args.steps} steps; stage kernel {r['stage_ms']:.3f} ms ({r['kernel']})")
+        res[form] = r
+    # accuracy legs, untimed: the fast form after 10 steps against the
+    # bit-exact one (which equals the reference restatement bit for bit), and
+    # the mixed form over the configuration's stated 100 steps
+    log("turbulence: accuracy legs")
+    dts10 = courant_dts(d64, P0, 10, CFL)
+    exact10 = run_fixed(d64, P0, dts10)
+    l1_fast10 = rel_l1_cons(d64, run_fixed(with_form(d64, "fp64_fast"), P0, dts10), exact10)
+    n_mixed = 100
+    dts100 = courant_dts(d64, P0, n_mixed, CFL)
+    l1_mixed100 = rel_l1_cons(d64, run_fixed(with_form(d64, "mixed"), P0, dts100), run_fixed(d64, P0, dts100))
+    return d64, res, transfers, l1_fast10, l1_mixed100, n_mixed
+
+
+# ---------------------------------------------------------------------------
+# the other configurations
+# ---------------------------------------------------------------------------
+def sod_leg(args):
+    d = problems.sod_desc(1024)
+    P0 = problems.sod_1d(d)
+    with Block(d) as blk:
+        blk.upload(P0)
+        blk.run(0.4, 0.2, 1 << 30)  # warm-up: kernel loading and graph capture
+        blk.upload(P0)
+        shape = None
+        if not args.no_autotune:
+            shape = blk.autotune(repeats=20)[0].as_dict()
+            blk.run(0.4, 0.2, 1 << 30)  # graph capture in the tuned shape
             blk.upload(P0)
-            for s in range(steps):
-                if d is d64:
-                    dts.append(CFL * blk.courant_dt())
-                blk.advance(dts[s])
-            res.append(blk.download())
-    g = d64.guards
-    inner = tuple(slice(g[a], g[a] + d64.cells[a]) for a in range(3))
-
-    def cons(P):
-        P = P[inner]
-        U = np.empty_like(P)
-        U[..., 0] = P[..., 0]
-        U[..., 1:4] = P[..., :1] * P[..., 1:4]
-        U[..., 4] = 0.5 * P[..., 0] * np.sum(P[..., 1:4] ** 2, axis=-1) + P[..., 4] / (d64.gamma - 1)
-        return U
-    a, b = cons(res[1]), cons(res[0])
-    return max(float(np.sum(np.abs(a[..., q] - b[..., q])) / np.sum(np.abs(b[..., q]))) for q in range(5))
-
-
-def cpu_model():
-    try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
-    except OSError:
-        pass
-    return "unknown"
-
-
-def _oracle_rate(n_sample, threads, target_s, max_steps=2000):
-    import oracle
-    oracle.set_threads(threads)
-    desc = problems.turbulence_desc(n_sample)
-    P = problems.turbulence_3d(desc)
-    err, dtc = oracle.courant_dt(desc, P)
-    oracle.advance(desc, P, 0.0, CFL * dtc)  # warm-up step
-    steps, t0 = 0, time.perf_counter()
-    while True:
-        err, dtc = oracle.courant_dt(desc, P)
-        err, _ = oracle.advance(desc, P, 0.0, CFL * dtc)
-        steps += 1
+        t0 = time.perf_counter()
+        t, steps, rep = blk.run(0.4, 0.2, 1 << 30)
         el = time.perf_counter() - t0
-        if el > target_s or steps >= max_steps:
-            break
-    return n_sample ** 3 * steps / el, steps, el
+    return {"workload": "1-D Sod tube 1024 zones, gamma 1.4, PLM + HLLE, RK2, CFL 0.4, to t = 0.2, fp64",
+            "time_to_solution_s": el, "steps": steps, "t": t, "value": 1024 * steps / el,
+            "unit": "zone-updates/s", "launch_shape": shape,
+            "roofline_frac": 1024 * steps / el * alg_bytes_per_zone_update(d) / HBM_PEAK}
 
 
-def cpu_baseline(n_sample, target_s=15.0):
-    """The oracle (CPU restatement of the reference step) on the same workload
-    at n_sample^3, OpenMP threads = the host's CPU share, plus a shorter
-    1-thread run."""
-    # the host's CPU share: OMP_NUM_THREADS where the environment sets it
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
-    rate, steps, el = _oracle_rate(n_sample, threads, target_s)
-    rate1, steps1, el1 = _oracle_rate(32, 1, 5.0, 200)
-    return {"value": rate, "unit": "zone-updates/s", "cores": threads, "kind": "port",
-            "cpu": cpu_model(), "one_thread_value": rate1,
-            "sample": f"turbulence_3d at {n_sample}^3 (same generator and scheme), {steps} steps after 1 warm-up "
-                      f"step, {el:.1f} s, CPU restatement of the reference step (oracle/), OpenMP over zones; "
-                      f"one thread: 32^3, {steps1} steps, {el1:.1f} s"}
-
-
-def other_workload(args):
-    """Secondary configurations (not the headline line): 2-D MHD with flux-CT,
-    3-D relativistic MHD with WENO5, and the 1-D Sod tube to t = 0.2."""
-    from mara2_amd.runtime import MaraError
-    w = args.workload
-    if w == "sod":
-        d = problems.sod_desc(1024)
-        P0 = problems.sod_1d(d)
-        with Block(d) as blk:
-            blk.set_graphs(not args.no_graphs)
-            blk.upload(P0)
-            blk.run(0.4, 0.2, 1 << 30)  # warm-up: kernel loading and graph capture
-            blk.upload(P0)
-            shape = None
-            if not args.no_autotune:
-                shape, _ = blk.autotune(repeats=20)
-                shape = shape.as_dict()
-                blk.run(0.4, 0.2, 1 << 30)  # graph capture in the tuned shape
-                blk.upload(P0)
-            t0 = time.perf_counter()
-            t, steps, rep = blk.run(0.4, 0.2, 1 << 30)
-            el = time.perf_counter() - t0
-        out = {"metric": METRIC, "value": 1024 * steps / el, "unit": "zone-updates/s", "n_gpus": 1,
-               "steps": steps, "warmup": 0, "ms_per_step": el / steps * 1e3, "higher_is_better": True,
-               "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic: Sod tube",
-               "config": {"workload": "1-D Sod tube 1024 zones, gamma 1.4, PLM + HLLE, RK2, CFL 0.4, to t = 0.2",
-                          "t_reached": t, "time_to_solution_s": el, "graphs": not args.no_graphs,
-                          "launch_shape": shape}}
-        print(json.dumps(out), flush=True)
-        return
-    if w == "mhd2d":
-        n = args.n or 2048
-        make_desc, make, cfl, label = problems.mhd_desc, problems.mhd_2d, 0.4, \
-            f"2-D ideal MHD {n}^2, PLM + HLLE, flux-CT, RK2, CFL 0.4, periodic"
-        gen = lambda d: make(d)
-    else:
-        n = args.n or 128
-        make_desc, make, cfl, label = problems.rmhd_desc, problems.rmhd_3d, 0.3, \
-            f"3-D relativistic MHD {n}^3, WENO5 + HLLE, flux-CT, RK3, CFL 0.3, periodic, max W {args.wmax}"
-        gen = lambda d: make(d, wmax=args.wmax)
-    d64 = make_desc(n, "fp64")
-    log(f"generating {label}")
-    P0 = gen(d64)
+def mhd2d_leg(args):
+    n = 2048
+    d64 = problems.mhd_desc(n, "fp64")
+    log(f"generating 2-D MHD {n}^2")
+    P0 = problems.mhd_2d(d64)
+    zones = n * n
+    out = {"workload": f"2-D ideal MHD {n}^2, PLM + HLLE, flux-CT, RK2, CFL 0.4, periodic"}
+    for form in ("fp64", "mixed"):
+        r = time_form(with_form(d64, form), P0, args.steps, 2, cfl=0.4, tune=not args.no_autotune)
+        v = zones * r["steps"] / r["el"]
+        out[form] = {"value": v, "ms_per_step": r["el"] / max(r["steps"], 1) * 1e3, "steps": r["steps"],
+                     "stage_ms": r["stage_ms"], "kernel": r["kernel"],
+                     "roofline_frac": v * alg_bytes_per_zone_update(d64) / HBM_PEAK,
+                     "launch_shape": r.get("launch_shape")}
+        log(f"mhd2d {form}: {v:.3e} zone-updates/s")
+    # the configuration's 200 steps, untimed, with the bit-exact form's dt
+    steps = 200
+    dts = courant_dts(d64, P0, steps, 0.4)
     res = {}
     for form in ("fp64", "mixed"):
-        d = make_desc(n, form)
-        with Block(d) as blk:
-            blk.set_graphs(not args.no_graphs)
-            # warm-up (kernel loading, graph capture) and the launch shape
+        with Block(with_form(d64, form)) as blk:
             blk.upload(P0)
-            blk.run(cfl, 1e300, 2, check=False)
-            shape = None
-            if not args.no_autotune:
-                shape = blk.autotune(repeats=5)[0].as_dict()
-            # the maraMainLoop contract: stops at the first failure, as the reference does
-            blk.upload(P0)
-            t0 = time.perf_counter()
-            try:
-                t, steps, rep = blk.run(cfl, 1e300, args.steps, check=False)
-            except MaraError as e:
-                t, steps, rep = 0.0, 0, None
-            el = time.perf_counter() - t0
-            P = blk.download()
-            mono = blk.monopole_max()
-            timed = None
-            if steps < args.steps:
-                # throughput when the loop stopped early: up to 10 steps from
-                # the initial state at its first dt, failures ignored.  Zones
-                # past an inversion failure run the Newton solver to its
-                # iteration limit every step and their number grows, so a
-                # longer fixed-dt run would time the failure, not the scheme
-                # (100 steps: 9.8 ms/step against 5.7 over the first 10).
-                n_fixed = min(args.steps, 10)
-                blk.upload(P0)
-                dt = cfl * blk.courant_dt()
+            for dt in dts:
                 blk.advance(dt, check=False)
-                t1 = time.perf_counter()
-                for _ in range(n_fixed):
-                    blk.advance(dt, check=False)
-                timed = (time.perf_counter() - t1, n_fixed)
+            res[form] = (blk.download(), blk.monopole_max())
+    out["fp64"]["max_abs_monopole_after_200_steps"] = res["fp64"][1]
+    out["mixed"]["max_abs_monopole_after_200_steps"] = res["mixed"][1]
+    out["mixed"]["rel_l1_vs_fp64_after_200_steps"] = rel_l1_cons(d64, res["mixed"][0], res["fp64"][0])
+    out["mixed"]["speedup_vs_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
+    return out
+
+
+def rmhd_leg(args):
+    n = 128
+    d64 = problems.rmhd_desc(n, "fp64")
+    log(f"generating 3-D RMHD {n}^3")
+    P0 = problems.rmhd_3d(d64)
+    zones = n ** 3
+    out = {"workload": f"3-D relativistic MHD {n}^3, WENO5 + HLLE, flux-CT, RK3, CFL 0.3, periodic, max W 5, "
+                       "mean sigma 1, flat spectra"}
+    for form in ("fp64", "mixed"):
+        d = with_form(d64, form)
+        with Block(d) as blk:
             blk.upload(P0)
+            blk.run(0.3, 1e300, 1, check=False)  # warm-up
+            # the maraMainLoop contract: stops at the first inversion failure
+            blk.upload(P0)
+            t, steps, rep = blk.run(0.3, 1e300, 100, check=False)
+            mono = blk.monopole_max()
+            # throughput at the first step's dt from the initial state over a
+            # fixed number of steps, failures counted (the reference would stop)
+            blk.upload(P0)
+            dt = 0.3 * blk.courant_dt()
+            blk.advance(dt, check=False)
+            n_fixed = min(args.steps, 10)
             blk.set_timing(True)
-            blk.run(cfl, 1e300, 2, check=False)
+            t1 = time.perf_counter()
+            fails = 0
+            for _ in range(n_fixed):
+                st, rp = blk.advance(dt, check=False)
+                fails += rp.c2p_failures
+            el = time.perf_counter() - t1
             ms, nl, name = blk.stage_kernel_time()
-        res[form] = dict(el=el, steps=steps, P=P, rep=rep.as_dict() if rep else None, mono=mono, ms=ms, name=name,
-                         timed=timed, shape=shape)
-        log(f"{form}: {steps} steps in {el:.3f} s, report {res[form]['rep']}, max |monopole| {mono:.3e}"
-            + (f"; fixed-dt timing {timed[1]} steps in {timed[0]:.3f} s" if timed else ""))
-    zones = int(np.prod([d64.cells[a] for a in range(3)]))
-    g = d64.guards
-    inner = tuple(slice(g[a], g[a] + d64.cells[a]) for a in range(3))
-    a, b = res["mixed"]["P"][inner], res["fp64"]["P"][inner]
-    l1 = max(float(np.sum(np.abs(a[..., q] - b[..., q])) / max(np.sum(np.abs(b[..., q])), 1e-300)) for q in range(8))
-    r64 = res["fp64"]
-    b_alg = alg_bytes_per_zone_update(d64)
-    per_launch = b_alg * zones / d64.rk_order
+        v = zones * n_fixed / el
+        out[form] = {"steps_before_failure": steps, "report_at_stop": rep.as_dict(),
+                     "max_abs_monopole_at_stop": mono, "fixed_dt_value": v, "fixed_dt_steps": n_fixed,
+                     "fixed_dt_c2p_failures": fails, "ms_per_step": el / n_fixed * 1e3, "stage_ms": ms,
+                     "kernel": name, "roofline_frac": v * alg_bytes_per_zone_update(d64) / HBM_PEAK}
+        log(f"rmhd3d {form}: {steps} steps before failure, fixed-dt {v:.3e} zone-updates/s")
+    out["mixed"]["speedup_vs_fp64"] = out["mixed"]["fixed_dt_value"] / out["fp64"]["fixed_dt_value"]
+    return out
 
-    def rate(r):
-        el, steps = r["timed"] if r["timed"] else (r["el"], r["steps"])
-        return zones * steps / el, el / max(steps, 1) * 1e3, steps
 
-    v64, ms64, st64 = rate(r64)
-    vmx, msmx, _ = rate(res["mixed"])
-    out = {"metric": METRIC, "value": v64, "unit": "zone-updates/s", "n_gpus": 1,
-           "steps": st64, "warmup": 0, "ms_per_step": ms64,
-           "steps_before_failure": {"fp64": r64["steps"], "mixed": res["mixed"]["steps"]},
-           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-           "data": "synthetic: seeded Gaussian random fields",
-           "config": {"workload": label, "zones": zones, "launch_shape": r64["shape"],
-                      "launch_shape_mixed": res["mixed"]["shape"]},
-           "roofline": {"bound": "hbm", "achieved": per_launch / (r64["ms"] * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9,
-                        "unit": "GB/s", "frac": per_launch / (r64["ms"] * 1e-3) / HBM_PEAK, "traffic": None,
-                        "kernel": r64["name"], "avg_launch_ms": r64["ms"]},
-           "fp64": {"max_abs_monopole": r64["mono"], "report": r64["rep"]},
-           "mixed": {"value": vmx, "speedup_vs_fp64": vmx / v64,
-                     "rel_l1_vs_fp64": l1, "max_abs_monopole": res["mixed"]["mono"], "report": res["mixed"]["rep"]}}
-    print(json.dumps(out), flush=True)
+def serving_leg(args, n_req, seq_sample, saturate=True, open_loop=True):
+    """Continuous batching over the seeded Riemann-problem stream: 70% 1-D
+    (256..16384 zones), 30% 2-D (128^2..1024^2), half MHD, PLM + HLLC / HLLE,
+    RK2, CFL 0.4, to t = 0.2, mixed form.  The one-at-a-time service rate is
+    measured on an evenly spaced sample of the stream (create, upload, run,
+    download, destroy per request); arrivals are Poisson at twice that rate."""
+    from mara2_amd import serving
+    log(f"serving: building {n_req} requests")
+    reqs = serving.make_stream(n_req, precision="mixed")
+    warm = serving.make_stream(4, precision="mixed", resize=lambda dim, n: 32 if dim == 1 else 16)
+    graphs = not args.no_graphs
+    serving.sequential(warm, graphs=graphs)
+    serving.serve(warm, 1 << 28, max_solves=8, chunk=args.chunk, graphs=graphs)
+    stride = max(1, n_req // seq_sample)
+    sample = reqs[::stride][:seq_sample]
+    log(f"serving: one-at-a-time service of {len(sample)} requests (every {stride}th)")
+    times = serving.sequential(sample, graphs=graphs)
+    seq_rate = len(sample) / sum(times)
+    seq_zone_rate = sum(r.zones * r.steps for r in sample) / sum(times)
+    budget = int(args.budget_gib * 2**30)
+    with Batch(1 << 20, rk_order=2, precision="mixed", max_solves=1) as b0:
+        batch_shape = b0.get_shape().as_dict()
+    out = {"workload": f"serving stream of {n_req} Riemann problems, 70% 1-D 256..16384 zones, 30% 2-D "
+                       "128^2..1024^2, half MHD, PLM + HLLC/HLLE, RK2, CFL 0.4, to t = 0.2, mixed form",
+           "unit": "solves/s", "hbm_pool_gib": args.budget_gib, "chunk_steps": args.chunk,
+           "launch_shape": batch_shape,
+           "sequential": {"solves_per_s": seq_rate, "zone_updates_per_s": seq_zone_rate,
+                          "sample": f"{len(sample)} requests, every {stride}th of the stream",
+                          "mean_s": float(np.mean(times)), "max_s": float(max(times)),
+                          "failed_requests": [r.index for r in serving.failures(sample)],
+                          "truncated_requests": [r.index for r in serving.truncated(sample)]},
+           "work_bound_per_request": {"steps": serving.MAX_STEPS, "zone_updates": serving.MAX_ZONE_UPDATES}}
+    if open_loop:
+        serving.set_arrivals(reqs, 2.0 * seq_rate)
+        log(f"serving: Poisson arrivals at {2 * seq_rate:.3f}/s (span {reqs[-1].arrival:.0f} s)")
+        wall, done = serving.serve(reqs, budget, chunk=args.chunk, log=log, graphs=graphs)
+        st = serving.latency_stats(done)
+        out["open_loop"] = {"arrival_rate": 2.0 * seq_rate, "solves_per_s": len(done) / wall, "wall_s": wall,
+                            "latency_s": st, "zone_updates_per_s": sum(r.zones * r.steps for r in done) / wall,
+                            "failed_requests": [r.index for r in serving.failures(done)],
+                            "truncated_requests": [r.index for r in serving.truncated(done)]}
+        log(f"serving open loop: {len(done) / wall:.2f} solves/s, p50 {st['p50_s']:.3f} s, p99 {st['p99_s']:.3f} s")
+    if saturate:
+        for r in reqs:
+            r.arrival = 0.0
+        log("serving: saturated run, every request at t = 0")
+        wall_s, done_s = serving.serve(reqs, budget, chunk=args.chunk, log=log, graphs=graphs)
+        zu_s = sum(r.zones * r.steps for r in done_s)
+        out["saturated"] = {"solves_per_s": len(done_s) / wall_s, "wall_s": wall_s,
+                            "speedup_vs_sequential": len(done_s) / wall_s / seq_rate,
+                            "work_speedup_vs_sequential": zu_s / wall_s / seq_zone_rate,
+                            "stream_vs_sample_work_per_solve":
+                                (zu_s / len(done_s)) / (sum(r.zones * r.steps for r in sample) / len(sample)),
+                            "latency_s": serving.latency_stats(done_s), "zone_updates_per_s": zu_s / wall_s,
+                            "failed_requests": [r.index for r in serving.failures(done_s)],
+                            "truncated_requests": [r.index for r in serving.truncated(done_s)]}
+        log(f"serving saturated: {len(done_s) / wall_s:.2f} solves/s "
+            f"({out['saturated']['speedup_vs_sequential']:.2f}x one at a time)")
+    out["value"] = out.get("saturated", out.get("open_loop", {})).get("solves_per_s")
+    return out
 
 
 def copy_workload(args):
@@ -302,202 +369,141 @@ def copy_workload(args):
                       "GB_per_s": nbytes / (ms * 1e-3) / 1e9}), flush=True)
 
 
-def serving_workload(args):
-    """Continuous batching over the seeded Riemann-problem stream: 70% 1-D
-    (256..16384 zones), 30% 2-D (128^2..1024^2), half MHD, PLM + HLLC / HLLE,
-    RK2, CFL 0.4, to t = 0.2, mixed form.  The one-at-a-time service rate is
-    measured on an evenly spaced sample of the stream (create, upload, run,
-    download, destroy per request); arrivals are Poisson at twice that rate."""
-    from mara2_amd import serving
-    n_req = args.requests
-    log(f"building {n_req} requests")
-    reqs = serving.make_stream(n_req, precision="mixed")
-    work = [r.zones for r in reqs]
-    log("warm-up")
-    warm = serving.make_stream(4, precision="mixed", resize=lambda dim, n: 32 if dim == 1 else 16)
-    graphs = not args.no_graphs
-    serving.sequential(warm, graphs=graphs)
-    serving.serve(warm, 1 << 28, max_solves=8, chunk=args.chunk, graphs=graphs)
-    stride = max(1, n_req // args.seq_sample)
-    sample = reqs[::stride][:args.seq_sample]
-    log(f"one-at-a-time service of {len(sample)} requests (every {stride}th)")
-    times = serving.sequential(sample, graphs=graphs)
-    seq_failed = [r.index for r in serving.failures(sample)]
-    seq_rate = len(sample) / sum(times)
-    seq_zone_rate = sum(r.zones * r.steps for r in sample) / sum(times)
-    log(f"sequential: {seq_rate:.3f} solves/s, mean {np.mean(times):.3f} s, max {max(times):.3f} s")
-    budget = int(args.budget_gib * 2**30)
-    # the batch keeps its own launch shape (pencils of 128 faces, 64 x 8 MHD
-    # tiles): tuned on one request alone, a block prefers longer pencils and
-    # taller tiles, which are slower once the batch fills the device
-    shape = None
-    with Batch(1 << 20, rk_order=2, precision="mixed", max_solves=1) as b0:
-        batch_shape = b0.get_shape().as_dict()
-    out = {"metric": METRIC, "value": None, "unit": "solves/s", "n_gpus": 1, "steps": None,
-           "warmup": None, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "f32/f64 mixed",
-           "data": f"synthetic: seeded Riemann-problem stream (seed {problems.SEED_SERVING})",
-           "config": {"workload": f"serving stream of {n_req} Riemann problems, 70% 1-D 256..16384 zones, "
-                                  "30% 2-D 128^2..1024^2, half MHD, PLM + HLLC/HLLE, RK2, CFL 0.4, to t = 0.2, "
-                                  "mixed form, Poisson arrivals at 2x the one-at-a-time rate",
-                      "zones_total": int(sum(work)), "hbm_pool_gib": args.budget_gib, "chunk_steps": args.chunk,
-                      "graphs": graphs, "launch_shape": batch_shape,
-                      "express_lane_max": args.express},
-           "arrival_rate": 2.0 * seq_rate,
-           "sequential": {"solves_per_s": seq_rate, "zone_updates_per_s": seq_zone_rate,
-                          "sample": f"{len(sample)} requests, every {stride}th of the stream",
-                          "mean_s": float(np.mean(times)), "max_s": float(max(times)),
-                          "failed_requests": seq_failed,
-                          "truncated_requests": [r.index for r in serving.truncated(sample)]},
-           "work_bound_per_request": {"steps": serving.MAX_STEPS, "zone_updates": serving.MAX_ZONE_UPDATES}}
-    if not args.no_open_loop:
-        serving.set_arrivals(reqs, 2.0 * seq_rate)
-        log(f"serving {n_req} requests, Poisson arrivals at {2 * seq_rate:.3f}/s "
-            f"(span {reqs[-1].arrival:.0f} s), pool {args.budget_gib:g} GiB")
-        wall, done = serving.serve(reqs, budget, chunk=args.chunk, log=log, graphs=graphs, shape=shape,
-                                   express=args.express)
-        failed = [r.index for r in serving.failures(done)]
-        st = serving.latency_stats(done)
-        zone_updates = sum(r.zones * r.steps for r in done)
-        log(f"served in {wall:.1f} s: p50 {st['p50_s']:.3f} s p99 {st['p99_s']:.3f} s")
-        out.update({"value": len(done) / wall, "latency_s": st, "wall_s": wall,
-                    "zone_updates_per_s": zone_updates / wall, "failed_requests": failed,
-                    "truncated_requests": [r.index for r in serving.truncated(done)],
-                    "slowest": serving.slowest(done),
-                    "floor_applied_zones": int(sum(r.floors for r in done))})
-    if args.saturate:
-        for r in reqs:
-            r.arrival = 0.0
-        log("saturated run: every request at t = 0")
-        wall_s, done_s = serving.serve(reqs, budget, chunk=args.chunk, log=log, graphs=graphs, shape=shape,
-                                       express=args.express)
-        zu_s = sum(r.zones * r.steps for r in done_s)
-        # solves/s against the sample's solves/s, and zone-updates/s against
-        # the sample's zone-updates/s: the second does not depend on how the
-        # sample's mean work per solve compares with the stream's
-        out["saturated"] = {"solves_per_s": len(done_s) / wall_s, "wall_s": wall_s,
-                            "speedup_vs_sequential": len(done_s) / wall_s / seq_rate,
-                            "work_speedup_vs_sequential": zu_s / wall_s / seq_zone_rate,
-                            "stream_vs_sample_work_per_solve":
-                                (zu_s / len(done_s)) / (sum(r.zones * r.steps for r in sample) / len(sample)),
-                            "latency_s": serving.latency_stats(done_s),
-                            "zone_updates_per_s": zu_s / wall_s,
-                            "failed_requests": [r.index for r in serving.failures(done_s)],
-                            "truncated_requests": [r.index for r in serving.truncated(done_s)],
-                            "slowest": serving.slowest(done_s)}
-        if out["value"] is None:
-            out["value"] = out["saturated"]["solves_per_s"]
-    print(json.dumps(out), flush=True)
+# ---------------------------------------------------------------------------
+# CPU baseline
+# ---------------------------------------------------------------------------
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _oracle_rate(n_sample, threads, target_s, min_steps=3, max_steps=2000):
+    import ctypes
+    import oracle
+    from mara2_amd.abi import MaraReport
+    L = oracle.lib(timing=True)
+    L.oracle_set_threads(threads)
+    desc = problems.turbulence_desc(n_sample)
+    P = problems.turbulence_3d(desc)
+    ptr = P.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    dt = ctypes.c_double(0)
+    rep = MaraReport()
+
+    def step():
+        L.oracle_courant_dt(ctypes.byref(desc), ptr, ctypes.byref(dt))
+        L.oracle_advance(ctypes.byref(desc), ptr, 0.0, CFL * dt.value, ctypes.byref(rep))
+    step()  # warm-up step
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        step()
+        steps += 1
+        el = time.perf_counter() - t0
+        if (el > target_s and steps >= min_steps) or steps >= max_steps:
+            break
+    return n_sample ** 3 * steps / el, steps, el
+
+
+def cpu_baseline(n_sample, target_s=12.0):
+    """The oracle (CPU restatement of the reference step, its -O3 timing
+    build) on the same workload, OpenMP threads = the host CPU share this
+    process may use (the affinity mask, capped by OMP_NUM_THREADS where the
+    environment sets one), plus a shorter one-thread run."""
+    affinity = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else list(range(os.cpu_count()))
+    env_cap = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = min(len(affinity), env_cap) if env_cap else len(affinity)
+    rate, steps, el = _oracle_rate(n_sample, threads, target_s)
+    rate1, steps1, el1 = _oracle_rate(32, 1, 4.0, min_steps=5, max_steps=200)
+    return {"value": rate, "unit": "zone-updates/s", "cores": threads, "kind": "port",
+            "cpu": cpu_model(), "os_cpu_count": os.cpu_count(), "affinity_cpus": len(affinity),
+            "omp_num_threads_env": env_cap or None, "one_thread_value": rate1,
+            "sample": f"turbulence_3d at {n_sample}^3 (same generator and scheme), {steps} steps after 1 warm-up "
+                      f"step, {el:.1f} s; the CPU restatement of the reference step (oracle/), built -O3 "
+                      f"(x86-64-v3, contraction on), OpenMP over zones on {threads} threads; one thread: 32^3, "
+                      f"{steps1} steps, {el1:.1f} s"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--n", type=int, default=0)
-    ap.add_argument("--cpu-sample", type=int, default=64)
+    ap.add_argument("--n", type=int, default=0, help="turbulence: cells per axis (default 256)")
+    ap.add_argument("--cpu-sample", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="turbulence",
-                    choices=["turbulence", "mhd2d", "rmhd3d", "sod", "serving", "copy"])
-    ap.add_argument("--wmax", type=float, default=5.0)
-    ap.add_argument("--requests", type=int, default=4096, help="serving: stream length")
-    ap.add_argument("--seq-sample", type=int, default=256, help="serving: requests timed one at a time")
+    ap.add_argument("--workload", default="all",
+                    choices=["all", "turbulence", "mhd2d", "rmhd3d", "sod", "serving", "copy"])
+    ap.add_argument("--requests", type=int, default=1024, help="serving: stream length")
+    ap.add_argument("--seq-sample", type=int, default=64, help="serving: requests timed one at a time")
     ap.add_argument("--budget-gib", type=float, default=64.0, help="serving: HBM pool")
     ap.add_argument("--chunk", type=int, default=16, help="serving: steps per batch launch sequence")
-    ap.add_argument("--saturate", action="store_true", help="serving: also run with every request at t = 0")
-    ap.add_argument("--no-open-loop", action="store_true", help="serving: skip the Poisson-arrival run")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
-    ap.add_argument("--express", type=int, default=0, help="serving: most solves on the batch's express lane")
-    ap.add_argument("--no-graphs", action="store_true",
-                    help="sod, mhd2d, rmhd3d, serving: launch every kernel from the host instead of replaying HIP graphs")
+    ap.add_argument("--no-graphs", action="store_true", help="launch every kernel from the host")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               f"(a multi-GPU run would shard the mesh into one block per GPU with a halo exchange)",
               file=sys.stderr)
         sys.exit(2)
-    if args.workload == "serving":
-        return serving_workload(args)
     if args.workload == "copy":
         return copy_workload(args)
-    if args.workload != "turbulence":
-        return other_workload(args)
+    base = {"metric": METRIC, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None}
+    if args.workload in ("sod", "mhd2d", "rmhd3d", "serving"):
+        leg = {"sod": sod_leg, "mhd2d": mhd2d_leg, "rmhd3d": rmhd_leg,
+               "serving": lambda a: serving_leg(a, a.requests, a.seq_sample)}[args.workload](args)
+        print(json.dumps(dict(base, workload=args.workload, **leg)), flush=True)
+        return
 
-    n = args.n or 256
-    d64 = problems.turbulence_desc(n, "fp64")
-    dmx = problems.turbulence_desc(n, "mixed")
-    log(f"generating {n}^3 initial data")
-    P0 = problems.turbulence_3d(d64)
-    zones = n ** 3
-
-    log("timing FP64 form")
-    transfers = {}
-    tune64 = None if args.no_autotune else {}
-    tunemx = None if args.no_autotune else {}
-    el64, P64, ms64, nl64, kname = time_form(d64, P0, args.steps, args.warmup, transfers, tune64)
-    log(f"FP64: {el64:.3f} s for {args.steps} steps; stage kernel {ms64:.3f} ms ({kname}); shape {tune64}")
-    dfast = problems.turbulence_desc(n, "fp64_fast")
-    tunefast = None if args.no_autotune else {}
-    log("timing fast FP64 form")
-    elfa, Pfa, msfa, nlfa, kfa = time_form(dfast, P0, args.steps, args.warmup, None, tunefast)
-    log(f"fast FP64: {elfa:.3f} s for {args.steps} steps; stage kernel {msfa:.3f} ms ({kfa}); shape {tunefast}")
-    log("timing mixed form")
-    elmx, Pmx, msmx, nlmx, _ = time_form(dmx, P0, args.steps, args.warmup, None, tunemx)
-    log(f"mixed: shape {tunemx}")
-    log(f"mixed: {elmx:.3f} s for {args.steps} steps; stage kernel {msmx:.3f} ms")
-
-    g = d64.guards[0]
-    inner = (slice(g, g + n),) * 3
-    U64 = P64[inner]
-    Umx = Pmx[inner]
-    l1 = max(float(np.sum(np.abs(Umx[..., q] - U64[..., q])) / np.sum(np.abs(U64[..., q]))) for q in range(5))
-
-    value = zones * args.steps / el64
-    b_alg = alg_bytes_per_zone_update(d64)
-    per_launch = b_alg * zones / d64.rk_order
-    achieved = per_launch / (ms64 * 1e-3)
-    workload = f"turbulence3d_{n}"
-    roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK, "traffic": traffic_from_profile("stage_fp64", workload),
-                "kernel": kname, "avg_launch_ms": ms64, "launches": nl64,
-                "alg_bytes_per_launch": per_launch}
-    out = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "zone-updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": el64 / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
+    d64, res, transfers, l1_fast10, l1_mixed100, n_mixed = turbulence_leg(args)
+    zones = int(np.prod([d64.cells[a] for a in range(3)]))
+    fast, exact, mixed = res["fp64_fast"], res["fp64"], res["mixed"]
+    workload = f"turbulence3d_{d64.cells[0]}"
+    out = dict(base)
+    out.update({
+        "value": fast["value"], "unit": "zone-updates/s", "ms_per_step": fast["el"] / args.steps * 1e3,
         "dtype": "f64",
         "data": f"synthetic: seeded Gaussian random velocity field (seed {problems.SEED_TURBULENCE})",
-        "config": {"workload": f"3-D decaying turbulence {n}^3, hydro gamma 5/3, PLM theta 1.5 + HLLC, "
-                               f"RK2, CFL {CFL}, periodic", "zones": zones, "form": "fp64",
-                   "launch_shape": tune64, "launch_shape_mixed": tunemx},
-        "roofline": roofline,
-        # the C ABI takes host arrays: PCIe-inclusive rate of one upload, the
-        # timed steps and one download of MeshData::P (not the headline value)
+        "config": {"workload": f"3-D decaying turbulence {d64.cells[0]}^3, hydro gamma 5/3, PLM theta 1.5 + HLLC, "
+                               f"RK2, CFL {CFL}, periodic", "zones": zones,
+                   "form": "fp64_fast: every operation binary64 (FMA, refined reciprocals and square roots), "
+                           "held to relative L1 <= 1e-12 of the reference after 10 steps",
+                   "rel_l1_vs_bit_exact_fp64_after_10_steps": l1_fast10,
+                   "launch_shape": fast.get("launch_shape")},
+        "roofline": roofline(d64, fast["stage_ms"], fast["kernel"], traffic_from_profile("stage_fast", workload)),
+        "fp64_exact": {"value": exact["value"], "ms_per_step": exact["el"] / args.steps * 1e3,
+                       "stage_ms": exact["stage_ms"], "kernel": exact["kernel"],
+                       "roofline_frac": roofline(d64, exact["stage_ms"], exact["kernel"])["frac"],
+                       "traffic": traffic_from_profile("stage_fp64", workload),
+                       "launch_shape": exact.get("launch_shape"),
+                       "note": "bit-identical to the CPU restatement of the reference step"},
+        "mixed": {"value": mixed["value"], "ms_per_step": mixed["el"] / args.steps * 1e3,
+                  "stage_ms": mixed["stage_ms"], "kernel": mixed["kernel"],
+                  "speedup_vs_fp64_exact": mixed["value"] / exact["value"],
+                  "speedup_vs_fp64_fast": mixed["value"] / fast["value"],
+                  f"rel_l1_vs_fp64_after_{n_mixed}_steps": l1_mixed100,
+                  "roofline_frac": roofline(d64, mixed["stage_ms"], mixed["kernel"])["frac"],
+                  "launch_shape": mixed.get("launch_shape")},
         "host_transfer": {"upload_s": transfers["upload_s"], "download_s": transfers["download_s"],
                           "pcie_inclusive_value": zones * args.steps
-                          / (el64 + transfers["upload_s"] + transfers["download_s"])},
-        "mixed": {"value": zones * args.steps / elmx, "ms_per_step": elmx / args.steps * 1e3,
-                  "speedup_vs_fp64": el64 / elmx, "rel_l1_vs_fp64": l1,
-                  "avg_stage_launch_ms": msmx,
-                  "roofline_frac": per_launch / (msmx * 1e-3) / HBM_PEAK},
-    }
-    # the fast form against the plain one (bit-identical to the reference
-    # restatement) after 10 steps of the same dt: the FP64 bound of 1e-12
-    l1_fast10 = fast_vs_plain_after(d64, dfast, P0, 10)
-    out["fp64_fast"] = {"value": zones * args.steps / elfa, "ms_per_step": elfa / args.steps * 1e3,
-                        "speedup_vs_fp64": el64 / elfa, "avg_stage_launch_ms": msfa, "kernel": kfa,
-                        "roofline_frac": per_launch / (msfa * 1e-3) / HBM_PEAK,
-                        "rel_l1_vs_fp64_after_10_steps": l1_fast10, "launch_shape": tunefast}
+                          / (fast["el"] + transfers["upload_s"] + transfers["download_s"])},
+    })
+    if args.workload == "all":
+        configs = {}
+        for name, fn in (("sod", sod_leg), ("mhd2d", mhd2d_leg), ("rmhd3d", rmhd_leg),
+                         ("serving", lambda a: serving_leg(a, a.requests, a.seq_sample))):
+            try:
+                configs[name] = fn(args)
+            except (MaraError, RuntimeError) as e:
+                configs[name] = {"error": str(e)}
+                log(f"{name}: {e}")
+        out["configs"] = configs
     if not args.no_cpu_baseline:
-        log("CPU baseline")
+        log(f"CPU baseline at {args.cpu_sample}^3")
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample)
     print(json.dumps(out), flush=True)
 

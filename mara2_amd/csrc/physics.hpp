@@ -102,112 +102,103 @@ __device__ __forceinline__ R weno5(R vm2, R vm1, R v0, R vp1, R vp2)
 }
 
 // ---------------------------------------------------------------------------
-// QuarticPolynomial (src/QuarticPolynomial.cpp:19-144), binary64 only
+// Real roots of a quartic (the job of QuarticPolynomial::solve,
+// src/QuarticPolynomial.cpp:19-144), binary64 only, by Ferrari's method.
 // ---------------------------------------------------------------------------
-__device__ inline void solve_cubic(double c3, double c2, double c1, double c0,
-                                   double& x1, double& x2, double& x3, int& nr)
+// The largest real root of y^3 + A y^2 + B y + C = 0.  With y = t - A/3 the
+// cubic is t^3 + 3 h t - 2 g = 0 and disc = h^3 + g^2 decides the case:
+// one real root (disc > 0, Cardano: t = cbrt(g + sqrt(disc)) + cbrt(g -
+// sqrt(disc))), three (disc < 0, Viete: the largest is 2 sqrt(-h)
+// cos(phi / 3) with cos(phi) = g / sqrt(-h)^3), or a repeated one (disc = 0:
+// t = 2 cbrt(g) and the double root -cbrt(g)).
+__device__ inline double cubic_largest_root(double A, double B, double C)
 {
-    const double a2 = c2 / c3;
-    const double a1 = c1 / c3;
-    const double a0 = c0 / c3;
-    const double q = a1 / 3.0 - a2 * a2 / 9.0;
-    const double r = (a1 * a2 - 3.0 * a0) / 6.0 - a2 * a2 * a2 / 27.0;
-    const double delta = q * q * q + r * r;
-    if (delta > 0.0)
+    const double shift = A / 3.0;
+    const double h = B / 3.0 - shift * shift;
+    const double g = 0.5 * (A * B / 3.0 - C) - shift * shift * shift;
+    const double disc = h * h * h + g * g;
+    double t;
+    if (disc > 0.0)
     {
-        double s1 = r + sqrt(delta);
-        double s2 = r - sqrt(delta);
-        s1 = (s1 >= 0.0) ? pow(s1, 1. / 3) : -pow(-s1, 1. / 3);
-        s2 = (s2 >= 0.0) ? pow(s2, 1. / 3) : -pow(-s2, 1. / 3);
-        x1 = (s1 + s2) - a2 / 3.0;
-        x2 = x3 = -0.5 * (s1 + s2) - a2 / 3.0;
-        nr = 1;
+        const double w = sqrt(disc);
+        t = cbrt(g + w) + cbrt(g - w);
     }
-    else if (delta < 0.0)
+    else if (disc < 0.0)
     {
-        const double theta = acos(r / sqrt(-q * q * q)) / 3.0;
-        const double costh = cos(theta);
-        const double sinth = sin(theta);
-        const double sq = sqrt(-q);
-        x1 = 2.0 * sq * costh - a2 / 3.0;
-        x2 = -sq * costh - a2 / 3.0 - sqrt(3.0) * sq * sinth;
-        x3 = -sq * costh - a2 / 3.0 + sqrt(3.0) * sq * sinth;
-        nr = 3;
+        const double m = sqrt(-h);
+        double c = g / (m * m * m);
+        c = c > 1.0 ? 1.0 : (c < -1.0 ? -1.0 : c);
+        t = 2.0 * m * cos(acos(c) / 3.0);
     }
     else
     {
-        const double s = (r >= 0.0) ? pow(r, 1. / 3) : -pow(-r, 1. / 3);
-        x1 = 2.0 * s - a2 / 3.0;
-        x2 = x3 = -s - a2 / 3.0;
-        nr = 3;
+        const double c = cbrt(g);
+        t = 2.0 * c > -c ? 2.0 * c : -c;
     }
+    return t - shift;
 }
 
-__device__ inline int quartic_solve(double d4, double d3, double d2, double d1, double d0, double roots[4])
+// Real roots of k4 x^4 + k3 x^3 + k2 x^2 + k1 x + k0 = 0, ascending in
+// roots[]; returns their number, 0, 2 or 4.  For the monic x^4 + b x^3 +
+// c x^2 + d x + e take the largest root y of the resolvent cubic
+// y^3 - c y^2 + (b d - 4 e) y + (4 c e - d^2 - b^2 e) = 0; then
+// x^4 + ... = (x^2 + b x / 2 + y / 2)^2 - (s x + u)^2 with
+// s^2 = b^2 / 4 - c + y, and the quartic is the product of the quadratics
+// x^2 + (b/2 - s) x + ... and x^2 + (b/2 + s) x + ..., whose roots are
+// -b/4 + s/2 +- sqrt(q+)/2 and -b/4 - s/2 +- sqrt(q-)/2 with
+// q+- = 3 b^2 / 4 - 2 c - s^2 +- (b c - 2 d - b^3 / 4) / s.  Where s = 0 the
+// linear term vanishes and q+- = 3 b^2 / 4 - 2 c +- 2 sqrt(y^2 - 4 e).  A
+// quadratic with a negative q contributes no real root.
+__device__ inline int quartic_real_roots(double k4, double k3, double k2, double k1, double k0, double roots[4])
 {
-    const double a3 = d3 / d4;
-    const double a2 = d2 / d4;
-    const double a1 = d1 / d4;
-    const double a0 = d0 / d4;
-    const double au2 = -a2;
-    const double au1 = (a1 * a3 - 4 * a0);
-    const double au0 = 4.0 * a0 * a2 - a1 * a1 - a0 * a3 * a3;
-    int nr, nr12, nr34;
-    double r1, r2, r3, r4, x1, x2, x3;
-    solve_cubic(1.0, au2, au1, au0, x1, x2, x3, nr);
-    const double u1 = (nr == 1) ? x1 : ((x1 > x3) ? x1 : x3);
-    const double R2 = 0.25 * a3 * a3 + u1 - a2;
-    const double R = (R2 > 0.0) ? sqrt(R2) : 0.0;
-    double D2, E2;
-    if (R != 0.0)
+    const double b = k3 / k4, c = k2 / k4, d = k1 / k4, e = k0 / k4;
+    const double y = cubic_largest_root(-c, b * d - 4.0 * e, 4.0 * c * e - d * d - b * b * e);
+    const double s2 = 0.25 * b * b + y - c;
+    const double s = s2 > 0.0 ? sqrt(s2) : 0.0;
+    const double common = 0.75 * b * b - 2.0 * c;
+    double qp, qm;
+    if (s != 0.0)
     {
-        const double foo1 = 0.75 * a3 * a3 - R2 - 2.0 * a2;
-        const double foo2 = 0.25 * (4.0 * a3 * a2 - 8.0 * a1 - a3 * a3 * a3) / R;
-        D2 = foo1 + foo2;
-        E2 = foo1 - foo2;
+        const double lin = 0.25 * (4.0 * b * c - 8.0 * d - b * b * b) / s;
+        qp = common - s2 + lin;
+        qm = common - s2 - lin;
     }
     else
     {
-        const double foo1 = 0.75 * a3 * a3 - 2.0 * a2;
-        const double foo2 = 2.00 * sqrt(u1 * u1 - 4.0 * a0);
-        D2 = foo1 + foo2;
-        E2 = foo1 - foo2;
+        const double w = 2.0 * sqrt(y * y - 4.0 * e);
+        qp = common + w;
+        qm = common - w;
     }
-    if (D2 >= 0.0)
+    const double centre = -0.25 * b;
+    int n = 0;
+    if (qp >= 0.0)
     {
-        const double D = sqrt(D2);
-        r1 = -0.25 * a3 + 0.5 * R - 0.5 * D;
-        r2 = -0.25 * a3 + 0.5 * R + 0.5 * D;
-        nr12 = 2;
+        const double half = 0.5 * sqrt(qp);
+        roots[n++] = centre + 0.5 * s - half;
+        roots[n++] = centre + 0.5 * s + half;
     }
-    else
+    if (qm >= 0.0)
     {
-        r1 = r2 = -0.25 * a3 + 0.5 * R;
-        nr12 = 0;
+        const double half = 0.5 * sqrt(qm);
+        roots[n++] = centre - 0.5 * s - half;
+        roots[n++] = centre - 0.5 * s + half;
     }
-    if (E2 >= 0.0)
+    if (n == 4)
     {
-        const double E = sqrt(E2);
-        r3 = -0.25 * a3 - 0.5 * R - 0.5 * E;
-        r4 = -0.25 * a3 - 0.5 * R + 0.5 * E;
-        nr34 = 2;
+        // two ascending pairs merged by compare-exchange: (0,2), (1,3), (1,2)
+        auto order = [&](int i, int j) {
+            if (roots[j] < roots[i])
+            {
+                const double x = roots[i];
+                roots[i] = roots[j];
+                roots[j] = x;
+            }
+        };
+        order(0, 2);
+        order(1, 3);
+        order(1, 2);
     }
-    else
-    {
-        r3 = r4 = -0.25 * a3 - 0.5 * R;
-        nr34 = 0;
-    }
-    int i = 0;
-    if (nr12 != 0) { roots[i++] = r1; roots[i++] = r2; }
-    if (nr34 != 0) { roots[i++] = r3; roots[i++] = r4; }
-    for (int a = 1; a < i; ++a)  // ascending, same order as std::sort
-    {
-        const double x = roots[a];
-        int b = a - 1;
-        while (b >= 0 && x < roots[b]) { roots[b + 1] = roots[b]; --b; }
-        roots[b + 1] = x;
-    }
-    return nr12 + nr34;
+    return n;
 }
 
 // ---------------------------------------------------------------------------
@@ -347,7 +338,7 @@ __device__ inline int rmhd_eigen(double gamma, const double* P, const double* dA
     const double A1 = -4 * K * v3 - L * vn * 2 - 2 * b0 * bn;
     const double A0 = K * v4 + L * v2 + bn * bn;
     double roots[4];
-    const int nr = quartic_solve(A4, A3, A2, A1, A0, roots);
+    const int nr = quartic_real_roots(A4, A3, A2, A1, A0, roots);
     const double C = sqrt(h0 + bb);
     if (nr == 4)
     {
@@ -648,10 +639,20 @@ __device__ __forceinline__ int prim_to_cons(double gamma, const double* P, doubl
     }
 }
 
-// ConsToPrimRMHD (src/ConsToPrimRMHD.cpp:77-323): Anton 2D Newton, then Noble.
-struct C2P
+// Conserved -> primitive inversion of the relativistic MHD law
+// (ConsToPrimRMHD, src/ConsToPrimRMHD.cpp:77-323): a 2-D Newton iteration on
+// (Z, W) = (rho h W^2, Lorentz factor) after Anton et al. (2006), and where it
+// fails a 1-D Newton iteration on Z after Noble et al. (2006); both stop at a
+// relative step below 1e-12 or after 250 iterations.  The operation order is
+// the reference's: which zones converge, and after how many iterations, is
+// part of the result.
+struct Inversion
 {
-    double gamf, pfloor, D, Tau, S2, B2, BS, BS2, Cons[8], Z_start, W_start;
+    double k_gas;     // (gamma - 1) / gamma: p = k_gas (rho h - rho)
+    double pfloor;    // pressure floor relative to rho, <= 0 off
+    double D, tau, S2, B2, BS, BS2;
+    double U[8];
+    double Z0, W0;    // starting point: Z = sqrt(S^2 + D^2), W = Z / D
 };
 
 __device__ __forceinline__ bool finite8(const double* x)
@@ -662,7 +663,9 @@ __device__ __forceinline__ bool finite8(const double* x)
     return ok;
 }
 
-__device__ __forceinline__ int c2p_check_cons(const double* U)
+// status codes of the inversion (0 success; 1 NaN in U, 2 D < 0, 3 tau < 0,
+// 4 NaN in P, 5 p < 0, 6 rho < 0, 7 v.v >= 1, 8 iteration limit)
+__device__ __forceinline__ int inversion_check_input(const double* U)
 {
     if (U[0] < 0.0) return 2;
     if (U[4] < 0.0) return 3;
@@ -670,18 +673,19 @@ __device__ __forceinline__ int c2p_check_cons(const double* U)
     return 0;
 }
 
-__device__ __forceinline__ int c2p_reconstruct(const C2P& s, double Z, double W, double* Pout)
+// the primitives of a converged (Z, W)
+__device__ __forceinline__ int inversion_primitives(const Inversion& s, double Z, double W, double* Pout)
 {
     double P[8];
-    const double b0 = s.BS * W / Z;
+    const double bw = s.BS * W / Z;
     P[0] = s.D / W;
-    P[4] = (s.D / W) * (Z / (s.D * W) - 1.0) * s.gamf;
-    P[1] = (s.Cons[1] + b0 * s.Cons[5] / W) / (Z + s.B2);
-    P[2] = (s.Cons[2] + b0 * s.Cons[6] / W) / (Z + s.B2);
-    P[3] = (s.Cons[3] + b0 * s.Cons[7] / W) / (Z + s.B2);
-    P[5] = s.Cons[5];
-    P[6] = s.Cons[6];
-    P[7] = s.Cons[7];
+    P[4] = (s.D / W) * (Z / (s.D * W) - 1.0) * s.k_gas;
+    P[1] = (s.U[1] + bw * s.U[5] / W) / (Z + s.B2);
+    P[2] = (s.U[2] + bw * s.U[6] / W) / (Z + s.B2);
+    P[3] = (s.U[3] + bw * s.U[7] / W) / (Z + s.B2);
+    P[5] = s.U[5];
+    P[6] = s.U[6];
+    P[7] = s.U[7];
     if (P[4] <= 0.0 && s.pfloor > 0.0) P[4] = P[0] * s.pfloor;
     const double v2 = P[1] * P[1] + P[2] * P[2] + P[3] * P[3];
     if (v2 >= 1.0) return 7;
@@ -693,89 +697,93 @@ __device__ __forceinline__ int c2p_reconstruct(const C2P& s, double Z, double W,
     return 0;
 }
 
-__device__ inline int c2p_anton(const C2P& s, double* P)
+// Newton on the residuals r_mom (momentum: S^2 as a function of Z, W) and
+// r_en (energy: tau) in (Z, W), with the inverse Jacobian written out
+__device__ inline int invert_zw(const Inversion& s, double* P)
 {
-    if (int bad = c2p_check_cons(s.Cons)) return bad;
+    if (int bad = inversion_check_input(s.U)) return bad;
     int iterations = 0;
-    double error = 1.0;
-    double W = s.W_start, Z = s.Z_start;
-    const double D = s.D, Tau = s.Tau, S2 = s.S2, B2 = s.B2, BS2 = s.BS2, gamf = s.gamf;
-    while (error > 1e-12)
+    double step = 1.0;
+    double W = s.W0, Z = s.Z0;
+    const double D = s.D, tau = s.tau, S2 = s.S2, B2 = s.B2, BS2 = s.BS2, kg = s.k_gas;
+    while (step > 1e-12)
     {
         const double Z2 = Z * Z;
         const double Z3 = Z * Z2;
         const double W2 = W * W;
         const double W3 = W * W2;
-        const double Pre = (D / W) * (Z / (D * W) - 1.0) * gamf;
-        const double df0dZ = 2 * (B2 + Z) * (BS2 * W2 + (W2 - 1) * Z3) / (W2 * Z3);
-        const double df0dW = 2 * (B2 + Z) * (B2 + Z) / W3;
-        const double df1dZ = 1.0 + BS2 / Z3 - gamf / W2;
-        const double df1dW = B2 / W3 + (2 * Z - D * W) / W3 * gamf;
-        const double f0 = -S2 + (Z + B2) * (Z + B2) * (W2 - 1) / W2 - (2 * Z + B2) * BS2 / Z2;
-        const double f1 = -Tau + Z + B2 - Pre - 0.5 * B2 / W2 - 0.5 * BS2 / Z2 - D;
-        const double det = df0dZ * df1dW - df0dW * df1dZ;
-        const double G0 = df1dW / det, G1 = -df0dW / det;
-        const double G2 = -df1dZ / det, G3 = df0dZ / det;
-        const double dZ = -(G0 * f0 + G1 * f1);
-        const double dW = -(G2 * f0 + G3 * f1);
-        double Z_new = Z + dZ;
-        double W_new = W + dW;
-        Z_new = (Z_new > 0.0) ? Z_new : -Z_new;
-        Z_new = (Z_new < 1e20) ? Z_new : Z;
-        W_new = (W_new > 1.0) ? W_new : 1.0;
-        W_new = (W_new < 1e12) ? W_new : 1e12;
-        Z = Z_new;
-        W = W_new;
-        error = fabs(dZ / Z) + fabs(dW / W);
+        const double p = (D / W) * (Z / (D * W) - 1.0) * kg;
+        const double J00 = 2 * (B2 + Z) * (BS2 * W2 + (W2 - 1) * Z3) / (W2 * Z3);
+        const double J01 = 2 * (B2 + Z) * (B2 + Z) / W3;
+        const double J10 = 1.0 + BS2 / Z3 - kg / W2;
+        const double J11 = B2 / W3 + (2 * Z - D * W) / W3 * kg;
+        const double r_mom = -S2 + (Z + B2) * (Z + B2) * (W2 - 1) / W2 - (2 * Z + B2) * BS2 / Z2;
+        const double r_en = -tau + Z + B2 - p - 0.5 * B2 / W2 - 0.5 * BS2 / Z2 - D;
+        const double det = J00 * J11 - J01 * J10;
+        const double K00 = J11 / det, K01 = -J01 / det;
+        const double K10 = -J10 / det, K11 = J00 / det;
+        const double dZ = -(K00 * r_mom + K01 * r_en);
+        const double dW = -(K10 * r_mom + K11 * r_en);
+        double Zn = Z + dZ;
+        double Wn = W + dW;
+        Zn = (Zn > 0.0) ? Zn : -Zn;
+        Zn = (Zn < 1e20) ? Zn : Z;
+        Wn = (Wn > 1.0) ? Wn : 1.0;
+        Wn = (Wn < 1e12) ? Wn : 1e12;
+        Z = Zn;
+        W = Wn;
+        step = fabs(dZ / Z) + fabs(dW / W);
         ++iterations;
         if (iterations == 250) return 8;
     }
-    return c2p_reconstruct(s, Z, W, P);
+    return inversion_primitives(s, Z, W, P);
 }
 
-__device__ inline int c2p_noble(const C2P& s, double* P)
+// Newton on the energy residual in Z alone, v^2 and W eliminated through the
+// momentum relation v^2 = num / den
+__device__ inline int invert_z(const Inversion& s, double* P)
 {
-    if (int bad = c2p_check_cons(s.Cons)) return bad;
+    if (int bad = inversion_check_input(s.U)) return bad;
     int iterations = 0;
-    double error = 1.0;
-    double Z = s.Z_start;
-    const double D = s.D, Tau = s.Tau, S2 = s.S2, B2 = s.B2, BS2 = s.BS2, gamf = s.gamf;
-    while (error > 1e-12)
+    double step = 1.0;
+    double Z = s.Z0;
+    const double D = s.D, tau = s.tau, S2 = s.S2, B2 = s.B2, BS2 = s.BS2, kg = s.k_gas;
+    while (step > 1e-12)
     {
         const double Z2 = Z * Z;
         const double Z3 = Z * Z2;
-        const double a = S2 * Z2 + BS2 * (B2 + 2 * Z);
-        const double b = (B2 + Z) * (B2 + Z) * Z2;
-        const double ap = 2 * (S2 * Z + BS2);
-        const double bp = 2 * Z * (B2 + Z) * (B2 + 2 * Z);
-        const double V2 = a / b;
-        const double W2 = 1.0 / (1.0 - V2);
+        const double num = S2 * Z2 + BS2 * (B2 + 2 * Z);
+        const double den = (B2 + Z) * (B2 + Z) * Z2;
+        const double dnum = 2 * (S2 * Z + BS2);
+        const double dden = 2 * Z * (B2 + Z) * (B2 + 2 * Z);
+        const double v2 = num / den;
+        const double W2 = 1.0 / (1.0 - v2);
         const double W = sqrt(W2);
         const double W3 = W * W2;
-        const double Pre = (D / W) * (Z / (D * W) - 1.0) * gamf;
-        const double dv2dZ = (ap * b - bp * a) / (b * b);
-        const double delPdelZ = gamf / W2;
-        const double delPdelW = gamf * (D / W2 - 2 * Z / W3);
-        const double dWdv2 = 0.5 * W3;
-        const double dPdZ = delPdelW * dWdv2 * dv2dZ + delPdelZ;
-        const double f = Tau + D - 0.5 * B2 * (1 + V2) + 0.5 * BS2 / Z2 - Z + Pre;
-        const double g = -0.5 * B2 * dv2dZ - BS2 / Z3 - 1.0 + dPdZ;
-        const double dZ = -f / g;
-        double Z_new = Z + dZ;
-        Z_new = (Z_new > 0.0) ? Z_new : -Z_new;
-        Z_new = (Z_new < 1e20) ? Z_new : Z;
-        Z = Z_new;
-        error = fabs(dZ / Z);
+        const double p = (D / W) * (Z / (D * W) - 1.0) * kg;
+        const double dv2_dZ = (dnum * den - dden * num) / (den * den);
+        const double dp_dZ_atW = kg / W2;
+        const double dp_dW_atZ = kg * (D / W2 - 2 * Z / W3);
+        const double dW_dv2 = 0.5 * W3;
+        const double dp_dZ = dp_dW_atZ * dW_dv2 * dv2_dZ + dp_dZ_atW;
+        const double r = tau + D - 0.5 * B2 * (1 + v2) + 0.5 * BS2 / Z2 - Z + p;
+        const double dr = -0.5 * B2 * dv2_dZ - BS2 / Z3 - 1.0 + dp_dZ;
+        const double dZ = -r / dr;
+        double Zn = Z + dZ;
+        Zn = (Zn > 0.0) ? Zn : -Zn;
+        Zn = (Zn < 1e20) ? Zn : Z;
+        Z = Zn;
+        step = fabs(dZ / Z);
         ++iterations;
         if (iterations == 250) return 8;
     }
     const double Z2 = Z * Z;
-    const double a = S2 * Z2 + BS2 * (B2 + 2 * Z);
-    const double b = (B2 + Z) * (B2 + Z) * Z2;
-    const double V2 = a / b;
-    const double W2 = 1.0 / (1.0 - V2);
+    const double num = S2 * Z2 + BS2 * (B2 + 2 * Z);
+    const double den = (B2 + Z) * (B2 + Z) * Z2;
+    const double v2 = num / den;
+    const double W2 = 1.0 / (1.0 - v2);
     const double W = sqrt(W2);
-    return c2p_reconstruct(s, Z, W, P);
+    return inversion_primitives(s, Z, W, P);
 }
 
 // fromConserved -> primitive (src/ConservationLaws.cpp:238-251, 370-414,
@@ -830,21 +838,21 @@ __device__ __forceinline__ int cons_to_prim(double gamma, double pfloor, const d
     }
     else
     {
-        C2P s;
-        s.gamf = (gamma - 1.0) / gamma;
+        Inversion s;
+        s.k_gas = (gamma - 1.0) / gamma;
         s.pfloor = pfloor;
         s.D = U[0];
-        s.Tau = U[4];
+        s.tau = U[4];
         s.S2 = U[1] * U[1] + U[2] * U[2] + U[3] * U[3];
         s.B2 = U[5] * U[5] + U[6] * U[6] + U[7] * U[7];
         s.BS = U[5] * U[1] + U[6] * U[2] + U[7] * U[3];
         s.BS2 = s.BS * s.BS;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) s.Cons[i] = U[i];
-        s.Z_start = sqrt(s.S2 + s.D * s.D);
-        s.W_start = s.Z_start / s.D;
-        if (c2p_anton(s, P) == 0) return 0;
-        if (c2p_noble(s, P) == 0) return 0;
+        for (int i = 0; i < 8; ++i) s.U[i] = U[i];
+        s.Z0 = sqrt(s.S2 + s.D * s.D);
+        s.W0 = s.Z0 / s.D;
+        if (invert_zw(s, P) == 0) return 0;
+        if (invert_z(s, P) == 0) return 0;
         return 1;
     }
 }

@@ -13,6 +13,7 @@ from mara2_amd.abi import MaraDesc, MaraReport, array_shape, num_fields
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
+_LIB_O3 = None
 _REF = None
 
 _d = ctypes.c_double
@@ -28,39 +29,48 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
-def lib():
-    global _LIB
+def lib(timing=False):
+    """The checker build (no contraction), or with timing=True the -O3 build
+    of the same source that bench.py times as the CPU baseline."""
+    global _LIB, _LIB_O3
+    if timing:
+        if _LIB_O3 is None:
+            _LIB_O3 = _load(os.path.join(_HERE, "liboracle_o3.so"))
+        return _LIB_O3
     if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
-        if not os.path.exists(path):
-            build()
-        L = ctypes.CDLL(path)
-        L.oracle_plm.restype = _d
-        L.oracle_weno5.restype = _d
-        L.oracle_weno5.argtypes = [_dp, _i, _i, _d]
-        L.oracle_plm.argtypes = [_dp, _d, _d]
-        L.oracle_quartic.argtypes = [_dp, _dp]
-        L.oracle_rmhd_c2p.argtypes = [_dp, _d, _d, _dp, _ip]
-        L.oracle_from_primitive.argtypes = [_i, _d, _dp, _dp, _dp, _dp, _dp]
-        L.oracle_from_conserved.argtypes = [_i, _d, _d, _dp, _dp, _ip]
-        L.oracle_riemann.argtypes = [_i, _i, _d, _dp, _dp, _dp, _dp]
-        L.oracle_ct_correct.argtypes = [_dp, _ip, _i, _i]
-        L.oracle_divergence.argtypes = [_dp, _ip, _i, _d, _dp, _d, _dp]
-        L.oracle_monopole_cell.argtypes = [_dp, _ip, _dp]
-        L.oracle_vector_potential_to_B.argtypes = [ctypes.POINTER(MaraDesc), _dp, _dp]
-        L.oracle_apply_bc.argtypes = [ctypes.POINTER(MaraDesc), _dp]
-        L.oracle_advance.argtypes = [ctypes.POINTER(MaraDesc), _dp, _d, _d, ctypes.POINTER(MaraReport)]
-        L.oracle_advance_src.argtypes = [ctypes.POINTER(MaraDesc), _dp, _d, _d, ctypes.c_void_p, _dp, _dp,
-                                         ctypes.POINTER(MaraReport)]
-        L.oracle_advance_visc.argtypes = [ctypes.POINTER(MaraDesc), _dp, _d, _d, ctypes.c_void_p, _dp, _dp,
-                                          ctypes.POINTER(MaraReport)]
-        L.oracle_courant_dt.argtypes = [ctypes.POINTER(MaraDesc), _dp, _dp]
-        L.oracle_monopole_max.argtypes = [ctypes.POINTER(MaraDesc), _dp, _dp]
-        L.oracle_set_threads.argtypes = [_i]
-        L.oracle_rmhd_c2p_codes.argtypes = [_dp, _d, _d, _ip]
-        L.oracle_last_failure.argtypes = [_dp]
-        _LIB = L
+        _LIB = _load(os.path.join(_HERE, "liboracle.so"))
     return _LIB
+
+
+def _load(path):
+    if not os.path.exists(path):
+        build()
+    L = ctypes.CDLL(path)
+    L.oracle_plm.restype = _d
+    L.oracle_weno5.restype = _d
+    L.oracle_weno5.argtypes = [_dp, _i, _i, _d]
+    L.oracle_plm.argtypes = [_dp, _d, _d]
+    L.oracle_quartic.argtypes = [_dp, _dp]
+    L.oracle_rmhd_c2p.argtypes = [_dp, _d, _d, _dp, _ip]
+    L.oracle_from_primitive.argtypes = [_i, _d, _dp, _dp, _dp, _dp, _dp]
+    L.oracle_from_conserved.argtypes = [_i, _d, _d, _dp, _dp, _ip]
+    L.oracle_riemann.argtypes = [_i, _i, _d, _dp, _dp, _dp, _dp]
+    L.oracle_ct_correct.argtypes = [_dp, _ip, _i, _i]
+    L.oracle_divergence.argtypes = [_dp, _ip, _i, _d, _dp, _d, _dp]
+    L.oracle_monopole_cell.argtypes = [_dp, _ip, _dp]
+    L.oracle_vector_potential_to_B.argtypes = [ctypes.POINTER(MaraDesc), _dp, _dp]
+    L.oracle_apply_bc.argtypes = [ctypes.POINTER(MaraDesc), _dp]
+    L.oracle_advance.argtypes = [ctypes.POINTER(MaraDesc), _dp, _d, _d, ctypes.POINTER(MaraReport)]
+    L.oracle_advance_src.argtypes = [ctypes.POINTER(MaraDesc), _dp, _d, _d, ctypes.c_void_p, _dp, _dp,
+                                     ctypes.POINTER(MaraReport)]
+    L.oracle_advance_visc.argtypes = [ctypes.POINTER(MaraDesc), _dp, _d, _d, ctypes.c_void_p, _dp, _dp,
+                                      ctypes.POINTER(MaraReport)]
+    L.oracle_courant_dt.argtypes = [ctypes.POINTER(MaraDesc), _dp, _dp]
+    L.oracle_monopole_max.argtypes = [ctypes.POINTER(MaraDesc), _dp, _dp]
+    L.oracle_set_threads.argtypes = [_i]
+    L.oracle_rmhd_c2p_codes.argtypes = [_dp, _d, _d, _ip]
+    L.oracle_last_failure.argtypes = [_dp]
+    return L
 
 
 def ref():

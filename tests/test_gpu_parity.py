@@ -157,7 +157,9 @@ def test_rmhd_3d_weno_ct():
     desc = problems.rmhd_desc(32)
     P0 = problems.rmhd_3d(desc, kmax=2, wmax=3.0)
     Pg, Po, dg, do = run_pair(desc, P0, 10, 0.3, Block)
-    assert abs(dg[0] - do[0]) <= 1e-14 * do[0]
+    # the eigenvalue quartic is solved in closed form on both sides, with
+    # different rounding: the CFL wave speeds agree to the FP64 bound
+    assert abs(dg[0] - do[0]) <= REL_L1_FP64 * do[0]
     _check_fp64(desc, Pg, Po, exact=False)
     _monopole_pair(desc, Pg, Po)
 
