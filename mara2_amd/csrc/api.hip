@@ -170,13 +170,15 @@ void launch_courant(mara_block* b)
     }
 }
 
-// The stage kernels of the block's form: binary32 fluxes (mixed), the fast
-// binary64 kernel where it serves the block (its fused hydro PLM stage), else
-// the plain binary64 kernels.
+// The stage kernels of the block's form: the fast tiles where they serve
+// the block (its fused hydro PLM stage) with binary32 faces in the mixed form
+// and binary64 faces in the fast binary64 form; else binary32 fluxes (mixed)
+// or the plain binary64 kernels.
 void run_stage_kernels(const StageArgs& s, bool mixed)
 {
-    if (mixed) stage_kernels_f32(s);
-    else if (s.d->precision == MARA_PREC_FP64_FAST && s.fused && fast64_supported(s.d)) stage_kernels_fast64(s);
+    const bool fast_tile = s.fused && fast64_supported(s.d);
+    if (mixed) fast_tile ? stage_kernels_fast32(s) : stage_kernels_f32(s);
+    else if (s.d->precision == MARA_PREC_FP64_FAST && fast_tile) stage_kernels_fast64(s);
     else stage_kernels<double>(s);
 }
 

@@ -15,6 +15,7 @@
 #include "kernels.hpp"
 #include "fused.hpp"
 #include "fused_mhd.hpp"
+#include "fused_fast.hpp"
 
 namespace mara {
 
@@ -117,6 +118,23 @@ kb_fused_plm(const BItem* __restrict__ items, const BTile* __restrict__ tiles, d
     double* Pout = last ? it.Pa : it.Pb;
     fused_plm_tile<Riem, R, Z>(Pin, it.Pa, it.U, Pout, G, b, first, !last, courant_out, it.ctl, T.x0, T.y0, T.z0,
                                kz);
+}
+
+// The mixed form's 2-D and 3-D hydro + PLM items: the single block's tiles
+// of the fast forms with binary32 faces (fast64::fast_plm_tile).
+template <int Riem, bool Z>
+static __global__ void __launch_bounds__(fast64::FastTile<float>::TX * fast64::FastTile<float>::TY)
+    __attribute__((amdgpu_waves_per_eu(fast64::FastTile<float>::WAVES, 8)))
+kb_fast_plm(const BItem* __restrict__ items, const BTile* __restrict__ tiles, double b, int first, int last,
+            int courant_out, int kz)
+{
+    const BTile T = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    const BItem& it = items[T.item];
+    if (!it.ctl->active) return;
+    const Geom G = it.G;
+    const double* Pin = first ? it.Pa : it.Pb;
+    double* Pout = last ? it.Pa : it.Pb;
+    fast64::fast_plm_tile<float, Riem, Z>(Pin, it.Pa, Pout, G, b, first, courant_out, it.ctl, T.x0, T.y0, T.z0, kz);
 }
 
 // 2-D MHD items of an RK2 batch: the single block's fused MHD tiles (fluxes,

@@ -204,8 +204,17 @@ void pencil_dispatch(mara_batch* B, const ClassKey& k, const Segment& s, double 
 template <int Riem, typename R, bool Z>
 void fused_launch(mara_batch* B, const BTile* t, int count, double b, int first, int last, int cout)
 {
-    kb_fused_plm<Riem, R, Z><<<count, dim3(FUSED_TX, PlmTile<R>::TY), 0, B->cur>>>(B->d_items, t, b, first, last, cout,
-                                                                                  B->kz);
+    // the mixed form runs the single block's tiles of the fast forms (the
+    // same 64 x 8 tile shape as PlmTile<float>), the binary64 form the plain ones
+    if constexpr (sizeof(R) == 4)
+    {
+        static_assert(fast64::FastTile<float>::TY == PlmTile<float>::TY, "the batch's tile table assumes one height");
+        kb_fast_plm<Riem, Z><<<count, dim3(64, fast64::FastTile<float>::TY), 0, B->cur>>>(B->d_items, t, b, first,
+                                                                                       last, cout, B->kz);
+    }
+    else
+        kb_fused_plm<Riem, R, Z><<<count, dim3(FUSED_TX, PlmTile<R>::TY), 0, B->cur>>>(B->d_items, t, b, first, last,
+                                                                                      cout, B->kz);
 }
 
 template <int Recon, typename R>

@@ -666,7 +666,7 @@ inline const char* fused_kernel_name(const mara_desc& d)
 {
     const bool mixed = d.precision == MARA_PREC_MIXED;
     if (d.precision == MARA_PREC_FP64_FAST && d.recon == MARA_RECON_PLM) return "k_fast_plm<double>";
-    if (d.recon == MARA_RECON_PLM) return mixed ? "k_fused_plm<float>" : "k_fused_plm<double>";
+    if (d.recon == MARA_RECON_PLM) return mixed ? "k_fast_plm<float>" : "k_fused_plm<double>";
     return mixed ? "k_fused_hydro<float>" : "k_fused_hydro<double>";
 }
 

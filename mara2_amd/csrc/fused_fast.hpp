@@ -1,16 +1,19 @@
-// fused_fast.hpp -- the binary64 stage kernel of the "fast" FP64 form for
-// the Newtonian hydro law with PLM on 2-D and 3-D blocks.
+// fused_fast.hpp -- stage kernel of the fast forms for the Newtonian hydro
+// law with PLM on 2-D and 3-D blocks: the fast binary64 form (R = double)
+// and the mixed form (R = float: reconstruction, face states and Riemann
+// fluxes in binary32, the zone update in binary64).
 //
 // Same method and the same 2.5-D tile march as fused.hpp (k_fused_plm):
 // half-slopes shared by a zone's two faces, plane k with its x-y halo in LDS,
 // the zone's own z column in registers, the next tile-row's faces on the
-// tile edge.  What differs is the arithmetic, which is binary64 throughout
-// but not the reference's rounding:
-//   - divisions are products with a reciprocal (hardware v_rcp_f64 refined by
-//     two Newton steps) and square roots come from v_rsq_f64 refined the same
-//     way: both within a few ulp, neither correctly rounded;
-//   - multiply-adds contract to FMA (the translation unit is compiled with
-//     -ffp-contract=fast);
+// tile edge.  What differs is the arithmetic, which does not keep the
+// reference's rounding:
+//   - divisions are products with a reciprocal and square roots come from a
+//     reciprocal square root; in binary64 both are the hardware
+//     approximations (v_rcp_f64, v_rsq_f64, relative error ~2^-23) refined by
+//     two Newton steps, within a few ulp, in binary32 the hardware ones;
+//   - multiply-adds contract to FMA (the translation units are compiled with
+//     -ffp-contract=on);
 //   - the limiter is minmod written with min/max, the face normals are the
 //     coordinate axes (the reference's yhat carries a 6e-17 x component that
 //     changes nothing above roundoff), the wave-speed extremes are vn -/+ cs;
@@ -18,12 +21,16 @@
 //     needs one reciprocal for its star state, HLLE one per face;
 //   - the zone update takes one reciprocal of the density;
 //   - the RK stage state is carried as the primitives alone: a stage's U is
-//     U(Pin) of the zone's own input value (where the reference carries U
-//     between stages; the two differ by the rounding of one P(U) -> U(P)
-//     round trip), so the kernel neither reads nor writes the U array, and
-//     the first stage takes U0 from that value instead of re-reading P0.
-// The form is held to relative L1 <= 1e-12 of the conserved fields against
-// the reference step after 10 steps, not to bit equality.
+//     U(Pin) of the zone's own binary64 input value (where the reference
+//     carries U between stages; the two differ by the rounding of one
+//     P(U) -> U(P) round trip), so the kernel neither reads nor writes the U
+//     array, and the binary64 form's first stage takes U0 from that value
+//     instead of re-reading P0;
+//   - arrays read once (P0) or written once (Pout) use non-temporal access,
+//     which leaves the L2 to the input planes that neighbouring tiles re-read
+//     as their halo.
+// The binary64 form is held to relative L1 <= 1e-12 of the reference's
+// conserved fields after 10 steps, the mixed form to 1e-5.
 #pragma once
 
 #include "fused.hpp"
@@ -31,105 +38,123 @@
 namespace mara {
 namespace fast64 {
 
-// 1/x: v_rcp_f64 and two Newton steps
-__device__ __forceinline__ double rcp(double x)
+template <typename R> struct Fast;
+
+template <> struct Fast<double>
 {
-    double y = __builtin_amdgcn_rcp(x);
-    double e = __builtin_fma(-x, y, 1.0);
-    y = __builtin_fma(y, e, y);
-    e = __builtin_fma(-x, y, 1.0);
-    return __builtin_fma(y, e, y);
+    // 1/x: v_rcp_f64 and two Newton steps
+    static __device__ __forceinline__ double rcp(double x)
+    {
+        double y = __builtin_amdgcn_rcp(x);
+        double e = __builtin_fma(-x, y, 1.0);
+        y = __builtin_fma(y, e, y);
+        e = __builtin_fma(-x, y, 1.0);
+        return __builtin_fma(y, e, y);
+    }
+    // 1/sqrt(s) for s > 0: v_rsq_f64 and two Newton steps y += y (1 - s y^2) / 2
+    static __device__ __forceinline__ double rsqrt(double s)
+    {
+        double r = __builtin_amdgcn_rsq(s);
+        double e = __builtin_fma(-(s * r), r, 1.0);
+        r = __builtin_fma(0.5 * r, e, r);
+        e = __builtin_fma(-(s * r), r, 1.0);
+        return __builtin_fma(0.5 * r, e, r);
+    }
+};
+
+template <> struct Fast<float>
+{
+    static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+    static __device__ __forceinline__ float rsqrt(float s) { return __builtin_amdgcn_rsqf(s); }
+};
+
+// sqrt(x / y) for x >= 0, y > 0 as x / sqrt(x y); 0 when x == 0
+template <typename R>
+__device__ __forceinline__ R sqrt_ratio(R x, R y)
+{
+    const R s = x * y;
+    return s == R(0) ? R(0) : x * Fast<R>::rsqrt(s);
 }
 
-// sqrt(x / y) for x >= 0, y > 0 as x * rsqrt(x y): v_rsq_f64 (relative
-// error about 2^-23) and two Newton steps y <- y + y (1 - s y^2) / 2; 0 when
-// x == 0
-__device__ __forceinline__ double sqrt_ratio(double x, double y)
+// 0.5 * minmod(theta (v0 - vl), (vr - vl) / 2, theta (vr - v0)): the value
+// of least magnitude when all three share a sign, else 0 (th2 = theta / 2):
+// max(min(a, b, c), min(max(a, b, c), 0))
+template <typename R>
+__device__ __forceinline__ R half_slope(R th2, R vl, R v0, R vr)
 {
-    const double s = x * y;
-    double r = __builtin_amdgcn_rsq(s);
-    double e = __builtin_fma(-(s * r), r, 1.0);
-    r = __builtin_fma(0.5 * r, e, r);
-    e = __builtin_fma(-(s * r), r, 1.0);
-    r = __builtin_fma(0.5 * r, e, r);
-    return s == 0.0 ? 0.0 : x * r;
+    const R dl = v0 - vl, dr = vr - v0;
+    const R a = th2 * dl, c = th2 * dr, b = R(0.25) * (dl + dr);
+    const R lo = __builtin_fmin(a, __builtin_fmin(b, c));
+    const R hi = __builtin_fmax(a, __builtin_fmax(b, c));
+    return __builtin_fmax(lo, __builtin_fmin(hi, R(0)));
 }
 
-// 0.5 * minmod(theta (v0 - vl), (vr - vl) / 2, theta (vr - v0)): the value of
-// least magnitude when all three share a sign, else 0 (th2 = theta / 2)
-__device__ __forceinline__ double half_slope(double th2, double vl, double v0, double vr)
-{
-    const double dl = v0 - vl, dr = vr - v0;
-    const double a = th2 * dl, c = th2 * dr, b = 0.25 * (dl + dr);
-    const double lo = __builtin_fmin(a, __builtin_fmin(b, c));
-    const double hi = __builtin_fmax(a, __builtin_fmax(b, c));
-    return __builtin_fmax(lo, 0.0) + __builtin_fmin(hi, 0.0);
-}
-
+template <typename R>
 struct Consts
 {
-    double gamma, igm1, gm1, th2;
+    R gamma, igm1, th2;
 };
 
 // sound speed sqrt(gamma p / rho)
-__device__ __forceinline__ double sound(const Consts& K, double rho, double p) { return sqrt_ratio(K.gamma * p, rho); }
+template <typename R>
+__device__ __forceinline__ R sound(const Consts<R>& K, R rho, R p) { return sqrt_ratio(K.gamma * p, rho); }
 
 // HLLC flux (Toro eqs. 10.33, 10.58; HllcNewtonianHydroRiemannSolver,
 // src/RiemannSolvers.cpp:63-118) through a face normal to axis AX from the
 // primitive states L and R.  The signal speeds are the extremes of the two
 // states' eigenvalues and 0 (am <= 0 <= ap); the flux is the contact side's
 // F + a (U* - U), which is that side's F where the side is supersonic (a = 0).
-template <int AX>
-__device__ __forceinline__ void hllc(const Consts& K, const double* L, const double* R, double* F)
+template <int AX, typename R>
+__device__ __forceinline__ void hllc(const Consts<R>& K, const R* L, const R* Rt, R* F)
 {
-    const double vnl = L[1 + AX], vnr = R[1 + AX];
-    const double csl = sound(K, L[0], L[4]), csr = sound(K, R[0], R[4]);
-    const double am = __builtin_fmin(0.0, __builtin_fmin(vnl - csl, vnr - csr));
-    const double ap = __builtin_fmax(0.0, __builtin_fmax(vnl + csl, vnr + csr));
-    const double dl = am - vnl, dr = ap - vnr;
-    const double lc = ((R[4] - R[0] * vnr * dr) - (L[4] - L[0] * vnl * dl)) * rcp(L[0] * dl - R[0] * dr);
-    const bool left = 0.0 <= lc;
-    double P[5];
+    const R vnl = L[1 + AX], vnr = Rt[1 + AX];
+    const R csl = sound(K, L[0], L[4]), csr = sound(K, Rt[0], Rt[4]);
+    const R am = __builtin_fmin(R(0), __builtin_fmin(vnl - csl, vnr - csr));
+    const R ap = __builtin_fmax(R(0), __builtin_fmax(vnl + csl, vnr + csr));
+    const R dl = am - vnl, dr = ap - vnr;
+    const R lc = ((Rt[4] - Rt[0] * vnr * dr) - (L[4] - L[0] * vnl * dl)) * Fast<R>::rcp(L[0] * dl - Rt[0] * dr);
+    const bool left = R(0) <= lc;
+    R P[5];
 #pragma unroll
-    for (int q = 0; q < 5; ++q) P[q] = left ? L[q] : R[q];
-    const double a = left ? am : ap;
-    const double D = left ? dl : dr;  // a - vn
-    const double r = P[0], p = P[4], vn = P[1 + AX];
-    const double E = r * 0.5 * (P[1] * P[1] + P[2] * P[2] + P[3] * P[3]) + p * K.igm1;
-    const double k = a - lc;
-    const double ik = k != 0.0 ? rcp(k) : 0.0;
-    const double f = r * D * ik;  // rho* = rho (a - vn) / (a - lc)
+    for (int q = 0; q < 5; ++q) P[q] = left ? L[q] : Rt[q];
+    const R a = left ? am : ap;
+    const R D = left ? dl : dr;  // a - vn
+    const R r = P[0], p = P[4], vn = P[1 + AX];
+    const R E = r * R(0.5) * (P[1] * P[1] + P[2] * P[2] + P[3] * P[3]) + p * K.igm1;
+    const R k = a - lc;
+    const R ik = k != R(0) ? Fast<R>::rcp(k) : R(0);
+    const R f = r * D * ik;  // rho* = rho (a - vn) / (a - lc)
     F[0] = vn * r + a * (f - r);
 #pragma unroll
     for (int t = 0; t < 3; ++t)
     {
-        const double Ut = r * P[1 + t];
-        const double Us = f * (t == AX ? lc : P[1 + t]);
-        F[1 + t] = vn * Ut + (t == AX ? p : 0.0) + a * (Us - Ut);
+        const R Ut = r * P[1 + t];
+        const R Us = f * (t == AX ? lc : P[1 + t]);
+        F[1 + t] = vn * Ut + (t == AX ? p : R(0)) + a * (Us - Ut);
     }
-    const double Es = ik * (D * E + (lc - vn) * (r * D * lc + p));
+    const R Es = ik * (D * E + (lc - vn) * (r * D * lc + p));
     F[4] = vn * (E + p) + a * (Es - E);
 }
 
 // HLLE flux (HlleRiemannSolver, src/RiemannSolvers.cpp:38-57)
-template <int AX>
-__device__ __forceinline__ void hlle(const Consts& K, const double* L, const double* R, double* F)
+template <int AX, typename R>
+__device__ __forceinline__ void hlle(const Consts<R>& K, const R* L, const R* Rt, R* F)
 {
-    const double vnl = L[1 + AX], vnr = R[1 + AX];
-    const double csl = sound(K, L[0], L[4]), csr = sound(K, R[0], R[4]);
-    const double am = __builtin_fmin(0.0, __builtin_fmin(vnl - csl, vnr - csr));
-    const double ap = __builtin_fmax(0.0, __builtin_fmax(vnl + csl, vnr + csr));
-    const double iw = rcp(ap - am);
-    const double El = L[0] * 0.5 * (L[1] * L[1] + L[2] * L[2] + L[3] * L[3]) + L[4] * K.igm1;
-    const double Er = R[0] * 0.5 * (R[1] * R[1] + R[2] * R[2] + R[3] * R[3]) + R[4] * K.igm1;
-    double Ul[5], Ur[5], Fl[5], Fr[5];
+    const R vnl = L[1 + AX], vnr = Rt[1 + AX];
+    const R csl = sound(K, L[0], L[4]), csr = sound(K, Rt[0], Rt[4]);
+    const R am = __builtin_fmin(R(0), __builtin_fmin(vnl - csl, vnr - csr));
+    const R ap = __builtin_fmax(R(0), __builtin_fmax(vnl + csl, vnr + csr));
+    const R iw = Fast<R>::rcp(ap - am);
+    const R El = L[0] * R(0.5) * (L[1] * L[1] + L[2] * L[2] + L[3] * L[3]) + L[4] * K.igm1;
+    const R Er = Rt[0] * R(0.5) * (Rt[1] * Rt[1] + Rt[2] * Rt[2] + Rt[3] * Rt[3]) + Rt[4] * K.igm1;
+    R Ul[5], Ur[5], Fl[5], Fr[5];
     Ul[0] = L[0];
-    Ur[0] = R[0];
+    Ur[0] = Rt[0];
 #pragma unroll
     for (int t = 0; t < 3; ++t)
     {
         Ul[1 + t] = L[0] * L[1 + t];
-        Ur[1 + t] = R[0] * R[1 + t];
+        Ur[1 + t] = Rt[0] * Rt[1 + t];
     }
     Ul[4] = El;
     Ur[4] = Er;
@@ -140,81 +165,86 @@ __device__ __forceinline__ void hlle(const Consts& K, const double* L, const dou
         Fr[q] = vnr * Ur[q];
     }
     Fl[1 + AX] += L[4];
-    Fr[1 + AX] += R[4];
+    Fr[1 + AX] += Rt[4];
     Fl[4] = vnl * (El + L[4]);
-    Fr[4] = vnr * (Er + R[4]);
-    const double apm = ap * am;
+    Fr[4] = vnr * (Er + Rt[4]);
+    const R apm = ap * am;
 #pragma unroll
     for (int q = 0; q < 5; ++q) F[q] = (ap * Fl[q] - am * Fr[q] + apm * (Ur[q] - Ul[q])) * iw;
 }
 
-template <int Riem, int AX>
-__device__ __forceinline__ void face_flux(const Consts& K, const double* L, const double* R, double* F)
+template <int Riem, int AX, typename R>
+__device__ __forceinline__ void face_flux(const Consts<R>& K, const R* L, const R* Rt, R* F)
 {
-    if constexpr (Riem == 1) hllc<AX>(K, L, R, F);
-    else hlle<AX>(K, L, R, F);
+    if constexpr (Riem == 1) hllc<AX>(K, L, Rt, F);
+    else hlle<AX>(K, L, Rt, F);
 }
 
-// U(P) of NewtonianHydro with 1 / (gamma - 1) as a product
-__device__ __forceinline__ void cons_of(const Consts& K, const double* P, double* U)
+// U(P) of NewtonianHydro with 1 / (gamma - 1) as a product, binary64
+__device__ __forceinline__ void cons_of(double igm1, const double* P, double* U)
 {
     U[0] = P[0];
     U[1] = P[0] * P[1];
     U[2] = P[0] * P[2];
     U[3] = P[0] * P[3];
-    U[4] = P[0] * 0.5 * (P[1] * P[1] + P[2] * P[2] + P[3] * P[3]) + P[4] * K.igm1;
+    U[4] = P[0] * 0.5 * (P[1] * P[1] + P[2] * P[2] + P[3] * P[3]) + P[4] * igm1;
 }
 
-// P(U) with one reciprocal of the density
-__device__ __forceinline__ void prim_of(const Consts& K, const double* U, double* P)
+// P(U) with one reciprocal of the density, binary64
+__device__ __forceinline__ void prim_of(double gm1, const double* U, double* P)
 {
-    const double ir = rcp(U[0]);
+    const double ir = Fast<double>::rcp(U[0]);
     P[0] = U[0];
     P[1] = U[1] * ir;
     P[2] = U[2] * ir;
     P[3] = U[3] * ir;
-    P[4] = (U[4] - 0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) * ir) * K.gm1;
+    P[4] = (U[4] - 0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) * ir) * gm1;
 }
 
 // CFL time step of a zone: Lmin / max over axes of (|v_axis| + cs), the
-// reference's min over zones of L / max |eigenvalue| with axis normals
-__device__ __forceinline__ double courant(const Consts& K, const double* P, double Lmin)
+// reference's min over zones of L / max |eigenvalue| with axis normals,
+// in the face arithmetic
+template <typename R>
+__device__ __forceinline__ double courant(const Consts<R>& K, const double* Pd, double Lmin)
 {
-    const double cs = sound(K, P[0], P[4]);
-    const double vm = __builtin_fmax(__builtin_fabs(P[1]), __builtin_fmax(__builtin_fabs(P[2]), __builtin_fabs(P[3])));
-    return Lmin * rcp(vm + cs);
+    const R P[5] = {R(Pd[0]), R(Pd[1]), R(Pd[2]), R(Pd[3]), R(Pd[4])};
+    const R cs = sound(K, P[0], P[4]);
+    const R vm = __builtin_fmax(__builtin_fabs(P[1]), __builtin_fmax(__builtin_fabs(P[2]), __builtin_fabs(P[3])));
+    return double(R(Lmin) * Fast<R>::rcp(vm + cs));
 }
 
-// Tile of 64 x TY zones marching over z planes; TY = 4 keeps three
-// work-groups per CU within the LDS.
-constexpr int FTX = 64, FTY = 4;
-
-template <int Riem, bool Z>
-static __global__ void __launch_bounds__(FTX * FTY) __attribute__((amdgpu_waves_per_eu(3, 8)))
-k_fast_plm(const double* __restrict__ Pin, const double* __restrict__ P0, double* __restrict__ Pout, Geom G,
-           double b, int first_stage, int courant_out, Ctl* ctl, int kz)
+// Tiles of 64 x TY zones marching over z planes: binary64 64 x 4 (three
+// work-groups per CU within the LDS), binary32 64 x 8.
+template <typename R> struct FastTile
 {
-    if (!ctl->active) return;
-    constexpr int NG = 2, TX = FTX, TY = FTY;
+    static constexpr int TX = 64;
+    static constexpr int TY = sizeof(R) == 8 ? 4 : 8;
+    static constexpr int WAVES = sizeof(R) == 8 ? 3 : 4;
+};
+
+// One tile (tx, ty, tz) of the stage; every thread of the work-group calls
+// it (it synchronises internally).
+template <typename R, int Riem, bool Z>
+__device__ __forceinline__ void fast_plm_tile(const double* __restrict__ Pin, const double* __restrict__ P0,
+                                              double* __restrict__ Pout, const Geom& G, double b, int first_stage,
+                                              int courant_out, Ctl* ctl, int tx, int ty, int tz, int kz)
+{
+    constexpr bool F64 = sizeof(R) == 8;
+    constexpr int NG = 2, TX = FastTile<R>::TX, TY = FastTile<R>::TY;
     constexpr int HX = TX + 2 * NG, HY = TY + 2 * NG;
     constexpr int NT = TX * TY;
-    __shared__ double plane[5][HY][HX];   // plane k, halo 2; zone (i, j) at [j - j0 + 2][i - i0 + 2]
-    __shared__ double sy[5][TY + 2][TX];  // y half-slopes of rows -1 .. TY at [r + 1]
-    __shared__ double sxe[5][TY][3];      // x half-slopes of zones i0 - 1, i0 + 63, i0 + 64
-    __shared__ double fy[5][TY + 1][TX];  // y-face fluxes, face j0 + r at [r]
-    __shared__ double fxe[5][TY];         // x-face i0 + 64 of each row
-    __shared__ double wmin[TX * TY / 64];
+    __shared__ R plane[5][HY][HX];   // plane k, halo 2; zone (i, j) at [j - j0 + 2][i - i0 + 2]
+    __shared__ R sy[5][TY + 2][TX];  // y half-slopes of rows -1 .. TY at [r + 1]
+    __shared__ R sxe[5][TY][3];      // x half-slopes of zones i0 - 1, i0 + 63, i0 + 64
+    __shared__ R fy[5][TY + 1][TX];  // y-face fluxes, face j0 + r at [r]
+    __shared__ R fxe[5][TY];         // x-face i0 + 64 of each row
+    __shared__ double wmin[NT / 64];
 
-    const int ntx = (G.n[0] + TX - 1) / TX, nty = (G.n[1] + TY - 1) / TY;
-    const int ntz = Z ? (G.n[2] + kz - 1) / kz : 1;
-    const int tile = xcd_tile(blockIdx.x, ntx * nty * ntz);
-    const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
-
-    Consts K;
-    K.gamma = G.gamma;
-    K.gm1 = G.gamma - 1.0;
-    K.igm1 = 1.0 / K.gm1;
-    K.th2 = 0.5 * G.theta;
+    Consts<R> K;
+    K.gamma = R(G.gamma);
+    K.igm1 = R(1.0 / (G.gamma - 1.0));
+    K.th2 = R(0.5 * G.theta);
+    const double gm1 = G.gamma - 1.0, igm1 = 1.0 / gm1;
     double dmin = 1.79769313486231570815e+308;
     const int lane = threadIdx.x;
     const int row = threadIdx.y;
@@ -229,30 +259,30 @@ k_fast_plm(const double* __restrict__ Pin, const double* __restrict__ P0, double
     const int kb = Z ? G.g[2] + tz * kz : 0;
     const int ke = Z ? min(kb + kz, G.N[2] - G.g[2]) : 1;
     const long long sxy = (long long)G.N[0] * G.N[1];
-    const double A0 = G.dA[0], A1 = G.dA[1], A2 = G.dA[2];
-    const double fac = -1.0 / G.dV;
+    const R A0 = R(G.dA[0]), A1 = R(G.dA[1]), A2 = R(G.dA[2]);
+    const R fac = R(-1.0 / G.dV);
     const double dt = ctl->dt;
     const int ic = min(i, G.N[0] - 1), jc = min(j, G.N[1] - 1);
     const long long col = (long long)jc * G.N[0] + ic;
 
-    double W[5][4];  // z window P(k - 1 .. k + 2) of the zone's column
-    double hzc[5];   // z half-slope of zone k
-    double Fzlo[5];
+    R W[5][4];  // z window P(k - 1 .. k + 2) of the zone's column
+    R hzc[5];   // z half-slope of zone k
+    R Fzlo[5];
     if constexpr (Z)
     {
-        double w0[5];
+        R w0[5];
 #pragma unroll
         for (int q = 0; q < 5; ++q)
         {
-            w0[q] = Pin[q * G.fs + col + (long long)(kb - 2) * sxy];
+            w0[q] = R(Pin[q * G.fs + col + (long long)(kb - 2) * sxy]);
 #pragma unroll
-            for (int m = 0; m < 3; ++m) W[q][m + 1] = Pin[q * G.fs + col + (long long)(kb - 1 + m) * sxy];
+            for (int m = 0; m < 3; ++m) W[q][m + 1] = R(Pin[q * G.fs + col + (long long)(kb - 1 + m) * sxy]);
         }
-        double PL[5], PR[5];
+        R PL[5], PR[5];
 #pragma unroll
         for (int q = 0; q < 5; ++q)
         {
-            const double hlo = half_slope(K.th2, w0[q], W[q][1], W[q][2]);
+            const R hlo = half_slope(K.th2, w0[q], W[q][1], W[q][2]);
             hzc[q] = half_slope(K.th2, W[q][1], W[q][2], W[q][3]);
             PL[q] = W[q][1] + hlo;
             PR[q] = W[q][2] - hzc[q];
@@ -274,7 +304,7 @@ k_fast_plm(const double* __restrict__ Pin, const double* __restrict__ P0, double
                 const int gi = min(i0 - NG + hx, G.N[0] - 1), gj = min(j0 - NG + hy, G.N[1] - 1);
                 const long long c = pbase + (long long)gj * G.N[0] + gi;
 #pragma unroll
-                for (int q = 0; q < 5; ++q) plane[q][hy][hx] = Pin[q * G.fs + c];
+                for (int q = 0; q < 5; ++q) plane[q][hy][hx] = R(Pin[q * G.fs + c]);
             }
         }
         else
@@ -285,21 +315,21 @@ k_fast_plm(const double* __restrict__ Pin, const double* __restrict__ P0, double
                 const int gi = min(i0 - NG + hx, G.N[0] - 1), gj = min(j0 - NG + hy, G.N[1] - 1);
                 const long long c = pbase + (long long)gj * G.N[0] + gi;
 #pragma unroll
-                for (int q = 0; q < 5; ++q) plane[q][hy][hx] = Pin[q * G.fs + c];
+                for (int q = 0; q < 5; ++q) plane[q][hy][hx] = R(Pin[q * G.fs + c]);
             }
         }
-        double Fzhi[5];
+        R Fzhi[5];
         if constexpr (Z)
         {
-            double PL[5], PR[5];
+            R PL[5], PR[5];
 #pragma unroll
             for (int q = 0; q < 5; ++q)
             {
                 W[q][0] = W[q][1];
                 W[q][1] = W[q][2];
                 W[q][2] = W[q][3];
-                W[q][3] = Pin[q * G.fs + col + (long long)(k + 2) * sxy];
-                const double hn = half_slope(K.th2, W[q][1], W[q][2], W[q][3]);
+                W[q][3] = R(Pin[q * G.fs + col + (long long)(k + 2) * sxy]);
+                const R hn = half_slope(K.th2, W[q][1], W[q][2], W[q][3]);
                 PL[q] = W[q][1] + hzc[q];
                 PR[q] = W[q][2] - hn;
                 hzc[q] = hn;
@@ -309,7 +339,7 @@ k_fast_plm(const double* __restrict__ Pin, const double* __restrict__ P0, double
         __syncthreads();
 
         // half-slopes: own zone in x (register) and y (LDS); tile-edge zones
-        double hx[5];
+        R hx[5];
 #pragma unroll
         for (int q = 0; q < 5; ++q)
         {
@@ -347,17 +377,17 @@ k_fast_plm(const double* __restrict__ Pin, const double* __restrict__ P0, double
         }
         __syncthreads();
 
-        double Fx[5], Fy[5];
-        double hl[5];
+        R Fx[5], Fy[5];
+        R hl[5];
 #pragma unroll
         for (int q = 0; q < 5; ++q)
         {
-            const double up = __shfl_up(hx[q], 1, 64);
+            const R up = __shfl_up(hx[q], 1, 64);
             hl[q] = lane == 0 ? sxe[q][row][0] : up;
         }
         if (xface)
         {
-            double PL[5], PR[5];
+            R PL[5], PR[5];
 #pragma unroll
             for (int q = 0; q < 5; ++q)
             {
@@ -368,7 +398,7 @@ k_fast_plm(const double* __restrict__ Pin, const double* __restrict__ P0, double
         }
         if (yface)
         {
-            double PL[5], PR[5];
+            R PL[5], PR[5];
 #pragma unroll
             for (int q = 0; q < 5; ++q)
             {
@@ -382,7 +412,7 @@ k_fast_plm(const double* __restrict__ Pin, const double* __restrict__ P0, double
         const int te = tid - (NT - TX - TY);
         if (te >= 0)
         {
-            double PL[5], PR[5], F[5];
+            R PL[5], PR[5], F[5];
             if (te < TY)
             {
                 const int r = te;
@@ -418,23 +448,28 @@ k_fast_plm(const double* __restrict__ Pin, const double* __restrict__ P0, double
         }
         __syncthreads();
 
-        double Fxr[5];
+        R Fxr[5];
 #pragma unroll
         for (int q = 0; q < 5; ++q)
         {
-            const double down = __shfl_down(Fx[q], 1, 64);
+            const R down = __shfl_down(Fx[q], 1, 64);
             Fxr[q] = lane == TX - 1 ? fxe[q][row] : down;
         }
         if (cell_xy)
         {
             const long long c = pbase + col;
-            // the stage's own state U(Pin) from the zone's input value (in the z
-            // window, or the LDS plane in 2-D); the step's start state U(P0) is
-            // the same value in the first stage and read from P0 after it
+            // the stage's own state U(Pin) of the zone's binary64 input value
+            // (in binary64 the z window or the LDS plane hold it; the binary32
+            // tiles re-read it); the step's start state U(P0) is that value
+            // again in the first stage, else read from P0
             double pin[5], Us[5], U0[5];
 #pragma unroll
-            for (int q = 0; q < 5; ++q) pin[q] = Z ? W[q][1] : plane[q][row + 2][lane + 2];
-            cons_of(K, pin, Us);
+            for (int q = 0; q < 5; ++q)
+            {
+                if constexpr (F64) pin[q] = Z ? W[q][1] : plane[q][row + 2][lane + 2];
+                else pin[q] = __builtin_nontemporal_load(&Pin[q * G.fs + c]);
+            }
+            cons_of(igm1, pin, Us);
             if (first_stage)
             {
 #pragma unroll
@@ -444,22 +479,22 @@ k_fast_plm(const double* __restrict__ Pin, const double* __restrict__ P0, double
             {
                 double p0[5];
 #pragma unroll
-                for (int q = 0; q < 5; ++q) p0[q] = P0[q * G.fs + c];
-                cons_of(K, p0, U0);
+                for (int q = 0; q < 5; ++q) p0[q] = __builtin_nontemporal_load(&P0[q * G.fs + c]);
+                cons_of(igm1, p0, U0);
             }
             double Un[5];
 #pragma unroll
             for (int q = 0; q < 5; ++q)
             {
-                double acc = (Fxr[q] - Fx[q]) * A0 + (fy[q][row + 1][lane] - Fy[q]) * A1;
+                R acc = (Fxr[q] - Fx[q]) * A0 + (fy[q][row + 1][lane] - Fy[q]) * A1;
                 if constexpr (Z) acc += (Fzhi[q] - Fzlo[q]) * A2;
-                const double L = fac * acc;
+                const double L = double(fac * acc);
                 Un[q] = U0[q] * (1 - b) + (Us[q] + dt * L) * b;
             }
             double Pn[5];
-            prim_of(K, Un, Pn);
+            prim_of(gm1, Un, Pn);
 #pragma unroll
-            for (int q = 0; q < 5; ++q) Pout[q * G.fs + c] = Pn[q];
+            for (int q = 0; q < 5; ++q) __builtin_nontemporal_store(Pn[q], &Pout[q * G.fs + c]);
             if (courant_out)
             {
                 const double d = courant(K, Pn, G.Lmin);
@@ -473,13 +508,51 @@ k_fast_plm(const double* __restrict__ Pin, const double* __restrict__ P0, double
         }
         __syncthreads();
     }
-    if (courant_out) courant_to_bins(dmin, wmin, TX * TY / 64, tid, ctl);
+    if (courant_out) courant_to_bins(dmin, wmin, NT / 64, tid, ctl);
+}
+
+template <typename R, int Riem, bool Z>
+static __global__ void __launch_bounds__(FastTile<R>::TX * FastTile<R>::TY)
+    __attribute__((amdgpu_waves_per_eu(FastTile<R>::WAVES, 8)))
+k_fast_plm(const double* __restrict__ Pin, const double* __restrict__ P0, double* __restrict__ Pout, Geom G,
+           double b, int first_stage, int courant_out, Ctl* ctl, int kz)
+{
+    if (!ctl->active) return;
+    constexpr int TX = FastTile<R>::TX, TY = FastTile<R>::TY;
+    const int ntx = (G.n[0] + TX - 1) / TX, nty = (G.n[1] + TY - 1) / TY;
+    const int ntz = Z ? (G.n[2] + kz - 1) / kz : 1;
+    const int tile = xcd_tile(blockIdx.x, ntx * nty * ntz);
+    const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
+    fast_plm_tile<R, Riem, Z>(Pin, P0, Pout, G, b, first_stage, courant_out, ctl, tx, ty, tz, kz);
 }
 
 // the blocks this kernel serves: hydro, PLM, 2-D or 3-D, two or three stages
 inline bool supported(const mara_desc* d)
 {
     return d->law == MARA_LAW_HYDRO && d->recon == MARA_RECON_PLM && d->cells[1] > 1 && d->rk_order >= 2;
+}
+
+// One stage on the block's stream: R = double for the fast binary64 form,
+// float for the mixed form.
+template <typename R>
+inline void launch(const Geom& G, hipStream_t stream, int riemann, const double* Pin, const double* P0,
+                   double* Pout, double b, int first, int cout, Ctl* ctl, int kz)
+{
+    constexpr int TX = FastTile<R>::TX, TY = FastTile<R>::TY;
+    const bool z = G.N[2] > 1;
+    const bool hllc = riemann == MARA_RIEMANN_HLLC;
+    const dim3 block(TX, TY, 1);
+    const dim3 grid(((G.n[0] + TX - 1) / TX) * ((G.n[1] + TY - 1) / TY) * (z ? (G.n[2] + kz - 1) / kz : 1));
+    if (z)
+    {
+        if (hllc) k_fast_plm<R, 1, true><<<grid, block, 0, stream>>>(Pin, P0, Pout, G, b, first, cout, ctl, kz);
+        else k_fast_plm<R, 0, true><<<grid, block, 0, stream>>>(Pin, P0, Pout, G, b, first, cout, ctl, kz);
+    }
+    else
+    {
+        if (hllc) k_fast_plm<R, 1, false><<<grid, block, 0, stream>>>(Pin, P0, Pout, G, b, first, cout, ctl, kz);
+        else k_fast_plm<R, 0, false><<<grid, block, 0, stream>>>(Pin, P0, Pout, G, b, first, cout, ctl, kz);
+    }
 }
 
 } // namespace fast64

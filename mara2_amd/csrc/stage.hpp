@@ -215,9 +215,12 @@ void stage_kernels(const StageArgs& s)
 // binary32 instantiation (mixed.hip)
 void stage_kernels_f32(const StageArgs& s);
 
-// the fast binary64 form's stage kernel (fast.hip) for the blocks it serves
-// (fast64_supported); the caller runs stage_kernels<double> for the others
+// the stage kernel of the fast forms (fast.hip) for the blocks it serves
+// (fast64_supported: fused hydro + PLM): binary64 faces for the fast
+// binary64 form, binary32 faces for the mixed form; the caller runs
+// stage_kernels<double> / stage_kernels_f32 for the others
 void stage_kernels_fast64(const StageArgs& s);
+void stage_kernels_fast32(const StageArgs& s);
 bool fast64_supported(const mara_desc* d);
 
 } // namespace mara
