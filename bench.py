@@ -456,7 +456,9 @@ def main():
     if args.workload in ("sod", "mhd2d", "rmhd3d", "serving"):
         leg = {"sod": sod_leg, "mhd2d": mhd2d_leg, "rmhd3d": rmhd_leg,
                "serving": lambda a: serving_leg(a, a.requests, a.seq_sample)}[args.workload](args)
-        print(json.dumps(dict(base, workload=args.workload, **leg)), flush=True)
+        out = dict(base)
+        out.update(leg)
+        print(json.dumps(out), flush=True)
         return
 
     d64, res, transfers, l1_fast10, l1_mixed100, n_mixed = turbulence_leg(args)
