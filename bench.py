@@ -32,7 +32,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from mara2_amd import abi, problems  # noqa: E402
-from mara2_amd.runtime import Batch, Block, MaraError  # noqa: E402
+from mara2_amd.runtime import Batch, Block, Group, MaraError  # noqa: E402
 
 METRIC = "zone-updates/s, FP32-mixed vs FP64 at stated L1 error; solves/s + p99 latency"
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
@@ -181,7 +181,88 @@ def turbulence_leg(args):
     n_mixed = 100
     dts100 = courant_dts(d64, P0, n_mixed, CFL)
     l1_mixed100 = rel_l1_cons(d64, run_fixed(with_form(d64, "mixed"), P0, dts100), run_fixed(d64, P0, dts100))
-    return d64, res, transfers, l1_fast10, l1_mixed100, n_mixed
+    return d64, res, transfers, l1_fast10, l1_mixed100, n_mixed, P0
+
+
+# ---------------------------------------------------------------------------
+# one GPU's share of a split mesh, and the mesh split into blocks on one GPU
+# ---------------------------------------------------------------------------
+def blocks_leg(args, d64, P0):
+    """(a) The block one GPU would hold if the n^3 mesh were split over 8 GPUs
+    along x: n/8 x n x n, periodic, timed like the headline.  (b) The n^3 mesh
+    as a 2 x 2 x 2 group of blocks on this GPU (guard exchange after every
+    stage) against the undivided block: host-timed steps of mara_group_advance
+    and mara_block_advance with the same time step, and the bitwise
+    comparison of the two states afterwards."""
+    n = d64.cells[0]
+    out = {}
+    share = abi.make_desc(law="hydro", gamma=d64.gamma, recon="plm", plm_theta=d64.plm_theta, riemann="hllc",
+                          bc="periodic", cells=(n // 8, n, n), rk_order=2, precision="fp64",
+                          upper=(1.0 / 8, 1.0, 1.0))
+    Ps = problems.empty(share)
+    Ps[...] = P0[:Ps.shape[0]]
+    Ps = problems.apply_bc(share, Ps)
+    zs = (n // 8) * n * n
+    for form in ("fp64_fast", "fp64"):
+        log(f"share block {n // 8}x{n}x{n}: timing {form}")
+        r = time_form(with_form(share, form), Ps, args.steps, args.warmup, tune=not args.no_autotune)
+        out[f"share_{form}"] = {"cells": [n // 8, n, n], "value": zs * r["steps"] / r["el"],
+                                "ms_per_step": r["el"] / max(r["steps"], 1) * 1e3, "stage_ms": r["stage_ms"],
+                                "kernel": r["kernel"],
+                                "roofline_frac": roofline(share, r["stage_ms"], r["kernel"])["frac"],
+                                "launch_shape": r.get("launch_shape")}
+    # the same count of zones cut along the slowest axis of the device layout
+    # (x is the contiguous one, and 64-lane tiles span it), and the cube of a
+    # 2 x 2 x 2 split
+    for cells in ((n, n, n // 8), (n // 2, n // 2, n // 2)):
+        dd = abi.make_desc(law="hydro", gamma=d64.gamma, recon="plm", plm_theta=d64.plm_theta, riemann="hllc",
+                           bc="periodic", cells=cells, rk_order=2, precision="fp64_fast",
+                           upper=tuple(c / n for c in cells))
+        Pd = problems.empty(dd)
+        Pd[...] = P0[:Pd.shape[0], :Pd.shape[1], :Pd.shape[2]]
+        Pd = problems.apply_bc(dd, Pd)
+        z = int(np.prod(cells))
+        log(f"share block {cells}: timing fp64_fast")
+        r = time_form(dd, Pd, args.steps, args.warmup, tune=not args.no_autotune)
+        out["share_fp64_fast_" + "x".join(str(c) for c in cells)] = {
+            "cells": list(cells), "value": z * r["steps"] / r["el"],
+            "ms_per_step": r["el"] / max(r["steps"], 1) * 1e3, "stage_ms": r["stage_ms"], "kernel": r["kernel"],
+            "roofline_frac": roofline(dd, r["stage_ms"], r["kernel"])["frac"], "launch_shape": r.get("launch_shape")}
+    dims = (2, 2, 2)
+    d = with_form(d64, "fp64_fast")
+    steps = args.steps
+    log(f"group {dims} of {n}^3 against one block: {steps} steps")
+    with Block(d) as blk:
+        blk.upload(P0)
+        dt = CFL * blk.courant_dt()
+        for _ in range(args.warmup):
+            blk.advance(dt, check=False)
+        blk.upload(P0)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            blk.advance(dt, check=False)
+        t_blk = time.perf_counter() - t0
+        Pb = blk.download()
+    with Group(d, dims) as grp:
+        grp.upload(P0)
+        for _ in range(args.warmup):
+            grp.advance(dt, check=False)
+        grp.upload(P0)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            grp.advance(dt, check=False)
+        t_grp = time.perf_counter() - t0
+        Pg = grp.download()
+    zones = n ** 3
+    out["group_2x2x2"] = {"form": "fp64_fast", "blocks": 8, "steps": steps,
+                          "value": zones * steps / t_grp, "ms_per_step": t_grp / steps * 1e3,
+                          "single_block_value": zones * steps / t_blk,
+                          "single_block_ms_per_step": t_blk / steps * 1e3,
+                          "group_over_single_time": t_grp / t_blk,
+                          "bitwise_equal_to_single_block": bool(np.array_equal(Pb, Pg)),
+                          "timing": "host clock around mara_group_advance / mara_block_advance calls, "
+                                    "one synchronisation per step on both sides"}
+    return out
 
 
 # ---------------------------------------------------------------------------
@@ -461,7 +542,7 @@ def main():
         print(json.dumps(out), flush=True)
         return
 
-    d64, res, transfers, l1_fast10, l1_mixed100, n_mixed = turbulence_leg(args)
+    d64, res, transfers, l1_fast10, l1_mixed100, n_mixed, turbulence_state = turbulence_leg(args)
     zones = int(np.prod([d64.cells[a] for a in range(3)]))
     fast, exact, mixed = res["fp64_fast"], res["fp64"], res["mixed"]
     workload = f"turbulence3d_{d64.cells[0]}"
@@ -494,6 +575,8 @@ def main():
                           "pcie_inclusive_value": zones * args.steps
                           / (fast["el"] + transfers["upload_s"] + transfers["download_s"])},
     })
+    if args.workload in ("all", "turbulence"):
+        out["blocks"] = blocks_leg(args, d64, turbulence_state)
     if args.workload == "all":
         configs = {}
         for name, fn in (("sod", sod_leg), ("mhd2d", mhd2d_leg), ("rmhd3d", rmhd_leg),

@@ -69,7 +69,8 @@ extern "C" {
                                reference's conserved fields after 10 steps.
                                Hydro blocks with PLM on 2-D and 3-D meshes run
                                their own stage kernel; every other block runs
-                               the MARA_PREC_FP64 kernels.  Single blocks only. */
+                               the MARA_PREC_FP64 kernels.  Blocks and block
+                               groups; not in batches. */
 
 /* ---- block descriptor ---------------------------------------------------- */
 typedef struct mara_desc
@@ -89,7 +90,7 @@ typedef struct mara_desc
     int    field_ct;       /* 1 unless MethodOfLinesTVD::setDisableFieldCT(true) */
     int    precision;      /* MARA_PREC_* */
     double cooling_rate;   /* NewtonianMHD::setCoolingRate; > 0 (MARA_LAW_MHD
-                              only, single blocks only) adds the auxiliary
+                              only; not in batches) adds the auxiliary
                               ninth field and the cooling source
                               (src/ConservationLaws.cpp:354-358,489-501);
                               <= 0 (the reference default) is off */
@@ -375,6 +376,62 @@ int mara_batch_finished(const mara_batch* batch, int* slots, int capacity, int* 
 /* Copy the solve's primitives into P (MeshData::P layout; skipped when P
    is null) and release its slot and memory. */
 int mara_batch_retire(mara_batch* batch, int slot, double* P);
+
+/* ---------------------------------------------------------------------------
+   Block groups: one mesh split into blocks on one GPU, the single-device half
+   of BlockDecomposition (src/BlockDecomposition.cpp).  Each block runs its
+   stages on a stream of its own, so the blocks' kernels overlap; once all of
+   a stage is done, the group's stream fills their guard zones from the
+   neighbours' edge layers axis by axis, left then right, and
+   applies the physical condition at the mesh's walls, as BlockDecomposedBC
+   under BoundaryCondition::applySimple does (src/BlockDecomposition.cpp:11-92,
+   src/BoundaryConditions.cpp:8-18).  Every block keeps the whole mesh's cell
+   sizes, so a group's step is the single block's step zone for zone.
+   --------------------------------------------------------------------------- */
+typedef struct mara_group mara_group;
+
+#define MARA_GROUP_MAX_BLOCKS 1024
+
+/* Partition `which` of n elements over p partitions: its element count and
+   first element.  BlockDecomposition::partition and ::startIndex
+   (src/BlockDecomposition.cpp:187-202): the first n % p partitions hold one
+   element more. */
+int mara_partition(int n, int p, int which, int* count, int* start);
+
+/* Split desc's mesh into dims[0] x dims[1] x dims[2] blocks (block
+   coordinates row-major, axis 2 fastest, as a Cartesian communicator ranks
+   them; BlockDecomposition::getPatchRegion, src/BlockDecomposition.cpp:162-175).
+   MARA_ERR_UNSUPPORTED for MHD and RMHD with flux-CT (the face fluxes of a
+   neighbour's edge are needed inside a stage) and for a split that leaves a
+   block fewer than two cells or fewer cells than guards along an axis;
+   MARA_ERR_CAPACITY beyond MARA_GROUP_MAX_BLOCKS blocks. */
+int mara_group_create(const mara_desc* desc, const int* dims, mara_group** out);
+int mara_group_destroy(mara_group* group);
+int mara_group_num_blocks(const mara_group* group);
+/* First interior cell and cell counts of block `index` in the mesh. */
+int mara_group_block_region(const mara_group* group, int index, int* start, int* cells);
+long long mara_group_device_bytes(const mara_group* group);
+
+/* Primitives of the whole mesh in MeshData::P layout, guards included.  The
+   upload gives each block its window of P (guards included); the download
+   writes every block's interior and the mesh's own guard zones. */
+int mara_group_upload_prim(mara_group* group, const double* P);
+int mara_group_download_prim(mara_group* group, double* P);
+
+/* The guard exchange on the resident primitives (MeshData::applyBoundaryCondition
+   with a BlockDecomposedBC, src/MeshData.cpp:148-151). */
+int mara_group_apply_bc(mara_group* group);
+
+/* One Runge-Kutta step of size dt of every block, the guard exchange after
+   each stage (MethodOfLinesTVD::advance, src/SolutionSchemes.cpp:108-275,
+   with a BlockDecomposedBC).  The report sums the blocks' counts; its first
+   failure is the one with the smallest index of the whole mesh, in the
+   mesh's interior coordinates. */
+int mara_group_advance(mara_group* group, double t0, double dt, mara_report* report);
+
+/* Minimum over the blocks of mara_block_courant_dt: the whole mesh's CFL
+   time step. */
+int mara_group_courant_dt(mara_group* group, double* dt);
 
 #ifdef __cplusplus
 }

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -98,6 +99,7 @@ struct mara_block
     int kz = FUSED_KZ;           // fused 3-D kernels: z planes per work-group
     int rows = ROWS_DEFAULT;     // flux-array kernels: work-group rows
     int pencil_shape = PENCIL_DEFAULT;  // requested pencil length (all laws)
+    bool in_group = false;    // a group member: the group's exchange fills its guards
     Ctl* ctl = nullptr;
     Ctl* ctl_host = nullptr;
     unsigned long long* cbins = nullptr;
@@ -182,6 +184,22 @@ void run_stage_kernels(const StageArgs& s, bool mixed)
     else stage_kernels<double>(s);
 }
 
+// The buffers stage rk reads and writes.
+void stage_io(const mara_block* b, int rk, double** Pin, double** Pout)
+{
+    const int last = rk == b->d.rk_order - 1;
+    *Pin = rk == 0 ? b->Pa : b->Pb;
+    *Pout = last ? b->Pa : b->Pb;
+    if (b->fused || b->pencil || b->mhd2d)
+    {
+        // the fused and pencil kernels read neighbouring zones of their input
+        // while other work-groups write their output, so input and output
+        // never alias: RK2 Pa -> Pb -> Pa, RK3 Pa -> Pb -> Pc -> Pa
+        *Pin = rk == 0 ? b->Pa : (rk == 1 ? b->Pb : b->Pc);
+        *Pout = last ? b->Pa : (rk == 0 ? b->Pb : b->Pc);
+    }
+}
+
 // One RK stage: fluxes, CT, update with ghost fill.  With courant_out the last
 // stage also reduces the next step's CFL time step into the control bins.
 void launch_stage(mara_block* b, int rk, int courant_out)
@@ -191,17 +209,10 @@ void launch_stage(mara_block* b, int rk, int courant_out)
     static const double b3[3] = {1.0, 1. / 4, 2. / 3};
     const double* bb = b->d.rk_order == 1 ? b1 : b->d.rk_order == 2 ? b2 : b3;
     const int last = rk == b->d.rk_order - 1;
-    const double* Pin = rk == 0 ? b->Pa : b->Pb;
-    double* Pout = last ? b->Pa : b->Pb;
+    double* Pin;
+    double* Pout;
+    stage_io(b, rk, &Pin, &Pout);
     const bool mixed = b->d.precision == MARA_PREC_MIXED;
-    if (b->fused || b->pencil || b->mhd2d)
-    {
-        // the fused and pencil kernels read neighbouring zones of their input
-        // while other work-groups write their output, so input and output
-        // never alias: RK2 Pa -> Pb -> Pa, RK3 Pa -> Pb -> Pc -> Pa
-        Pin = rk == 0 ? b->Pa : (rk == 1 ? b->Pb : b->Pc);
-        Pout = last ? b->Pa : (rk == 0 ? b->Pb : b->Pc);
-    }
 
     // NewtonianMHD cooling (ConservationLaw::addSourceTerms) into S
     const bool cooling = b->G.nq == 9;
@@ -253,8 +264,10 @@ void launch_stage(mara_block* b, int rk, int courant_out)
         b->ev_used += 1;
     }
     // the flux-array update writes the ghost zones of Pout itself
-    // (write_guard_images); the fused kernels leave them to a ghost-fill pass
-    if (b->fused) launch_bc(b, Pout);
+    // (write_guard_images); the fused kernels leave them to a ghost-fill pass,
+    // and a group member's guards come from the group's exchange after the
+    // stage (the images the update wrote are then overwritten)
+    if (b->fused && !b->in_group) launch_bc(b, Pout);
 }
 
 // ---- tuning table ----------------------------------------------------------
@@ -1150,3 +1163,421 @@ extern "C" int mara_advance_noalloc(const mara_desc* d, const double* P_in, doub
     if (status) return status;
     return fill_report(&b, report);
 }
+
+// ===========================================================================
+// Block groups: one mesh split into blocks on one device.  The single-device
+// half of BlockDecomposition (src/BlockDecomposition.cpp): the blocks take
+// the partition of every axis and run their stages on streams of their own,
+// and after each stage the group's stream fills their guards from the neighbours' edge
+// layers in BoundaryCondition::applySimple order (axis 0, 1, 2; left, then
+// right), with the physical condition at the mesh's walls.
+// ===========================================================================
+struct mara_group
+{
+    mara_desc d;                      // the whole mesh
+    int dims[3] = {1, 1, 1};
+    int count = 0;
+    hipStream_t stream = nullptr;
+    std::vector<mara_block*> blocks;  // row-major over the block coordinates, axis 2 fastest
+    std::vector<std::array<int, 3>> start, cells;
+    hipEvent_t fork = nullptr;        // recorded on the group's stream, awaited by every block's stream
+    std::vector<hipEvent_t> join;     // recorded on each block's stream, awaited by the group's stream
+    HaloEntry* halo = nullptr;        // device tables [buffer role][axis][block]
+    long long halo_threads[3] = {0, 0, 0};  // largest entry count per axis
+    int roles = 2;                    // Pa, Pb (and Pc on the three-buffer RK3 schedule)
+};
+
+namespace {
+
+// BlockDecomposition::partition and ::startIndex (src/BlockDecomposition.cpp:187-202):
+// the first n % p partitions hold one element more than the others.
+void partition_of(int n, int p, int i, int* count, int* start)
+{
+    const int q = n / p, r = n % p;
+    *count = q + (i < r ? 1 : 0);
+    const int big = i < r ? i : r;  // partitions of size q + 1 before i
+    *start = (q + 1) * big + q * (i - big);
+}
+
+int group_coord(const mara_group* g, int m, int axis)
+{
+    const int c2 = m % g->dims[2];
+    const int c1 = (m / g->dims[2]) % g->dims[1];
+    const int c0 = m / (g->dims[2] * g->dims[1]);
+    return axis == 0 ? c0 : axis == 1 ? c1 : c2;
+}
+
+int group_index(const mara_group* g, int c0, int c1, int c2) { return (c0 * g->dims[1] + c1) * g->dims[2] + c2; }
+
+// the buffer of block b in role r: 0 Pa, 1 Pb, 2 Pc
+double* role_buffer(mara_block* b, int r) { return r == 0 ? b->Pa : r == 1 ? b->Pb : b->Pc; }
+
+int role_of(const mara_block* b, const double* P) { return P == b->Pa ? 0 : P == b->Pb ? 1 : 2; }
+
+// The blocks' streams wait for the work so far on the group's stream ...
+int group_fork(mara_group* g)
+{
+    HIP_TRY(hipEventRecord(g->fork, g->stream));
+    for (mara_block* b : g->blocks) HIP_TRY(hipStreamWaitEvent(b->stream, g->fork, 0));
+    return MARA_OK;
+}
+
+// ... and the group's stream for the work so far on every block's stream.
+int group_join(mara_group* g)
+{
+    for (int m = 0; m < g->count; ++m)
+    {
+        HIP_TRY(hipEventRecord(g->join[m], g->blocks[m]->stream));
+        HIP_TRY(hipStreamWaitEvent(g->stream, g->join[m], 0));
+    }
+    return MARA_OK;
+}
+
+void group_exchange(mara_group* g, int role)
+{
+    const int nq = g->blocks[0]->G.nq;
+    for (int axis = 0; axis < 3; ++axis)
+    {
+        if (g->d.guards[axis] == 0 || g->halo_threads[axis] == 0) continue;
+        const dim3 grid((unsigned)((g->halo_threads[axis] + kThreads - 1) / kThreads), (unsigned)g->count);
+        k_halo<<<grid, kThreads, 0, g->stream>>>(g->halo + ((long long)role * 3 + axis) * g->count, axis, nq);
+    }
+}
+
+// after group_join: the members' control blocks to the host
+int read_group_ctls(mara_group* g)
+{
+    for (mara_block* b : g->blocks)
+        HIP_TRY(hipMemcpyAsync(b->ctl_host, b->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return MARA_OK;
+}
+
+// the members' reports as one: counts summed, the first failure the one with
+// the smallest linear index of the whole mesh (i slowest)
+int group_report(mara_group* g, mara_report* r)
+{
+    int status = MARA_OK;
+    mara_report all = {};
+    all.first_failure[0] = all.first_failure[1] = all.first_failure[2] = -1;
+    unsigned long long best = ~0ull;
+    for (int m = 0; m < g->count; ++m)
+    {
+        mara_report one;
+        const int s = fill_report(g->blocks[m], &one);
+        // ctl_status's precedence over the members
+        if (s == MARA_ERR_SUPERLUMINAL || (s == MARA_ERR_BAD_EIGENVALUES && status != MARA_ERR_SUPERLUMINAL)
+            || (s == MARA_ERR_INVERSION_FAILURE && status == MARA_OK))
+            status = s;
+        all.c2p_failures += one.c2p_failures;
+        all.floor_applied += one.floor_applied;
+        all.eigen_failures += one.eigen_failures;
+        all.superluminal += one.superluminal;
+        if (one.c2p_failures)
+        {
+            int f[3];
+            for (int a = 0; a < 3; ++a) f[a] = one.first_failure[a] + g->start[m][a];
+            const unsigned long long lin =
+                ((unsigned long long)f[0] * g->d.cells[1] + f[1]) * g->d.cells[2] + f[2];
+            if (lin < best)
+            {
+                best = lin;
+                for (int a = 0; a < 3; ++a) all.first_failure[a] = f[a];
+            }
+        }
+    }
+    all.status = status;
+    if (r) *r = all;
+    return status;
+}
+
+// Padded local extent along an axis.
+inline int padded(const mara_group* g, int m, int a) { return g->cells[m][a] + 2 * g->d.guards[a]; }
+
+} // namespace
+
+extern "C" {
+
+int mara_partition(int n, int p, int which, int* count, int* start)
+{
+    if (!count || !start || n < 1 || p < 1 || which < 0 || which >= p) return MARA_ERR_INVALID_ARGUMENT;
+    partition_of(n, p, which, count, start);
+    return MARA_OK;
+}
+
+int mara_group_destroy(mara_group* g)
+{
+    if (!g) return MARA_ERR_INVALID_ARGUMENT;
+    if (g->stream) hipStreamSynchronize(g->stream);
+    for (mara_block* b : g->blocks)
+        if (b) mara_block_destroy(b);
+    if (g->fork) hipEventDestroy(g->fork);
+    for (hipEvent_t e : g->join)
+        if (e) hipEventDestroy(e);
+    hipFree(g->halo);
+    if (g->stream) hipStreamDestroy(g->stream);
+    delete g;
+    return MARA_OK;
+}
+
+int mara_group_create(const mara_desc* d, const int* dims, mara_group** out)
+{
+    if (!out || !dims) return MARA_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    int v = validate(d);
+    if (v) return v;
+    long long total = 1;
+    for (int a = 0; a < 3; ++a)
+    {
+        if (dims[a] < 1) return MARA_ERR_INVALID_ARGUMENT;
+        if (dims[a] > 1 && d->cells[a] == 1) return MARA_ERR_INVALID_ARGUMENT;  // a flat axis is not split
+        total *= dims[a];
+    }
+    if (total > MARA_GROUP_MAX_BLOCKS) return MARA_ERR_CAPACITY;
+    // flux-CT needs the face fluxes of the neighbours' edges inside a stage,
+    // which the guard exchange between stages does not carry
+    if (d->law != MARA_LAW_HYDRO && d->field_ct) return MARA_ERR_UNSUPPORTED;
+    for (int a = 0; a < 3; ++a)
+    {
+        // every block keeps at least two cells and the guard depth along a
+        // split axis, so all blocks take the same stage path and every
+        // exchange reads interior layers
+        const int smallest = d->cells[a] / dims[a];
+        if (dims[a] > 1 && (smallest < 2 || smallest < d->guards[a])) return MARA_ERR_UNSUPPORTED;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MARA_ERR_NO_DEVICE;
+
+    mara_group* g = new (std::nothrow) mara_group();
+    if (!g) return MARA_ERR_HIP;
+    g->d = *d;
+    for (int a = 0; a < 3; ++a) g->dims[a] = dims[a];
+    g->count = (int)total;
+    auto fail = [&](int code) { mara_group_destroy(g); return code; };
+    if (!hip_ok(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking))) return fail(MARA_ERR_HIP);
+    if (!hip_ok(hipEventCreateWithFlags(&g->fork, hipEventDisableTiming))) return fail(MARA_ERR_HIP);
+    g->join.assign(g->count, nullptr);
+    for (hipEvent_t& e : g->join)
+        if (!hip_ok(hipEventCreateWithFlags(&e, hipEventDisableTiming))) return fail(MARA_ERR_HIP);
+
+    // the whole mesh's cell sizes, which every block keeps exactly
+    const Geom whole = make_geom(*d);
+    g->blocks.assign(g->count, nullptr);
+    g->start.resize(g->count);
+    g->cells.resize(g->count);
+    for (int m = 0; m < g->count; ++m)
+    {
+        mara_desc local = *d;
+        for (int a = 0; a < 3; ++a)
+        {
+            int n, s;
+            partition_of(d->cells[a], dims[a], group_coord(g, m, a), &n, &s);
+            g->start[m][a] = s;
+            g->cells[m][a] = n;
+            local.cells[a] = n;
+            // CartesianMeshGeometry of the block (BlockDecomposition::decompose,
+            // src/BlockDecomposition.cpp:141-159): its own lower and upper
+            const double dx = (d->upper[a] - d->lower[a]) / d->cells[a];
+            local.lower[a] = d->lower[a] + s * dx;
+            local.upper[a] = d->lower[a] + (s + n) * dx;
+        }
+        // each block on a stream of its own, so the blocks' stages overlap
+        int e = mara_block_create(&local, &g->blocks[m]);
+        if (e) return fail(e);
+        mara_block* b = g->blocks[m];
+        b->in_group = true;
+        b->graphs = false;
+        for (int a = 0; a < 3; ++a) b->G.dA[a] = whole.dA[a];
+        b->G.dV = whole.dV;
+        b->G.Lmin = whole.Lmin;
+    }
+    const mara_block* b0 = g->blocks[0];
+    g->roles = b0->Pc ? 3 : 2;
+
+    // exchange tables, one per buffer role and axis
+    std::vector<HaloEntry> tab((size_t)g->roles * 3 * g->count);
+    for (int r = 0; r < g->roles; ++r)
+        for (int axis = 0; axis < 3; ++axis)
+            for (int m = 0; m < g->count; ++m)
+            {
+                HaloEntry& e = tab[((size_t)r * 3 + axis) * g->count + m];
+                std::memset(&e, 0, sizeof e);
+                mara_block* b = g->blocks[m];
+                e.P = role_buffer(b, r);
+                e.fs = b->G.fs;
+                for (int a = 0; a < 3; ++a) e.N[a] = b->G.N[a];
+                e.g = d->guards[axis];
+                const int t1 = axis == 0 ? 1 : 0, t2 = axis == 2 ? 1 : 2;
+                e.count = 2ll * e.g * e.N[t1] * e.N[t2];
+                g->halo_threads[axis] = std::max(g->halo_threads[axis], e.count);
+                int c[3] = {group_coord(g, m, 0), group_coord(g, m, 1), group_coord(g, m, 2)};
+                for (int side = 0; side < 2; ++side)
+                {
+                    const bool wall = side == 0 ? c[axis] == 0 : c[axis] == dims[axis] - 1;
+                    if (wall && d->bc != MARA_BC_PERIODIC)
+                    {
+                        e.kind[side] = d->bc;  // outflow 1, reflecting 2
+                        continue;
+                    }
+                    int nc[3] = {c[0], c[1], c[2]};
+                    nc[axis] = (c[axis] + (side == 0 ? dims[axis] - 1 : 1)) % dims[axis];
+                    mara_block* nb = g->blocks[group_index(g, nc[0], nc[1], nc[2])];
+                    e.kind[side] = 0;
+                    e.src[side] = role_buffer(nb, r);
+                    e.src_fs[side] = nb->G.fs;
+                    e.src_N[side] = nb->G.N[axis];
+                    if (nb->G.N[t1] != b->G.N[t1] || nb->G.N[t2] != b->G.N[t2]) return fail(MARA_ERR_INVALID_ARGUMENT);
+                }
+            }
+    const size_t tb = tab.size() * sizeof(HaloEntry);
+    if (!hip_ok(hipMalloc(&g->halo, tb))) return fail(MARA_ERR_HIP);
+    if (!hip_ok(hipMemcpy(g->halo, tab.data(), tb, hipMemcpyHostToDevice))) return fail(MARA_ERR_HIP);
+    *out = g;
+    return MARA_OK;
+}
+
+int mara_group_num_blocks(const mara_group* g) { return g ? g->count : -1; }
+
+int mara_group_block_region(const mara_group* g, int index, int* start, int* cells)
+{
+    if (!g || !start || !cells || index < 0 || index >= g->count) return MARA_ERR_INVALID_ARGUMENT;
+    for (int a = 0; a < 3; ++a)
+    {
+        start[a] = g->start[index][a];
+        cells[a] = g->cells[index][a];
+    }
+    return MARA_OK;
+}
+
+long long mara_group_device_bytes(const mara_group* g)
+{
+    if (!g) return -1;
+    long long s = (long long)g->roles * 3 * g->count * sizeof(HaloEntry);
+    for (const mara_block* b : g->blocks) s += b->bytes;
+    return s;
+}
+
+int mara_group_upload_prim(mara_group* g, const double* P)
+{
+    if (!g || !P) return MARA_ERR_INVALID_ARGUMENT;
+    const int nq = g->blocks[0]->G.nq;
+    const int GN1 = g->d.cells[1] + 2 * g->d.guards[1], GN2 = g->d.cells[2] + 2 * g->d.guards[2];
+    std::vector<double> local;
+    for (int m = 0; m < g->count; ++m)
+    {
+        // the block's window of the whole array, its guards included: padded
+        // index l of the block is padded index start + l of the mesh
+        const int N0 = padded(g, m, 0), N1 = padded(g, m, 1), N2 = padded(g, m, 2);
+        local.resize((size_t)N0 * N1 * N2 * nq);
+        const auto& s = g->start[m];
+        for (int i = 0; i < N0; ++i)
+            for (int j = 0; j < N1; ++j)
+            {
+                const double* src = P + (((size_t)(s[0] + i) * GN1 + (s[1] + j)) * GN2 + s[2]) * nq;
+                std::memcpy(&local[(((size_t)i * N1 + j) * N2) * nq], src, (size_t)N2 * nq * sizeof(double));
+            }
+        int e = mara_block_upload_prim(g->blocks[m], local.data());
+        if (e) return e;
+    }
+    return MARA_OK;
+}
+
+int mara_group_download_prim(mara_group* g, double* P)
+{
+    if (!g || !P) return MARA_ERR_INVALID_ARGUMENT;
+    const int nq = g->blocks[0]->G.nq;
+    const int GN1 = g->d.cells[1] + 2 * g->d.guards[1], GN2 = g->d.cells[2] + 2 * g->d.guards[2];
+    std::vector<double> local;
+    for (int m = 0; m < g->count; ++m)
+    {
+        const int N0 = padded(g, m, 0), N1 = padded(g, m, 1), N2 = padded(g, m, 2);
+        local.resize((size_t)N0 * N1 * N2 * nq);
+        int e = mara_block_download_prim(g->blocks[m], local.data());
+        if (e) return e;
+        // the block's interior, and its guards where they are the mesh's
+        int lo[3], hi[3];
+        for (int a = 0; a < 3; ++a)
+        {
+            const int c = group_coord(g, m, a);
+            lo[a] = c == 0 ? 0 : g->d.guards[a];
+            hi[a] = c == g->dims[a] - 1 ? padded(g, m, a) : g->d.guards[a] + g->cells[m][a];
+        }
+        const auto& s = g->start[m];
+        for (int i = lo[0]; i < hi[0]; ++i)
+            for (int j = lo[1]; j < hi[1]; ++j)
+            {
+                double* dst = P + (((size_t)(s[0] + i) * GN1 + (s[1] + j)) * GN2 + s[2] + lo[2]) * nq;
+                std::memcpy(dst, &local[(((size_t)i * N1 + j) * N2 + lo[2]) * nq],
+                            (size_t)(hi[2] - lo[2]) * nq * sizeof(double));
+            }
+    }
+    return MARA_OK;
+}
+
+int mara_group_apply_bc(mara_group* g)
+{
+    if (!g) return MARA_ERR_INVALID_ARGUMENT;
+    int e = group_join(g);
+    if (e) return e;
+    group_exchange(g, 0);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return MARA_OK;
+}
+
+int mara_group_advance(mara_group* g, double t0, double dt, mara_report* report)
+{
+    (void)t0;
+    if (!g) return MARA_ERR_INVALID_ARGUMENT;
+    // per stage: every block's stage on its own stream, then the exchange on
+    // the group's stream once all of them are done
+    int e = group_fork(g);
+    if (e) return e;
+    for (mara_block* b : g->blocks) k_ctl_begin_advance<<<1, 1, 0, b->stream>>>(b->ctl, dt);
+    const int rk_order = g->d.rk_order;
+    for (int rk = 0; rk < rk_order; ++rk)
+    {
+        if (rk > 0 && (e = group_fork(g))) return e;
+        for (mara_block* b : g->blocks) launch_stage(b, rk, 0);
+        if ((e = group_join(g))) return e;
+        double* Pin;
+        double* Pout;
+        stage_io(g->blocks[0], rk, &Pin, &Pout);
+        group_exchange(g, role_of(g->blocks[0], Pout));
+    }
+    HIP_TRY(hipGetLastError());
+    e = read_group_ctls(g);
+    if (e) return e;
+    return group_report(g, report);
+}
+
+int mara_group_courant_dt(mara_group* g, double* dt)
+{
+    if (!g || !dt) return MARA_ERR_INVALID_ARGUMENT;
+    int e = group_fork(g);
+    if (e) return e;
+    for (mara_block* b : g->blocks)
+    {
+        k_ctl_begin_courant<<<1, 1, 0, b->stream>>>(b->ctl);
+        launch_courant(b);
+        k_ctl_fold_bins<<<1, 1, 0, b->stream>>>(b->ctl);
+    }
+    if ((e = group_join(g))) return e;
+    HIP_TRY(hipGetLastError());
+    e = read_group_ctls(g);
+    if (e) return e;
+    double best = 1.79769313486231570815e+308;
+    int status = MARA_OK;
+    for (mara_block* b : g->blocks)
+    {
+        union { unsigned long long u; double d; } cv;
+        cv.u = b->ctl_host->courant_bits;
+        best = std::min(best, cv.d);
+        if (b->ctl_host->superluminal) status = MARA_ERR_SUPERLUMINAL;
+        else if (b->ctl_host->eigen_fail && status == MARA_OK) status = MARA_ERR_BAD_EIGENVALUES;
+    }
+    *dt = best;
+    return status;
+}
+
+} // extern "C"

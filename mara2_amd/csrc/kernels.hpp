@@ -783,6 +783,83 @@ k_bc(double* __restrict__ P, Geom G, int axis, int kind, const Ctl* __restrict__
 }
 
 // ---------------------------------------------------------------------------
+// Guard exchange of a group of blocks that tile one mesh on one device
+// (BlockDecomposedBC::apply, src/BlockDecomposition.cpp:11-92).  Along `axis`
+// the left guard layers [0, g) of a block take layers [Ns - 2g, Ns - g) of its
+// left neighbour and the right layers [N - g, N) take [g, 2g) of its right
+// neighbour, over the full transverse extent, guards included; at a wall of a
+// non-periodic mesh the side takes the physical condition instead (outflow or
+// reflecting, as bc_guard).  The reads are interior along the axis and the
+// writes are guards along it, so one launch serves every block and side.
+// ---------------------------------------------------------------------------
+struct HaloEntry
+{
+    double* P;              // the block's buffer of this stage
+    const double* src[2];   // left / right neighbour's buffer (unused at a wall)
+    long long fs;           // the block's field stride
+    long long src_fs[2];    // the neighbours' field strides
+    int N[3];               // the block's extents, guards included
+    int src_N[2];           // the neighbours' extents along the axis
+    int kind[2];            // per side: 0 neighbour copy, 1 outflow, 2 reflecting
+    int g;                  // guard layers along the axis
+    long long count;        // threads of this block: 2 g N_t1 N_t2
+};
+
+// Work-group row blockIdx.y serves table entry blockIdx.y.
+static __global__ void __launch_bounds__(256)
+k_halo(const HaloEntry* __restrict__ tab, int axis, int nq)
+{
+    const HaloEntry& e = tab[blockIdx.y];
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= e.count) return;
+    const int g = e.g;
+    const int N = e.N[axis];
+    const int t1 = axis == 0 ? 1 : 0;
+    const int t2 = axis == 2 ? 1 : 2;
+    const long long slab = (long long)e.N[t1] * e.N[t2];
+    const int a1 = (int)(t % e.N[t1]);
+    const int a2 = (int)((t / e.N[t1]) % e.N[t2]);
+    const int layer = (int)((t / slab) % g);
+    const int side = (int)(t / (slab * g));
+    const int kind = e.kind[side];
+    const double* S = e.P;
+    long long sfs = e.fs;
+    int sN[3] = {e.N[0], e.N[1], e.N[2]};
+    int dst, src;
+    if (kind == 0)
+    {
+        S = e.src[side];
+        sfs = e.src_fs[side];
+        sN[axis] = e.src_N[side];
+        dst = side == 0 ? layer : N - g + layer;
+        src = side == 0 ? sN[axis] - 2 * g + layer : g + layer;
+    }
+    else if (kind == 1)
+    {
+        dst = side == 0 ? layer : N - 1 - layer;
+        src = side == 0 ? g : N - g - 1;
+    }
+    else
+    {
+        dst = side == 0 ? layer : N - 1 - layer;
+        src = side == 0 ? 2 * g - layer - 1 : N - 2 * g + layer;
+    }
+    int I[3], J[3];
+    I[axis] = dst;
+    J[axis] = src;
+    I[t1] = J[t1] = a1;
+    I[t2] = J[t2] = a2;
+    const long long cd = ((long long)I[2] * e.N[1] + I[1]) * e.N[0] + I[0];
+    const long long cs = ((long long)J[2] * sN[1] + J[1]) * sN[0] + J[0];
+    for (int q = 0; q < nq; ++q)
+    {
+        double v = S[q * sfs + cs];
+        if (kind == 2 && q == 1 + axis) v *= -1;
+        e.P[q * e.fs + cd] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // CFL reduction (FieldOperator::getCourantTimestep, src/FieldOperator.cpp:125-169):
 // per interior zone L / max(lambda_x, lambda_y, lambda_z), minimum over zones.
 // ---------------------------------------------------------------------------

@@ -92,6 +92,20 @@ def load():
                                    ctypes.POINTER(ctypes.c_int), ctypes.POINTER(MaraReport)]
     L.mara_batch_retire.argtypes = [bat, ctypes.c_int, _dp]
     L.mara_batch_finished.argtypes = [bat, ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    grp = ctypes.c_void_p
+    ip = ctypes.POINTER(ctypes.c_int)
+    L.mara_partition.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ip, ip]
+    L.mara_group_create.argtypes = [ctypes.POINTER(MaraDesc), ip, ctypes.POINTER(grp)]
+    L.mara_group_destroy.argtypes = [grp]
+    L.mara_group_num_blocks.argtypes = [grp]
+    L.mara_group_block_region.argtypes = [grp, ctypes.c_int, ip, ip]
+    L.mara_group_device_bytes.argtypes = [grp]
+    L.mara_group_device_bytes.restype = ctypes.c_longlong
+    L.mara_group_upload_prim.argtypes = [grp, _dp]
+    L.mara_group_download_prim.argtypes = [grp, _dp]
+    L.mara_group_apply_bc.argtypes = [grp]
+    L.mara_group_advance.argtypes = [grp, ctypes.c_double, ctypes.c_double, ctypes.POINTER(MaraReport)]
+    L.mara_group_courant_dt.argtypes = [grp, _dp]
     _LIB = L
     return L
 
@@ -361,3 +375,81 @@ class Batch:
 
 def desc_device_bytes(desc):
     return load().mara_desc_device_bytes(ctypes.byref(desc))
+
+
+def partition(n, p, which):
+    """(count, start) of partition `which` of n elements over p partitions
+    (BlockDecomposition::partition / startIndex)."""
+    c, s = ctypes.c_int(0), ctypes.c_int(0)
+    _check(load().mara_partition(n, p, which, ctypes.byref(c), ctypes.byref(s)), "mara_partition")
+    return c.value, s.value
+
+
+class Group:
+    """One mesh split into dims[0] x dims[1] x dims[2] blocks on one device
+    (the single-device half of BlockDecomposition), with the guard exchange
+    after every stage."""
+
+    def __init__(self, desc: MaraDesc, dims):
+        self.lib = load()
+        self.desc = desc
+        self.shape = array_shape(desc)
+        self.dims = tuple(int(x) for x in dims)
+        d3 = (ctypes.c_int * 3)(*self.dims)
+        self._h = ctypes.c_void_p()
+        _check(self.lib.mara_group_create(ctypes.byref(desc), d3, ctypes.byref(self._h)), "mara_group_create")
+
+    def close(self):
+        if self._h:
+            self.lib.mara_group_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def num_blocks(self):
+        return self.lib.mara_group_num_blocks(self._h)
+
+    @property
+    def device_bytes(self):
+        return self.lib.mara_group_device_bytes(self._h)
+
+    def block_region(self, index):
+        s, c = (ctypes.c_int * 3)(), (ctypes.c_int * 3)()
+        _check(self.lib.mara_group_block_region(self._h, index, s, c), "mara_group_block_region")
+        return tuple(s), tuple(c)
+
+    def upload(self, P):
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        assert P.shape == self.shape, (P.shape, self.shape)
+        _check(self.lib.mara_group_upload_prim(self._h, P.ctypes.data_as(_dp)), "mara_group_upload_prim")
+
+    def download(self):
+        P = np.zeros(self.shape)
+        _check(self.lib.mara_group_download_prim(self._h, P.ctypes.data_as(_dp)), "mara_group_download_prim")
+        return P
+
+    def apply_bc(self):
+        _check(self.lib.mara_group_apply_bc(self._h), "mara_group_apply_bc")
+
+    def advance(self, dt, t0=0.0, check=True):
+        rep = MaraReport()
+        st = self.lib.mara_group_advance(self._h, t0, dt, ctypes.byref(rep))
+        if check:
+            _check(st, "mara_group_advance", rep)
+        return st, rep
+
+    def courant_dt(self):
+        dt = ctypes.c_double(0)
+        _check(self.lib.mara_group_courant_dt(self._h, ctypes.byref(dt)), "mara_group_courant_dt")
+        return dt.value
