@@ -154,6 +154,17 @@ void launch_bc(mara_block* b, double* P)
     if (total > 0) k_ghost<<<grid_for(total), kThreads, 0, b->stream>>>(P, G, total, b->ctl);
 }
 
+// The arithmetic of the CFL time step the block's last stage produces
+// (the values of fast64::CflArith: 0 the reference's binary64, 1 binary32
+// wave speeds, 2 and 3 the fast tiles' binary64 and binary32).
+int cfl_arith(const mara_block* b)
+{
+    const bool fast_tile = b->fused && fast64_supported(&b->d);
+    if (b->d.precision == MARA_PREC_MIXED) return fast_tile ? 3 : (law_code(b->d) < 2 ? 1 : 0);
+    if (b->d.precision == MARA_PREC_FP64_FAST && fast_tile) return 2;
+    return 0;
+}
+
 void launch_courant(mara_block* b)
 {
     const Geom& G = b->G;
@@ -161,6 +172,13 @@ void launch_courant(mara_block* b)
     long long blocks = (total + kThreads - 1) / kThreads;
     if (blocks > 2048) blocks = 2048;  // 8 work-groups per CU, grid-stride beyond
     const unsigned nb = (unsigned)blocks;
+    // the first step of a run takes its dt in the arithmetic of the later
+    // ones, which the form's last stage computes
+    if (const int arith = cfl_arith(b))
+    {
+        courant_kernels_form(b->Pa, G, G.nq == 9 ? 3 : b->d.law, b->ctl, b->stream, arith, nb);
+        return;
+    }
     switch (b->d.law)
     {
         case 0: k_courant<0><<<nb, kThreads, 0, b->stream>>>(b->Pa, G, b->ctl); break;

@@ -413,6 +413,14 @@ int mara_batch_admit(mara_batch* B, const mara_desc* d, const double* P, double 
     it.ctl = B->d_ctl + n;
     it.bc = d->bc;
     it.lane = B->classes[sl.cls].lane;
+    // the first step's CFL in the arithmetic of the item's stage kernels:
+    // the mixed form's hydro PLM tiles are the fast binary32 ones
+    {
+        const mara::batch::ClassKey& key = B->classes[sl.cls];
+        if (B->precision != MARA_PREC_MIXED) it.cfl = fast64::CFL_EXACT;
+        else if (key.fused == 1 || key.fused == 2) it.cfl = fast64::CFL_FAST32;
+        else it.cfl = key.law < 2 ? fast64::CFL_MIXED : fast64::CFL_EXACT;
+    }
     sl.express = false;
 
     Ctl& c = B->h_ctl[n];

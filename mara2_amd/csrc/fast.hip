@@ -23,4 +23,19 @@ void stage_kernels_fast32(const StageArgs& s)
     fast64::launch<float>(*s.G, s.stream, s.d->riemann, s.Pin, s.P0, s.Pout, s.b, s.first, s.cout, s.ctl, s.kz);
 }
 
+// The CFL reduction in the arithmetic of a form other than the reference's
+// (fast64::CflArith), compiled with the same flags as the stage kernels
+// that produce the form's later time steps.
+void courant_kernels_form(const double* P, const Geom& G, int law, Ctl* ctl, hipStream_t stream, int arith,
+                          unsigned blocks)
+{
+    switch (law)
+    {
+        case 0: fast64::k_courant_form<0><<<blocks, 256, 0, stream>>>(P, G, ctl, arith); break;
+        case 1: fast64::k_courant_form<1><<<blocks, 256, 0, stream>>>(P, G, ctl, arith); break;
+        case 3: fast64::k_courant_form<3><<<blocks, 256, 0, stream>>>(P, G, ctl, arith); break;
+        default: fast64::k_courant_form<2><<<blocks, 256, 0, stream>>>(P, G, ctl, arith); break;
+    }
+}
+
 } // namespace mara

@@ -555,5 +555,77 @@ inline void launch(const Geom& G, hipStream_t stream, int riemann, const double*
     }
 }
 
+// ---------------------------------------------------------------------------
+// The CFL time step in the arithmetic of the form whose stage kernels produce
+// it in their last stage, so the first step of a run (or of a restart from a
+// checkpoint) takes the same dt a later step would take for the same state:
+// the reference's binary64 (k_courant), binary32 wave speeds (the mixed
+// flux-array, pencil, 2-D MHD and non-PLM fused kernels), or the fast tiles'
+// courant<double> / courant<float>.
+// ---------------------------------------------------------------------------
+enum CflArith { CFL_EXACT = 0, CFL_MIXED = 1, CFL_FAST64 = 2, CFL_FAST32 = 3 };
+
+template <int Law>
+__device__ __forceinline__ double zone_courant_form(int arith, const double* __restrict__ P, const Geom& G,
+                                                    long long c, int& err)
+{
+    if constexpr (Law == 0)
+    {
+        if (arith == CFL_FAST64 || arith == CFL_FAST32)
+        {
+            double p[5];
+#pragma unroll
+            for (int q = 0; q < 5; ++q) p[q] = P[q * G.fs + c];
+            if (arith == CFL_FAST64)
+            {
+                Consts<double> K;
+                K.gamma = G.gamma;
+                return courant(K, p, G.Lmin);
+            }
+            Consts<float> K;
+            K.gamma = float(G.gamma);
+            return courant(K, p, G.Lmin);
+        }
+    }
+    if constexpr (Law < 2)
+    {
+        if (arith == CFL_MIXED)
+        {
+            constexpr int NQ = LawNQ<Law>::value;
+            double p[NQ];
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) p[q] = P[q * G.fs + c];
+            return courant_of<Law, float>(p, G, err);
+        }
+    }
+    return zone_courant<Law>(P, G, c, err);
+}
+
+// k_courant with the form's arithmetic
+template <int Law>
+static __global__ void __launch_bounds__(256) k_courant_form(const double* __restrict__ P, Geom G, Ctl* ctl, int arith)
+{
+    __shared__ double wmin[4];
+    if (!ctl->active) return;
+    const long long total = (long long)G.n[0] * G.n[1] * G.n[2];
+    const long long nxy = (long long)G.n[0] * G.n[1];
+    double dtc = 1.79769313486231570815e+308;
+    int err = 0;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x)
+    {
+        const long long kk = t / nxy;
+        const long long r = t - kk * nxy;
+        const int j = (int)(r / G.n[0]);
+        const int i = (int)(r - (long long)j * G.n[0]);
+        const long long c = cell_index(G, G.g[0] + i, G.g[1] + j, G.g[2] + (int)kk);
+        const double d = zone_courant_form<Law>(arith, P, G, c, err);
+        if (d < dtc) dtc = d;
+    }
+    if (err & ERR_SUPERLUMINAL) atomicAdd(&ctl->superluminal, 1ull);
+    if (err & ERR_EIGEN) atomicAdd(&ctl->eigen_fail, 1ull);
+    courant_to_bins(dtc, wmin, 4, threadIdx.x, ctl);
+}
+
 } // namespace fast64
 } // namespace mara

@@ -223,4 +223,9 @@ void stage_kernels_fast64(const StageArgs& s);
 void stage_kernels_fast32(const StageArgs& s);
 bool fast64_supported(const mara_desc* d);
 
+// the CFL reduction in a form's own arithmetic (fast.hip; arith one of
+// fast64::CflArith other than CFL_EXACT), law 3 for cooled NewtonianMHD
+void courant_kernels_form(const double* P, const Geom& G, int law, Ctl* ctl, hipStream_t stream, int arith,
+                          unsigned blocks);
+
 } // namespace mara

@@ -218,6 +218,126 @@ Array MeshOperator::generate(InitialDataFunction F, MeshLocation location, Shape
     return P;
 }
 
+Array MeshOperator::generate(InitialDataFunction F, MeshLocation location, VectorMode, Shape bs) const
+{
+    if (!geometry) throw std::logic_error("MeshOperator has no MeshGeometry instance");
+    if (location != MeshLocation::face) return generate(F, location, bs);
+    const int nq = (int)F(0.0, 0.0, 0.0).size();
+    const Shape s = geometry->cellsShape();
+    Array A(s[0] + 2 * bs[0] + 1, s[1] + 2 * bs[1] + 1, s[2] + 2 * bs[2] + 1, nq, 3);
+    for (int i = 0; i < A.size(0); ++i)
+    for (int j = 0; j < A.size(1); ++j)
+    for (int k = 0; k < A.size(2); ++k)
+    {
+        const double c[3] = {double(i - bs[0]), double(j - bs[1]), double(k - bs[2])};
+        for (int a = 0; a < 3; ++a)
+        {
+            // the face of axis a on the left of cell c: half a cell back along a
+            const Coordinate x = geometry->coordinateAtIndex(c[0] - (a == 0 ? 0.5 : 0.0), c[1] - (a == 1 ? 0.5 : 0.0),
+                                                             c[2] - (a == 2 ? 0.5 : 0.0));
+            const std::vector<double> v = F(x[0], x[1], x[2]);
+            for (int q = 0; q < nq; ++q) A(i, j, k, q, a) = v[q];
+        }
+    }
+    return A;
+}
+
+Array MeshOperator::divergence(const Array& flux, double factor) const
+{
+    if (!geometry) throw std::logic_error("MeshOperator has no MeshGeometry instance");
+    if (flux.size(4) != 3) throw std::logic_error("Attempt to compute divergence from data whose size(4) != 3");
+    const int nq = flux.size(3);
+    Array D(flux.size(0) - 1, flux.size(1) - 1, flux.size(2) - 1, nq);
+    for (int i = 0; i < D.size(0); ++i)
+    for (int j = 0; j < D.size(1); ++j)
+    for (int k = 0; k < D.size(2); ++k)
+    {
+        const double a0 = geometry->faceArea(i, j, k, 0), a1 = geometry->faceArea(i, j, k, 1);
+        const double a2 = geometry->faceArea(i, j, k, 2);
+        const double f = factor / geometry->cellVolume(i, j, k);
+        for (int q = 0; q < nq; ++q)
+        {
+            double s = +flux(i + 1, j, k, q, 0) * a0;
+            s -= flux(i, j, k, q, 0) * a0;
+            s += flux(i, j + 1, k, q, 1) * a1;
+            s -= flux(i, j, k, q, 1) * a1;
+            s += flux(i, j, k + 1, q, 2) * a2;
+            s -= flux(i, j, k, q, 2) * a2;
+            D(i, j, k, q) = f * s;
+        }
+    }
+    return D;
+}
+
+// ---------------------------------------------------------------------------
+Array CellCenteredFieldCT::vectorPotentialToFluxes(const Array& A) const
+{
+    Array F(A.size(0), A.size(1), A.size(2), 3, 3);
+    // flux of B_q through faces of axis a: (e_a x A)_q with a sign, i.e.
+    // F(q, a) = -eps(a, q, r) A_r for the cyclic (a, q, r)
+    for (int i = 0; i < A.size(0); ++i)
+    for (int j = 0; j < A.size(1); ++j)
+    for (int k = 0; k < A.size(2); ++k)
+        for (int a = 0; a < 3; ++a)
+        {
+            const int q = (a + 1) % 3, r = (a + 2) % 3;
+            F(i, j, k, q, a) = -A(i, j, k, r, a);
+            F(i, j, k, r, a) = +A(i, j, k, q, a);
+        }
+    return F;
+}
+
+Array CellCenteredFieldCT::generateGodunovFluxes(const Array& F, int magneticIndex) const
+{
+    Array G = F;
+    correctGodunovFluxes(G, magneticIndex);
+    return G;
+}
+
+// Toth (2000) eq. 25: the flux of B_t through faces of axis a becomes the
+// average of the four edge values around the face, each edge value the mean
+// of the two adjacent flux estimates of the same electric field (+F of B_t
+// on axis a, -F of B_a on axis t).  The normal component's own flux is zero.
+void CellCenteredFieldCT::correctGodunovFluxes(Array& F, int mi) const
+{
+    const Array G = F;  // read the uncorrected fluxes throughout
+    const int n[3] = {F.size(0), F.size(1), F.size(2)};
+    for (int i = 0; i < n[0]; ++i)
+    for (int j = 0; j < n[1]; ++j)
+    for (int k = 0; k < n[2]; ++k)
+        for (int a = 0; a < 3; ++a) F(i, j, k, mi + a, a) = 0.0;
+    for (int a = 0; a < 3; ++a)
+    {
+        if (n[a] < 3) continue;
+        for (int t = 0; t < 3; ++t)
+        {
+            if (t == a) continue;
+            int e_a[3] = {0, 0, 0}, e_t[3] = {0, 0, 0};
+            e_a[a] = 1;
+            e_t[t] = 1;
+            int lo[3] = {0, 0, 0}, hi[3] = {n[0], n[1], n[2]};
+            lo[a] = 1;
+            lo[t] = 1;
+            hi[t] = n[t] - 1;
+            for (int i = lo[0]; i < hi[0]; ++i)
+            for (int j = lo[1]; j < hi[1]; ++j)
+            for (int k = lo[2]; k < hi[2]; ++k)
+            {
+                auto g = [&](int q, int ax, int di, int dj, int dk) { return G(i + di, j + dj, k + dk, q, ax); };
+                const int bt = mi + t, ba = mi + a;
+                double s = 2 * g(bt, a, 0, 0, 0);
+                s += g(bt, a, e_t[0], e_t[1], e_t[2]);
+                s += g(bt, a, -e_t[0], -e_t[1], -e_t[2]);
+                s -= g(ba, t, e_t[0], e_t[1], e_t[2]);
+                s -= g(ba, t, e_t[0] - e_a[0], e_t[1] - e_a[1], e_t[2] - e_a[2]);
+                s -= g(ba, t, 0, 0, 0);
+                s -= g(ba, t, -e_a[0], -e_a[1], -e_a[2]);
+                F(i, j, k, bt, a) = 0.125 * s;
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 MeshData::MeshData(Shape cs, Shape bs, int nq) : boundaryShape(bs)
 {
@@ -233,6 +353,19 @@ void MeshData::assignPrimitive(const Array& Pi)
     for (int q = 0; q < Pi.size(3); ++q)
         P(i + b[0], j + b[1], k + b[2], q) = Pi(i, j, k, q);
     deviceVersion = 0;  // host copy replaced
+}
+
+void MeshData::assignMagneticField(const Array& B, MeshLocation location, int flags)
+{
+    if (location != MeshLocation::cell) throw std::logic_error("MeshData::assignMagneticField: cell location only");
+    if (magneticIndex < 0) throw std::logic_error("MeshData::assignMagneticField: no magnetic index set");
+    const Shape& b = boundaryShape;
+    const int o[3] = {flags & includeGuard ? 0 : b[0], flags & includeGuard ? 0 : b[1], flags & includeGuard ? 0 : b[2]};
+    for (int i = 0; i < B.size(0); ++i)
+    for (int j = 0; j < B.size(1); ++j)
+    for (int k = 0; k < B.size(2); ++k)
+        for (int d = 0; d < 3; ++d) P(i + o[0], j + o[1], k + o[2], magneticIndex + d) = B(i, j, k, d);
+    deviceVersion = 0;
 }
 
 // ---------------------------------------------------------------------------

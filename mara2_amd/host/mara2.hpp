@@ -178,6 +178,8 @@ public:
     virtual Coordinate coordinateAtIndex(double i, double j, double k) const = 0;
     virtual unsigned long totalCellsInMesh() const = 0;
     virtual double cellLength(int i, int j, int k, int axis) const = 0;
+    virtual double faceArea(int i, int j, int k, int axis) const = 0;
+    virtual double cellVolume(int i, int j, int k) const = 0;
 };
 
 // src/CartesianMeshGeometry.cpp:61-157
@@ -192,6 +194,9 @@ public:
     Coordinate coordinateAtIndex(double i, double j, double k) const override;
     unsigned long totalCellsInMesh() const override;
     double cellLength(int, int, int, int axis) const override { return dx[axis]; }
+    // src/CartesianMeshGeometry.cpp:79-92: uniform areas and volume
+    double faceArea(int, int, int, int axis) const override { return axis == 0 ? dx[1] * dx[2] : axis == 1 ? dx[2] * dx[0] : dx[0] * dx[1]; }
+    double cellVolume(int, int, int) const override { return dx[0] * dx[1] * dx[2]; }
     Coordinate lowerCorner() const { return lower; }
     Coordinate upperCorner() const { return upper; }
 private:
@@ -244,16 +249,44 @@ public:
     Array linearCellDimension() const;
     // src/MeshOperator.cpp:47-195, cell location
     Array generate(InitialDataFunction F, MeshLocation location, Shape boundaryShape = {{0, 0, 0, 0, 0}}) const;
+    // face location with every component on the faces of every axis
+    // (VectorMode::scalars): an array (N1+1, N2+1, N3+1, nq, 3), guards
+    // included, component q on the faces of axis a at the face centres
+    enum class VectorMode { scalars };
+    Array generate(InitialDataFunction F, MeshLocation location, VectorMode mode, Shape boundaryShape) const;
+    // src/MeshOperator.cpp:323-381: factor / V * (sum over axes of F A on the
+    // right face minus F A on the left), flux (M1+1, M2+1, M3+1, nq, 3) ->
+    // (M1, M2, M3, nq)
+    Array divergence(const Array& flux, double factor) const;
 private:
     std::shared_ptr<MeshGeometry> geometry;
+};
+
+// The host half of CellCenteredFieldCT (src/CellCenteredFieldCT.cpp:14-157)
+// that initial data uses: fluxes of B from a vector potential on faces, and
+// Toth's flux-CT correction of them; the step's own flux-CT runs on the device.
+class CellCenteredFieldCT
+{
+public:
+    // A (…, 3, 3) [component, face axis] -> the induction fluxes F = -n x A
+    Array vectorPotentialToFluxes(const Array& A) const;
+    // a corrected copy of F (correctGodunovFluxes with B at magneticIndex)
+    Array generateGodunovFluxes(const Array& F, int magneticIndex) const;
+    void correctGodunovFluxes(Array& F, int magneticIndex) const;
 };
 
 // src/MeshData.cpp:8-28: primitives with guard zones, reference layout.
 class MeshData
 {
 public:
+    enum { includeGuard = 1 };
     MeshData(Shape cellsShape, Shape boundaryShape, int numComponents);
     void assignPrimitive(const Array& interiorP);
+    void setMagneticIndex(int index) { magneticIndex = index; }
+    // cell-centred B (M1, M2, M3, 3) into the magnetic fields of P, over the
+    // whole array with includeGuard, else over the interior
+    void assignMagneticField(const Array& B, MeshLocation location, int flags);
+    int magneticIndex = -1;
     void applyBoundaryCondition(BoundaryCondition& bc) { bc.applySimple(P, boundaryShape); }
     Shape getBoundaryShape() const { return boundaryShape; }
     Array P;

@@ -13,6 +13,7 @@
 // the loop's clock to out.bin, so a test can replay the same initial data on
 // the oracle:  int32 header[8] = {N1, N2, N3, nq, steps, 0, 0, 0},
 // float64 t_final_reached, float64 P0[...], float64 P[...].
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -33,6 +34,7 @@ struct Problem
     double finalTime;
     std::shared_ptr<BoundaryCondition> bc;
     InitialDataFunction idf;
+    InitialDataFunction ivp = nullptr;  // vector potential, when B comes from one
 };
 
 std::vector<double> pick(bool left, std::vector<double> a, std::vector<double> b) { return left ? a : b; }
@@ -74,8 +76,8 @@ std::map<std::string, Problem> problems2d()
     return p;
 }
 
-// src/Problems.cpp:491-499 (the problems with a vector potential initialise B
-// through flux-CT on the host in the reference; the Python drivers cover them)
+// src/Problems.cpp:489-552; FieldLoop and AbcField take B from a vector
+// potential on faces (runProblem, src/Problems.cpp:660-667)
 std::map<std::string, Problem> problemsMhd()
 {
     auto periodic = std::make_shared<PeriodicBoundaryCondition>();
@@ -86,6 +88,40 @@ std::map<std::string, Problem> problemsMhd()
         const double R = std::sqrt(x * x + y * y);
         return pick(R < 0.125, {1.000, 0., 0., 0., 1.000, 1.33, 1.33, 0.}, {0.125, 0., 0., 0., 0.100, 1.33, 1.33, 0.});
     }};
+    // a weak loop of B_phi = 3 B0 k (kR)^2 exp(-(kR)^3) advected diagonally;
+    // A_z = B0 exp(-(kR)^3)
+    const double B0 = 1e-5, kl = 10.0;
+    p["FieldLoop"] = {1.0, periodic,
+        [=](double x, double y, double) {
+            x -= 0.5;
+            y -= 0.5;
+            const double R = std::sqrt(x * x + y * y);
+            const double bphi = B0 * 3 * kl * std::pow(kl * R, 2) * std::exp(-std::pow(kl * R, 3));
+            return std::vector<double>{1.0, 1.0, 1.0, 0.0, 1.0, bphi * (-y / R), bphi * (x / R), 0.0};
+        },
+        [=](double x, double y, double) {
+            x -= 0.5;
+            y -= 0.5;
+            const double R = std::sqrt(x * x + y * y);
+            return std::vector<double>{0.0, 0.0, B0 * std::exp(-std::pow(kl * R, 3))};
+        }};
+    // the Arnold-Beltrami-Childress field with (A, B, C) = (1, 1, 0), k = 2 pi,
+    // and its vector potential B / k
+    auto abc = [](double x, double y, double z) {
+        const double A = 1.0, B = 1.0, C = 0.0, k = 2 * M_PI;
+        return std::array<double, 3>{C * std::cos(k * z) - B * std::sin(k * y), A * std::cos(k * x) - C * std::sin(k * z),
+                                     B * std::cos(k * y) - A * std::sin(k * x)};
+    };
+    p["AbcField"] = {0.1, periodic,
+        [=](double x, double y, double z) {
+            const auto b = abc(x, y, z);
+            return std::vector<double>{1., 0., 0., 0., 1., b[0], b[1], b[2]};
+        },
+        [=](double x, double y, double z) {
+            const auto b = abc(x, y, z);
+            const double k = 2 * M_PI;
+            return std::vector<double>{b[0] / k, b[1] / k, b[2] / k};
+        }};
     return p;
 }
 
@@ -197,6 +233,15 @@ int main(int argc, const char* argv[])
     status.totalCellsInMesh = (long)mg->totalCellsInMesh();
     auto L = mo->linearCellDimension();
     md->assignPrimitive(mo->generate(problem.idf, MeshLocation::cell));
+    if (problem.ivp)
+    {
+        // B = div(CT(F(A))) over the whole array, guards included
+        CellCenteredFieldCT ct;
+        md->setMagneticIndex(cl->getIndexFor(ConservationLaw::VariableType::magnetic));
+        const Array A = mo->generate(problem.ivp, MeshLocation::face, MeshOperator::VectorMode::scalars, bs);
+        const Array G = ct.generateGodunovFluxes(ct.vectorPotentialToFluxes(A), 0);
+        md->assignMagneticField(mo->divergence(G, 1.0), MeshLocation::cell, MeshData::includeGuard);
+    }
     md->applyBoundaryCondition(*problem.bc);
     const Array P0 = md->P;
 

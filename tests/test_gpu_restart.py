@@ -1,0 +1,107 @@
+"""Restart through the GPU (CheckpointWriter::writeCheckpoint / readCheckpoint,
+src/Checkpoint.cpp:76-273, and the restart path of a program such as
+src/Problems/MagneticBraiding.cpp:234-236): run N steps on the device
+without a pause, and separately run N1 steps, write a checkpoint of the
+state, the clock, user parameters and the scheduler's task records, read it
+back into a fresh block (interior from the file, guards from the boundary
+condition) and continue for N - N1 steps.  Both runs must end bit for bit in
+the same state at the same time, and the groups must come back as written."""
+import numpy as np
+import pytest
+
+from mara2_amd import abi, checkpoint, problems
+from mara2_amd.runtime import Block, Group
+
+pytestmark = pytest.mark.gpu
+
+
+CASES = {
+    "turbulence3d-fp64": lambda: (problems.turbulence_desc(32), None),
+    "turbulence3d-mixed": lambda: (problems.turbulence_desc(32, "mixed"), None),
+    "turbulence3d-fp64_fast": lambda: (problems.turbulence_desc(32, "fp64_fast"), None),
+    "sod-rk3": lambda: (abi.make_desc(law="hydro", gamma=1.4, riemann="hlle", bc="outflow", cells=(256, 1, 1),
+                                      rk_order=3), "sod"),
+    "mhd2d-ct": lambda: (problems.mhd_desc(48), "mhd"),
+    "rmhd3d-weno5-ct": lambda: (abi.make_desc(law="rmhd", recon="weno5", riemann="hlle", bc="periodic",
+                                              cells=(16, 16, 16), rk_order=3, field_ct=1), "rmhd"),
+}
+
+
+def _initial(desc, kind):
+    if kind == "sod":
+        return problems.sod_1d(desc)
+    if kind == "mhd":
+        return problems.mhd_2d(desc)
+    if kind == "rmhd":
+        rng = np.random.default_rng(21)
+        P = problems.empty(desc)
+        shp = P.shape[:3]
+        P[..., 0] = 1.0 + 0.1 * rng.random(shp)
+        P[..., 4] = 1.0 + 0.1 * rng.random(shp)
+        P[..., 1:4] = 0.05 * rng.standard_normal(shp + (3,))
+        P[..., 5:8] = 0.0  # field-free: the CT path runs with B = 0 exactly
+        return problems.apply_bc(desc, P)
+    return problems.turbulence_3d(desc, kmax=4)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_restart_equals_uninterrupted_run(tmp_path, name):
+    desc, kind = CASES[name]()
+    P0 = _initial(desc, kind)
+    cfl, n, n1 = 0.3, 12, 5
+    with Block(desc) as b:
+        b.upload(P0)
+        t_full, steps, _ = b.run(cfl, 1e300, n)
+        assert steps == n
+        P_full = b.download()
+    with Block(desc) as b:
+        b.upload(P0)
+        t1, s1, _ = b.run(cfl, 1e300, n1)
+        P1 = b.download()
+    user = {"cfl": cfl, "problem": name, "steps": n}
+    sched = {"checkpoint": {"nextPhysTime": 2 * t1, "nextWallTime": 0.0, "nextIteration": s1 + 1, "repetition": 1}}
+    path = tmp_path / "chkpt.0001.npz"
+    checkpoint.write(path, desc, P1, t1, s1, user=user, scheduler=sched)
+    Pr, tr, itr = checkpoint.read(path, desc)
+    assert (tr, itr) == (t1, s1)
+    groups = checkpoint.read_groups(path)
+    assert groups["user"] == user
+    assert groups["scheduler"] == sched
+    with Block(desc) as b:
+        b.upload(Pr)
+        t2, s2, _ = b.run(cfl, 1e300, n - itr, t=tr)
+        assert s2 == n - n1
+        P2 = b.download()
+    assert t2 == t_full
+    assert np.array_equal(P2, P_full), f"{name}: restarted state differs in {np.count_nonzero(P2 != P_full)} values"
+
+
+def test_restart_of_a_block_group(tmp_path):
+    """The same through a group of blocks: the checkpoint holds the whole
+    mesh, and a group of another shape continues it."""
+    desc = problems.turbulence_desc(24)
+    P0 = problems.turbulence_3d(desc, kmax=4)
+    cfl, n, n1 = 0.3, 8, 3
+    with Block(desc) as b:
+        b.upload(P0)
+        dts = []
+        for _ in range(n):
+            dts.append(cfl * b.courant_dt())
+            b.advance(dts[-1])
+        P_full = b.download()
+    with Group(desc, (2, 1, 2)) as g:
+        g.upload(P0)
+        for k in range(n1):
+            assert cfl * g.courant_dt() == dts[k]
+            g.advance(dts[k])
+        P1 = g.download()
+    path = tmp_path / "group.npz"
+    checkpoint.write(path, desc, P1, sum(dts[:n1]), n1, user={"dims": "2x1x2"})
+    Pr, tr, itr = checkpoint.read(path, desc)
+    with Group(desc, (1, 3, 1)) as g:
+        g.upload(Pr)
+        for k in range(itr, n):
+            assert cfl * g.courant_dt() == dts[k]
+            g.advance(dts[k])
+        P2 = g.download()
+    assert np.array_equal(P2, P_full)
