@@ -57,6 +57,11 @@ extern "C" {
 #define MARA_BC_OUTFLOW    1 /* OutflowBoundaryCondition    src/BoundaryConditions.cpp:66-105 */
 #define MARA_BC_REFLECTING 2 /* ReflectingBoundaryCondition src/BoundaryConditions.cpp:111-176 */
 
+#define MARA_GEOM_CARTESIAN 0 /* CartesianMeshGeometry src/CartesianMeshGeometry.cpp */
+#define MARA_GEOM_SPHERICAL 1 /* SphericalMeshGeometry src/SphericalMeshGeometry.cpp:
+                                 axes (r, theta, phi), lower/upper their
+                                 ranges, optional logarithmic radial edges */
+
 #define MARA_PREC_FP64   0  /* every operation in IEEE binary64, no contraction */
 #define MARA_PREC_MIXED  1  /* reconstruction, Riemann fluxes and flux divergence
                                in binary32; RK accumulation, flux-CT, primitive
@@ -94,6 +99,15 @@ typedef struct mara_desc
                               ninth field and the cooling source
                               (src/ConservationLaws.cpp:354-358,489-501);
                               <= 0 (the reference default) is off */
+    int    geometry;       /* MARA_GEOM_*; 0 (Cartesian) when zeroed.  The
+                              spherical mesh runs the flux-array kernels with
+                              the reference's face areas, volumes and cell
+                              lengths (SphericalMeshGeometry::faceArea,
+                              cellVolume, cellLength) and hands source terms
+                              its cell centres (r, theta, phi); not with
+                              flux-CT, viscous fluxes, batches, block groups
+                              or mara_advance_noalloc */
+    int    log_radial;     /* SphericalMeshGeometry::setUseLogarithmicRadialBinning */
 } mara_desc;
 
 /* Per-call diagnostics.  The reference throws at the first failure; the GPU

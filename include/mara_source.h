@@ -42,6 +42,9 @@ typedef struct mara_source_args
     double dx[3];      /* cell size: (upper - lower) / n */
     double t[8];       /* auxiliary data (mara_block_set_source_aux) */
     double params[MARA_SOURCE_MAX_PARAMS];
+    const double* centre[3]; /* null on a Cartesian mesh; on a spherical one
+                                the n[a] cell-centre coordinates of each axis
+                                (SphericalMeshGeometry::coordinateAtIndex) */
 } mara_source_args;
 
 /* MARA_SOURCE_KERNEL(name, fn): an extern "C" kernel `name` that evaluates
@@ -61,9 +64,9 @@ typedef struct mara_source_args
             P[q] = q < a.nq ? a.P[q * a.fs + c] : 0.0;                                            \
             S[q] = 0.0;                                                                           \
         }                                                                                         \
-        const double x = a.lower[0] + a.dx[0] * (ic + 0.5);                                       \
-        const double y = a.lower[1] + a.dx[1] * (jc + 0.5);                                       \
-        const double zz = a.lower[2] + a.dx[2] * (kc + 0.5);                                      \
+        const double x = a.centre[0] ? a.centre[0][ic] : a.lower[0] + a.dx[0] * (ic + 0.5);      \
+        const double y = a.centre[1] ? a.centre[1][jc] : a.lower[1] + a.dx[1] * (jc + 0.5);      \
+        const double zz = a.centre[2] ? a.centre[2][kc] : a.lower[2] + a.dx[2] * (kc + 0.5);     \
         fn(x, y, zz, a.t, P, S, a.params);                                                        \
         for (int q = 0; q < a.nq; ++q) a.S[q * a.fs + c] = S[q];                                  \
     }

@@ -24,6 +24,9 @@ RECONS = {"pcm": RECON_PCM, "plm": RECON_PLM, "weno5": RECON_WENO5}
 RIEMANNS = {"hlle": RIEMANN_HLLE, "hllc": RIEMANN_HLLC}
 BCS = {"periodic": BC_PERIODIC, "outflow": BC_OUTFLOW, "reflecting": BC_REFLECTING}
 PRECISIONS = {"fp64": PREC_FP64, "mixed": PREC_MIXED, "fp64_fast": PREC_FP64_FAST}
+GEOM_CARTESIAN = 0
+GEOM_SPHERICAL = 1
+GEOMETRIES = {"cartesian": GEOM_CARTESIAN, "spherical": GEOM_SPHERICAL}
 
 
 def num_fields(law):
@@ -53,6 +56,8 @@ class MaraDesc(ctypes.Structure):
         ("field_ct", ctypes.c_int),
         ("precision", ctypes.c_int),
         ("cooling_rate", ctypes.c_double),
+        ("geometry", ctypes.c_int),
+        ("log_radial", ctypes.c_int),
     ]
 
 
@@ -103,7 +108,8 @@ class MaraReport(ctypes.Structure):
 def make_desc(law="hydro", gamma=5.0 / 3.0, pressure_floor=-1.0, recon="plm",
               plm_theta=1.5, riemann="hlle", bc="periodic", cells=(128, 1, 1),
               guards=None, lower=(0.0, 0.0, 0.0), upper=(1.0, 1.0, 1.0),
-              rk_order=2, field_ct=1, precision="fp64", cooling_rate=0.0):
+              rk_order=2, field_ct=1, precision="fp64", cooling_rate=0.0, geometry="cartesian",
+              log_radial=False):
     """Build a MaraDesc from names.  guards defaults to the scheme's stencil
     size on every axis with more than one cell (the reference programs'
     boundary shapes, src/Problems.cpp:155-156)."""
@@ -128,6 +134,8 @@ def make_desc(law="hydro", gamma=5.0 / 3.0, pressure_floor=-1.0, recon="plm",
     d.field_ct = field_ct
     d.precision = PRECISIONS[precision] if isinstance(precision, str) else precision
     d.cooling_rate = cooling_rate
+    d.geometry = GEOMETRIES[geometry] if isinstance(geometry, str) else geometry
+    d.log_radial = 1 if log_radial else 0
     return d
 
 

@@ -100,6 +100,7 @@ struct mara_block
     int rows = ROWS_DEFAULT;     // flux-array kernels: work-group rows
     int pencil_shape = PENCIL_DEFAULT;  // requested pencil length (all laws)
     bool in_group = false;    // a group member: the group's exchange fills its guards
+    double* sph = nullptr;    // spherical mesh: the geometry tables (G.sph)
     Ctl* ctl = nullptr;
     Ctl* ctl_host = nullptr;
     unsigned long long* cbins = nullptr;
@@ -483,6 +484,14 @@ int mara_block_create(const mara_desc* d, mara_block** out)
         if (!hip_ok(hipMalloc(&b->Ssrc, cellbytes))) return fail(MARA_ERR_HIP);
         if (!hip_ok(hipMemset(b->Ssrc, 0, cellbytes))) return fail(MARA_ERR_HIP);
     }
+    if (d->geometry == MARA_GEOM_SPHERICAL)
+    {
+        const std::vector<double> tab = spherical_tables(*d, b->G);
+        const size_t tb = tab.size() * sizeof(double);
+        if (!hip_ok(hipMalloc(&b->sph, tb))) return fail(MARA_ERR_HIP);
+        if (!hip_ok(hipMemcpy(b->sph, tab.data(), tb, hipMemcpyHostToDevice))) return fail(MARA_ERR_HIP);
+        b->G.sph = b->sph;
+    }
     if (!hip_ok(hipMalloc(&b->ctl, sizeof(Ctl)))) return fail(MARA_ERR_HIP);
     if (!hip_ok(hipMalloc(&b->cbins, CBINS * CBIN_STRIDE * sizeof(unsigned long long)))) return fail(MARA_ERR_HIP);
     if (!hip_ok(hipHostMalloc((void**)&b->ctl_host, sizeof(Ctl), hipHostMallocDefault))) return fail(MARA_ERR_HIP);
@@ -514,6 +523,7 @@ int mara_block_destroy(mara_block* b)
     if (b->src_module) (void)hipModuleUnload(b->src_module);
     if (b->vis_module) (void)hipModuleUnload(b->vis_module);
     hipFree(b->Ssrc);
+    hipFree(b->sph);
     if (b->ctl_host) hipHostFree(b->ctl_host);
     for (int e = 0; e < 2 * mara_block::kTimingPairs; ++e)
         if (b->ev[e]) hipEventDestroy(b->ev[e]);
@@ -887,6 +897,15 @@ int mara_block_set_source(mara_block* b, const char* code_object, const char* ke
         a.n[e] = b->G.n[e];
         a.lower[e] = b->d.lower[e];
         a.dx[e] = (b->d.upper[e] - b->d.lower[e]) / b->d.cells[e];  // CartesianMeshGeometry
+        a.centre[e] = nullptr;
+    }
+    if (b->sph)
+    {
+        // SphericalMeshGeometry::coordinateAtIndex of the interior cells
+        const SphView s(b->G);
+        a.centre[0] = s.rc;
+        a.centre[1] = s.qc;
+        a.centre[2] = s.pc;
     }
     a.nq = b->S.nq;
     for (int q = 0; q < MARA_SOURCE_MAX_PARAMS; ++q) a.params[q] = q < n_params ? params[q] : 0.0;
@@ -908,7 +927,7 @@ int mara_block_set_viscous_flux(mara_block* b, const char* code_object, const ch
     if (!b || n_params < 0 || n_params > MARA_SOURCE_MAX_PARAMS || (n_params > 0 && !params)
         || (code_object && !kernel))
         return MARA_ERR_INVALID_ARGUMENT;
-    if (code_object && (b->d.precision == MARA_PREC_MIXED || b->S.ct)) return MARA_ERR_UNSUPPORTED;
+    if (code_object && (b->d.precision == MARA_PREC_MIXED || b->S.ct || b->sph)) return MARA_ERR_UNSUPPORTED;
     HIP_TRY(hipStreamSynchronize(b->stream));
     if (b->vis_module) (void)hipModuleUnload(b->vis_module);
     b->vis_module = nullptr;
@@ -1124,6 +1143,8 @@ extern "C" int mara_advance_noalloc(const mara_desc* d, const double* P_in, doub
     if (!P_in || !P_out) return MARA_ERR_INVALID_ARGUMENT;
     const int v = validate(d);
     if (v) return v;
+    // a spherical block's geometry tables would need a scratch range of their own
+    if (d->geometry != MARA_GEOM_CARTESIAN) return MARA_ERR_UNSUPPORTED;
     // the block's state, set up as mara_block_create does, on the stack; its
     // buffers are carved from the scratch array below
     mara_block b;
@@ -1353,8 +1374,10 @@ int mara_group_create(const mara_desc* d, const int* dims, mara_group** out)
     }
     if (total > MARA_GROUP_MAX_BLOCKS) return MARA_ERR_CAPACITY;
     // flux-CT needs the face fluxes of the neighbours' edges inside a stage,
-    // which the guard exchange between stages does not carry
+    // which the guard exchange between stages does not carry; the blocks
+    // of a spherical mesh would need their own edge tables
     if (d->law != MARA_LAW_HYDRO && d->field_ct) return MARA_ERR_UNSUPPORTED;
+    if (d->geometry != MARA_GEOM_CARTESIAN) return MARA_ERR_UNSUPPORTED;
     for (int a = 0; a < 3; ++a)
     {
         // every block keeps at least two cells and the guard depth along a

@@ -262,6 +262,7 @@ int validate_item(mara_batch* B, const mara_desc* d)
     if (v) return v;
     if (d->rk_order != B->rk_order || d->precision != B->precision) return MARA_ERR_UNSUPPORTED;
     if (law_code(*d) == LAW_MHD_COOLING) return MARA_ERR_UNSUPPORTED;  // single blocks only
+    if (d->geometry != MARA_GEOM_CARTESIAN) return MARA_ERR_UNSUPPORTED;
     return MARA_OK;
 }
 

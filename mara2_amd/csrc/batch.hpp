@@ -177,7 +177,8 @@ kb_courant(const BItem* __restrict__ items, const BTile* __restrict__ tiles)
     int i, j, k;
     double dtc = 1.79769313486231570815e+308;
     int err = 0;
-    if (tile_thread(T, i, j, k)) dtc = fast64::zone_courant_form<Law>(it.cfl, it.Pa, it.G, cell_index(it.G, i, j, k), err);
+    if (tile_thread(T, i, j, k))
+        dtc = fast64::zone_courant_form<Law>(it.cfl, it.Pa, it.G, cell_index(it.G, i, j, k), it.G.Lmin, err);
     if (err & ERR_SUPERLUMINAL) atomicAdd(&it.ctl->superluminal, 1ull);
     if (err & ERR_EIGEN) atomicAdd(&it.ctl->eigen_fail, 1ull);
     courant_to_bins(dtc, wmin, 4, threadIdx.x, it.ctl);

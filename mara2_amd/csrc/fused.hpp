@@ -650,7 +650,7 @@ k_fused_plm(const double* __restrict__ Pin, const double* __restrict__ P0, doubl
 // buffer, so RK1 blocks take the flux-array path, which updates in place.
 inline bool fused_supported(const mara_desc* d)
 {
-    return d->law == MARA_LAW_HYDRO && d->cells[1] > 1 && d->rk_order >= 2;
+    return d->law == MARA_LAW_HYDRO && d->cells[1] > 1 && d->rk_order >= 2 && d->geometry == MARA_GEOM_CARTESIAN;
 }
 
 // 1-D blocks of two or three stages run each stage as pencil tiles
@@ -659,7 +659,7 @@ inline bool fused_supported(const mara_desc* d)
 inline bool pencil_supported(const mara_desc* d)
 {
     return d->cells[1] == 1 && d->cells[2] == 1 && d->cells[0] > 1 && d->guards[0] > 0 && d->rk_order >= 2
-           && !(d->cooling_rate > 0.0);
+           && !(d->cooling_rate > 0.0) && d->geometry == MARA_GEOM_CARTESIAN;
 }
 
 inline const char* fused_kernel_name(const mara_desc& d)

@@ -529,7 +529,8 @@ k_fast_plm(const double* __restrict__ Pin, const double* __restrict__ P0, double
 // the blocks this kernel serves: hydro, PLM, 2-D or 3-D, two or three stages
 inline bool supported(const mara_desc* d)
 {
-    return d->law == MARA_LAW_HYDRO && d->recon == MARA_RECON_PLM && d->cells[1] > 1 && d->rk_order >= 2;
+    return d->law == MARA_LAW_HYDRO && d->recon == MARA_RECON_PLM && d->cells[1] > 1 && d->rk_order >= 2
+           && d->geometry == MARA_GEOM_CARTESIAN;
 }
 
 // One stage on the block's stream: R = double for the fast binary64 form,
@@ -567,7 +568,7 @@ enum CflArith { CFL_EXACT = 0, CFL_MIXED = 1, CFL_FAST64 = 2, CFL_FAST32 = 3 };
 
 template <int Law>
 __device__ __forceinline__ double zone_courant_form(int arith, const double* __restrict__ P, const Geom& G,
-                                                    long long c, int& err)
+                                                    long long c, double Lz, int& err)
 {
     if constexpr (Law == 0)
     {
@@ -595,10 +596,10 @@ __device__ __forceinline__ double zone_courant_form(int arith, const double* __r
             double p[NQ];
 #pragma unroll
             for (int q = 0; q < NQ; ++q) p[q] = P[q * G.fs + c];
-            return courant_of<Law, float>(p, G, err);
+            return courant_of<Law, float>(p, G, Lz, err);
         }
     }
-    return zone_courant<Law>(P, G, c, err);
+    return zone_courant<Law>(P, G, c, Lz, err);
 }
 
 // k_courant with the form's arithmetic
@@ -619,7 +620,8 @@ static __global__ void __launch_bounds__(256) k_courant_form(const double* __res
         const int j = (int)(r / G.n[0]);
         const int i = (int)(r - (long long)j * G.n[0]);
         const long long c = cell_index(G, G.g[0] + i, G.g[1] + j, G.g[2] + (int)kk);
-        const double d = zone_courant_form<Law>(arith, P, G, c, err);
+        const double d =
+            zone_courant_form<Law>(arith, P, G, c, zone_length(G, G.g[0] + i, G.g[1] + j, G.g[2] + (int)kk), err);
         if (d < dtc) dtc = d;
     }
     if (err & ERR_SUPERLUMINAL) atomicAdd(&ctl->superluminal, 1ull);

@@ -198,7 +198,7 @@ k_mhd2d(const double* __restrict__ Pin, const double* __restrict__ P0, double* _
 inline bool mhd2d_supported(const mara_desc* d)
 {
     return d->law == MARA_LAW_MHD && d->cells[0] > 1 && d->cells[1] > 1 && d->cells[2] == 1 && d->guards[0] > 0
-           && d->guards[1] > 0 && d->rk_order >= 2 && !(d->cooling_rate > 0.0);
+           && d->guards[1] > 0 && d->rk_order >= 2 && !(d->cooling_rate > 0.0) && d->geometry == MARA_GEOM_CARTESIAN;
 }
 
 template <int Recon, typename R, int TY>

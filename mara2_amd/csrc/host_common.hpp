@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <vector>
 
 #include "../../include/mara_gpu.h"
 #include "kernels.hpp"
@@ -78,7 +79,52 @@ inline int validate(const mara_desc* d)
         if (d->guards[a] > 0 && d->cells[a] < d->guards[a]) return MARA_ERR_UNSUPPORTED;
     }
     if (d->cells[0] == 1 && (d->cells[1] > 1 || d->cells[2] > 1)) return MARA_ERR_UNSUPPORTED;
+    if (d->geometry < MARA_GEOM_CARTESIAN || d->geometry > MARA_GEOM_SPHERICAL) return MARA_ERR_INVALID_ARGUMENT;
+    if (d->geometry == MARA_GEOM_SPHERICAL)
+    {
+        // CellCenteredFieldCT is the Cartesian construction
+        if (d->law != MARA_LAW_HYDRO && d->field_ct) return MARA_ERR_UNSUPPORTED;
+        if (d->log_radial && !(d->lower[0] > 0.0)) return MARA_ERR_INVALID_ARGUMENT;
+    }
     return MARA_OK;
+}
+
+// SphericalMeshGeometry::getEdge (src/SphericalMeshGeometry.cpp:190-205):
+// edge n (interior-relative, fractional for centres) along an axis.
+inline double sph_edge(const mara_desc& d, double n, int axis)
+{
+    if (axis == 0 && d.log_radial) return d.lower[0] * std::pow(d.upper[0] / d.lower[0], n / d.cells[0]);
+    return d.lower[axis] + (d.upper[axis] - d.lower[axis]) * n / d.cells[axis];
+}
+
+// The tables of SphView for a spherical block (host libm), in its layout.
+inline std::vector<double> spherical_tables(const mara_desc& d, const Geom& G)
+{
+    std::vector<double> t;
+    t.reserve((size_t)SphView::size(G));
+    const int* g = G.g;
+    for (int n = 0; n <= G.N[0]; ++n) t.push_back(sph_edge(d, n - g[0], 0));
+    for (int n = 0; n <= G.N[1]; ++n) t.push_back(sph_edge(d, n - g[1], 1));
+    for (int n = 0; n <= G.N[1]; ++n) t.push_back(std::cos(sph_edge(d, n - g[1], 1)));
+    for (int n = 0; n <= G.N[1]; ++n) t.push_back(std::sin(sph_edge(d, n - g[1], 1)));
+    for (int n = 0; n <= G.N[2]; ++n) t.push_back(sph_edge(d, n - g[2], 2));
+    // cellLength's radius (src/SphericalMeshGeometry.cpp:79-81); the linear
+    // case takes its second edge from axis 1, as the reference does
+    for (int i = 0; i < G.N[0]; ++i)
+    {
+        const int ic = i - g[0];
+        t.push_back(d.log_radial ? std::sqrt(sph_edge(d, ic, 0) * sph_edge(d, ic + 1, 0))
+                                 : 0.5 * (sph_edge(d, ic, 0) + sph_edge(d, ic + 1, 1)));
+    }
+    for (int j = 0; j < G.N[1]; ++j)
+    {
+        const int jc = j - g[1];
+        t.push_back(std::sin(0.5 * (sph_edge(d, jc, 1) + sph_edge(d, jc + 1, 1))));
+    }
+    // coordinateAtIndex of the interior cells
+    for (int a = 0; a < 3; ++a)
+        for (int n = 0; n < G.n[a]; ++n) t.push_back(sph_edge(d, n + 0.5, a));
+    return t;
 }
 
 // Extents, strides and the constants of CartesianMeshGeometry

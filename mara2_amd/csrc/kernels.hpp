@@ -60,7 +60,87 @@ struct Geom
     double cool;    // NewtonianMHD cooling rate (law 3), else 0
     double dA[3], dV, Lmin;
     double nhat[9]; // UnitVector xhat, yhat, zhat
+    const double* sph;  // spherical mesh: the edge tables of SphView, else null (Cartesian)
 };
+
+// ---------------------------------------------------------------------------
+// Spherical mesh (SphericalMeshGeometry, src/SphericalMeshGeometry.cpp).  The
+// host evaluates the edges getEdge(n, axis) and their cosines and sines once
+// with the host libm; the kernels form face areas, volumes and cell lengths
+// from them with the reference's expressions in the reference's operation
+// order, so the FP64 form stays bit-equal to the CPU step.  Table layout,
+// array index i <-> geometry index i - g:
+//   re[N0 + 1]  radial edges          qe, cq, sq [N1 + 1]  theta edges, cos, sin
+//   pe[N2 + 1]  phi edges             ri[N0]  cellLength's radius of cell i
+//   sqm[N1]     sin of the cell's mid-theta
+//   rc[n0], qc[n1], pc[n2]  interior cell centres (coordinateAtIndex)
+// ---------------------------------------------------------------------------
+struct SphView
+{
+    const double *re, *qe, *cq, *sq, *pe, *ri, *sqm, *rc, *qc, *pc;
+    __host__ __device__ explicit SphView(const Geom& G)
+    {
+        re = G.sph;
+        qe = re + G.N[0] + 1;
+        cq = qe + G.N[1] + 1;
+        sq = cq + G.N[1] + 1;
+        pe = sq + G.N[1] + 1;
+        ri = pe + G.N[2] + 1;
+        sqm = ri + G.N[0];
+        rc = sqm + G.N[1];
+        qc = rc + G.n[0];
+        pc = qc + G.n[1];
+    }
+    static long long size(const Geom& G)
+    {
+        return (G.N[0] + 1) + 3ll * (G.N[1] + 1) + (G.N[2] + 1) + G.N[0] + G.N[1] + G.n[0] + G.n[1] + G.n[2];
+    }
+};
+
+// Face areas of cell (i, j, k) on its low and high side along each axis and
+// its volume: MeshOperator::divergence's A1000, A0000, A0101, A0001, A0012,
+// A0002 and V (src/MeshOperator.cpp:356-362) from SphericalMeshGeometry::
+// faceArea and cellVolume (src/SphericalMeshGeometry.cpp:93-129).
+struct CellAreas
+{
+    double x1, x0, y1, y0, z1, z0, vol;
+};
+
+__device__ __forceinline__ CellAreas spherical_areas(const Geom& G, int i, int j, int k)
+{
+    const SphView s(G);
+    const double r0 = s.re[i], r1 = s.re[i + 1];
+    const double dp = s.pe[k + 1] - s.pe[k];
+    const double dc = s.cq[j + 1] - s.cq[j];
+    CellAreas A;
+    // faceArea(i + 1, j, k, 0) and (i, j, k, 0): -r0 r0 (p1 - p0) (cos q1 - cos q0)
+    A.x1 = -r1 * r1 * dp * dc;
+    A.x0 = -r0 * r0 * dp * dc;
+    const double dr3 = r1 * r1 * r1 - r0 * r0 * r0;
+    // faceArea(i, j + 1, k, 1) and (i, j, k, 1): 1/3 (r1^3 - r0^3) sin q0 (p1 - p0)
+    A.y1 = +1. / 3 * dr3 * s.sq[j + 1] * dp;
+    A.y0 = +1. / 3 * dr3 * s.sq[j] * dp;
+    // faceArea(i, j, k + 1, 2) = faceArea(i, j, k, 2): -1/3 (r1^3 - r0^3) (cos q1 - cos q0)
+    A.z1 = A.z0 = -1. / 3 * dr3 * dc;
+    A.vol = -1. / 3 * dr3 * dc * dp;
+    return A;
+}
+
+// MeshOperator::linearCellDimension of interior cell (i, j, k): the smallest
+// of SphericalMeshGeometry::cellLength along the three axes
+// (src/SphericalMeshGeometry.cpp:77-91); G.Lmin on a Cartesian mesh.
+__device__ __forceinline__ double zone_length(const Geom& G, int i, int j, int k)
+{
+    if (!G.sph) return G.Lmin;
+    const SphView s(G);
+    const double l0 = 1. * (s.re[i + 1] - s.re[i]);
+    const double l1 = s.ri[i] * (s.qe[j + 1] - s.qe[j]);
+    const double l2 = s.ri[i] * (s.pe[k + 1] - s.pe[k]) * s.sqm[j];
+    double L = l0;
+    if (l1 < L) L = l1;
+    if (l2 < L) L = l2;
+    return L;
+}
 
 __device__ __forceinline__ long long cell_index(const Geom& G, int i, int j, int k)
 {
@@ -245,7 +325,7 @@ __device__ __forceinline__ double ct_value(const R* __restrict__ Fx, const R* __
 // R is the arithmetic of the wave speeds: binary64 for the FP64 form (the
 // reference's), binary32 for the hydro and MHD laws in the mixed form.
 template <int Law, typename R = double>
-__device__ __forceinline__ double courant_of(const double* p, const Geom& G, int& err)
+__device__ __forceinline__ double courant_of(const double* p, const Geom& G, double Lz, int& err)
 {
     constexpr int NQ = LawNQ<Law>::value;
     R pr[NQ];
@@ -270,18 +350,26 @@ __device__ __forceinline__ double courant_of(const double* p, const Geom& G, int
     R lmax = lam[0];
     if (lmax < lam[1]) lmax = lam[1];
     if (lmax < lam[2]) lmax = lam[2];
-    if constexpr (sizeof(R) == 8) return G.Lmin / lmax;
-    else return double(Math<R>::div(R(G.Lmin), lmax));
+    if constexpr (sizeof(R) == 8) return Lz / lmax;
+    else return double(Math<R>::div(R(Lz), lmax));
+}
+
+// on a Cartesian mesh (one cell length for every zone)
+template <int Law, typename R = double>
+__device__ __forceinline__ double courant_of(const double* p, const Geom& G, int& err)
+{
+    return courant_of<Law, R>(p, G, G.Lmin, err);
 }
 
 template <int Law>
-__device__ __forceinline__ double zone_courant(const double* __restrict__ P, const Geom& G, long long c, int& err)
+__device__ __forceinline__ double zone_courant(const double* __restrict__ P, const Geom& G, long long c, double Lz,
+                                               int& err)
 {
     constexpr int NQ = LawNQ<Law>::value;
     double p[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) p[q] = P[q * G.fs + c];
-    return courant_of<Law>(p, G, err);
+    return courant_of<Law>(p, G, Lz, err);
 }
 
 // Work-group minimum of v (every thread of the group must call), then one
@@ -427,8 +515,9 @@ __device__ __forceinline__ double finish_zone(const double* __restrict__ P0, dou
             // the next step's CFL from the zone's new primitives (interior only,
             // as FieldOperator::getCourantTimestep reads them)
             int err = 0;
-            if constexpr (MIX) dtc = courant_of<Law, float>(Pn, G, err);
-            else dtc = courant_of<Law>(Pn, G, err);
+            const double Lz = zone_length(G, i, j, k);
+            if constexpr (MIX) dtc = courant_of<Law, float>(Pn, G, Lz, err);
+            else dtc = courant_of<Law>(Pn, G, Lz, err);
             if (err & ERR_SUPERLUMINAL) atomicAdd(&ctl->superluminal, 1ull);
             if (err & ERR_EIGEN) atomicAdd(&ctl->eigen_fail, 1ull);
         }
@@ -471,6 +560,18 @@ __device__ __forceinline__ double divergence_R(const Geom& G, R x1, R x0, R y1, 
     return double(R(fac) * (+x1 * A0 - x0 * A0 + y1 * A1 - y0 * A1 + z1 * A2 - z0 * A2));
 }
 
+// the same with the face areas and volume of one cell (spherical mesh)
+template <typename R>
+__device__ __forceinline__ double divergence_R(const CellAreas& A, R x1, R x0, R y1, R y0, R z1, R z0)
+{
+    const double fac = -1.0 / A.vol;
+    if constexpr (sizeof(R) == 8)
+        return fac * (+x1 * A.x1 - x0 * A.x0 + y1 * A.y1 - y0 * A.y0 + z1 * A.z1 - z0 * A.z0);
+    else
+        return double(R(fac) * (+x1 * R(A.x1) - x0 * R(A.x0) + y1 * R(A.y1) - y0 * R(A.y0) + z1 * R(A.z1)
+                                - z0 * R(A.z0)));
+}
+
 __device__ __forceinline__ double divergence_ct(const Geom& G, double x1, double x0, double y1, double y0,
                                                 double z1, double z0)
 {
@@ -493,6 +594,8 @@ __device__ __forceinline__ double update_zone(const double* __restrict__ P0, dou
     const long long f = face_index(G, i, j, k);
     const long long fx = f + 1, fy = f + (G.N[0] + 1), fz = f + (long long)(G.N[0] + 1) * (G.N[1] + 1);
     const bool hx = G.N[0] > 1, hy = G.N[1] > 1, hz = G.N[2] > 1;
+    CellAreas A{};
+    if (G.sph) A = spherical_areas(G, i, j, k);
     double L[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
@@ -513,7 +616,7 @@ __device__ __forceinline__ double update_zone(const double* __restrict__ P0, dou
             const R x1 = hx ? Fx[q * G.ffs + fx] : R(0), x0 = hx ? Fx[q * G.ffs + f] : R(0);
             const R y1 = hy ? Fy[q * G.ffs + fy] : R(0), y0 = hy ? Fy[q * G.ffs + f] : R(0);
             const R z1 = hz ? Fz[q * G.ffs + fz] : R(0), z0 = hz ? Fz[q * G.ffs + f] : R(0);
-            L[q] = divergence_R<R>(G, x1, x0, y1, y0, z1, z0);
+            L[q] = G.sph ? divergence_R<R>(A, x1, x0, y1, y0, z1, z0) : divergence_R<R>(G, x1, x0, y1, y0, z1, z0);
         }
     }
     return finish_zone<Law, R>(P0, U, Pout, G, b, first_stage, write_u, courant_out, ctl, i, j, k, L, Ssrc);
@@ -883,7 +986,7 @@ k_courant(const double* __restrict__ P, Geom G, Ctl* ctl)
         const int j = (int)(r / G.n[0]);
         const int i = (int)(r - (long long)j * G.n[0]);
         const long long c = cell_index(G, G.g[0] + i, G.g[1] + j, G.g[2] + (int)kk);
-        const double d = zone_courant<Law>(P, G, c, err);
+        const double d = zone_courant<Law>(P, G, c, zone_length(G, G.g[0] + i, G.g[1] + j, G.g[2] + (int)kk), err);
         if (d < dtc) dtc = d;
     }
     if (err & ERR_SUPERLUMINAL) atomicAdd(&ctl->superluminal, 1ull);

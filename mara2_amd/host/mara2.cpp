@@ -106,6 +106,96 @@ Coordinate CartesianMeshGeometry::coordinateAtIndex(double i, double j, double k
     return Coordinate{{lower[0] + dx[0] * (i + 0.5), lower[1] + dx[1] * (j + 0.5), lower[2] + dx[2] * (k + 0.5)}};
 }
 
+void MeshGeometry::describe(mara_desc& d) const
+{
+    const Shape cs = cellsShape();
+    const Coordinate lo = lowerCorner(), hi = upperCorner();
+    for (int a = 0; a < 3; ++a)
+    {
+        d.cells[a] = cs[a];
+        d.lower[a] = lo[a];
+        d.upper[a] = hi[a];
+    }
+    d.geometry = geometryCode();
+    d.log_radial = logRadial();
+}
+
+// ---------------------------------------------------------------------------
+SphericalMeshGeometry::SphericalMeshGeometry() : shape{{128, 1, 1, 1, 1}}, lower{{1.0, 0.0, 0.0}}, upper{{10.0, M_PI, 2 * M_PI}} {}
+
+SphericalMeshGeometry::SphericalMeshGeometry(Shape S)
+    : shape{{S[0], S[1], S[2], 1, 1}}, lower{{1.0, 0.0, 0.0}}, upper{{10.0, M_PI, 2 * M_PI}}
+{
+}
+
+SphericalMeshGeometry::SphericalMeshGeometry(int ni, int nj, int nk)
+    : shape{{ni, nj, nk, 1, 1}}, lower{{1.0, 0.0, 0.0}}, upper{{10.0, M_PI, 2 * M_PI}}
+{
+}
+
+void SphericalMeshGeometry::setCellsShape(Shape S)
+{
+    shape[0] = S[0];
+    shape[1] = S[1];
+    shape[2] = S[2];
+}
+
+// edge n of an axis, fractional n for centres (:190-205)
+double SphericalMeshGeometry::getEdge(double n, int axis) const
+{
+    if (axis == 0 && logarithmic) return lower[0] * std::pow(upper[0] / lower[0], n / shape[0]);
+    return lower[axis] + (upper[axis] - lower[axis]) * n / shape[axis];
+}
+
+Coordinate SphericalMeshGeometry::coordinateAtIndex(double i, double j, double k) const
+{
+    return Coordinate{{getEdge(i + 0.5, 0), getEdge(j + 0.5, 1), getEdge(k + 0.5, 2)}};
+}
+
+unsigned long SphericalMeshGeometry::totalCellsInMesh() const
+{
+    return (unsigned long)shape[0] * shape[1] * shape[2];
+}
+
+// (:77-91); with linear edges the reference averages radial edge i with
+// theta edge i + 1, which the GPU step reproduces
+double SphericalMeshGeometry::cellLength(int i, int j, int k, int axis) const
+{
+    const double ri = logarithmic ? std::sqrt(getEdge(i, 0) * getEdge(i + 1, 0)) : 0.5 * (getEdge(i, 0) + getEdge(i + 1, 1));
+    const double qi = 0.5 * (getEdge(j, 1) + getEdge(j + 1, 1));
+    if (axis == 0) return 1. * (getEdge(i + 1, 0) - getEdge(i, 0));
+    if (axis == 1) return ri * (getEdge(j + 1, 1) - getEdge(j, 1));
+    if (axis == 2) return ri * (getEdge(k + 1, 2) - getEdge(k, 2)) * std::sin(qi);
+    throw std::logic_error("Axis argument not 0, 1, or 2");
+}
+
+// (:106-123): the face on the low side of cell (i, j, k)
+double SphericalMeshGeometry::faceArea(int i, int j, int k, int axis) const
+{
+    const double r0 = getEdge(i, 0), r1 = getEdge(i + 1, 0);
+    const double q0 = getEdge(j, 1), q1 = getEdge(j + 1, 1);
+    const double p0 = getEdge(k, 2), p1 = getEdge(k + 1, 2);
+    if (axis == 0) return -r0 * r0 * (p1 - p0) * (std::cos(q1) - std::cos(q0));
+    if (axis == 1) return +1. / 3 * (r1 * r1 * r1 - r0 * r0 * r0) * std::sin(q0) * (p1 - p0);
+    if (axis == 2) return -1. / 3 * (r1 * r1 * r1 - r0 * r0 * r0) * (std::cos(q1) - std::cos(q0));
+    throw std::logic_error("Invalid axis");
+}
+
+double SphericalMeshGeometry::cellVolume(int i, int j, int k) const
+{
+    const double r0 = getEdge(i, 0), r1 = getEdge(i + 1, 0);
+    const double q0 = getEdge(j, 1), q1 = getEdge(j + 1, 1);
+    const double p0 = getEdge(k, 2), p1 = getEdge(k + 1, 2);
+    return -1. / 3 * (r1 * r1 * r1 - r0 * r0 * r0) * (std::cos(q1) - std::cos(q0)) * (p1 - p0);
+}
+
+double SphericalMeshGeometry::meshVolume() const
+{
+    return -1. / 3 * (upper[0] * upper[0] * upper[0] - lower[0] * lower[0] * lower[0])
+           * (std::cos(upper[1]) - std::cos(lower[1])) * (upper[2] - lower[2]);
+}
+
+// ---------------------------------------------------------------------------
 unsigned long CartesianMeshGeometry::totalCellsInMesh() const
 {
     return (unsigned long)shape[0] * shape[1] * shape[2];
@@ -421,8 +511,8 @@ mara_desc GpuMethodOfLinesTVD::describe(const MeshData& md) const
     if (!fluxScheme) throw std::logic_error("No IntercellFluxScheme instance");
     auto law = fieldOperator->getConservationLaw();
     if (!law) throw std::logic_error("FieldOperator has no ConservationLaw");
-    auto geom = std::dynamic_pointer_cast<CartesianMeshGeometry>(meshOperator->getMeshGeometry());
-    if (!geom) throw std::logic_error("GPU step needs a CartesianMeshGeometry");
+    auto geom = meshOperator->getMeshGeometry();
+    if (!geom) throw std::logic_error("MeshOperator has no MeshGeometry instance");
     mara_desc d;
     std::memset(&d, 0, sizeof d);
     d.law = law->lawCode();
@@ -434,13 +524,10 @@ mara_desc GpuMethodOfLinesTVD::describe(const MeshData& md) const
     d.riemann = fluxScheme->getRiemannSolver() ? fluxScheme->getRiemannSolver()->solverCode() : MARA_RIEMANN_HLLE;
     d.bc = boundaryCondition->bcCode();
     const Shape cs = geom->cellsShape();
-    const Coordinate lo = geom->lowerCorner(), hi = geom->upperCorner();
+    geom->describe(d);
     for (int a = 0; a < 3; ++a)
     {
-        d.cells[a] = cs[a];
         d.guards[a] = md.boundaryShape[a];
-        d.lower[a] = lo[a];
-        d.upper[a] = hi[a];
         if (md.P.size(a) != cs[a] + 2 * md.boundaryShape[a])
             throw std::logic_error("MeshData shape does not match the mesh geometry");
     }
@@ -623,12 +710,14 @@ FieldOperator::~FieldOperator()
 double FieldOperator::getCourantTimestep(const MeshData& md, const Array& L) const
 {
     if (!law) throw std::logic_error("FieldOperator has no ConservationLaw");
-    if (!geometry) throw std::logic_error("FieldOperator needs the CartesianMeshGeometry (setMeshGeometry)");
+    if (!geometry) throw std::logic_error("FieldOperator needs the MeshGeometry (setMeshGeometry)");
     const Shape cs = geometry->cellsShape();
     if (L.size(0) != cs[0] || L.size(1) != cs[1] || L.size(2) != cs[2])
         throw std::logic_error("Primitive and linear cell dimension arrays have different sizes");
-    // the device reduction uses one length for the uniform Cartesian mesh
-    for (std::size_t n = 1; n < L.size(); ++n)
+    // the device evaluates linearCellDimension of the mesh itself: one length
+    // on a Cartesian mesh, the geometry's per-cell lengths on a spherical one
+    const bool cartesian = geometry->getType() == "cartesian";
+    for (std::size_t n = 1; cartesian && n < L.size(); ++n)
         if (L.data()[n] != L.data()[0]) throw std::logic_error("non-uniform linear cell dimension is not supported");
     mara_desc d;
     std::memset(&d, 0, sizeof d);
@@ -640,14 +729,8 @@ double FieldOperator::getCourantTimestep(const MeshData& md, const Array& L) con
     d.plm_theta = 1.5;
     d.bc = MARA_BC_OUTFLOW;
     d.rk_order = 1;
-    const Coordinate lo = geometry->lowerCorner(), hi = geometry->upperCorner();
-    for (int a = 0; a < 3; ++a)
-    {
-        d.cells[a] = cs[a];
-        d.guards[a] = md.boundaryShape[a];
-        d.lower[a] = lo[a];
-        d.upper[a] = hi[a];
-    }
+    geometry->describe(d);
+    for (int a = 0; a < 3; ++a) d.guards[a] = md.boundaryShape[a];
     if (block && std::memcmp(&d, &blockDesc, sizeof d) != 0)
     {
         mara_block_destroy(block);

@@ -180,6 +180,14 @@ public:
     virtual double cellLength(int i, int j, int k, int axis) const = 0;
     virtual double faceArea(int i, int j, int k, int axis) const = 0;
     virtual double cellVolume(int i, int j, int k) const = 0;
+    virtual Coordinate lowerCorner() const = 0;
+    virtual Coordinate upperCorner() const = 0;
+    virtual std::string getType() const = 0;
+    // the mesh's part of a block descriptor: cells, lower, upper, geometry
+    void describe(mara_desc& d) const;
+protected:
+    virtual int geometryCode() const = 0;
+    virtual int logRadial() const { return 0; }
 };
 
 // src/CartesianMeshGeometry.cpp:61-157
@@ -197,13 +205,48 @@ public:
     // src/CartesianMeshGeometry.cpp:79-92: uniform areas and volume
     double faceArea(int, int, int, int axis) const override { return axis == 0 ? dx[1] * dx[2] : axis == 1 ? dx[2] * dx[0] : dx[0] * dx[1]; }
     double cellVolume(int, int, int) const override { return dx[0] * dx[1] * dx[2]; }
-    Coordinate lowerCorner() const { return lower; }
-    Coordinate upperCorner() const { return upper; }
+    Coordinate lowerCorner() const override { return lower; }
+    Coordinate upperCorner() const override { return upper; }
+    std::string getType() const override { return "cartesian"; }
+protected:
+    int geometryCode() const override { return MARA_GEOM_CARTESIAN; }
 private:
     void cacheSpacing();
     Shape shape;
     Coordinate lower, upper;
     double dx[3];
+};
+
+// src/SphericalMeshGeometry.cpp: axes (r, theta, phi); linear edges, or
+// logarithmic radial ones.  The GPU step evaluates the same areas, volumes
+// and lengths from the descriptor (MARA_GEOM_SPHERICAL).
+class SphericalMeshGeometry : public MeshGeometry
+{
+public:
+    SphericalMeshGeometry();
+    explicit SphericalMeshGeometry(Shape S);
+    SphericalMeshGeometry(int ni, int nj, int nk);
+    void setUseLogarithmicRadialBinning(bool shouldUse) { logarithmic = shouldUse; }
+    void setCellsShape(Shape S) override;
+    void setLowerUpper(Coordinate L, Coordinate U) override { lower = L; upper = U; }
+    Shape cellsShape() const override { return shape; }
+    Coordinate coordinateAtIndex(double i, double j, double k) const override;
+    unsigned long totalCellsInMesh() const override;
+    double cellLength(int i, int j, int k, int axis) const override;
+    double faceArea(int i, int j, int k, int axis) const override;
+    double cellVolume(int i, int j, int k) const override;
+    double meshVolume() const;
+    Coordinate lowerCorner() const override { return lower; }
+    Coordinate upperCorner() const override { return upper; }
+    std::string getType() const override { return "spherical"; }
+    double getEdge(double n, int axis) const;
+protected:
+    int geometryCode() const override { return MARA_GEOM_SPHERICAL; }
+    int logRadial() const override { return logarithmic ? 1 : 0; }
+private:
+    Shape shape;
+    Coordinate lower, upper;
+    bool logarithmic = false;
 };
 
 class BoundaryCondition
@@ -311,10 +354,10 @@ public:
     // P is the MeshData primitive array with guards, L from
     // MeshOperator::linearCellDimension.  Evaluated on the GPU.
     virtual double getCourantTimestep(const MeshData& md, const Array& L) const;
-    void setMeshGeometry(std::shared_ptr<CartesianMeshGeometry> g) { geometry = g; }
+    void setMeshGeometry(std::shared_ptr<MeshGeometry> g) { geometry = g; }
 private:
     std::shared_ptr<ConservationLaw> law;
-    std::shared_ptr<CartesianMeshGeometry> geometry;
+    std::shared_ptr<MeshGeometry> geometry;
     mutable mara_block* block = nullptr;  // reused while the descriptor is unchanged
     mutable mara_desc blockDesc{};
 };

@@ -125,9 +125,29 @@ std::map<std::string, Problem> problemsMhd()
     return p;
 }
 
+// The ThermalConvection program (src/Problems/ThermalConvection.cpp:176-300)
+// with its default user values: a polytrope rho = r^-1.5, p = K rho^gamma
+// (K = r0 g0 (gamma - 1) / gamma) at rest on a spherical wedge, r in
+// [1, 100] with logarithmic edges, theta within pi/12 of the equator, phi in
+// [0, 0.1].  Its gravity, heating and geometric source terms come from the
+// code object given on the command line (mara2_amd/sources/
+// thermal_convection.hip).  The program's acoustic-driver boundary is not
+// part of this path: outflow stands in for it.
+std::map<std::string, Problem> problemsSpherical()
+{
+    std::map<std::string, Problem> p;
+    const double g0 = 0.2, d0 = 1.0, gamma = 5. / 3, gm = 5. / 3, r0 = 1.0;
+    const double K = r0 * g0 * (gm - 1) / gm / std::pow(d0, gm - 1);
+    p["ThermalConvection"] = {0.5, std::make_shared<OutflowBoundaryCondition>(), [=](double r, double, double) {
+        const double d = d0 * std::pow(r, -1.5);
+        return std::vector<double>{d, 0, 0, 0, K * std::pow(d, gamma)};
+    }};
+    return p;
+}
+
 int usage()
 {
-    std::cerr << "usage: mara_regress <regress-1d|regress-rel|regress-2d|regress-mhd|sod> <problem> <scheme> <out.bin>"
+    std::cerr << "usage: mara_regress <regress-1d|regress-rel|regress-2d|regress-mhd|sod|thermal-convection> <problem> <scheme> <out.bin>"
                  " [max_steps [[viscous=]code.hsaco kernel [parameters...]]]\n";
     return 2;
 }
@@ -208,11 +228,29 @@ int main(int argc, const char* argv[])
         table["Sod"] = {0.2, outflow, [](double x, double, double) { return pick(x < 0.5, {1.0, 0.0, 0.0, 0.0, 1.0}, {0.125, 0.0, 0.0, 0.0, 0.1}); }};
         cfl = 0.4;
     }
+    else if (program == "thermal-convection")
+    {
+        cs = {{64, 64, 1, 1, 1}};
+        bs = {{2, 2, 0, 0, 0}};
+        cl = std::make_shared<NewtonianHydro>();
+        table = problemsSpherical();
+        fs = std::make_shared<MethodOfLinesPlm>();
+        fs->setPlmTheta(2.0);
+        fs->setRiemannSolver(std::make_shared<HllcNewtonianHydroRiemannSolver>());
+    }
     else return usage();
     if (!table.count(pname)) return usage();
     const Problem& problem = table[pname];
 
-    auto mg = std::make_shared<CartesianMeshGeometry>();
+    std::shared_ptr<MeshGeometry> mg;
+    if (program == "thermal-convection")
+    {
+        auto sg = std::make_shared<SphericalMeshGeometry>();
+        sg->setLowerUpper({{1.0, M_PI * 0.5 - M_PI / 12.0, 0}}, {{100.0, M_PI * 0.5 + M_PI / 12.0, 0.1}});
+        sg->setUseLogarithmicRadialBinning(true);
+        mg = sg;
+    }
+    else mg = std::make_shared<CartesianMeshGeometry>();
     auto mo = std::make_shared<MeshOperator>();
     auto fo = std::make_shared<FieldOperator>();
     auto md = std::make_shared<MeshData>(cs, bs, cl->getNumConserved());

@@ -70,6 +70,8 @@ def _load(path):
     L.oracle_set_threads.argtypes = [_i]
     L.oracle_rmhd_c2p_codes.argtypes = [_dp, _d, _d, _ip]
     L.oracle_last_failure.argtypes = [_dp]
+    L.oracle_cell_geometry.argtypes = [ctypes.POINTER(MaraDesc), _i, _i, _i, _dp]
+    L.oracle_divergence_desc.argtypes = [ctypes.POINTER(MaraDesc), _dp, _i, _d, _dp]
     return L
 
 
@@ -168,6 +170,24 @@ def divergence(F, factor, dA, dV):
     return D
 
 
+def cell_geometry(desc, i, j, k):
+    """(low-side face areas [3], volume, cell lengths [3], centre [3]) of
+    interior-relative cell (i, j, k) of the descriptor's mesh."""
+    out = np.zeros(10)
+    lib().oracle_cell_geometry(ctypes.byref(desc), i, j, k, _ptr(out))
+    return out[:3].copy(), out[3], out[4:7].copy(), out[7:10].copy()
+
+
+def divergence_desc(desc, F, factor=1.0):
+    """MeshOperator::divergence with the mesh's geometry: F of shape
+    (N1+1, N2+1, N3+1, nq, 3) over the padded array."""
+    N = [desc.cells[a] + 2 * desc.guards[a] for a in range(3)]
+    assert F.shape[:3] == (N[0] + 1, N[1] + 1, N[2] + 1) and F.shape[4] == 3
+    D = np.zeros((N[0], N[1], N[2], F.shape[3]))
+    lib().oracle_divergence_desc(ctypes.byref(desc), _ptr(np.ascontiguousarray(F)), F.shape[3], factor, _ptr(D))
+    return D
+
+
 def monopole_cell(B):
     s = (ctypes.c_int * 3)(*B.shape[:3])
     M = np.zeros(B.shape[:3])
@@ -187,7 +207,8 @@ def apply_bc(desc, P):
 
 
 # test sources compiled into the oracle (mara_oracle.c): name -> C symbol
-SOURCES = {"uniform_gravity": "oracle_source_uniform_gravity", "point_mass": "oracle_source_point_mass"}
+SOURCES = {"uniform_gravity": "oracle_source_uniform_gravity", "point_mass": "oracle_source_point_mass",
+           "thermal_convection": "oracle_source_thermal_convection"}
 # test viscous-flux functions compiled into the oracle
 VISCOUS = {"constant": "oracle_viscous_constant"}
 

@@ -1052,6 +1052,128 @@ void oracle_monopole_cell(const double* B, const int s[3], double* Mc)
     free(M);
 }
 
+/* ==========================================================================
+   SphericalMeshGeometry (src/SphericalMeshGeometry.cpp), axes (r, theta, phi).
+   Indices are the geometry's (interior-relative, negative in the guards);
+   every quantity is evaluated from the edges each time, as the reference does.
+   ========================================================================== */
+/* getEdge (:190-205) */
+static double sph_edge(const mara_desc* d, double n, int axis)
+{
+    if (axis == 0 && d->log_radial) return d->lower[0] * pow(d->upper[0] / d->lower[0], n / d->cells[0]);
+    return d->lower[axis] + (d->upper[axis] - d->lower[axis]) * n / d->cells[axis];
+}
+
+/* faceArea (:106-123): the face on the low side of cell (i, j, k) along axis */
+static double sph_face_area(const mara_desc* d, int i, int j, int k, int axis)
+{
+    const double r0 = sph_edge(d, i, 0), r1 = sph_edge(d, i + 1, 0);
+    const double q0 = sph_edge(d, j, 1), q1 = sph_edge(d, j + 1, 1);
+    const double p0 = sph_edge(d, k, 2), p1 = sph_edge(d, k + 1, 2);
+    if (axis == 0) return -r0 * r0 * (p1 - p0) * (cos(q1) - cos(q0));
+    if (axis == 1) return +1. / 3 * (r1 * r1 * r1 - r0 * r0 * r0) * sin(q0) * (p1 - p0);
+    return -1. / 3 * (r1 * r1 * r1 - r0 * r0 * r0) * (cos(q1) - cos(q0));
+}
+
+/* cellVolume (:93-102) */
+static double sph_cell_volume(const mara_desc* d, int i, int j, int k)
+{
+    const double r0 = sph_edge(d, i, 0), r1 = sph_edge(d, i + 1, 0);
+    const double q0 = sph_edge(d, j, 1), q1 = sph_edge(d, j + 1, 1);
+    const double p0 = sph_edge(d, k, 2), p1 = sph_edge(d, k + 1, 2);
+    return -1. / 3 * (r1 * r1 * r1 - r0 * r0 * r0) * (cos(q1) - cos(q0)) * (p1 - p0);
+}
+
+/* cellLength (:77-91); with linear radial edges the reference averages edge i
+   of axis 0 with edge i + 1 of axis 1, kept as it is */
+static double sph_cell_length(const mara_desc* d, int i, int j, int k, int axis)
+{
+    const double ri = d->log_radial ? sqrt(sph_edge(d, i, 0) * sph_edge(d, i + 1, 0))
+                                    : 0.5 * (sph_edge(d, i, 0) + sph_edge(d, i + 1, 1));
+    const double qi = 0.5 * (sph_edge(d, j, 1) + sph_edge(d, j + 1, 1));
+    if (axis == 0) return 1. * (sph_edge(d, i + 1, 0) - sph_edge(d, i, 0));
+    if (axis == 1) return ri * (sph_edge(d, j + 1, 1) - sph_edge(d, j, 1));
+    return ri * (sph_edge(d, k + 1, 2) - sph_edge(d, k, 2)) * sin(qi);
+}
+
+/* MeshOperator::linearCellDimension (src/MeshOperator.cpp:292-307) of one
+   interior cell: *std::min_element of the three lengths */
+static double linear_cell_dimension(const mara_desc* d, const double dx[3], int i, int j, int k)
+{
+    double L3[3];
+    for (int a = 0; a < 3; ++a) L3[a] = d->geometry == MARA_GEOM_SPHERICAL ? sph_cell_length(d, i, j, k, a) : dx[a];
+    double L = L3[0];
+    if (L3[1] < L) L = L3[1];
+    if (L3[2] < L) L = L3[2];
+    return L;
+}
+
+/* coordinateAtIndex (src/CartesianMeshGeometry.cpp:61-67,
+   src/SphericalMeshGeometry.cpp:59-66) */
+static void coordinate_at(const mara_desc* d, const double dx[3], int i, int j, int k, double x[3])
+{
+    const int ix[3] = {i, j, k};
+    for (int a = 0; a < 3; ++a)
+        x[a] = d->geometry == MARA_GEOM_SPHERICAL ? sph_edge(d, ix[a] + 0.5, a) : d->lower[a] + dx[a] * (ix[a] + 0.5);
+}
+
+/* Geometry of interior-relative cell (i, j, k) for tests: low-side face areas
+   of the three axes, volume, the three cell lengths, the centre. */
+void oracle_cell_geometry(const mara_desc* d, int i, int j, int k, double out[10])
+{
+    double dx[3];
+    for (int a = 0; a < 3; ++a) dx[a] = (d->upper[a] - d->lower[a]) / d->cells[a];
+    const int sph = d->geometry == MARA_GEOM_SPHERICAL;
+    for (int a = 0; a < 3; ++a)
+        out[a] = sph ? sph_face_area(d, i, j, k, a) : dx[(a + 1) % 3] * dx[(a + 2) % 3];
+    out[3] = sph ? sph_cell_volume(d, i, j, k) : dx[0] * dx[1] * dx[2];
+    for (int a = 0; a < 3; ++a) out[4 + a] = sph ? sph_cell_length(d, i, j, k, a) : dx[a];
+    coordinate_at(d, dx, i, j, k, out + 7);
+}
+
+/* MeshOperator::divergence with the mesh's own geometry: F over the whole
+   padded array (N1+1, N2+1, N3+1, nq, 3), D over the padded cells, geometry
+   index = array index - guards (startIndex of makeFootprint). */
+void oracle_divergence_desc(const mara_desc* d, const double* F, int nq, double factor, double* D)
+{
+    const int s[3] = {d->cells[0] + 2 * d->guards[0] + 1, d->cells[1] + 2 * d->guards[1] + 1,
+                      d->cells[2] + 2 * d->guards[2] + 1};
+    if (d->geometry != MARA_GEOM_SPHERICAL)
+    {
+        double dx[3], dA[3], dV;
+        for (int a = 0; a < 3; ++a) dx[a] = (d->upper[a] - d->lower[a]) / d->cells[a];
+        dA[0] = dx[1] * dx[2];
+        dA[1] = dx[2] * dx[0];
+        dA[2] = dx[0] * dx[1];
+        dV = dx[0] * dx[1] * dx[2];
+        oracle_divergence(F, s, nq, factor, dA, dV, D);
+        return;
+    }
+    const int n0 = s[0] - 1, n1 = s[1] - 1, n2 = s[2] - 1;
+    const int g0 = d->guards[0], g1 = d->guards[1], g2 = d->guards[2];
+#define FF(i, j, k, q, a) F[((((long)(i) * s[1] + (j)) * s[2] + (k)) * nq + (q)) * 3 + (a)]
+    #pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 0; i < n0; ++i)
+    for (int j = 0; j < n1; ++j)
+    for (int k = 0; k < n2; ++k)
+    {
+        const int ic = i - g0, jc = j - g1, kc = k - g2;
+        const double A1000 = sph_face_area(d, ic + 1, jc, kc, 0), A0000 = sph_face_area(d, ic, jc, kc, 0);
+        const double A0101 = sph_face_area(d, ic, jc + 1, kc, 1), A0001 = sph_face_area(d, ic, jc, kc, 1);
+        const double A0012 = sph_face_area(d, ic, jc, kc + 1, 2), A0002 = sph_face_area(d, ic, jc, kc, 2);
+        const double vol = sph_cell_volume(d, ic, jc, kc);
+        for (int q = 0; q < nq; ++q)
+            D[(((long)i * n1 + j) * n2 + k) * nq + q] = factor / vol * (
+                +FF(i + 1, j, k, q, 0) * A1000
+                -FF(i, j, k, q, 0) * A0000
+                +FF(i, j + 1, k, q, 1) * A0101
+                -FF(i, j, k, q, 1) * A0001
+                +FF(i, j, k + 1, q, 2) * A0012
+                -FF(i, j, k, q, 2) * A0002);
+    }
+#undef FF
+}
+
 /* ========================================================================== */
 static void mesh_spacing(const mara_desc* d, double dx[3], double dA[3], double* dV)
 {
@@ -1177,6 +1299,30 @@ void oracle_source_point_mass(double x, double y, double z, const double* t, con
     S[2] = P[0] * a1;
     S[3] = P[0] * a2;
     S[4] = P[0] * (P[1] * a0 + P[2] * a1 + P[3] * a2);
+}
+
+/* The source terms of the ThermalConvection program
+   (src/Problems/ThermalConvection.cpp:218-245) on a spherical mesh, x = (r,
+   theta, phi): the geometric terms of the momentum equations in spherical
+   coordinates, gravity g0 / r^2 = params[0] / r^2 on the radial momentum and
+   the energy, and heating params[1] exp(-r) of the energy; cot(q) is
+   tan(pi/2 - q) as the program writes it. */
+void oracle_source_thermal_convection(double r, double q, double p, const double* t, const double* P, double* S,
+                                      const double* params)
+{
+    (void)p; (void)t;
+    const double g0 = params[0], q0 = params[1];
+    const double dg = P[0], vr = P[1], vq = P[2], vp = P[3], pg = P[4];
+    const double cotq = tan(M_PI_2 - q);
+    S[0] = 0.0;
+    S[1] = (2 * pg + dg * (vq * vq + vp * vp)) / r;
+    S[2] = (pg * cotq + dg * (vp * vp * cotq - vr * vq)) / r;
+    S[3] = -dg * vp * (vr + vq * cotq) / r;
+    S[4] = 0.0;
+    const double g = g0 * pow(r, -2.0);
+    S[1] += -dg * g;
+    S[4] += -dg * g * vr;
+    S[4] += q0 * exp(-r);
 }
 
 /* A viscous-flux function of the reference's setViscousFluxFunction shape
@@ -1374,7 +1520,7 @@ static int advance_impl(const mara_desc* d, double* P, double dt, oracle_source_
         if (visc) visc(d, P, F, taux, params);
 
         /* L = divergence(F, -1), U = U0 (1-b) + (U + dt L) b */
-        oracle_divergence(F, fs, nq, -1.0, dA, dV, L);
+        oracle_divergence_desc(d, F, nq, -1.0, L);
         /* ConservationLaw::addSourceTerms: NewtonianMHD cooling
            (src/ConservationLaws.cpp:489-501), every cell */
         if (law_of(d) == LAW_MHD_COOLING)
@@ -1390,18 +1536,17 @@ static int advance_impl(const mara_desc* d, double* P, double dt, oracle_source_
         }
         if (src)
         {
-            /* coordinateAtIndex(i + start) with start = -guards (src/CartesianMeshGeometry.cpp:61-67) */
+            /* coordinateAtIndex(i + start) with start = -guards */
             #pragma omp parallel for collapse(2) schedule(static)
             for (int i = 0; i < l.N[0]; ++i)
             for (int j = 0; j < l.N[1]; ++j)
             for (int k = 0; k < l.N[2]; ++k)
             {
-                const double x = d->lower[0] + dx[0] * ((i - d->guards[0]) + 0.5);
-                const double y = d->lower[1] + dx[1] * ((j - d->guards[1]) + 0.5);
-                const double z = d->lower[2] + dx[2] * ((k - d->guards[2]) + 0.5);
+                double x[3];
+                coordinate_at(d, dx, i - d->guards[0], j - d->guards[1], k - d->guards[2], x);
                 const long c = cidx(&l, i, j, k);
                 double Sq[NF] = {0};
-                src(x, y, z, taux, P + c, Sq, params);
+                src(x[0], x[1], x[2], taux, P + c, Sq, params);
                 for (int q = 0; q < nq; ++q) L[c + q] += Sq[q];
             }
         }
@@ -1465,10 +1610,6 @@ int oracle_courant_dt(const mara_desc* d, const double* P, double* dt_out)
     for (int a = 0; a < 3; ++a) l.N[a] = d->cells[a] + 2 * d->guards[a];
     double dx[3], dA[3], dV;
     mesh_spacing(d, dx, dA, &dV);
-    /* linearCellDimension: *std::min_element of (dx, dy, dz) */
-    double Lc = dx[0];
-    if (dx[1] < Lc) Lc = dx[1];
-    if (dx[2] < Lc) Lc = dx[2];
     double nhat[9];
     oracle_unit_vectors(nhat);
     double courant = DBL_MAX;
@@ -1495,6 +1636,7 @@ int oracle_courant_dt(const mara_desc* d, const double* P, double* dt_out)
         double lmax = lam[0];
         if (lmax < lam[1]) lmax = lam[1];
         if (lmax < lam[2]) lmax = lam[2];
+        const double Lc = linear_cell_dimension(d, dx, i - g0, j - g1, k - g2);
         const double dtc = Lc / lmax;
         if (dtc < courant) courant = dtc;
     }

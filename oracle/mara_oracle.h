@@ -71,6 +71,11 @@ void oracle_ct_correct(double* F, const int s[3], int nq, int mag_index);
 
 /* MeshOperator::divergence (src/MeshOperator.cpp:323-381) on a Cartesian
    mesh: flux (s0,s1,s2,nq,3) -> D (s0-1,s1-1,s2-1,nq). */
+/* face areas (low side, axes 0-2), volume, cell lengths and centre of
+   interior-relative cell (i, j, k) of the descriptor's mesh */
+void oracle_cell_geometry(const mara_desc* d, int i, int j, int k, double out[10]);
+/* MeshOperator::divergence over the padded array with the mesh's geometry */
+void oracle_divergence_desc(const mara_desc* d, const double* F, int nq, double factor, double* D);
 void oracle_divergence(const double* F, const int s[3], int nq, double factor,
                        const double dA[3], double dV, double* D);
 

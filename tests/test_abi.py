@@ -46,6 +46,7 @@ int main(void) {
   F(mara_desc, plm_theta) F(mara_desc, riemann) F(mara_desc, bc) F(mara_desc, cells)
   F(mara_desc, guards) F(mara_desc, lower) F(mara_desc, upper) F(mara_desc, rk_order)
   F(mara_desc, field_ct) F(mara_desc, precision) F(mara_desc, cooling_rate)
+  F(mara_desc, geometry) F(mara_desc, log_radial)
   F(mara_report, status) F(mara_report, c2p_failures) F(mara_report, first_failure)
   F(mara_report, floor_applied) F(mara_report, eigen_failures) F(mara_report, superluminal)
   return 0; }
@@ -95,6 +96,27 @@ def test_create_rejects_small_guards_and_hllc_mhd():
     assert lib.mara_block_create(ctypes.byref(d), ctypes.byref(h)) == abi.ERR_GUARD_TOO_SMALL
     d = abi.make_desc(law="mhd", riemann="hllc")
     assert lib.mara_block_create(ctypes.byref(d), ctypes.byref(h)) == abi.ERR_UNSUPPORTED
+
+
+def test_spherical_mesh_checks():
+    """Geometry codes beyond the two meshes, flux-CT on a spherical mesh
+    (CellCenteredFieldCT is the Cartesian construction), logarithmic radial
+    edges from r = 0, and the entry points that do not take a spherical
+    mesh: block groups and the no-allocation step."""
+    lib = runtime.load()
+    h = ctypes.c_void_p()
+    d = abi.make_desc(geometry=5)
+    assert lib.mara_block_create(ctypes.byref(d), ctypes.byref(h)) == abi.ERR_INVALID_ARGUMENT
+    d = abi.make_desc(law="mhd", cells=(16, 16, 1), lower=(1, 0.2, 0), upper=(2, 2.9, 1), geometry="spherical")
+    assert lib.mara_block_create(ctypes.byref(d), ctypes.byref(h)) == abi.ERR_UNSUPPORTED
+    d = abi.make_desc(cells=(16, 1, 1), lower=(0, 0, 0), upper=(2, 3, 6), geometry="spherical", log_radial=True)
+    assert lib.mara_block_create(ctypes.byref(d), ctypes.byref(h)) == abi.ERR_INVALID_ARGUMENT
+    d = abi.make_desc(cells=(16, 16, 1), lower=(1, 0.2, 0), upper=(2, 2.9, 1), geometry="spherical")
+    dims = (ctypes.c_int * 3)(2, 2, 1)
+    assert lib.mara_group_create(ctypes.byref(d), dims, ctypes.byref(h)) == abi.ERR_UNSUPPORTED
+    buf = (ctypes.c_double * 8)()
+    assert lib.mara_advance_noalloc(ctypes.byref(d), buf, buf, ctypes.c_double(0.0), ctypes.c_double(0.1), None,
+                                    None) == abi.ERR_UNSUPPORTED
 
 
 def test_device_bytes_estimate():
