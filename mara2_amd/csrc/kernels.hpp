@@ -272,16 +272,34 @@ __device__ __forceinline__ void flux_face(const double* __restrict__ P, R* __res
     for (int q = 0; q < NQ; ++q) F[q * G.ffs + f] = Fq[q];
 }
 
+// Threads over the (m0, m1, m2) face box: 64 x rows work-groups with z from
+// the grid, or (linear != 0) a 1-D grid over the box's faces in x-fastest
+// order, which keeps every lane busy where m0 is not a multiple of 64 (the
+// flux-CT ring makes it 2 more than the interior).
 template <int Law, int Recon, int Riem, typename R>
 static __global__ void __launch_bounds__(256)
 k_flux(const double* __restrict__ P, R* __restrict__ F, const Geom G, int axis,
-       int lo0, int lo1, int lo2, int m0, int m1, int m2, Ctl* ctl)
+       int lo0, int lo1, int lo2, int m0, int m1, int m2, int linear, Ctl* ctl)
 {
     if (!ctl->active) return;
-    const int ti = blockIdx.x * blockDim.x + threadIdx.x;
-    const int tj = blockIdx.y * blockDim.y + threadIdx.y;
-    if (ti >= m0 || tj >= m1) return;
-    flux_face<Law, Recon, Riem, R>(P, F, G, axis, lo0 + ti, lo1 + tj, lo2 + (int)blockIdx.z, ctl);
+    int ti, tj, tk;
+    if (linear)
+    {
+        const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+        const unsigned row = t / (unsigned)m0;
+        ti = (int)(t - row * (unsigned)m0);
+        tj = (int)(row % (unsigned)m1);
+        tk = (int)(row / (unsigned)m1);
+        if (tk >= m2) return;
+    }
+    else
+    {
+        ti = blockIdx.x * blockDim.x + threadIdx.x;
+        tj = blockIdx.y * blockDim.y + threadIdx.y;
+        tk = (int)blockIdx.z;
+        if (ti >= m0 || tj >= m1) return;
+    }
+    flux_face<Law, Recon, Riem, R>(P, F, G, axis, lo0 + ti, lo1 + tj, lo2 + tk, ctl);
 }
 
 // ---------------------------------------------------------------------------

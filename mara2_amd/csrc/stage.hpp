@@ -66,8 +66,16 @@ void stage_flux_t(const StageArgs& s, int axis)
         }
     }
     const int m0 = hi[0] - lo[0], m1 = hi[1] - lo[1], m2 = hi[2] - lo[2];
-    k_flux<Law, Recon, Riem, R><<<stage_grid3(m0, m1, m2, s.rows), stage_block3(s.rows), 0, s.stream>>>(
-        s.Pin, (R*)s.F[axis], G, axis, lo[0], lo[1], lo[2], m0, m1, m2, s.ctl);
+    // rows of 64 lanes leave (64 - m0 % 64) lanes of each row's last
+    // wavefront idle; past an eighth of the lanes the faces go in one line
+    const int padded = (m0 + 63) / 64 * 64;
+    const long long faces = (long long)m0 * m1 * m2;
+    if (8 * (padded - m0) > padded && faces < (1ll << 31))
+        k_flux<Law, Recon, Riem, R><<<(unsigned)((faces + 255) / 256), 256, 0, s.stream>>>(
+            s.Pin, (R*)s.F[axis], G, axis, lo[0], lo[1], lo[2], m0, m1, m2, 1, s.ctl);
+    else
+        k_flux<Law, Recon, Riem, R><<<stage_grid3(m0, m1, m2, s.rows), stage_block3(s.rows), 0, s.stream>>>(
+            s.Pin, (R*)s.F[axis], G, axis, lo[0], lo[1], lo[2], m0, m1, m2, 0, s.ctl);
 }
 
 template <int Law, typename R>
