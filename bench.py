@@ -424,6 +424,32 @@ def serving_leg(args, n_req, seq_sample, saturate=True, open_loop=True):
     return out
 
 
+def persistent_leg(args):
+    """The serving batch's 2-D hydro tiles as a work queue
+    (mara_batch_set_persistent: a fixed grid of work-groups per CU claiming
+    the tiles from per-XCD counters) against one work-group per tile.  Every
+    request at t = 0; the 2-D hydro requests of the stream alone (the tiles
+    the queue serves) and the whole stream."""
+    from mara2_amd import serving
+    reqs = serving.make_stream(args.requests, precision="mixed")
+    hydro2d = [r for r in reqs if r.params["dim"] == 2 and r.params["law"] == "hydro"]
+    warm = serving.make_stream(4, precision="mixed", resize=lambda dim, n: 32 if dim == 1 else 16)
+    serving.serve(warm, 1 << 28, max_solves=8, chunk=args.chunk)
+    budget = int(args.budget_gib * 2**30)
+    out = {"workload": f"saturated batches of the serving stream's first {args.requests} requests (mixed form)",
+           "unit": "solves/s"}
+    for name, subset in ((f"hydro_2d_{len(hydro2d)}", hydro2d), (f"stream_{len(reqs)}", reqs)):
+        out[name] = {}
+        for wpc in (0, 1, 2, 4):
+            for r in subset:
+                r.arrival = 0.0
+            wall, done = serving.serve(subset, budget, chunk=args.chunk, persistent=wpc)
+            key = "one_work_group_per_tile" if wpc == 0 else f"queue_{wpc}_per_cu"
+            out[name][key] = {"solves_per_s": len(done) / wall, "wall_s": wall}
+            log(f"persistent {name} {key}: {len(done) / wall:.2f} solves/s ({wall:.2f} s)")
+    return out
+
+
 def copy_workload(args):
     """A plain device-to-device copy of known size: 1 GiB read and 1 GiB
     written per launch (torch's elementwise copy kernel).  Under rocprofv3
@@ -517,7 +543,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="all",
-                    choices=["all", "turbulence", "mhd2d", "rmhd3d", "sod", "serving", "copy"])
+                    choices=["all", "turbulence", "mhd2d", "rmhd3d", "sod", "serving", "copy", "persistent"])
     ap.add_argument("--requests", type=int, default=1024, help="serving: stream length")
     ap.add_argument("--seq-sample", type=int, default=64, help="serving: requests timed one at a time")
     ap.add_argument("--budget-gib", type=float, default=64.0, help="serving: HBM pool")
@@ -534,8 +560,8 @@ def main():
         return copy_workload(args)
     base = {"metric": METRIC, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None}
-    if args.workload in ("sod", "mhd2d", "rmhd3d", "serving"):
-        leg = {"sod": sod_leg, "mhd2d": mhd2d_leg, "rmhd3d": rmhd_leg,
+    if args.workload in ("sod", "mhd2d", "rmhd3d", "serving", "persistent"):
+        leg = {"sod": sod_leg, "mhd2d": mhd2d_leg, "rmhd3d": rmhd_leg, "persistent": persistent_leg,
                "serving": lambda a: serving_leg(a, a.requests, a.seq_sample)}[args.workload](args)
         out = dict(base)
         out.update(leg)

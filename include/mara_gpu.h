@@ -356,6 +356,15 @@ int mara_batch_step(mara_batch* batch, int n_steps, int* n_active);
    the same either way.  No counterpart in the reference. */
 int mara_batch_set_graphs(mara_batch* batch, int enable);
 
+/* With workgroups_per_cu > 0 (at most 16) the 2-D and 3-D hydro + PLM
+   tiles of a stage run as a work queue: a fixed grid of workgroups_per_cu
+   work-groups per CU that claim the same tiles in the same per-XCD order
+   from atomic counters, instead of one work-group per tile.  Results are
+   unchanged; 0 (the default) launches one work-group per tile.  No
+   counterpart in the reference (a persistent-kernel experiment for the
+   serving batch). */
+int mara_batch_set_persistent(mara_batch* batch, int workgroups_per_cu);
+
 /* The most solves the express lane may hold (default 0: every 2-D and 3-D
    solve stays on the tile lane).  Solves already there stay. */
 int mara_batch_set_express(mara_batch* batch, int max_solves);

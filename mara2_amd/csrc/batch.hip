@@ -296,6 +296,7 @@ int mara_batch_create(long long hbm_budget, int rk_order, int precision, int max
     if (!hip_ok(hipMalloc(&B->d_ctl, max_solves * sizeof(Ctl)))) return fail(MARA_ERR_HIP);
     if (!hip_ok(hipMalloc(&B->d_bins, (size_t)max_solves * CBINS * CBIN_STRIDE * sizeof(unsigned long long)))) return fail(MARA_ERR_HIP);
     if (!hip_ok(hipHostMalloc((void**)&B->h_ctl, max_solves * sizeof(Ctl), hipHostMallocDefault))) return fail(MARA_ERR_HIP);
+    if (!hip_ok(hipMalloc(&B->d_queue, 2 * NUM_XCD * sizeof(unsigned)))) return fail(MARA_ERR_HIP);
     std::memset(B->h_ctl, 0, max_solves * sizeof(Ctl));
     B->items.resize(max_solves);
     B->slots.resize(max_solves);
@@ -314,6 +315,7 @@ int mara_batch_destroy(mara_batch* B)
     (void)hipFree(B->d_ctl);
     (void)hipFree(B->d_bins);
     (void)hipFree(B->d_tiles);
+    (void)hipFree(B->d_queue);
     if (B->round_exec) (void)hipGraphExecDestroy(B->round_exec);
     if (B->h_ctl) (void)hipHostFree(B->h_ctl);
     if (B->stream) (void)hipStreamDestroy(B->stream);
@@ -323,6 +325,19 @@ int mara_batch_destroy(mara_batch* B)
     if (B->join) (void)hipEventDestroy(B->join);
     if (B->join3) (void)hipEventDestroy(B->join3);
     delete B;
+    return MARA_OK;
+}
+
+int mara_batch_set_persistent(mara_batch* B, int workgroups_per_cu)
+{
+    if (!B || workgroups_per_cu < 0 || workgroups_per_cu > 16) return MARA_ERR_INVALID_ARGUMENT;
+    if (B->persistent != workgroups_per_cu && B->round_exec)
+    {
+        // the captured round holds the other launches
+        (void)hipGraphExecDestroy(B->round_exec);
+        B->round_exec = nullptr;
+    }
+    B->persistent = workgroups_per_cu;
     return MARA_OK;
 }
 

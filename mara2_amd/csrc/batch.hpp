@@ -138,6 +138,74 @@ kb_fast_plm(const BItem* __restrict__ items, const BTile* __restrict__ tiles, do
     fast64::fast_plm_tile<float, Riem, Z>(Pin, it.Pa, Pout, G, b, first, courant_out, it.ctl, T.x0, T.y0, T.z0, kz);
 }
 
+// ---------------------------------------------------------------------------
+// Work-queue form of the fused hydro tile kernels (mara_batch_set_persistent):
+// a fixed grid of work-groups that loop over the class's tiles instead of
+// one work-group per tile.  Work-group b serves XCD x = b % NUM_XCD and takes,
+// in order, the tiles xcd_tile gives that XCD (the same tiles and order as
+// the one-per-tile launch), numbering them from the XCD's own counter
+// q[x]; it returns once the counter passes the XCD's share, which every
+// work-group reaches because each claim raises the counter by one.  The
+// counters start at zero (kb_queue_reset before each launch, on the same
+// stream).
+// ---------------------------------------------------------------------------
+static __global__ void __launch_bounds__(64) kb_queue_reset(unsigned* __restrict__ q)
+{
+    if (threadIdx.x < NUM_XCD) q[threadIdx.x] = 0u;
+}
+
+// The next tile of this work-group's XCD share, or -1 when the share is
+// done; every thread of the work-group calls it.
+__device__ __forceinline__ int queue_next_tile(unsigned* __restrict__ q, int count, int* claim)
+{
+    const int x = (int)(blockIdx.x % NUM_XCD);
+    const int per = count / NUM_XCD, extra = count % NUM_XCD;
+    const int size = per + (x < extra ? 1 : 0);
+    const int start = x * per + (x < extra ? x : extra);
+    if (threadIdx.x == 0 && threadIdx.y == 0) *claim = (int)atomicAdd(&q[x], 1u);
+    __syncthreads();
+    const int l = *claim;
+    __syncthreads();  // every thread has read the claim before the next one overwrites it
+    return l < size ? start + l : -1;
+}
+
+template <int Riem, typename R, bool Z>
+static __global__ void __launch_bounds__(FUSED_TX * PlmTile<R>::TY) __attribute__((amdgpu_waves_per_eu(PlmTile<R>::WAVES, 8)))
+kbq_fused_plm(const BItem* __restrict__ items, const BTile* __restrict__ tiles, int count, unsigned* __restrict__ q,
+              double b, int first, int last, int courant_out, int kz)
+{
+    __shared__ int claim;
+    for (int t = queue_next_tile(q, count, &claim); t >= 0; t = queue_next_tile(q, count, &claim))
+    {
+        const BTile T = tiles[t];
+        const BItem& it = items[T.item];
+        if (!it.ctl->active) continue;
+        const double* Pin = first ? it.Pa : it.Pb;
+        double* Pout = last ? it.Pa : it.Pb;
+        fused_plm_tile<Riem, R, Z>(Pin, it.Pa, it.U, Pout, it.G, b, first, !last, courant_out, it.ctl, T.x0, T.y0,
+                                   T.z0, kz);
+    }
+}
+
+template <int Riem, bool Z>
+static __global__ void __launch_bounds__(fast64::FastTile<float>::TX * fast64::FastTile<float>::TY)
+    __attribute__((amdgpu_waves_per_eu(fast64::FastTile<float>::WAVES, 8)))
+kbq_fast_plm(const BItem* __restrict__ items, const BTile* __restrict__ tiles, int count, unsigned* __restrict__ q,
+             double b, int first, int last, int courant_out, int kz)
+{
+    __shared__ int claim;
+    for (int t = queue_next_tile(q, count, &claim); t >= 0; t = queue_next_tile(q, count, &claim))
+    {
+        const BTile T = tiles[t];
+        const BItem& it = items[T.item];
+        if (!it.ctl->active) continue;
+        const double* Pin = first ? it.Pa : it.Pb;
+        double* Pout = last ? it.Pa : it.Pb;
+        fast64::fast_plm_tile<float, Riem, Z>(Pin, it.Pa, Pout, it.G, b, first, courant_out, it.ctl, T.x0, T.y0,
+                                              T.z0, kz);
+    }
+}
+
 // 2-D MHD items of an RK2 batch: the single block's fused MHD tiles (fluxes,
 // flux-CT and update in one pass, ghost images included), tile (x0, y0).
 template <int Recon, typename R, bool CT, int TY>

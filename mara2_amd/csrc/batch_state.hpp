@@ -111,6 +111,8 @@ struct mara_batch
     int kz = mara::FUSED_KZ;            // z planes per fused tile of the 3-D hydro solves
     int rows = mara::MHD_ROWS_DEFAULT;  // 2-D MHD tiles of 64 x (4 * rows) zones
     int express_max = 0;                // solves the express lane may hold (0: none)
+    int persistent = 0;                 // fused hydro tiles from a work queue: work-groups per CU (0: off)
+    unsigned* d_queue = nullptr;        // the queue's counters, one set of NUM_XCD per tile lane
     hipStream_t stream = nullptr;   // tile kernels, transfers, control
     hipStream_t stream2 = nullptr;  // pencil kernels of 1-D solves
     hipStream_t stream3 = nullptr;  // express lane: the longest chains of 2-D/3-D solves
@@ -206,6 +208,21 @@ void fused_launch(mara_batch* B, const BTile* t, int count, double b, int first,
 {
     // the mixed form runs the single block's tiles of the fast forms (the
     // same 64 x 8 tile shape as PlmTile<float>), the binary64 form the plain ones
+    if (B->persistent)
+    {
+        // a fixed grid drawing the same tiles from per-XCD counters
+        unsigned* q = B->d_queue + (B->cur == B->stream3 ? NUM_XCD : 0);
+        const int wgs = 256 * B->persistent;
+        const int grid = std::max(NUM_XCD, std::min(count + NUM_XCD - 1, wgs) / NUM_XCD * NUM_XCD);
+        kb_queue_reset<<<1, 64, 0, B->cur>>>(q);
+        if constexpr (sizeof(R) == 4)
+            kbq_fast_plm<Riem, Z><<<grid, dim3(64, fast64::FastTile<float>::TY), 0, B->cur>>>(
+                B->d_items, t, count, q, b, first, last, cout, B->kz);
+        else
+            kbq_fused_plm<Riem, R, Z><<<grid, dim3(FUSED_TX, PlmTile<R>::TY), 0, B->cur>>>(
+                B->d_items, t, count, q, b, first, last, cout, B->kz);
+        return;
+    }
     if constexpr (sizeof(R) == 4)
     {
         static_assert(fast64::FastTile<float>::TY == PlmTile<float>::TY, "the batch's tile table assumes one height");

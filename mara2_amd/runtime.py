@@ -79,6 +79,7 @@ def load():
     L.mara_batch_get_shape.argtypes = [bat, ctypes.POINTER(MaraShape)]
     L.mara_batch_lane.argtypes = [bat, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     L.mara_batch_set_express.argtypes = [bat, ctypes.c_int]
+    L.mara_batch_set_persistent.argtypes = [bat, ctypes.c_int]
     L.mara_batch_set_shape.argtypes = [bat, ctypes.POINTER(MaraShape)]
     L.mara_batch_destroy.argtypes = [bat]
     L.mara_batch_item_bytes.argtypes = [ctypes.POINTER(MaraDesc), ctypes.c_int]
@@ -316,6 +317,11 @@ class Batch:
     def set_express(self, max_solves):
         """The most long-chain solves the express lane may hold (0: none)."""
         _check(self.lib.mara_batch_set_express(self._h, max_solves), "mara_batch_set_express")
+
+    def set_persistent(self, workgroups_per_cu):
+        """Run the 2-D/3-D hydro + PLM tiles as a work queue over a fixed grid
+        of workgroups_per_cu work-groups per CU (0: one work-group per tile)."""
+        _check(self.lib.mara_batch_set_persistent(self._h, int(workgroups_per_cu)), "mara_batch_set_persistent")
 
     def lane(self, slot):
         """0 tile lane, 1 pencil lane, 2 express lane."""

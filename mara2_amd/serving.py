@@ -111,14 +111,15 @@ def sequential(requests, keep_result=False, graphs=True):
 
 def serve(requests, hbm_budget, max_solves=1024, chunk=16, rk_order=2, precision="mixed",
           keep_results=False, log=None, clock=time.perf_counter, graphs=True, shape=None, express=0,
-          batch_factory=Batch):
+          batch_factory=Batch, persistent=0):
     """Continuous batching over a request stream with arrival times set;
     shape (optional) is the batch's launch shape.  Returns (wall seconds,
     list of requests with admitted/finished stamps).  A request the pool
     cannot take even when it is empty (larger than the HBM budget) is not
     waited on: it finishes at once with status MARA_ERR_CAPACITY.
     batch_factory(hbm_budget, rk_order=, precision=, max_solves=) makes the
-    batch (the device Batch by default)."""
+    batch (the device Batch by default); persistent > 0 runs the batch's
+    hydro tiles as a work queue of that many work-groups per CU."""
     pending = deque(sorted(requests, key=lambda r: r.arrival))
     waiting = []
     in_flight = {}
@@ -126,6 +127,8 @@ def serve(requests, hbm_budget, max_solves=1024, chunk=16, rk_order=2, precision
     with batch_factory(hbm_budget, rk_order=rk_order, precision=precision, max_solves=max_solves) as batch:
         batch.set_graphs(graphs)
         batch.set_express(express)
+        if persistent:
+            batch.set_persistent(persistent)
         if shape is not None:
             batch.set_shape(shape)
         t0 = clock()

@@ -329,3 +329,35 @@ def test_rk3_batch_uses_flux_arrays_and_matches_block():
     for s, (d, r) in slots.items():
         Pw, _, _ = _alone_vs_block(d, r.P, 30)
         assert np.array_equal(got[s], Pw), r.index
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("wpc", [1, 3])
+def test_work_queue_tiles_equal_one_tile_per_work_group(fp64_stream, fp64_mix, precision, wpc):
+    """mara_batch_set_persistent: the 2-D and 3-D hydro tiles drawn from a
+    work queue by a fixed grid give the results of one work-group per tile,
+    bit for bit, in a mix with 1-D, MHD and a 3-D solve, with graphs and
+    host launches."""
+    reqs = fp64_stream if precision == "fp64" else _stream("mixed")
+    desc3 = problems.turbulence_desc(24, precision)
+    P3 = problems.turbulence_3d(problems.turbulence_desc(24, "fp64"), kmax=4)
+
+    def run(persistent, graphs):
+        out = {}
+        with Batch(BUDGET, rk_order=2, precision=precision, max_solves=64) as b:
+            b.set_graphs(graphs)
+            b.set_persistent(persistent)
+            slots = {b.admit(r.desc, r.P, serving.CFL, serving.T_FINAL, serving.MAX_STEPS): r.index for r in reqs}
+            slots[b.admit(desc3, P3, 0.3, 1e300, 8)] = "3d"
+            while slots:
+                b.step(7)
+                for s in b.finished():
+                    out[slots.pop(s)] = b.retire(s)
+        return out
+
+    ref = run(0, True)
+    for graphs in (True, False):
+        got = run(wpc, graphs)
+        assert set(got) == set(ref)
+        for k in ref:
+            assert np.array_equal(got[k], ref[k]), (k, graphs)
