@@ -207,11 +207,12 @@ int mara_block_set_source_aux(mara_block* block, const double* t);
    MARA_VISCOUS_FLUX_KERNEL), run on every stage after the Godunov fluxes and
    before the divergence, with up to MARA_SOURCE_MAX_PARAMS parameters and the
    auxiliary data of mara_block_set_source_aux.  The block takes the
-   flux-array path while it is set.  A null code_object removes it.
-   MARA_ERR_UNSUPPORTED in the mixed form (binary32 flux arrays) and for a
-   magnetic law with flux-CT (the reference applies it to the CT-corrected
-   fluxes, which this path forms inside the zone update);
-   MARA_ERR_INVALID_ARGUMENT for a file or kernel that cannot be loaded. */
+   flux-array path while it is set.  A null code_object removes it.  With
+   flux-CT the function sees the CT-corrected fluxes, as the reference's
+   fluxCorrection hands them over; in the mixed form it sees binary64 copies
+   of the binary32 fluxes, and the update reads their binary32 rounding.
+   MARA_ERR_UNSUPPORTED on a spherical mesh; MARA_ERR_INVALID_ARGUMENT for a
+   file or kernel that cannot be loaded. */
 int mara_block_set_viscous_flux(mara_block* block, const char* code_object, const char* kernel,
                                 const double* params, int n_params);
 

@@ -94,6 +94,7 @@ struct mara_block
     double* Pc = nullptr;  // third stage buffer of the fused RK3 schedule
     double* U = nullptr;
     void* F[3] = {nullptr, nullptr, nullptr};
+    void* Fv[3] = {nullptr, nullptr, nullptr};  // binary64 flux staging of a viscous hook (mixed form or flux-CT)
     int pencil = 0;  // 1-D pencil tile length (pencil_supported), else 0
     bool mhd2d = false;  // fused 2-D MHD tiles (mhd2d_supported)
     int kz = FUSED_KZ;           // fused 3-D kernels: z planes per work-group
@@ -275,6 +276,7 @@ void launch_stage(mara_block* b, int rk, int courant_out)
     s.S = b->src_fn || cooling ? b->Ssrc : nullptr;
     b->vis_args.P = Pin;
     s.after_flux = b->vis_fn ? launch_viscous : nullptr;
+    for (int a = 0; a < 3; ++a) s.Fv[a] = b->vis_fn ? b->Fv[a] : nullptr;
     s.after_flux_ctx = b;
     run_stage_kernels(s, mixed);
     if (timed)
@@ -517,6 +519,7 @@ int mara_block_destroy(mara_block* b)
     hipFree(b->Pc);
     hipFree(b->U);
     for (int a = 0; a < 3; ++a) hipFree(b->F[a]);
+    for (int a = 0; a < 3; ++a) hipFree(b->Fv[a]);
     hipFree(b->ctl);
     hipFree(b->cbins);
     if (b->steps_exec) (void)hipGraphExecDestroy(b->steps_exec);
@@ -927,7 +930,7 @@ int mara_block_set_viscous_flux(mara_block* b, const char* code_object, const ch
     if (!b || n_params < 0 || n_params > MARA_SOURCE_MAX_PARAMS || (n_params > 0 && !params)
         || (code_object && !kernel))
         return MARA_ERR_INVALID_ARGUMENT;
-    if (code_object && (b->d.precision == MARA_PREC_MIXED || b->S.ct || b->sph)) return MARA_ERR_UNSUPPORTED;
+    if (code_object && b->sph) return MARA_ERR_UNSUPPORTED;
     HIP_TRY(hipStreamSynchronize(b->stream));
     if (b->vis_module) (void)hipModuleUnload(b->vis_module);
     b->vis_module = nullptr;
@@ -958,13 +961,25 @@ int mara_block_set_viscous_flux(mara_block* b, const char* code_object, const ch
         b->vis_fn = nullptr;
         return MARA_ERR_INVALID_ARGUMENT;
     }
-    // the flux-array path: flux arrays of every axis with zones
+    // the flux-array path: flux arrays of every axis with zones, and a
+    // binary64 staging copy where the arrays are binary32 or flux-CT must be
+    // applied before the hook
+    const bool staged = b->d.precision == MARA_PREC_MIXED || b->S.ct;
     for (int a = 0; a < 3; ++a)
     {
-        if (!b->S.fleshed[a] || b->F[a]) continue;
-        const size_t fb = (size_t)b->S.nq * b->S.ffs * b->S.real;
-        HIP_TRY(hipMalloc(&b->F[a], fb));
-        b->bytes += (long long)fb;
+        if (!b->S.fleshed[a]) continue;
+        if (!b->F[a])
+        {
+            const size_t fb = (size_t)b->S.nq * b->S.ffs * b->S.real;
+            HIP_TRY(hipMalloc(&b->F[a], fb));
+            b->bytes += (long long)fb;
+        }
+        if (staged && !b->Fv[a])
+        {
+            const size_t fb = (size_t)b->S.nq * b->S.ffs * sizeof(double);
+            HIP_TRY(hipMalloc(&b->Fv[a], fb));
+            b->bytes += (long long)fb;
+        }
     }
     b->fused = false;
     b->pencil = 0;
@@ -976,7 +991,7 @@ int mara_block_set_viscous_flux(mara_block* b, const char* code_object, const ch
     b->vis_faces = 0;
     for (int e = 0; e < 3; ++e)
     {
-        a.F[e] = b->S.fleshed[e] ? static_cast<double*>(b->F[e]) : nullptr;
+        a.F[e] = !b->S.fleshed[e] ? nullptr : static_cast<double*>(staged ? b->Fv[e] : b->F[e]);
         a.N[e] = b->G.N[e];
         a.g[e] = b->G.g[e];
         a.n[e] = b->G.n[e];

@@ -338,6 +338,52 @@ __device__ __forceinline__ double ct_value(const R* __restrict__ Fx, const R* __
 }
 
 // ---------------------------------------------------------------------------
+// Staging for a viscous-flux hook (MethodOfLinesTVD's fluxCorrection: flux-CT,
+// then the viscous function, src/SolutionSchemes.cpp:173-187).  Over the
+// faces the zone update reads along `axis` (faces g .. g + n of the axis on
+// the interior rows of the others, the hook's own face set), the binary64
+// copy Fv of the axis' fluxes, with the B fluxes flux-CT-corrected when ct
+// (G_a^a = 0, G_a^c from ct_value), so the hook sees the fluxes the
+// reference hands it; k_visc_unstage rounds them back into binary32 arrays.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool visc_face(const Geom& G, int axis, long long t, int& i, int& j, int& k)
+{
+    const int m0 = G.n[0] + (axis == 0), m1 = G.n[1] + (axis == 1), m2 = G.n[2] + (axis == 2);
+    if (t >= (long long)m0 * m1 * m2) return false;
+    i = G.g[0] + (int)(t % m0);
+    j = G.g[1] + (int)((t / m0) % m1);
+    k = G.g[2] + (int)(t / ((long long)m0 * m1));
+    return true;
+}
+
+template <typename R>
+static __global__ void __launch_bounds__(256)
+k_visc_stage(const R* __restrict__ Fx, const R* __restrict__ Fy, const R* __restrict__ Fz, double* __restrict__ Fv,
+             Geom G, int axis, int ct)
+{
+    int i, j, k;
+    if (!visc_face(G, axis, (long long)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    const long long f = face_index(G, i, j, k);
+    const R* Fa = axis == 0 ? Fx : axis == 1 ? Fy : Fz;
+    for (int q = 0; q < G.nq; ++q)
+    {
+        double v = double(Fa[q * G.ffs + f]);
+        if (ct && q >= 5 && q < 8) v = q - 5 == axis ? 0.0 : ct_value(Fx, Fy, Fz, G, axis, q - 5, f);
+        Fv[q * G.ffs + f] = v;
+    }
+}
+
+template <typename R>
+static __global__ void __launch_bounds__(256)
+k_visc_unstage(const double* __restrict__ Fv, R* __restrict__ F, Geom G, int axis)
+{
+    int i, j, k;
+    if (!visc_face(G, axis, (long long)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    const long long f = face_index(G, i, j, k);
+    for (int q = 0; q < G.nq; ++q) F[q * G.ffs + f] = R(Fv[q * G.ffs + f]);
+}
+
+// ---------------------------------------------------------------------------
 // CFL time step of one zone (FieldOperator::getCourantTimestep, src/FieldOperator.cpp:125-169)
 // ---------------------------------------------------------------------------
 // R is the arithmetic of the wave speeds: binary64 for the FP64 form (the

@@ -38,6 +38,12 @@ struct StageArgs
     // flux-array path: called between the flux sweeps and the zone update
     // (the viscous-flux hook enqueues its kernel there), or null
     void (*after_flux)(void*);
+    // binary64 staging arrays of the fluxes for the hook where the flux
+    // arrays are binary32 or flux-CT must come first (null otherwise): the
+    // hook then sees and updates these (k_visc_stage), and the zone update
+    // reads them (binary64) or their binary32 rounding (k_visc_unstage)
+    // without applying flux-CT again
+    void* Fv[3] = {nullptr, nullptr, nullptr};
     void* after_flux_ctx;
 };
 
@@ -210,13 +216,43 @@ void stage_kernels(const StageArgs& s)
             default: stage_flux_law<2, R>(s, axis); break;
         }
     }
+    const bool staged = s.after_flux && (s.Fv[0] || s.Fv[1] || s.Fv[2]);
+    if (staged)
+    {
+        for (int axis = 0; axis < 3; ++axis)
+            if (s.Fv[axis])
+            {
+                const long long faces = (long long)(s.G->n[0] + (axis == 0)) * (s.G->n[1] + (axis == 1))
+                                        * (s.G->n[2] + (axis == 2));
+                k_visc_stage<R><<<(unsigned)((faces + 255) / 256), 256, 0, s.stream>>>(
+                    (const R*)s.F[0], (const R*)s.F[1], (const R*)s.F[2], (double*)s.Fv[axis], *s.G, axis, s.ct);
+            }
+    }
     if (s.after_flux) s.after_flux(s.after_flux_ctx);
+    StageArgs u = s;
+    if (staged)
+    {
+        // flux-CT is in the staged fluxes already
+        u.ct = false;
+        for (int axis = 0; axis < 3; ++axis)
+        {
+            if (!s.Fv[axis]) continue;
+            if constexpr (sizeof(R) == 8) u.F[axis] = s.Fv[axis];
+            else
+            {
+                const long long faces = (long long)(s.G->n[0] + (axis == 0)) * (s.G->n[1] + (axis == 1))
+                                        * (s.G->n[2] + (axis == 2));
+                k_visc_unstage<R><<<(unsigned)((faces + 255) / 256), 256, 0, s.stream>>>(
+                    (const double*)s.Fv[axis], (R*)s.F[axis], *s.G, axis);
+            }
+        }
+    }
     switch (law)
     {
-        case 0: stage_update_law<0, R>(s); break;
-        case 1: stage_update_law<1, R>(s); break;
-        case 3: stage_update_law<3, R>(s); break;
-        default: stage_update_law<2, R>(s); break;
+        case 0: stage_update_law<0, R>(u); break;
+        case 1: stage_update_law<1, R>(u); break;
+        case 3: stage_update_law<3, R>(u); break;
+        default: stage_update_law<2, R>(u); break;
     }
 }
 

@@ -3,9 +3,11 @@ gfx950 code object (mara2_amd/sources/viscous_constant.hsaco, written with
 include/mara_source.h) against the same function in the oracle
 (oracle_advance_visc), bit for bit in FP64.  A block with the hook takes the
 flux-array path whatever its own kernels are (fused 2-D/3-D hydro, 1-D
-pencils, fused 2-D MHD without CT); the hook is refused where the reference
-would apply it to CT-corrected fluxes and in the mixed form, and removing it
-restores the block's own kernels."""
+pencils, fused 2-D MHD); with flux-CT the hook sees the CT-corrected fluxes,
+as the reference's fluxCorrection hands them over (binary64 staging,
+k_visc_stage); in the mixed form it sees binary64 copies of the binary32
+fluxes and the update reads their binary32 rounding (held to 1e-5 of the
+oracle).  Removing it restores the block's own kernels."""
 import os
 
 import numpy as np
@@ -39,6 +41,10 @@ CASES = {
     "hydro2d_weno_rk3": dict(law="hydro", recon="weno5", riemann="hlle", bc="reflecting", cells=(40, 24, 1),
                              rk_order=3),
     "mhd2d_noct_to_flux": dict(law="mhd", recon="plm", riemann="hlle", bc="periodic", cells=(70, 37, 1), field_ct=0),
+    "mhd2d_ct_to_flux": dict(law="mhd", recon="plm", riemann="hlle", bc="periodic", cells=(70, 37, 1), field_ct=1),
+    "mhd3d_ct": dict(law="mhd", recon="plm", riemann="hlle", bc="outflow", cells=(20, 12, 10), field_ct=1),
+    "mhd2d_ct_cooled": dict(law="mhd", recon="plm", riemann="hlle", bc="periodic", cells=(36, 28, 1), field_ct=1,
+                            cooling_rate=0.3),
 }
 
 
@@ -113,13 +119,41 @@ def test_zero_viscosity_gives_the_plain_flux_array_step():
     assert np.array_equal(out[0], out[1])
 
 
+@pytest.mark.parametrize("case", ["hydro2d_fused_to_flux", "hydro3d_fused_to_flux", "mhd2d_ct_to_flux"])
+def test_mixed_form_with_viscosity_close_to_oracle(case):
+    """The mixed form with the hook: binary32 fluxes, the hook on their
+    binary64 copies (CT-corrected under flux-CT), within the mixed form's 1e-5
+    of the binary64 oracle."""
+    import oracle
+    from helpers import interior, prim_to_cons, rel_l1
+    desc = abi.make_desc(**CASES[case])
+    dmix = abi.make_desc(**CASES[case], precision="mixed")
+    P0 = _state(desc, 73)
+    nu = 0.01
+    Po = P0.copy()
+    with Block(dmix) as blk:
+        blk.set_viscous_flux(*VISC, (nu,))
+        blk.upload(P0)
+        for _ in range(4):
+            err, do = oracle.courant_dt(desc, Po)
+            assert err == 0
+            blk.advance(0.3 * do)
+            err, _ = oracle.advance(desc, Po, 0.0, 0.3 * do, None, ("constant", [nu]))
+            assert err == 0
+        Pg = blk.download()
+    err = rel_l1(prim_to_cons(desc, interior(desc, Pg)), prim_to_cons(desc, interior(desc, Po)))
+    assert err <= 1e-5, err
+    # the hook took effect: without it the state differs by more than that
+    Pn = P0.copy()
+    for _ in range(4):
+        err, do = oracle.courant_dt(desc, Pn)
+        oracle.advance(desc, Pn, 0.0, 0.3 * do)
+    assert rel_l1(prim_to_cons(desc, interior(desc, Pn)), prim_to_cons(desc, interior(desc, Po))) > 1e-4
+
+
 def test_refused_configurations():
-    with Block(abi.make_desc(law="mhd", recon="plm", riemann="hlle", bc="periodic", cells=(32, 32, 1),
-                             field_ct=1)) as blk:
-        with pytest.raises(runtime.MaraError):
-            blk.set_viscous_flux(*VISC, (0.01,))
-    with Block(abi.make_desc(law="hydro", recon="plm", riemann="hllc", bc="periodic", cells=(32, 32, 1),
-                             precision="mixed")) as blk:
+    with Block(abi.make_desc(law="hydro", cells=(16, 8, 1), lower=(1, 0.4, 0), upper=(2, 2.7, 1),
+                             geometry="spherical")) as blk:
         with pytest.raises(runtime.MaraError):
             blk.set_viscous_flux(*VISC, (0.01,))
     desc = abi.make_desc(**CASES["hydro1d_pencil_to_flux"])
