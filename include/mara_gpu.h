@@ -95,7 +95,7 @@ typedef struct mara_desc
     int    field_ct;       /* 1 unless MethodOfLinesTVD::setDisableFieldCT(true) */
     int    precision;      /* MARA_PREC_* */
     double cooling_rate;   /* NewtonianMHD::setCoolingRate; > 0 (MARA_LAW_MHD
-                              only; not in batches) adds the auxiliary
+                              only) adds the auxiliary
                               ninth field and the cooling source
                               (src/ConservationLaws.cpp:354-358,489-501);
                               <= 0 (the reference default) is off */

@@ -25,12 +25,15 @@ int class_of(mara_batch* B, const mara_desc& d, bool express = false)
 {
     const bool one_d = d.cells[1] == 1 && d.cells[2] == 1;
     const bool rk2 = B->rk_order == 2;
+    // the kernel law: 3 for NewtonianMHD with cooling, which takes the
+    // flux-array kernels (nine fields and the cooling terms)
+    const int law = law_code(d);
     // fused tiles in RK2 batches: hydro + PLM in 2-D (1) and 3-D (2), MHD in 2-D (3)
     int fused = 0;
-    if (!one_d && rk2 && d.law == MARA_LAW_HYDRO && d.recon == MARA_RECON_PLM) fused = d.cells[2] > 1 ? 2 : 1;
-    else if (!one_d && rk2 && d.law == MARA_LAW_MHD && d.cells[2] == 1) fused = 3;
-    const int pencil = (one_d && rk2) ? 1 : 0;
-    const ClassKey k{d.law, d.recon, d.riemann, (d.law != MARA_LAW_HYDRO && d.field_ct) ? 1 : 0,
+    if (!one_d && rk2 && law == MARA_LAW_HYDRO && d.recon == MARA_RECON_PLM) fused = d.cells[2] > 1 ? 2 : 1;
+    else if (!one_d && rk2 && law == MARA_LAW_MHD && d.cells[2] == 1) fused = 3;
+    const int pencil = (one_d && rk2 && law != LAW_MHD_COOLING) ? 1 : 0;
+    const ClassKey k{law, d.recon, d.riemann, (d.law != MARA_LAW_HYDRO && d.field_ct) ? 1 : 0,
                      pencil, fused, pencil ? 1 : (express ? 2 : 0)};
     for (size_t c = 0; c < B->classes.size(); ++c)
         if (!(B->classes[c] < k) && !(k < B->classes[c])) return (int)c;
@@ -42,12 +45,13 @@ long long item_bytes(const mara_desc& d, int precision)
 {
     int N[3];
     for (int a = 0; a < 3; ++a) N[a] = d.cells[a] + 2 * d.guards[a];
-    const int nq = law_nq(d.law);
+    const int nq = law_nq(law_code(d));
     const long long fs = round_up((long long)N[0] * N[1] * N[2], 64);
     const long long ffs = round_up((long long)(N[0] + 1) * (N[1] + 1) * (N[2] + 1), 64);
     const int real = precision == MARA_PREC_MIXED ? 4 : 8;
     long long b = 3 * round_up(nq * fs * 8, 256);
     for (int a = 0; a < 3; ++a) if (N[a] > 1) b += round_up(nq * ffs * real, 256);
+    if (nq == 9) b += round_up(nq * fs * 8, 256);  // the cooling terms
     return b;
 }
 
@@ -261,7 +265,6 @@ int validate_item(mara_batch* B, const mara_desc* d)
     const int v = validate(d);
     if (v) return v;
     if (d->rk_order != B->rk_order || d->precision != B->precision) return MARA_ERR_UNSUPPORTED;
-    if (law_code(*d) == LAW_MHD_COOLING) return MARA_ERR_UNSUPPORTED;  // single blocks only
     if (d->geometry != MARA_GEOM_CARTESIAN) return MARA_ERR_UNSUPPORTED;
     return MARA_OK;
 }
@@ -426,6 +429,14 @@ int mara_batch_admit(mara_batch* B, const mara_desc* d, const double* P, double 
     const int real = B->precision == MARA_PREC_MIXED ? 4 : 8;
     for (int a = 0; a < 3; ++a)
         if (it.G.N[a] > 1) { it.F[a] = p; p += round_up(it.G.nq * it.G.ffs * real, 256); }
+    it.S = nullptr;
+    if (it.G.nq == 9)
+    {
+        // the cooling terms; fields other than the energy and the auxiliary
+        // one stay zero
+        it.S = (double*)p;
+        p += cb;
+    }
     it.ctl = B->d_ctl + n;
     it.bc = d->bc;
     it.lane = B->classes[sl.cls].lane;
@@ -464,6 +475,7 @@ int mara_batch_admit(mara_batch* B, const mara_desc* d, const double* P, double 
         return undo();
     k_aos_to_soa<<<(unsigned)((sl.ncell + 255) / 256), 256, 0, B->stream>>>(it.U, it.Pa, it.G);
     if (!hip_ok(hipMemcpyAsync(it.Pb, it.Pa, cb, hipMemcpyDeviceToDevice, B->stream))) return undo();
+    if (it.S && !hip_ok(hipMemsetAsync(it.S, 0, cb, B->stream))) return undo();
     if (!hip_ok(hipGetLastError())) return undo();
     // the host buffer may go away after return
     if (!hip_ok(hipStreamSynchronize(B->stream))) return undo();
@@ -493,7 +505,7 @@ int mara_batch_step(mara_batch* B, int n_steps, int* n_active)
         const Geom& G = B->items[n].G;
         int lo[3], hi[3];
         for (int e = 0; e < 3; ++e) { lo[e] = G.g[e]; hi[e] = G.g[e] + G.n[e]; }
-        box_tiles(fresh[sl.d.law], n, lo, hi);
+        box_tiles(fresh[sl.d.law], n, lo, hi);  // a cooled MHD solve's CFL is the MHD one
         sl.fresh = false;
     }
     std::vector<BTile> ftiles;

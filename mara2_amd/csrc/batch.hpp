@@ -30,6 +30,7 @@ struct BItem
     int bc;
     int lane;  // 0: tile kernels on the batch's first stream, 1: pencil kernels on the second
     int cfl;   // arithmetic of the CFL time step of the item's stage kernels (fast64::CflArith)
+    double* S; // NewtonianMHD cooling terms of the stage (nine-field items), else null
 };
 
 struct BTile
@@ -78,8 +79,26 @@ kb_update(const BItem* __restrict__ items, const BTile* __restrict__ tiles, doub
     double dtc = 1.79769313486231570815e+308;
     if (tile_thread(T, i, j, k))
         dtc = update_zone<Law, R, CT>(it.Pa, it.U, last ? it.Pa : it.Pb, (const R*)it.F[0], (const R*)it.F[1],
-                                      (const R*)it.F[2], it.G, b, first, !last, courant_out, it.ctl, i, j, k, nullptr);
+                                      (const R*)it.F[2], it.G, b, first, !last, courant_out, it.ctl, i, j, k, it.S);
     if (courant_out) courant_to_bins(dtc, wmin, 4, threadIdx.x, it.ctl);
+}
+
+// The cooling terms of a nine-field item from the stage's primitives, before
+// its flux sweeps (k_cooling_source per tile of the update's zones).
+static __global__ void __launch_bounds__(256)
+kb_cooling(const BItem* __restrict__ items, const BTile* __restrict__ tiles, int rk)
+{
+    const BTile T = tiles[blockIdx.x];
+    const BItem& it = items[T.item];
+    if (!it.ctl->active || !it.S) return;
+    int i, j, k;
+    if (!tile_thread(T, i, j, k)) return;
+    const double* Pin = rk == 0 ? it.Pa : it.Pb;
+    const long long c = cell_index(it.G, i, j, k);
+    const double gm1 = it.G.gamma - 1.0;
+    const double q = Pin[4 * it.G.fs + c] / gm1 * it.G.cool;
+    it.S[4 * it.G.fs + c] = -q;
+    it.S[8 * it.G.fs + c] = q;
 }
 
 // 1-D items of an RK2 batch: reconstruction, fluxes and update in one pass

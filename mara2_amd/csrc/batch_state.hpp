@@ -163,6 +163,7 @@ void flux_dispatch(mara_batch* B, const ClassKey& k, const Segment& s, int axis,
     FL(0, 0, 0) FL(0, 0, 1) FL(0, 1, 0) FL(0, 1, 1) FL(0, 2, 0) FL(0, 2, 1)
     FL(1, 0, 0) FL(1, 1, 0) FL(1, 2, 0)
     FL(2, 0, 0) FL(2, 1, 0) FL(2, 2, 0)
+    FL(3, 0, 0) FL(3, 1, 0) FL(3, 2, 0)
 #undef FL
 }
 
@@ -285,7 +286,7 @@ void update_dispatch(mara_batch* B, const ClassKey& k, const Segment& s, double 
     }
     const BTile* t = B->d_tiles + s.offset;
 #define UP(L, C) if (k.law == L && k.ct == C) { kb_update<L, R, (C != 0)><<<s.count, 256, 0, B->cur>>>(B->d_items, t, b, first, last, cout); return; }
-    UP(0, 0) UP(1, 0) UP(1, 1) UP(2, 0) UP(2, 1)
+    UP(0, 0) UP(1, 0) UP(1, 1) UP(2, 0) UP(2, 1) UP(3, 0) UP(3, 1)
 #undef UP
 }
 
@@ -299,6 +300,11 @@ void run_stage(mara_batch* B, int rk, int lane)
     static const double b3[3] = {1.0, 1. / 4, 2. / 3};
     const double* bb = B->rk_order == 1 ? b1 : B->rk_order == 2 ? b2 : b3;
     const int last = rk == B->rk_order - 1;
+    // NewtonianMHD cooling (ConservationLaw::addSourceTerms) of the stage's
+    // primitives, over the update's tiles of the nine-field classes
+    for (auto& e : B->update)
+        if (B->classes[e.first].lane == lane && B->classes[e.first].law == 3)
+            kb_cooling<<<e.second.count, 256, 0, B->cur>>>(B->d_items, B->d_tiles + e.second.offset, rk);
     for (int axis = 0; axis < 3; ++axis)
         for (auto& e : B->flux[axis])
             if (B->classes[e.first].lane == lane) flux_dispatch<R>(B, B->classes[e.first], e.second, axis, rk);

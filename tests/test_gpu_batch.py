@@ -361,3 +361,50 @@ def test_work_queue_tiles_equal_one_tile_per_work_group(fp64_stream, fp64_mix, p
         assert set(got) == set(ref)
         for k in ref:
             assert np.array_equal(got[k], ref[k]), (k, graphs)
+
+
+@pytest.mark.parametrize("rk_order", [2, 3])
+def test_cooled_mhd_solves_in_a_batch(rk_order):
+    """NewtonianMHD with a cooling rate (nine fields, the cooling terms added
+    to L in every stage) in a batch beside ordinary solves: bit-equal to the
+    single block in FP64, 1-D and 2-D, flux-CT on and off."""
+    def cooled(cells, ct, seed):
+        d = abi.make_desc(law="mhd", recon="plm", riemann="hlle", bc="periodic", cells=cells, field_ct=ct,
+                          cooling_rate=0.4, rk_order=rk_order)
+        rng = np.random.default_rng(seed)
+        P = problems.empty(d)
+        shp = P.shape[:3]
+        P[..., 0] = 1.0 + 0.3 * rng.random(shp)
+        P[..., 4] = 1.0 + 0.3 * rng.random(shp)
+        P[..., 1:4] = 0.2 * rng.standard_normal(shp + (3,))
+        P[..., 5:8] = 0.3 + 0.1 * rng.standard_normal(shp + (3,))
+        P[..., 8] = 0.1 * rng.random(shp)
+        return d, problems.apply_bc(d, P)
+
+    solves = [cooled((40, 1, 1), 1, 1), cooled((24, 20, 1), 1, 2), cooled((20, 18, 1), 0, 3)]
+    plain = serving.make_stream(3, precision="fp64", resize=SMALL)
+    plain = [(problems.riemann_desc(r.params, precision="fp64", rk_order=rk_order), r.P) for r in plain]
+    with Batch(BUDGET, rk_order=rk_order, precision="fp64") as b:
+        slots = {}
+        for n, (d, P) in enumerate(solves + plain):
+            if n < len(solves):
+                slots[b.admit(d, P, 0.3, 1e300, 6)] = n
+            else:
+                slots[b.admit(d, P, serving.CFL, serving.T_FINAL, serving.MAX_STEPS)] = n
+        got = {}
+        while len(got) < len(slots):
+            b.step(4)
+            for s in b.finished():
+                done, t, steps, st, rep = b.query(s)
+                assert st == 0, rep.as_dict()
+                got[slots[s]] = (b.retire(s), t, steps)
+    for n, (d, P) in enumerate(solves + plain):
+        with Block(d) as blk:
+            blk.upload(P)
+            if n < len(solves):
+                t, steps, _ = blk.run(0.3, 1e300, 6)
+            else:
+                t, steps, _ = blk.run(serving.CFL, serving.T_FINAL, serving.MAX_STEPS)
+            Pb = blk.download()
+        assert (t, steps) == got[n][1:], n
+        assert np.array_equal(Pb, got[n][0]), n

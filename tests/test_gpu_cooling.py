@@ -90,10 +90,25 @@ def test_mixed_cooling_close_to_the_fp64_oracle(case):
 def test_refused_configurations():
     with pytest.raises(runtime.MaraError):
         Block(abi.make_desc(law="hydro", cells=(32, 32, 1), cooling_rate=0.5))
+
+
+def test_batched_cooled_solve_matches_oracle():
+    """A cooled solve in a batch (the batch's flux-array kernels with the
+    cooling terms) against the oracle, bit for bit."""
+    import oracle
     desc = abi.make_desc(law="mhd", riemann="hlle", cooling_rate=0.5, **CASES["2d_plm_ct_rk2"])
+    P0 = _state(desc, 83)
     with Batch(1 << 28, rk_order=2, precision="fp64", max_solves=4) as b:
-        with pytest.raises(runtime.MaraError):
-            b.admit(desc, _state(desc, 83), 0.4, 0.1, 10)
+        s = b.admit(desc, P0, 0.4, 1e300, 5)
+        while not b.finished():
+            b.step(5)
+        done, t, steps, st, _ = b.query(s)
+        assert st == 0 and steps == 5
+        Pg = b.retire(s)
+    Po = P0.copy()
+    err, to, so = oracle.run(desc, Po, 0.4, 1e300, max_steps=5)
+    assert err == 0 and (to, so) == (t, steps)
+    assert np.array_equal(Pg, Po)
 
 
 def test_cooling_with_viscous_fluxes_matches_oracle():
