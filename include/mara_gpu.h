@@ -357,6 +357,21 @@ int mara_batch_step(mara_batch* batch, int n_steps, int* n_active);
    the same either way.  No counterpart in the reference. */
 int mara_batch_set_graphs(mara_batch* batch, int enable);
 
+/* Source terms for every solve of a batch (the batch form of
+   mara_block_set_source, MethodOfLinesTVD::setSourceTermsFunction,
+   src/SolutionSchemes.cpp:203-217): the code object's kernel <kernel>_batch,
+   which MARA_SOURCE_KERNEL defines beside <kernel> (include/mara_source.h),
+   evaluates the function at every interior zone of every solve before each
+   stage, one launch for all of them; the update adds S to L.  Set before
+   the first admission (MARA_ERR_INVALID_ARGUMENT once solves are resident);
+   solves admitted with a source take the flux-array kernels.  A null
+   code_object removes it.  MARA_ERR_UNSUPPORTED at admission for a cooled
+   NewtonianMHD solve, as for a single block. */
+int mara_batch_set_source(mara_batch* batch, const char* code_object, const char* kernel, const double* params,
+                          int n_params);
+/* The 8 auxiliary values passed to the batch's source function. */
+int mara_batch_set_source_aux(mara_batch* batch, const double* t);
+
 /* With workgroups_per_cu > 0 (at most 16) the 2-D and 3-D hydro + PLM
    tiles of a stage run as a work queue: a fixed grid of workgroups_per_cu
    work-groups per CU that claim the same tiles in the same per-XCD order

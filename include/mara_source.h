@@ -49,26 +49,40 @@ typedef struct mara_source_args
 
 /* MARA_SOURCE_KERNEL(name, fn): an extern "C" kernel `name` that evaluates
    fn(x, y, z, t, P, S, params) at every interior zone, one thread per zone,
-   256 threads per work-group. */
-#define MARA_SOURCE_KERNEL(name, fn)                                                              \
-    extern "C" __global__ void __launch_bounds__(256) name(mara_source_args a)                    \
+   256 threads per work-group; and a kernel `name_batch` (for
+   mara_batch_set_source) that does the same for solve blockIdx.y of an
+   array of launch records. */
+#define MARA_SOURCE_ZONE(a, fn)                                                                   \
+    do                                                                                            \
     {                                                                                             \
         const long long z = (long long)blockIdx.x * blockDim.x + threadIdx.x;                      \
-        const long long nx = a.n[0], nxy = (long long)a.n[0] * a.n[1];                             \
-        if (z >= nxy * a.n[2]) return;                                                            \
-        const int ic = (int)(z % nx), jc = (int)((z / nx) % a.n[1]), kc = (int)(z / nxy);          \
-        const long long c = ((long long)(kc + a.g[2]) * a.N[1] + (jc + a.g[1])) * a.N[0] + (ic + a.g[0]); \
+        const long long nx = (a).n[0], nxy = (long long)(a).n[0] * (a).n[1];                       \
+        if (z >= nxy * (a).n[2]) break;                                                           \
+        const int ic = (int)(z % nx), jc = (int)((z / nx) % (a).n[1]), kc = (int)(z / nxy);        \
+        const long long c = ((long long)(kc + (a).g[2]) * (a).N[1] + (jc + (a).g[1])) * (a).N[0]   \
+                            + (ic + (a).g[0]);                                                    \
         double P[8], S[8];                                                                        \
         for (int q = 0; q < 8; ++q)                                                               \
         {                                                                                         \
-            P[q] = q < a.nq ? a.P[q * a.fs + c] : 0.0;                                            \
+            P[q] = q < (a).nq ? (a).P[q * (a).fs + c] : 0.0;                                      \
             S[q] = 0.0;                                                                           \
         }                                                                                         \
-        const double x = a.centre[0] ? a.centre[0][ic] : a.lower[0] + a.dx[0] * (ic + 0.5);      \
-        const double y = a.centre[1] ? a.centre[1][jc] : a.lower[1] + a.dx[1] * (jc + 0.5);      \
-        const double zz = a.centre[2] ? a.centre[2][kc] : a.lower[2] + a.dx[2] * (kc + 0.5);     \
-        fn(x, y, zz, a.t, P, S, a.params);                                                        \
-        for (int q = 0; q < a.nq; ++q) a.S[q * a.fs + c] = S[q];                                  \
+        const double x = (a).centre[0] ? (a).centre[0][ic] : (a).lower[0] + (a).dx[0] * (ic + 0.5); \
+        const double y = (a).centre[1] ? (a).centre[1][jc] : (a).lower[1] + (a).dx[1] * (jc + 0.5); \
+        const double zz = (a).centre[2] ? (a).centre[2][kc] : (a).lower[2] + (a).dx[2] * (kc + 0.5); \
+        fn(x, y, zz, (a).t, P, S, (a).params);                                                    \
+        for (int q = 0; q < (a).nq; ++q) (a).S[q * (a).fs + c] = S[q];                            \
+    } while (0)
+
+#define MARA_SOURCE_KERNEL(name, fn)                                                              \
+    extern "C" __global__ void __launch_bounds__(256) name(mara_source_args a)                    \
+    {                                                                                             \
+        MARA_SOURCE_ZONE(a, fn);                                                                  \
+    }                                                                                             \
+    extern "C" __global__ void __launch_bounds__(256) name##_batch(const mara_source_args* items) \
+    {                                                                                             \
+        const mara_source_args& b = items[blockIdx.y];                                           \
+        MARA_SOURCE_ZONE(b, fn);                                                                  \
     }
 
 /* ---------------------------------------------------------------------------

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <new>
@@ -29,10 +30,13 @@ int class_of(mara_batch* B, const mara_desc& d, bool express = false)
     // flux-array kernels (nine fields and the cooling terms)
     const int law = law_code(d);
     // fused tiles in RK2 batches: hydro + PLM in 2-D (1) and 3-D (2), MHD in 2-D (3)
+    // (with a source every solve takes the flux-array kernels, whose update
+    // adds S)
+    const bool plain = !B->src_fn;
     int fused = 0;
-    if (!one_d && rk2 && law == MARA_LAW_HYDRO && d.recon == MARA_RECON_PLM) fused = d.cells[2] > 1 ? 2 : 1;
-    else if (!one_d && rk2 && law == MARA_LAW_MHD && d.cells[2] == 1) fused = 3;
-    const int pencil = (one_d && rk2 && law != LAW_MHD_COOLING) ? 1 : 0;
+    if (plain && !one_d && rk2 && law == MARA_LAW_HYDRO && d.recon == MARA_RECON_PLM) fused = d.cells[2] > 1 ? 2 : 1;
+    else if (plain && !one_d && rk2 && law == MARA_LAW_MHD && d.cells[2] == 1) fused = 3;
+    const int pencil = (plain && one_d && rk2 && law != LAW_MHD_COOLING) ? 1 : 0;
     const ClassKey k{law, d.recon, d.riemann, (d.law != MARA_LAW_HYDRO && d.field_ct) ? 1 : 0,
                      pencil, fused, pencil ? 1 : (express ? 2 : 0)};
     for (size_t c = 0; c < B->classes.size(); ++c)
@@ -41,7 +45,7 @@ int class_of(mara_batch* B, const mara_desc& d, bool express = false)
     return (int)B->classes.size() - 1;
 }
 
-long long item_bytes(const mara_desc& d, int precision)
+long long item_bytes(const mara_desc& d, int precision, bool with_source = false)
 {
     int N[3];
     for (int a = 0; a < 3; ++a) N[a] = d.cells[a] + 2 * d.guards[a];
@@ -51,7 +55,7 @@ long long item_bytes(const mara_desc& d, int precision)
     const int real = precision == MARA_PREC_MIXED ? 4 : 8;
     long long b = 3 * round_up(nq * fs * 8, 256);
     for (int a = 0; a < 3; ++a) if (N[a] > 1) b += round_up(nq * ffs * real, 256);
-    if (nq == 9) b += round_up(nq * fs * 8, 256);  // the cooling terms
+    if (nq == 9 || with_source) b += round_up(nq * fs * 8, 256);  // the cooling or source terms
     return b;
 }
 
@@ -226,6 +230,7 @@ bool finished(const Ctl& c)
 
 void promote_long_chains(mara_batch* B)
 {
+    if (B->src_fn) return;  // the source launch serves the tile lane's solves
     std::vector<std::pair<double, int>> rem;
     int express = 0;
     for (int k = 0; k < B->n_items; ++k)
@@ -266,6 +271,45 @@ int validate_item(mara_batch* B, const mara_desc* d)
     if (v) return v;
     if (d->rk_order != B->rk_order || d->precision != B->precision) return MARA_ERR_UNSUPPORTED;
     if (d->geometry != MARA_GEOM_CARTESIAN) return MARA_ERR_UNSUPPORTED;
+    // the reference adds the cooling to L before the function's S, an order
+    // one S array cannot keep (as for a single block)
+    if (B->src_fn && law_code(*d) == LAW_MHD_COOLING) return MARA_ERR_UNSUPPORTED;
+    return MARA_OK;
+}
+
+// The source kernel's launch records of slot n (both stage inputs), from its
+// item; zones = 0 leaves the slot out.
+void source_records(mara_batch* B, int n, bool used)
+{
+    const BItem& it = B->items[n];
+    for (int r = 0; r < 2; ++r)
+    {
+        mara_source_args& a = B->src_args[r][n];
+        std::memset(&a, 0, sizeof a);
+        if (!used) continue;
+        a.P = r == 0 ? it.Pa : it.Pb;
+        a.S = it.S;
+        a.fs = it.G.fs;
+        a.nq = it.G.nq;
+        const mara_desc& d = B->slots[n].d;
+        for (int e = 0; e < 3; ++e)
+        {
+            a.N[e] = it.G.N[e];
+            a.g[e] = it.G.g[e];
+            a.n[e] = it.G.n[e];
+            a.lower[e] = d.lower[e];
+            a.dx[e] = (d.upper[e] - d.lower[e]) / d.cells[e];  // CartesianMeshGeometry
+        }
+        for (int q = 0; q < 8; ++q) a.t[q] = B->src_t[q];
+        for (int q = 0; q < MARA_SOURCE_MAX_PARAMS; ++q) a.params[q] = B->src_params[q];
+    }
+}
+
+int upload_source_records(mara_batch* B, int first, int count)
+{
+    for (int r = 0; r < 2; ++r)
+        HIP_TRY(hipMemcpyAsync(B->d_src[r] + first, B->src_args[r].data() + first, count * sizeof(mara_source_args),
+                               hipMemcpyHostToDevice, B->stream));
     return MARA_OK;
 }
 
@@ -319,6 +363,9 @@ int mara_batch_destroy(mara_batch* B)
     (void)hipFree(B->d_bins);
     (void)hipFree(B->d_tiles);
     (void)hipFree(B->d_queue);
+    (void)hipFree(B->d_src[0]);
+    (void)hipFree(B->d_src[1]);
+    if (B->src_module) (void)hipModuleUnload(B->src_module);
     if (B->round_exec) (void)hipGraphExecDestroy(B->round_exec);
     if (B->h_ctl) (void)hipHostFree(B->h_ctl);
     if (B->stream) (void)hipStreamDestroy(B->stream);
@@ -328,6 +375,70 @@ int mara_batch_destroy(mara_batch* B)
     if (B->join) (void)hipEventDestroy(B->join);
     if (B->join3) (void)hipEventDestroy(B->join3);
     delete B;
+    return MARA_OK;
+}
+
+int mara_batch_set_source(mara_batch* B, const char* code_object, const char* kernel, const double* params,
+                          int n_params)
+{
+    if (!B || n_params < 0 || n_params > MARA_SOURCE_MAX_PARAMS || (n_params > 0 && !params)
+        || (code_object && !kernel))
+        return MARA_ERR_INVALID_ARGUMENT;
+    // the solves' layout (S arrays, flux-array classes) is fixed at admission
+    for (int s = 0; s < B->max_solves; ++s)
+        if (B->slots[s].used) return MARA_ERR_INVALID_ARGUMENT;
+    HIP_TRY(hipStreamSynchronize(B->stream));
+    if (B->src_module) (void)hipModuleUnload(B->src_module);
+    B->src_module = nullptr;
+    B->src_fn = nullptr;
+    B->src_zones = 0;
+    B->classes.clear();  // class keys depend on whether a source is set
+    B->dirty = true;
+    if (B->round_exec)
+    {
+        (void)hipGraphExecDestroy(B->round_exec);
+        B->round_exec = nullptr;
+    }
+    if (!code_object) return MARA_OK;
+    if (hipModuleLoad(&B->src_module, code_object) != hipSuccess)
+    {
+        std::fprintf(stderr, "mara_gpu: cannot load source code object %s\n", code_object);
+        (void)hipGetLastError();
+        B->src_module = nullptr;
+        return MARA_ERR_INVALID_ARGUMENT;
+    }
+    // MARA_SOURCE_KERNEL's batched kernel, <kernel>_batch
+    char name[256];
+    std::snprintf(name, sizeof name, "%s_batch", kernel);
+    if (hipModuleGetFunction(&B->src_fn, B->src_module, name) != hipSuccess)
+    {
+        std::fprintf(stderr, "mara_gpu: no kernel %s in %s\n", name, code_object);
+        (void)hipGetLastError();
+        (void)hipModuleUnload(B->src_module);
+        B->src_module = nullptr;
+        B->src_fn = nullptr;
+        return MARA_ERR_INVALID_ARGUMENT;
+    }
+    for (int r = 0; r < 2; ++r)
+    {
+        if (!B->d_src[r]) HIP_TRY(hipMalloc(&B->d_src[r], B->max_solves * sizeof(mara_source_args)));
+        B->src_args[r].assign(B->max_solves, mara_source_args{});
+    }
+    for (int q = 0; q < MARA_SOURCE_MAX_PARAMS; ++q) B->src_params[q] = q < n_params ? params[q] : 0.0;
+    return upload_source_records(B, 0, B->max_solves) ? MARA_ERR_HIP : MARA_OK;
+}
+
+int mara_batch_set_source_aux(mara_batch* B, const double* t)
+{
+    if (!B || !t) return MARA_ERR_INVALID_ARGUMENT;
+    for (int q = 0; q < 8; ++q) B->src_t[q] = t[q];
+    if (!B->src_fn) return MARA_OK;
+    for (int n = 0; n < B->max_solves; ++n)
+        for (int r = 0; r < 2; ++r)
+            for (int q = 0; q < 8; ++q) B->src_args[r][n].t[q] = t[q];
+    int e = upload_source_records(B, 0, B->max_solves);
+    if (e) return e;
+    HIP_TRY(hipStreamSynchronize(B->stream));
     return MARA_OK;
 }
 
@@ -406,7 +517,7 @@ int mara_batch_admit(mara_batch* B, const mara_desc* d, const double* P, double 
     for (int s = 0; s < B->max_solves; ++s)
         if (!B->slots[s].used) { n = s; break; }
     if (n < 0) return MARA_ERR_CAPACITY;
-    const long long bytes = item_bytes(*d, B->precision);
+    const long long bytes = item_bytes(*d, B->precision, B->src_fn != nullptr);
     const long long off = B->pool.alloc(bytes);
     if (off < 0) return MARA_ERR_CAPACITY;
 
@@ -430,7 +541,7 @@ int mara_batch_admit(mara_batch* B, const mara_desc* d, const double* P, double 
     for (int a = 0; a < 3; ++a)
         if (it.G.N[a] > 1) { it.F[a] = p; p += round_up(it.G.nq * it.G.ffs * real, 256); }
     it.S = nullptr;
-    if (it.G.nq == 9)
+    if (it.G.nq == 9 || B->src_fn)
     {
         // the cooling terms; fields other than the energy and the auxiliary
         // one stay zero
@@ -477,6 +588,18 @@ int mara_batch_admit(mara_batch* B, const mara_desc* d, const double* P, double 
     if (!hip_ok(hipMemcpyAsync(it.Pb, it.Pa, cb, hipMemcpyDeviceToDevice, B->stream))) return undo();
     if (it.S && !hip_ok(hipMemsetAsync(it.S, 0, cb, B->stream))) return undo();
     if (!hip_ok(hipGetLastError())) return undo();
+    if (B->src_fn)
+    {
+        source_records(B, n, true);
+        if (upload_source_records(B, n, 1)) return undo();
+        B->src_zones = std::max(B->src_zones, (long long)it.G.n[0] * it.G.n[1] * it.G.n[2]);
+        if (B->round_exec)
+        {
+            // the captured round's source launch covers the old slot count and zone count
+            (void)hipGraphExecDestroy(B->round_exec);
+            B->round_exec = nullptr;
+        }
+    }
     // the host buffer may go away after return
     if (!hip_ok(hipStreamSynchronize(B->stream))) return undo();
     if (n + 1 > B->n_items) B->n_items = n + 1;
@@ -684,6 +807,12 @@ int mara_batch_retire(mara_batch* B, int slot, double* P)
     std::memset(&c, 0, sizeof c);
     c.cbins = B->d_bins + (size_t)slot * CBINS * CBIN_STRIDE;
     HIP_TRY(hipMemcpyAsync(B->d_ctl + slot, &c, sizeof(Ctl), hipMemcpyHostToDevice, B->stream));
+    if (B->src_fn)
+    {
+        source_records(B, slot, false);
+        int e = upload_source_records(B, slot, 1);
+        if (e) return e;
+    }
     HIP_TRY(hipStreamSynchronize(B->stream));
     B->pool.release(sl.off, sl.bytes);
     sl.used = false;

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/mara_gpu.h"
+#include "../../include/mara_source.h"
 #include "batch.hpp"
 #include "host_common.hpp"
 
@@ -112,6 +113,16 @@ struct mara_batch
     int rows = mara::MHD_ROWS_DEFAULT;  // 2-D MHD tiles of 64 x (4 * rows) zones
     int express_max = 0;                // solves the express lane may hold (0: none)
     int persistent = 0;                 // fused hydro tiles from a work queue: work-groups per CU (0: off)
+    // source terms of every solve (mara_batch_set_source): the code object's
+    // batched kernel and one launch record per slot for each stage input
+    // (Pa in a step's first stage, Pb after), zones = 0 in unused slots
+    hipModule_t src_module = nullptr;
+    hipFunction_t src_fn = nullptr;
+    double src_params[MARA_SOURCE_MAX_PARAMS] = {};
+    double src_t[8] = {};
+    std::vector<mara_source_args> src_args[2];
+    mara_source_args* d_src[2] = {nullptr, nullptr};
+    long long src_zones = 0;            // largest interior zone count of a solve admitted with the source
     unsigned* d_queue = nullptr;        // the queue's counters, one set of NUM_XCD per tile lane
     hipStream_t stream = nullptr;   // tile kernels, transfers, control
     hipStream_t stream2 = nullptr;  // pencil kernels of 1-D solves
@@ -300,6 +311,15 @@ void run_stage(mara_batch* B, int rk, int lane)
     static const double b3[3] = {1.0, 1. / 4, 2. / 3};
     const double* bb = B->rk_order == 1 ? b1 : B->rk_order == 2 ? b2 : b3;
     const int last = rk == B->rk_order - 1;
+    // source terms of the stage's primitives (MeshOperator::generateSourceTerms),
+    // every solve in one launch: solve blockIdx.y of the launch records
+    if (B->src_fn && lane == 0 && B->n_items > 0)
+    {
+        mara_source_args* recs = B->d_src[rk == 0 ? 0 : 1];
+        void* args[] = {&recs};
+        (void)hipModuleLaunchKernel(B->src_fn, (unsigned)((B->src_zones + 255) / 256), (unsigned)B->n_items, 1, 256, 1,
+                                    1, 0, B->cur, args, nullptr);
+    }
     // NewtonianMHD cooling (ConservationLaw::addSourceTerms) of the stage's
     // primitives, over the update's tiles of the nine-field classes
     for (auto& e : B->update)

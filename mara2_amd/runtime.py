@@ -80,6 +80,8 @@ def load():
     L.mara_batch_lane.argtypes = [bat, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     L.mara_batch_set_express.argtypes = [bat, ctypes.c_int]
     L.mara_batch_set_persistent.argtypes = [bat, ctypes.c_int]
+    L.mara_batch_set_source.argtypes = [bat, ctypes.c_char_p, ctypes.c_char_p, _dp, ctypes.c_int]
+    L.mara_batch_set_source_aux.argtypes = [bat, _dp]
     L.mara_batch_set_shape.argtypes = [bat, ctypes.POINTER(MaraShape)]
     L.mara_batch_destroy.argtypes = [bat]
     L.mara_batch_item_bytes.argtypes = [ctypes.POINTER(MaraDesc), ctypes.c_int]
@@ -317,6 +319,20 @@ class Batch:
     def set_express(self, max_solves):
         """The most long-chain solves the express lane may hold (0: none)."""
         _check(self.lib.mara_batch_set_express(self._h, max_solves), "mara_batch_set_express")
+
+    def set_source(self, code_object, kernel=None, params=()):
+        """Source terms for every solve (include/mara_source.h; the code
+        object's <kernel>_batch kernel); before the first admission.
+        code_object=None removes them."""
+        pr = np.ascontiguousarray(params, dtype=np.float64)
+        _check(self.lib.mara_batch_set_source(
+            self._h, os.fsencode(code_object) if code_object else None, kernel.encode() if kernel else None,
+            pr.ctypes.data_as(_dp) if len(pr) else None, len(pr)), "mara_batch_set_source")
+
+    def set_source_aux(self, t):
+        t8 = np.zeros(8)
+        t8[:len(t)] = t
+        _check(self.lib.mara_batch_set_source_aux(self._h, t8.ctypes.data_as(_dp)), "mara_batch_set_source_aux")
 
     def set_persistent(self, workgroups_per_cu):
         """Run the 2-D/3-D hydro + PLM tiles as a work queue over a fixed grid

@@ -140,3 +140,139 @@ def test_bad_code_object_or_kernel_is_refused():
             blk.set_source(GRAVITY[0], GRAVITY[1], np.zeros(17))
         blk.upload(_state(desc, 65))
         blk.advance(0.3 * blk.courant_dt())
+
+
+# ---------------------------------------------------------------------------
+# Source terms in batches (mara_batch_set_source): the code object's
+# <kernel>_batch kernel evaluates the function for every solve of the batch in
+# one launch per stage.
+
+BATCH_MIX = {
+    "hydro1d": dict(law="hydro", recon="plm", riemann="hllc", bc="outflow", cells=(300, 1, 1)),
+    "hydro2d": dict(law="hydro", recon="plm", riemann="hllc", bc="periodic", cells=(70, 37, 1)),
+    "mhd2d_ct": dict(law="mhd", recon="plm", riemann="hlle", bc="periodic", cells=(48, 30, 1), field_ct=1),
+    "mhd1d": dict(law="mhd", recon="plm", riemann="hlle", bc="outflow", cells=(120, 1, 1)),
+    "hydro3d_weno": dict(law="hydro", recon="weno5", riemann="hlle", bc="periodic", cells=(20, 12, 10)),
+}
+
+
+def _batch(items, code, params, taux=(), precision="fp64", cfl=0.3, steps=5):
+    """Admit every (name, desc, P0) at once under one source; step to the end."""
+    from mara2_amd.runtime import Batch
+    out = {}
+    with Batch(1 << 28, rk_order=2, precision=precision, max_solves=8) as b:
+        if code is not None:
+            b.set_source(code[0], code[1], params)
+        if taux:
+            b.set_source_aux(taux)
+        slots = {b.admit(d, P, cfl, 1e300, steps): name for name, d, P in items}
+        while slots:
+            b.step(3)
+            for s in b.finished():
+                done, t, n, st, rep = b.query(s)
+                assert done and st == 0, rep.as_dict()
+                out[slots.pop(s)] = (b.retire(s), t, n)
+    return out
+
+
+def _mix(seed, precision="fp64"):
+    items = []
+    for i, (name, kw) in enumerate(sorted(BATCH_MIX.items())):
+        d = abi.make_desc(precision=precision, **kw)
+        items.append((name, d, _state(d, seed + i)))
+    return items
+
+
+@pytest.mark.parametrize("which", ["gravity", "point_mass"])
+def test_batched_sources_match_oracle(which):
+    """Every solve of a 1-D/2-D/3-D hydro and MHD mix under one source,
+    bit for bit against the oracle's loop with the same source."""
+    import oracle
+    if which == "gravity":
+        code, params, taux, name = GRAVITY, (0.3, -0.2, 0.1), (), "uniform_gravity"
+    else:
+        code, params, taux, name = POINT, (0.1, 0.01), (0.4, 0.35, 0.5), "point_mass"
+    items = _mix(71)
+    got = _batch(items, code, params, taux)
+    for nm, d, P0 in items:
+        Po = P0.copy()
+        err, to, so = oracle.run(d, Po, 0.3, 1e300, max_steps=5, source=(name, list(taux), list(params)))
+        assert err == 0
+        Pg, t, n = got[nm]
+        assert (t, n) == (to, so), nm
+        diff = np.argwhere(Pg != Po)
+        assert diff.size == 0, f"{nm}: {len(diff)} differ, first {diff[0]}"
+
+
+def test_batched_source_equals_single_block():
+    items = _mix(72)
+    got = _batch(items, GRAVITY, (0.5, 0.25, -0.125))
+    for nm, d, P0 in items:
+        with Block(d) as blk:
+            blk.set_source(*GRAVITY, (0.5, 0.25, -0.125))
+            blk.upload(P0)
+            t, n, _ = blk.run(0.3, 1e300, 5)
+            Pb = blk.download()
+        assert (t, n) == got[nm][1:], nm
+        assert np.array_equal(Pb, got[nm][0]), nm
+
+
+def test_mixed_batch_with_source_close_to_oracle():
+    """The mixed form's batch (binary32 faces) with the source within the
+    smooth-flow bound 1e-5 of the FP64 oracle after 5 steps."""
+    import oracle
+    from helpers import interior, prim_to_cons, rel_l1
+    items = _mix(73, precision="mixed")
+    got = _batch(items, GRAVITY, (0.3, -0.2, 0.1), precision="mixed")
+    for nm, d, P0 in items:
+        d64 = abi.make_desc(**BATCH_MIX[nm])
+        Po = P0.copy()
+        err, to, so = oracle.run(d64, Po, 0.3, 1e300, max_steps=5, source=("uniform_gravity", [], [0.3, -0.2, 0.1]))
+        assert err == 0 and so == got[nm][2]
+        e = rel_l1(prim_to_cons(d64, interior(d64, got[nm][0])), prim_to_cons(d64, interior(d64, Po)))
+        assert e <= 1e-5, (nm, e)
+
+
+def test_removed_batch_source_restores_the_plain_kernels():
+    """Setting a source and removing it again before admission gives the
+    plain batch (fused tiles and pencils included) bit for bit."""
+    items = _mix(74)
+    plain = _batch(items, None, ())
+    from mara2_amd.runtime import Batch
+    out = {}
+    with Batch(1 << 28, rk_order=2, precision="fp64", max_solves=8) as b:
+        b.set_source(*GRAVITY, (1.0, 0.0, 0.0))
+        b.set_source(None)
+        slots = {b.admit(d, P, 0.3, 1e300, 5): nm for nm, d, P in items}
+        while slots:
+            b.step(3)
+            for s in b.finished():
+                out[slots.pop(s)] = b.retire(s)
+    for nm in plain:
+        assert np.array_equal(out[nm], plain[nm][0]), nm
+
+
+def test_batch_source_refusals():
+    from mara2_amd.runtime import Batch
+    d = abi.make_desc(**BATCH_MIX["hydro2d"])
+    dc = abi.make_desc(law="mhd", riemann="hlle", cooling_rate=0.5, cells=(40, 24, 1), bc="periodic", field_ct=1)
+    with Batch(1 << 28, rk_order=2, precision="fp64", max_solves=4) as b:
+        with pytest.raises(runtime.MaraError):
+            b.set_source(GRAVITY[0], "no_such_kernel")
+        with pytest.raises(runtime.MaraError):
+            b.set_source(os.path.join(SRC, "missing.hsaco"), "k")
+        b.set_source(*GRAVITY, (0.1, 0.0, 0.0))
+        P = problems.empty(dc)
+        P[..., 0] = 1.0
+        P[..., 4] = 1.0
+        with pytest.raises(runtime.MaraError) as e:
+            b.admit(dc, problems.apply_bc(dc, P), 0.3, 1e300, 5)
+        assert e.value.status == abi.ERR_UNSUPPORTED
+        s = b.admit(d, _state(d, 75), 0.3, 1e300, 2)
+        with pytest.raises(runtime.MaraError) as e:
+            b.set_source(None)
+        assert e.value.status == abi.ERR_INVALID_ARGUMENT
+        while not b.finished():
+            b.step(2)
+        b.retire(s)
+        b.set_source(None)  # allowed again once no solve is resident
