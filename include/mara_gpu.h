@@ -245,7 +245,9 @@ typedef struct mara_shape
     int pencil;  /* 1-D stages: faces per pencil work-group, 128, 256 or 512
                     (the relativistic law runs 256 whatever is set) */
     int zplanes; /* fused 3-D hydro stages: z planes a work-group marches,
-                    4, 8, 16 or 32 */
+                    1 to 1024 (the autotuner tries 4, 8, 16, 32 and the
+                    counts that make the tile count a whole number of
+                    device-filling rounds) */
     int rows;    /* flux-array stages: work-group rows of 64 zones; fused
                     2-D MHD stages: tiles of 64 x (4 * rows) zones; 1, 2 or 4 */
 } mara_shape;

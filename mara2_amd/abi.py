@@ -75,7 +75,8 @@ class MaraShape(ctypes.Structure):
 
 
 PENCIL_SHAPES = (128, 256, 512)
-ZPLANE_SHAPES = (4, 8, 16, 32)
+ZPLANE_SHAPES = (4, 8, 16, 32)  # the autotuner's fixed candidates; set_shape takes 1 .. ZPLANES_MAX
+ZPLANES_MAX = 1024
 ROW_SHAPES = (1, 2, 4)
 MHD_ROWS_DEFAULT = 2  # fused 2-D MHD tiles of 64 x 8 zones
 

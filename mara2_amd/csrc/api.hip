@@ -204,6 +204,18 @@ void run_stage_kernels(const StageArgs& s, bool mixed)
     else stage_kernels<double>(s);
 }
 
+} // namespace
+
+void mara::stage_round(const StageArgs& s, bool mixed, int* per_cu, int* per_layer)
+{
+    const bool fast_tile = s.fused && fast64_supported(s.d);
+    if (fast_tile && (mixed || s.d->precision == MARA_PREC_FP64_FAST)) stage_round_fast(s, mixed, per_cu, per_layer);
+    else if (mixed) stage_round_f32(s, per_cu, per_layer);
+    else fused_round_r<double>(*s.d, *s.G, per_cu, per_layer);
+}
+
+namespace {
+
 // The buffers stage rk reads and writes.
 void stage_io(const mara_block* b, int rk, double** Pin, double** Pout)
 {
@@ -560,7 +572,31 @@ int mara_block_autotune(mara_block* b, int repeats, mara_shape* chosen, double* 
     if (b->pencil && b->d.law != MARA_LAW_RMHD)
         for (int p : PENCIL_SHAPES) { mara_shape s = cur; s.pencil = p; cands.push_back(s); }
     else if (b->fused && b->G.N[2] > 1)
+    {
         for (int z : KZ_SHAPES) { mara_shape s = cur; s.zplanes = z; cands.push_back(s); }
+        // z-plane counts whose tile count is a whole number (1 to 3) of device
+        // rounds, each round every CU holding as many work-groups as the
+        // kernel's occupancy allows: no partly filled last round, and fewer
+        // tiles re-reading the z window at their start
+        StageArgs q;
+        q.d = &b->d;
+        q.G = &b->G;
+        int per_cu = 0, per_layer = 0, cus = 0, dev = 0;
+        stage_round(q, b->d.precision == MARA_PREC_MIXED, &per_cu, &per_layer);
+        if (hipGetDevice(&dev) != hipSuccess
+            || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            cus = 0;
+        (void)hipGetLastError();
+        for (int r = 1; r <= 3 && per_cu > 0 && per_layer > 0 && cus > 0; ++r)
+        {
+            const long long layers = (long long)r * per_cu * cus / per_layer;
+            if (layers < 1) continue;
+            const int z = (int)((b->G.n[2] + layers - 1) / layers);
+            bool have = z < 4 || z > KZ_MAX;
+            for (const mara_shape& c : cands) have = have || c.zplanes == z;
+            if (!have) { mara_shape s = cur; s.zplanes = z; cands.push_back(s); }
+        }
+    }
     else if (!b->fused && !b->pencil)  // flux-array rows or fused 2-D MHD tile height
         for (int r : ROWS_SHAPES) { mara_shape s = cur; s.rows = r; cands.push_back(s); }
     else

@@ -23,6 +23,12 @@ void stage_kernels_fast32(const StageArgs& s)
     fast64::launch<float>(*s.G, s.stream, s.d->riemann, s.Pin, s.P0, s.Pout, s.b, s.first, s.cout, s.ctl, s.kz);
 }
 
+void stage_round_fast(const StageArgs& s, bool f32, int* per_cu, int* per_layer)
+{
+    if (f32) fast64::tile_round<float>(*s.G, s.d->riemann, per_cu, per_layer);
+    else fast64::tile_round<double>(*s.G, s.d->riemann, per_cu, per_layer);
+}
+
 // The CFL reduction in the arithmetic of a form other than the reference's
 // (fast64::CflArith), compiled with the same flags as the stage kernels
 // that produce the form's later time steps.

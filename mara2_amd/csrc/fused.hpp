@@ -29,7 +29,8 @@ namespace mara {
 constexpr int FUSED_TX = 64;
 constexpr int FUSED_TY = 8;
 constexpr int FUSED_KZ = 16;  // default z planes per work-group (a launch shape)
-constexpr int KZ_SHAPES[4] = {4, 8, 16, 32};
+constexpr int KZ_SHAPES[4] = {4, 8, 16, 32};  // the tuner's fixed candidates
+constexpr int KZ_MAX = 1024;                   // set_shape accepts 1 .. KZ_MAX
 
 // Work-group -> tile order that keeps neighbouring tiles on one XCD.  The
 // dispatcher hands work-group L of a 1-D grid to XCD L % 8, and each XCD has
@@ -670,8 +671,8 @@ inline const char* fused_kernel_name(const mara_desc& d)
     return mixed ? "k_fused_hydro<float>" : "k_fused_hydro<double>";
 }
 
-// kz: z planes per work-group (one of KZ_SHAPES; a launch shape, the
-// result does not depend on it)
+// kz: z planes per work-group (1 .. KZ_MAX; a launch shape, the result does
+// not depend on it)
 template <int Recon, int Riem, typename R>
 inline void launch_fused_t(const Geom& G, hipStream_t s, const double* Pin, const double* P0, double* U,
                            double* Pout, double b, int first, int write_u, int cout, Ctl* ctl, int kz)
@@ -693,6 +694,53 @@ inline void launch_fused_t(const Geom& G, hipStream_t s, const double* Pin, cons
     {
         if (z) k_fused_hydro<Recon, Riem, R, true><<<grid, block, 0, s>>>(Pin, P0, U, Pout, G, b, first, write_u, cout, ctl, kz);
         else k_fused_hydro<Recon, Riem, R, false><<<grid, block, 0, s>>>(Pin, P0, U, Pout, G, b, first, write_u, cout, ctl, kz);
+    }
+}
+
+// The fused 3-D kernel's work-groups resident on one CU (the occupancy its
+// registers and LDS allow) and its tiles per z layer of the block, from which
+// the autotuner derives z-plane counts whose tile count fills whole rounds of
+// the device.
+template <int Recon, int Riem, typename R>
+inline void fused_round_t(const Geom& G, int* per_cu, int* per_layer)
+{
+    int n = 0;
+    if constexpr (Recon == 1)
+    {
+        constexpr int TY = PlmTile<R>::TY;
+        *per_layer = ((G.n[0] + FUSED_TX - 1) / FUSED_TX) * ((G.n[1] + TY - 1) / TY);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused_plm<Riem, R, true>, FUSED_TX * TY, 0) != hipSuccess)
+            n = 0;
+    }
+    else
+    {
+        *per_layer = ((G.n[0] + FUSED_TX - 1) / FUSED_TX) * ((G.n[1] + FUSED_TY - 1) / FUSED_TY);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused_hydro<Recon, Riem, R, true>, FUSED_TX * FUSED_TY, 0)
+            != hipSuccess)
+            n = 0;
+    }
+    (void)hipGetLastError();
+    *per_cu = n;
+}
+
+template <typename R>
+inline void fused_round_r(const mara_desc& d, const Geom& G, int* per_cu, int* per_layer)
+{
+    const bool hllc = d.riemann == MARA_RIEMANN_HLLC;
+    switch (d.recon)
+    {
+        case MARA_RECON_PCM:
+            if (hllc) fused_round_t<0, 1, R>(G, per_cu, per_layer);
+            else fused_round_t<0, 0, R>(G, per_cu, per_layer);
+            break;
+        case MARA_RECON_PLM:
+            if (hllc) fused_round_t<1, 1, R>(G, per_cu, per_layer);
+            else fused_round_t<1, 0, R>(G, per_cu, per_layer);
+            break;
+        default:
+            if (hllc) fused_round_t<2, 1, R>(G, per_cu, per_layer);
+            else fused_round_t<2, 0, R>(G, per_cu, per_layer);
+            break;
     }
 }
 

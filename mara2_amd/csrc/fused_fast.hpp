@@ -290,34 +290,75 @@ __device__ __forceinline__ void fast_plm_tile(const double* __restrict__ Pin, co
         face_flux<Riem, 2>(K, PL, PR, Fzlo);
     }
 
+    // The zones of plane k staged from HBM: in 3-D the ring around the tile
+    // (its own zones come from the z window), else the whole plane.  Every
+    // load of a plane (ring, the z window's next value) is issued before the
+    // first LDS write waits on one, so the plane costs one memory latency.
+    constexpr int NH = Z ? HX * HY - NT : HX * HY;
+    constexpr int PH = (NH + NT - 1) / NT;
     for (int k = kb; k < ke; ++k)
     {
         const long long pbase = (long long)k * sxy;
+        // the zone's offset in the plane, opaque per plane like the ring's
+        int colk = (int)col;
+        asm volatile("" : "+v"(colk));
+        R hv[PH][5];
+        int hpos[PH];
+#pragma unroll
+        for (int p = 0; p < PH; ++p)
+        {
+            const int h = tid + p * NT;
+            hpos[p] = -1;
+            if ((p + 1) * NT <= NH || h < NH)
+            {
+                int hx, hy;
+                if constexpr (Z)
+                {
+                    // rows 0, 1 and TY + 2, TY + 3 whole; columns 0, 1, HX - 2,
+                    // HX - 1 of rows 2 .. TY + 1
+                    if (h < 4 * HX)
+                    {
+                        const int r = h / HX;
+                        hy = r < 2 ? r : TY + r;
+                        hx = h - r * HX;
+                    }
+                    else
+                    {
+                        const int e = h - 4 * HX, c = e & 3;
+                        hy = NG + (e >> 2);
+                        hx = c < 2 ? c : HX - 4 + c;
+                    }
+                }
+                else
+                {
+                    hy = h / HX;
+                    hx = h - hy * HX;
+                }
+                int off = min(j0 - NG + hy, G.N[1] - 1) * G.N[0] + min(i0 - NG + hx, G.N[0] - 1);
+                // an opaque copy: the offset is recomputed per plane rather
+                // than its five 64-bit field addresses held across the march
+                asm volatile("" : "+v"(off));
+                const double* src = Pin + pbase + off;
+#pragma unroll
+                for (int q = 0; q < 5; ++q) hv[p][q] = R(src[q * G.fs]);
+                hpos[p] = hy * HX + hx;
+            }
+        }
+        R wn[5];
         if constexpr (Z)
         {
 #pragma unroll
+            for (int q = 0; q < 5; ++q) wn[q] = R(Pin[q * G.fs + colk + (long long)(k + 2) * sxy]);
+#pragma unroll
             for (int q = 0; q < 5; ++q) plane[q][row + NG][lane + NG] = W[q][2];
-            for (int t = tid; t < HX * HY; t += NT)
-            {
-                const int hx = t % HX, hy = t / HX;
-                if (hx >= NG && hx < NG + TX && hy >= NG && hy < NG + TY) continue;
-                const int gi = min(i0 - NG + hx, G.N[0] - 1), gj = min(j0 - NG + hy, G.N[1] - 1);
-                const long long c = pbase + (long long)gj * G.N[0] + gi;
-#pragma unroll
-                for (int q = 0; q < 5; ++q) plane[q][hy][hx] = R(Pin[q * G.fs + c]);
-            }
         }
-        else
-        {
-            for (int t = tid; t < HX * HY; t += NT)
-            {
-                const int hx = t % HX, hy = t / HX;
-                const int gi = min(i0 - NG + hx, G.N[0] - 1), gj = min(j0 - NG + hy, G.N[1] - 1);
-                const long long c = pbase + (long long)gj * G.N[0] + gi;
 #pragma unroll
-                for (int q = 0; q < 5; ++q) plane[q][hy][hx] = R(Pin[q * G.fs + c]);
+        for (int p = 0; p < PH; ++p)
+            if (hpos[p] >= 0)
+            {
+#pragma unroll
+                for (int q = 0; q < 5; ++q) (&plane[q][0][0])[hpos[p]] = hv[p][q];
             }
-        }
         R Fzhi[5];
         if constexpr (Z)
         {
@@ -328,7 +369,7 @@ __device__ __forceinline__ void fast_plm_tile(const double* __restrict__ Pin, co
                 W[q][0] = W[q][1];
                 W[q][1] = W[q][2];
                 W[q][2] = W[q][3];
-                W[q][3] = R(Pin[q * G.fs + col + (long long)(k + 2) * sxy]);
+                W[q][3] = wn[q];
                 const R hn = half_slope(K.th2, W[q][1], W[q][2], W[q][3]);
                 PL[q] = W[q][1] + hzc[q];
                 PR[q] = W[q][2] - hn;
@@ -457,7 +498,7 @@ __device__ __forceinline__ void fast_plm_tile(const double* __restrict__ Pin, co
         }
         if (cell_xy)
         {
-            const long long c = pbase + col;
+            const long long c = pbase + colk;
             // the stage's own state U(Pin) of the zone's binary64 input value
             // (in binary64 the z window or the LDS plane hold it; the binary32
             // tiles re-read it); the step's start state U(P0) is that value
@@ -531,6 +572,21 @@ inline bool supported(const mara_desc* d)
 {
     return d->law == MARA_LAW_HYDRO && d->recon == MARA_RECON_PLM && d->cells[1] > 1 && d->rk_order >= 2
            && d->geometry == MARA_GEOM_CARTESIAN;
+}
+
+// Work-groups of the 3-D fast kernel resident on one CU and its tiles per z
+// layer (fused_round_t's counterpart).
+template <typename R>
+inline void tile_round(const Geom& G, int riemann, int* per_cu, int* per_layer)
+{
+    constexpr int TX = FastTile<R>::TX, TY = FastTile<R>::TY;
+    *per_layer = ((G.n[0] + TX - 1) / TX) * ((G.n[1] + TY - 1) / TY);
+    int n = 0;
+    const hipError_t e = riemann == MARA_RIEMANN_HLLC
+                             ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fast_plm<R, 1, true>, TX * TY, 0)
+                             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fast_plm<R, 0, true>, TX * TY, 0);
+    (void)hipGetLastError();
+    *per_cu = e == hipSuccess ? n : 0;
 }
 
 // One stage on the block's stream: R = double for the fast binary64 form,

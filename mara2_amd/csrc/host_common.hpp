@@ -171,7 +171,7 @@ inline bool valid_shape(const mara_shape& s)
             if (list[i] == v) return true;
         return false;
     };
-    return in(s.pencil, PENCIL_SHAPES, 3) && in(s.zplanes, KZ_SHAPES, 4) && in(s.rows, ROWS_SHAPES, 3);
+    return in(s.pencil, PENCIL_SHAPES, 3) && s.zplanes >= 1 && s.zplanes <= KZ_MAX && in(s.rows, ROWS_SHAPES, 3);
 }
 
 inline int ctl_status(const Ctl& c)

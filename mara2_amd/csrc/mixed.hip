@@ -12,6 +12,7 @@
 namespace mara {
 
 void stage_kernels_f32(const StageArgs& s) { stage_kernels<float>(s); }
+void stage_round_f32(const StageArgs& s, int* per_cu, int* per_layer) { fused_round_r<float>(*s.d, *s.G, per_cu, per_layer); }
 
 namespace batch {
 void run_stage_f32(mara_batch* B, int rk, int lane) { run_stage<float>(B, rk, lane); }

@@ -267,6 +267,13 @@ void stage_kernels_fast64(const StageArgs& s);
 void stage_kernels_fast32(const StageArgs& s);
 bool fast64_supported(const mara_desc* d);
 
+// Work-groups resident per CU and tiles per z layer of the fused 3-D kernel a
+// stage launches (for the autotuner's z-plane candidates): the fast kernels
+// (fast.hip), the binary32 and binary64 fused kernels (mixed.hip, api.hip)
+void stage_round_fast(const StageArgs& s, bool f32, int* per_cu, int* per_layer);
+void stage_round_f32(const StageArgs& s, int* per_cu, int* per_layer);
+void stage_round(const StageArgs& s, bool mixed, int* per_cu, int* per_layer);
+
 // the CFL reduction in a form's own arithmetic (fast.hip; arith one of
 // fast64::CflArith other than CFL_EXACT), law 3 for cooled NewtonianMHD
 void courant_kernels_form(const double* P, const Geom& G, int law, Ctl* ctl, hipStream_t stream, int arith,

@@ -113,7 +113,7 @@ def test_launch_shapes_give_the_same_bits():
     desc = _fast(problems.turbulence_desc(40))
     P0 = problems.turbulence_3d(problems.turbulence_desc(40))
     results = []
-    for kz in abi.ZPLANE_SHAPES:
+    for kz in abi.ZPLANE_SHAPES + (13, 40):
         with Block(desc) as blk:
             blk.set_shape(abi.make_shape(zplanes=kz))
             blk.upload(P0)
