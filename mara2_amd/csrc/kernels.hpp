@@ -218,8 +218,10 @@ __device__ __forceinline__ void face_flux(const R (&S)[LawNQ<Law>::value][2 * St
         }
         else
         {
-            PL[q] = weno5<R, true>(S[q][0], S[q][1], S[q][2], S[q][3], S[q][4]);
-            PR[q] = weno5<R, false>(S[q][1], S[q][2], S[q][3], S[q][4], S[q][5]);
+            // the relativistic law's divisions are Rel's (physics.hpp)
+            using D = typename std::conditional<Law == 2, typename RelMath<R>::type, Math<R>>::type;
+            PL[q] = weno5<R, true, D>(S[q][0], S[q][1], S[q][2], S[q][3], S[q][4]);
+            PR[q] = weno5<R, false, D>(S[q][1], S[q][2], S[q][3], S[q][4], S[q][5]);
         }
     }
     const R n[3] = {R(G.nhat[3 * axis + 0]), R(G.nhat[3 * axis + 1]), R(G.nhat[3 * axis + 2])};
@@ -236,7 +238,7 @@ __device__ __forceinline__ void face_flux(const R (&S)[LawNQ<Law>::value][2 * St
     else if constexpr (Riem == 1 && Law == 0)
         hllc(SL, SR, n, Fq);
     else
-        hlle<R, NQ>(SL, SR, Fq);
+        hlle<R, NQ, typename std::conditional<Law == 2, typename RelMath<R>::type, Math<R>>::type>(SL, SR, Fq);
 }
 
 // Flux through the face left of cell (i, j, k) along axis, from the stencil

@@ -10,6 +10,7 @@
 
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <type_traits>
 
 namespace mara {
 
@@ -33,6 +34,44 @@ template <> struct Math<float>
     static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
     static __device__ __forceinline__ float abs(float x) { return __builtin_fabsf(x); }
 };
+
+// Binary64 arithmetic of the relativistic law.  Its results are held to
+// relative L1 1e-12 of the reference, not to the reference's rounding (its
+// quartic already goes through the device libm), so its divisions and
+// square roots take the hardware reciprocal (reciprocal square root) refined
+// by two Newton steps, within a few ulp, in place of the correctly rounded
+// sequences (about ten instructions each), and its expressions contract to
+// fused multiply-adds where they are written as multiply-adds.
+struct Rel
+{
+    // 1/x; x = 0, +-inf and NaN give the hardware value (+-inf, 0, NaN)
+    static __device__ __forceinline__ double rcp(double x)
+    {
+        const double y0 = __builtin_amdgcn_rcp(x);
+        double e = __builtin_fma(-x, y0, 1.0);
+        double y = __builtin_fma(y0, e, y0);
+        e = __builtin_fma(-x, y, 1.0);
+        y = __builtin_fma(y, e, y);
+        return __builtin_isfinite(y) ? y : y0;
+    }
+    static __device__ __forceinline__ double div(double a, double b) { return a * rcp(b); }
+    // sqrt(x) as x / sqrt(x); 0 and +inf as they are, NaN below 0
+    static __device__ __forceinline__ double sqrt(double x)
+    {
+        double r = __builtin_amdgcn_rsq(x);
+        double e = __builtin_fma(-(x * r), r, 1.0);
+        r = __builtin_fma(0.5 * r, e, r);
+        e = __builtin_fma(-(x * r), r, 1.0);
+        r = __builtin_fma(0.5 * r, e, r);
+        return (x == 0.0 || x == __builtin_inf()) ? x : x * r;
+    }
+    static __device__ __forceinline__ double abs(double x) { return __builtin_fabs(x); }
+};
+
+// the relativistic law's arithmetic in working precision R: Rel in
+// binary64, the mixed form's binary32 one otherwise
+template <typename R> struct RelMath { using type = Math<R>; };
+template <> struct RelMath<double> { using type = Rel; };
 
 template <typename R> __device__ __forceinline__ R min2(R a, R b) { return a < b ? a : b; }
 template <typename R> __device__ __forceinline__ R max2(R a, R b) { return a > b ? a : b; }
@@ -64,8 +103,9 @@ __device__ __forceinline__ R plm(R theta, R vl, R v0, R vr, R sgnv)
 
 // weno5 with the FD tables and the ImprovedShenZha10 indicator, A = 50
 // (src/Reconstruction.cpp:68-132, 154-169).  Right = true selects C2R
-// (value at the right edge of the centre cell), else C2L.  v[-2..2].
-template <typename R, bool Right>
+// (value at the right edge of the centre cell), else C2L.  v[-2..2].  D
+// supplies the division (Math<R>, or the relativistic law's Rel).
+template <typename R, bool Right, typename D = Math<R>>
 __device__ __forceinline__ R weno5(R vm2, R vm1, R v0, R vp1, R vp2)
 {
     // coefficient rows c[r][0..2] and weights d[r]
@@ -89,16 +129,16 @@ __device__ __forceinline__ R weno5(R vm2, R vm1, R v0, R vp1, R vp2)
     R B2 = R(13. / 12.) * (t2a * t2a) + R(1. / 4.) * (t2b * t2b);
     const R minB = min3(B0, B1, B2);
     const R maxB = max3(B0, B1, B2);
-    const R R0 = Math<R>::div(minB, maxB + eps_prime);
+    const R R0 = D::div(minB, maxB + eps_prime);
     B0 = R0 * A * minB + B0;
     B1 = R0 * A * minB + B1;
     B2 = R0 * A * minB + B2;
     const R e0 = eps_prime + B0, e1 = eps_prime + B1, e2 = eps_prime + B2;
-    const R w0 = Math<R>::div(d0, e0 * e0);
-    const R w1 = Math<R>::div(d1, e1 * e1);
-    const R w2 = Math<R>::div(d2, e2 * e2);
+    const R w0 = D::div(d0, e0 * e0);
+    const R w1 = D::div(d1, e1 * e1);
+    const R w2 = D::div(d2, e2 * e2);
     const R wtot = w0 + w1 + w2;
-    return Math<R>::div(w0 * vs0 + w1 * vs1 + w2 * vs2, wtot);
+    return D::div(w0 * vs0 + w1 * vs1 + w2 * vs2, wtot);
 }
 
 // ---------------------------------------------------------------------------
@@ -113,22 +153,23 @@ __device__ __forceinline__ R weno5(R vm2, R vm1, R v0, R vp1, R vp2)
 // t = 2 cbrt(g) and the double root -cbrt(g)).
 __device__ inline double cubic_largest_root(double A, double B, double C)
 {
-    const double shift = A / 3.0;
-    const double h = B / 3.0 - shift * shift;
-    const double g = 0.5 * (A * B / 3.0 - C) - shift * shift * shift;
+#pragma clang fp contract(fast)
+    const double shift = Rel::div(A, 3.0);
+    const double h = Rel::div(B, 3.0) - shift * shift;
+    const double g = 0.5 * (Rel::div(A * B, 3.0) - C) - shift * shift * shift;
     const double disc = h * h * h + g * g;
     double t;
     if (disc > 0.0)
     {
-        const double w = sqrt(disc);
+        const double w = Rel::sqrt(disc);
         t = cbrt(g + w) + cbrt(g - w);
     }
     else if (disc < 0.0)
     {
-        const double m = sqrt(-h);
-        double c = g / (m * m * m);
+        const double m = Rel::sqrt(-h);
+        double c = Rel::div(g, m * m * m);
         c = c > 1.0 ? 1.0 : (c < -1.0 ? -1.0 : c);
-        t = 2.0 * m * cos(acos(c) / 3.0);
+        t = 2.0 * m * cos(Rel::div(acos(c), 3.0));
     }
     else
     {
@@ -151,21 +192,23 @@ __device__ inline double cubic_largest_root(double A, double B, double C)
 // quadratic with a negative q contributes no real root.
 __device__ inline int quartic_real_roots(double k4, double k3, double k2, double k1, double k0, double roots[4])
 {
-    const double b = k3 / k4, c = k2 / k4, d = k1 / k4, e = k0 / k4;
+#pragma clang fp contract(fast)
+    const double ik4 = Rel::rcp(k4);
+    const double b = k3 * ik4, c = k2 * ik4, d = k1 * ik4, e = k0 * ik4;
     const double y = cubic_largest_root(-c, b * d - 4.0 * e, 4.0 * c * e - d * d - b * b * e);
     const double s2 = 0.25 * b * b + y - c;
-    const double s = s2 > 0.0 ? sqrt(s2) : 0.0;
+    const double s = s2 > 0.0 ? Rel::sqrt(s2) : 0.0;
     const double common = 0.75 * b * b - 2.0 * c;
     double qp, qm;
     if (s != 0.0)
     {
-        const double lin = 0.25 * (4.0 * b * c - 8.0 * d - b * b * b) / s;
+        const double lin = Rel::div(0.25 * (4.0 * b * c - 8.0 * d - b * b * b), s);
         qp = common - s2 + lin;
         qm = common - s2 - lin;
     }
     else
     {
-        const double w = 2.0 * sqrt(y * y - 4.0 * e);
+        const double w = 2.0 * Rel::sqrt(y * y - 4.0 * e);
         qp = common + w;
         qm = common - w;
     }
@@ -173,13 +216,13 @@ __device__ inline int quartic_real_roots(double k4, double k3, double k2, double
     int n = 0;
     if (qp >= 0.0)
     {
-        const double half = 0.5 * sqrt(qp);
+        const double half = 0.5 * Rel::sqrt(qp);
         roots[n++] = centre + 0.5 * s - half;
         roots[n++] = centre + 0.5 * s + half;
     }
     if (qm >= 0.0)
     {
-        const double half = 0.5 * sqrt(qm);
+        const double half = 0.5 * Rel::sqrt(qm);
         roots[n++] = centre - 0.5 * s - half;
         roots[n++] = centre - 0.5 * s + half;
     }
@@ -300,38 +343,39 @@ __device__ __forceinline__ void mhd_state(R gamma, const R* P, const R* dAA, Sta
 // Returns ERR_NONE or ERR_EIGEN.
 __device__ inline int rmhd_eigen(double gamma, const double* P, const double* dAA, double* A)
 {
+#pragma clang fp contract(fast)
     const double gm0 = gamma;
     const double gm1 = gamma - 1.0;
     const double vv = P[1] * P[1] + P[2] * P[2] + P[3] * P[3];
     const double BB = P[5] * P[5] + P[6] * P[6] + P[7] * P[7];
     const double Bv = P[5] * P[1] + P[6] * P[2] + P[7] * P[3];
     const double vn = P[1] * dAA[0] + P[2] * dAA[1] + P[3] * dAA[2];
-    const double W2 = 1.0 / (1.0 - vv);
-    const double W0 = sqrt(W2);
+    const double W2 = Rel::rcp(1.0 - vv);
+    const double W0 = Rel::sqrt(W2);
     const double b0 = W0 * Bv;
-    const double bb = (BB + b0 * b0) / W2;
-    const double b1 = (P[5] + b0 * W0 * P[1]) / W0;
-    const double b2 = (P[6] + b0 * W0 * P[2]) / W0;
-    const double b3 = (P[7] + b0 * W0 * P[3]) / W0;
+    const double bb = Rel::div(BB + b0 * b0, W2);
+    const double b1 = Rel::div(P[5] + b0 * W0 * P[1], W0);
+    const double b2 = Rel::div(P[6] + b0 * W0 * P[2], W0);
+    const double b3 = Rel::div(P[7] + b0 * W0 * P[3], W0);
     const double bn = b1 * dAA[0] + b2 * dAA[1] + b3 * dAA[2];
-    const double e0 = (P[4] / P[0]) / gm1;
+    const double e0 = Rel::div(Rel::div(P[4], P[0]), gm1);
     const double p0 = P[4];
-    const double h0 = 1.0 + e0 + p0 / P[0];
-    const double cs2 = gm0 * p0 / (P[0] * h0);
+    const double h0 = 1.0 + e0 + Rel::div(p0, P[0]);
+    const double cs2 = Rel::div(gm0 * p0, P[0] * h0);
     const double W4 = W2 * W2;
     const double v2 = vn * vn;
     const double v3 = vn * v2;
     const double v4 = vn * v3;
     if (BB == 0.0)
     {
-        const double K = sqrt(cs2 * (1 - vv) * (1 - vv * cs2 - v2 * (1 - cs2)));
-        A[0] = (vn * (1 - cs2) - K) / (1 - vv * cs2);
+        const double K = Rel::sqrt(cs2 * (1 - vv) * (1 - vv * cs2 - v2 * (1 - cs2)));
+        A[0] = Rel::div(vn * (1 - cs2) - K, 1 - vv * cs2);
         for (int n = 1; n < 7; ++n) A[n] = vn;
-        A[7] = (vn * (1 - cs2) + K) / (1 - vv * cs2);
+        A[7] = Rel::div(vn * (1 - cs2) + K, 1 - vv * cs2);
         return ERR_NONE;
     }
-    const double K = W4 * (P[0] * h0 * (1. / cs2 - 1.));
-    const double L = -W2 * (P[0] * h0 + bb / cs2);
+    const double K = W4 * (P[0] * h0 * (Rel::rcp(cs2) - 1.));
+    const double L = -W2 * (P[0] * h0 + Rel::div(bb, cs2));
     const double A4 = K - L - b0 * b0;
     const double A3 = -4 * K * vn + L * vn * 2 + 2 * b0 * bn;
     const double A2 = 6 * K * v2 + L * (1. - v2) + b0 * b0 - bn * bn;
@@ -339,28 +383,28 @@ __device__ inline int rmhd_eigen(double gamma, const double* P, const double* dA
     const double A0 = K * v4 + L * v2 + bn * bn;
     double roots[4];
     const int nr = quartic_real_roots(A4, A3, A2, A1, A0, roots);
-    const double C = sqrt(h0 + bb);
+    const double C = Rel::sqrt(h0 + bb);
     if (nr == 4)
     {
         A[0] = roots[0];
-        A[1] = (bn - C * vn * W0) / (b0 - C * W0);
+        A[1] = Rel::div(bn - C * vn * W0, b0 - C * W0);
         A[2] = roots[1];
         A[3] = vn;
         A[4] = vn;
         A[5] = roots[2];
-        A[6] = (bn + C * vn * W0) / (b0 + C * W0);
+        A[6] = Rel::div(bn + C * vn * W0, b0 + C * W0);
         A[7] = roots[3];
         return ERR_NONE;
     }
     if (nr == 2)
     {
         A[0] = roots[0];
-        A[1] = (bn - C * vn * W0) / (b0 - C * W0);
+        A[1] = Rel::div(bn - C * vn * W0, b0 - C * W0);
         A[2] = vn;
         A[3] = vn;
         A[4] = vn;
         A[5] = vn;
-        A[6] = (bn + C * vn * W0) / (b0 + C * W0);
+        A[6] = Rel::div(bn + C * vn * W0, b0 + C * W0);
         A[7] = roots[1];
         return ERR_NONE;
     }
@@ -372,24 +416,26 @@ __device__ inline int rmhd_eigen(double gamma, const double* P, const double* dA
 template <typename R>
 __device__ __forceinline__ int rmhd_state(R gamma, const R* P, const R* dAA, State<R, 8>& S)
 {
+#pragma clang fp contract(fast)
+    using RM = typename RelMath<R>::type;
     const R gm1 = gamma - R(1.0);
     const R vv = P[1] * P[1] + P[2] * P[2] + P[3] * P[3];
     const R BB = P[5] * P[5] + P[6] * P[6] + P[7] * P[7];
     const R Bv = P[5] * P[1] + P[6] * P[2] + P[7] * P[3];
     const R vn = P[1] * dAA[0] + P[2] * dAA[1] + P[3] * dAA[2];
     const R Bn = P[5] * dAA[0] + P[6] * dAA[1] + P[7] * dAA[2];
-    const R W2 = Math<R>::div(R(1.0), R(1.0) - vv);
-    const R W0 = Math<R>::sqrt(W2);
+    const R W2 = RM::div(R(1.0), R(1.0) - vv);
+    const R W0 = RM::sqrt(W2);
     const R b0 = W0 * Bv;
-    const R bb = Math<R>::div(BB + b0 * b0, W2);
-    const R b1 = Math<R>::div(P[5] + b0 * W0 * P[1], W0);
-    const R b2 = Math<R>::div(P[6] + b0 * W0 * P[2], W0);
-    const R b3 = Math<R>::div(P[7] + b0 * W0 * P[3], W0);
-    const R e0 = Math<R>::div(Math<R>::div(P[4], P[0]), gm1);
-    const R es = e0 + Math<R>::div(R(0.5) * bb, P[0]);
+    const R bb = RM::div(BB + b0 * b0, W2);
+    const R b1 = RM::div(P[5] + b0 * W0 * P[1], W0);
+    const R b2 = RM::div(P[6] + b0 * W0 * P[2], W0);
+    const R b3 = RM::div(P[7] + b0 * W0 * P[3], W0);
+    const R e0 = RM::div(RM::div(P[4], P[0]), gm1);
+    const R es = e0 + RM::div(R(0.5) * bb, P[0]);
     const R p0 = P[4];
     const R ps = p0 + R(0.5) * bb;
-    const R hs = R(1.0) + es + Math<R>::div(ps, P[0]);
+    const R hs = R(1.0) + es + RM::div(ps, P[0]);
     if (vv >= R(1)) return ERR_SUPERLUMINAL;
 #pragma unroll
     for (int q = 0; q < 8; ++q) S.P[q] = P[q];
@@ -402,10 +448,10 @@ __device__ __forceinline__ int rmhd_state(R gamma, const R* P, const R* dAA, Sta
     S.U[6] = P[6];
     S.U[7] = P[7];
     S.F[0] = vn * S.U[0];
-    S.F[1] = vn * S.U[1] - Math<R>::div(Bn * b1, W0) + ps * dAA[0];
-    S.F[2] = vn * S.U[2] - Math<R>::div(Bn * b2, W0) + ps * dAA[1];
-    S.F[3] = vn * S.U[3] - Math<R>::div(Bn * b3, W0) + ps * dAA[2];
-    S.F[4] = vn * S.U[4] - Math<R>::div(Bn * b0, W0) + ps * vn;
+    S.F[1] = vn * S.U[1] - RM::div(Bn * b1, W0) + ps * dAA[0];
+    S.F[2] = vn * S.U[2] - RM::div(Bn * b2, W0) + ps * dAA[1];
+    S.F[3] = vn * S.U[3] - RM::div(Bn * b3, W0) + ps * dAA[2];
+    S.F[4] = vn * S.U[4] - RM::div(Bn * b0, W0) + ps * vn;
     S.F[5] = vn * S.U[5] - Bn * P[1];
     S.F[6] = vn * S.U[6] - Bn * P[2];
     S.F[7] = vn * S.U[7] - Bn * P[3];
@@ -494,7 +540,7 @@ __device__ __forceinline__ R min_elem(const R* A)
     return m;
 }
 
-template <typename R, int NQ>
+template <typename R, int NQ, typename D = Math<R>>
 __device__ __forceinline__ void hlle(const State<R, NQ>& L, const State<R, NQ>& Rs, R* F)
 {
     const R epl = max_elem<R, NQ>(L.A);
@@ -505,7 +551,7 @@ __device__ __forceinline__ void hlle(const State<R, NQ>& L, const State<R, NQ>& 
     const R am = min3(R(0.0), eml, emr);
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
-        F[q] = Math<R>::div(ap * L.F[q] - am * Rs.F[q] + ap * am * (Rs.U[q] - L.U[q]), ap - am);
+        F[q] = D::div(ap * L.F[q] - am * Rs.F[q] + ap * am * (Rs.U[q] - L.U[q]), ap - am);
 }
 
 template <typename R>
@@ -610,22 +656,23 @@ __device__ __forceinline__ int prim_to_cons(double gamma, const double* P, doubl
     }
     else
     {
+#pragma clang fp contract(fast)
         const double gm1 = gamma - 1.0;
         const double vv = P[1] * P[1] + P[2] * P[2] + P[3] * P[3];
         const double BB = P[5] * P[5] + P[6] * P[6] + P[7] * P[7];
         const double Bv = P[5] * P[1] + P[6] * P[2] + P[7] * P[3];
-        const double W2 = 1.0 / (1.0 - vv);
-        const double W0 = sqrt(W2);
+        const double W2 = Rel::rcp(1.0 - vv);
+        const double W0 = Rel::sqrt(W2);
         const double b0 = W0 * Bv;
-        const double bb = (BB + b0 * b0) / W2;
-        const double b1 = (P[5] + b0 * W0 * P[1]) / W0;
-        const double b2 = (P[6] + b0 * W0 * P[2]) / W0;
-        const double b3 = (P[7] + b0 * W0 * P[3]) / W0;
-        const double e0 = (P[4] / P[0]) / gm1;
-        const double es = e0 + 0.5 * bb / P[0];
+        const double bb = Rel::div(BB + b0 * b0, W2);
+        const double b1 = Rel::div(P[5] + b0 * W0 * P[1], W0);
+        const double b2 = Rel::div(P[6] + b0 * W0 * P[2], W0);
+        const double b3 = Rel::div(P[7] + b0 * W0 * P[3], W0);
+        const double e0 = Rel::div(Rel::div(P[4], P[0]), gm1);
+        const double es = e0 + Rel::div(0.5 * bb, P[0]);
         const double p0 = P[4];
         const double ps = p0 + 0.5 * bb;
-        const double hs = 1.0 + es + ps / P[0];
+        const double hs = 1.0 + es + Rel::div(ps, P[0]);
         if (vv >= 1) return ERR_SUPERLUMINAL;
         U[0] = P[0] * W0;
         U[4] = P[0] * hs * W2 - ps - b0 * b0 - U[0];
@@ -676,13 +723,14 @@ __device__ __forceinline__ int inversion_check_input(const double* U)
 // the primitives of a converged (Z, W)
 __device__ __forceinline__ int inversion_primitives(const Inversion& s, double Z, double W, double* Pout)
 {
+#pragma clang fp contract(fast)
     double P[8];
-    const double bw = s.BS * W / Z;
-    P[0] = s.D / W;
-    P[4] = (s.D / W) * (Z / (s.D * W) - 1.0) * s.k_gas;
-    P[1] = (s.U[1] + bw * s.U[5] / W) / (Z + s.B2);
-    P[2] = (s.U[2] + bw * s.U[6] / W) / (Z + s.B2);
-    P[3] = (s.U[3] + bw * s.U[7] / W) / (Z + s.B2);
+    const double bw = Rel::div(s.BS * W, Z);
+    P[0] = Rel::div(s.D, W);
+    P[4] = Rel::div(s.D, W) * (Rel::div(Z, s.D * W) - 1.0) * s.k_gas;
+    P[1] = Rel::div(s.U[1] + Rel::div(bw * s.U[5], W), Z + s.B2);
+    P[2] = Rel::div(s.U[2] + Rel::div(bw * s.U[6], W), Z + s.B2);
+    P[3] = Rel::div(s.U[3] + Rel::div(bw * s.U[7], W), Z + s.B2);
     P[5] = s.U[5];
     P[6] = s.U[6];
     P[7] = s.U[7];
@@ -701,6 +749,7 @@ __device__ __forceinline__ int inversion_primitives(const Inversion& s, double Z
 // r_en (energy: tau) in (Z, W), with the inverse Jacobian written out
 __device__ inline int invert_zw(const Inversion& s, double* P)
 {
+#pragma clang fp contract(fast)
     if (int bad = inversion_check_input(s.U)) return bad;
     int iterations = 0;
     double step = 1.0;
@@ -712,16 +761,16 @@ __device__ inline int invert_zw(const Inversion& s, double* P)
         const double Z3 = Z * Z2;
         const double W2 = W * W;
         const double W3 = W * W2;
-        const double p = (D / W) * (Z / (D * W) - 1.0) * kg;
-        const double J00 = 2 * (B2 + Z) * (BS2 * W2 + (W2 - 1) * Z3) / (W2 * Z3);
-        const double J01 = 2 * (B2 + Z) * (B2 + Z) / W3;
-        const double J10 = 1.0 + BS2 / Z3 - kg / W2;
-        const double J11 = B2 / W3 + (2 * Z - D * W) / W3 * kg;
-        const double r_mom = -S2 + (Z + B2) * (Z + B2) * (W2 - 1) / W2 - (2 * Z + B2) * BS2 / Z2;
-        const double r_en = -tau + Z + B2 - p - 0.5 * B2 / W2 - 0.5 * BS2 / Z2 - D;
+        const double p = Rel::div(D, W) * (Rel::div(Z, D * W) - 1.0) * kg;
+        const double J00 = Rel::div(2 * (B2 + Z) * (BS2 * W2 + (W2 - 1) * Z3), W2 * Z3);
+        const double J01 = Rel::div(2 * (B2 + Z) * (B2 + Z), W3);
+        const double J10 = 1.0 + Rel::div(BS2, Z3) - Rel::div(kg, W2);
+        const double J11 = Rel::div(B2, W3) + Rel::div(2 * Z - D * W, W3) * kg;
+        const double r_mom = -S2 + Rel::div((Z + B2) * (Z + B2) * (W2 - 1), W2) - Rel::div((2 * Z + B2) * BS2, Z2);
+        const double r_en = -tau + Z + B2 - p - Rel::div(0.5 * B2, W2) - Rel::div(0.5 * BS2, Z2) - D;
         const double det = J00 * J11 - J01 * J10;
-        const double K00 = J11 / det, K01 = -J01 / det;
-        const double K10 = -J10 / det, K11 = J00 / det;
+        const double K00 = Rel::div(J11, det), K01 = Rel::div(-J01, det);
+        const double K10 = Rel::div(-J10, det), K11 = Rel::div(J00, det);
         const double dZ = -(K00 * r_mom + K01 * r_en);
         const double dW = -(K10 * r_mom + K11 * r_en);
         double Zn = Z + dZ;
@@ -732,7 +781,7 @@ __device__ inline int invert_zw(const Inversion& s, double* P)
         Wn = (Wn < 1e12) ? Wn : 1e12;
         Z = Zn;
         W = Wn;
-        step = fabs(dZ / Z) + fabs(dW / W);
+        step = fabs(Rel::div(dZ, Z)) + fabs(Rel::div(dW, W));
         ++iterations;
         if (iterations == 250) return 8;
     }
@@ -743,6 +792,7 @@ __device__ inline int invert_zw(const Inversion& s, double* P)
 // momentum relation v^2 = num / den
 __device__ inline int invert_z(const Inversion& s, double* P)
 {
+#pragma clang fp contract(fast)
     if (int bad = inversion_check_input(s.U)) return bad;
     int iterations = 0;
     double step = 1.0;
@@ -756,33 +806,33 @@ __device__ inline int invert_z(const Inversion& s, double* P)
         const double den = (B2 + Z) * (B2 + Z) * Z2;
         const double dnum = 2 * (S2 * Z + BS2);
         const double dden = 2 * Z * (B2 + Z) * (B2 + 2 * Z);
-        const double v2 = num / den;
-        const double W2 = 1.0 / (1.0 - v2);
-        const double W = sqrt(W2);
+        const double v2 = Rel::div(num, den);
+        const double W2 = Rel::rcp(1.0 - v2);
+        const double W = Rel::sqrt(W2);
         const double W3 = W * W2;
-        const double p = (D / W) * (Z / (D * W) - 1.0) * kg;
-        const double dv2_dZ = (dnum * den - dden * num) / (den * den);
-        const double dp_dZ_atW = kg / W2;
-        const double dp_dW_atZ = kg * (D / W2 - 2 * Z / W3);
+        const double p = Rel::div(D, W) * (Rel::div(Z, D * W) - 1.0) * kg;
+        const double dv2_dZ = Rel::div(dnum * den - dden * num, den * den);
+        const double dp_dZ_atW = Rel::div(kg, W2);
+        const double dp_dW_atZ = kg * (Rel::div(D, W2) - Rel::div(2 * Z, W3));
         const double dW_dv2 = 0.5 * W3;
         const double dp_dZ = dp_dW_atZ * dW_dv2 * dv2_dZ + dp_dZ_atW;
-        const double r = tau + D - 0.5 * B2 * (1 + v2) + 0.5 * BS2 / Z2 - Z + p;
-        const double dr = -0.5 * B2 * dv2_dZ - BS2 / Z3 - 1.0 + dp_dZ;
-        const double dZ = -r / dr;
+        const double r = tau + D - 0.5 * B2 * (1 + v2) + Rel::div(0.5 * BS2, Z2) - Z + p;
+        const double dr = -0.5 * B2 * dv2_dZ - Rel::div(BS2, Z3) - 1.0 + dp_dZ;
+        const double dZ = Rel::div(-r, dr);
         double Zn = Z + dZ;
         Zn = (Zn > 0.0) ? Zn : -Zn;
         Zn = (Zn < 1e20) ? Zn : Z;
         Z = Zn;
-        step = fabs(dZ / Z);
+        step = fabs(Rel::div(dZ, Z));
         ++iterations;
         if (iterations == 250) return 8;
     }
     const double Z2 = Z * Z;
     const double num = S2 * Z2 + BS2 * (B2 + 2 * Z);
     const double den = (B2 + Z) * (B2 + Z) * Z2;
-    const double v2 = num / den;
-    const double W2 = 1.0 / (1.0 - v2);
-    const double W = sqrt(W2);
+    const double v2 = Rel::div(num, den);
+    const double W2 = Rel::rcp(1.0 - v2);
+    const double W = Rel::sqrt(W2);
     return inversion_primitives(s, Z, W, P);
 }
 
@@ -838,6 +888,7 @@ __device__ __forceinline__ int cons_to_prim(double gamma, double pfloor, const d
     }
     else
     {
+#pragma clang fp contract(fast)
         Inversion s;
         s.k_gas = (gamma - 1.0) / gamma;
         s.pfloor = pfloor;
@@ -849,8 +900,8 @@ __device__ __forceinline__ int cons_to_prim(double gamma, double pfloor, const d
         s.BS2 = s.BS * s.BS;
 #pragma unroll
         for (int i = 0; i < 8; ++i) s.U[i] = U[i];
-        s.Z0 = sqrt(s.S2 + s.D * s.D);
-        s.W0 = s.Z0 / s.D;
+        s.Z0 = Rel::sqrt(s.S2 + s.D * s.D);
+        s.W0 = Rel::div(s.Z0, s.D);
         if (invert_zw(s, P) == 0) return 0;
         if (invert_z(s, P) == 0) return 0;
         return 1;
