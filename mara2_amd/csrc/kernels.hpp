@@ -649,17 +649,20 @@ __device__ __forceinline__ double divergence_ct(const Geom& G, double x1, double
 // U = U0 (1-b) + (U + dt L) b (src/SolutionSchemes.cpp:232-235), and
 // recoverPrimitive (src/FieldOperator.cpp:41-82) on interior zones.
 // U0 is regenerated from the step's starting primitives P0.
-template <int Law, typename R, bool CT>
-__device__ __forceinline__ double update_zone(const double* __restrict__ P0, double* __restrict__ U, double* __restrict__ Pout,
-                                              const R* __restrict__ Fx, const R* __restrict__ Fy, const R* __restrict__ Fz,
-                                              const Geom& G, double b, int first_stage,
-                                              int write_u, int courant_out, Ctl* ctl, int i, int j, int k,
-                                              const double* __restrict__ Ssrc)
+// Full3: the block has all three axes (a compile-time fact in that copy, so
+// every flux the zone reads is loaded without a branch and the loads issue
+// together).
+template <int Law, typename R, bool CT, bool Full3>
+__device__ __forceinline__ double update_zone_t(const double* __restrict__ P0, double* __restrict__ U, double* __restrict__ Pout,
+                                                const R* __restrict__ Fx, const R* __restrict__ Fy, const R* __restrict__ Fz,
+                                                const Geom& G, double b, int first_stage,
+                                                int write_u, int courant_out, Ctl* ctl, int i, int j, int k,
+                                                const double* __restrict__ Ssrc)
 {
     constexpr int NQ = LawNQ<Law>::value;
     const long long f = face_index(G, i, j, k);
     const long long fx = f + 1, fy = f + (G.N[0] + 1), fz = f + (long long)(G.N[0] + 1) * (G.N[1] + 1);
-    const bool hx = G.N[0] > 1, hy = G.N[1] > 1, hz = G.N[2] > 1;
+    const bool hx = Full3 || G.N[0] > 1, hy = Full3 || G.N[1] > 1, hz = Full3 || G.N[2] > 1;
     CellAreas A{};
     if (G.sph) A = spherical_areas(G, i, j, k);
     double L[NQ];
@@ -686,6 +689,20 @@ __device__ __forceinline__ double update_zone(const double* __restrict__ P0, dou
         }
     }
     return finish_zone<Law, R>(P0, U, Pout, G, b, first_stage, write_u, courant_out, ctl, i, j, k, L, Ssrc);
+}
+
+template <int Law, typename R, bool CT>
+__device__ __forceinline__ double update_zone(const double* __restrict__ P0, double* __restrict__ U, double* __restrict__ Pout,
+                                              const R* __restrict__ Fx, const R* __restrict__ Fy, const R* __restrict__ Fz,
+                                              const Geom& G, double b, int first_stage,
+                                              int write_u, int courant_out, Ctl* ctl, int i, int j, int k,
+                                              const double* __restrict__ Ssrc)
+{
+    if (G.N[0] > 1 && G.N[1] > 1 && G.N[2] > 1)
+        return update_zone_t<Law, R, CT, true>(P0, U, Pout, Fx, Fy, Fz, G, b, first_stage, write_u, courant_out, ctl,
+                                               i, j, k, Ssrc);
+    return update_zone_t<Law, R, CT, false>(P0, U, Pout, Fx, Fy, Fz, G, b, first_stage, write_u, courant_out, ctl, i,
+                                            j, k, Ssrc);
 }
 
 // ---------------------------------------------------------------------------
