@@ -65,13 +65,31 @@ __device__ __forceinline__ void mhd2d_tile(const double* __restrict__ Pin, const
     __shared__ R sP[STAGE ? NQ : 1][STAGE ? HY : 1][STAGE ? HX : 1];
     if constexpr (STAGE)
     {
-        for (int t = tid; t < HX * HY; t += NT)
-        {
-            const int hx = t % HX, hy = t / HX;
-            const int gi = min(i0 - NG + hx, G.N[0] - 1), gj = min(j0 - NG + hy, G.N[1] - 1);
-            const long long c = cell_index(G, gi, gj, k);
+        // every load issued before the first LDS write waits on one
+        constexpr int PH = (HX * HY + NT - 1) / NT;
+        R hv[PH][NQ];
 #pragma unroll
-            for (int q = 0; q < NQ; ++q) sP[q][hy][hx] = R(Pin[q * G.fs + c]);
+        for (int p = 0; p < PH; ++p)
+        {
+            const int t = tid + p * NT;
+            if ((p + 1) * NT <= HX * HY || t < HX * HY)
+            {
+                const int hx = t % HX, hy = t / HX;
+                const int gi = min(i0 - NG + hx, G.N[0] - 1), gj = min(j0 - NG + hy, G.N[1] - 1);
+                const long long c = cell_index(G, gi, gj, k);
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) hv[p][q] = R(Pin[q * G.fs + c]);
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < PH; ++p)
+        {
+            const int t = tid + p * NT;
+            if ((p + 1) * NT <= HX * HY || t < HX * HY)
+            {
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) (&sP[q][0][0])[t] = hv[p][q];
+            }
         }
         __syncthreads();
     }
