@@ -298,7 +298,9 @@ k_fused_hydro(const double* __restrict__ Pin, const double* __restrict__ P0, dou
 #pragma unroll
             for (int q = 0; q < 5; ++q) Fzlo[q] = Fzhi[q];
         }
-        __syncthreads();
+        // the update reads registers and fy / fxe only, which the next plane
+        // rewrites after another barrier every wave must reach: the next
+        // plane's staging need not wait here (a 2-D tile has one plane)
     }
     if (courant_out) courant_to_bins(dmin, wmin, TX * TY / 64, tid, ctl);
 }
@@ -622,7 +624,9 @@ __device__ __forceinline__ void fused_plm_tile(const double* __restrict__ Pin, c
 #pragma unroll
             for (int q = 0; q < 5; ++q) Fzlo[q] = Fzhi[q];
         }
-        __syncthreads();
+        // the update reads registers and fy / fxe only, which the next plane
+        // rewrites after another barrier every wave must reach: the next
+        // plane's staging need not wait here (a 2-D tile has one plane)
     }
     if (courant_out) courant_to_bins(dmin, wmin, TX * TY / 64, tid, ctl);
 }

@@ -547,7 +547,11 @@ __device__ __forceinline__ void fast_plm_tile(const double* __restrict__ Pin, co
 #pragma unroll
             for (int q = 0; q < 5; ++q) Fzlo[q] = Fzhi[q];
         }
-        __syncthreads();
+        // In 3-D the update reads only registers and fy / fxe, which the next
+        // plane rewrites after two more barriers every wave must reach, so the
+        // next plane's staging need not wait here; the 2-D update reads the
+        // plane itself.
+        if constexpr (!Z) __syncthreads();
     }
     if (courant_out) courant_to_bins(dmin, wmin, NT / 64, tid, ctl);
 }
