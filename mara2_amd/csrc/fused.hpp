@@ -305,6 +305,78 @@ k_fused_hydro(const double* __restrict__ Pin, const double* __restrict__ P0, dou
     if (courant_out) courant_to_bins(dmin, wmin, TX * TY / 64, tid, ctl);
 }
 
+// The zones of plane k that a TX x TY tile (halo NG = 2) stages from HBM into
+// its LDS plane[5][TY + 4][TX + 4]: in 3-D the ring around the tile (its own
+// zones come from the z window), else the whole plane; zones past the block
+// are clamped to its last zone and feed no face that is used.  load() issues
+// every load of the plane before store() waits on one, so staging a plane
+// costs one memory latency rather than one per pass of a strided loop.
+template <typename R, int TX, int TY, bool Z>
+struct PlaneRing
+{
+    static constexpr int NG = 2, HX = TX + 2 * NG, HY = TY + 2 * NG, NT = TX * TY;
+    static constexpr int NH = Z ? HX * HY - NT : HX * HY;
+    static constexpr int PH = (NH + NT - 1) / NT;
+    R v[PH][5];
+    int pos[PH];
+
+    __device__ __forceinline__ void load(const double* __restrict__ Pin, const Geom& G, long long pbase, int i0,
+                                         int j0, int tid)
+    {
+#pragma unroll
+        for (int p = 0; p < PH; ++p)
+        {
+            const int h = tid + p * NT;
+            pos[p] = -1;
+            if ((p + 1) * NT <= NH || h < NH)
+            {
+                int hx, hy;
+                if constexpr (Z)
+                {
+                    // rows 0, 1 and TY + 2, TY + 3 whole; columns 0, 1,
+                    // HX - 2, HX - 1 of rows 2 .. TY + 1
+                    if (h < 4 * HX)
+                    {
+                        const int r = h / HX;
+                        hy = r < 2 ? r : TY + r;
+                        hx = h - r * HX;
+                    }
+                    else
+                    {
+                        const int e = h - 4 * HX, c = e & 3;
+                        hy = NG + (e >> 2);
+                        hx = c < 2 ? c : HX - 4 + c;
+                    }
+                }
+                else
+                {
+                    hy = h / HX;
+                    hx = h - hy * HX;
+                }
+                int off = min(j0 - NG + hy, G.N[1] - 1) * G.N[0] + min(i0 - NG + hx, G.N[0] - 1);
+                // an opaque copy: the offset is recomputed per plane rather
+                // than its five 64-bit field addresses held across the march
+                asm volatile("" : "+v"(off));
+                const double* src = Pin + pbase + off;
+#pragma unroll
+                for (int q = 0; q < 5; ++q) v[p][q] = R(src[q * G.fs]);
+                pos[p] = hy * HX + hx;
+            }
+        }
+    }
+
+    __device__ __forceinline__ void store(R (*plane)[HY][HX]) const
+    {
+#pragma unroll
+        for (int p = 0; p < PH; ++p)
+            if (pos[p] >= 0)
+            {
+#pragma unroll
+                for (int q = 0; q < 5; ++q) (&plane[q][0][0])[pos[p]] = v[p][q];
+            }
+    }
+};
+
 // ---------------------------------------------------------------------------
 // PLM variant: every zone's half-slope h = 0.5 * minmod (one per axis and
 // field) is computed once and shared by its two faces, where the reference
@@ -417,33 +489,23 @@ __device__ __forceinline__ void fused_plm_tile(const double* __restrict__ Pin, c
     for (int k = kb; k < ke; ++k)
     {
         const long long pbase = (long long)k * sxy;
+        // the zone's offset in the plane, opaque per plane like the ring's
+        int colk = (int)col;
+        asm volatile("" : "+v"(colk));
+        // plane k: the ring from memory and (3-D) the own zone from the z
+        // window, loaded two planes ago; the window's next value is loaded
+        // with the ring
+        PlaneRing<R, TX, TY, Z> ring;
+        ring.load(Pin, G, pbase, i0, j0, tid);
+        R wn[5];
         if constexpr (Z)
         {
-            // own zone of plane k from the z window (loaded two planes ago),
-            // only the halo ring from memory
+#pragma unroll
+            for (int q = 0; q < 5; ++q) wn[q] = R(Pin[q * G.fs + colk + (long long)(k + 2) * sxy]);
 #pragma unroll
             for (int q = 0; q < 5; ++q) plane[q][row + NG][lane + NG] = W[q][2];
-            for (int t = tid; t < HX * HY; t += NT)
-            {
-                const int hx = t % HX, hy = t / HX;
-                if (hx >= NG && hx < NG + TX && hy >= NG && hy < NG + TY) continue;
-                const int gi = min(i0 - NG + hx, G.N[0] - 1), gj = min(j0 - NG + hy, G.N[1] - 1);
-                const long long c = pbase + (long long)gj * G.N[0] + gi;
-#pragma unroll
-                for (int q = 0; q < 5; ++q) plane[q][hy][hx] = R(Pin[q * G.fs + c]);
-            }
         }
-        else
-        {
-            for (int t = tid; t < HX * HY; t += NT)
-            {
-                const int hx = t % HX, hy = t / HX;
-                const int gi = min(i0 - NG + hx, G.N[0] - 1), gj = min(j0 - NG + hy, G.N[1] - 1);
-                const long long c = pbase + (long long)gj * G.N[0] + gi;
-#pragma unroll
-                for (int q = 0; q < 5; ++q) plane[q][hy][hx] = R(Pin[q * G.fs + c]);
-            }
-        }
+        ring.store(plane);
         // z: slide the window to k-1..k+2, half-slope of zone k+1, upper z-face
         R Fzhi[5];
         if constexpr (Z)
@@ -455,7 +517,7 @@ __device__ __forceinline__ void fused_plm_tile(const double* __restrict__ Pin, c
                 W[q][0] = W[q][1];
                 W[q][1] = W[q][2];
                 W[q][2] = W[q][3];
-                W[q][3] = R(Pin[q * G.fs + col + (long long)(k + 2) * sxy]);
+                W[q][3] = wn[q];
                 const R hn = half_slope(theta, W[q][1], W[q][2], W[q][3]);
                 PL[q] = W[q][1] + hzc[q];
                 PR[q] = W[q][2] + (-hn);
@@ -583,7 +645,7 @@ __device__ __forceinline__ void fused_plm_tile(const double* __restrict__ Pin, c
         }
         if (cell_xy)
         {
-            const long long c = pbase + col;
+            const long long c = pbase + colk;
             double p0[5], U0[5];
 #pragma unroll
             for (int q = 0; q < 5; ++q) p0[q] = P0[q * G.fs + c];

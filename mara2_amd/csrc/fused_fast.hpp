@@ -286,60 +286,16 @@ __device__ __forceinline__ void fast_plm_tile(const double* __restrict__ Pin, co
         face_flux<Riem, 2>(K, PL, PR, Fzlo);
     }
 
-    // The zones of plane k staged from HBM: in 3-D the ring around the tile
-    // (its own zones come from the z window), else the whole plane.  Every
-    // load of a plane (ring, the z window's next value) is issued before the
-    // first LDS write waits on one, so the plane costs one memory latency.
-    constexpr int NH = Z ? HX * HY - NT : HX * HY;
-    constexpr int PH = (NH + NT - 1) / NT;
     for (int k = kb; k < ke; ++k)
     {
         const long long pbase = (long long)k * sxy;
         // the zone's offset in the plane, opaque per plane like the ring's
         int colk = (int)col;
         asm volatile("" : "+v"(colk));
-        R hv[PH][5];
-        int hpos[PH];
-#pragma unroll
-        for (int p = 0; p < PH; ++p)
-        {
-            const int h = tid + p * NT;
-            hpos[p] = -1;
-            if ((p + 1) * NT <= NH || h < NH)
-            {
-                int hx, hy;
-                if constexpr (Z)
-                {
-                    // rows 0, 1 and TY + 2, TY + 3 whole; columns 0, 1, HX - 2,
-                    // HX - 1 of rows 2 .. TY + 1
-                    if (h < 4 * HX)
-                    {
-                        const int r = h / HX;
-                        hy = r < 2 ? r : TY + r;
-                        hx = h - r * HX;
-                    }
-                    else
-                    {
-                        const int e = h - 4 * HX, c = e & 3;
-                        hy = NG + (e >> 2);
-                        hx = c < 2 ? c : HX - 4 + c;
-                    }
-                }
-                else
-                {
-                    hy = h / HX;
-                    hx = h - hy * HX;
-                }
-                int off = min(j0 - NG + hy, G.N[1] - 1) * G.N[0] + min(i0 - NG + hx, G.N[0] - 1);
-                // an opaque copy: the offset is recomputed per plane rather
-                // than its five 64-bit field addresses held across the march
-                asm volatile("" : "+v"(off));
-                const double* src = Pin + pbase + off;
-#pragma unroll
-                for (int q = 0; q < 5; ++q) hv[p][q] = R(src[q * G.fs]);
-                hpos[p] = hy * HX + hx;
-            }
-        }
+        // plane k: the ring from memory (PlaneRing) and (3-D) the own zone
+        // from the z window; the window's next value is loaded with the ring
+        PlaneRing<R, TX, TY, Z> ring;
+        ring.load(Pin, G, pbase, i0, j0, tid);
         R wn[5];
         if constexpr (Z)
         {
@@ -348,13 +304,7 @@ __device__ __forceinline__ void fast_plm_tile(const double* __restrict__ Pin, co
 #pragma unroll
             for (int q = 0; q < 5; ++q) plane[q][row + NG][lane + NG] = W[q][2];
         }
-#pragma unroll
-        for (int p = 0; p < PH; ++p)
-            if (hpos[p] >= 0)
-            {
-#pragma unroll
-                for (int q = 0; q < 5; ++q) (&plane[q][0][0])[hpos[p]] = hv[p][q];
-            }
+        ring.store(plane);
         R Fzhi[5];
         if constexpr (Z)
         {
