@@ -23,6 +23,8 @@ template <> struct Math<double>
 {
     static __device__ __forceinline__ double sqrt(double x) { return __builtin_sqrt(x); }
     static __device__ __forceinline__ double div(double a, double b) { return a / b; }
+    // a / b where b is positive and finite (the same division here)
+    static __device__ __forceinline__ double div_pos(double a, double b) { return a / b; }
     static __device__ __forceinline__ double abs(double x) { return __builtin_fabs(x); }
 };
 
@@ -32,6 +34,7 @@ template <> struct Math<float>
 {
     static __device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
     static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+    static __device__ __forceinline__ float div_pos(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
     static __device__ __forceinline__ float abs(float x) { return __builtin_fabsf(x); }
 };
 
@@ -53,6 +56,14 @@ struct Rel
         return __builtin_isfinite(y) ? y : y0;
     }
     static __device__ __forceinline__ double div(double a, double b) { return a * rcp(b); }
+    // a / b for b known positive and finite (WENO5's weights, HLLE's wave-speed
+    // spread): the refined reciprocal without the zero and infinity fix-up,
+    // which such b never needs (a zero spread gives NaN either way)
+    static __device__ __forceinline__ double div_pos(double a, double b)
+    {
+        const double y0 = __builtin_amdgcn_rcp(b);
+        return a * __builtin_fma(y0, __builtin_fma(-b, y0, 1.0), y0);
+    }
     // sqrt(x) as x / sqrt(x); 0 and +inf as they are, NaN below 0
     static __device__ __forceinline__ double sqrt(double x)
     {
@@ -124,16 +135,16 @@ __device__ __forceinline__ R weno5(R vm2, R vm1, R v0, R vp1, R vp2)
     R B2 = R(13. / 12.) * (t2a * t2a) + R(1. / 4.) * (t2b * t2b);
     const R minB = min3(B0, B1, B2);
     const R maxB = max3(B0, B1, B2);
-    const R R0 = D::div(minB, maxB + eps_prime);
+    const R R0 = D::div_pos(minB, maxB + eps_prime);
     B0 = R0 * A * minB + B0;
     B1 = R0 * A * minB + B1;
     B2 = R0 * A * minB + B2;
     const R e0 = eps_prime + B0, e1 = eps_prime + B1, e2 = eps_prime + B2;
-    const R w0 = D::div(d0, e0 * e0);
-    const R w1 = D::div(d1, e1 * e1);
-    const R w2 = D::div(d2, e2 * e2);
+    const R w0 = D::div_pos(d0, e0 * e0);
+    const R w1 = D::div_pos(d1, e1 * e1);
+    const R w2 = D::div_pos(d2, e2 * e2);
     const R wtot = w0 + w1 + w2;
-    return D::div(w0 * vs0 + w1 * vs1 + w2 * vs2, wtot);
+    return D::div_pos(w0 * vs0 + w1 * vs1 + w2 * vs2, wtot);
 }
 
 // ---------------------------------------------------------------------------
@@ -546,7 +557,7 @@ __device__ __forceinline__ void hlle(const State<R, NQ>& L, const State<R, NQ>& 
     const R am = min3(R(0.0), eml, emr);
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
-        F[q] = D::div(ap * L.F[q] - am * Rs.F[q] + ap * am * (Rs.U[q] - L.U[q]), ap - am);
+        F[q] = D::div_pos(ap * L.F[q] - am * Rs.F[q] + ap * am * (Rs.U[q] - L.U[q]), ap - am);
 }
 
 template <typename R>
