@@ -160,9 +160,9 @@ __device__ __forceinline__ R weno5(R vm2, R vm1, R v0, R vp1, R vp2)
 __device__ inline double cubic_largest_root(double A, double B, double C)
 {
 #pragma clang fp contract(fast)
-    const double shift = Rel::div(A, 3.0);
-    const double h = Rel::div(B, 3.0) - shift * shift;
-    const double g = 0.5 * (Rel::div(A * B, 3.0) - C) - shift * shift * shift;
+    const double shift = Rel::div_pos(A, 3.0);
+    const double h = Rel::div_pos(B, 3.0) - shift * shift;
+    const double g = 0.5 * (Rel::div_pos(A * B, 3.0) - C) - shift * shift * shift;
     const double disc = h * h * h + g * g;
     double t;
     if (disc > 0.0)
@@ -173,9 +173,9 @@ __device__ inline double cubic_largest_root(double A, double B, double C)
     else if (disc < 0.0)
     {
         const double m = Rel::sqrt(-h);
-        double c = Rel::div(g, m * m * m);
+        double c = Rel::div_pos(g, m * m * m);
         c = c > 1.0 ? 1.0 : (c < -1.0 ? -1.0 : c);
-        t = 2.0 * m * cos(Rel::div(acos(c), 3.0));
+        t = 2.0 * m * cos(Rel::div_pos(acos(c), 3.0));
     }
     else
     {
@@ -208,7 +208,7 @@ __device__ inline int quartic_real_roots(double k4, double k3, double k2, double
     double qp, qm;
     if (s != 0.0)
     {
-        const double lin = Rel::div(0.25 * (4.0 * b * c - 8.0 * d - b * b * b), s);
+        const double lin = Rel::div_pos(0.25 * (4.0 * b * c - 8.0 * d - b * b * b), s);
         qp = common - s2 + lin;
         qm = common - s2 - lin;
     }
@@ -356,18 +356,18 @@ __device__ inline int rmhd_eigen(double gamma, const double* P, const double* dA
     const double BB = P[5] * P[5] + P[6] * P[6] + P[7] * P[7];
     const double Bv = P[5] * P[1] + P[6] * P[2] + P[7] * P[3];
     const double vn = P[1] * dAA[0] + P[2] * dAA[1] + P[3] * dAA[2];
-    const double W2 = Rel::rcp(1.0 - vv);
+    const double W2 = Rel::div_pos(1.0, 1.0 - vv);
     const double W0 = Rel::sqrt(W2);
     const double b0 = W0 * Bv;
-    const double bb = Rel::div(BB + b0 * b0, W2);
-    const double b1 = Rel::div(P[5] + b0 * W0 * P[1], W0);
-    const double b2 = Rel::div(P[6] + b0 * W0 * P[2], W0);
-    const double b3 = Rel::div(P[7] + b0 * W0 * P[3], W0);
+    const double bb = Rel::div_pos(BB + b0 * b0, W2);
+    const double b1 = Rel::div_pos(P[5] + b0 * W0 * P[1], W0);
+    const double b2 = Rel::div_pos(P[6] + b0 * W0 * P[2], W0);
+    const double b3 = Rel::div_pos(P[7] + b0 * W0 * P[3], W0);
     const double bn = b1 * dAA[0] + b2 * dAA[1] + b3 * dAA[2];
-    const double e0 = Rel::div(Rel::div(P[4], P[0]), gm1);
+    const double e0 = Rel::div_pos(Rel::div_pos(P[4], P[0]), gm1);
     const double p0 = P[4];
-    const double h0 = 1.0 + e0 + Rel::div(p0, P[0]);
-    const double cs2 = Rel::div(gm0 * p0, P[0] * h0);
+    const double h0 = 1.0 + e0 + Rel::div_pos(p0, P[0]);
+    const double cs2 = Rel::div_pos(gm0 * p0, P[0] * h0);
     const double W4 = W2 * W2;
     const double v2 = vn * vn;
     const double v3 = vn * v2;
@@ -375,13 +375,13 @@ __device__ inline int rmhd_eigen(double gamma, const double* P, const double* dA
     if (BB == 0.0)
     {
         const double K = Rel::sqrt(cs2 * (1 - vv) * (1 - vv * cs2 - v2 * (1 - cs2)));
-        A[0] = Rel::div(vn * (1 - cs2) - K, 1 - vv * cs2);
+        A[0] = Rel::div_pos(vn * (1 - cs2) - K, 1 - vv * cs2);
         for (int n = 1; n < 7; ++n) A[n] = vn;
-        A[7] = Rel::div(vn * (1 - cs2) + K, 1 - vv * cs2);
+        A[7] = Rel::div_pos(vn * (1 - cs2) + K, 1 - vv * cs2);
         return ERR_NONE;
     }
-    const double K = W4 * (P[0] * h0 * (Rel::rcp(cs2) - 1.));
-    const double L = -W2 * (P[0] * h0 + Rel::div(bb, cs2));
+    const double K = W4 * (P[0] * h0 * (Rel::div_pos(1.0, cs2) - 1.));
+    const double L = -W2 * (P[0] * h0 + Rel::div_pos(bb, cs2));
     const double A4 = K - L - b0 * b0;
     const double A3 = -4 * K * vn + L * vn * 2 + 2 * b0 * bn;
     const double A2 = 6 * K * v2 + L * (1. - v2) + b0 * b0 - bn * bn;
@@ -430,18 +430,18 @@ __device__ __forceinline__ int rmhd_state(R gamma, const R* P, const R* dAA, Sta
     const R Bv = P[5] * P[1] + P[6] * P[2] + P[7] * P[3];
     const R vn = P[1] * dAA[0] + P[2] * dAA[1] + P[3] * dAA[2];
     const R Bn = P[5] * dAA[0] + P[6] * dAA[1] + P[7] * dAA[2];
-    const R W2 = RM::div(R(1.0), R(1.0) - vv);
+    const R W2 = RM::div_pos(R(1.0), R(1.0) - vv);
     const R W0 = RM::sqrt(W2);
     const R b0 = W0 * Bv;
-    const R bb = RM::div(BB + b0 * b0, W2);
-    const R b1 = RM::div(P[5] + b0 * W0 * P[1], W0);
-    const R b2 = RM::div(P[6] + b0 * W0 * P[2], W0);
-    const R b3 = RM::div(P[7] + b0 * W0 * P[3], W0);
-    const R e0 = RM::div(RM::div(P[4], P[0]), gm1);
-    const R es = e0 + RM::div(R(0.5) * bb, P[0]);
+    const R bb = RM::div_pos(BB + b0 * b0, W2);
+    const R b1 = RM::div_pos(P[5] + b0 * W0 * P[1], W0);
+    const R b2 = RM::div_pos(P[6] + b0 * W0 * P[2], W0);
+    const R b3 = RM::div_pos(P[7] + b0 * W0 * P[3], W0);
+    const R e0 = RM::div_pos(RM::div_pos(P[4], P[0]), gm1);
+    const R es = e0 + RM::div_pos(R(0.5) * bb, P[0]);
     const R p0 = P[4];
     const R ps = p0 + R(0.5) * bb;
-    const R hs = R(1.0) + es + RM::div(ps, P[0]);
+    const R hs = R(1.0) + es + RM::div_pos(ps, P[0]);
     if (vv >= R(1)) return ERR_SUPERLUMINAL;
 #pragma unroll
     for (int q = 0; q < 8; ++q) S.P[q] = P[q];
@@ -454,10 +454,10 @@ __device__ __forceinline__ int rmhd_state(R gamma, const R* P, const R* dAA, Sta
     S.U[6] = P[6];
     S.U[7] = P[7];
     S.F[0] = vn * S.U[0];
-    S.F[1] = vn * S.U[1] - RM::div(Bn * b1, W0) + ps * dAA[0];
-    S.F[2] = vn * S.U[2] - RM::div(Bn * b2, W0) + ps * dAA[1];
-    S.F[3] = vn * S.U[3] - RM::div(Bn * b3, W0) + ps * dAA[2];
-    S.F[4] = vn * S.U[4] - RM::div(Bn * b0, W0) + ps * vn;
+    S.F[1] = vn * S.U[1] - RM::div_pos(Bn * b1, W0) + ps * dAA[0];
+    S.F[2] = vn * S.U[2] - RM::div_pos(Bn * b2, W0) + ps * dAA[1];
+    S.F[3] = vn * S.U[3] - RM::div_pos(Bn * b3, W0) + ps * dAA[2];
+    S.F[4] = vn * S.U[4] - RM::div_pos(Bn * b0, W0) + ps * vn;
     S.F[5] = vn * S.U[5] - Bn * P[1];
     S.F[6] = vn * S.U[6] - Bn * P[2];
     S.F[7] = vn * S.U[7] - Bn * P[3];
