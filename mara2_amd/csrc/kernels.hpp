@@ -193,6 +193,34 @@ __device__ __forceinline__ T wave_max(T v)
 // ---------------------------------------------------------------------------
 template <int Recon> struct Stencil { static constexpr int NG = Recon == 0 ? 1 : Recon == 1 ? 2 : 3; };
 
+// The relativistic law's divisions are Rel's (physics.hpp), the others the
+// working precision's.
+template <int Law, typename R>
+using LawMath = typename std::conditional<Law == 2, typename RelMath<R>::type, Math<R>>::type;
+
+// Flux through one face from the reconstructed primitives on its two sides:
+// the law's face states and the Riemann solver.
+template <int Law, int Riem, typename R>
+__device__ __forceinline__ void face_flux_lr(const R* PL, const R* PR, const Geom& G, int axis, R* Fq, Ctl* ctlw)
+{
+    constexpr int NQ = LawNQ<Law>::value;
+    const R n[3] = {R(G.nhat[3 * axis + 0]), R(G.nhat[3 * axis + 1]), R(G.nhat[3 * axis + 2])};
+    State<R, NQ> SL, SR;
+    const int eL = law_state<Law, R>(R(G.gamma), PL, n, SL);
+    const int eR = law_state<Law, R>(R(G.gamma), PR, n, SR);
+    if (eL | eR)
+    {
+        if ((eL | eR) & ERR_SUPERLUMINAL) atomicAdd(&ctlw->superluminal, 1ull);
+        if ((eL | eR) & ERR_EIGEN) atomicAdd(&ctlw->eigen_fail, 1ull);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) Fq[q] = R(0);
+    }
+    else if constexpr (Riem == 1 && Law == 0)
+        hllc(SL, SR, n, Fq);
+    else
+        hlle<R, NQ, LawMath<Law, R>>(SL, SR, Fq);
+}
+
 // Flux through one face from its stencil S[q][m] = P(first stencil zone + m),
 // m = 0 .. 2 NG - 1, in R (reconstruction, law state, Riemann solver).
 template <int Law, int Recon, int Riem, typename R>
@@ -218,27 +246,11 @@ __device__ __forceinline__ void face_flux(const R (&S)[LawNQ<Law>::value][2 * St
         }
         else
         {
-            // the relativistic law's divisions are Rel's (physics.hpp)
-            using D = typename std::conditional<Law == 2, typename RelMath<R>::type, Math<R>>::type;
-            PL[q] = weno5<R, true, D>(S[q][0], S[q][1], S[q][2], S[q][3], S[q][4]);
-            PR[q] = weno5<R, false, D>(S[q][1], S[q][2], S[q][3], S[q][4], S[q][5]);
+            PL[q] = weno5<R, true, LawMath<Law, R>>(S[q][0], S[q][1], S[q][2], S[q][3], S[q][4]);
+            PR[q] = weno5<R, false, LawMath<Law, R>>(S[q][1], S[q][2], S[q][3], S[q][4], S[q][5]);
         }
     }
-    const R n[3] = {R(G.nhat[3 * axis + 0]), R(G.nhat[3 * axis + 1]), R(G.nhat[3 * axis + 2])};
-    State<R, NQ> SL, SR;
-    const int eL = law_state<Law, R>(R(G.gamma), PL, n, SL);
-    const int eR = law_state<Law, R>(R(G.gamma), PR, n, SR);
-    if (eL | eR)
-    {
-        if ((eL | eR) & ERR_SUPERLUMINAL) atomicAdd(&ctlw->superluminal, 1ull);
-        if ((eL | eR) & ERR_EIGEN) atomicAdd(&ctlw->eigen_fail, 1ull);
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) Fq[q] = R(0);
-    }
-    else if constexpr (Riem == 1 && Law == 0)
-        hllc(SL, SR, n, Fq);
-    else
-        hlle<R, NQ, typename std::conditional<Law == 2, typename RelMath<R>::type, Math<R>>::type>(SL, SR, Fq);
+    face_flux_lr<Law, Riem, R>(PL, PR, G, axis, Fq, ctlw);
 }
 
 // Flux through the face left of cell (i, j, k) along axis, from the stencil
@@ -251,6 +263,28 @@ __device__ __forceinline__ void face_flux_at(const double* __restrict__ P, const
     constexpr int NG = Stencil<Recon>::NG;
     const long long stride = axis == 0 ? 1 : axis == 1 ? (long long)G.N[0] : (long long)G.N[0] * G.N[1];
     const long long c0 = cell_index(G, i, j, k) - NG * stride;  // first stencil cell
+    if constexpr (Law == 2 && Recon == 2)
+    {
+        // the relativistic WENO5 face field by field: each field's six values
+        // are loaded and reconstructed before the next field's loads issue
+        // (an opaque use of the two values orders them), so the 48 stencil
+        // values are never live together and the kernel keeps more
+        // wavefronts resident
+        R PL[NQ], PR[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+        {
+            const double* Pq = P + q * G.fs + c0;
+            R v[6];
+#pragma unroll
+            for (int m = 0; m < 6; ++m) v[m] = R(Pq[m * stride]);
+            PL[q] = weno5<R, true, LawMath<Law, R>>(v[0], v[1], v[2], v[3], v[4]);
+            PR[q] = weno5<R, false, LawMath<Law, R>>(v[1], v[2], v[3], v[4], v[5]);
+            asm volatile("" : : "v"(PL[q]), "v"(PR[q]) : "memory");
+        }
+        face_flux_lr<Law, Riem, R>(PL, PR, G, axis, Fq, ctlw);
+        return;
+    }
     R S[NQ][2 * NG];
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
