@@ -26,6 +26,16 @@ template <> struct Math<double>
     // a / b where b is positive and finite (the same division here)
     static __device__ __forceinline__ double div_pos(double a, double b) { return a / b; }
     static __device__ __forceinline__ double abs(double x) { return __builtin_fabs(x); }
+    // the reference's MIN3 / MAX3 and comparison order (NaN semantics kept)
+    static __device__ __forceinline__ double min3(double a, double b, double c)
+    {
+        return a < b ? (a < c ? a : c) : (b < c ? b : c);
+    }
+    static __device__ __forceinline__ double max3(double a, double b, double c)
+    {
+        return a > b ? (a > c ? a : c) : (b > c ? b : c);
+    }
+    static constexpr bool ieee_minmax = false;
 };
 
 // binary32 in the mixed form: hardware square root and reciprocal (about one
@@ -36,6 +46,15 @@ template <> struct Math<float>
     static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
     static __device__ __forceinline__ float div_pos(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
     static __device__ __forceinline__ float abs(float x) { return __builtin_fabsf(x); }
+    static __device__ __forceinline__ float min3(float a, float b, float c)
+    {
+        return a < b ? (a < c ? a : c) : (b < c ? b : c);
+    }
+    static __device__ __forceinline__ float max3(float a, float b, float c)
+    {
+        return a > b ? (a > c ? a : c) : (b > c ? b : c);
+    }
+    static constexpr bool ieee_minmax = false;
 };
 
 // Binary64 arithmetic of the relativistic law.  Its results are held to
@@ -72,6 +91,18 @@ struct Rel
         return (x == 0.0 || x == __builtin_inf()) ? x : x * r;
     }
     static __device__ __forceinline__ double abs(double x) { return __builtin_fabs(x); }
+    // minima and maxima as single min/max instructions; they differ from the
+    // comparison forms only where an operand is NaN (or for zeros of both
+    // signs, which these operands never are), a state already lost
+    static __device__ __forceinline__ double min3(double a, double b, double c)
+    {
+        return __builtin_fmin(a, __builtin_fmin(b, c));
+    }
+    static __device__ __forceinline__ double max3(double a, double b, double c)
+    {
+        return __builtin_fmax(a, __builtin_fmax(b, c));
+    }
+    static constexpr bool ieee_minmax = true;
 };
 
 // the relativistic law's arithmetic in working precision R: Rel in
@@ -133,8 +164,8 @@ __device__ __forceinline__ R weno5(R vm2, R vm1, R v0, R vp1, R vp2)
     R B0 = R(13. / 12.) * (t0a * t0a) + R(1. / 4.) * (t0b * t0b);
     R B1 = R(13. / 12.) * (t1a * t1a) + R(1. / 4.) * (t1b * t1b);
     R B2 = R(13. / 12.) * (t2a * t2a) + R(1. / 4.) * (t2b * t2b);
-    const R minB = min3(B0, B1, B2);
-    const R maxB = max3(B0, B1, B2);
+    const R minB = D::min3(B0, B1, B2);
+    const R maxB = D::max3(B0, B1, B2);
     const R R0 = D::div_pos(minB, maxB + eps_prime);
     B0 = R0 * A * minB + B0;
     B1 = R0 * A * minB + B1;
@@ -549,12 +580,29 @@ __device__ __forceinline__ R min_elem(const R* A)
 template <typename R, int NQ, typename D = Math<R>>
 __device__ __forceinline__ void hlle(const State<R, NQ>& L, const State<R, NQ>& Rs, R* F)
 {
-    const R epl = max_elem<R, NQ>(L.A);
-    const R eml = min_elem<R, NQ>(L.A);
-    const R epr = max_elem<R, NQ>(Rs.A);
-    const R emr = min_elem<R, NQ>(Rs.A);
-    const R ap = max3(R(0.0), epl, epr);
-    const R am = min3(R(0.0), eml, emr);
+    R ap, am;
+    if constexpr (D::ieee_minmax)
+    {
+        // the same extremes over both states' eigenvalues and 0 as single
+        // min/max instructions (Rel's arithmetic)
+        ap = R(0);
+        am = R(0);
+#pragma unroll
+        for (int n = 0; n < NQ; ++n)
+        {
+            ap = __builtin_fmax(ap, __builtin_fmax(L.A[n], Rs.A[n]));
+            am = __builtin_fmin(am, __builtin_fmin(L.A[n], Rs.A[n]));
+        }
+    }
+    else
+    {
+        const R epl = max_elem<R, NQ>(L.A);
+        const R eml = min_elem<R, NQ>(L.A);
+        const R epr = max_elem<R, NQ>(Rs.A);
+        const R emr = min_elem<R, NQ>(Rs.A);
+        ap = max3(R(0.0), epl, epr);
+        am = min3(R(0.0), eml, emr);
+    }
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
         F[q] = D::div_pos(ap * L.F[q] - am * Rs.F[q] + ap * am * (Rs.U[q] - L.U[q]), ap - am);
