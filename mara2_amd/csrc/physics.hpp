@@ -46,15 +46,17 @@ template <> struct Math<float>
     static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
     static __device__ __forceinline__ float div_pos(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
     static __device__ __forceinline__ float abs(float x) { return __builtin_fabsf(x); }
+    // minima and maxima as single min/max instructions (the comparison
+    // forms differ only where an operand is NaN, a state already lost)
     static __device__ __forceinline__ float min3(float a, float b, float c)
     {
-        return a < b ? (a < c ? a : c) : (b < c ? b : c);
+        return __builtin_fminf(a, __builtin_fminf(b, c));
     }
     static __device__ __forceinline__ float max3(float a, float b, float c)
     {
-        return a > b ? (a > c ? a : c) : (b > c ? b : c);
+        return __builtin_fmaxf(a, __builtin_fmaxf(b, c));
     }
-    static constexpr bool ieee_minmax = false;
+    static constexpr bool ieee_minmax = true;
 };
 
 // Binary64 arithmetic of the relativistic law.  Its results are held to
@@ -126,6 +128,16 @@ __device__ __forceinline__ R minmod(R theta, R ul, R u0, R ur)
     const R a = theta * (u0 - ul);
     const R b = R(0.5) * (ur - ul);
     const R c = theta * (ur - u0);
+    if constexpr (sizeof(R) == 4)
+    {
+        // the mixed form: the same value (the argument of least magnitude
+        // when all three share a sign, else zero) as min/max instructions,
+        // max(min(a, b, c), min(max(a, b, c), 0)); only the sign of a zero
+        // result can differ from the reference's product form
+        const R lo = __builtin_fminf(a, __builtin_fminf(b, c));
+        const R hi = __builtin_fmaxf(a, __builtin_fmaxf(b, c));
+        return __builtin_fmaxf(lo, __builtin_fminf(hi, R(0)));
+    }
     const R m = min3(Math<R>::abs(a), Math<R>::abs(b), Math<R>::abs(c));
     const int sa = sgn(a), sb = sgn(b), sc = sgn(c);
     return R(0.25) * Math<R>::abs(R(sa + sb)) * R(sa + sc) * m;
