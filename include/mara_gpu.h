@@ -73,7 +73,8 @@ extern "C" {
                                normals; held to relative L1 <= 1e-12 of the
                                reference's conserved fields after 10 steps.
                                Hydro blocks with PLM on 2-D and 3-D meshes run
-                               their own stage kernel; every other block runs
+                               their own stage kernel, 2-D MHD blocks the MHD
+                               tiles in this arithmetic; every other block runs
                                the MARA_PREC_FP64 kernels.  Blocks and block
                                groups; not in batches. */
 

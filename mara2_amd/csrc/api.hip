@@ -164,6 +164,7 @@ int cfl_arith(const mara_block* b)
     const bool fast_tile = b->fused && fast64_supported(&b->d);
     if (b->d.precision == MARA_PREC_MIXED) return fast_tile ? 3 : (law_code(b->d) < 2 ? 1 : 0);
     if (b->d.precision == MARA_PREC_FP64_FAST && fast_tile) return 2;
+    if (b->d.precision == MARA_PREC_FP64_FAST && b->mhd2d) return 4;
     return 0;
 }
 
@@ -201,6 +202,7 @@ void run_stage_kernels(const StageArgs& s, bool mixed)
     const bool fast_tile = s.fused && fast64_supported(s.d);
     if (mixed) fast_tile ? stage_kernels_fast32(s) : stage_kernels_f32(s);
     else if (s.d->precision == MARA_PREC_FP64_FAST && fast_tile) stage_kernels_fast64(s);
+    else if (s.d->precision == MARA_PREC_FP64_FAST && s.mhd2d) stage_kernels_fast64_mhd2d(s);
     else stage_kernels<double>(s);
 }
 

@@ -23,6 +23,16 @@ void stage_kernels_fast32(const StageArgs& s)
     fast64::launch<float>(*s.G, s.stream, s.d->riemann, s.Pin, s.P0, s.Pout, s.b, s.first, s.cout, s.ctl, s.kz);
 }
 
+// 2-D MHD blocks of the fast binary64 form: the fused MHD + CT tiles with the
+// relativistic law's arithmetic (refined reciprocals and square roots,
+// min/max extremes and minmod, one reciprocal of the density in the
+// recovery) and contraction
+void stage_kernels_fast64_mhd2d(const StageArgs& s)
+{
+    launch_mhd2d<double, Rel>(*s.d, *s.G, s.ct, s.rows, s.stream, s.Pin, s.P0, s.U, s.Pout, s.b, s.first, s.write_u,
+                              s.cout, s.ctl, s.S);
+}
+
 void stage_round_fast(const StageArgs& s, bool f32, int* per_cu, int* per_layer)
 {
     if (f32) fast64::tile_round<float>(*s.G, s.d->riemann, per_cu, per_layer);

@@ -567,10 +567,11 @@ inline void launch(const Geom& G, hipStream_t stream, int riemann, const double*
 // it in their last stage, so the first step of a run (or of a restart from a
 // checkpoint) takes the same dt a later step would take for the same state:
 // the reference's binary64 (k_courant), binary32 wave speeds (the mixed
-// flux-array, pencil, 2-D MHD and non-PLM fused kernels), or the fast tiles'
-// courant<double> / courant<float>.
+// flux-array, pencil, 2-D MHD and non-PLM fused kernels), the fast tiles'
+// courant<double> / courant<float>, or the fast binary64 2-D MHD tiles'
+// wave speeds in Rel arithmetic.
 // ---------------------------------------------------------------------------
-enum CflArith { CFL_EXACT = 0, CFL_MIXED = 1, CFL_FAST64 = 2, CFL_FAST32 = 3 };
+enum CflArith { CFL_EXACT = 0, CFL_MIXED = 1, CFL_FAST64 = 2, CFL_FAST32 = 3, CFL_REL64 = 4 };
 
 template <int Law>
 __device__ __forceinline__ double zone_courant_form(int arith, const double* __restrict__ P, const Geom& G,
@@ -596,6 +597,15 @@ __device__ __forceinline__ double zone_courant_form(int arith, const double* __r
     }
     if constexpr (Law < 2)
     {
+        // the fast binary64 form's 2-D MHD tiles: wave speeds in Rel
+        if (arith == CFL_REL64)
+        {
+            constexpr int NQ = LawNQ<Law>::value;
+            double p[NQ];
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) p[q] = P[q * G.fs + c];
+            return courant_of<Law, double, Rel>(p, G, Lz, err);
+        }
         if (arith == CFL_MIXED)
         {
             constexpr int NQ = LawNQ<Law>::value;

@@ -35,7 +35,9 @@ namespace mara {
 constexpr int MHD_ROWS_DEFAULT = 2;
 inline constexpr int mhd_tile_height(int rows) { return 4 * rows; }
 
-template <int Recon, typename R, bool CT, int TY>
+// A: the arithmetic of the face states and the zone recovery (Math<R>; Rel
+// in the fast binary64 form)
+template <int Recon, typename R, bool CT, int TY, typename A = Math<R>>
 __device__ __forceinline__ void mhd2d_tile(const double* __restrict__ Pin, const double* __restrict__ P0,
                                            double* __restrict__ U, double* __restrict__ Pout, const Geom& G, double b,
                                            int first_stage, int write_u, int courant_out, Ctl* ctl, int tx, int ty,
@@ -127,10 +129,10 @@ __device__ __forceinline__ void mhd2d_tile(const double* __restrict__ Pin, const
                 for (int q = 0; q < NQ; ++q)
 #pragma unroll
                     for (int m = 0; m < 2 * NG; ++m) S[q][m] = sP[q][ly][c + m];
-                face_flux<1, Recon, 0, R>(S, G, 0, Fq, ctl);
+                face_flux<1, Recon, 0, R, A>(S, G, 0, Fq, ctl);
             }
             else
-                face_flux_at<1, Recon, 0, R>(Pin, G, 0, i0 + c, jj, k, Fq, ctl);
+                face_flux_at<1, Recon, 0, R, A>(Pin, G, 0, i0 + c, jj, k, Fq, ctl);
 #pragma unroll
             for (int q = 0; q < NQ; ++q) fx[q][xr_lo + r][c] = Fq[q];
         }
@@ -146,10 +148,10 @@ __device__ __forceinline__ void mhd2d_tile(const double* __restrict__ Pin, const
                 for (int q = 0; q < NQ; ++q)
 #pragma unroll
                     for (int m = 0; m < 2 * NG; ++m) S[q][m] = sP[q][r + m][lx];
-                face_flux<1, Recon, 0, R>(S, G, 1, Fq, ctl);
+                face_flux<1, Recon, 0, R, A>(S, G, 1, Fq, ctl);
             }
             else
-                face_flux_at<1, Recon, 0, R>(Pin, G, 1, ii, j0 + r, k, Fq, ctl);
+                face_flux_at<1, Recon, 0, R, A>(Pin, G, 1, ii, j0 + r, k, Fq, ctl);
 #pragma unroll
             for (int q = 0; q < NQ; ++q) fy[q][r][yc_lo + c] = Fq[q];
         }
@@ -193,12 +195,12 @@ __device__ __forceinline__ void mhd2d_tile(const double* __restrict__ Pin, const
                 L[q] = divergence_R<R>(G, fx[q][xr][xc + 1], fx[q][xr][xc], fy[q][yr + 1][yc], fy[q][yr][yc], R(0),
                                        R(0));
         }
-        dtc = finish_zone<1, R>(P0, U, Pout, G, b, first_stage, write_u, courant_out, ctl, i, j, k, L, Ssrc);
+        dtc = finish_zone<1, R, A>(P0, U, Pout, G, b, first_stage, write_u, courant_out, ctl, i, j, k, L, Ssrc);
     }
     if (courant_out) courant_to_bins(dtc, wmin, NT / 64, tid, ctl);
 }
 
-template <int Recon, typename R, bool CT, int TY>
+template <int Recon, typename R, bool CT, int TY, typename A = Math<R>>
 static __global__ void __launch_bounds__(64 * TY)
 k_mhd2d(const double* __restrict__ Pin, const double* __restrict__ P0, double* __restrict__ U,
         double* __restrict__ Pout, Geom G, double b, int first_stage, int write_u, int courant_out, Ctl* ctl,
@@ -207,8 +209,8 @@ k_mhd2d(const double* __restrict__ Pin, const double* __restrict__ P0, double* _
     if (!ctl->active) return;
     const int ntx = (G.n[0] + 63) / 64, nty = (G.n[1] + TY - 1) / TY;
     const int tile = xcd_tile(blockIdx.x, ntx * nty);
-    mhd2d_tile<Recon, R, CT, TY>(Pin, P0, U, Pout, G, b, first_stage, write_u, courant_out, ctl, tile % ntx,
-                                 tile / ntx, Ssrc);
+    mhd2d_tile<Recon, R, CT, TY, A>(Pin, P0, U, Pout, G, b, first_stage, write_u, courant_out, ctl, tile % ntx,
+                                    tile / ntx, Ssrc);
 }
 
 // 2-D MHD blocks of two or three stages (the z axis flat, x and y fleshed):
@@ -219,32 +221,32 @@ inline bool mhd2d_supported(const mara_desc* d)
            && d->guards[1] > 0 && d->rk_order >= 2 && !(d->cooling_rate > 0.0) && d->geometry == MARA_GEOM_CARTESIAN;
 }
 
-template <int Recon, typename R, int TY>
+template <int Recon, typename R, int TY, typename A = Math<R>>
 inline void launch_mhd2d_ty(const Geom& G, bool ct, hipStream_t s, const double* Pin, const double* P0, double* U,
                             double* Pout, double b, int first, int write_u, int cout, Ctl* ctl, const double* Ssrc)
 {
     const dim3 block(64, TY, 1);
     const dim3 grid(((G.n[0] + 63) / 64) * ((G.n[1] + TY - 1) / TY));
     if (ct)
-        k_mhd2d<Recon, R, true, TY><<<grid, block, 0, s>>>(Pin, P0, U, Pout, G, b, first, write_u, cout, ctl, Ssrc);
+        k_mhd2d<Recon, R, true, TY, A><<<grid, block, 0, s>>>(Pin, P0, U, Pout, G, b, first, write_u, cout, ctl, Ssrc);
     else
-        k_mhd2d<Recon, R, false, TY><<<grid, block, 0, s>>>(Pin, P0, U, Pout, G, b, first, write_u, cout, ctl, Ssrc);
+        k_mhd2d<Recon, R, false, TY, A><<<grid, block, 0, s>>>(Pin, P0, U, Pout, G, b, first, write_u, cout, ctl, Ssrc);
 }
 
-template <int Recon, typename R>
+template <int Recon, typename R, typename A = Math<R>>
 inline void launch_mhd2d_t(const Geom& G, bool ct, int rows, hipStream_t s, const double* Pin, const double* P0,
                            double* U, double* Pout, double b, int first, int write_u, int cout, Ctl* ctl,
                            const double* Ssrc)
 {
     switch (rows)
     {
-        case 1: launch_mhd2d_ty<Recon, R, 4>(G, ct, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, Ssrc); break;
-        case 2: launch_mhd2d_ty<Recon, R, 8>(G, ct, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, Ssrc); break;
-        default: launch_mhd2d_ty<Recon, R, 16>(G, ct, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, Ssrc); break;
+        case 1: launch_mhd2d_ty<Recon, R, 4, A>(G, ct, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, Ssrc); break;
+        case 2: launch_mhd2d_ty<Recon, R, 8, A>(G, ct, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, Ssrc); break;
+        default: launch_mhd2d_ty<Recon, R, 16, A>(G, ct, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, Ssrc); break;
     }
 }
 
-template <typename R>
+template <typename R, typename A = Math<R>>
 inline void launch_mhd2d(const mara_desc& d, const Geom& G, bool ct, int rows, hipStream_t s, const double* Pin,
                          const double* P0, double* U, double* Pout, double b, int first, int write_u, int cout,
                          Ctl* ctl, const double* Ssrc)
@@ -252,12 +254,12 @@ inline void launch_mhd2d(const mara_desc& d, const Geom& G, bool ct, int rows, h
     switch (d.recon)
     {
         case MARA_RECON_PCM:
-            launch_mhd2d_t<0, R>(G, ct, rows, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, Ssrc);
+            launch_mhd2d_t<0, R, A>(G, ct, rows, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, Ssrc);
             break;
         case MARA_RECON_PLM:
-            launch_mhd2d_t<1, R>(G, ct, rows, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, Ssrc);
+            launch_mhd2d_t<1, R, A>(G, ct, rows, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, Ssrc);
             break;
-        default: launch_mhd2d_t<2, R>(G, ct, rows, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, Ssrc); break;
+        default: launch_mhd2d_t<2, R, A>(G, ct, rows, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, Ssrc); break;
     }
 }
 

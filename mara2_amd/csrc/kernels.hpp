@@ -193,21 +193,17 @@ __device__ __forceinline__ T wave_max(T v)
 // ---------------------------------------------------------------------------
 template <int Recon> struct Stencil { static constexpr int NG = Recon == 0 ? 1 : Recon == 1 ? 2 : 3; };
 
-// The relativistic law's divisions are Rel's (physics.hpp), the others the
-// working precision's.
-template <int Law, typename R>
-using LawMath = typename std::conditional<Law == 2, typename RelMath<R>::type, Math<R>>::type;
-
 // Flux through one face from the reconstructed primitives on its two sides:
-// the law's face states and the Riemann solver.
-template <int Law, int Riem, typename R>
+// the law's face states and the Riemann solver, in arithmetic A (LawMath by
+// default, physics.hpp).
+template <int Law, int Riem, typename R, typename A = LawMath<Law, R>>
 __device__ __forceinline__ void face_flux_lr(const R* PL, const R* PR, const Geom& G, int axis, R* Fq, Ctl* ctlw)
 {
     constexpr int NQ = LawNQ<Law>::value;
     const R n[3] = {R(G.nhat[3 * axis + 0]), R(G.nhat[3 * axis + 1]), R(G.nhat[3 * axis + 2])};
     State<R, NQ> SL, SR;
-    const int eL = law_state<Law, R>(R(G.gamma), PL, n, SL);
-    const int eR = law_state<Law, R>(R(G.gamma), PR, n, SR);
+    const int eL = law_state<Law, R, A>(R(G.gamma), PL, n, SL);
+    const int eR = law_state<Law, R, A>(R(G.gamma), PR, n, SR);
     if (eL | eR)
     {
         if ((eL | eR) & ERR_SUPERLUMINAL) atomicAdd(&ctlw->superluminal, 1ull);
@@ -218,12 +214,12 @@ __device__ __forceinline__ void face_flux_lr(const R* PL, const R* PR, const Geo
     else if constexpr (Riem == 1 && Law == 0)
         hllc(SL, SR, n, Fq);
     else
-        hlle<R, NQ, LawMath<Law, R>>(SL, SR, Fq);
+        hlle<R, NQ, A>(SL, SR, Fq);
 }
 
 // Flux through one face from its stencil S[q][m] = P(first stencil zone + m),
 // m = 0 .. 2 NG - 1, in R (reconstruction, law state, Riemann solver).
-template <int Law, int Recon, int Riem, typename R>
+template <int Law, int Recon, int Riem, typename R, typename A = LawMath<Law, R>>
 __device__ __forceinline__ void face_flux(const R (&S)[LawNQ<Law>::value][2 * Stencil<Recon>::NG], const Geom& G,
                                           int axis, R* Fq, Ctl* ctlw)
 {
@@ -241,21 +237,21 @@ __device__ __forceinline__ void face_flux(const R (&S)[LawNQ<Law>::value][2 * St
         }
         else if constexpr (NG == 2)
         {
-            PL[q] = plm(theta, S[q][0], S[q][1], S[q][2], R(+1.0));
-            PR[q] = plm(theta, S[q][1], S[q][2], S[q][3], R(-1.0));
+            PL[q] = plm<R, A>(theta, S[q][0], S[q][1], S[q][2], R(+1.0));
+            PR[q] = plm<R, A>(theta, S[q][1], S[q][2], S[q][3], R(-1.0));
         }
         else
         {
-            PL[q] = weno5<R, true, LawMath<Law, R>>(S[q][0], S[q][1], S[q][2], S[q][3], S[q][4]);
-            PR[q] = weno5<R, false, LawMath<Law, R>>(S[q][1], S[q][2], S[q][3], S[q][4], S[q][5]);
+            PL[q] = weno5<R, true, A>(S[q][0], S[q][1], S[q][2], S[q][3], S[q][4]);
+            PR[q] = weno5<R, false, A>(S[q][1], S[q][2], S[q][3], S[q][4], S[q][5]);
         }
     }
-    face_flux_lr<Law, Riem, R>(PL, PR, G, axis, Fq, ctlw);
+    face_flux_lr<Law, Riem, R, A>(PL, PR, G, axis, Fq, ctlw);
 }
 
 // Flux through the face left of cell (i, j, k) along axis, from the stencil
 // of 2 NG cells read from P.
-template <int Law, int Recon, int Riem, typename R>
+template <int Law, int Recon, int Riem, typename R, typename A = LawMath<Law, R>>
 __device__ __forceinline__ void face_flux_at(const double* __restrict__ P, const Geom& G, int axis, int i, int j,
                                              int k, R* Fq, Ctl* ctlw)
 {
@@ -278,11 +274,11 @@ __device__ __forceinline__ void face_flux_at(const double* __restrict__ P, const
             R v[6];
 #pragma unroll
             for (int m = 0; m < 6; ++m) v[m] = R(Pq[m * stride]);
-            PL[q] = weno5<R, true, LawMath<Law, R>>(v[0], v[1], v[2], v[3], v[4]);
-            PR[q] = weno5<R, false, LawMath<Law, R>>(v[1], v[2], v[3], v[4], v[5]);
+            PL[q] = weno5<R, true, A>(v[0], v[1], v[2], v[3], v[4]);
+            PR[q] = weno5<R, false, A>(v[1], v[2], v[3], v[4], v[5]);
             asm volatile("" : : "v"(PL[q]), "v"(PR[q]) : "memory");
         }
-        face_flux_lr<Law, Riem, R>(PL, PR, G, axis, Fq, ctlw);
+        face_flux_lr<Law, Riem, R, A>(PL, PR, G, axis, Fq, ctlw);
         return;
     }
     R S[NQ][2 * NG];
@@ -293,7 +289,7 @@ __device__ __forceinline__ void face_flux_at(const double* __restrict__ P, const
 #pragma unroll
         for (int m = 0; m < 2 * NG; ++m) S[q][m] = R(Pq[m * stride]);
     }
-    face_flux<Law, Recon, Riem, R>(S, G, axis, Fq, ctlw);
+    face_flux<Law, Recon, Riem, R, A>(S, G, axis, Fq, ctlw);
 }
 
 template <int Law, int Recon, int Riem, typename R>
@@ -423,8 +419,9 @@ k_visc_unstage(const double* __restrict__ Fv, R* __restrict__ F, Geom G, int axi
 // CFL time step of one zone (FieldOperator::getCourantTimestep, src/FieldOperator.cpp:125-169)
 // ---------------------------------------------------------------------------
 // R is the arithmetic of the wave speeds: binary64 for the FP64 form (the
-// reference's), binary32 for the hydro and MHD laws in the mixed form.
-template <int Law, typename R = double>
+// reference's), binary32 for the hydro and MHD laws in the mixed form; A that
+// of the face states (LawMath, or Rel in the fast binary64 form).
+template <int Law, typename R = double, typename A = LawMath<Law, R>>
 __device__ __forceinline__ double courant_of(const double* p, const Geom& G, double Lz, int& err)
 {
     constexpr int NQ = LawNQ<Law>::value;
@@ -437,7 +434,7 @@ __device__ __forceinline__ double courant_of(const double* p, const Geom& G, dou
     {
         const R n[3] = {R(G.nhat[3 * a]), R(G.nhat[3 * a + 1]), R(G.nhat[3 * a + 2])};
         State<R, NQ> S;
-        err |= law_state<Law, R>(R(G.gamma), pr, n, S);
+        err |= law_state<Law, R, A>(R(G.gamma), pr, n, S);
         R m = R(0);
 #pragma unroll
         for (int q = 0; q < NQ; ++q)
@@ -455,10 +452,10 @@ __device__ __forceinline__ double courant_of(const double* p, const Geom& G, dou
 }
 
 // on a Cartesian mesh (one cell length for every zone)
-template <int Law, typename R = double>
+template <int Law, typename R = double, typename A = LawMath<Law, R>>
 __device__ __forceinline__ double courant_of(const double* p, const Geom& G, int& err)
 {
-    return courant_of<Law, R>(p, G, G.Lmin, err);
+    return courant_of<Law, R, A>(p, G, G.Lmin, err);
 }
 
 template <int Law>
@@ -565,7 +562,10 @@ __device__ __forceinline__ void write_guard_images(double* __restrict__ P, const
 // arithmetic: with binary32 fluxes (the mixed form) the hydro and MHD laws
 // recover primitives with cons_to_prim_mixed and take the CFL wave speeds in
 // binary32.
-template <int Law, typename R = double>
+// A: the arithmetic of the zone's CFL wave speeds (LawMath; Rel in the fast
+// binary64 form, which also recovers the primitives with one reciprocal of
+// the density, as the mixed form does).
+template <int Law, typename R = double, typename A = LawMath<Law, R>>
 __device__ __forceinline__ double finish_zone(const double* __restrict__ P0, double* __restrict__ U,
                                               double* __restrict__ Pout, const Geom& G, double b, int first_stage,
                                               int write_u, int courant_out, Ctl* ctl, int i, int j, int k,
@@ -591,10 +591,11 @@ __device__ __forceinline__ double finish_zone(const double* __restrict__ P0, dou
         Un[q] = U0[q] * (1 - b) + (Uq + dt * Lq) * b;
     }
     constexpr bool MIX = sizeof(R) == 4 && Law < 2;
+    constexpr bool FAST = sizeof(R) == 8 && Law < 2 && A::ieee_minmax;
     double Pn[NQ];
     int floor = 0;
     int failed;
-    if constexpr (MIX) failed = cons_to_prim_mixed<Law>(G.gamma, G.pfloor, Un, Pn, floor);
+    if constexpr (MIX || FAST) failed = cons_to_prim_mixed<Law>(G.gamma, G.pfloor, Un, Pn, floor);
     else failed = cons_to_prim<Law>(G.gamma, G.pfloor, Un, Pn, floor);
     if (failed != 0)
     {
@@ -617,7 +618,7 @@ __device__ __forceinline__ double finish_zone(const double* __restrict__ P0, dou
             int err = 0;
             const double Lz = zone_length(G, i, j, k);
             if constexpr (MIX) dtc = courant_of<Law, float>(Pn, G, Lz, err);
-            else dtc = courant_of<Law>(Pn, G, Lz, err);
+            else dtc = courant_of<Law, double, A>(Pn, G, Lz, err);
             if (err & ERR_SUPERLUMINAL) atomicAdd(&ctl->superluminal, 1ull);
             if (err & ERR_EIGEN) atomicAdd(&ctl->eigen_fail, 1ull);
         }

@@ -112,6 +112,25 @@ struct Rel
 template <typename R> struct RelMath { using type = Math<R>; };
 template <> struct RelMath<double> { using type = Rel; };
 
+// A law's arithmetic by default: Rel for the relativistic law (its binary64
+// form is held to a bound, not to the reference's rounding), the working
+// precision's own for the others.  The fast binary64 form passes Rel for the
+// MHD law as well.
+template <int Law, typename R>
+using LawMath = typename std::conditional<Law == 2, typename RelMath<R>::type, Math<R>>::type;
+
+// max / min of two values as one instruction in the working precision
+template <typename R> __device__ __forceinline__ R fmax2(R a, R b)
+{
+    if constexpr (sizeof(R) == 4) return __builtin_fmaxf(a, b);
+    else return __builtin_fmax(a, b);
+}
+template <typename R> __device__ __forceinline__ R fmin2(R a, R b)
+{
+    if constexpr (sizeof(R) == 4) return __builtin_fminf(a, b);
+    else return __builtin_fmin(a, b);
+}
+
 template <typename R> __device__ __forceinline__ R min2(R a, R b) { return a < b ? a : b; }
 template <typename R> __device__ __forceinline__ R max2(R a, R b) { return a > b ? a : b; }
 // MIN3 / MAX3 of src/RiemannSolvers.cpp:3-6
@@ -122,21 +141,21 @@ template <typename R> __device__ __forceinline__ int sgn(R x) { return (x > R(0)
 // ---------------------------------------------------------------------------
 // Reconstruction (src/Reconstruction.cpp:55-132)
 // ---------------------------------------------------------------------------
-template <typename R>
+template <typename R, typename A = Math<R>>
 __device__ __forceinline__ R minmod(R theta, R ul, R u0, R ur)
 {
     const R a = theta * (u0 - ul);
     const R b = R(0.5) * (ur - ul);
     const R c = theta * (ur - u0);
-    if constexpr (sizeof(R) == 4)
+    if constexpr (sizeof(R) == 4 || A::ieee_minmax)
     {
-        // the mixed form: the same value (the argument of least magnitude
-        // when all three share a sign, else zero) as min/max instructions,
-        // max(min(a, b, c), min(max(a, b, c), 0)); only the sign of a zero
-        // result can differ from the reference's product form
-        const R lo = __builtin_fminf(a, __builtin_fminf(b, c));
-        const R hi = __builtin_fmaxf(a, __builtin_fmaxf(b, c));
-        return __builtin_fmaxf(lo, __builtin_fminf(hi, R(0)));
+        // the mixed and fast forms: the same value (the argument of least
+        // magnitude when all three share a sign, else zero) as min/max
+        // instructions, max(min(a, b, c), min(max(a, b, c), 0)); only the
+        // sign of a zero result can differ from the reference's product form
+        const R lo = fmin2(a, fmin2(b, c));
+        const R hi = fmax2(a, fmax2(b, c));
+        return fmax2(lo, fmin2(hi, R(0)));
     }
     const R m = min3(Math<R>::abs(a), Math<R>::abs(b), Math<R>::abs(c));
     const int sa = sgn(a), sb = sgn(b), sc = sgn(c);
@@ -144,10 +163,10 @@ __device__ __forceinline__ R minmod(R theta, R ul, R u0, R ur)
 }
 
 // PLM_C2R (sgn=+1) / PLM_C2L (sgn=-1): v0 + sgn * 0.5 * minmod
-template <typename R>
+template <typename R, typename A = Math<R>>
 __device__ __forceinline__ R plm(R theta, R vl, R v0, R vr, R sgnv)
 {
-    return v0 + sgnv * R(0.5) * minmod(theta, vl, v0, vr);
+    return v0 + sgnv * R(0.5) * minmod<R, A>(theta, vl, v0, vr);
 }
 
 // weno5 with the FD tables and the ImprovedShenZha10 indicator, A = 50
@@ -334,12 +353,13 @@ __device__ __forceinline__ void hydro_state(R gamma, const R* P, const R* dAA, S
 }
 
 // NewtonianMHD::fromPrimitive (src/ConservationLaws.cpp:416-487)
-template <typename R>
+// A: the arithmetic (Math<R>, or Rel in the fast binary64 form)
+template <typename R, typename A = Math<R>>
 __device__ __forceinline__ void mhd_state(R gamma, const R* P, const R* dAA, State<R, 8>& S)
 {
     const R gm0 = gamma;
     const R gm1 = gamma - R(1.0);
-    const R cs2 = Math<R>::div(gm0 * P[4], P[0]);
+    const R cs2 = A::div_pos(gm0 * P[4], P[0]);
     const R vv = P[1] * P[1] + P[2] * P[2] + P[3] * P[3];
     const R BB = P[5] * P[5] + P[6] * P[6] + P[7] * P[7];
     const R Bv = P[5] * P[1] + P[6] * P[2] + P[7] * P[3];
@@ -352,7 +372,7 @@ __device__ __forceinline__ void mhd_state(R gamma, const R* P, const R* dAA, Sta
     S.U[1] = P[0] * P[1];
     S.U[2] = P[0] * P[2];
     S.U[3] = P[0] * P[3];
-    S.U[4] = P[0] * R(0.5) * vv + R(0.5) * BB + Math<R>::div(P[4], gm1);
+    S.U[4] = P[0] * R(0.5) * vv + R(0.5) * BB + A::div_pos(P[4], gm1);
     S.U[5] = P[5];
     S.U[6] = P[6];
     S.U[7] = P[7];
@@ -365,19 +385,20 @@ __device__ __forceinline__ void mhd_state(R gamma, const R* P, const R* dAA, Sta
     S.F[6] = vn * S.U[6] - Bn * P[2];
     S.F[7] = vn * S.U[7] - Bn * P[3];
     const R Bn2 = Bn * Bn;
-    const R ca2 = Math<R>::div(BB, P[0]);
-    const R cn2 = Math<R>::div(Bn2, P[0]);
+    const R ca2 = A::div_pos(BB, P[0]);
+    const R cn2 = A::div_pos(Bn2, P[0]);
     const R cw4 = (cs2 + ca2) * (cs2 + ca2);
     // (cs2 + ca2)^2 - 4 cs2 cn2 >= (cs2 - ca2)^2 >= 0 since cn2 <= ca2; in
-    // binary32 it can round below zero where cs2 ~ ca2 and B is along the
-    // normal, and the square root would then poison the fast speed.  The
-    // mixed form clamps it; binary64 keeps the reference's expression.
+    // binary32, or with contraction, it can round below zero where cs2 ~ ca2
+    // and B is along the normal, and the square root would then poison the
+    // fast speed.  The mixed and fast forms clamp it; the binary64 form
+    // keeps the reference's expression.
     R disc = cw4 - R(4) * cs2 * cn2;
-    if constexpr (sizeof(R) == 4) disc = disc < R(0) ? R(0) : disc;
-    const R root = Math<R>::sqrt(disc);
+    if constexpr (sizeof(R) == 4 || A::ieee_minmax) disc = disc < R(0) ? R(0) : disc;
+    const R root = A::sqrt(disc);
     const R cF2 = R(0.5) * (cs2 + ca2 + root);
     const R cS2 = R(0.5) * (cs2 + ca2 - root);
-    const R sF = Math<R>::sqrt(cF2), sA = Math<R>::sqrt(cn2), sS = Math<R>::sqrt(cS2);
+    const R sF = A::sqrt(cF2), sA = A::sqrt(cn2), sS = A::sqrt(cS2);
     S.A[0] = vn - sF;
     S.A[1] = vn - sA;
     S.A[2] = vn - sS;
@@ -520,15 +541,15 @@ __device__ __forceinline__ int rmhd_state(R gamma, const R* P, const R* dAA, Sta
 // numAuxiliary = 1, src/ConservationLaws.cpp:479-485).
 template <int Law> struct LawNQ { static constexpr int value = Law == 0 ? 5 : Law == 3 ? 9 : 8; };
 
-template <int Law, typename R>
+template <int Law, typename R, typename A = LawMath<Law, R>>
 __device__ __forceinline__ int law_state(R gamma, const R* P, const R* n, State<R, LawNQ<Law>::value>& S)
 {
     if constexpr (Law == 0) { hydro_state(gamma, P, n, S); return ERR_NONE; }
-    else if constexpr (Law == 1) { mhd_state(gamma, P, n, S); return ERR_NONE; }
+    else if constexpr (Law == 1) { mhd_state<R, A>(gamma, P, n, S); return ERR_NONE; }
     else if constexpr (Law == 3)
     {
         State<R, 8> m;
-        mhd_state(gamma, P, n, m);
+        mhd_state<R, A>(gamma, P, n, m);
 #pragma unroll
         for (int q = 0; q < 8; ++q)
         {

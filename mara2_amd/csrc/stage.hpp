@@ -265,6 +265,8 @@ void stage_kernels_f32(const StageArgs& s);
 // stage_kernels<double> / stage_kernels_f32 for the others
 void stage_kernels_fast64(const StageArgs& s);
 void stage_kernels_fast32(const StageArgs& s);
+// the fused 2-D MHD tiles in the fast binary64 form's arithmetic (fast.hip)
+void stage_kernels_fast64_mhd2d(const StageArgs& s);
 bool fast64_supported(const mara_desc* d);
 
 // Work-groups resident per CU and tiles per z layer of the fused 3-D kernel a

@@ -123,12 +123,24 @@ def test_launch_shapes_give_the_same_bits():
         assert np.array_equal(r, results[0])
 
 
+def _mhd_1d(desc, seed):
+    rng = np.random.default_rng(seed)
+    P = problems.empty(desc)
+    shp = P.shape[:3]
+    P[..., 0] = 1.0 + 0.5 * rng.random(shp)
+    P[..., 4] = 1.0 + 0.5 * rng.random(shp)
+    P[..., 1:4] = 0.3 * rng.standard_normal(shp + (3,))
+    P[..., 5:8] = 0.5 + 0.2 * rng.standard_normal(shp + (3,))
+    return problems.apply_bc(desc, P)
+
+
 def test_blocks_without_a_fast_kernel_run_the_plain_one():
-    """A fast-form block the fast kernel does not serve (WENO5 hydro, MHD)
-    runs the plain binary64 kernels and so equals the oracle bit for bit."""
+    """A fast-form block no fast kernel serves (WENO5 hydro, 1-D MHD) runs
+    the plain binary64 kernels and so equals the oracle bit for bit."""
     for desc in (abi.make_desc(law="hydro", recon="weno5", riemann="hllc", cells=(40, 30, 1), rk_order=3),
-                 problems.mhd_desc(32)):
-        P0 = _shock_2d(desc) if desc.law == abi.LAW_HYDRO else problems.mhd_2d(desc)
+                 abi.make_desc(law="mhd", recon="plm", riemann="hlle", bc="periodic", cells=(200, 1, 1),
+                               field_ct=1)):
+        P0 = _shock_2d(desc) if desc.law == abi.LAW_HYDRO else _mhd_1d(desc, 31)
         Po = P0.copy()
         with Block(_fast(desc)) as blk:
             blk.upload(P0)
@@ -137,3 +149,60 @@ def test_blocks_without_a_fast_kernel_run_the_plain_one():
                 blk.advance(0.3 * dto)
                 oracle.advance(desc, Po, 0.0, 0.3 * dto)
             assert np.array_equal(blk.download(), Po)
+
+
+# ---------------------------------------------------------------------------
+# 2-D MHD blocks: the fused MHD + flux-CT tiles in the fast form's arithmetic
+# (refined reciprocals and square roots, min/max minmod and wave-speed
+# extremes, one reciprocal of the density in the recovery, contraction).
+
+@pytest.mark.parametrize("n,rk", [(48, 2), (40, 3)])
+def test_mhd_2d_fast_tiles_ten_steps(n, rk):
+    desc = problems.mhd_desc(n)
+    desc.rk_order = rk
+    P0 = problems.mhd_2d(desc)
+    err, Pg, Po = _compare(desc, P0, 10, 0.3)
+    print(f"fast FP64 MHD {n}^2 RK{rk}: rel L1 {err:.3e}")
+    assert err <= REL_L1_FP64
+    assert not np.array_equal(Pg, Po)  # the fast tiles ran, not the plain ones
+    # flux-CT keeps the corner monopole at roundoff
+    e, mono = oracle.monopole_max(desc, Pg)
+    assert e == 0 and mono <= 1e-12 * np.abs(Pg[..., 5:8]).max()
+
+
+def test_mhd_2d_fast_tiles_without_ct_and_other_walls():
+    desc = abi.make_desc(law="mhd", recon="plm", riemann="hlle", bc="outflow", cells=(50, 36, 1), field_ct=0)
+    P0 = _mhd_1d(desc, 33)
+    err, _, _ = _compare(desc, P0, 10, 0.3)
+    assert err <= REL_L1_FP64, err
+
+
+def test_mhd_2d_fast_device_loop_against_oracle_loop():
+    """mara_block_run on the fast MHD tiles: their own CFL (in the same
+    arithmetic on the first step and in the last stage of the others) takes
+    as many steps to the same time as the oracle's maraMainLoop."""
+    desc = problems.mhd_desc(48)
+    P0 = problems.mhd_2d(desc)
+    Po = P0.copy()
+    err, to, so = oracle.run(desc, Po, 0.4, 0.02, max_steps=15)
+    assert err == 0
+    with Block(_fast(desc)) as blk:
+        blk.upload(P0)
+        tg, sg, rep = blk.run(0.4, 0.02, 15)
+        Pg = blk.download()
+    assert sg == so and abs(tg - to) <= 1e-13 * to
+    assert rel_l1(prim_to_cons(desc, interior(desc, Pg)), prim_to_cons(desc, interior(desc, Po))) <= REL_L1_FP64
+
+
+def test_mhd_2d_fast_tile_heights_give_the_same_bits():
+    desc = _fast(problems.mhd_desc(40))
+    P0 = problems.mhd_2d(problems.mhd_desc(40))
+    results = []
+    for rows in abi.ROW_SHAPES:
+        with Block(desc) as blk:
+            blk.set_shape(abi.make_shape(rows=rows))
+            blk.upload(P0)
+            blk.run(0.3, 1e30, 4)
+            results.append(blk.download())
+    for r in results[1:]:
+        assert np.array_equal(r, results[0])

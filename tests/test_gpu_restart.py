@@ -22,6 +22,7 @@ CASES = {
     "sod-rk3": lambda: (abi.make_desc(law="hydro", gamma=1.4, riemann="hlle", bc="outflow", cells=(256, 1, 1),
                                       rk_order=3), "sod"),
     "mhd2d-ct": lambda: (problems.mhd_desc(48), "mhd"),
+    "mhd2d-ct-fp64_fast": lambda: (problems.mhd_desc(48, "fp64_fast"), "mhd"),
     "rmhd3d-weno5-ct": lambda: (abi.make_desc(law="rmhd", recon="weno5", riemann="hlle", bc="periodic",
                                               cells=(16, 16, 16), rk_order=3, field_ct=1), "rmhd"),
 }
