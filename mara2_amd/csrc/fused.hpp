@@ -46,6 +46,32 @@ __device__ __forceinline__ int xcd_tile(int L, int ntiles)
     return x * q + (x < r ? x : r) + l;
 }
 
+// Whole-wavefront lane shifts by one through DPP (wave_shr:1 / wave_shl:1 on
+// every dword): lane l takes v of lane l - 1 (shift_up) or l + 1
+// (shift_down), and the lane with no source, 0 or 63, keeps `edge` (bound_ctrl
+// off).  One v_mov_b32_dpp per dword, where __shfl_up/__shfl_down take an
+// LDS permute and the edge a select (the fast tiles use these; the bit-exact
+// tiles keep the shuffles, which measured no slower there).  Called with all
+// 64 lanes active.
+template <int CTRL>
+__device__ __forceinline__ float dpp_shift(float v, float edge)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, edge),
+                                                                 __builtin_bit_cast(int, v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_shift(double v, double edge)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned long long e = __builtin_bit_cast(unsigned long long, edge);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp((int)(unsigned)e, (int)(unsigned)b, CTRL, 0xf, 0xf, false);
+    const unsigned hi =
+        (unsigned)__builtin_amdgcn_update_dpp((int)(unsigned)(e >> 32), (int)(unsigned)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+template <typename R> __device__ __forceinline__ R shift_up(R v, R edge) { return dpp_shift<0x138>(v, edge); }
+template <typename R> __device__ __forceinline__ R shift_down(R v, R edge) { return dpp_shift<0x130>(v, edge); }
+
 // Flux through one face from a stencil of 2 NG zones per field (reference
 // IntercellFluxScheme + RiemannSolver, src/IntercellFluxSchemes.cpp:45-117).
 template <int Recon, int Riem, typename R>

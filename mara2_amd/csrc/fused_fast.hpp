@@ -369,8 +369,7 @@ __device__ __forceinline__ void fast_plm_tile(const double* __restrict__ Pin, co
 #pragma unroll
         for (int q = 0; q < 5; ++q)
         {
-            const R up = __shfl_up(hx[q], 1, 64);
-            hl[q] = lane == 0 ? sxe[q][row][0] : up;
+            hl[q] = shift_up(hx[q], sxe[q][row][0]);
         }
         if (xface)
         {
@@ -439,8 +438,7 @@ __device__ __forceinline__ void fast_plm_tile(const double* __restrict__ Pin, co
 #pragma unroll
         for (int q = 0; q < 5; ++q)
         {
-            const R down = __shfl_down(Fx[q], 1, 64);
-            Fxr[q] = lane == TX - 1 ? fxe[q][row] : down;
+            Fxr[q] = shift_down(Fx[q], fxe[q][row]);
         }
         if (cell_xy)
         {
