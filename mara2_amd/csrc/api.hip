@@ -288,6 +288,11 @@ void launch_stage(mara_block* b, int rk, int courant_out)
     s.kz = b->kz;
     s.rows = b->rows;
     s.S = b->src_fn || cooling ? b->Ssrc : nullptr;
+    // the fast tiles of a periodic single block read their halos from the
+    // interior (StageArgs::wrap): the guards of an intermediate stage's
+    // output are then never read, and only the last stage fills them
+    const bool fast_tile = b->fused && fast64_supported(&b->d) && (mixed || b->d.precision == MARA_PREC_FP64_FAST);
+    s.wrap = fast_tile && b->G.bc == 0 && !b->in_group;
     b->vis_args.P = Pin;
     s.after_flux = b->vis_fn ? launch_viscous : nullptr;
     for (int a = 0; a < 3; ++a) s.Fv[a] = b->vis_fn ? b->Fv[a] : nullptr;
@@ -301,8 +306,9 @@ void launch_stage(mara_block* b, int rk, int courant_out)
     // the flux-array update writes the ghost zones of Pout itself
     // (write_guard_images); the fused kernels leave them to a ghost-fill pass,
     // and a group member's guards come from the group's exchange after the
-    // stage (the images the update wrote are then overwritten)
-    if (b->fused && !b->in_group) launch_bc(b, Pout);
+    // stage (the images the update wrote are then overwritten).  Each step
+    // ends with its guards filled.
+    if (b->fused && !b->in_group && (last || !s.wrap)) launch_bc(b, Pout);
 }
 
 // ---- tuning table ----------------------------------------------------------

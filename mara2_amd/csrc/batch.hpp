@@ -154,7 +154,8 @@ kb_fast_plm(const BItem* __restrict__ items, const BTile* __restrict__ tiles, do
     const Geom G = it.G;
     const double* Pin = first ? it.Pa : it.Pb;
     double* Pout = last ? it.Pa : it.Pb;
-    fast64::fast_plm_tile<float, Riem, Z>(Pin, it.Pa, Pout, G, b, first, courant_out, it.ctl, T.x0, T.y0, T.z0, kz);
+    fast64::fast_plm_tile<float, Riem, Z>(Pin, it.Pa, Pout, G, b, first, courant_out, it.ctl, T.x0, T.y0, T.z0, kz,
+                                          false);
 }
 
 // ---------------------------------------------------------------------------
@@ -221,7 +222,7 @@ kbq_fast_plm(const BItem* __restrict__ items, const BTile* __restrict__ tiles, i
         const double* Pin = first ? it.Pa : it.Pb;
         double* Pout = last ? it.Pa : it.Pb;
         fast64::fast_plm_tile<float, Riem, Z>(Pin, it.Pa, Pout, it.G, b, first, courant_out, it.ctl, T.x0, T.y0,
-                                              T.z0, kz);
+                                              T.z0, kz, false);
     }
 }
 

@@ -15,12 +15,14 @@ bool fast64_supported(const mara_desc* d) { return fast64::supported(d); }
 
 void stage_kernels_fast64(const StageArgs& s)
 {
-    fast64::launch<double>(*s.G, s.stream, s.d->riemann, s.Pin, s.P0, s.Pout, s.b, s.first, s.cout, s.ctl, s.kz);
+    fast64::launch<double>(*s.G, s.stream, s.d->riemann, s.Pin, s.P0, s.Pout, s.b, s.first, s.cout, s.ctl, s.kz,
+                          s.wrap);
 }
 
 void stage_kernels_fast32(const StageArgs& s)
 {
-    fast64::launch<float>(*s.G, s.stream, s.d->riemann, s.Pin, s.P0, s.Pout, s.b, s.first, s.cout, s.ctl, s.kz);
+    fast64::launch<float>(*s.G, s.stream, s.d->riemann, s.Pin, s.P0, s.Pout, s.b, s.first, s.cout, s.ctl, s.kz,
+                          s.wrap);
 }
 
 // 2-D MHD blocks of the fast binary64 form: the fused MHD + CT tiles with the

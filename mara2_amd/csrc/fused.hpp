@@ -72,6 +72,18 @@ __device__ __forceinline__ double dpp_shift(double v, double edge)
 template <typename R> __device__ __forceinline__ R shift_up(R v, R edge) { return dpp_shift<0x138>(v, edge); }
 template <typename R> __device__ __forceinline__ R shift_down(R v, R edge) { return dpp_shift<0x130>(v, edge); }
 
+// Index s along axis a of a periodic block, with a guard zone replaced by the
+// interior zone whose value the periodic ghost fill copies into it (s < g:
+// s + n; s >= g + n: s - n), then clamped to the array (ragged tiles read
+// past the mesh into lanes that update nothing).  Tiles that read their
+// halos through it do not need the guards filled.
+__device__ __forceinline__ int periodic_index(const Geom& G, int a, int s)
+{
+    const int g = G.g[a], n = G.n[a];
+    s = s < g ? s + n : (s >= g + n ? s - n : s);
+    return min(max(s, 0), G.N[a] - 1);
+}
+
 // Flux through one face from a stencil of 2 NG zones per field (reference
 // IntercellFluxScheme + RiemannSolver, src/IntercellFluxSchemes.cpp:45-117).
 template <int Recon, int Riem, typename R>
@@ -346,8 +358,10 @@ struct PlaneRing
     R v[PH][5];
     int pos[PH];
 
+    // wrap: read the halo of a periodic block from the interior images of
+    // its guard zones (periodic_index) instead of the guards
     __device__ __forceinline__ void load(const double* __restrict__ Pin, const Geom& G, long long pbase, int i0,
-                                         int j0, int tid)
+                                         int j0, int tid, bool wrap = false)
     {
 #pragma unroll
         for (int p = 0; p < PH; ++p)
@@ -379,7 +393,8 @@ struct PlaneRing
                     hy = h / HX;
                     hx = h - hy * HX;
                 }
-                int off = min(j0 - NG + hy, G.N[1] - 1) * G.N[0] + min(i0 - NG + hx, G.N[0] - 1);
+                int off = wrap ? periodic_index(G, 1, j0 - NG + hy) * G.N[0] + periodic_index(G, 0, i0 - NG + hx)
+                               : min(j0 - NG + hy, G.N[1] - 1) * G.N[0] + min(i0 - NG + hx, G.N[0] - 1);
                 // an opaque copy: the offset is recomputed per plane rather
                 // than its five 64-bit field addresses held across the march
                 asm volatile("" : "+v"(off));

@@ -223,7 +223,8 @@ template <typename R> struct FastTile
 template <typename R, int Riem, bool Z>
 __device__ __forceinline__ void fast_plm_tile(const double* __restrict__ Pin, const double* __restrict__ P0,
                                               double* __restrict__ Pout, const Geom& G, double b, int first_stage,
-                                              int courant_out, Ctl* ctl, int tx, int ty, int tz, int kz)
+                                              int courant_out, Ctl* ctl, int tx, int ty, int tz, int kz,
+                                              bool wrap)
 {
     constexpr bool F64 = sizeof(R) == 8;
     constexpr int NG = 2, TX = FastTile<R>::TX, TY = FastTile<R>::TY;
@@ -258,8 +259,13 @@ __device__ __forceinline__ void fast_plm_tile(const double* __restrict__ Pin, co
     const R A0 = R(G.dA[0]), A1 = R(G.dA[1]), A2 = R(G.dA[2]);
     const R fac = R(-1.0 / G.dV);
     const double dt = ctl->dt;
-    const int ic = min(i, G.N[0] - 1), jc = min(j, G.N[1] - 1);
+    // wrap (periodic blocks): the halo, the zone columns past the tile's
+    // edges and the z window's guard planes come from their interior images,
+    // so the stage reads no guard zone (periodic_index)
+    const int ic = wrap ? periodic_index(G, 0, i) : min(i, G.N[0] - 1);
+    const int jc = wrap ? periodic_index(G, 1, j) : min(j, G.N[1] - 1);
     const long long col = (long long)jc * G.N[0] + ic;
+    const auto zplane = [&](int kk) { return (long long)(wrap ? periodic_index(G, 2, kk) : kk) * sxy; };
 
     R W[5][4];  // z window P(k - 1 .. k + 2) of the zone's column
     R hzc[5];   // z half-slope of zone k
@@ -270,9 +276,9 @@ __device__ __forceinline__ void fast_plm_tile(const double* __restrict__ Pin, co
 #pragma unroll
         for (int q = 0; q < 5; ++q)
         {
-            w0[q] = R(Pin[q * G.fs + col + (long long)(kb - 2) * sxy]);
+            w0[q] = R(Pin[q * G.fs + col + zplane(kb - 2)]);
 #pragma unroll
-            for (int m = 0; m < 3; ++m) W[q][m + 1] = R(Pin[q * G.fs + col + (long long)(kb - 1 + m) * sxy]);
+            for (int m = 0; m < 3; ++m) W[q][m + 1] = R(Pin[q * G.fs + col + zplane(kb - 1 + m)]);
         }
         R PL[5], PR[5];
 #pragma unroll
@@ -295,12 +301,12 @@ __device__ __forceinline__ void fast_plm_tile(const double* __restrict__ Pin, co
         // plane k: the ring from memory (PlaneRing) and (3-D) the own zone
         // from the z window; the window's next value is loaded with the ring
         PlaneRing<R, TX, TY, Z> ring;
-        ring.load(Pin, G, pbase, i0, j0, tid);
+        ring.load(Pin, G, pbase, i0, j0, tid, wrap);
         R wn[5];
         if constexpr (Z)
         {
 #pragma unroll
-            for (int q = 0; q < 5; ++q) wn[q] = R(Pin[q * G.fs + colk + (long long)(k + 2) * sxy]);
+            for (int q = 0; q < 5; ++q) wn[q] = R(Pin[q * G.fs + colk + zplane(k + 2)]);
 #pragma unroll
             for (int q = 0; q < 5; ++q) plane[q][row + NG][lane + NG] = W[q][2];
         }
@@ -504,7 +510,7 @@ template <typename R, int Riem, bool Z>
 static __global__ void __launch_bounds__(FastTile<R>::TX * FastTile<R>::TY)
     __attribute__((amdgpu_waves_per_eu(FastTile<R>::WAVES, 8)))
 k_fast_plm(const double* __restrict__ Pin, const double* __restrict__ P0, double* __restrict__ Pout, Geom G,
-           double b, int first_stage, int courant_out, Ctl* ctl, int kz)
+           double b, int first_stage, int courant_out, Ctl* ctl, int kz, int wrap)
 {
     if (!ctl->active) return;
     constexpr int TX = FastTile<R>::TX, TY = FastTile<R>::TY;
@@ -512,7 +518,7 @@ k_fast_plm(const double* __restrict__ Pin, const double* __restrict__ P0, double
     const int ntz = Z ? (G.n[2] + kz - 1) / kz : 1;
     const int tile = xcd_tile(blockIdx.x, ntx * nty * ntz);
     const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
-    fast_plm_tile<R, Riem, Z>(Pin, P0, Pout, G, b, first_stage, courant_out, ctl, tx, ty, tz, kz);
+    fast_plm_tile<R, Riem, Z>(Pin, P0, Pout, G, b, first_stage, courant_out, ctl, tx, ty, tz, kz, wrap != 0);
 }
 
 // the blocks this kernel serves: hydro, PLM, 2-D or 3-D, two or three stages
@@ -541,7 +547,7 @@ inline void tile_round(const Geom& G, int riemann, int* per_cu, int* per_layer)
 // float for the mixed form.
 template <typename R>
 inline void launch(const Geom& G, hipStream_t stream, int riemann, const double* Pin, const double* P0,
-                   double* Pout, double b, int first, int cout, Ctl* ctl, int kz)
+                   double* Pout, double b, int first, int cout, Ctl* ctl, int kz, int wrap)
 {
     constexpr int TX = FastTile<R>::TX, TY = FastTile<R>::TY;
     const bool z = G.N[2] > 1;
@@ -550,13 +556,13 @@ inline void launch(const Geom& G, hipStream_t stream, int riemann, const double*
     const dim3 grid(((G.n[0] + TX - 1) / TX) * ((G.n[1] + TY - 1) / TY) * (z ? (G.n[2] + kz - 1) / kz : 1));
     if (z)
     {
-        if (hllc) k_fast_plm<R, 1, true><<<grid, block, 0, stream>>>(Pin, P0, Pout, G, b, first, cout, ctl, kz);
-        else k_fast_plm<R, 0, true><<<grid, block, 0, stream>>>(Pin, P0, Pout, G, b, first, cout, ctl, kz);
+        if (hllc) k_fast_plm<R, 1, true><<<grid, block, 0, stream>>>(Pin, P0, Pout, G, b, first, cout, ctl, kz, wrap);
+        else k_fast_plm<R, 0, true><<<grid, block, 0, stream>>>(Pin, P0, Pout, G, b, first, cout, ctl, kz, wrap);
     }
     else
     {
-        if (hllc) k_fast_plm<R, 1, false><<<grid, block, 0, stream>>>(Pin, P0, Pout, G, b, first, cout, ctl, kz);
-        else k_fast_plm<R, 0, false><<<grid, block, 0, stream>>>(Pin, P0, Pout, G, b, first, cout, ctl, kz);
+        if (hllc) k_fast_plm<R, 1, false><<<grid, block, 0, stream>>>(Pin, P0, Pout, G, b, first, cout, ctl, kz, wrap);
+        else k_fast_plm<R, 0, false><<<grid, block, 0, stream>>>(Pin, P0, Pout, G, b, first, cout, ctl, kz, wrap);
     }
 }
 

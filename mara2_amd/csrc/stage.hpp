@@ -35,6 +35,9 @@ struct StageArgs
     int rows;    // flux-array kernels: work-group rows of 64 zones; fused 2-D
                  // MHD tiles: 64 x (4 * rows) zones
     const double* S;  // source terms added to L (same layout as P), or null
+    // fast tiles of a periodic block: read the halos from the interior images
+    // of the guard zones, so Pin's guards need not be filled
+    bool wrap = false;
     // flux-array path: called between the flux sweeps and the zone update
     // (the viscous-flux hook enqueues its kernel there), or null
     void (*after_flux)(void*);
