@@ -101,6 +101,10 @@ struct mara_block
     int rows = ROWS_DEFAULT;     // flux-array kernels: work-group rows
     int pencil_shape = PENCIL_DEFAULT;  // requested pencil length (all laws)
     bool in_group = false;    // a group member: the group's exchange fills its guards
+    // the guards of Pa may not hold the ghost fill of its interior: the
+    // stages of the fast tiles of a periodic block neither read nor fill them
+    // (StageArgs::wrap); whatever reads them next fills them first (fill_guards)
+    bool guards_stale = false;
     double* sph = nullptr;    // spherical mesh: the geometry tables (G.sph)
     Ctl* ctl = nullptr;
     Ctl* ctl_host = nullptr;
@@ -154,6 +158,27 @@ void launch_bc(mara_block* b, double* P)
     const long long total = 2ll * G.g[0] * G.N[1] * G.N[2] + (long long)G.n[0] * 2 * G.g[1] * G.N[2]
                             + (long long)G.n[0] * G.n[1] * 2 * G.g[2];
     if (total > 0) k_ghost<<<grid_for(total), kThreads, 0, b->stream>>>(P, G, total, b->ctl);
+}
+
+// The ghost fill of Pa where its guards may be stale, whatever the state of
+// the device loop (a run's stop condition leaves the control block inactive)
+void fill_guards(mara_block* b)
+{
+    if (!b->guards_stale) return;
+    const Geom& G = b->G;
+    const long long total = 2ll * G.g[0] * G.N[1] * G.N[2] + (long long)G.n[0] * 2 * G.g[1] * G.N[2]
+                            + (long long)G.n[0] * G.n[1] * 2 * G.g[2];
+    if (total > 0) k_ghost<<<grid_for(total), kThreads, 0, b->stream>>>(b->Pa, G, total, nullptr);
+    b->guards_stale = false;
+}
+
+// whether the stage kernels of the block read their halos from the interior
+// (the fast tiles of a periodic single block, StageArgs::wrap)
+bool stage_wraps(const mara_block* b)
+{
+    const bool mixed = b->d.precision == MARA_PREC_MIXED;
+    const bool fast_tile = b->fused && fast64_supported(&b->d) && (mixed || b->d.precision == MARA_PREC_FP64_FAST);
+    return fast_tile && b->G.bc == 0 && !b->in_group;
 }
 
 // The arithmetic of the CFL time step the block's last stage produces
@@ -247,6 +272,9 @@ void launch_stage(mara_block* b, int rk, int courant_out)
     double* Pout;
     stage_io(b, rk, &Pin, &Pout);
     const bool mixed = b->d.precision == MARA_PREC_MIXED;
+    const bool wrap = stage_wraps(b);
+    // a stage that reads guards starts from filled ones
+    if (rk == 0 && !wrap) fill_guards(b);
 
     // NewtonianMHD cooling (ConservationLaw::addSourceTerms) into S
     const bool cooling = b->G.nq == 9;
@@ -288,11 +316,7 @@ void launch_stage(mara_block* b, int rk, int courant_out)
     s.kz = b->kz;
     s.rows = b->rows;
     s.S = b->src_fn || cooling ? b->Ssrc : nullptr;
-    // the fast tiles of a periodic single block read their halos from the
-    // interior (StageArgs::wrap): the guards of an intermediate stage's
-    // output are then never read, and only the last stage fills them
-    const bool fast_tile = b->fused && fast64_supported(&b->d) && (mixed || b->d.precision == MARA_PREC_FP64_FAST);
-    s.wrap = fast_tile && b->G.bc == 0 && !b->in_group;
+    s.wrap = wrap;
     b->vis_args.P = Pin;
     s.after_flux = b->vis_fn ? launch_viscous : nullptr;
     for (int a = 0; a < 3; ++a) s.Fv[a] = b->vis_fn ? b->Fv[a] : nullptr;
@@ -306,9 +330,14 @@ void launch_stage(mara_block* b, int rk, int courant_out)
     // the flux-array update writes the ghost zones of Pout itself
     // (write_guard_images); the fused kernels leave them to a ghost-fill pass,
     // and a group member's guards come from the group's exchange after the
-    // stage (the images the update wrote are then overwritten).  Each step
-    // ends with its guards filled.
-    if (b->fused && !b->in_group && (last || !s.wrap)) launch_bc(b, Pout);
+    // stage (the images the update wrote are then overwritten).  The fast
+    // tiles of a periodic block read no guards: their output's are filled
+    // when something reads them (fill_guards).
+    if (b->fused && !b->in_group)
+    {
+        if (!wrap) launch_bc(b, Pout);
+        else if (last) b->guards_stale = true;
+    }
 }
 
 // ---- tuning table ----------------------------------------------------------
@@ -663,6 +692,7 @@ int mara_block_autotune(mara_block* b, int repeats, mara_shape* chosen, double* 
         s.kz = c.zplanes;
         s.rows = c.rows;
         s.S = b->src_fn || b->G.nq == 9 ? b->Ssrc : nullptr;
+        s.wrap = stage_wraps(b);
         s.after_flux = nullptr;  // the tuner times the stage kernels only
         s.after_flux_ctx = nullptr;
         const bool mixed = b->d.precision == MARA_PREC_MIXED;
@@ -763,6 +793,7 @@ int mara_block_upload_prim(mara_block* b, const double* P)
     const size_t n = (size_t)b->S.ncell * b->S.nq;
     HIP_TRY(hipMemcpyAsync(b->U, P, n * sizeof(double), hipMemcpyHostToDevice, b->stream));
     k_aos_to_soa<<<grid_for(b->S.ncell), kThreads, 0, b->stream>>>(b->U, b->Pa, b->G);
+    b->guards_stale = false;
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(b->stream));
     return MARA_OK;
@@ -772,6 +803,7 @@ int mara_block_download_prim(mara_block* b, double* P)
 {
     if (!b || !P) return MARA_ERR_INVALID_ARGUMENT;
     const size_t n = (size_t)b->S.ncell * b->S.nq;
+    fill_guards(b);
     k_soa_to_aos<<<grid_for(b->S.ncell), kThreads, 0, b->stream>>>(b->Pa, b->U, b->G);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(P, b->U, n * sizeof(double), hipMemcpyDeviceToHost, b->stream));
@@ -784,6 +816,7 @@ int mara_block_apply_bc(mara_block* b)
     if (!b) return MARA_ERR_INVALID_ARGUMENT;
     k_ctl_begin_courant<<<1, 1, 0, b->stream>>>(b->ctl);  // sets active
     launch_bc(b, b->Pa);
+    b->guards_stale = false;
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(b->stream));
     return MARA_OK;
@@ -1252,6 +1285,7 @@ extern "C" int mara_advance_noalloc(const mara_desc* d, const double* P_in, doub
     k_aos_to_soa<<<grid_for(b.S.ncell), kThreads, 0, b.stream>>>(P_in, b.Pa, b.G);
     k_ctl_begin_advance<<<1, 1, 0, b.stream>>>(b.ctl, dt);
     for (int rk = 0; rk < b.d.rk_order; ++rk) launch_stage(&b, rk, 0);
+    fill_guards(&b);
     k_soa_to_aos<<<grid_for(b.S.ncell), kThreads, 0, b.stream>>>(b.Pa, P_out, b.G);
     int status = MARA_OK;
     if (!hip_ok(hipGetLastError())) status = MARA_ERR_HIP;

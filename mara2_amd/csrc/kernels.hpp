@@ -981,7 +981,7 @@ __device__ __forceinline__ long long ghost_count(const Geom& G)
 static __global__ void __launch_bounds__(256)
 k_ghost(double* __restrict__ P, Geom G, long long total, const Ctl* __restrict__ ctl)
 {
-    if (!ctl->active) return;
+    if (ctl && !ctl->active) return;  // a null control block: fill unconditionally
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total) return;
     ghost_zone(P, G, t);
