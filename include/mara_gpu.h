@@ -151,7 +151,10 @@ int mara_block_create(const mara_desc* desc, mara_block** out);
 int mara_block_destroy(mara_block* block);
 
 /* Copy primitives, guards included, in MeshData::P layout (see above),
-   between host and device.  Count of doubles = prod(n+2g) * nq. */
+   between host and device.  Count of doubles = prod(n+2g) * nq.  The
+   download's guards hold the boundary condition of its interior, as after
+   the reference's advance (a periodic block on the fast tiles fills them
+   here rather than after every stage). */
 int mara_block_upload_prim(mara_block* block, const double* P);
 int mara_block_download_prim(mara_block* block, double* P);
 
