@@ -5,6 +5,8 @@ reference's FP64 parity is stated with -- relative L1 <= 1e-12 of the
 conserved fields after 10 steps -- instead of bit equality.  Both sides take
 the oracle's time steps, so the comparison is of the scheme alone; the
 device loop (its own CFL) is compared separately."""
+import os
+
 import numpy as np
 import pytest
 
@@ -206,3 +208,35 @@ def test_mhd_2d_fast_tile_heights_give_the_same_bits():
             results.append(blk.download())
     for r in results[1:]:
         assert np.array_equal(r, results[0])
+
+
+GRAVITY = (os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "mara2_amd", "sources",
+                        "uniform_gravity.hsaco"), "mara_source_uniform_gravity")
+
+
+@pytest.mark.parametrize("form", ["fp64_fast", "mixed"])
+def test_periodic_guards_filled_when_read(form):
+    """The fast tiles of a periodic block read their halos from the interior
+    and leave the guards to whatever reads them next: the download holds the
+    periodic ghost fill of its interior, and a stage path that reads guards
+    (a source term moves the block to the flux arrays) starts from filled
+    ones -- the same result as a block uploaded from the downloaded state."""
+    desc = abi.make_desc(law="hydro", riemann="hllc", bc="periodic", cells=(40, 36, 20), rk_order=2,
+                         precision=form)
+    P0 = _random_state(desc, 31)
+    with Block(desc) as blk:
+        blk.upload(P0)
+        blk.run(0.3, 1e300, 3, check=False)
+        P1 = blk.download()
+        assert np.array_equal(P1, problems.apply_bc(desc, P1.copy())), "download guards are not the ghost fill"
+    with Block(desc) as a, Block(desc) as b:
+        a.upload(P0)
+        a.run(0.3, 1e300, 3, check=False)
+        a.set_source(GRAVITY[0], GRAVITY[1], (0.0, 0.0, 0.0))
+        a.advance(2e-3)
+        b.upload(P1)
+        b.set_source(GRAVITY[0], GRAVITY[1], (0.0, 0.0, 0.0))
+        b.advance(2e-3)
+        Pa, Pb = a.download(), b.download()
+    assert np.all(np.isfinite(Pa))
+    assert np.array_equal(Pa, Pb), f"{np.count_nonzero(Pa != Pb)} values differ"
