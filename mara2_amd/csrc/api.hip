@@ -102,7 +102,7 @@ struct mara_block
     int pencil_shape = PENCIL_DEFAULT;  // requested pencil length (all laws)
     bool in_group = false;    // a group member: the group's exchange fills its guards
     // the guards of Pa may not hold the ghost fill of its interior: the
-    // stages of the fast tiles of a periodic block neither read nor fill them
+    // stages of the PLM tiles of a periodic block neither read nor fill them
     // (StageArgs::wrap); whatever reads them next fills them first (fill_guards)
     bool guards_stale = false;
     double* sph = nullptr;    // spherical mesh: the geometry tables (G.sph)
@@ -173,12 +173,14 @@ void fill_guards(mara_block* b)
 }
 
 // whether the stage kernels of the block read their halos from the interior
-// (the fast tiles of a periodic single block, StageArgs::wrap)
+// (the fast tiles and the bit-exact PLM tile of a periodic single block,
+// StageArgs::wrap)
 bool stage_wraps(const mara_block* b)
 {
     const bool mixed = b->d.precision == MARA_PREC_MIXED;
     const bool fast_tile = b->fused && fast64_supported(&b->d) && (mixed || b->d.precision == MARA_PREC_FP64_FAST);
-    return fast_tile && b->G.bc == 0 && !b->in_group;
+    const bool exact_plm = b->fused && b->d.precision == MARA_PREC_FP64 && b->d.recon == MARA_RECON_PLM;
+    return (fast_tile || exact_plm) && b->G.bc == 0 && !b->in_group;
 }
 
 // The arithmetic of the CFL time step the block's last stage produces
@@ -330,7 +332,7 @@ void launch_stage(mara_block* b, int rk, int courant_out)
     // the flux-array update writes the ghost zones of Pout itself
     // (write_guard_images); the fused kernels leave them to a ghost-fill pass,
     // and a group member's guards come from the group's exchange after the
-    // stage (the images the update wrote are then overwritten).  The fast
+    // stage (the images the update wrote are then overwritten).  The PLM
     // tiles of a periodic block read no guards: their output's are filled
     // when something reads them (fill_guards).
     if (b->fused && !b->in_group)

@@ -137,7 +137,7 @@ kb_fused_plm(const BItem* __restrict__ items, const BTile* __restrict__ tiles, d
     const double* Pin = first ? it.Pa : it.Pb;
     double* Pout = last ? it.Pa : it.Pb;
     fused_plm_tile<Riem, R, Z>(Pin, it.Pa, it.U, Pout, G, b, first, !last, courant_out, it.ctl, T.x0, T.y0, T.z0,
-                               kz);
+                               kz, false);
 }
 
 // The mixed form's 2-D and 3-D hydro + PLM items: the single block's tiles
@@ -203,7 +203,7 @@ kbq_fused_plm(const BItem* __restrict__ items, const BTile* __restrict__ tiles, 
         const double* Pin = first ? it.Pa : it.Pb;
         double* Pout = last ? it.Pa : it.Pb;
         fused_plm_tile<Riem, R, Z>(Pin, it.Pa, it.U, Pout, it.G, b, first, !last, courant_out, it.ctl, T.x0, T.y0,
-                                   T.z0, kz);
+                                   T.z0, kz, false);
     }
 }
 

@@ -462,7 +462,7 @@ template <int Riem, typename R, bool Z>
 __device__ __forceinline__ void fused_plm_tile(const double* __restrict__ Pin, const double* __restrict__ P0,
                                                double* __restrict__ U, double* __restrict__ Pout, const Geom& G,
                                                double b, int first_stage, int write_u, int courant_out, Ctl* ctl,
-                                               int tx, int ty, int tz, int kz)
+                                               int tx, int ty, int tz, int kz, bool wrap)
 {
     constexpr int NG = 2;
     constexpr int TX = FUSED_TX, TY = PlmTile<R>::TY;
@@ -497,8 +497,12 @@ __device__ __forceinline__ void fused_plm_tile(const double* __restrict__ Pin, c
     const R A0 = R(G.dA[0]), A1 = R(G.dA[1]), A2 = R(G.dA[2]);
     const R fac = R(-1.0 / G.dV);
     const double dt = ctl->dt;
-    const int ic = min(i, G.N[0] - 1), jc = min(j, G.N[1] - 1);
+    // wrap (periodic blocks): halos, columns past the tile and the z
+    // window's guard planes from their interior images (periodic_index)
+    const int ic = wrap ? periodic_index(G, 0, i) : min(i, G.N[0] - 1);
+    const int jc = wrap ? periodic_index(G, 1, j) : min(j, G.N[1] - 1);
     const long long col = (long long)jc * G.N[0] + ic;
+    const auto zplane = [&](int kk) { return (long long)(wrap ? periodic_index(G, 2, kk) : kk) * sxy; };
 
     // z window W[q][m] = P(k - 1 + m), m = 0..3, and the carried half-slope of zone k
     R W[5][4];
@@ -510,9 +514,9 @@ __device__ __forceinline__ void fused_plm_tile(const double* __restrict__ Pin, c
 #pragma unroll
         for (int q = 0; q < 5; ++q)
         {
-            w0[q] = R(Pin[q * G.fs + col + (long long)(kb - 2) * sxy]);
+            w0[q] = R(Pin[q * G.fs + col + zplane(kb - 2)]);
 #pragma unroll
-            for (int m = 0; m < 3; ++m) W[q][m + 1] = R(Pin[q * G.fs + col + (long long)(kb - 1 + m) * sxy]);
+            for (int m = 0; m < 3; ++m) W[q][m + 1] = R(Pin[q * G.fs + col + zplane(kb - 1 + m)]);
         }
         // lower z-face of plane kb: zones kb-1 and kb
         R PL[5], PR[5];
@@ -537,12 +541,12 @@ __device__ __forceinline__ void fused_plm_tile(const double* __restrict__ Pin, c
         // window, loaded two planes ago; the window's next value is loaded
         // with the ring
         PlaneRing<R, TX, TY, Z> ring;
-        ring.load(Pin, G, pbase, i0, j0, tid);
+        ring.load(Pin, G, pbase, i0, j0, tid, wrap);
         R wn[5];
         if constexpr (Z)
         {
 #pragma unroll
-            for (int q = 0; q < 5; ++q) wn[q] = R(Pin[q * G.fs + colk + (long long)(k + 2) * sxy]);
+            for (int q = 0; q < 5; ++q) wn[q] = R(Pin[q * G.fs + colk + zplane(k + 2)]);
 #pragma unroll
             for (int q = 0; q < 5; ++q) plane[q][row + NG][lane + NG] = W[q][2];
         }
@@ -738,7 +742,7 @@ template <int Riem, typename R, bool Z>
 static __global__ void __launch_bounds__(FUSED_TX * PlmTile<R>::TY) __attribute__((amdgpu_waves_per_eu(PlmTile<R>::WAVES, 8)))
 k_fused_plm(const double* __restrict__ Pin, const double* __restrict__ P0, double* __restrict__ U,
             double* __restrict__ Pout, Geom G, double b, int first_stage, int write_u, int courant_out, Ctl* ctl,
-            int kz)
+            int kz, int wrap)
 {
     if (!ctl->active) return;
     constexpr int TX = FUSED_TX, TY = PlmTile<R>::TY;
@@ -746,7 +750,8 @@ k_fused_plm(const double* __restrict__ Pin, const double* __restrict__ P0, doubl
     const int ntz = Z ? (G.n[2] + kz - 1) / kz : 1;
     const int tile = xcd_tile(blockIdx.x, ntx * nty * ntz);
     const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
-    fused_plm_tile<Riem, R, Z>(Pin, P0, U, Pout, G, b, first_stage, write_u, courant_out, ctl, tx, ty, tz, kz);
+    fused_plm_tile<Riem, R, Z>(Pin, P0, U, Pout, G, b, first_stage, write_u, courant_out, ctl, tx, ty, tz, kz,
+                               wrap != 0);
 }
 
 // The fused path covers the hydro law on 2-D and 3-D blocks; 1-D blocks and
@@ -782,7 +787,7 @@ inline const char* fused_kernel_name(const mara_desc& d)
 // not depend on it)
 template <int Recon, int Riem, typename R>
 inline void launch_fused_t(const Geom& G, hipStream_t s, const double* Pin, const double* P0, double* U,
-                           double* Pout, double b, int first, int write_u, int cout, Ctl* ctl, int kz)
+                           double* Pout, double b, int first, int write_u, int cout, Ctl* ctl, int kz, int wrap)
 {
     const bool z = G.N[2] > 1;
     dim3 block(FUSED_TX, FUSED_TY, 1);
@@ -794,8 +799,13 @@ inline void launch_fused_t(const Geom& G, hipStream_t s, const double* Pin, cons
         const dim3 pblock(FUSED_TX, TY, 1);
         const dim3 pgrid(((G.n[0] + FUSED_TX - 1) / FUSED_TX) * ((G.n[1] + TY - 1) / TY)
                          * (z ? (G.n[2] + kz - 1) / kz : 1));
-        if (z) k_fused_plm<Riem, R, true><<<pgrid, pblock, 0, s>>>(Pin, P0, U, Pout, G, b, first, write_u, cout, ctl, kz);
-        else k_fused_plm<Riem, R, false><<<pgrid, pblock, 0, s>>>(Pin, P0, U, Pout, G, b, first, write_u, cout, ctl, kz);
+        // wrap: the PLM tile reads a periodic block's halos from the interior
+        if (z)
+            k_fused_plm<Riem, R, true>
+                <<<pgrid, pblock, 0, s>>>(Pin, P0, U, Pout, G, b, first, write_u, cout, ctl, kz, wrap);
+        else
+            k_fused_plm<Riem, R, false>
+                <<<pgrid, pblock, 0, s>>>(Pin, P0, U, Pout, G, b, first, write_u, cout, ctl, kz, wrap);
     }
     else
     {
@@ -853,22 +863,23 @@ inline void fused_round_r(const mara_desc& d, const Geom& G, int* per_cu, int* p
 
 template <typename R>
 inline void launch_fused_r(const mara_desc& d, const Geom& G, hipStream_t s, const double* Pin, const double* P0,
-                           double* U, double* Pout, double b, int first, int write_u, int cout, Ctl* ctl, int kz)
+                           double* U, double* Pout, double b, int first, int write_u, int cout, Ctl* ctl, int kz,
+                           int wrap = 0)
 {
     const bool hllc = d.riemann == MARA_RIEMANN_HLLC;
     switch (d.recon)
     {
         case MARA_RECON_PCM:
-            if (hllc) launch_fused_t<0, 1, R>(G, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, kz);
-            else launch_fused_t<0, 0, R>(G, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, kz);
+            if (hllc) launch_fused_t<0, 1, R>(G, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, kz, wrap);
+            else launch_fused_t<0, 0, R>(G, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, kz, wrap);
             break;
         case MARA_RECON_PLM:
-            if (hllc) launch_fused_t<1, 1, R>(G, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, kz);
-            else launch_fused_t<1, 0, R>(G, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, kz);
+            if (hllc) launch_fused_t<1, 1, R>(G, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, kz, wrap);
+            else launch_fused_t<1, 0, R>(G, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, kz, wrap);
             break;
         default:
-            if (hllc) launch_fused_t<2, 1, R>(G, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, kz);
-            else launch_fused_t<2, 0, R>(G, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, kz);
+            if (hllc) launch_fused_t<2, 1, R>(G, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, kz, wrap);
+            else launch_fused_t<2, 0, R>(G, s, Pin, P0, U, Pout, b, first, write_u, cout, ctl, kz, wrap);
             break;
     }
 }

@@ -35,8 +35,9 @@ struct StageArgs
     int rows;    // flux-array kernels: work-group rows of 64 zones; fused 2-D
                  // MHD tiles: 64 x (4 * rows) zones
     const double* S;  // source terms added to L (same layout as P), or null
-    // fast tiles of a periodic block: read the halos from the interior images
-    // of the guard zones, so Pin's guards need not be filled
+    // the fast tiles and the bit-exact PLM tile of a periodic block: read the
+    // halos from the interior images of the guard zones, so Pin's guards need
+    // not be filled
     bool wrap = false;
     // flux-array path: called between the flux sweeps and the zone update
     // (the viscous-flux hook enqueues its kernel there), or null
@@ -187,7 +188,7 @@ void stage_kernels(const StageArgs& s)
     if (s.fused)
     {
         launch_fused_r<R>(*s.d, *s.G, s.stream, s.Pin, s.P0, s.U, s.Pout, s.b, s.first, s.write_u, s.cout, s.ctl,
-                          s.kz);
+                          s.kz, s.wrap);
         return;
     }
     if (s.mhd2d)

@@ -214,9 +214,9 @@ GRAVITY = (os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__
                         "uniform_gravity.hsaco"), "mara_source_uniform_gravity")
 
 
-@pytest.mark.parametrize("form", ["fp64_fast", "mixed"])
+@pytest.mark.parametrize("form", ["fp64_fast", "mixed", "fp64"])
 def test_periodic_guards_filled_when_read(form):
-    """The fast tiles of a periodic block read their halos from the interior
+    """The PLM tiles of a periodic block read their halos from the interior
     and leave the guards to whatever reads them next: the download holds the
     periodic ghost fill of its interior, and a stage path that reads guards
     (a source term moves the block to the flux arrays) starts from filled
