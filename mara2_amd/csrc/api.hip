@@ -399,30 +399,29 @@ void apply_shape(mara_block* b, const mara_shape& s)
     b->steps_exec = nullptr;
 }
 
-// One device-driven step: dt from the CFL bins and the clock, the RK stages
-// (the last one reduces the next step's CFL), then the clock advance.
-void enqueue_step(mara_block* b)
+// n device-driven steps: dt from the CFL bins and the clock, the RK stages
+// (the last one reduces the next step's CFL), then the clock advance, with
+// each step's end and the next one's start in one launch (k_ctl_step_next).
+void enqueue_steps(mara_block* b, int n)
 {
+    if (n <= 0) return;
     k_ctl_step_start<<<1, 64, 0, b->stream>>>(b->ctl);
-    for (int rk = 0; rk < b->d.rk_order; ++rk) launch_stage(b, rk, 1);
-    k_ctl_step_end<<<1, 1, 0, b->stream>>>(b->ctl);
+    for (int s = 0; s < n; ++s)
+    {
+        for (int rk = 0; rk < b->d.rk_order; ++rk) launch_stage(b, rk, 1);
+        if (s + 1 < n) k_ctl_step_next<<<1, 64, 0, b->stream>>>(b->ctl);
+        else k_ctl_step_end<<<1, 1, 0, b->stream>>>(b->ctl);
+    }
 }
 
 // Capture kGraphSteps steps of the block's stream into an executable graph:
-// the steps of enqueue_step, with each step's end and the next one's start
-// in one launch (k_ctl_step_next).  Nothing in a step is a memset or a host
+// the launches of enqueue_steps.  Nothing in a step is a memset or a host
 // transfer.
 int capture_steps(mara_block* b)
 {
     hipGraph_t g = nullptr;
     HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    k_ctl_step_start<<<1, 64, 0, b->stream>>>(b->ctl);
-    for (int s = 0; s < mara_block::kGraphSteps; ++s)
-    {
-        for (int rk = 0; rk < b->d.rk_order; ++rk) launch_stage(b, rk, 1);
-        if (s + 1 < mara_block::kGraphSteps) k_ctl_step_next<<<1, 64, 0, b->stream>>>(b->ctl);
-        else k_ctl_step_end<<<1, 1, 0, b->stream>>>(b->ctl);
-    }
+    enqueue_steps(b, mara_block::kGraphSteps);
     const hipError_t launched = hipGetLastError();
     const hipError_t ended = hipStreamEndCapture(b->stream, &g);
     if (!hip_ok(launched) || !hip_ok(ended))
@@ -894,7 +893,7 @@ int mara_block_run(mara_block* b, double cfl, double t_final, int max_steps,
         if (graph)
             for (; s + mara_block::kGraphSteps <= chunk; s += mara_block::kGraphSteps)
                 HIP_TRY(hipGraphLaunch(b->steps_exec, b->stream));
-        for (; s < chunk; ++s) enqueue_step(b);
+        enqueue_steps(b, chunk - s);
         queued += chunk;
         HIP_TRY(hipGetLastError());
         int e = read_ctl(b);
