@@ -605,6 +605,7 @@ def main():
                   "speedup_vs_fp64_fast": mixed["value"] / fast["value"],
                   f"rel_l1_vs_fp64_after_{n_mixed}_steps": l1_mixed100,
                   "roofline_frac": roofline(d64, mixed["stage_ms"], mixed["kernel"])["frac"],
+                  "traffic": traffic_from_profile("stage_mixed", workload),
                   "launch_shape": mixed.get("launch_shape")},
         "host_transfer": {"upload_s": transfers["upload_s"], "download_s": transfers["download_s"],
                           "pcie_inclusive_value": zones * args.steps
