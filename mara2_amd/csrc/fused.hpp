@@ -50,9 +50,9 @@ __device__ __forceinline__ int xcd_tile(int L, int ntiles)
 // every dword): lane l takes v of lane l - 1 (shift_up) or l + 1
 // (shift_down), and the lane with no source, 0 or 63, keeps `edge` (bound_ctrl
 // off).  One v_mov_b32_dpp per dword, where __shfl_up/__shfl_down take an
-// LDS permute and the edge a select (the fast tiles use these; the bit-exact
-// tiles keep the shuffles, which measured no slower there).  Called with all
-// 64 lanes active.
+// LDS permute and the edge a select.  The fast tiles use these; the
+// bit-exact tile keeps the shuffles, against which these measured 0.5 %
+// slower there.  Called with all 64 lanes active.
 template <int CTRL>
 __device__ __forceinline__ float dpp_shift(float v, float edge)
 {
