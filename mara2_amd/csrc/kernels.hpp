@@ -1038,10 +1038,24 @@ k_halo(const HaloEntry* __restrict__ tab, int axis, int nq)
     const int t1 = axis == 0 ? 1 : 0;
     const int t2 = axis == 2 ? 1 : 2;
     const long long slab = (long long)e.N[t1] * e.N[t2];
-    const int a1 = (int)(t % e.N[t1]);
-    const int a2 = (int)((t / e.N[t1]) % e.N[t2]);
-    const int layer = (int)((t / slab) % g);
-    const int side = (int)(t / (slab * g));
+    // thread order: along x the layer is fastest, so that neighbouring lanes
+    // copy the contiguous g values of one row; along y and z the x index is
+    int a1, a2, layer, side;
+    if (axis == 0)
+    {
+        const long long r = t / g;
+        layer = (int)(t - r * g);
+        a1 = (int)(r % e.N[t1]);
+        a2 = (int)((r / e.N[t1]) % e.N[t2]);
+        side = (int)(r / slab);
+    }
+    else
+    {
+        a1 = (int)(t % e.N[t1]);
+        a2 = (int)((t / e.N[t1]) % e.N[t2]);
+        layer = (int)((t / slab) % g);
+        side = (int)(t / (slab * g));
+    }
     const int kind = e.kind[side];
     const double* S = e.P;
     long long sfs = e.fs;
