@@ -180,6 +180,8 @@ bool stage_wraps(const mara_block* b)
     const bool mixed = b->d.precision == MARA_PREC_MIXED;
     const bool fast_tile = b->fused && fast64_supported(&b->d) && (mixed || b->d.precision == MARA_PREC_FP64_FAST);
     const bool exact_plm = b->fused && b->d.precision == MARA_PREC_FP64 && b->d.recon == MARA_RECON_PLM;
+    // every guard's periodic image is an interior zone: validate() rejects
+    // axes with fewer cells than guard layers
     return (fast_tile || exact_plm) && b->G.bc == 0 && !b->in_group;
 }
 
