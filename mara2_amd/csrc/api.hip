@@ -1320,6 +1320,9 @@ struct mara_group
     HaloEntry* halo = nullptr;        // device tables [buffer role][axis][block]
     long long halo_threads[3] = {0, 0, 0};  // largest entry count per axis
     int roles = 2;                    // Pa, Pb (and Pc on the three-buffer RK3 schedule)
+    Ctl** ctls = nullptr;             // device array of the members' control blocks
+    Ctl* ctl_copy = nullptr;          // device copies of them, gathered for one transfer
+    Ctl* ctl_copy_host = nullptr;     // pinned host copy of ctl_copy
 };
 
 namespace {
@@ -1382,9 +1385,12 @@ void group_exchange(mara_group* g, int role)
 // after group_join: the members' control blocks to the host
 int read_group_ctls(mara_group* g)
 {
-    for (mara_block* b : g->blocks)
-        HIP_TRY(hipMemcpyAsync(b->ctl_host, b->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, g->stream));
+    constexpr int W = sizeof(Ctl) / sizeof(unsigned long long);
+    k_ctl_gather<<<(g->count * W + 255) / 256, 256, 0, g->stream>>>(g->ctls, g->count, g->ctl_copy);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(g->ctl_copy_host, g->ctl_copy, g->count * sizeof(Ctl), hipMemcpyDeviceToHost, g->stream));
     HIP_TRY(hipStreamSynchronize(g->stream));
+    for (int m = 0; m < g->count; ++m) std::memcpy(g->blocks[m]->ctl_host, &g->ctl_copy_host[m], sizeof(Ctl));
     return MARA_OK;
 }
 
@@ -1450,6 +1456,9 @@ int mara_group_destroy(mara_group* g)
     for (hipEvent_t e : g->join)
         if (e) hipEventDestroy(e);
     hipFree(g->halo);
+    hipFree(g->ctls);
+    hipFree(g->ctl_copy);
+    if (g->ctl_copy_host) hipHostFree(g->ctl_copy_host);
     if (g->stream) hipStreamDestroy(g->stream);
     delete g;
     return MARA_OK;
@@ -1569,6 +1578,15 @@ int mara_group_create(const mara_desc* d, const int* dims, mara_group** out)
     const size_t tb = tab.size() * sizeof(HaloEntry);
     if (!hip_ok(hipMalloc(&g->halo, tb))) return fail(MARA_ERR_HIP);
     if (!hip_ok(hipMemcpy(g->halo, tab.data(), tb, hipMemcpyHostToDevice))) return fail(MARA_ERR_HIP);
+    // the members' control blocks, addressed together from the group's stream
+    std::vector<Ctl*> ctls(g->count);
+    for (int m = 0; m < g->count; ++m) ctls[m] = g->blocks[m]->ctl;
+    if (!hip_ok(hipMalloc(&g->ctls, g->count * sizeof(Ctl*)))) return fail(MARA_ERR_HIP);
+    if (!hip_ok(hipMemcpy(g->ctls, ctls.data(), g->count * sizeof(Ctl*), hipMemcpyHostToDevice)))
+        return fail(MARA_ERR_HIP);
+    if (!hip_ok(hipMalloc(&g->ctl_copy, g->count * sizeof(Ctl)))) return fail(MARA_ERR_HIP);
+    if (!hip_ok(hipHostMalloc((void**)&g->ctl_copy_host, g->count * sizeof(Ctl), hipHostMallocDefault)))
+        return fail(MARA_ERR_HIP);
     *out = g;
     return MARA_OK;
 }
@@ -1590,6 +1608,7 @@ long long mara_group_device_bytes(const mara_group* g)
 {
     if (!g) return -1;
     long long s = (long long)g->roles * 3 * g->count * sizeof(HaloEntry);
+    s += (long long)g->count * (sizeof(Ctl*) + sizeof(Ctl));  // the control-block table and copies
     for (const mara_block* b : g->blocks) s += b->bytes;
     return s;
 }
@@ -1668,9 +1687,9 @@ int mara_group_advance(mara_group* g, double t0, double dt, mara_report* report)
     if (!g) return MARA_ERR_INVALID_ARGUMENT;
     // per stage: every block's stage on its own stream, then the exchange on
     // the group's stream once all of them are done
+    k_ctl_begin_advance_n<<<(g->count + 63) / 64, 64, 0, g->stream>>>(g->ctls, g->count, dt);
     int e = group_fork(g);
     if (e) return e;
-    for (mara_block* b : g->blocks) k_ctl_begin_advance<<<1, 1, 0, b->stream>>>(b->ctl, dt);
     const int rk_order = g->d.rk_order;
     for (int rk = 0; rk < rk_order; ++rk)
     {

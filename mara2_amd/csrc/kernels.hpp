@@ -1217,7 +1217,7 @@ k_soa_to_aos(const double* __restrict__ P, double* __restrict__ A, Geom G)
 // ---------------------------------------------------------------------------
 // Control kernels (single thread): host-given dt, and the device-driven loop.
 // ---------------------------------------------------------------------------
-static __global__ void k_ctl_begin_advance(Ctl* ctl, double dt)
+__device__ __forceinline__ void begin_advance(Ctl* ctl, double dt)
 {
     ctl->dt = dt;
     ctl->active = 1;
@@ -1226,6 +1226,26 @@ static __global__ void k_ctl_begin_advance(Ctl* ctl, double dt)
     ctl->eigen_fail = 0;
     ctl->superluminal = 0;
     ctl->first_fail = ~0ull;
+}
+
+static __global__ void k_ctl_begin_advance(Ctl* ctl, double dt) { begin_advance(ctl, dt); }
+
+// The control blocks of a block group: k_ctl_begin_advance for all of them in
+// one launch, and their copies gathered into one array for one transfer.
+static __global__ void k_ctl_begin_advance_n(Ctl* const* __restrict__ ctls, int count, double dt)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < count) begin_advance(ctls[m], dt);
+}
+
+static __global__ void k_ctl_gather(Ctl* const* __restrict__ ctls, int count, Ctl* __restrict__ out)
+{
+    constexpr int W = sizeof(Ctl) / sizeof(unsigned long long);
+    static_assert(sizeof(Ctl) % sizeof(unsigned long long) == 0, "Ctl is copied in 8-byte words");
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count * W) return;
+    const int m = t / W, w = t - m * W;
+    reinterpret_cast<unsigned long long*>(out + m)[w] = reinterpret_cast<const unsigned long long*>(ctls[m])[w];
 }
 
 __device__ __forceinline__ void reset_bins(Ctl* ctl)
